@@ -1,0 +1,169 @@
+"""Throughput of the FDTD time-stepper on one MI355X: 3D Yee + CPML in f32, Gcell-updates/s and
+the fused step kernel's fraction of the HBM roofline, beside the CPU baseline.
+
+Workload (the single-GPU configuration the metric is quoted on): a 1x2 MMI on SOI,
+384 x 256 x 192 cells of 20 nm (18.9 M cells), a 220 nm Si core (n = 3.48) in SiO2 (n = 1.444),
+500 nm arms, a 3 um wide MMI body, a 12-cell CPML on every face, Courant 0.9, a Gaussian-pulse
+current at 1.55 um (200 nm bandwidth) with a fundamental-mode-like profile across the input arm,
+and 21-frequency transform monitors on both output ports. The MMI body is 5 um long: 10 um does
+not fit 384 x 20 nm = 7.68 um. The permittivity is sampled at each E value (no subpixel average:
+the geometry is grid-aligned, and the kernel's work does not depend on it).
+
+A step is one pass of the whole path over the grid: the fused Yee + CPML kernel, the monitors'
+taps and their FP64 accumulation. `value` is cells x steps / wall time of K steps with the fields
+resident in HBM. The roofline entry is for the fused kernel: algorithmic bytes per launch
+(E and H read and written, 48 B, plus 3 B of coefficient ids, plus 8 B per CPML psi value) over
+its average duration from HIP events on its stream.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12):
+    import photonoxide_amd as P
+    g = P.Grid3d(nx, ny, nz, h, h, h, 0.0, -ny * h / 2, -nz * h / 2)
+    n_si, n_ox = 3.48, 1.444
+    e_si, e_ox = n_si**2, n_ox**2
+    x_body0, x_body1 = 1.2, 6.2
+    w_arm, w_body = 0.5, 3.0
+    y_out = w_body / 4
+    t = 0.22
+
+    def eps_at(c):
+        i = np.arange(nx)[None, None, :]
+        j = np.arange(ny)[None, :, None]
+        k = np.arange(nz)[:, None, None]
+        x = g.x0 + i * h + (0.5 * h if c == 0 else 0.0)
+        y = g.y0 + j * h + (0.5 * h if c == 1 else 0.0)
+        z = g.z0 + k * h + (0.5 * h if c == 2 else 0.0)
+        core = np.abs(z) < t / 2
+        inp = (x < x_body0) & (np.abs(y) < w_arm / 2)
+        body = (x >= x_body0) & (x <= x_body1) & (np.abs(y) < w_body / 2)
+        out = (x > x_body1) & ((np.abs(y - y_out) < w_arm / 2) | (np.abs(y + y_out) < w_arm / 2))
+        si = core & (inp | body | out)
+        return np.where(si, e_si, e_ox).astype(np.float64)
+
+    eps = np.stack([np.broadcast_to(eps_at(c), (nz, ny, nx)) for c in range(3)])
+    s = P.Simulation.with_eps(g, eps, P.Boundaries.cpml_all(pml), 0.9)
+    # the source: E_y across the input arm with a Gaussian profile, a pulse at 1.55 um
+    f0 = 1 / 1.55
+    wave = P.Waveform.pulse(f0, 1 / 1.45 - 1 / 1.65)
+    i_src = pml + 10
+    jj, kk = np.meshgrid(np.arange(ny), np.arange(nz))
+    y = g.y0 + jj * h + 0.5 * h
+    z = g.z0 + kk * h
+    prof = np.exp(-(y / 0.3) ** 2 - (z / 0.15) ** 2)
+    keep = prof > 1e-3
+    idx = (kk[keep] * ny + jj[keep]) * nx + i_src
+    s.add_current_indices(P.Field.E, np.full(idx.size, 1), idx, prof[keep].astype(complex), wave)
+    # 21-frequency monitors on the two output ports, over the arm and its surroundings
+    freqs = 1 / np.linspace(1.50, 1.60, 21)
+    i_mon = nx - pml - 10
+    for yc in (y_out, -y_out):
+        jc = int(round((yc - g.y0) / h))
+        s.add_dft((i_mon, jc - 25, nz // 2 - 15), (i_mon, jc + 24, nz // 2 + 14), freqs)
+    return s, g
+
+
+def algorithmic_bytes_per_step(nx, ny, nz, pml, coef_bytes=3):
+    n = nx * ny * nz
+    psi = 0
+    for axis_n in (nx, ny, nz):
+        psi += 4 * (n // axis_n) * (2 * pml)  # 2 fields x 2 components, both sides
+    return n * (48 + coef_bytes) + 8 * psi, psi
+
+
+def cpu_baseline(nx, ny, nz, pml):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    threads = min(16, os.cpu_count() or 1)
+    steps = 3
+    sec = O.lib.or_cpu_rate_f32(nx, ny, nz, pml, steps, 3, threads)
+    rate = nx * ny * nz / sec / 1e9
+    return {"value": round(rate, 4), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"the reference's CPU row kernel restated in C/OpenMP (oracle/cpu_baseline.c), f32, "
+                      f"{nx}x{ny}x{nz} cells with a {pml}-cell CPML, best of 3 runs of {steps} steps"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--planes", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: each box has one MI355X, and "
+                          "--gpus above 1 is not supported (the design shards a grid by z slabs; "
+                          "see DESIGN.md)"}))
+        sys.exit(2)
+    import photonoxide_amd as P
+    from photonoxide_amd import _native as N
+
+    nx, ny, nz, pml = 384, 256, 192, 12
+    s, g = mmi_problem(nx, ny, nz, 0.02, pml)
+    gpu = P.Gpu(0)
+    s.set_device(gpu, P.Precision.F32)
+    if args.rows or args.planes:
+        s.set_tiling(args.rows, args.planes)
+    res = s.resident()
+    s.run(args.warmup)
+    N.check(N.lib.pxfdtd_resident_synchronize(res))
+    N.check(N.lib.pxfdtd_resident_set_profiling(res, 1))
+    t0 = time.perf_counter()
+    s.run(args.steps)
+    N.check(N.lib.pxfdtd_resident_synchronize(res))
+    elapsed = time.perf_counter() - t0
+    kernel_ms = N.lib.pxfdtd_resident_kernel_ms(res)
+    cells = nx * ny * nz
+    value = cells * args.steps / elapsed / 1e9
+    mode = N.lib.pxfdtd_resident_coefficient_mode(res)
+    bytes_step, psi = algorithmic_bytes_per_step(nx, ny, nz, pml, 3 if mode == 0 else 24)
+    achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_fused_step_mmi_f32.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    out = {
+        "metric": "3D FDTD Gcell-updates/s (% HBM roofline); batched-job p50/p99 latency",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "mmi_1x2_soi_384x256x192_dx20nm_cpml12_f32",
+                   "cells": cells, "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
+                   "monitors": "2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums",
+                   "psi_values": psi},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "fused_step<float,0,4>", "kernel_ms": round(kernel_ms, 5),
+                     "bytes_per_launch": bytes_step,
+                     "gcell_per_s_kernel": round(cells / (kernel_ms * 1e-3) / 1e9, 3)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(nx, ny, nz, pml)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

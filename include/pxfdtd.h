@@ -1,0 +1,256 @@
+/* pxfdtd: the C ABI of the MI355X-native FDTD time-stepper.
+ *
+ * Two levels, both plain C (opaque handles, plain pointers and sizes, int status codes):
+ *
+ *  1. The device seam, pxfdtd_device_* and pxfdtd_resident_*: what photonoxide's GPU resident
+ *     state does for a Simulation (src/fdtd/gpu/resident.rs:43-870, reached through
+ *     Simulation::set_device / run, src/fdtd.rs:1142-1158,1517-1540, and gpu::Resident::new,
+ *     upload, step, download, record). The caller (the reference's Rust Simulation, or the C++
+ *     host below) owns the problem and computes its tables in FP64, as the reference's host
+ *     does (resident.rs:543-843); the device steps them in f32 or f64.
+ *
+ *  2. The host mirror, pxfdtd_simulation_*: photonoxide's fdtd::Simulation (src/fdtd.rs:470-1590)
+ *     in C++: grid, boundaries, averaged permittivity, CPML, conductivity, conductors, point
+ *     sources, currents, dipoles, probes, transform monitors and dispersive media, with the
+ *     same meaning and errors, stepping on the device through level 1.
+ *
+ * Every function returning int returns PXFDTD_OK (0) or an error class; pxfdtd_last_error()
+ * gives the calling thread's last message. A refused feature or an allocation failure is a
+ * status, never an abort (the reference panics on device failure, gpu.rs:440-450).
+ *
+ * Threading: a pxfdtd_resident or pxfdtd_simulation is used from one host thread at a time and
+ * owns one HIP stream; a pxfdtd_device may be shared.
+ *
+ * Layout of every host array: the reference's, component by component (x, y, z), then k, j, i
+ * with i fastest: index(c, (i, j, k)) = c N + (k ny + j) nx + i (src/fdfd/three.rs:122-124).
+ */
+#ifndef PXFDTD_ABI_H
+#define PXFDTD_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PXFDTD_OK = 0,
+    PXFDTD_ERR_INVALID = 1,     /* a value out of range: Error::InvalidValue */
+    PXFDTD_ERR_UNSUPPORTED = 2, /* what the device doesn't step: Error::Gpu */
+    PXFDTD_ERR_MEMORY = 3,      /* it doesn't fit */
+    PXFDTD_ERR_DEVICE = 4       /* the HIP runtime failed */
+};
+enum { PXFDTD_F32 = 0, PXFDTD_F64 = 1 };
+enum { PXFDTD_E = 0, PXFDTD_H = 1 };
+
+const char *pxfdtd_last_error(void);
+
+/* ------------------------------------------------------------------ level 1: the device */
+
+typedef struct pxfdtd_device pxfdtd_device;
+typedef struct pxfdtd_resident pxfdtd_resident;
+
+typedef struct {
+    char name[256];
+    char arch[64];
+    int32_t ordinal;
+    int32_t compute_units;
+    int64_t hbm_bytes;
+    int64_t free_bytes;
+} pxfdtd_device_info;
+
+/* Opens HIP device `ordinal` (replaces Gpu::new, src/fdtd/gpu.rs:149-234). */
+int pxfdtd_device_open(int ordinal, pxfdtd_device **out);
+int pxfdtd_device_query(const pxfdtd_device *dev, pxfdtd_device_info *info);
+void pxfdtd_device_close(pxfdtd_device *dev);
+
+/* A dispersive medium's values and poles (src/fdtd/media.rs:592-607 Group): E's update there
+ * uses p_e, p_m as its ca, cb (given in the problem's ca, cb), then the ADE rest. */
+typedef struct {
+    int32_t poles;           /* at most 4 */
+    double chi;              /* sum gamma_p / (4 dt) */
+    double coef[4][3];       /* alpha_p, xi_p, gamma_p */
+    int64_t count;
+    const int32_t *component;
+    const int64_t *index;    /* within a component: (k ny + j) nx + i */
+} pxfdtd_medium;
+
+/* Everything Resident::new and plan read (resident.rs:154-293,543-793). */
+typedef struct {
+    int32_t nx, ny, nz;
+    int32_t precision;       /* PXFDTD_F32 or PXFDTD_F64 */
+    int32_t periodic[3];     /* Bloch k = 0 sides */
+    int32_t pml_low[3], pml_high[3];
+    double dt;
+    double step[3];          /* dx, dy, dz */
+    const double *factors_e[3];   /* kappa/step at the nodes, n_axis each */
+    const double *factors_h[3];   /* kappa/step halfway after them */
+    /* the CPML recursion's b and a per [axis][side low, high][field E, H], count values each */
+    const double *cpml_b[3][2][2];
+    const double *cpml_a[3][2][2];
+    const double *ca;        /* [3][N] */
+    const double *cb;        /* [3][N] */
+    int32_t nmedia;
+    const pxfdtd_medium *media;
+} pxfdtd_problem;
+
+/* A source value (Applied, src/fdtd/sources.rs:132-136): after `field`'s curl update each step,
+ * H -= dt Re(a w) or E -= cb Re(a w), w the emitter's value that step; entries apply in
+ * order. A point source is a = 1 with w = Waveform::at. */
+typedef struct {
+    int32_t field, component;
+    int64_t index;
+    double a_re, a_im;
+    int32_t emitter;
+} pxfdtd_injection;
+
+typedef struct {
+    int32_t field, component;
+    int64_t index;
+} pxfdtd_probe;
+
+/* A transform series (src/fdtd/monitors.rs:45-55): one component over the box lo..hi
+ * (hi exclusive), its sums [f][k][j][i] complex, at monitor `monitor`'s frequencies. */
+typedef struct {
+    int32_t field, component;
+    int32_t lo[3], hi[3];
+    int32_t monitor;
+} pxfdtd_series;
+
+/* The HBM a problem needs on the device, bytes: for admission by footprint. */
+int pxfdtd_resident_footprint(const pxfdtd_problem *p, size_t *bytes);
+/* Allocates and tables the problem on the device (Resident::new). */
+int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *p, pxfdtd_resident **out);
+void pxfdtd_resident_destroy(pxfdtd_resident *r);
+/* Fields [3][N] and the CPML's psi, all slabs concatenated in the reference's slab order
+ * (axis x, y, z; side low, high; field E, H; component axis+1, axis+2; each [k][j][i] over its
+ * box): Resident::upload / download (resident.rs:378-413). psi may be NULL (zeros). */
+int pxfdtd_resident_upload(pxfdtd_resident *r, const double *e, const double *h,
+                           const double *psi);
+int pxfdtd_resident_download(pxfdtd_resident *r, double *e, double *h, double *psi);
+int64_t pxfdtd_resident_psi_len(const pxfdtd_resident *r);
+int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_injection *inj, int64_t n,
+                                   int32_t emitters);
+int pxfdtd_resident_set_probes(pxfdtd_resident *r, const pxfdtd_probe *probes, int32_t n);
+/* monitor_nf[m]: monitor m's number of frequencies. Phase slots per step: for each monitor,
+ * nf slots at E's time then nf at H's. Resets the sums. */
+int pxfdtd_resident_set_series(pxfdtd_resident *r, const pxfdtd_series *series, int32_t n,
+                               const int32_t *monitor_nf, int32_t monitors);
+int32_t pxfdtd_resident_phase_slots(const pxfdtd_resident *r);
+/* `count` steps (Resident::step, resident.rs:430-539): waves [count][emitters][2] (each
+ * emitter's w at its field's time that step), phases [count][slots][2] (e^(i w t) dt in FP64,
+ * E at (n+1) dt, H at (n+1/2) dt). */
+int pxfdtd_resident_step(pxfdtd_resident *r, uint32_t count, const double *waves,
+                         const double *phases);
+/* the steps recorded since the last read of the probes */
+int64_t pxfdtd_resident_probe_steps(const pxfdtd_resident *r);
+/* probe values recorded since the last read, [steps][probes]; *steps on return */
+int pxfdtd_resident_read_probes(pxfdtd_resident *r, double *out, int64_t capacity,
+                                int64_t *steps);
+/* the transform sums, FP64, series after series, each [f][k][j][i] complex */
+int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_t capacity);
+int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r);
+/* the tiling: rows per tile (1, 2, 4, 8) and planes per chunk; 0 keeps the current value */
+int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes);
+int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes);
+/* the coefficient storage chosen: 0 uint8 ids into a table, 1 full ca/cb arrays */
+int32_t pxfdtd_resident_coefficient_mode(const pxfdtd_resident *r);
+/* the average time per step of the last pxfdtd_resident_step call (all its kernels), ms, from
+ * HIP events on the resident's stream */
+double pxfdtd_resident_last_kernel_ms(const pxfdtd_resident *r);
+int pxfdtd_resident_synchronize(pxfdtd_resident *r);
+/* time every fused step kernel with HIP events on the resident's stream (off by default) */
+int pxfdtd_resident_set_profiling(pxfdtd_resident *r, int32_t on);
+/* the fused step kernel's average duration over the last pxfdtd_resident_step call, ms */
+double pxfdtd_resident_kernel_ms(const pxfdtd_resident *r);
+
+/* ------------------------------------------------------------ level 2: the host mirror */
+
+typedef struct pxfdtd_simulation pxfdtd_simulation;
+
+typedef struct {
+    int32_t nx, ny, nz;
+    double dx, dy, dz, x0, y0, z0;
+} pxfdtd_grid;
+
+/* Boundaries (src/fdtd.rs:177-232) with the Cpml (src/fdtd.rs:84-110) */
+typedef struct {
+    int32_t periodic[3];
+    int32_t low[3], high[3];
+    double reflection;
+    int32_t has_sigma;
+    double sigma, order, kappa, alpha;
+} pxfdtd_boundaries;
+
+/* A region of the scene: kind 0 the open box lo < p < hi, kind 1 a cylinder along z,
+ * (x - lo0)^2 + (y - lo1)^2 < hi0^2 and lo2 < z < hi2; value[] its diagonal eps (or sigma in
+ * value[0], or membership != 0). Later regions replace earlier ones. */
+typedef struct {
+    int32_t kind;
+    double lo[3], hi[3];
+    double value[3];
+} pxfdtd_shape;
+
+typedef struct {
+    double background[3];
+    int32_t n;
+    const pxfdtd_shape *shapes;
+} pxfdtd_scene;
+
+/* Waveform (src/fdtd.rs:259-424): kind 0 Gaussian, 1 differentiated Gaussian, 2 continuous */
+typedef struct {
+    int32_t kind;
+    double frequency, width, delay;
+    double amp_re, amp_im, ramp;
+} pxfdtd_waveform;
+
+/* Simulation::diagonal with the scene's eps averaged per component over 8^3 samples
+ * (src/fdtd.rs:570-653) */
+int pxfdtd_simulation_new(const pxfdtd_grid *g, const pxfdtd_boundaries *b,
+                          const pxfdtd_scene *eps, double courant, pxfdtd_simulation **out);
+/* the permittivity each value of E sees given directly, [3][N] */
+int pxfdtd_simulation_new_eps(const pxfdtd_grid *g, const pxfdtd_boundaries *b,
+                              const double *eps3n, double courant, pxfdtd_simulation **out);
+void pxfdtd_simulation_destroy(pxfdtd_simulation *s);
+int pxfdtd_simulation_conductivity(pxfdtd_simulation *s, const pxfdtd_scene *sigma);
+int pxfdtd_simulation_conductor(pxfdtd_simulation *s, int32_t comp, int32_t i, int32_t j,
+                                int32_t k);
+int pxfdtd_simulation_add_source(pxfdtd_simulation *s, int32_t field, int32_t comp, int32_t i,
+                                 int32_t j, int32_t k, const pxfdtd_waveform *w);
+int pxfdtd_simulation_add_current(pxfdtd_simulation *s, int32_t field, int64_t n,
+                                  const int32_t *comps, const int64_t *index,
+                                  const double *amps, const pxfdtd_waveform *w);
+int pxfdtd_simulation_add_dipole(pxfdtd_simulation *s, int32_t field, int32_t comp,
+                                 const double position[3], double amp_re, double amp_im,
+                                 const pxfdtd_waveform *w);
+int pxfdtd_simulation_add_probe(pxfdtd_simulation *s, int32_t field, int32_t comp, int32_t i,
+                                int32_t j, int32_t k, int32_t *number);
+int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low[3], const int32_t high[3],
+                              int32_t nf, const double *freqs, int32_t *number);
+/* poles [np][4]: (0, strength, resonance, damping) Lorentz or (1, plasma, damping, 0) Drude */
+int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf, int32_t np,
+                                 const double *poles, const pxfdtd_scene *inside);
+/* Steps on `dev` in `precision` from now on (Simulation::set_device); NULL: back to none */
+int pxfdtd_simulation_set_device(pxfdtd_simulation *s, pxfdtd_device *dev, int32_t precision);
+int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count);
+double pxfdtd_simulation_dt(const pxfdtd_simulation *s);
+int64_t pxfdtd_simulation_steps(const pxfdtd_simulation *s);
+int64_t pxfdtd_simulation_cells(const pxfdtd_simulation *s);
+/* fields: copies into out[N] (brought back from the device if it steps there) */
+int pxfdtd_simulation_field(pxfdtd_simulation *s, int32_t which, int32_t comp, double *out);
+int pxfdtd_simulation_set_field(pxfdtd_simulation *s, int32_t which, int32_t comp,
+                                const double *values);
+int pxfdtd_simulation_coefficients(const pxfdtd_simulation *s, int32_t comp, double *ca,
+                                   double *cb);
+int64_t pxfdtd_simulation_probe_len(pxfdtd_simulation *s, int32_t n);
+int pxfdtd_simulation_probe(pxfdtd_simulation *s, int32_t n, double *out);
+/* transform monitor n's series (0..5: E x, y, z, H x, y, z), [f][k][j][i] complex */
+int64_t pxfdtd_simulation_dft_len(pxfdtd_simulation *s, int32_t n, int32_t series);
+int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t series, double *out);
+pxfdtd_resident *pxfdtd_simulation_resident(pxfdtd_simulation *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,11 @@
+"""photonoxide_amd: an MI355X-native FDTD time-stepper behind photonoxide's FDTD interface.
+
+The compute path is hand-written HIP for gfx950 in ``csrc/`` behind a C ABI
+(``include/pxfdtd.h``); this package binds it with ctypes and mirrors the reference's
+``fdtd`` API in :mod:`photonoxide_amd.fdtd`.
+"""
+from .fdtd import (Axis, Box, Boundaries, Cpml, Current, Cylinder, Dipole, Field, Gpu, Grid3d,
+                   Periodic, Pml, Precision, Scene, Simulation, Source, Waveform)
+
+__all__ = ["Axis", "Box", "Boundaries", "Cpml", "Current", "Cylinder", "Dipole", "Field", "Gpu",
+           "Grid3d", "Periodic", "Pml", "Precision", "Scene", "Simulation", "Source", "Waveform"]

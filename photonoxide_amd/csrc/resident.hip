@@ -1,0 +1,970 @@
+// Level 1 of the C ABI (include/pxfdtd.h): the device seam. A resident problem's buffers on
+// one MI355X, its tables, and the stepping loop: the fused Yee + CPML kernel once per step,
+// the taps that copy probe and transform values out, and every few steps the FP64 transform
+// accumulation. It replaces photonoxide's gpu::Resident (src/fdtd/gpu/resident.rs:43-870):
+// new (:154-293), upload/download (:378-413), step (:430-539) and record (:846-869).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/pxfdtd.h"
+#include "kernels.hpp"
+
+using namespace pxfdtd;
+
+// ------------------------------------------------------------------------------ errors
+static thread_local std::string g_error;
+
+extern "C" const char *pxfdtd_last_error(void) { return g_error.c_str(); }
+
+int px_fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                        \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return px_fail(e_ == hipErrorOutOfMemory ? PXFDTD_ERR_MEMORY : PXFDTD_ERR_DEVICE, \
+                           "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+// ------------------------------------------------------------------------------ device
+struct pxfdtd_device {
+    int ordinal;
+    hipDeviceProp_t prop;
+};
+
+extern "C" int pxfdtd_device_open(int ordinal, pxfdtd_device **out) {
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (ordinal < 0 || ordinal >= count)
+        return px_fail(PXFDTD_ERR_INVALID, "no HIP device %d (%d present)", ordinal, count);
+    auto *d = new pxfdtd_device();
+    d->ordinal = ordinal;
+    hipError_t e = hipGetDeviceProperties(&d->prop, ordinal);
+    if (e != hipSuccess) {
+        delete d;
+        return px_fail(PXFDTD_ERR_DEVICE, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    }
+    *out = d;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_device_query(const pxfdtd_device *d, pxfdtd_device_info *info) {
+    memset(info, 0, sizeof *info);
+    snprintf(info->name, sizeof info->name, "%s", d->prop.name);
+    snprintf(info->arch, sizeof info->arch, "%s", d->prop.gcnArchName);
+    info->ordinal = d->ordinal;
+    info->compute_units = d->prop.multiProcessorCount;
+    info->hbm_bytes = (int64_t)d->prop.totalGlobalMem;
+    size_t free_b = 0, total = 0;
+    HIPCHK(hipSetDevice(d->ordinal));
+    HIPCHK(hipMemGetInfo(&free_b, &total));
+    info->free_bytes = (int64_t)free_b;
+    return PXFDTD_OK;
+}
+
+extern "C" void pxfdtd_device_close(pxfdtd_device *d) { delete d; }
+
+// ------------------------------------------------------------------------------ buffers
+struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    Buf() = default;
+    Buf(const Buf &) = delete;
+    Buf &operator=(const Buf &) = delete;
+    ~Buf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t n) {
+        release();
+        if (n == 0) return PXFDTD_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return px_fail(PXFDTD_ERR_MEMORY, "hipMalloc of %zu bytes: %s", n, hipGetErrorString(e));
+        }
+        bytes = n;
+        return PXFDTD_OK;
+    }
+    // grows to at least n bytes, keeping nothing
+    int reserve(size_t n) { return n <= bytes ? PXFDTD_OK : alloc(n); }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+#define TRY(x)                       \
+    do {                             \
+        int rc_ = (x);               \
+        if (rc_ != PXFDTD_OK) return rc_; \
+    } while (0)
+
+struct SlabDesc {
+    int field, comp, axis, side, from, count;
+    long long offset, len;
+};
+
+struct pxfdtd_resident {
+    pxfdtd_device *dev = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int prec = PXFDTD_F32;
+    size_t esz = 4;
+    int nx = 0, ny = 0, nz = 0, pitch = 0;
+    long long plane = 0, nval = 0, ncells = 0;
+    int periodic[3] = {0, 0, 0}, low[3] = {0, 0, 0}, high[3] = {0, 0, 0};
+    double dt = 0, step[3] = {0, 0, 0};
+    Buf field[2][6];
+    int mode = 0;
+    int table_stride = 256;
+    Buf ids[3], table, group_of, ca[3], cb[3], grp[3];
+    bool has_grp = false;
+    Buf fe[3], fh[3];
+    Buf pb[3][2][2], pa[3][2][2];
+    Buf psi[2];
+    long long psi_off[2][3][3][2];
+    long long psi_len = 0;
+    std::vector<SlabDesc> slabs;
+    // media
+    Buf groups, m_before, m_cur, m_prev;
+    int ngroups = 0, max_poles = 0;
+    // injections
+    std::vector<pxfdtd_injection> inj_host;
+    int emitters = 0;
+    Buf inj_off, inj;
+    bool have_inj = false;
+    Buf waves;
+    // taps, probes and transforms
+    int nprobes = 0;
+    long long nvals = 0, ntaps = 0;
+    Buf tap_index, tap_which, tap_fields[2];
+    Buf staging;
+    int block_steps = 32;
+    Buf series_of, series_info, sums, phases;
+    long long sums_len = 0;
+    int slots = 0;
+    Buf probe_rec;
+    long long probe_rows = 0, probe_cap = 0;
+    // stepping
+    int p = 0;
+    int ty = 4, kz = 16;
+    double last_ms = 0;
+    // per-launch timing of the fused kernel, when asked for
+    bool profile = false;
+    std::vector<hipEvent_t> kev;
+    double kernel_ms = 0;
+};
+
+struct TapSpec;
+static std::map<const pxfdtd_resident *, TapSpec> &specs();
+
+// ---------------------------------------------------------------------- problem checks
+static long long slab_volume(const pxfdtd_problem *p, int axis, int count) {
+    long long n[3] = {p->nx, p->ny, p->nz};
+    n[axis] = count;
+    return n[0] * n[1] * n[2];
+}
+
+static int check_problem(const pxfdtd_problem *p) {
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
+        return px_fail(PXFDTD_ERR_INVALID, "the grid needs cells, got %d x %d x %d", p->nx, p->ny, p->nz);
+    if (p->precision != PXFDTD_F32 && p->precision != PXFDTD_F64)
+        return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");
+    if (!(p->dt > 0) || !(p->step[0] > 0) || !(p->step[1] > 0) || !(p->step[2] > 0))
+        return px_fail(PXFDTD_ERR_INVALID, "dt and the steps must be positive");
+    if (!p->ca || !p->cb) return px_fail(PXFDTD_ERR_INVALID, "ca and cb are required");
+    const int n[3] = {p->nx, p->ny, p->nz};
+    for (int a = 0; a < 3; a++) {
+        if (!p->factors_e[a] || !p->factors_h[a])
+            return px_fail(PXFDTD_ERR_INVALID, "the difference factors are required");
+        if (p->periodic[a] && (p->pml_low[a] || p->pml_high[a]))
+            return px_fail(PXFDTD_ERR_INVALID, "a periodic axis has no CPML");
+        if (p->pml_low[a] < 0 || p->pml_high[a] < 0 || p->pml_low[a] + p->pml_high[a] >= n[a])
+            return px_fail(PXFDTD_ERR_INVALID, "CPMLs of %d and %d cells leave nothing of %d cells",
+                           p->pml_low[a], p->pml_high[a], n[a]);
+        const int cnt[2] = {p->pml_low[a], p->pml_high[a]};
+        for (int s = 0; s < 2; s++)
+            for (int f = 0; f < 2; f++)
+                if (cnt[s] && (!p->cpml_b[a][s][f] || !p->cpml_a[a][s][f]))
+                    return px_fail(PXFDTD_ERR_INVALID, "CPML coefficients missing for axis %d", a);
+    }
+    if (p->nmedia > 127) return px_fail(PXFDTD_ERR_UNSUPPORTED, "at most 127 dispersive media");
+    for (int m = 0; m < p->nmedia; m++)
+        if (p->media[m].poles < 1 || p->media[m].poles > 4)
+            return px_fail(PXFDTD_ERR_UNSUPPORTED, "a dispersive medium needs 1 to 4 poles here");
+    return PXFDTD_OK;
+}
+
+static int pitch_of(int nx) { return (nx + 63) / 64 * 64; }
+
+extern "C" int pxfdtd_resident_footprint(const pxfdtd_problem *p, size_t *bytes) {
+    TRY(check_problem(p));
+    const size_t esz = p->precision == PXFDTD_F64 ? 8 : 4;
+    const size_t nval = (size_t)pitch_of(p->nx) * p->ny * p->nz;
+    size_t b = 12 * nval * esz;  // two copies of E and H
+    b += 3 * nval;               // ids (the table mode; the array mode adds more, see create)
+    size_t psi = 0;
+    for (int a = 0; a < 3; a++) psi += 4 * (size_t)slab_volume(p, a, p->pml_low[a] + p->pml_high[a]);
+    b += 2 * psi * esz;
+    int maxp = 0;
+    for (int m = 0; m < p->nmedia; m++) maxp = std::max(maxp, (int)p->media[m].poles);
+    if (maxp) b += 3 * nval * esz * (1 + 2 * (size_t)maxp);
+    *bytes = b + (1u << 20);
+    return PXFDTD_OK;
+}
+
+template <typename T>
+static std::vector<T> converted(const double *v, size_t n) {
+    std::vector<T> o(n);
+    for (size_t q = 0; q < n; q++) o[q] = (T)v[q];
+    return o;
+}
+
+template <typename T>
+static int upload_table(Buf &b, const double *v, size_t n, hipStream_t s) {
+    TRY(b.alloc(std::max<size_t>(n, 1) * sizeof(T)));
+    if (!n) return PXFDTD_OK;
+    std::vector<T> c = converted<T>(v, n);
+    HIPCHK(hipMemcpyAsync(b.p, c.data(), n * sizeof(T), hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return PXFDTD_OK;
+}
+
+template <typename T>
+static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
+    const long long N = (long long)p->nx * p->ny * p->nz;
+    const long long nval = r->nval;
+    hipStream_t s = r->stream;
+    for (int c = 0; c < 2; c++)
+        for (int q = 0; q < 6; q++) {
+            TRY(r->field[c][q].alloc(nval * sizeof(T)));
+            HIPCHK(hipMemsetAsync(r->field[c][q].p, 0, nval * sizeof(T), s));
+        }
+    // media membership per value: group + 1, 0 outside
+    std::vector<int8_t> member;
+    if (p->nmedia) {
+        member.assign(3 * N, 0);
+        for (int m = 0; m < p->nmedia; m++) {
+            const pxfdtd_medium &M = p->media[m];
+            for (long long v = 0; v < M.count; v++) {
+                const int c = M.component[v];
+                const long long q = M.index[v];
+                if (c < 0 || c > 2 || q < 0 || q >= N)
+                    return px_fail(PXFDTD_ERR_INVALID, "a medium's value is outside the grid");
+                member[c * N + q] = (int8_t)(m + 1);
+            }
+        }
+    }
+    // E's coefficients: uint8 ids if every component has at most 256 distinct
+    // (ca, cb, medium) triples, else whole arrays
+    std::vector<std::vector<uint8_t>> idv(3);
+    std::vector<double> tab(6 * 256, 0.0);
+    std::vector<int8_t> tgrp(3 * 256, -1);
+    bool table_ok = true;
+    for (int c = 0; c < 3 && table_ok; c++) {
+        std::map<std::tuple<uint64_t, uint64_t, int>, int> seen;
+        idv[c].resize(nval, 0);
+        for (long long q = 0; q < N && table_ok; q++) {
+            const double a = p->ca[c * N + q], b = p->cb[c * N + q];
+            const int g = member.empty() ? 0 : member[c * N + q];
+            uint64_t ab, bb;
+            memcpy(&ab, &a, 8);
+            memcpy(&bb, &b, 8);
+            auto key = std::make_tuple(ab, bb, g);
+            auto it = seen.find(key);
+            int id;
+            if (it == seen.end()) {
+                id = (int)seen.size();
+                if (id >= 256) { table_ok = false; break; }
+                seen.emplace(key, id);
+                tab[(c * 256 + id) * 2] = a;
+                tab[(c * 256 + id) * 2 + 1] = b;
+                tgrp[c * 256 + id] = (int8_t)(g - 1);
+            } else {
+                id = it->second;
+            }
+            const long long i = q % p->nx, rest = q / p->nx;
+            idv[c][(rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i] = (uint8_t)id;
+        }
+    }
+    r->mode = table_ok ? 0 : 1;
+    if (r->mode == 0) {
+        for (int c = 0; c < 3; c++) {
+            TRY(r->ids[c].alloc(nval));
+            HIPCHK(hipMemcpyAsync(r->ids[c].p, idv[c].data(), nval, hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        TRY(upload_table<T>(r->table, tab.data(), tab.size(), s));
+        TRY(r->group_of.alloc(tgrp.size()));
+        HIPCHK(hipMemcpyAsync(r->group_of.p, tgrp.data(), tgrp.size(), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } else {
+        std::vector<T> a(nval, T(0)), b(nval, T(0));
+        std::vector<int8_t> g(nval, -1);
+        for (int c = 0; c < 3; c++) {
+            for (long long q = 0; q < N; q++) {
+                const long long i = q % p->nx, rest = q / p->nx;
+                const long long d = (rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i;
+                a[d] = (T)p->ca[c * N + q];
+                b[d] = (T)p->cb[c * N + q];
+                if (!member.empty()) g[d] = (int8_t)(member[c * N + q] - 1);
+            }
+            TRY(r->ca[c].alloc(nval * sizeof(T)));
+            TRY(r->cb[c].alloc(nval * sizeof(T)));
+            HIPCHK(hipMemcpyAsync(r->ca[c].p, a.data(), nval * sizeof(T), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(r->cb[c].p, b.data(), nval * sizeof(T), hipMemcpyHostToDevice, s));
+            if (!member.empty()) {
+                TRY(r->grp[c].alloc(nval));
+                HIPCHK(hipMemcpyAsync(r->grp[c].p, g.data(), nval, hipMemcpyHostToDevice, s));
+            }
+            HIPCHK(hipStreamSynchronize(s));
+        }
+        r->has_grp = !member.empty();
+    }
+    const int n[3] = {p->nx, p->ny, p->nz};
+    for (int a = 0; a < 3; a++) {
+        TRY(upload_table<T>(r->fe[a], p->factors_e[a], n[a], s));
+        TRY(upload_table<T>(r->fh[a], p->factors_h[a], n[a], s));
+        const int cnt[2] = {p->pml_low[a], p->pml_high[a]};
+        for (int sd = 0; sd < 2; sd++)
+            for (int f = 0; f < 2; f++)
+                if (cnt[sd]) {
+                    TRY(upload_table<T>(r->pb[a][sd][f], p->cpml_b[a][sd][f], cnt[sd], s));
+                    TRY(upload_table<T>(r->pa[a][sd][f], p->cpml_a[a][sd][f], cnt[sd], s));
+                }
+    }
+    // psi: the reference's slab order (src/fdtd.rs:838-866)
+    for (int F = 0; F < 2; F++)
+        for (int c = 0; c < 3; c++)
+            for (int w = 0; w < 3; w++)
+                for (int sd = 0; sd < 2; sd++) r->psi_off[F][c][w][sd] = -1;
+    long long off = 0;
+    for (int a = 0; a < 3; a++) {
+        const int cnt[2] = {p->pml_low[a], p->pml_high[a]};
+        const int from[2] = {0, n[a] - p->pml_high[a]};
+        for (int sd = 0; sd < 2; sd++) {
+            if (!cnt[sd]) continue;
+            for (int F = 0; F < 2; F++)
+                for (int cc = 1; cc <= 2; cc++) {
+                    const int c = (a + cc) % 3;
+                    const long long len = slab_volume(p, a, cnt[sd]);
+                    r->slabs.push_back({F, c, a, sd, from[sd], cnt[sd], off, len});
+                    r->psi_off[F][c][a][sd] = off;
+                    off += len;
+                }
+        }
+    }
+    r->psi_len = off;
+    for (int q = 0; q < 2; q++) {
+        TRY(r->psi[q].alloc(std::max<long long>(off, 1) * sizeof(T)));
+        HIPCHK(hipMemsetAsync(r->psi[q].p, 0, std::max<long long>(off, 1) * sizeof(T), s));
+    }
+    // media: groups and dense state
+    if (p->nmedia) {
+        std::vector<MediumGroup> gs(p->nmedia);
+        for (int m = 0; m < p->nmedia; m++) {
+            memset(&gs[m], 0, sizeof(MediumGroup));
+            gs[m].poles = p->media[m].poles;
+            gs[m].chi = p->media[m].chi;
+            memcpy(gs[m].coef, p->media[m].coef, sizeof gs[m].coef);
+            r->max_poles = std::max(r->max_poles, (int)p->media[m].poles);
+        }
+        r->ngroups = p->nmedia;
+        TRY(r->groups.alloc(gs.size() * sizeof(MediumGroup)));
+        HIPCHK(hipMemcpyAsync(r->groups.p, gs.data(), gs.size() * sizeof(MediumGroup),
+                              hipMemcpyHostToDevice, s));
+        TRY(r->m_before.alloc(3 * nval * sizeof(T)));
+        TRY(r->m_cur.alloc(3 * nval * r->max_poles * sizeof(T)));
+        TRY(r->m_prev.alloc(3 * nval * r->max_poles * sizeof(T)));
+        HIPCHK(hipMemsetAsync(r->m_before.p, 0, r->m_before.bytes, s));
+        HIPCHK(hipMemsetAsync(r->m_cur.p, 0, r->m_cur.bytes, s));
+        HIPCHK(hipMemsetAsync(r->m_prev.p, 0, r->m_prev.bytes, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *p,
+                                      pxfdtd_resident **out) {
+    TRY(check_problem(p));
+    HIPCHK(hipSetDevice(dev->ordinal));
+    auto *r = new pxfdtd_resident();
+    r->dev = dev;
+    r->prec = p->precision;
+    r->esz = p->precision == PXFDTD_F64 ? 8 : 4;
+    r->nx = p->nx;
+    r->ny = p->ny;
+    r->nz = p->nz;
+    r->pitch = pitch_of(p->nx);
+    r->plane = (long long)r->pitch * p->ny;
+    r->nval = r->plane * p->nz;
+    r->ncells = (long long)p->nx * p->ny * p->nz;
+    for (int a = 0; a < 3; a++) {
+        r->periodic[a] = p->periodic[a] ? 1 : 0;
+        r->low[a] = p->pml_low[a];
+        r->high[a] = p->pml_high[a];
+        r->step[a] = p->step[a];
+    }
+    r->dt = p->dt;
+    // tiling defaults: 4 rows per tile, chunks that give the chip many workgroups
+    r->ty = p->ny >= 4 ? 4 : (p->ny >= 2 ? 2 : 1);
+    {
+        const long long tiles = (long long)((p->nx + 63) / 64) * ((p->ny + r->ty - 1) / r->ty);
+        int kz = p->nz;
+        while (kz > 8 && tiles * ((p->nz + kz - 1) / kz) < 2048) kz = (kz + 1) / 2;
+        r->kz = std::max(1, kz);
+    }
+    hipError_t e = hipStreamCreate(&r->stream);
+    if (e != hipSuccess) {
+        delete r;
+        return px_fail(PXFDTD_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    (void)hipEventCreate(&r->ev0);
+    (void)hipEventCreate(&r->ev1);
+    int rc = r->prec == PXFDTD_F64 ? build<double>(r, p) : build<float>(r, p);
+    if (rc != PXFDTD_OK) {
+        std::string msg = g_error;
+        pxfdtd_resident_destroy(r);
+        g_error = msg;
+        return rc;
+    }
+    *out = r;
+    return PXFDTD_OK;
+}
+
+extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
+    if (!r) return;
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    if (r->ev0) (void)hipEventDestroy(r->ev0);
+    if (r->ev1) (void)hipEventDestroy(r->ev1);
+    for (auto e : r->kev) (void)hipEventDestroy(e);
+    hipStream_t s = r->stream;
+    specs().erase(r);
+    delete r;  // buffers free themselves
+    if (s) (void)hipStreamDestroy(s);
+}
+
+extern "C" int64_t pxfdtd_resident_psi_len(const pxfdtd_resident *r) { return r->psi_len; }
+extern "C" int32_t pxfdtd_resident_coefficient_mode(const pxfdtd_resident *r) { return r->mode; }
+extern "C" double pxfdtd_resident_last_kernel_ms(const pxfdtd_resident *r) { return r->last_ms; }
+
+extern "C" int pxfdtd_resident_synchronize(pxfdtd_resident *r) {
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return PXFDTD_OK;
+}
+
+// ------------------------------------------------------------------- upload / download
+template <typename T>
+static int upload_fields(pxfdtd_resident *r, const double *e, const double *h, const double *psi) {
+    const long long N = r->ncells;
+    std::vector<T> buf(r->nval, T(0));
+    for (int q = 0; q < 6; q++) {
+        const double *src = (q < 3 ? e : h);
+        if (!src) continue;
+        src += (q % 3) * N;
+        for (long long v = 0; v < N; v++) {
+            const long long i = v % r->nx, rest = v / r->nx;
+            buf[(rest / r->ny) * r->plane + (rest % r->ny) * r->pitch + i] = (T)src[v];
+        }
+        HIPCHK(hipMemcpyAsync(r->field[r->p][q].p, buf.data(), r->nval * sizeof(T),
+                              hipMemcpyHostToDevice, r->stream));
+        HIPCHK(hipStreamSynchronize(r->stream));
+    }
+    if (r->psi_len) {
+        std::vector<T> ps(r->psi_len, T(0));
+        if (psi)
+            for (long long q = 0; q < r->psi_len; q++) ps[q] = (T)psi[q];
+        HIPCHK(hipMemcpyAsync(r->psi[r->p].p, ps.data(), r->psi_len * sizeof(T),
+                              hipMemcpyHostToDevice, r->stream));
+        HIPCHK(hipStreamSynchronize(r->stream));
+    }
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_upload(pxfdtd_resident *r, const double *e, const double *h,
+                                      const double *psi) {
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    return r->prec == PXFDTD_F64 ? upload_fields<double>(r, e, h, psi)
+                                 : upload_fields<float>(r, e, h, psi);
+}
+
+template <typename T>
+static int download_fields(pxfdtd_resident *r, double *e, double *h, double *psi) {
+    const long long N = r->ncells;
+    std::vector<T> buf(r->nval);
+    HIPCHK(hipStreamSynchronize(r->stream));
+    for (int q = 0; q < 6; q++) {
+        double *dst = (q < 3 ? e : h);
+        if (!dst) continue;
+        dst += (q % 3) * N;
+        HIPCHK(hipMemcpyAsync(buf.data(), r->field[r->p][q].p, r->nval * sizeof(T),
+                              hipMemcpyDeviceToHost, r->stream));
+        HIPCHK(hipStreamSynchronize(r->stream));
+        for (long long v = 0; v < N; v++) {
+            const long long i = v % r->nx, rest = v / r->nx;
+            dst[v] = (double)buf[(rest / r->ny) * r->plane + (rest % r->ny) * r->pitch + i];
+        }
+    }
+    if (psi && r->psi_len) {
+        std::vector<T> ps(r->psi_len);
+        HIPCHK(hipMemcpyAsync(ps.data(), r->psi[r->p].p, r->psi_len * sizeof(T),
+                              hipMemcpyDeviceToHost, r->stream));
+        HIPCHK(hipStreamSynchronize(r->stream));
+        for (long long q = 0; q < r->psi_len; q++) psi[q] = (double)ps[q];
+    }
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_download(pxfdtd_resident *r, double *e, double *h, double *psi) {
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    return r->prec == PXFDTD_F64 ? download_fields<double>(r, e, h, psi)
+                                 : download_fields<float>(r, e, h, psi);
+}
+
+// ------------------------------------------------------------------------- injections
+// Each tile's list holds the injections on its own values and on its halo (the row below and
+// the column left of it, where H is recomputed), sorted by plane, each plane's in the
+// caller's order: a value's injections apply in that order.
+static int build_injection_lists(pxfdtd_resident *r) {
+    r->have_inj = !r->inj_host.empty();
+    if (!r->have_inj) return PXFDTD_OK;
+    const int tx = (r->nx + TX - 1) / TX, tyn = (r->ny + r->ty - 1) / r->ty;
+    const long long tiles = (long long)tx * tyn;
+    struct Item { long long tile; int k; long long seq; Injection in; };
+    std::vector<Item> items;
+    const long long N = r->ncells;
+    for (size_t q = 0; q < r->inj_host.size(); q++) {
+        const pxfdtd_injection &s = r->inj_host[q];
+        const long long v = s.index;
+        const int i = (int)(v % r->nx), j = (int)((v / r->nx) % r->ny), k = (int)(v / ((long long)r->nx * r->ny));
+        Injection in;
+        in.k = k; in.i = i; in.j = j; in.code = s.field * 4 + s.component; in.emitter = s.emitter;
+        in.pad = 0; in.a_re = s.a_re; in.a_im = s.a_im;
+        (void)N;
+        const int bx = i / TX, by = j / r->ty;
+        items.push_back({(long long)by * tx + bx, k, (long long)q, in});
+        if (s.field == PXFDTD_H) {
+            // halo row of the tile above: j == j0' - 1
+            int jn = j + 1, byn = -1;
+            if (jn < r->ny && jn % r->ty == 0) byn = jn / r->ty;
+            else if (jn == r->ny && r->periodic[1]) byn = 0;
+            if (byn >= 0 && byn != by) items.push_back({(long long)byn * tx + bx, k, (long long)q, in});
+            // halo column of the tile to the right: i == i0' - 1
+            int in_ = i + 1, bxn = -1;
+            if (in_ < r->nx && in_ % TX == 0) bxn = in_ / TX;
+            else if (in_ == r->nx && r->periodic[0]) bxn = 0;
+            if (bxn >= 0 && bxn != bx) items.push_back({(long long)by * tx + bxn, k, (long long)q, in});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+        return a.tile != b.tile ? a.tile < b.tile : (a.k != b.k ? a.k < b.k : a.seq < b.seq);
+    });
+    // a tile's own value and its halo are different positions, so an entry never appears
+    // twice for one value within a tile
+    std::vector<int32_t> off(tiles * (r->nz + 1), 0);
+    std::vector<Injection> flat(items.size());
+    size_t q = 0;
+    for (long long t = 0; t < tiles; t++)
+        for (int k = 0; k <= r->nz; k++) {
+            off[t * (r->nz + 1) + k] = (int32_t)q;
+            if (k == r->nz) break;
+            while (q < items.size() && items[q].tile == t && items[q].k == k) {
+                flat[q] = items[q].in;
+                q++;
+            }
+        }
+    TRY(r->inj_off.alloc(off.size() * sizeof(int32_t)));
+    TRY(r->inj.alloc(std::max<size_t>(flat.size(), 1) * sizeof(Injection)));
+    HIPCHK(hipMemcpyAsync(r->inj_off.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, r->stream));
+    if (!flat.empty())
+        HIPCHK(hipMemcpyAsync(r->inj.p, flat.data(), flat.size() * sizeof(Injection), hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_injection *inj,
+                                              int64_t n, int32_t emitters) {
+    for (int64_t q = 0; q < n; q++) {
+        const pxfdtd_injection &s = inj[q];
+        if (s.field < 0 || s.field > 1 || s.component < 0 || s.component > 2 || s.index < 0 ||
+            s.index >= r->ncells || s.emitter < 0 || s.emitter >= emitters)
+            return px_fail(PXFDTD_ERR_INVALID, "injection %lld is out of range", (long long)q);
+    }
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    r->inj_host.assign(inj, inj + n);
+    r->emitters = emitters;
+    return build_injection_lists(r);
+}
+
+extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes) {
+    if (rows) {
+        if (rows != 1 && rows != 2 && rows != 4 && rows != 8)
+            return px_fail(PXFDTD_ERR_INVALID, "rows per tile must be 1, 2, 4 or 8, got %d", rows);
+        r->ty = rows;
+    }
+    if (planes) {
+        if (planes < 1) return px_fail(PXFDTD_ERR_INVALID, "planes per chunk must be >= 1");
+        r->kz = std::min(planes, r->nz);
+    }
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    return build_injection_lists(r);
+}
+
+extern "C" int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes) {
+    *rows = r->ty;
+    *planes = r->kz;
+    return PXFDTD_OK;
+}
+
+// ---------------------------------------------------------------- probes and transforms
+static int build_taps(pxfdtd_resident *r, const std::vector<int64_t> &index,
+                      const std::vector<int8_t> &which) {
+    r->ntaps = (long long)index.size();
+    if (!r->ntaps) return PXFDTD_OK;
+    TRY(r->tap_index.alloc(index.size() * 8));
+    TRY(r->tap_which.alloc(which.size()));
+    HIPCHK(hipMemcpyAsync(r->tap_index.p, index.data(), index.size() * 8, hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipMemcpyAsync(r->tap_which.p, which.data(), which.size(), hipMemcpyHostToDevice, r->stream));
+    for (int c = 0; c < 2; c++) {
+        void *ptrs[6];
+        for (int q = 0; q < 6; q++) ptrs[q] = r->field[c][q].p;
+        TRY(r->tap_fields[c].alloc(sizeof ptrs));
+        HIPCHK(hipMemcpyAsync(r->tap_fields[c].p, ptrs, sizeof ptrs, hipMemcpyHostToDevice, r->stream));
+    }
+    TRY(r->staging.alloc((size_t)r->block_steps * r->ntaps * r->esz));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return PXFDTD_OK;
+}
+
+struct TapSpec {
+    std::vector<pxfdtd_probe> probes;
+    std::vector<pxfdtd_series> series;
+    std::vector<int32_t> monitor_nf;
+};
+
+static int rebuild_taps(pxfdtd_resident *r, const TapSpec &t) {
+    std::vector<int64_t> index;
+    std::vector<int8_t> which;
+    for (const auto &pr : t.probes) {
+        const long long v = pr.index;
+        const long long i = v % r->nx, rest = v / r->nx;
+        index.push_back((rest / r->ny) * r->plane + (rest % r->ny) * r->pitch + i);
+        which.push_back((int8_t)(pr.field * 3 + pr.component));
+    }
+    r->nprobes = (int)t.probes.size();
+    std::vector<int32_t> series_of;
+    std::vector<SeriesInfo> info;
+    std::vector<int32_t> mbase(t.monitor_nf.size() + 1, 0);
+    for (size_t m = 0; m < t.monitor_nf.size(); m++) mbase[m + 1] = mbase[m] + 2 * t.monitor_nf[m];
+    r->slots = mbase.back();
+    long long sums_off = 0, base_v = 0;
+    for (size_t q = 0; q < t.series.size(); q++) {
+        const pxfdtd_series &s = t.series[q];
+        SeriesInfo in;
+        in.base_v = base_v;
+        in.volume = (long long)(s.hi[0] - s.lo[0]) * (s.hi[1] - s.lo[1]) * (s.hi[2] - s.lo[2]);
+        in.sums_off = sums_off;
+        in.nf = t.monitor_nf[s.monitor];
+        in.slot = mbase[s.monitor] + (s.field == PXFDTD_H ? in.nf : 0);
+        info.push_back(in);
+        for (int k = s.lo[2]; k < s.hi[2]; k++)
+            for (int j = s.lo[1]; j < s.hi[1]; j++)
+                for (int i = s.lo[0]; i < s.hi[0]; i++) {
+                    index.push_back((long long)k * r->plane + (long long)j * r->pitch + i);
+                    which.push_back((int8_t)(s.field * 3 + s.component));
+                    series_of.push_back((int32_t)q);
+                }
+        base_v += in.volume;
+        sums_off += in.volume * in.nf;
+    }
+    r->nvals = base_v;
+    r->sums_len = 2 * sums_off;
+    TRY(build_taps(r, index, which));
+    if (r->nvals) {
+        TRY(r->series_of.alloc(series_of.size() * 4));
+        TRY(r->series_info.alloc(info.size() * sizeof(SeriesInfo)));
+        TRY(r->sums.alloc(r->sums_len * 8));
+        HIPCHK(hipMemcpyAsync(r->series_of.p, series_of.data(), series_of.size() * 4, hipMemcpyHostToDevice, r->stream));
+        HIPCHK(hipMemcpyAsync(r->series_info.p, info.data(), info.size() * sizeof(SeriesInfo), hipMemcpyHostToDevice, r->stream));
+        HIPCHK(hipMemsetAsync(r->sums.p, 0, r->sums_len * 8, r->stream));
+        HIPCHK(hipStreamSynchronize(r->stream));
+    }
+    return PXFDTD_OK;
+}
+
+// the resident keeps its tap specification to rebuild when either part changes
+static std::map<const pxfdtd_resident *, TapSpec> &specs() {
+    static std::map<const pxfdtd_resident *, TapSpec> m;
+    return m;
+}
+
+extern "C" int pxfdtd_resident_set_probes(pxfdtd_resident *r, const pxfdtd_probe *probes, int32_t n) {
+    for (int q = 0; q < n; q++)
+        if (probes[q].index < 0 || probes[q].index >= r->ncells || probes[q].field < 0 ||
+            probes[q].field > 1 || probes[q].component < 0 || probes[q].component > 2)
+            return px_fail(PXFDTD_ERR_INVALID, "probe %d is out of range", q);
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    TapSpec &t = specs()[r];
+    t.probes.assign(probes, probes + n);
+    r->probe_rows = 0;
+    return rebuild_taps(r, t);
+}
+
+extern "C" int pxfdtd_resident_set_series(pxfdtd_resident *r, const pxfdtd_series *series,
+                                          int32_t n, const int32_t *monitor_nf, int32_t monitors) {
+    const int dims[3] = {r->nx, r->ny, r->nz};
+    for (int q = 0; q < n; q++) {
+        const pxfdtd_series &s = series[q];
+        if (s.monitor < 0 || s.monitor >= monitors || s.field < 0 || s.field > 1 ||
+            s.component < 0 || s.component > 2)
+            return px_fail(PXFDTD_ERR_INVALID, "series %d is out of range", q);
+        for (int a = 0; a < 3; a++)
+            if (s.lo[a] < 0 || s.hi[a] > dims[a] || s.lo[a] >= s.hi[a])
+                return px_fail(PXFDTD_ERR_INVALID, "series %d's box is not on the grid", q);
+    }
+    for (int m = 0; m < monitors; m++)
+        if (monitor_nf[m] < 1) return px_fail(PXFDTD_ERR_INVALID, "a monitor needs frequencies");
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    TapSpec &t = specs()[r];
+    t.series.assign(series, series + n);
+    t.monitor_nf.assign(monitor_nf, monitor_nf + monitors);
+    return rebuild_taps(r, t);
+}
+
+extern "C" int32_t pxfdtd_resident_phase_slots(const pxfdtd_resident *r) { return r->slots; }
+extern "C" int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r) { return r->sums_len; }
+
+extern "C" int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_t capacity) {
+    if (capacity < r->sums_len)
+        return px_fail(PXFDTD_ERR_INVALID, "the sums need %lld values", (long long)r->sums_len);
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    if (r->sums_len)
+        HIPCHK(hipMemcpyAsync(out, r->sums.p, r->sums_len * 8, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return PXFDTD_OK;
+}
+
+template <typename T>
+static int read_probes_t(pxfdtd_resident *r, double *out, int64_t capacity, int64_t *steps) {
+    *steps = r->probe_rows;
+    if (!r->nprobes || !r->probe_rows) return PXFDTD_OK;
+    if (capacity < r->probe_rows * r->nprobes)
+        return px_fail(PXFDTD_ERR_INVALID, "the probes need %lld values", (long long)(r->probe_rows * r->nprobes));
+    std::vector<T> buf(r->probe_rows * r->nprobes);
+    HIPCHK(hipMemcpyAsync(buf.data(), r->probe_rec.p, buf.size() * sizeof(T), hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    for (size_t q = 0; q < buf.size(); q++) out[q] = (double)buf[q];
+    r->probe_rows = 0;
+    return PXFDTD_OK;
+}
+
+extern "C" int64_t pxfdtd_resident_probe_steps(const pxfdtd_resident *r) { return r->probe_rows; }
+
+extern "C" int pxfdtd_resident_read_probes(pxfdtd_resident *r, double *out, int64_t capacity,
+                                           int64_t *steps) {
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    return r->prec == PXFDTD_F64 ? read_probes_t<double>(r, out, capacity, steps)
+                                 : read_probes_t<float>(r, out, capacity, steps);
+}
+
+// -------------------------------------------------------------------------- stepping
+template <typename T>
+static StepArgs<T> args_of(const pxfdtd_resident *r) {
+    StepArgs<T> A;
+    memset(&A, 0, sizeof A);
+    A.nx = r->nx; A.ny = r->ny; A.nz = r->nz; A.pitch = r->pitch; A.plane = r->plane;
+    for (int a = 0; a < 3; a++) {
+        A.periodic[a] = r->periodic[a];
+        A.low[a] = r->low[a];
+        A.high[a] = r->high[a];
+        A.step[a] = (T)r->step[a];
+        A.fe[a] = r->fe[a].as<T>();
+        A.fh[a] = r->fh[a].as<T>();
+        for (int s = 0; s < 2; s++)
+            for (int f = 0; f < 2; f++) {
+                A.pb[a][s][f] = r->pb[a][s][f].as<T>();
+                A.pa[a][s][f] = r->pa[a][s][f].as<T>();
+            }
+    }
+    A.tiles_x = (r->nx + TX - 1) / TX;
+    A.tiles_y = (r->ny + r->ty - 1) / r->ty;
+    A.kz = r->kz;
+    A.chunks = (r->nz + r->kz - 1) / r->kz;
+    A.dt = (T)r->dt;
+    A.table_stride = r->table_stride;
+    for (int c = 0; c < 3; c++) {
+        A.ids[c] = r->ids[c].as<uint8_t>();
+        A.ca[c] = r->ca[c].as<T>();
+        A.cb[c] = r->cb[c].as<T>();
+        A.grp[c] = r->has_grp ? r->grp[c].as<int8_t>() : nullptr;
+    }
+    A.table = r->table.as<T>();
+    A.group_of = r->group_of.as<int8_t>();
+    memcpy(A.psi_off, r->psi_off, sizeof A.psi_off);
+    A.inj_off = r->have_inj ? r->inj_off.as<int32_t>() : nullptr;
+    A.inj = r->inj.as<Injection>();
+    A.groups = r->groups.as<MediumGroup>();
+    A.ngroups = r->ngroups;
+    A.max_poles = r->max_poles;
+    A.m_before = r->m_before.as<T>();
+    A.m_cur = r->m_cur.as<T>();
+    A.m_prev = r->m_prev.as<T>();
+    A.mstride = r->nval;
+    return A;
+}
+
+template <typename T, int MODE, int TY>
+static int launch_ty(const StepArgs<T> &A, hipStream_t s, size_t lds) {
+    const int blocks = A.tiles_x * A.tiles_y * A.chunks;
+    hipLaunchKernelGGL((fused_step<T, MODE, TY>), dim3(blocks), dim3(TX, TY + 2), lds, s, A);
+    return PXFDTD_OK;
+}
+
+template <typename T, int MODE>
+static int launch_mode(const StepArgs<T> &A, int ty, hipStream_t s, size_t lds) {
+    switch (ty) {
+    case 1: return launch_ty<T, MODE, 1>(A, s, lds);
+    case 2: return launch_ty<T, MODE, 2>(A, s, lds);
+    case 4: return launch_ty<T, MODE, 4>(A, s, lds);
+    default: return launch_ty<T, MODE, 8>(A, s, lds);
+    }
+}
+
+template <typename T>
+static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const double *phases) {
+    hipStream_t s = r->stream;
+    if (r->emitters && r->have_inj) {
+        if (!waves) return px_fail(PXFDTD_ERR_INVALID, "the injections need their waves");
+        const size_t b = (size_t)count * r->emitters * 2 * 8;
+        TRY(r->waves.reserve(b));
+        HIPCHK(hipMemcpyAsync(r->waves.p, waves, b, hipMemcpyHostToDevice, s));
+    }
+    if (r->nvals) {
+        if (!phases) return px_fail(PXFDTD_ERR_INVALID, "the transforms need their phases");
+        const size_t b = (size_t)count * r->slots * 2 * 8;
+        TRY(r->phases.reserve(b));
+        HIPCHK(hipMemcpyAsync(r->phases.p, phases, b, hipMemcpyHostToDevice, s));
+    }
+    if (r->nprobes) {
+        const long long need = (r->probe_rows + count) * r->nprobes * (long long)sizeof(T);
+        if ((long long)r->probe_rec.bytes < need) {
+            Buf grown;
+            TRY(grown.alloc((size_t)std::max<long long>(need, 2 * (long long)r->probe_rec.bytes)));
+            if (r->probe_rows)
+                HIPCHK(hipMemcpyAsync(grown.p, r->probe_rec.p, r->probe_rows * r->nprobes * sizeof(T),
+                                      hipMemcpyDeviceToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));
+            std::swap(grown.p, r->probe_rec.p);
+            std::swap(grown.bytes, r->probe_rec.bytes);
+        }
+    }
+    const size_t lds = r->mode == 0 ? (size_t)r->table_stride * (6 * sizeof(T) + 3) : 0;
+    int in_block = 0;
+    long long block_first = 0;
+    if (r->profile)
+        while (r->kev.size() < 2 * (size_t)count) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            r->kev.push_back(e);
+        }
+    HIPCHK(hipEventRecord(r->ev0, s));
+    for (uint32_t n = 0; n < count; n++) {
+        StepArgs<T> A = args_of<T>(r);
+        const int p = r->p, q = 1 - p;
+        for (int c = 0; c < 3; c++) {
+            A.e_in[c] = r->field[p][c].as<T>();
+            A.h_in[c] = r->field[p][3 + c].as<T>();
+            A.e_out[c] = r->field[q][c].as<T>();
+            A.h_out[c] = r->field[q][3 + c].as<T>();
+        }
+        A.psi_in = r->psi[p].as<T>();
+        A.psi_out = r->psi[q].as<T>();
+        A.waves = r->waves.as<double>() + (size_t)n * r->emitters * 2;
+        if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
+        if (r->mode == 0)
+            TRY((launch_mode<T, 0>(A, r->ty, s, lds)));
+        else
+            TRY((launch_mode<T, 1>(A, r->ty, s, lds)));
+        HIPCHK(hipGetLastError());
+        if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
+        r->p = q;
+        if (r->ntaps) {
+            const int threads = 256;
+            const long long blocks = (r->ntaps + threads - 1) / threads;
+            hipLaunchKernelGGL((tap_values<T>), dim3((unsigned)blocks), dim3(threads), 0, s,
+                               r->tap_fields[q].as<const T *const>(), r->tap_index.as<int64_t>(),
+                               r->tap_which.as<int8_t>(), r->ntaps,
+                               r->staging.as<T>() + (size_t)in_block * r->ntaps);
+            HIPCHK(hipGetLastError());
+            in_block++;
+            if (in_block == r->block_steps || n + 1 == count) {
+                if (r->nvals) {
+                    const long long blocks2 = (r->nvals + threads - 1) / threads;
+                    hipLaunchKernelGGL((dft_accumulate<T>), dim3((unsigned)blocks2), dim3(threads), 0, s,
+                                       r->staging.as<T>() + r->nprobes, r->ntaps, r->nvals, in_block,
+                                       r->series_of.as<int32_t>(), r->series_info.as<SeriesInfo>(),
+                                       r->phases.as<double>() + (size_t)block_first * r->slots * 2,
+                                       r->slots, r->sums.as<double>());
+                    HIPCHK(hipGetLastError());
+                }
+                if (r->nprobes) {
+                    HIPCHK(hipMemcpy2DAsync(r->probe_rec.as<T>() + (size_t)r->probe_rows * r->nprobes,
+                                            r->nprobes * sizeof(T), r->staging.p, r->ntaps * sizeof(T),
+                                            r->nprobes * sizeof(T), in_block, hipMemcpyDeviceToDevice, s));
+                    r->probe_rows += in_block;
+                }
+                block_first = n + 1;
+                in_block = 0;
+            }
+        }
+    }
+    HIPCHK(hipEventRecord(r->ev1, s));
+    HIPCHK(hipEventSynchronize(r->ev1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    r->last_ms = count ? ms / count : 0.0;
+    if (r->profile) {
+        double sum = 0;
+        for (uint32_t n = 0; n < count; n++) {
+            float k = 0;
+            HIPCHK(hipEventElapsedTime(&k, r->kev[2 * n], r->kev[2 * n + 1]));
+            sum += k;
+        }
+        r->kernel_ms = sum / count;
+    }
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_set_profiling(pxfdtd_resident *r, int32_t on) {
+    r->profile = on != 0;
+    return PXFDTD_OK;
+}
+
+extern "C" double pxfdtd_resident_kernel_ms(const pxfdtd_resident *r) { return r->kernel_ms; }
+
+extern "C" int pxfdtd_resident_step(pxfdtd_resident *r, uint32_t count, const double *waves,
+                                    const double *phases) {
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    if (count == 0) return PXFDTD_OK;
+    return r->prec == PXFDTD_F64 ? step_t<double>(r, count, waves, phases)
+                                 : step_t<float>(r, count, waves, phases);
+}

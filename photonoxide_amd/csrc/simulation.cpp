@@ -1,0 +1,954 @@
+// Level 2 of the C ABI (include/pxfdtd.h): the host mirror of photonoxide's fdtd::Simulation
+// (src/fdtd.rs:470-1590). It owns the problem in FP64 as the reference does: the averaged
+// permittivity, dt, the walls, the CPML's coefficients and slabs, conductivity, conductors,
+// sources, currents, dipoles, probes, transform monitors and dispersive media. It steps only on
+// a device, through level 1 (pxfdtd_resident_*): there is no CPU stepping here, and running
+// without a device is an error.
+//
+// Each set-up computation follows the reference's operation order (cited per function), so the
+// tables it hands the device are the reference's bits.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pxfdtd.h"
+#include "portable.hpp"
+
+using pxfdtd::C64;
+
+int px_fail(int code, const char *fmt, ...);  // resident.hip
+
+#define TRY(x)                            \
+    do {                                  \
+        int rc_ = (x);                    \
+        if (rc_ != PXFDTD_OK) return rc_; \
+    } while (0)
+
+namespace {
+
+constexpr double TAU = 6.283185307179586;
+
+struct Slab {
+    int field, comp, axis, side, from, count;
+    std::vector<double> b, a, psi;
+};
+
+struct SourceRec {
+    int field, comp;
+    long long r;
+    pxfdtd_waveform w;
+};
+
+struct CurrentRec {
+    int field;
+    std::vector<int32_t> comps;
+    std::vector<int64_t> r;
+    std::vector<C64> amps;
+    pxfdtd_waveform w;
+};
+
+struct ProbeRec {
+    int field, comp;
+    long long r;
+    std::vector<double> values;
+};
+
+struct DftRec {
+    std::vector<double> freqs;
+    int lo[3], hi[3];
+    std::vector<double> acc;  // host-held sums of earlier device layouts, 6 series concatenated
+};
+
+struct Group {
+    std::vector<int32_t> comp;
+    std::vector<int64_t> r;
+    double chi;
+    int poles;
+    double coef[4][3];
+};
+
+double smoothstep(double t, double ramp) {
+    if (ramp > 0.0) {
+        double s = std::clamp(t / ramp, 0.0, 1.0);
+        return s * s * (3.0 - 2.0 * s);
+    }
+    return 1.0;
+}
+
+// Waveform::at (src/fdtd.rs:399-423)
+double wave_at(const pxfdtd_waveform &w, double t) {
+    const double omega = TAU * w.frequency;
+    if (w.kind == 0) {
+        const double u = (t - w.delay) / w.width;
+        double s, c;
+        pxfdtd::psincos(omega * (t - w.delay), s, c);
+        return pxfdtd::pexp(-u * u) * s;
+    }
+    if (w.kind == 1) {
+        const double u = (t - w.delay) / w.width;
+        return -u * pxfdtd::pexp(-u * u) * std::sqrt(2.0 * 2.718281828459045);
+    }
+    const C64 e = pxfdtd::pcexp({0.0, -omega * t});
+    return smoothstep(t, w.ramp) * (w.amp_re * e.re - w.amp_im * e.im);
+}
+
+// Waveform::complex_at (src/fdtd.rs:329-351)
+C64 wave_complex_at(const pxfdtd_waveform &w, double t) {
+    const double omega = TAU * w.frequency;
+    if (w.kind == 0) {
+        const double u = (t - w.delay) / w.width;
+        const double g = pxfdtd::pexp(-u * u);
+        const C64 e = pxfdtd::pcexp({0.0, -omega * (t - w.delay)});
+        return {0.0 * e.re - g * e.im, 0.0 * e.im + g * e.re};
+    }
+    if (w.kind == 1) return {wave_at(w, t), 0.0};
+    const double s = smoothstep(t, w.ramp);
+    const C64 e = pxfdtd::pcexp({0.0, -omega * t});
+    const double ar = s * w.amp_re, ai = s * w.amp_im;
+    return {ar * e.re - ai * e.im, ar * e.im + ai * e.re};
+}
+
+double scene_value(const pxfdtd_scene *sc, int comp, double x, double y, double z) {
+    double v = sc->background[comp];
+    for (int q = 0; q < sc->n; q++) {
+        const pxfdtd_shape &s = sc->shapes[q];
+        bool in;
+        if (s.kind == 0)
+            in = s.lo[0] < x && x < s.hi[0] && s.lo[1] < y && y < s.hi[1] && s.lo[2] < z && z < s.hi[2];
+        else {
+            const double dx = x - s.lo[0], dy = y - s.lo[1];
+            in = dx * dx + dy * dy < s.hi[0] * s.hi[0] && s.lo[2] < z && z < s.hi[2];
+        }
+        if (in) v = s.value[comp];
+    }
+    return v;
+}
+
+// the permittivity of E's value's cell from samples^3 points, harmonic along the component,
+// arithmetic across, with num-complex's division of a real by a complex, e / (e e + 0 0)
+// (src/fdfd/three.rs:563-588)
+double averaged(const pxfdtd_scene *sc, int comp, const double centre[3], const double h[3]) {
+    const int samples = 8;
+    const double n = samples;
+    const int a = comp, b = (comp + 1) % 3, c = (comp + 2) % 3;
+    double sum = 0.0, p[3] = {centre[0], centre[1], centre[2]};
+    for (int sb = 0; sb < samples; sb++) {
+        p[b] = centre[b] + ((sb + 0.5) / n - 0.5) * h[b];
+        for (int sc_ = 0; sc_ < samples; sc_++) {
+            p[c] = centre[c] + ((sc_ + 0.5) / n - 0.5) * h[c];
+            double inverse = 0.0;
+            for (int sa = 0; sa < samples; sa++) {
+                p[a] = centre[a] + ((sa + 0.5) / n - 0.5) * h[a];
+                const double e = scene_value(sc, comp, p[0], p[1], p[2]);
+                inverse += 1.0 * e / (e * e + 0.0 * 0.0);
+            }
+            sum += n * inverse / (inverse * inverse + 0.0 * 0.0);
+        }
+    }
+    return sum / (n * n);
+}
+
+}  // namespace
+
+struct pxfdtd_simulation {
+    pxfdtd_grid g;
+    pxfdtd_boundaries b;
+    int n[3];
+    long long N;
+    double dt;
+    long long steps = 0;
+    std::vector<double> e, h, ca, cb;
+    std::vector<double> kn[3], kh[3], fe[3], fh[3];
+    std::vector<Slab> slabs;
+    std::vector<double> cpml_b[3][2][2], cpml_a[3][2][2];
+    std::vector<SourceRec> sources;
+    std::vector<CurrentRec> currents;
+    std::vector<ProbeRec> probes;
+    std::vector<DftRec> dfts;
+    std::vector<Group> groups;
+    std::vector<int32_t> member;  // [3][N] group or -1, when media exist
+    // the device
+    pxfdtd_device *dev = nullptr;
+    int precision = PXFDTD_F32;
+    pxfdtd_resident *res = nullptr;
+    bool device_newer = false;  // the device's fields are newer than the host's
+    bool host_newer = false;    // the host's fields changed since the last upload
+    bool inj_dirty = true, taps_dirty = true;
+    int emitters = 0;
+
+    void e_position(int comp, long long r, double p[3]) const {
+        const int i = (int)(r % g.nx), j = (int)((r / g.nx) % g.ny), k = (int)(r / ((long long)g.nx * g.ny));
+        p[0] = g.x0 + i * g.dx;
+        p[1] = g.y0 + j * g.dy;
+        p[2] = g.z0 + k * g.dz;
+        const double hs[3] = {g.dx, g.dy, g.dz};
+        p[comp] += 0.5 * hs[comp];
+    }
+};
+
+namespace {
+
+int check(const pxfdtd_grid *g, const pxfdtd_boundaries *b, double courant) {
+    const int n[3] = {g->nx, g->ny, g->nz};
+    if ((long long)g->nx * g->ny * g->nz <= 0 || !(g->dx > 0) || !(g->dy > 0) || !(g->dz > 0))
+        return px_fail(PXFDTD_ERR_INVALID, "the grid needs cells and positive steps");
+    if (!(b->reflection > 0 && b->reflection < 1) || !(b->order >= 0 && std::isfinite(b->order)) ||
+        !(b->kappa >= 1 && std::isfinite(b->kappa)) || !(b->alpha >= 0 && std::isfinite(b->alpha)) ||
+        (b->has_sigma && !(b->sigma >= 0 && std::isfinite(b->sigma))))
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the CPML needs R in (0, 1), an order >= 0, kappa >= 1, alpha >= 0 and sigma >= 0");
+    for (int a = 0; a < 3; a++)
+        if (!b->periodic[a] && (b->low[a] < 0 || b->high[a] < 0 || b->low[a] + b->high[a] >= n[a]))
+            return px_fail(PXFDTD_ERR_INVALID, "CPMLs of %d and %d cells leave nothing of %d cells",
+                           b->low[a], b->high[a], n[a]);
+    if (!(courant > 0 && courant <= 1))
+        return px_fail(PXFDTD_ERR_INVALID, "the Courant number must be in (0, 1], got %g", courant);
+    return PXFDTD_OK;
+}
+
+// src/fdtd.rs:799-869
+void build_cpml(pxfdtd_simulation *s) {
+    const pxfdtd_boundaries &cp = s->b;
+    const double hs[3] = {s->g.dx, s->g.dy, s->g.dz};
+    for (int a = 0; a < 3; a++) {
+        s->kn[a].assign(s->n[a], 1.0);
+        s->kh[a].assign(s->n[a], 1.0);
+    }
+    for (int axis = 0; axis < 3; axis++) {
+        if (cp.periodic[axis]) continue;
+        const int low = cp.low[axis], high = cp.high[axis], n = s->n[axis];
+        const double h = hs[axis];
+        auto profile = [&](double depth, int cells, double &sig, double &kap) {
+            const double d = cells * h;
+            const double x = std::pow(std::clamp(depth / cells, 0.0, 1.0), cp.order);
+            const double smax = cp.has_sigma ? cp.sigma : (cp.order + 1.0) * (-std::log(cp.reflection)) / (2.0 * d);
+            sig = smax * x;
+            kap = 1.0 + (cp.kappa - 1.0) * x;
+        };
+        auto at = [&](double pos, double &sig, double &kap) {
+            if (low > 0 && pos < low)
+                profile(low - pos, low, sig, kap);
+            else if (high > 0 && pos > n - high)
+                profile(pos - (n - high), high, sig, kap);
+            else {
+                sig = 0.0;
+                kap = 1.0;
+            }
+        };
+        for (int m = 0; m < n; m++) {
+            double sg, kp;
+            at(m, sg, kp);
+            s->kn[axis][m] = 1.0 / kp;
+            at(m + 0.5, sg, kp);
+            s->kh[axis][m] = 1.0 / kp;
+        }
+        const int froms[2] = {0, n - high}, counts[2] = {low, high};
+        for (int side = 0; side < 2; side++) {
+            if (!counts[side]) continue;
+            for (int f = 0; f < 2; f++) {
+                std::vector<double> bb, aa;
+                for (int m = froms[side]; m < froms[side] + counts[side]; m++) {
+                    double sg, kp;
+                    at(m + (f == 0 ? 0.0 : 0.5), sg, kp);
+                    const double bv = pxfdtd::pexp(-(sg / kp + cp.alpha) * s->dt);
+                    const double den = sg * kp + kp * kp * cp.alpha;
+                    bb.push_back(bv);
+                    aa.push_back(sg > 0.0 && den > 0.0 ? sg * (bv - 1.0) / den : 0.0);
+                }
+                s->cpml_b[axis][side][f] = bb;
+                s->cpml_a[axis][side][f] = aa;
+                for (int cc = 1; cc <= 2; cc++) {
+                    Slab sl;
+                    sl.field = f;
+                    sl.comp = (axis + cc) % 3;
+                    sl.axis = axis;
+                    sl.side = side;
+                    sl.from = froms[side];
+                    sl.count = counts[side];
+                    sl.b = bb;
+                    sl.a = aa;
+                    sl.psi.assign((size_t)counts[side] * (s->N / n), 0.0);
+                    s->slabs.push_back(std::move(sl));
+                }
+            }
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        s->fe[a].resize(s->n[a]);
+        s->fh[a].resize(s->n[a]);
+        for (int m = 0; m < s->n[a]; m++) {
+            s->fe[a][m] = s->kn[a][m] / hs[a];
+            s->fh[a][m] = s->kh[a][m] / hs[a];
+        }
+    }
+}
+
+// src/fdtd.rs:680-796 with the permittivity per value from `eps(c, r)`
+template <typename F>
+int make(const pxfdtd_grid *g, const pxfdtd_boundaries *b, double courant, F eps,
+         pxfdtd_simulation **out) {
+    TRY(check(g, b, courant));
+    auto *s = new pxfdtd_simulation();
+    s->g = *g;
+    s->b = *b;
+    s->n[0] = g->nx;
+    s->n[1] = g->ny;
+    s->n[2] = g->nz;
+    s->N = (long long)g->nx * g->ny * g->nz;
+    const double hs[3] = {g->dx, g->dy, g->dz};
+    double sum = 0.0;
+    for (int a = 0; a < 3; a++)
+        if (!(b->periodic[a] && s->n[a] == 1)) sum += pxfdtd::ppowi(hs[a], -2);
+    if (sum == 0.0) {
+        delete s;
+        return px_fail(PXFDTD_ERR_INVALID, "the field varies along no axis");
+    }
+    s->dt = courant / std::sqrt(sum);
+    const long long N = s->N;
+    s->e.assign(3 * N, 0.0);
+    s->h.assign(3 * N, 0.0);
+    s->ca.assign(3 * N, 1.0);
+    s->cb.assign(3 * N, 0.0);
+    int bad = 0;
+    for (int c = 0; c < 3; c++) {
+        // each value's average alone: the same bits on any number of threads
+#pragma omp parallel for schedule(static) reduction(| : bad)
+        for (long long r = 0; r < N; r++) {
+            const double v = eps(c, r);
+            if (!(std::isfinite(v) && v > 0)) bad |= 1;
+            s->cb[c * N + r] = s->dt / v;
+        }
+    }
+    if (bad) {
+        delete s;
+        return px_fail(PXFDTD_ERR_INVALID, "the permittivity must be finite and positive");
+    }
+    for (int c = 0; c < 3; c++)
+        for (int axis = 0; axis < 3; axis++) {
+            if (axis == c || b->periodic[axis]) continue;
+            for (long long r = 0; r < N; r++) {
+                const long long at[3] = {r % g->nx, (r / g->nx) % g->ny, r / ((long long)g->nx * g->ny)};
+                if (at[axis] == 0) s->ca[c * N + r] = s->cb[c * N + r] = 0.0;
+            }
+        }
+    build_cpml(s);
+    *out = s;
+    return PXFDTD_OK;
+}
+
+}  // namespace
+
+extern "C" int pxfdtd_simulation_new(const pxfdtd_grid *g, const pxfdtd_boundaries *b,
+                                     const pxfdtd_scene *eps, double courant,
+                                     pxfdtd_simulation **out) {
+    if (!eps) return px_fail(PXFDTD_ERR_INVALID, "a scene is required");
+    const double hs[3] = {g->dx, g->dy, g->dz};
+    const pxfdtd_grid gg = *g;
+    return make(g, b, courant, [&](int c, long long r) {
+        const int i = (int)(r % gg.nx), j = (int)((r / gg.nx) % gg.ny), k = (int)(r / ((long long)gg.nx * gg.ny));
+        double p[3] = {gg.x0 + i * gg.dx, gg.y0 + j * gg.dy, gg.z0 + k * gg.dz};
+        p[c] += 0.5 * hs[c];
+        return averaged(eps, c, p, hs);
+    }, out);
+}
+
+extern "C" int pxfdtd_simulation_new_eps(const pxfdtd_grid *g, const pxfdtd_boundaries *b,
+                                         const double *eps3n, double courant,
+                                         pxfdtd_simulation **out) {
+    const long long N = (long long)g->nx * g->ny * g->nz;
+    return make(g, b, courant, [&](int c, long long r) { return eps3n[c * N + r]; }, out);
+}
+
+extern "C" void pxfdtd_simulation_destroy(pxfdtd_simulation *s) {
+    if (!s) return;
+    if (s->res) pxfdtd_resident_destroy(s->res);
+    delete s;
+}
+
+static void drop_resident(pxfdtd_simulation *s);
+static int sync_host(pxfdtd_simulation *s);
+
+// the sums the device has taken since its series were set, added to the host's; the device's
+// restart from zero when its series are set next
+static int bank_sums(pxfdtd_simulation *s) {
+    if (!s->res || s->taps_dirty || s->dfts.empty()) return PXFDTD_OK;
+    std::vector<double> sums(pxfdtd_resident_sums_len(s->res));
+    TRY(pxfdtd_resident_read_sums(s->res, sums.data(), (int64_t)sums.size()));
+    size_t at = 0;
+    for (auto &d : s->dfts)
+        for (auto &v : d.acc) v += sums[at++];
+    s->taps_dirty = true;
+    return PXFDTD_OK;
+}
+
+// src/fdtd.rs:949-992
+extern "C" int pxfdtd_simulation_conductivity(pxfdtd_simulation *s, const pxfdtd_scene *sigma) {
+    if (!s->groups.empty())
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the conductivity comes before the dispersive media: conductivity, then medium");
+    TRY(sync_host(s));
+    const long long N = s->N;
+    for (int c = 0; c < 3; c++)
+        for (long long r = 0; r < N; r++) {
+            double &cb = s->cb[c * N + r];
+            if (cb == 0.0) continue;
+            double p[3];
+            s->e_position(c, r, p);
+            const double sg = scene_value(sigma, 0, p[0], p[1], p[2]);
+            if (!(std::isfinite(sg) && sg >= 0))
+                return px_fail(PXFDTD_ERR_INVALID, "the conductivity must be finite and >= 0, got %g", sg);
+            const double half = sg * cb / 2.0;
+            s->ca[c * N + r] = (1.0 - half) / (1.0 + half);
+            cb /= 1.0 + half;
+        }
+    drop_resident(s);
+    return PXFDTD_OK;
+}
+
+static void media_remove(pxfdtd_simulation *s, int c, long long r) {
+    if (s->member.empty()) return;
+    const int g = s->member[c * s->N + r];
+    if (g < 0) return;
+    s->member[c * s->N + r] = -1;
+    Group &G = s->groups[g];
+    size_t w = 0;
+    for (size_t v = 0; v < G.r.size(); v++) {
+        if (G.comp[v] == c && G.r[v] == r) continue;
+        G.comp[w] = G.comp[v];
+        G.r[w] = G.r[v];
+        w++;
+    }
+    G.comp.resize(w);
+    G.r.resize(w);
+}
+
+// src/fdtd.rs:995-1010
+extern "C" int pxfdtd_simulation_conductor(pxfdtd_simulation *s, int32_t comp, int32_t i,
+                                           int32_t j, int32_t k) {
+    if (comp < 0 || comp > 2 || i < 0 || j < 0 || k < 0 || i >= s->n[0] || j >= s->n[1] || k >= s->n[2])
+        return px_fail(PXFDTD_ERR_INVALID, "(%d, %d, %d) is outside the grid", i, j, k);
+    TRY(sync_host(s));
+    const long long r = ((long long)k * s->n[1] + j) * s->n[0] + i;
+    s->ca[comp * s->N + r] = 0.0;
+    s->cb[comp * s->N + r] = 0.0;
+    s->e[comp * s->N + r] = 0.0;
+    media_remove(s, comp, r);
+    drop_resident(s);
+    return PXFDTD_OK;
+}
+
+static int inside(const pxfdtd_simulation *s, int i, int j, int k) {
+    if (i >= 0 && j >= 0 && k >= 0 && i < s->n[0] && j < s->n[1] && k < s->n[2]) return PXFDTD_OK;
+    return px_fail(PXFDTD_ERR_INVALID, "(%d, %d, %d) is outside the %d x %d x %d grid", i, j, k,
+                   s->n[0], s->n[1], s->n[2]);
+}
+
+static int check_waveform(const pxfdtd_waveform *w) {
+    if (w->kind < 0 || w->kind > 2) return px_fail(PXFDTD_ERR_INVALID, "unknown waveform kind %d", w->kind);
+    if (w->kind != 1 && !(w->frequency > 0 && std::isfinite(w->frequency)))
+        return px_fail(PXFDTD_ERR_INVALID, "a waveform's frequency must be positive and finite");
+    if (w->kind != 2 && !(w->width > 0))
+        return px_fail(PXFDTD_ERR_INVALID, "a pulse's width must be positive");
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_add_source(pxfdtd_simulation *s, int32_t field, int32_t comp,
+                                            int32_t i, int32_t j, int32_t k,
+                                            const pxfdtd_waveform *w) {
+    TRY(inside(s, i, j, k));
+    TRY(check_waveform(w));
+    if (field < 0 || field > 1 || comp < 0 || comp > 2)
+        return px_fail(PXFDTD_ERR_INVALID, "a source needs a field and a component");
+    s->sources.push_back({field, comp, ((long long)k * s->n[1] + j) * s->n[0] + i, *w});
+    s->inj_dirty = true;
+    return PXFDTD_OK;
+}
+
+// src/fdtd/sources.rs:223-240
+extern "C" int pxfdtd_simulation_add_current(pxfdtd_simulation *s, int32_t field, int64_t n,
+                                             const int32_t *comps, const int64_t *index,
+                                             const double *amps, const pxfdtd_waveform *w) {
+    TRY(check_waveform(w));
+    if (field < 0 || field > 1) return px_fail(PXFDTD_ERR_INVALID, "a current needs a field");
+    CurrentRec c;
+    c.field = field;
+    c.w = *w;
+    for (int64_t v = 0; v < n; v++) {
+        if (comps[v] < 0 || comps[v] > 2 || index[v] < 0 || index[v] >= s->N)
+            return px_fail(PXFDTD_ERR_INVALID, "a current's value %lld is outside the grid", (long long)v);
+        if (!std::isfinite(amps[2 * v]) || !std::isfinite(amps[2 * v + 1]))
+            return px_fail(PXFDTD_ERR_INVALID, "a current's amplitude must be finite");
+        c.comps.push_back(comps[v]);
+        c.r.push_back(index[v]);
+        c.amps.push_back({amps[2 * v], amps[2 * v + 1]});
+    }
+    s->currents.push_back(std::move(c));
+    s->inj_dirty = true;
+    return PXFDTD_OK;
+}
+
+// a dipole restricted to the grid by its trilinear weights (src/fdtd/sources.rs:248-315)
+extern "C" int pxfdtd_simulation_add_dipole(pxfdtd_simulation *s, int32_t field, int32_t comp,
+                                            const double position[3], double amp_re,
+                                            double amp_im, const pxfdtd_waveform *w) {
+    TRY(check_waveform(w));
+    if (!std::isfinite(amp_re) || !std::isfinite(amp_im) || !std::isfinite(position[0]) ||
+        !std::isfinite(position[1]) || !std::isfinite(position[2]))
+        return px_fail(PXFDTD_ERR_INVALID, "a dipole must be finite");
+    const double hs[3] = {s->g.dx, s->g.dy, s->g.dz};
+    const double origin[3] = {s->g.x0, s->g.y0, s->g.z0};
+    struct W { long long m; double w; };
+    W along[3][2];
+    double volume = 1.0;
+    for (int axis = 0; axis < 3; axis++) {
+        const int n = s->n[axis];
+        const bool varies = !(n == 1 && s->b.periodic[axis]);
+        if (!varies) {
+            along[axis][0] = {0, 1.0};
+            along[axis][1] = {0, 0.0};
+            continue;
+        }
+        volume *= hs[axis];
+        const bool half = field == PXFDTD_E ? comp == axis : comp != axis;
+        const double first = origin[axis] + 0 * hs[axis] + (half ? 0.5 * hs[axis] : 0.0);
+        const double sp = (position[axis] - first) / hs[axis];
+        const bool periodic = s->b.periodic[axis];
+        if (!periodic && !(sp >= 0.0 && sp <= n - 1))
+            return px_fail(PXFDTD_ERR_INVALID, "the dipole is outside its component's values along axis %d", axis);
+        const double m = std::floor(sp);
+        const double f = sp - m;
+        long long lo, hi;
+        if (periodic) {
+            lo = (((long long)m % n) + n) % n;
+            hi = (((long long)m + 1) % n + n) % n;
+        } else {
+            lo = (long long)m;
+            hi = std::min<long long>((long long)m + 1, n - 1);
+        }
+        along[axis][0] = {lo, 1.0 * (1.0 - f)};
+        along[axis][1] = {hi, 1.0 * f};
+    }
+    // weights summed per value, values in index order (a BTreeMap in the reference)
+    std::map<long long, C64> values;
+    for (auto &x : along[0])
+        for (auto &y : along[1])
+            for (auto &z : along[2]) {
+                // complex weights with zero imaginary parts: wx * wy * wz
+                const double wv = x.w * y.w * z.w;
+                if (wv == 0.0) continue;
+                const long long r = (z.m * s->n[1] + y.m) * s->n[0] + x.m;
+                // a * (w / volume): (w/volume) real, a complex
+                const double q = wv / volume;
+                C64 &acc = values[r];
+                acc.re += amp_re * q - amp_im * 0.0;
+                acc.im += amp_re * 0.0 + amp_im * q;
+            }
+    CurrentRec c;
+    c.field = field;
+    c.w = *w;
+    for (auto &kv : values) {
+        c.comps.push_back(comp);
+        c.r.push_back(kv.first);
+        c.amps.push_back(kv.second);
+    }
+    s->currents.push_back(std::move(c));
+    s->inj_dirty = true;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_add_probe(pxfdtd_simulation *s, int32_t field, int32_t comp,
+                                           int32_t i, int32_t j, int32_t k, int32_t *number) {
+    TRY(inside(s, i, j, k));
+    if (field < 0 || field > 1 || comp < 0 || comp > 2)
+        return px_fail(PXFDTD_ERR_INVALID, "a probe needs a field and a component");
+    s->probes.push_back({field, comp, ((long long)k * s->n[1] + j) * s->n[0] + i, {}});
+    *number = (int32_t)s->probes.size() - 1;
+    s->taps_dirty = true;
+    return PXFDTD_OK;
+}
+
+// src/fdtd/monitors.rs:395-450
+extern "C" int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low[3],
+                                         const int32_t high[3], int32_t nf, const double *freqs,
+                                         int32_t *number) {
+    if (nf <= 0) return px_fail(PXFDTD_ERR_INVALID, "a monitor needs at least one frequency");
+    for (int f = 0; f < nf; f++)
+        if (!(freqs[f] > 0) || TAU * freqs[f] * s->dt >= 3.141592653589793)
+            return px_fail(PXFDTD_ERR_INVALID, "%g c/um is beyond the time step's Nyquist frequency", freqs[f]);
+    TRY(inside(s, high[0], high[1], high[2]));
+    for (int a = 0; a < 3; a++)
+        if (low[a] < 0 || low[a] > high[a])
+            return px_fail(PXFDTD_ERR_INVALID, "a transform's box must run from low to high indices");
+    DftRec d;
+    d.freqs.assign(freqs, freqs + nf);
+    long long vol = 1;
+    for (int a = 0; a < 3; a++) {
+        d.lo[a] = low[a];
+        d.hi[a] = high[a] + 1;
+        vol *= d.hi[a] - d.lo[a];
+    }
+    d.acc.assign(6 * 2 * (size_t)nf * vol, 0.0);
+    // sums so far on the device are kept on the host before the device's layout changes
+    TRY(bank_sums(s));
+    s->dfts.push_back(std::move(d));
+    *number = (int32_t)s->dfts.size() - 1;
+    s->taps_dirty = true;
+    return PXFDTD_OK;
+}
+
+// src/fdtd/media.rs:61-109,733-825
+extern "C" int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf, int32_t np,
+                                            const double *poles, const pxfdtd_scene *inside_) {
+    if (s->steps > 0) return px_fail(PXFDTD_ERR_INVALID, "a dispersive medium goes in before the run starts");
+    if (!(eps_inf > 0 && std::isfinite(eps_inf)))
+        return px_fail(PXFDTD_ERR_INVALID, "a medium's eps_inf must be positive and finite");
+    if (np < 0 || np > 4) return px_fail(PXFDTD_ERR_UNSUPPORTED, "a medium of 0 to 4 terms steps here");
+    TRY(sync_host(s));
+    const double dt = s->dt;
+    Group G;
+    G.poles = np;
+    double gsum = 0.0;
+    double top_w0dt = 0.0;
+    for (int q = 0; q < np; q++) {
+        const double *p = poles + 4 * q;
+        double a, w0sq, gamma;
+        if (p[0] == 0.0) {
+            if (!(p[1] >= 0) || !(p[2] > 0) || !(p[3] >= 0))
+                return px_fail(PXFDTD_ERR_INVALID, "a Lorentz term needs strength >= 0, resonance > 0, damping >= 0");
+            const double w0 = TAU * p[2];
+            a = p[1] * w0 * w0;
+            w0sq = w0 * w0;
+            gamma = TAU * p[3];
+            top_w0dt = std::max(top_w0dt, w0 * dt);
+        } else {
+            if (!(p[1] > 0) || !(p[2] >= 0))
+                return px_fail(PXFDTD_ERR_INVALID, "a Drude term needs plasma > 0, damping >= 0");
+            const double wp = TAU * p[1];
+            a = wp * wp;
+            w0sq = 0.0;
+            gamma = TAU * p[2];
+        }
+        const double d = 1.0 + gamma * dt / 2.0;
+        G.coef[q][0] = (2.0 - w0sq * dt * dt) / d;
+        G.coef[q][1] = (gamma * dt / 2.0 - 1.0) / d;
+        G.coef[q][2] = a * dt * dt / d;
+        gsum += G.coef[q][2];
+    }
+    // stability, as the reference states it in practice (media.rs:36-40): w0 dt <= 2 for each
+    // term and a Courant number within sqrt(eps_inf)
+    double curl = 0.0;
+    const double hs[3] = {s->g.dx, s->g.dy, s->g.dz};
+    for (int a = 0; a < 3; a++)
+        if (!(s->b.periodic[a] && s->n[a] == 1)) curl += 1.0 / (hs[a] * hs[a]);
+    if (top_w0dt > 2.0 || dt * std::sqrt(curl) > std::sqrt(eps_inf) * (1.0 + 1e-12))
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the medium isn't stable at the time step %g um/c (a term's w0 dt above 2, or a "
+                       "Courant number above sqrt(eps_inf)): take a smaller step", dt);
+    G.chi = gsum / (4.0 * dt);
+    const long long N = s->N;
+    std::vector<double> sig;
+    for (int c = 0; c < 3; c++)
+        for (long long r = 0; r < N; r++) {
+            if (s->cb[c * N + r] == 0.0) continue;
+            double p[3];
+            s->e_position(c, r, p);
+            if (scene_value(inside_, 0, p[0], p[1], p[2]) == 0.0) continue;
+            double old = 0.0;
+            if (!s->member.empty() && s->member[c * N + r] >= 0) old = s->groups[s->member[c * N + r]].chi;
+            const double sigma = (1.0 - s->ca[c * N + r]) / s->cb[c * N + r] - old;
+            G.comp.push_back(c);
+            G.r.push_back(r);
+            sig.push_back(sigma > 0.0 ? sigma : 0.0);
+        }
+    const double e = eps_inf;
+    for (size_t v = 0; v < G.r.size(); v++) {
+        const int c = G.comp[v];
+        const long long r = G.r[v];
+        media_remove(s, c, r);
+        const double nu = 1.0 + dt / e * (sig[v] / 2.0 + G.chi);
+        s->ca[c * N + r] = (1.0 - dt * sig[v] / (2.0 * e)) / nu;
+        s->cb[c * N + r] = dt / e / nu;
+    }
+    if (np > 0 && !G.r.empty()) {
+        if (s->member.empty()) s->member.assign(3 * N, -1);
+        const int gi = (int)s->groups.size();
+        for (size_t v = 0; v < G.r.size(); v++) s->member[G.comp[v] * N + G.r[v]] = gi;
+        s->groups.push_back(std::move(G));
+    }
+    drop_resident(s);
+    return PXFDTD_OK;
+}
+
+// ------------------------------------------------------------------------------ device
+
+static void drop_resident(pxfdtd_simulation *s) {
+    (void)bank_sums(s);
+    if (s->res) {
+        pxfdtd_resident_destroy(s->res);
+        s->res = nullptr;
+    }
+    s->inj_dirty = s->taps_dirty = true;
+    s->host_newer = true;
+}
+
+// the device's fields, psi and transform sums back on the host
+static int sync_host(pxfdtd_simulation *s) {
+    if (!s->res || !s->device_newer) return PXFDTD_OK;
+    std::vector<double> psi(pxfdtd_resident_psi_len(s->res));
+    TRY(pxfdtd_resident_download(s->res, s->e.data(), s->h.data(), psi.data()));
+    size_t at = 0;
+    for (auto &sl : s->slabs)
+        for (auto &v : sl.psi) v = psi[at++];
+    TRY(bank_sums(s));
+    s->device_newer = false;
+    // the dispersive media's currents live on the device only: keep it
+    return PXFDTD_OK;
+}
+
+static int ensure_resident(pxfdtd_simulation *s) {
+    if (!s->dev) return px_fail(PXFDTD_ERR_UNSUPPORTED, "no device: set one with pxfdtd_simulation_set_device");
+    if (!s->res) {
+        pxfdtd_problem p;
+        std::memset(&p, 0, sizeof p);
+        p.nx = s->n[0];
+        p.ny = s->n[1];
+        p.nz = s->n[2];
+        p.precision = s->precision;
+        p.dt = s->dt;
+        p.step[0] = s->g.dx;
+        p.step[1] = s->g.dy;
+        p.step[2] = s->g.dz;
+        for (int a = 0; a < 3; a++) {
+            p.periodic[a] = s->b.periodic[a];
+            p.pml_low[a] = s->b.periodic[a] ? 0 : s->b.low[a];
+            p.pml_high[a] = s->b.periodic[a] ? 0 : s->b.high[a];
+            p.factors_e[a] = s->fe[a].data();
+            p.factors_h[a] = s->fh[a].data();
+            for (int sd = 0; sd < 2; sd++)
+                for (int f = 0; f < 2; f++) {
+                    p.cpml_b[a][sd][f] = s->cpml_b[a][sd][f].empty() ? nullptr : s->cpml_b[a][sd][f].data();
+                    p.cpml_a[a][sd][f] = s->cpml_a[a][sd][f].empty() ? nullptr : s->cpml_a[a][sd][f].data();
+                }
+        }
+        p.ca = s->ca.data();
+        p.cb = s->cb.data();
+        std::vector<pxfdtd_medium> media(s->groups.size());
+        for (size_t g = 0; g < s->groups.size(); g++) {
+            const Group &G = s->groups[g];
+            pxfdtd_medium &m = media[g];
+            std::memset(&m, 0, sizeof m);
+            m.poles = G.poles;
+            m.chi = G.chi;
+            std::memcpy(m.coef, G.coef, sizeof m.coef);
+            m.count = (int64_t)G.r.size();
+            m.component = G.comp.data();
+            m.index = G.r.data();
+        }
+        // media with no terms left are plain values
+        std::vector<pxfdtd_medium> live;
+        for (auto &m : media)
+            if (m.count && m.poles) live.push_back(m);
+        p.nmedia = (int32_t)live.size();
+        p.media = live.data();
+        TRY(pxfdtd_resident_create(s->dev, &p, &s->res));
+        s->host_newer = true;
+        s->inj_dirty = s->taps_dirty = true;
+    }
+    if (s->host_newer) {
+        std::vector<double> psi;
+        for (auto &sl : s->slabs) psi.insert(psi.end(), sl.psi.begin(), sl.psi.end());
+        TRY(pxfdtd_resident_upload(s->res, s->e.data(), s->h.data(), psi.empty() ? nullptr : psi.data()));
+        s->host_newer = false;
+    }
+    if (s->inj_dirty) {
+        // H's then E's order within a field: point sources, then currents, each in order
+        std::vector<pxfdtd_injection> inj;
+        int em = 0;
+        for (auto &so : s->sources) inj.push_back({so.field, so.comp, so.r, 1.0, 0.0, em++});
+        for (auto &cu : s->currents) {
+            for (size_t v = 0; v < cu.r.size(); v++)
+                inj.push_back({cu.field, cu.comps[v], cu.r[v], cu.amps[v].re, cu.amps[v].im, em});
+            em++;
+        }
+        s->emitters = em;
+        TRY(pxfdtd_resident_set_injections(s->res, inj.data(), (int64_t)inj.size(), std::max(em, 1)));
+        s->inj_dirty = false;
+    }
+    if (s->taps_dirty) {
+        std::vector<pxfdtd_probe> pr;
+        for (auto &p : s->probes) pr.push_back({p.field, p.comp, p.r});
+        TRY(pxfdtd_resident_set_probes(s->res, pr.data(), (int32_t)pr.size()));
+        std::vector<pxfdtd_series> se;
+        std::vector<int32_t> nf;
+        for (size_t m = 0; m < s->dfts.size(); m++) {
+            const DftRec &d = s->dfts[m];
+            nf.push_back((int32_t)d.freqs.size());
+            for (int q = 0; q < 6; q++) {
+                pxfdtd_series x;
+                x.field = q < 3 ? PXFDTD_E : PXFDTD_H;
+                x.component = q % 3;
+                for (int a = 0; a < 3; a++) {
+                    x.lo[a] = d.lo[a];
+                    x.hi[a] = d.hi[a];
+                }
+                x.monitor = (int32_t)m;
+                se.push_back(x);
+            }
+        }
+        TRY(pxfdtd_resident_set_series(s->res, se.data(), (int32_t)se.size(), nf.data(), (int32_t)nf.size()));
+        s->taps_dirty = false;
+    }
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_set_device(pxfdtd_simulation *s, pxfdtd_device *dev, int32_t precision) {
+    if (precision != PXFDTD_F32 && precision != PXFDTD_F64)
+        return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");
+    TRY(sync_host(s));
+    drop_resident(s);
+    s->dev = dev;
+    s->precision = precision;
+    if (dev) {
+        const int rc = ensure_resident(s);
+        if (rc != PXFDTD_OK) {  // the device stays as it was: none
+            s->dev = nullptr;
+            drop_resident(s);
+            return rc;
+        }
+    }
+    return PXFDTD_OK;
+}
+
+static int read_probes(pxfdtd_simulation *s) {
+    if (!s->res || s->probes.empty()) return PXFDTD_OK;
+    const int64_t rows = pxfdtd_resident_probe_steps(s->res);
+    if (!rows) return PXFDTD_OK;
+    const size_t np = s->probes.size();
+    std::vector<double> buf((size_t)rows * np);
+    int64_t steps = 0;
+    TRY(pxfdtd_resident_read_probes(s->res, buf.data(), (int64_t)buf.size(), &steps));
+    for (int64_t n = 0; n < steps; n++)
+        for (size_t q = 0; q < np; q++) s->probes[q].values.push_back(buf[n * np + q]);
+    return PXFDTD_OK;
+}
+
+// Simulation::run on the device (src/fdtd.rs:1142-1158, resident.rs:430-539): waveforms and
+// phases in FP64 on the host, in chunks of steps
+extern "C" int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count) {
+    if (count < 0) return px_fail(PXFDTD_ERR_INVALID, "a negative step count");
+    TRY(ensure_resident(s));
+    const int slots = pxfdtd_resident_phase_slots(s->res);
+    const int64_t chunk = 512;
+    std::vector<double> waves, phases;
+    for (int64_t done = 0; done < count;) {
+        const int64_t m = std::min(chunk, count - done);
+        waves.assign((size_t)m * std::max(1, s->emitters) * 2, 0.0);
+        phases.assign((size_t)m * std::max(1, slots) * 2, 0.0);
+        for (int64_t q = 0; q < m; q++) {
+            const long long n = s->steps + done + q;
+            const double t = (double)n * s->dt;
+            const double half = t + 0.5 * s->dt;
+            int em = 0;
+            double *w = waves.data() + (size_t)q * std::max(1, s->emitters) * 2;
+            for (auto &so : s->sources) {
+                w[2 * em] = wave_at(so.w, so.field == PXFDTD_H ? t : half);
+                w[2 * em + 1] = 0.0;
+                em++;
+            }
+            for (auto &cu : s->currents) {
+                const C64 v = wave_complex_at(cu.w, cu.field == PXFDTD_H ? t : half);
+                w[2 * em] = v.re;
+                w[2 * em + 1] = v.im;
+                em++;
+            }
+            const double te = (double)(n + 1) * s->dt;
+            const double th = te - 0.5 * s->dt;
+            double *ph = phases.data() + (size_t)q * std::max(1, slots) * 2;
+            int slot = 0;
+            for (auto &d : s->dfts) {
+                for (int f2 = 0; f2 < 2; f2++)
+                    for (double fr : d.freqs) {
+                        const C64 p = pxfdtd::pcexp({0.0, TAU * fr * (f2 == 0 ? te : th)});
+                        ph[2 * slot] = p.re * s->dt;
+                        ph[2 * slot + 1] = p.im * s->dt;
+                        slot++;
+                    }
+            }
+        }
+        TRY(pxfdtd_resident_step(s->res, (uint32_t)m, waves.data(), phases.data()));
+        done += m;
+        TRY(read_probes(s));
+    }
+    s->steps += count;
+    if (count) s->device_newer = true;
+    return PXFDTD_OK;
+}
+
+extern "C" double pxfdtd_simulation_dt(const pxfdtd_simulation *s) { return s->dt; }
+extern "C" int64_t pxfdtd_simulation_steps(const pxfdtd_simulation *s) { return s->steps; }
+extern "C" int64_t pxfdtd_simulation_cells(const pxfdtd_simulation *s) { return s->N; }
+extern "C" pxfdtd_resident *pxfdtd_simulation_resident(pxfdtd_simulation *s) { return s->res; }
+
+extern "C" int pxfdtd_simulation_field(pxfdtd_simulation *s, int32_t which, int32_t comp, double *out) {
+    if (which < 0 || which > 1 || comp < 0 || comp > 2) return px_fail(PXFDTD_ERR_INVALID, "no such field");
+    TRY(sync_host(s));
+    const double *src = (which == 0 ? s->e.data() : s->h.data()) + comp * s->N;
+    std::memcpy(out, src, s->N * sizeof(double));
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_set_field(pxfdtd_simulation *s, int32_t which, int32_t comp,
+                                           const double *values) {
+    if (which < 0 || which > 1 || comp < 0 || comp > 2) return px_fail(PXFDTD_ERR_INVALID, "no such field");
+    TRY(sync_host(s));
+    double *dst = (which == 0 ? s->e.data() : s->h.data()) + comp * s->N;
+    std::memcpy(dst, values, s->N * sizeof(double));
+    s->host_newer = true;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_coefficients(const pxfdtd_simulation *s, int32_t comp, double *ca, double *cb) {
+    if (comp < 0 || comp > 2) return px_fail(PXFDTD_ERR_INVALID, "no such component");
+    std::memcpy(ca, s->ca.data() + comp * s->N, s->N * sizeof(double));
+    std::memcpy(cb, s->cb.data() + comp * s->N, s->N * sizeof(double));
+    return PXFDTD_OK;
+}
+
+extern "C" int64_t pxfdtd_simulation_probe_len(pxfdtd_simulation *s, int32_t n) {
+    if (n < 0 || n >= (int32_t)s->probes.size()) return -1;
+    return (int64_t)s->probes[n].values.size();
+}
+
+extern "C" int pxfdtd_simulation_probe(pxfdtd_simulation *s, int32_t n, double *out) {
+    if (n < 0 || n >= (int32_t)s->probes.size()) return px_fail(PXFDTD_ERR_INVALID, "no probe %d", n);
+    std::memcpy(out, s->probes[n].values.data(), s->probes[n].values.size() * sizeof(double));
+    return PXFDTD_OK;
+}
+
+extern "C" int64_t pxfdtd_simulation_dft_len(pxfdtd_simulation *s, int32_t n, int32_t series) {
+    if (n < 0 || n >= (int32_t)s->dfts.size() || series < 0 || series > 5) return -1;
+    return (int64_t)s->dfts[n].acc.size() / 6;
+}
+
+extern "C" int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t series, double *out) {
+    if (n < 0 || n >= (int32_t)s->dfts.size() || series < 0 || series > 5)
+        return px_fail(PXFDTD_ERR_INVALID, "no transform %d series %d", n, series);
+    const DftRec &d = s->dfts[n];
+    const size_t len = d.acc.size() / 6;
+    std::memcpy(out, d.acc.data() + series * len, len * sizeof(double));
+    if (s->res && !s->taps_dirty) {
+        // add what the device has summed since its series were set
+        std::vector<double> sums(pxfdtd_resident_sums_len(s->res));
+        TRY(pxfdtd_resident_read_sums(s->res, sums.data(), (int64_t)sums.size()));
+        size_t at = 0;
+        for (int32_t m = 0; m < n; m++) at += s->dfts[m].acc.size();
+        at += series * len;
+        for (size_t q = 0; q < len; q++) out[q] += sums[at + q];
+    }
+    return PXFDTD_OK;
+}

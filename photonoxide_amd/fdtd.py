@@ -1,0 +1,447 @@
+"""photonoxide's FDTD interface (src/fdtd.rs), stepping on an MI355X through the native library.
+
+The names, arguments and errors follow the reference's ``fdtd`` module: a :class:`Simulation` on
+a :class:`Grid3d` inside :class:`Boundaries`, with :class:`Source`, :class:`Current` and
+:class:`Dipole` sources, probes, :meth:`Simulation.add_dft` transform monitors and
+:meth:`Simulation.with_medium` dispersive media, stepped by :meth:`Simulation.run` on the
+device set by :meth:`Simulation.set_device`. Fields are numpy arrays in the reference's layout
+(k, j, i with i fastest). There is no CPU stepping: ``run`` without a device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field as dc_field
+from enum import IntEnum
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+
+class Axis(IntEnum):
+    X = 0
+    Y = 1
+    Z = 2
+
+    def others(self):
+        """The next two axes in cyclic order (src/fdfd/three.rs:52-58)."""
+        return Axis((self + 1) % 3), Axis((self + 2) % 3)
+
+
+class Field(IntEnum):
+    E = 0
+    H = 1
+
+
+class Precision(IntEnum):
+    F32 = N.F32
+    F64 = N.F64
+
+
+@dataclass(frozen=True)
+class Grid3d:
+    """A uniform grid of nx x ny x nz cells of dx x dy x dz um, low corner (x0, y0, z0)
+    (src/fdfd/three.rs:70-89)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    x0: float = 0.0
+    y0: float = 0.0
+    z0: float = 0.0
+
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    def n(self, axis) -> int:
+        return (self.nx, self.ny, self.nz)[axis]
+
+    def step(self, axis) -> float:
+        return (self.dx, self.dy, self.dz)[axis]
+
+    def index(self, i, j, k) -> int:
+        return (k * self.ny + j) * self.nx + i
+
+    def e_position(self, component, i, j, k):
+        p = [self.x0 + i * self.dx, self.y0 + j * self.dy, self.z0 + k * self.dz]
+        p[component] += 0.5 * self.step(component)
+        return p
+
+    @staticmethod
+    def centred(n, h):
+        """The grid of n cells of h um centred on the origin (src/fdtd/checks.rs:8-20)."""
+        return Grid3d(n[0], n[1], n[2], h, h, h, -n[0] * h / 2.0, -n[1] * h / 2.0, -n[2] * h / 2.0)
+
+
+@dataclass(frozen=True)
+class Pml:
+    """A CPML of low and high cells inside the grid (0 cells: a wall)."""
+    low: int = 0
+    high: int = 0
+
+
+@dataclass(frozen=True)
+class Periodic:
+    """A periodic side (a Bloch boundary with k = 0)."""
+
+
+@dataclass(frozen=True)
+class Cpml:
+    """The convolutional PML's parameters (src/fdtd.rs:84-110): FDFD's R = 1e-8, order 3."""
+    reflection: float = 1e-8
+    sigma: Optional[float] = None
+    order: float = 3.0
+    kappa: float = 1.0
+    alpha: float = 0.0
+
+
+@dataclass(frozen=True)
+class Boundaries:
+    x: object = Pml()
+    y: object = Pml()
+    z: object = Pml()
+    cpml: Cpml = Cpml()
+
+    @staticmethod
+    def cpml_all(cells: int) -> "Boundaries":
+        e = Pml(cells, cells)
+        return Boundaries(e, e, e)
+
+    @staticmethod
+    def walls() -> "Boundaries":
+        return Boundaries.cpml_all(0)
+
+    @staticmethod
+    def cpml_2d(cells: int) -> "Boundaries":
+        e = Pml(cells, cells)
+        return Boundaries(e, e, Periodic())
+
+    def native(self) -> N.Bounds:
+        b = N.Bounds()
+        for a, e in enumerate((self.x, self.y, self.z)):
+            if isinstance(e, Periodic):
+                b.periodic[a] = 1
+            else:
+                b.low[a], b.high[a] = e.low, e.high
+        c = self.cpml
+        b.reflection, b.order, b.kappa, b.alpha = c.reflection, c.order, c.kappa, c.alpha
+        b.has_sigma = 0 if c.sigma is None else 1
+        b.sigma = 0.0 if c.sigma is None else c.sigma
+        return b
+
+
+class Waveform:
+    """How a source's value varies in time (src/fdtd.rs:259-424)."""
+
+    def __init__(self, kind, frequency=0.0, width=0.0, delay=0.0, amplitude=1 + 0j, ramp=0.0):
+        self.kind, self.frequency, self.width, self.delay = kind, frequency, width, delay
+        self.amplitude, self.ramp = complex(amplitude), ramp
+
+    @staticmethod
+    def gaussian(frequency, width, delay):
+        return Waveform(0, frequency, width, delay)
+
+    @staticmethod
+    def differentiated_gaussian(width, delay):
+        return Waveform(1, 0.0, width, delay)
+
+    @staticmethod
+    def continuous(frequency, amplitude=1 + 0j, ramp=0.0):
+        return Waveform(2, frequency, 0.0, 0.0, amplitude, ramp)
+
+    @staticmethod
+    def pulse(frequency, bandwidth):
+        """A Gaussian on a carrier whose power spectrum is `bandwidth` wide at half its peak:
+        width sqrt(2 ln 2)/(pi bandwidth), its peak 6 widths in (src/fdtd.rs:300-312)."""
+        if not (math.isfinite(bandwidth) and bandwidth > 0):
+            raise ValueError(f"a pulse's bandwidth must be positive and finite, got {bandwidth}")
+        width = math.sqrt(2.0 * math.log(2.0)) / (math.pi * bandwidth)
+        return Waveform(0, frequency, width, 6.0 * width)
+
+    def native(self) -> N.Wave:
+        return N.Wave(self.kind, self.frequency, self.width, self.delay,
+                      self.amplitude.real, self.amplitude.imag, self.ramp)
+
+
+@dataclass
+class Source:
+    field: Field
+    component: Axis
+    at: tuple
+    waveform: Waveform
+
+
+@dataclass
+class Current:
+    """A current over many values: (component, (i, j, k), complex amplitude) each."""
+    field: Field
+    values: list
+    waveform: Waveform
+
+
+@dataclass
+class Dipole:
+    field: Field
+    component: Axis
+    position: tuple
+    amplitude: complex
+    waveform: Waveform
+
+
+@dataclass
+class Box:
+    """An open box lo < p < hi (inf allowed) with a diagonal permittivity (or one value)."""
+    lo: tuple
+    hi: tuple
+    value: object
+
+    def native(self) -> N.Shape:
+        v = self.value if isinstance(self.value, (tuple, list)) else (self.value,) * 3
+        return N.Shape(0, N.d3(*self.lo), N.d3(*self.hi), N.d3(*v))
+
+
+@dataclass
+class Cylinder:
+    """A cylinder along z: (x - cx)^2 + (y - cy)^2 < r^2 and zlo < z < zhi."""
+    centre: tuple
+    radius: float
+    z: tuple
+    value: object
+
+    def native(self) -> N.Shape:
+        v = self.value if isinstance(self.value, (tuple, list)) else (self.value,) * 3
+        return N.Shape(1, N.d3(self.centre[0], self.centre[1], self.z[0]),
+                       N.d3(self.radius, 0.0, self.z[1]), N.d3(*v))
+
+
+class Scene:
+    """A background value and regions, later ones replacing earlier ones."""
+
+    def __init__(self, background, shapes=()):
+        bg = background if isinstance(background, (tuple, list)) else (background,) * 3
+        self.background = tuple(float(b) for b in bg)
+        self.shapes = list(shapes)
+
+    def native(self):
+        arr = (N.Shape * max(1, len(self.shapes)))(*[s.native() for s in self.shapes])
+        sc = N.Scene(N.d3(*self.background), len(self.shapes), arr)
+        return sc, arr  # keep arr alive with the scene
+
+
+class Gpu:
+    """A HIP device (replaces gpu::Gpu, src/fdtd/gpu.rs:84-234)."""
+
+    def __init__(self, ordinal: int = 0):
+        h = C.c_void_p()
+        N.check(N.lib.pxfdtd_device_open(ordinal, C.byref(h)))
+        self.handle = h
+
+    def info(self) -> dict:
+        i = N.DeviceInfo()
+        N.check(N.lib.pxfdtd_device_query(self.handle, C.byref(i)))
+        return {"name": i.name.decode(), "arch": i.arch.decode(), "ordinal": i.ordinal,
+                "compute_units": i.compute_units, "hbm_bytes": i.hbm_bytes,
+                "free_bytes": i.free_bytes}
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            N.lib.pxfdtd_device_close(h)
+            self.handle = None
+
+
+class Simulation:
+    """An FDTD problem (src/fdtd.rs:470-512) on the host, stepping on a device."""
+
+    def __init__(self, handle, grid: Grid3d, boundaries: Boundaries):
+        self._h = handle
+        self.grid = grid
+        self.boundaries = boundaries
+        self._device = None
+
+    # -- construction ---------------------------------------------------------------------
+    @staticmethod
+    def new(grid: Grid3d, eps, boundaries: Boundaries, courant: float) -> "Simulation":
+        """The permittivity averaged per component over 8^3 samples of each cell
+        (Simulation::new / diagonal, src/fdtd.rs:570-653). `eps` is a :class:`Scene`."""
+        h = C.c_void_p()
+        scene, keep = eps.native()
+        N.check(N.lib.pxfdtd_simulation_new(C.byref(_grid(grid)), C.byref(boundaries.native()),
+                                            C.byref(scene), courant, C.byref(h)))
+        return Simulation(h, grid, boundaries)
+
+    @staticmethod
+    def with_eps(grid: Grid3d, eps3n, boundaries: Boundaries, courant: float) -> "Simulation":
+        """The permittivity each value of E sees, given per component as [3][nz][ny][nx]."""
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps3n, dtype=np.float64),
+                                                 (3, grid.nz, grid.ny, grid.nx)))
+        h = C.c_void_p()
+        N.check(N.lib.pxfdtd_simulation_new_eps(C.byref(_grid(grid)), C.byref(boundaries.native()),
+                                                N.dptr(e), courant, C.byref(h)))
+        return Simulation(h, grid, boundaries)
+
+    def with_conductivity(self, sigma: Scene) -> "Simulation":
+        sc, keep = sigma.native()
+        N.check(N.lib.pxfdtd_simulation_conductivity(self._h, C.byref(sc)))
+        return self
+
+    def with_medium(self, eps_inf: float, poles: Sequence[tuple], inside: Scene) -> "Simulation":
+        """A dispersive medium (src/fdtd/media.rs:733-825). Each pole is
+        ('lorentz', strength, resonance, damping) or ('drude', plasma, damping)."""
+        arr = np.zeros((max(1, len(poles)), 4))
+        for q, p in enumerate(poles):
+            if p[0] == "lorentz":
+                arr[q] = (0.0, p[1], p[2], p[3])
+            elif p[0] == "drude":
+                arr[q] = (1.0, p[1], p[2], 0.0)
+            else:
+                raise ValueError(f"unknown pole {p!r}")
+        sc, keep = inside.native()
+        N.check(N.lib.pxfdtd_simulation_add_medium(self._h, eps_inf, len(poles), N.dptr(arr),
+                                                   C.byref(sc)))
+        return self
+
+    def conductor(self, component, at):
+        N.check(N.lib.pxfdtd_simulation_conductor(self._h, int(component), *at))
+
+    def add_source(self, source: Source):
+        N.check(N.lib.pxfdtd_simulation_add_source(self._h, int(source.field), int(source.component),
+                                                   *source.at, C.byref(source.waveform.native())))
+
+    def add_current(self, current: Current):
+        n = len(current.values)
+        comps = np.array([int(v[0]) for v in current.values], dtype=np.int32)
+        idx = np.array([self.grid.index(*v[1]) for v in current.values], dtype=np.int64)
+        for v in current.values:
+            i, j, k = v[1]
+            if not (0 <= i < self.grid.nx and 0 <= j < self.grid.ny and 0 <= k < self.grid.nz):
+                raise N.NativeError(N.ERR_INVALID, f"{v[1]} is outside the grid")
+        amps = np.array([[complex(v[2]).real, complex(v[2]).imag] for v in current.values],
+                        dtype=np.float64).reshape(n, 2)
+        N.check(N.lib.pxfdtd_simulation_add_current(
+            self._h, int(current.field), n, comps.ctypes.data_as(N.pi32),
+            idx.ctypes.data_as(N.pi64), N.dptr(amps), C.byref(current.waveform.native())))
+
+    def add_current_indices(self, field, comps, index, amps, waveform: Waveform):
+        """A current given as arrays: components, flat indices within a component, complex
+        amplitudes (the form a mode source's restriction takes)."""
+        comps = np.ascontiguousarray(comps, dtype=np.int32)
+        index = np.ascontiguousarray(index, dtype=np.int64)
+        a = np.ascontiguousarray(np.stack([np.real(amps), np.imag(amps)], axis=-1), dtype=np.float64)
+        N.check(N.lib.pxfdtd_simulation_add_current(
+            self._h, int(field), len(index), comps.ctypes.data_as(N.pi32),
+            index.ctypes.data_as(N.pi64), N.dptr(a), C.byref(waveform.native())))
+
+    def add_dipole(self, dipole: Dipole):
+        pos = np.array(dipole.position, dtype=np.float64)
+        a = complex(dipole.amplitude)
+        N.check(N.lib.pxfdtd_simulation_add_dipole(self._h, int(dipole.field), int(dipole.component),
+                                                   N.dptr(pos), a.real, a.imag,
+                                                   C.byref(dipole.waveform.native())))
+
+    def add_probe(self, field, component, at) -> int:
+        n = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_add_probe(self._h, int(field), int(component), *at, C.byref(n)))
+        return n.value
+
+    def probe(self, n: int) -> np.ndarray:
+        m = N.lib.pxfdtd_simulation_probe_len(self._h, n)
+        if m < 0:
+            raise IndexError(f"no probe {n}")
+        out = np.zeros(m)
+        N.check(N.lib.pxfdtd_simulation_probe(self._h, n, N.dptr(out)))
+        return out
+
+    def add_dft(self, low, high, frequencies) -> int:
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        lo = (C.c_int32 * 3)(*low)
+        hi = (C.c_int32 * 3)(*high)
+        n = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_add_dft(self._h, lo, hi, len(f), N.dptr(f), C.byref(n)))
+        self._dft_shapes = getattr(self, "_dft_shapes", [])
+        self._dft_shapes.append((len(f), tuple(h - l + 1 for l, h in zip(low, high))))
+        return n.value
+
+    def dft(self, n: int, field, component) -> np.ndarray:
+        """Monitor n's transform of field's component: [f][k][j][i] complex."""
+        series = int(field) * 3 + int(component)
+        m = N.lib.pxfdtd_simulation_dft_len(self._h, n, series)
+        if m < 0:
+            raise IndexError(f"no transform {n}")
+        out = np.zeros(m)
+        N.check(N.lib.pxfdtd_simulation_dft(self._h, n, series, N.dptr(out)))
+        nf, (lx, ly, lz) = self._dft_shapes[n]
+        return (out[0::2] + 1j * out[1::2]).reshape(nf, lz, ly, lx)
+
+    # -- stepping -------------------------------------------------------------------------
+    def set_device(self, device: Optional[Gpu], precision: Precision = Precision.F32):
+        """Steps on `device` from now on (src/fdtd.rs:1517-1540); refused problems leave none."""
+        N.check(N.lib.pxfdtd_simulation_set_device(self._h, device.handle if device else None,
+                                                   int(precision)))
+        self._device = device
+
+    def run(self, count: int):
+        N.check(N.lib.pxfdtd_simulation_run(self._h, int(count)))
+
+    def dt(self) -> float:
+        return N.lib.pxfdtd_simulation_dt(self._h)
+
+    def steps(self) -> int:
+        return N.lib.pxfdtd_simulation_steps(self._h)
+
+    def _field(self, which, component) -> np.ndarray:
+        out = np.zeros(self.grid.cells())
+        N.check(N.lib.pxfdtd_simulation_field(self._h, which, int(component), N.dptr(out)))
+        return out
+
+    def e(self, component) -> np.ndarray:
+        return self._field(0, component)
+
+    def h(self, component) -> np.ndarray:
+        return self._field(1, component)
+
+    def set_e(self, component, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        N.check(N.lib.pxfdtd_simulation_set_field(self._h, 0, int(component), N.dptr(v)))
+
+    def set_h(self, component, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        N.check(N.lib.pxfdtd_simulation_set_field(self._h, 1, int(component), N.dptr(v)))
+
+    def coefficients(self, component):
+        ca = np.zeros(self.grid.cells())
+        cb = np.zeros(self.grid.cells())
+        N.check(N.lib.pxfdtd_simulation_coefficients(self._h, int(component), N.dptr(ca), N.dptr(cb)))
+        return ca, cb
+
+    def resident(self):
+        return N.lib.pxfdtd_simulation_resident(self._h)
+
+    def set_tiling(self, rows=0, planes=0):
+        r = self.resident()
+        if not r:
+            raise N.NativeError(N.ERR_UNSUPPORTED, "no device")
+        N.check(N.lib.pxfdtd_resident_set_tiling(r, rows, planes))
+
+    def tiling(self):
+        r = self.resident()
+        rows, planes = C.c_int32(), C.c_int32()
+        N.check(N.lib.pxfdtd_resident_get_tiling(r, C.byref(rows), C.byref(planes)))
+        return rows.value, planes.value
+
+    def last_step_ms(self) -> float:
+        return N.lib.pxfdtd_resident_last_kernel_ms(self.resident())
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            N.lib.pxfdtd_simulation_destroy(h)
+            self._h = None
+
+
+def _grid(g: Grid3d) -> N.Grid:
+    return N.Grid(g.nx, g.ny, g.nz, g.dx, g.dy, g.dz, g.x0, g.y0, g.z0)

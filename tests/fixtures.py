@@ -1,0 +1,136 @@
+"""Shared problems for the parity tests, built the same way on the oracle and on the device.
+
+`busy` follows the reference's GPU check problem (src/fdtd/gpu/checks.rs:36-147): a dielectric
+block and a lossy one, a conductor, CPMLs with kappa and alpha, a wall and (in 2D) periodic
+sides, random fields to start from, point sources on E and H and a current sharing a value with
+one of them, probes and a transform box, at sizes that leave tiles part-filled.
+"""
+import math
+
+import numpy as np
+
+import oracle_lib as O
+
+INF = float("inf")
+
+
+def noise(r):
+    """A pseudo-random value in [-1, 1) from an integer (src/fdtd/checks.rs:84-90)."""
+    r = np.asarray(r, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        x = (r * np.uint64(0x9E3779B97F4A7C15)) ^ np.uint64(0x5DEECE66)
+        x ^= x >> np.uint64(31)
+        x = x * np.uint64(0xBF58476D1CE4E5B9)
+        x ^= x >> np.uint64(29)
+    return (x >> np.uint64(11)).astype(np.float64) / float(1 << 52) - 1.0
+
+
+def pulse():
+    width = math.sqrt(2.0 * math.log(2.0)) / (math.pi * 0.6)
+    return dict(kind=0, frequency=1.0, width=width, delay=6.0 * width)
+
+
+def busy_spec(two_d=False):
+    h = 0.05
+    if two_d:
+        n = (70, 45, 1)
+        b = dict(periodic=(1, 0, 1), low=(0, 6, 0), high=(0, 6, 0), kappa=2.0, alpha=0.3)
+    else:
+        n = (37, 21, 19)
+        b = dict(periodic=(0, 0, 0), low=(4, 4, 3), high=(5, 0, 3), kappa=3.0, alpha=0.2)
+    grid = (n[0], n[1], n[2], h, h, h, -n[0] * h / 2, -n[1] * h / 2, -n[2] * h / 2)
+    nx, ny, nz = n
+    mid = (nx // 2, ny // 2, nz // 2)
+    spec = dict(
+        grid=grid, bounds=b, courant=0.9,
+        eps=(1.0, [(0, (-0.3, -0.2, -INF), (0.3, 0.2, INF), 12.0)]),
+        sigma=(0.0, [(0, (0.6, 0.3, -INF), (INF, INF, INF), 2.0)]),
+        conductor=(0, (nx // 3, ny // 3, nz // 2)),
+        sources=[(0, 2, mid, pulse()),
+                 (1, 0, (nx // 4, ny // 2, nz // 2), dict(kind=1, frequency=0.0, width=0.3, delay=1.2))],
+        currents=[(0, [(2, mid, 0.5 - 0.25j), (1, (nx // 2 + 3, ny // 2, nz // 2), -0.3 + 0.7j)], pulse())],
+        probes=[(f, c, at) for at in [(3, 3, 0), (nx - 8, ny // 3, nz // 3), mid] for (f, c) in [(0, 2), (1, 1)]],
+        dft=((nx // 2 - 6, ny // 2 - 4, 0), (nx // 2 + 6, ny // 2 + 4, nz - 1), [0.8, 1.0, 1.3]),
+    )
+    return spec
+
+
+def initial_fields(n):
+    """0.01 noise in E and H, as the reference's check problem starts."""
+    r = np.arange(n)
+    e = [0.01 * noise(r + 3 * c) for c in range(3)]
+    h = [0.01 * noise(r + 7 * c + 50_000) for c in range(3)]
+    return e, h
+
+
+def index(grid, at):
+    return (at[2] * grid[1] + at[1]) * grid[0] + at[0]
+
+
+def oracle(spec, with_fields=True):
+    s = O.Sim(spec["grid"], O.bounds(**spec["bounds"]), O.scene(*spec["eps"]), spec["courant"])
+    if spec.get("sigma"):
+        s.conductivity(O.scene(*spec["sigma"]))
+    n = s.n
+    if with_fields:
+        e, h = initial_fields(n)
+        for c in range(3):
+            s.set_field(0, c, e[c])
+            s.set_field(1, c, h[c])
+    if spec.get("conductor"):
+        c, at = spec["conductor"]
+        s.conductor(c, *at)
+    for f, c, at, w in spec.get("sources", []):
+        s.add_source(f, c, at, O.wave(**{k: v for k, v in w.items() if k != "kind"}, kind=w["kind"]))
+    for f, vals, w in spec.get("currents", []):
+        s.add_current(f, [v[0] for v in vals], [index(spec["grid"], v[1]) for v in vals],
+                      [v[2] for v in vals], O.wave(**{k: v for k, v in w.items() if k != "kind"}, kind=w["kind"]))
+    for f, c, at in spec.get("probes", []):
+        s.add_probe(f, c, at)
+    if spec.get("dft"):
+        lo, hi, fr = spec["dft"]
+        s.add_dft(lo, hi, fr)
+    return s
+
+
+def product(spec, with_fields=True):
+    import photonoxide_amd as P
+    g = P.Grid3d(*spec["grid"])
+    b = spec["bounds"]
+    edges = []
+    for a in range(3):
+        edges.append(P.Periodic() if b["periodic"][a] else P.Pml(b["low"][a], b["high"][a]))
+    bd = P.Boundaries(edges[0], edges[1], edges[2],
+                      P.Cpml(kappa=b.get("kappa", 1.0), alpha=b.get("alpha", 0.0)))
+
+    def to_scene(sc):
+        bg, shapes = sc
+        return P.Scene(bg, [P.Box(lo, hi, v) if k == 0 else P.Cylinder((lo[0], lo[1]), hi[0], (lo[2], hi[2]), v)
+                            for k, lo, hi, v in shapes])
+
+    s = P.Simulation.new(g, to_scene(spec["eps"]), bd, spec["courant"])
+    if spec.get("sigma"):
+        s.with_conductivity(to_scene(spec["sigma"]))
+    if with_fields:
+        e, h = initial_fields(g.cells())
+        for c in range(3):
+            s.set_e(c, e[c])
+            s.set_h(c, h[c])
+    if spec.get("conductor"):
+        c, at = spec["conductor"]
+        s.conductor(c, at)
+    for f, c, at, w in spec.get("sources", []):
+        s.add_source(P.Source(P.Field(f), P.Axis(c), at, P.Waveform(w["kind"], w["frequency"], w["width"], w["delay"])))
+    for f, vals, w in spec.get("currents", []):
+        s.add_current(P.Current(P.Field(f), [(P.Axis(v[0]), v[1], v[2]) for v in vals],
+                                P.Waveform(w["kind"], w["frequency"], w["width"], w["delay"])))
+    for f, c, at in spec.get("probes", []):
+        s.add_probe(f, c, at)
+    if spec.get("dft"):
+        lo, hi, fr = spec["dft"]
+        s.add_dft(lo, hi, fr)
+    return s
+
+
+def product_fields(s):
+    return np.stack([s.e(c) for c in range(3)] + [s.h(c) for c in range(3)])

@@ -1,0 +1,142 @@
+"""The device path against the oracle, through the C ABI (GPU).
+
+- f64 on the device is the oracle's f64, bit for bit: fields, CPML, sources, probes, transforms
+  and dispersive media (every update the same operations in the same order, no contraction).
+- f32 on the device is the oracle's f32 arithmetic, bit for bit, and within the north star's
+  1e-5 relative L2 of the FP64 oracle.
+- The tiling, splitting a run in pieces, and repeating a run change no bit.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fixtures import INF, busy_spec, oracle, product, product_fields
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import photonoxide_amd as P
+    return P.Gpu(0)
+
+
+def dispersive_spec():
+    spec = busy_spec(False)
+    spec = dict(spec)
+    spec["media"] = [
+        (2.0, [("lorentz", 1.2, 1.6, 0.05), ("lorentz", 0.4, 2.5, 0.0)],
+         (0.0, [(0, (-0.5, -0.3, -INF), (-0.1, 0.3, INF), 1.0)])),
+        (1.0, [("drude", 2.0, 0.1), ("lorentz", 0.5, 1.9, 0.1), ("lorentz", 0.3, 2.4, 0.2)],
+         (0.0, [(1, (0.35, -0.1, -INF), (0.2, 0, INF), 1.0)])),
+    ]
+    return spec
+
+
+def with_media(spec, o, p):
+    import photonoxide_amd as P
+    for eps_inf, poles, inside in spec.get("media", []):
+        o.add_medium(eps_inf, poles, O.scene(*inside))
+        bg, shapes = inside
+        p.with_medium(eps_inf, poles, P.Scene(bg, [P.Box(lo, hi, v) if k == 0 else
+                                                    P.Cylinder((lo[0], lo[1]), hi[0], (lo[2], hi[2]), v)
+                                                    for k, lo, hi, v in shapes]))
+
+
+def build_pair(spec, gpu, precision):
+    import photonoxide_amd as P
+    o = oracle(spec)
+    p = product(spec)
+    with_media(spec, o, p)
+    if precision == P.Precision.F32:
+        o.use_f32()
+    p.set_device(gpu, precision)
+    return o, p
+
+
+def same_bits(a, b):
+    return np.array_equal(np.asarray(a, dtype=np.float64).view(np.uint64),
+                          np.asarray(b, dtype=np.float64).view(np.uint64))
+
+
+@pytest.mark.parametrize("two_d", [False, True], ids=["3d", "2d"])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_device_is_the_oracle_bit_for_bit(gpu, two_d, prec):
+    import photonoxide_amd as P
+    precision = P.Precision.F64 if prec == "f64" else P.Precision.F32
+    spec = busy_spec(two_d)
+    print(f"case busy two_d={two_d} {prec}", flush=True)
+    o, p = build_pair(spec, gpu, precision)
+    o.run(80)
+    p.run(80)
+    fo, fp = o.fields(), product_fields(p)
+    diff = np.abs(fo - fp).max()
+    assert same_bits(fo, fp), f"largest difference {diff}"
+    for n in range(len(spec["probes"])):
+        assert same_bits(o.probe(n), p.probe(n)), n
+    for f in (0, 1):
+        for c in range(3):
+            assert same_bits(o.dft(0, f, c).view(np.float64), p.dft(0, f, c).view(np.float64)), (f, c)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dispersive_media_bit_for_bit(gpu, prec):
+    import photonoxide_amd as P
+    precision = P.Precision.F64 if prec == "f64" else P.Precision.F32
+    print(f"case dispersive {prec}", flush=True)
+    o, p = build_pair(dispersive_spec(), gpu, precision)
+    o.run(120)
+    p.run(120)
+    fo, fp = o.fields(), product_fields(p)
+    assert same_bits(fo, fp), np.abs(fo - fp).max()
+
+
+def test_f32_within_north_star_tolerance_of_f64_oracle(gpu):
+    """Fields in f32 within 1e-5 relative L2 of the FP64 reference scheme after N steps."""
+    import photonoxide_amd as P
+    spec = busy_spec(False)
+    o = oracle(spec)
+    p = product(spec)
+    p.set_device(gpu, P.Precision.F32)
+    steps = 400
+    o.run(steps)
+    p.run(steps)
+    fo, fp = o.fields(), product_fields(p)
+    rel = np.linalg.norm(fo - fp) / np.linalg.norm(fo)
+    print(f"f32 vs f64 oracle, {steps} steps: relative L2 {rel:.3e}")
+    assert rel < 1e-5, rel
+
+
+@pytest.mark.parametrize("rows,planes", [(1, 1), (2, 3), (8, 19), (4, 5)])
+def test_tiling_changes_no_bit(gpu, rows, planes):
+    import photonoxide_amd as P
+    spec = busy_spec(False)
+    ref = product(spec)
+    ref.set_device(gpu, P.Precision.F32)
+    ref.run(50)
+    p = product(spec)
+    p.set_device(gpu, P.Precision.F32)
+    p.set_tiling(rows, planes)
+    assert p.tiling() == (rows, planes)
+    p.run(50)
+    assert same_bits(product_fields(ref), product_fields(p))
+
+
+def test_pieces_equal_one_run_and_repeat(gpu):
+    import photonoxide_amd as P
+    spec = busy_spec(False)
+    a = product(spec)
+    a.set_device(gpu, P.Precision.F32)
+    a.run(60)
+    b = product(spec)
+    b.set_device(gpu, P.Precision.F32)
+    b.run(17)
+    b.run(43)
+    c = product(spec)
+    c.set_device(gpu, P.Precision.F32)
+    c.run(60)
+    fa = product_fields(a)
+    assert same_bits(fa, product_fields(b))
+    assert same_bits(fa, product_fields(c))
+    for n in range(len(spec["probes"])):
+        assert same_bits(a.probe(n), b.probe(n))

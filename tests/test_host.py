@@ -1,0 +1,77 @@
+"""The native library on the CPU: it loads and exports what include/pxfdtd.h declares, and the
+host mirror's set-up (dt, averaged permittivity, walls, conductivity, CPML) is the oracle's, bit
+for bit. No GPU call is made."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from fixtures import INF, busy_spec, oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "pxfdtd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pxfdtd_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    from photonoxide_amd import _native as N
+    out = subprocess.run(["nm", "-D", "--defined-only", N.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (pxfdtd_\w+)", out))
+    declared = declared_functions()
+    assert declared, "no declarations found"
+    missing = [f for f in declared if f not in exported]
+    assert not missing, missing
+    for f in declared:
+        assert hasattr(N.lib, f)
+        assert f in N.SIGNATURES, f"{f} has no ctypes signature"
+
+
+def test_setup_is_the_oracles_bit_for_bit():
+    import photonoxide_amd as P
+    from fixtures import product
+    for two_d in (False, True):
+        spec = busy_spec(two_d)
+        a = oracle(spec, with_fields=False)
+        b = product(spec, with_fields=False)
+        assert a.dt() == b.dt()
+        for c in range(3):
+            ca0, cb0 = a.coefficients(c)
+            ca1, cb1 = b.coefficients(c)
+            assert np.array_equal(ca0.view(np.uint64), ca1.view(np.uint64)), c
+            assert np.array_equal(cb0.view(np.uint64), cb1.view(np.uint64)), c
+
+
+def test_canary_setup_matches():
+    import photonoxide_amd as P
+    g = P.Grid3d(48, 40, 1, 0.05, 0.05, 0.05, -1.2, -1.0, -0.025)
+    b = P.Boundaries(P.Pml(), P.Pml(), P.Periodic())
+    s = P.Simulation.new(g, P.Scene(1.0, [P.Box((-0.3, -0.2, -INF), (INF, 0.2, INF), 6.25)]), b, 0.9)
+    o = O.Sim(tuple(vars(g).values()), O.bounds(periodic=(0, 0, 1)),
+              O.scene(1.0, [(0, (-0.3, -0.2, -INF), (INF, 0.2, INF), 6.25)]), 0.9)
+    assert s.dt() == o.dt()
+    for c in range(3):
+        assert np.array_equal(s.coefficients(c)[1], o.coefficients(c)[1])
+
+
+def test_errors_are_statuses():
+    import photonoxide_amd as P
+    from photonoxide_amd._native import NativeError
+    g = P.Grid3d(8, 8, 8, 0.05, 0.05, 0.05)
+    with pytest.raises(NativeError, match="Courant"):
+        P.Simulation.new(g, P.Scene(1.0), P.Boundaries.walls(), 1.5)
+    with pytest.raises(NativeError, match="leave nothing"):
+        P.Simulation.new(g, P.Scene(1.0), P.Boundaries.cpml_all(4), 0.9)
+    s = P.Simulation.new(g, P.Scene(1.0), P.Boundaries.cpml_all(2), 0.9)
+    with pytest.raises(NativeError, match="outside"):
+        s.add_probe(0, 0, (8, 0, 0))
+    with pytest.raises(NativeError, match="no device"):
+        s.run(1)
+    with pytest.raises(NativeError, match="Nyquist"):
+        s.add_dft((0, 0, 0), (1, 1, 1), [1e6])
