@@ -32,14 +32,16 @@ double or_cpu_rate_f32(int nx, int ny, int nz, int pml, int steps, int repeats, 
         f[q] = aligned_alloc(64, ((N * 4 + 63) / 64) * 64);
         for (long r = 0; r < N; r++) f[q][r] = 1e-3f * (float)((r * 2654435761u + q) % 1000) - 0.5f;
     }
-    float *cb[3];
+    float *cb[3], *ca[3];
     for (int c = 0; c < 3; c++) {
         cb[c] = aligned_alloc(64, ((N * 4 + 63) / 64) * 64);
-        for (long r = 0; r < N; r++) cb[c][r] = dt / 2.1f;
+        ca[c] = aligned_alloc(64, ((N * 4 + 63) / 64) * 64);
+        for (long r = 0; r < N; r++) { cb[c][r] = dt / 2.1f; ca[c][r] = 1.0f; }
     }
-    float fac[3][1024];
+    float *zeros = calloc(nx, sizeof(float));
+    static float fac[3][4096];
     for (int a = 0; a < 3; a++)
-        for (int m = 0; m < n[a] && m < 1024; m++) fac[a][m] = 1.0f / h;
+        for (int m = 0; m < n[a] && m < 4096; m++) fac[a][m] = 1.0f / h;
     cslab slabs[24];
     int ns = 0;
     for (int a = 0; a < 3; a++)
@@ -71,33 +73,38 @@ double or_cpu_rate_f32(int nx, int ny, int nz, int pml, int steps, int repeats, 
                     for (long row = chunk * rows_per; row < (chunk + 1) * rows_per && row < nrows; row++) {
                         const int j = (int)(row % ny), k = (int)(row / ny);
                         const long base = row * nx;
-                        const long oy = forward ? (j + 1 < ny ? nx : 0) : (j > 0 ? -nx : 0);
-                        const long oz = forward ? (k + 1 < nz ? plane : 0) : (k > 0 ? -plane : 0);
-                        const float ky = (forward ? (j + 1 < ny) : (j > 0)) ? 1.0f : 0.0f;
-                        const float kz = (forward ? (k + 1 < nz) : (k > 0)) ? 1.0f : 0.0f;
                         const float fy = fac[1][j], fz = fac[2][k];
-                        for (int c = 0; c < 3; c++) {
-                            float *o = out[c] + base;
-                            const float *fa = in[(c + 1) % 3] + base, *fb = in[(c + 2) % 3] + base;
-                            const float *cbr = cb[c] + base;
-                            for (int i = 0; i < nx; i++) {
-                                float da, db;
-                                /* D_a f_b - D_b f_a, with a = c + 1, b = c + 2 */
-                                const int a = (c + 1) % 3, b = (c + 2) % 3;
-                                float nb, na;
-                                if (a == 0) nb = forward ? (i + 1 < nx ? fb[i + 1] : 0.f) : (i > 0 ? fb[i - 1] : 0.f);
-                                else if (a == 1) nb = ky * fb[i + oy];
-                                else nb = kz * fb[i + oz];
-                                if (b == 0) na = forward ? (i + 1 < nx ? fa[i + 1] : 0.f) : (i > 0 ? fa[i - 1] : 0.f);
-                                else if (b == 1) na = ky * fa[i + oy];
-                                else na = kz * fa[i + oz];
-                                da = forward ? nb - fb[i] : fb[i] - nb;
-                                db = forward ? na - fa[i] : fa[i] - na;
-                                const float fa_ = a == 0 ? fac[0][i] : (a == 1 ? fy : fz);
-                                const float fb_ = b == 0 ? fac[0][i] : (b == 1 ? fy : fz);
-                                const float curl = da * fa_ - db * fb_;
-                                o[i] = forward ? o[i] - dt * curl : o[i] + cbr[i] * curl;
-                            }
+                        const float *fx = fac[0];
+                        if (forward) {
+                            /* H -= dt curl E, forward differences; rows j+1, k+1 (zeros beyond) */
+                            const float *ex = f[0] + base, *ey = f[1] + base, *ez = f[2] + base;
+                            const float *exy = j + 1 < ny ? ex + nx : zeros, *ezy = j + 1 < ny ? ez + nx : zeros;
+                            const float *exz = k + 1 < nz ? ex + plane : zeros, *eyz = k + 1 < nz ? ey + plane : zeros;
+                            float *hx = f[3] + base, *hy = f[4] + base, *hz = f[5] + base;
+                            for (int i = 0; i < nx; i++)
+                                hx[i] = hx[i] - dt * ((ezy[i] - ez[i]) * fy - (eyz[i] - ey[i]) * fz);
+                            for (int i = 0; i < nx - 1; i++)
+                                hy[i] = hy[i] - dt * ((exz[i] - ex[i]) * fz - (ez[i + 1] - ez[i]) * fx[i]);
+                            hy[nx - 1] = hy[nx - 1] - dt * ((exz[nx - 1] - ex[nx - 1]) * fz - (0.f - ez[nx - 1]) * fx[nx - 1]);
+                            for (int i = 0; i < nx - 1; i++)
+                                hz[i] = hz[i] - dt * ((ey[i + 1] - ey[i]) * fx[i] - (exy[i] - ex[i]) * fy);
+                            hz[nx - 1] = hz[nx - 1] - dt * ((0.f - ey[nx - 1]) * fx[nx - 1] - (exy[nx - 1] - ex[nx - 1]) * fy);
+                        } else {
+                            /* E = ca E + cb curl H, back differences; rows j-1, k-1 */
+                            const float *hx = f[3] + base, *hy = f[4] + base, *hz = f[5] + base;
+                            const float *hxy = j > 0 ? hx - nx : zeros, *hzy = j > 0 ? hz - nx : zeros;
+                            const float *hxz = k > 0 ? hx - plane : zeros, *hyz = k > 0 ? hy - plane : zeros;
+                            float *ex = f[0] + base, *ey = f[1] + base, *ez = f[2] + base;
+                            const float *ca0 = ca[0] + base, *ca1 = ca[1] + base, *ca2 = ca[2] + base;
+                            const float *cb0 = cb[0] + base, *cb1 = cb[1] + base, *cb2 = cb[2] + base;
+                            for (int i = 0; i < nx; i++)
+                                ex[i] = ca0[i] * ex[i] + cb0[i] * ((hz[i] - hzy[i]) * fy - (hy[i] - hyz[i]) * fz);
+                            ey[0] = ca1[0] * ey[0] + cb1[0] * ((hx[0] - hxz[0]) * fz - (hz[0] - 0.f) * fx[0]);
+                            for (int i = 1; i < nx; i++)
+                                ey[i] = ca1[i] * ey[i] + cb1[i] * ((hx[i] - hxz[i]) * fz - (hz[i] - hz[i - 1]) * fx[i]);
+                            ez[0] = ca2[0] * ez[0] + cb2[0] * ((hy[0] - 0.f) * fx[0] - (hx[0] - hxy[0]) * fy);
+                            for (int i = 1; i < nx; i++)
+                                ez[i] = ca2[i] * ez[i] + cb2[i] * ((hy[i] - hy[i - 1]) * fx[i] - (hx[i] - hxy[i]) * fy);
                         }
                     }
                 }
@@ -140,7 +147,8 @@ double or_cpu_rate_f32(int nx, int ny, int nz, int pml, int steps, int repeats, 
         if (el < best) best = el;
     }
     for (int q = 0; q < 6; q++) free(f[q]);
-    for (int c = 0; c < 3; c++) free(cb[c]);
+    for (int c = 0; c < 3; c++) { free(cb[c]); free(ca[c]); }
+    free(zeros);
     for (int q = 0; q < ns; q++) { free(slabs[q].b); free(slabs[q].a); free(slabs[q].psi); }
     return best;
 }
