@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12):
+def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12, plain=False):
     import photonoxide_amd as P
     g = P.Grid3d(nx, ny, nz, h, h, h, 0.0, -ny * h / 2, -nz * h / 2)
     n_si, n_ox = 3.48, 1.444
@@ -56,6 +56,8 @@ def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12):
 
     eps = np.stack([np.broadcast_to(eps_at(c), (nz, ny, nx)) for c in range(3)])
     s = P.Simulation.with_eps(g, eps, P.Boundaries.cpml_all(pml), 0.9)
+    if plain:
+        return s, g
     # the source: E_y across the input arm with a Gaussian profile, a pulse at 1.55 um
     f0 = 1 / 1.55
     wave = P.Waveform.pulse(f0, 1 / 1.45 - 1 / 1.65)
@@ -104,6 +106,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--planes", type=int, default=0)
+    ap.add_argument("--plain", action="store_true",
+                    help="the kernel alone: no source, no monitors (for characterization)")
+    ap.add_argument("--pml", type=int, default=12)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: each box has one MI355X, and "
@@ -113,8 +118,8 @@ def main():
     import photonoxide_amd as P
     from photonoxide_amd import _native as N
 
-    nx, ny, nz, pml = 384, 256, 192, 12
-    s, g = mmi_problem(nx, ny, nz, 0.02, pml)
+    nx, ny, nz, pml = 384, 256, 192, args.pml
+    s, g = mmi_problem(nx, ny, nz, 0.02, pml, plain=args.plain)
     gpu = P.Gpu(0)
     s.set_device(gpu, P.Precision.F32)
     if args.rows or args.planes:
@@ -150,7 +155,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "mmi_1x2_soi_384x256x192_dx20nm_cpml12_f32",
+        "config": {"workload": f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else ""),
                    "cells": cells, "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
                    "monitors": "2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums",
                    "psi_values": psi},

@@ -12,16 +12,23 @@
 // pitch = nx rounded up to 64 values, so a wave's 64 lanes read whole aligned rows), in two
 // copies: a step reads copy p and writes copy 1 - p. The CPML's psi lives only in the slabs,
 // one array per (field, component, axis, side) as the reference keeps it ([k][j][i] over the
-// slab's box), also in two copies. E's coefficients are a per-component material id (uint8,
-// or uint16 when a component has more than 256 distinct (ca, cb) pairs) into a table staged
-// in LDS: 1 byte per value instead of 8.
+// slab's box), also in two copies. E's coefficients are, where at most 256 distinct
+// (ca, cb, medium) triples over the three components occur, one uint8 material id per cell
+// into a table staged in LDS (1 byte per cell instead of 24); otherwise whole ca, cb arrays.
 //
-// One workgroup owns a tile of 64 x TY values in x and y and marches along z over a chunk of
-// planes. At each plane it computes H's new values over the tile and the halo one value
-// below in x and y (the neighbours E's back differences read), exchanges them through LDS,
-// then updates E over the tile. H at the plane below is kept in registers from the previous
-// plane; the chunk's first plane computes it once (a prologue). Only the tile's own values
-// are written, so the redundant halo work never races.
+// One workgroup owns a tile of 64 x TY values in x and y (a wavefront per row, plus one
+// wavefront for the halo) and marches along z over a chunk of planes. At each plane:
+//   1. E of the next plane goes into an LDS ring of two planes over the tile and its halo
+//      (one value around it), from registers loaded one plane ahead;
+//   2. the loads for the plane after are issued, so HBM traffic overlaps the arithmetic;
+//   3. H's new values over the tile, and over the row below and the column left of it (the
+//      neighbours E's back differences read, recomputed here rather than exchanged), from the
+//      E planes in LDS; the tile's own go to HBM, all to LDS;
+//   4. E's new values over the tile from the new H in LDS and, along z, the new H of the
+//      plane below kept in registers.
+// The chunk's first plane computes H of the plane below it once (a prologue). Only the
+// tile's own values are written, so the redundant halo work never races, and every value is
+// the same bits whatever the tiling.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -46,55 +53,60 @@ struct MediumGroup {
     double coef[4][3];  // alpha, xi, gamma per pole
 };
 
+// What a slab cell, an injection or a dispersive value needs: kept in device memory and read
+// where used, so that it holds no scalar registers across the plane loop.
 template <typename T>
-struct StepArgs {
-    int nx, ny, nz, pitch;
-    long long plane;          // pitch * ny
-    int periodic[3];
-    int tiles_x, tiles_y, chunks, kz;
+struct StepCold {
     int low[3], high[3];      // CPML cells per axis side
-    const T *e_in[3], *h_in[3];
-    T *e_out[3], *h_out[3];
-    // coefficients, mode 0: a uint8 id per value into a table staged in LDS
-    const uint8_t *ids[3];
-    const T *table;           // [3][table_stride][2]: ca, cb
-    const int8_t *group_of;   // [3][table_stride]: media group or -1
-    int table_stride;
-    // mode 1: ca, cb and the media group per value
-    const T *ca[3], *cb[3];
-    const int8_t *grp[3];     // may be null: no media
-    T dt;
-    T step[3];
-    const T *fe[3], *fh[3];   // kappa/step along each axis, nodes and halves
-    const T *pb[3][2][2];     // CPML b per [axis][side][field E=0,H=1], count values
-    const T *pa[3][2][2];
+    int cpml_off[3][2][2];    // b of [axis][side][field E=0,H=1] in coef; a follows, count each
+    int psi_off[2][3][3][2];  // [field][component][axis][side], -1 if none
     const T *psi_in;
     T *psi_out;
-    long long psi_off[2][3][3][2];  // [field][component][axis][side], -1 if none
     // injections: per tile a list sorted by plane, inj_off[tile * (nz + 1) + k]
     const int32_t *inj_off;
     const Injection *inj;
-    const double *waves;      // this step's emitters' w, complex
+    const double *waves;      // the call's emitters' w, [steps][emitters] complex
+    int emitters;
     // media (dense per-value state, read only where a value's group is set)
     const MediumGroup *groups;
-    int ngroups;
     int max_poles;
-    T *m_before;              // [3][N]
-    T *m_cur, *m_prev;        // [3][N][max_poles]
-    long long mstride;        // N with pitch (values per component)
+    T *m_before;              // [3][nval]
+    T *m_cur, *m_prev;        // [3][nval][max_poles]
 };
 
+// The kernel's arguments: what every plane reads.
 template <typename T>
-__device__ __forceinline__ T ld(const T *a, long long idx) {
-    return idx >= 0 ? a[idx] : T(0);
-}
+struct StepArgs {
+    int nx, ny, nz, pitch;
+    int plane;                // pitch * ny
+    int nval;                 // values per component: a field's components are nval apart
+    int flags;                // bit a: axis a periodic; bit 3: CPML slabs; bit 4: injections
+    int tiles_x, tiles_y, kz;
+    int blocks;               // tiles_x * tiles_y * chunks
+    int per_xcd;              // blocks dealt to each XCD, contiguous in the tile order
+    const T *ein, *hin;       // copy p, [3][nval] each
+    T *eout, *hout;           // copy 1 - p
+    // coefficients, mode 0: a uint8 material id per cell into a table staged in LDS,
+    // [256][3][2] (ca, cb per component) then the media groups [256][3] as int8
+    const uint8_t *ids;
+    const T *table;
+    // mode 1: ca then cb, [2][3][nval], and the media group per value [3][nval] (or null)
+    const T *cacb;
+    const int8_t *grp;
+    T dt;
+    T step[3];
+    // kappa/step at the nodes (fe) and halves (fh) along x, y, z: fe_x, fh_x, fe_y, fh_y,
+    // fe_z, fh_z, n_axis values each; then the CPML's b and a
+    const T *coef;
+    const StepCold<T> *cold;  // the one for this step's parity
+    int wave_step;            // this step's row of cold->waves
+};
 
-// index of the neighbour one step forward/back along an axis, -1 beyond a wall
-__device__ __forceinline__ long long fwd(long long r, int m, int n, long long stride, int per) {
-    return m + 1 < n ? r + stride : (per ? r - (long long)(n - 1) * stride : -1);
-}
-__device__ __forceinline__ long long bwd(long long r, int m, int n, long long stride, int per) {
-    return m > 0 ? r - stride : (per ? r + (long long)(n - 1) * stride : -1);
+// A workgroup barrier that waits only for this wave's LDS operations, not for its global
+// loads: the loads issued for the next plane stay in flight across it. (__syncthreads also
+// waits for every outstanding global load, which would drain that prefetch.)
+__device__ __forceinline__ void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // psi's slab side for coordinate m along an axis: 0 low, 1 high, -1 none
@@ -102,250 +114,434 @@ __device__ __forceinline__ int side_of(int m, int n, int low, int high) {
     return m < low ? 0 : (m >= n - high ? 1 : -1);
 }
 
-template <typename T>
-__device__ __forceinline__ T psi_update(const StepArgs<T> &A, int F, int c, int w, int side,
-                                        int i, int j, int k, T d, bool write) {
-    const int m[3] = {i, j, k};
-    const int n[3] = {A.nx, A.ny, A.nz};
-    int from[3] = {0, 0, 0}, cnt[3] = {n[0], n[1], n[2]};
-    from[w] = side == 0 ? 0 : n[w] - A.high[w];
-    cnt[w] = side == 0 ? A.low[w] : A.high[w];
-    const long long q = ((long long)(m[2] - from[2]) * cnt[1] + (m[1] - from[1])) * cnt[0] +
-                        (m[0] - from[0]);
-    const long long off = A.psi_off[F][c][w][side] + q;
-    const int s = m[w] - from[w];
-    const T b = A.pb[w][side][F][s], a = A.pa[w][side][F][s];
-    const T v = b * A.psi_in[off] + a * d;
-    if (write) A.psi_out[off] = v;
-    return v;
+// a coordinate wrapped onto the grid across a periodic side, or -1 beyond a wall
+__device__ __forceinline__ int wrap(int m, int n, int per) {
+    if (m >= 0 && m < n) return m;
+    if (!per) return -1;
+    return m < 0 ? m + n : m - n;
 }
 
-// H's new components at (i, j, k) in `mask` (bit c), from copy p; psi written if `write`.
+// Where a value (i, j, k) sits in the CPML slabs: per axis its side (-1: none), its index
+// along the axis within the slab, and its index within the slab's psi box.
+struct SlabAt {
+    int side[3], s[3], q[3];
+};
+
 template <typename T>
-__device__ __forceinline__ void h_new(const StepArgs<T> &A, int i, int j, int k, int mask,
-                                      bool write, T out[3]) {
-    const long long r = (long long)k * A.plane + (long long)j * A.pitch + i;
-    const long long rx = fwd(r, i, A.nx, 1, A.periodic[0]);
-    const long long ry = fwd(r, j, A.ny, A.pitch, A.periodic[1]);
-    const long long rz = fwd(r, k, A.nz, A.plane, A.periodic[2]);
-    const T ex = A.e_in[0][r], ey = A.e_in[1][r], ez = A.e_in[2][r];
-    const T dt = A.dt;
-    const int sx = side_of(i, A.nx, A.low[0], A.high[0]);
-    const int sy = side_of(j, A.ny, A.low[1], A.high[1]);
-    const int sz = side_of(k, A.nz, A.low[2], A.high[2]);
-    if (mask & 1) {  // Hx: D_y(Ez) fy - D_z(Ey) fz
-        const T dy = ld(A.e_in[2], ry) - ez, dz = ld(A.e_in[1], rz) - ey;
-        T v = A.h_in[0][r];
-        v = v - dt * (dy * A.fh[1][j] - dz * A.fh[2][k]);
-        if (sy >= 0) v = v - dt * (T(1) * psi_update(A, 1, 0, 1, sy, i, j, k, dy / A.step[1], write));
-        if (sz >= 0) v = v - dt * (T(-1) * psi_update(A, 1, 0, 2, sz, i, j, k, dz / A.step[2], write));
-        out[0] = v;
+__device__ __forceinline__ SlabAt slab_at(const StepArgs<T> &A, const StepCold<T> &K, int i,
+                                          int j, int k) {
+    SlabAt P;
+    const int n[3] = {A.nx, A.ny, A.nz};
+    const int m[3] = {i, j, k};
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        P.side[w] = side_of(m[w], n[w], K.low[w], K.high[w]);
+        const int from = P.side[w] == 1 ? n[w] - K.high[w] : 0;
+        const int cnt = P.side[w] == 1 ? K.high[w] : K.low[w];
+        P.s[w] = m[w] - from;
     }
-    if (mask & 2) {  // Hy: D_z(Ex) fz - D_x(Ez) fx
-        const T dz = ld(A.e_in[0], rz) - ex, dx = ld(A.e_in[2], rx) - ez;
-        T v = A.h_in[1][r];
-        v = v - dt * (dz * A.fh[2][k] - dx * A.fh[0][i]);
-        if (sx >= 0) v = v - dt * (T(-1) * psi_update(A, 1, 1, 0, sx, i, j, k, dx / A.step[0], write));
-        if (sz >= 0) v = v - dt * (T(1) * psi_update(A, 1, 1, 2, sz, i, j, k, dz / A.step[2], write));
-        out[1] = v;
-    }
-    if (mask & 4) {  // Hz: D_x(Ey) fx - D_y(Ex) fy
-        const T dx = ld(A.e_in[1], rx) - ey, dy = ld(A.e_in[0], ry) - ex;
-        T v = A.h_in[2][r];
-        v = v - dt * (dx * A.fh[0][i] - dy * A.fh[1][j]);
-        if (sx >= 0) v = v - dt * (T(1) * psi_update(A, 1, 2, 0, sx, i, j, k, dx / A.step[0], write));
-        if (sy >= 0) v = v - dt * (T(-1) * psi_update(A, 1, 2, 1, sy, i, j, k, dy / A.step[1], write));
-        out[2] = v;
-    }
+    const int cx = P.side[0] == 1 ? K.high[0] : K.low[0];
+    const int cy = P.side[1] == 1 ? K.high[1] : K.low[1];
+    P.q[0] = (k * A.ny + j) * cx + P.s[0];
+    P.q[1] = (k * cy + P.s[1]) * A.nx + i;
+    P.q[2] = (P.s[2] * A.ny + j) * A.nx + i;
+    return P;
+}
+
+// psi <- b psi + a d for (field F, component C, axis W) at a slab value; written if asked
+template <int F, int C, int W, typename T>
+__device__ __forceinline__ T psi_at(const StepArgs<T> &A, const StepCold<T> &K, const SlabAt &P,
+                                    T d, bool write) {
+    const int side = P.side[W];
+    const int off = (side == 1 ? K.psi_off[F][C][W][1] : K.psi_off[F][C][W][0]) + P.q[W];
+    const int co = side == 1 ? K.cpml_off[W][1][F] : K.cpml_off[W][0][F];
+    const int cnt = side == 1 ? K.high[W] : K.low[W];
+    const T v = A.coef[co + P.s[W]] * K.psi_in[off] + A.coef[co + cnt + P.s[W]] * d;
+    if (write) K.psi_out[off] = v;
+    return v;
 }
 
 // the injections of plane k on the value (field, i, j) in `mask`, in the tile's list order
 template <typename T, int FIELD>
-__device__ __forceinline__ void inject(const StepArgs<T> &A, int tile, int i, int j, int k,
-                                       int mask, T v[3], const T cb[3]) {
-    if (!A.inj_off) return;
-    const int32_t *off = A.inj_off + (long long)tile * (A.nz + 1);
+__device__ __forceinline__ void inject(const StepArgs<T> &A, const StepCold<T> &K, int tile, int i,
+                                       int j, int k, int mask, T &v0, T &v1, T &v2, T cb0, T cb1,
+                                       T cb2) {
+    const int32_t *off = K.inj_off + (long long)tile * (A.nz + 1);
     const int lo = off[k], hi = off[k + 1];
     for (int q = lo; q < hi; q++) {
-        const Injection in = A.inj[q];
+        const Injection in = K.inj[q];
         const int c = in.code & 3;
         if ((in.code >> 2) != FIELD || in.i != i || in.j != j || !((mask >> c) & 1)) continue;
-        const double wr = A.waves[2 * in.emitter], wi = A.waves[2 * in.emitter + 1];
+        const double *w = K.waves + 2 * ((long long)A.wave_step * K.emitters + in.emitter);
+        const double wr = w[0], wi = w[1];
         const T amount = T(in.a_re * wr - in.a_im * wi);
-        if (FIELD == 1)
-            v[c] = v[c] - A.dt * amount;
-        else
-            v[c] = v[c] - cb[c] * amount;
+        // selects, not an indexed array: the three values stay in registers
+        if (FIELD == 1) {
+            const T d = A.dt * amount;
+            v0 = c == 0 ? v0 - d : v0;
+            v1 = c == 1 ? v1 - d : v1;
+            v2 = c == 2 ? v2 - d : v2;
+        } else {
+            v0 = c == 0 ? v0 - cb0 * amount : v0;
+            v1 = c == 1 ? v1 - cb1 * amount : v1;
+            v2 = c == 2 ? v2 - cb2 * amount : v2;
+        }
     }
 }
 
+// the ADE rest of E's update and the polarization currents' step at one value
+// (src/fdtd/media.rs:670-705); eold is E^n, the value saved before the update
+template <typename T>
+__device__ __forceinline__ T media_update(const StepArgs<T> &A, const StepCold<T> &K, int g, int c,
+                                          int r, T v, T cb, T eold) {
+    const MediumGroup &G = K.groups[g];
+    const long long s = (long long)c * A.nval + r;
+    const int mp = K.max_poles;
+    T *m_cur = K.m_cur, *m_prev = K.m_prev;
+    const T before = K.m_before[s];
+    T rest = T(G.chi) * before;
+    T cur[4], prv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q >= G.poles) break;
+        cur[q] = m_cur[s * mp + q];
+        prv[q] = m_prev[s * mp + q];
+        const T alpha = T(G.coef[q][0]), xi = T(G.coef[q][1]);
+        rest = rest - T(0.5) * ((alpha + T(1)) * cur[q] + xi * prv[q]);
+    }
+    const T next = v + cb * rest;
+    const T slope = (next - before) / (T(2) * A.dt);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q >= G.poles) break;
+        const T alpha = T(G.coef[q][0]), xi = T(G.coef[q][1]), gamma = T(G.coef[q][2]);
+        const T jn = alpha * cur[q] + xi * prv[q] + gamma * slope;
+        m_prev[s * mp + q] = cur[q];
+        m_cur[s * mp + q] = jn;
+    }
+    K.m_before[s] = eold;
+    return next;
+}
+
+// What a thread loads one or two planes ahead: E of a plane (its own value and a halo
+// value), old H of another (own, and the halo row's Hx, Hz and column's Hy, Hz) and the
+// material id.
+template <typename T>
+struct Ring {
+    T e0, e1, e2;       // own E
+    T x0, x1, x2;       // halo E
+    T h0, h1, h2;       // own old H
+    T r0, r2, l1, l2;   // halo row Hx, Hz; halo column Hy, Hz
+    uint8_t id;
+};
+
 template <typename T, int MODE, int TY>
-__global__ __launch_bounds__(TX *(TY + 2)) void fused_step(StepArgs<T> A) {
-    __shared__ T hs[3][TY + 1][TX + 1];
+__global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
+    constexpr int W = TX + 2;   // LDS columns: i0 - 1 .. i0 + 64
+    constexpr int HR = TY + 2;  // LDS rows: j0 - 1 .. j0 + TY
+    constexpr int NT = TX * (TY + 1);
+    constexpr int NHALO = 2 * W + 2 * TY;  // E positions around the tile
+    static_assert(NHALO <= NT, "the halo has one thread per position");
+    __shared__ T es[2][3][HR][W];        // E, two planes
+    __shared__ T hs[3][TY + 1][TX + 1];  // new H: rows j0 - 1 .., columns i0 - 1 ..
+    __shared__ StepCold<T> K;            // the slab, injection and media parameters
     extern __shared__ unsigned char smem_raw[];
-    T *tab = reinterpret_cast<T *>(smem_raw);  // [3][stride][2]
-    int8_t *grp = reinterpret_cast<int8_t *>(tab + 6 * A.table_stride);
+    T *tab = reinterpret_cast<T *>(smem_raw);                 // [256][3][2]
+    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);  // [256][3]
+
     const int lane = threadIdx.x, wave = threadIdx.y;
+    const int tid = wave * TX + lane;
     if (MODE == 0) {
-        for (int q = wave * TX + lane; q < 6 * A.table_stride; q += TX * (TY + 2)) tab[q] = A.table[q];
-        for (int q = wave * TX + lane; q < 3 * A.table_stride; q += TX * (TY + 2)) grp[q] = A.group_of[q];
+        for (int q = tid; q < 256 * 6; q += NT) tab[q] = A.table[q];
+        const int8_t *g = reinterpret_cast<const int8_t *>(A.table + 256 * 6);
+        for (int q = tid; q < 256 * 3; q += NT) grp[q] = g[q];
+    }
+    {
+        const int *src = reinterpret_cast<const int *>(A.cold);
+        int *dst = reinterpret_cast<int *>(&K);
+        for (int q = tid; q < (int)(sizeof(StepCold<T>) / 4); q += NT) dst[q] = src[q];
     }
 
-    const int b = blockIdx.x;
+    // blocks are dealt round-robin to the 8 XCDs: give each XCD a contiguous run of tiles,
+    // so neighbouring tiles (which read each other's edges) share an L2
+    const int b = (int)(blockIdx.x % 8) * A.per_xcd + (int)(blockIdx.x / 8);
+    if (b >= A.blocks) return;
     const int tiles = A.tiles_x * A.tiles_y;
     const int bz = b / tiles, t = b % tiles;
     const int by = t / A.tiles_x, bx = t % A.tiles_x;
     const int i0 = bx * TX, j0 = by * TY;
     const int kz0 = bz * A.kz;
     const int kz1 = min(A.nz, kz0 + A.kz);
+    const int nx = A.nx, ny = A.ny, nz = A.nz, pitch = A.pitch, plane = A.plane;
+    const int perx = A.flags & 1, pery = (A.flags >> 1) & 1, perz = (A.flags >> 2) & 1;
+    const bool slabs = (A.flags & 8) != 0, injects = (A.flags & 16) != 0;
 
-    // what this thread computes: own value, a value of the halo row (j0 - 1) or of the halo
-    // column (i0 - 1)
-    int role, hi_ = -1, hj = -1, mask = 0;
-    if (wave < TY) {
-        role = 0;
-        hi_ = i0 + lane;
-        hj = j0 + wave;
-        mask = 7;
-        if (hi_ >= A.nx || hj >= A.ny) role = -1;
-    } else if (wave == TY) {
-        role = 1;  // halo row: Hz, Hx
-        hi_ = i0 + lane;
-        hj = j0 - 1;
-        mask = 5;
-        if (hj < 0) hj = A.periodic[1] ? A.ny - 1 : -2;
-        if (hi_ >= A.nx || hj == -2) role = -1;
-    } else {
-        role = 2;  // halo column: Hz, Hy
-        hi_ = i0 - 1;
-        hj = j0 + lane - 1;
-        mask = 6;
-        if (hi_ < 0) hi_ = A.periodic[0] ? A.nx - 1 : -2;
-        if (lane == 0 || lane > TY || hj >= A.ny || hi_ == -2) role = -1;
+    // own value
+    const bool own = wave < TY;
+    const int oi = i0 + lane, oj = j0 + wave;
+    const bool own_ok = own && oi < nx && oj < ny;
+    const int orow = oj * pitch + oi;  // offset within a plane
+    // halo E position this thread loads: row j0-1, row j0+TY, column i0-1, column i0+64
+    int hrho = -1, hkap = -1, hoff = -1;
+    if (tid < NHALO) {
+        if (tid < W) { hrho = 0; hkap = tid; }
+        else if (tid < 2 * W) { hrho = HR - 1; hkap = tid - W; }
+        else if (tid < 2 * W + TY) { hrho = 1 + tid - 2 * W; hkap = 0; }
+        else { hrho = 1 + tid - 2 * W - TY; hkap = W - 1; }
+        const int wi = wrap(i0 - 1 + hkap, nx, perx);
+        const int wj = wrap(j0 - 1 + hrho, ny, pery);
+        hoff = (wi < 0 || wj < 0) ? -1 : wj * pitch + wi;
     }
-    // LDS slot
-    const int li = role == 2 ? 0 : lane + 1;
-    const int lj = role == 0 ? wave + 1 : (role == 1 ? 0 : lane);
-    __syncthreads();
-
-    T hprev[3] = {T(0), T(0), T(0)};
-    if (role == 0) {
-        if (kz0 > 0 || A.periodic[2]) {
-            const int kp = kz0 > 0 ? kz0 - 1 : A.nz - 1;
-            T v[3];
-            h_new(A, hi_, hj, kp, 3, false, v);
-            const T nocb[3] = {T(0), T(0), T(0)};
-            inject<T, 1>(A, t, hi_, hj, kp, 3, v, nocb);
-            hprev[0] = v[0];
-            hprev[1] = v[1];
-        }
+    // own lanes beyond the grid hold the wrapped value (periodic sides) or zero
+    int eoff = -1;  // where this own lane's LDS E value comes from
+    if (own) {
+        const int wi = wrap(oi, nx, perx), wj = wrap(oj, ny, pery);
+        eoff = (wi < 0 || wj < 0) ? -1 : wj * pitch + wi;
     }
+    // halo H positions this thread computes (halo wave): row j0-1 at i0+lane (Hz, Hx);
+    // column i0-1 at j0+lane (Hz, Hy) for lane < TY
+    const bool hw = wave == TY;
+    const int hrj = wrap(j0 - 1, ny, pery);
+    const bool hrow_ok = hw && oi < nx && hrj >= 0;
+    const int hci = wrap(i0 - 1, nx, perx);
+    const int hcj = j0 + lane;
+    const bool hcol_ok = hw && lane < TY && hcj < ny && hci >= 0;
+    const int hrow_off = hrow_ok ? hrj * pitch + oi : -1, hcol_off = hcol_ok ? hcj * pitch + hci : -1;
+    const int own_off = own_ok ? orow : -1;
 
-    for (int k = kz0; k < kz1; k++) {
-        // phase 1: H at the plane over the tile and its halo
-        if (role >= 0) {
-            T v[3] = {T(0), T(0), T(0)};
-            h_new(A, hi_, hj, k, mask, role == 0, v);
-            const T nocb[3] = {T(0), T(0), T(0)};
-            inject<T, 1>(A, t, hi_, hj, k, mask, v, nocb);
-            if (role == 0) {
-                const long long r = (long long)k * A.plane + (long long)hj * A.pitch + hi_;
-                A.h_out[0][r] = v[0];
-                A.h_out[1][r] = v[1];
-                A.h_out[2][r] = v[2];
-            }
-            if (mask & 1) hs[0][lj][li] = v[0];
-            if (mask & 2) hs[1][lj][li] = v[1];
-            if (mask & 4) hs[2][lj][li] = v[2];
+    auto zplane = [&](int kk) -> int { return wrap(kk, nz, perz); };
+    // loads from a valid address always, the value zeroed beyond a wall: no branch per load
+    auto ld = [&](const T *a, int kp, int off) -> T {
+        const bool ok = kp >= 0 && off >= 0;
+        const T v = a[ok ? kp * plane + off : 0];
+        return ok ? v : T(0);
+    };
+    const int nv = A.nval;
+    const T *ein = A.ein, *hin = A.hin;
+    auto load_e = [&](Ring<T> &R, int kk) {
+        const int kp = zplane(kk);
+        R.e0 = ld(ein, kp, eoff); R.e1 = ld(ein + nv, kp, eoff); R.e2 = ld(ein + 2 * nv, kp, eoff);
+        R.x0 = ld(ein, kp, hoff); R.x1 = ld(ein + nv, kp, hoff); R.x2 = ld(ein + 2 * nv, kp, hoff);
+    };
+    auto load_h = [&](Ring<T> &R, int kk) {
+        const int kp = zplane(kk);
+        if (own) {
+            R.h0 = ld(hin, kp, own_off); R.h1 = ld(hin + nv, kp, own_off); R.h2 = ld(hin + 2 * nv, kp, own_off);
         } else {
-            // beyond a wall or the grid: zero neighbours for the back differences
-            if (wave == TY && lane < TX) { hs[0][0][lane + 1] = T(0); hs[2][0][lane + 1] = T(0); }
-            if (wave == TY + 1 && lane <= TY) { hs[1][lane][0] = T(0); hs[2][lane][0] = T(0); }
+            R.r0 = ld(hin, kp, hrow_off); R.r2 = ld(hin + 2 * nv, kp, hrow_off);
+            R.l1 = ld(hin + nv, kp, hcol_off); R.l2 = ld(hin + 2 * nv, kp, hcol_off);
         }
-        __syncthreads();
-        // phase 2: E over the tile
-        if (role == 0) {
-            const int i = hi_, j = hj;
-            const long long r = (long long)k * A.plane + (long long)j * A.pitch + i;
-            const T hx = hs[0][lj][li], hy = hs[1][lj][li], hz = hs[2][lj][li];
-            const T hz_jm = hs[2][lj - 1][li], hx_jm = hs[0][lj - 1][li];
-            const T hz_im = hs[2][lj][li - 1], hy_im = hs[1][lj][li - 1];
-            const int sx = side_of(i, A.nx, A.low[0], A.high[0]);
-            const int sy = side_of(j, A.ny, A.low[1], A.high[1]);
-            const int sz = side_of(k, A.nz, A.low[2], A.high[2]);
-            T ca[3], cb[3];
-            int grp_c[3];
-            for (int c = 0; c < 3; c++) {
-                if (MODE == 0) {
-                    const int id = (int)A.ids[c][r];
-                    ca[c] = tab[(c * A.table_stride + id) * 2];
-                    cb[c] = tab[(c * A.table_stride + id) * 2 + 1];
-                    grp_c[c] = grp[c * A.table_stride + id];
-                } else {
-                    ca[c] = A.ca[c][r];
-                    cb[c] = A.cb[c][r];
-                    grp_c[c] = A.grp[c] ? (int)A.grp[c][r] : -1;
-                }
-            }
-            T v[3];
-            const T eold[3] = {A.e_in[0][r], A.e_in[1][r], A.e_in[2][r]};
-            {  // Ex: D_y(Hz) fy - D_z(Hy) fz
-                const T dy = hz - hz_jm, dz = hy - hprev[1];
-                T e = ca[0] * eold[0] + cb[0] * (dy * A.fe[1][j] - dz * A.fe[2][k]);
-                if (sy >= 0) e = e + cb[0] * (T(1) * psi_update(A, 0, 0, 1, sy, i, j, k, dy / A.step[1], true));
-                if (sz >= 0) e = e + cb[0] * (T(-1) * psi_update(A, 0, 0, 2, sz, i, j, k, dz / A.step[2], true));
-                v[0] = e;
-            }
-            {  // Ey: D_z(Hx) fz - D_x(Hz) fx
-                const T dz = hx - hprev[0], dx = hz - hz_im;
-                T e = ca[1] * eold[1] + cb[1] * (dz * A.fe[2][k] - dx * A.fe[0][i]);
-                if (sx >= 0) e = e + cb[1] * (T(-1) * psi_update(A, 0, 1, 0, sx, i, j, k, dx / A.step[0], true));
-                if (sz >= 0) e = e + cb[1] * (T(1) * psi_update(A, 0, 1, 2, sz, i, j, k, dz / A.step[2], true));
-                v[1] = e;
-            }
-            {  // Ez: D_x(Hy) fx - D_y(Hx) fy
-                const T dx = hy - hy_im, dy = hx - hx_jm;
-                T e = ca[2] * eold[2] + cb[2] * (dx * A.fe[0][i] - dy * A.fe[1][j]);
-                if (sx >= 0) e = e + cb[2] * (T(1) * psi_update(A, 0, 2, 0, sx, i, j, k, dx / A.step[0], true));
-                if (sy >= 0) e = e + cb[2] * (T(-1) * psi_update(A, 0, 2, 1, sy, i, j, k, dy / A.step[1], true));
-                v[2] = e;
-            }
-            inject<T, 0>(A, t, i, j, k, 7, v, cb);
-            // dispersive media: the rest of E's update and the currents' step
-            // (src/fdtd/media.rs:670-705); E's old value is the saved E^n
-            for (int c = 0; c < 3; c++) {
-                const int g = grp_c[c];
-                if (g < 0) continue;
-                const MediumGroup &G = A.groups[g];
-                const long long s = (long long)c * A.mstride + r;
-                const T before = A.m_before[s];
-                const T chi = T(G.chi);
-                T rest = chi * before;
-                T cur[4], prv[4];
-                for (int q = 0; q < G.poles; q++) {
-                    cur[q] = A.m_cur[s * A.max_poles + q];
-                    prv[q] = A.m_prev[s * A.max_poles + q];
-                    const T alpha = T(G.coef[q][0]), xi = T(G.coef[q][1]);
-                    rest = rest - T(0.5) * ((alpha + T(1)) * cur[q] + xi * prv[q]);
-                }
-                const T next = v[c] + cb[c] * rest;
-                const T slope = (next - before) / (T(2) * A.dt);
-                for (int q = 0; q < G.poles; q++) {
-                    const T alpha = T(G.coef[q][0]), xi = T(G.coef[q][1]), gamma = T(G.coef[q][2]);
-                    const T jn = alpha * cur[q] + xi * prv[q] + gamma * slope;
-                    A.m_prev[s * A.max_poles + q] = cur[q];
-                    A.m_cur[s * A.max_poles + q] = jn;
-                }
-                A.m_before[s] = eold[c];
-                v[c] = next;
-            }
-            A.e_out[0][r] = v[0];
-            A.e_out[1][r] = v[1];
-            A.e_out[2][r] = v[2];
-            hprev[0] = hx;
-            hprev[1] = hy;
+    };
+    auto load_id = [&](Ring<T> &R, int kk) {
+        if (MODE == 0) {
+            const int kp = zplane(kk);
+            R.id = A.ids[own_ok && kp >= 0 ? kp * plane + orow : 0];
         }
-        __syncthreads();
+    };
+
+    const bool has_pro = kz0 > 0 || perz;
+    const int kfirst = has_pro ? kz0 - 1 : kz0;
+
+    // E of the first plane into LDS; the two rings take the planes after
+    {
+        Ring<T> R0;
+        load_e(R0, kfirst);
+        if (own) {
+            es[kfirst & 1][0][wave + 1][lane + 1] = R0.e0;
+            es[kfirst & 1][1][wave + 1][lane + 1] = R0.e1;
+            es[kfirst & 1][2][wave + 1][lane + 1] = R0.e2;
+        }
+        if (tid < NHALO) {
+            es[kfirst & 1][0][hrho][hkap] = R0.x0;
+            es[kfirst & 1][1][hrho][hkap] = R0.x1;
+            es[kfirst & 1][2][hrho][hkap] = R0.x2;
+        }
+    }
+    Ring<T> RA, RB;  // RA: E(kk+1), H(kk), id(kk) for kk - kfirst even; RB for odd
+    RA.e0 = RA.e1 = RA.e2 = RA.x0 = RA.x1 = RA.x2 = T(0);
+    RA.h0 = RA.h1 = RA.h2 = RA.r0 = RA.r2 = RA.l1 = RA.l2 = T(0);
+    RA.id = 0;
+    RB = RA;
+    load_e(RA, kfirst + 1);
+    load_h(RA, kfirst);
+    load_id(RA, kfirst);
+    if (kfirst + 1 < kz1) {
+        load_e(RB, kfirst + 2);
+        load_h(RB, kfirst + 1);
+        load_id(RB, kfirst + 1);
+    }
+    T ec0 = T(0), ec1 = T(0), ec2 = T(0);  // E of this plane, own (E^n)
+    T hp0 = T(0), hp1 = T(0);               // new H of the plane below, own (x, y)
+    const T dt = A.dt;
+    const T stx = A.step[0], sty = A.step[1], stz = A.step[2];
+    const T *coef = A.coef;
+    const int fex = 0, fhx = nx, fey = 2 * nx, fhy = 2 * nx + ny, fez = 2 * (nx + ny),
+              fhz = 2 * (nx + ny) + nz;
+    __syncthreads();  // the parameters and the first plane in LDS
+    if (own) {
+        ec0 = es[kfirst & 1][0][wave + 1][lane + 1];
+        ec1 = es[kfirst & 1][1][wave + 1][lane + 1];
+        ec2 = es[kfirst & 1][2][wave + 1][lane + 1];
+    }
+
+    auto iteration = [&](const int kk, Ring<T> &R) {
+        const bool pro = kk < kz0;
+        const int k = zplane(kk);
+        const int s0 = kk & 1, s1 = (kk + 1) & 1;
+        // 1. E of the next plane into LDS, then the ring's E takes the plane 3 ahead
+        if (own) {
+            es[s1][0][wave + 1][lane + 1] = R.e0;
+            es[s1][1][wave + 1][lane + 1] = R.e1;
+            es[s1][2][wave + 1][lane + 1] = R.e2;
+        }
+        if (tid < NHALO) {
+            es[s1][0][hrho][hkap] = R.x0;
+            es[s1][1][hrho][hkap] = R.x1;
+            es[s1][2][hrho][hkap] = R.x2;
+        }
+        if (kk + 2 < kz1) load_e(R, kk + 3);
+        lds_barrier();
+        // 2. new H over the tile and its halo
+        const T fzh = coef[fhz + k];
+        if (own_ok) {
+            const int ka = lane + 1, ra = wave + 1;
+            const T ex = es[s0][0][ra][ka], ey = es[s0][1][ra][ka], ez = es[s0][2][ra][ka];
+            const T dyx = es[s0][2][ra + 1][ka] - ez, dzx = es[s1][1][ra][ka] - ey;  // Hx
+            const T dzy = es[s1][0][ra][ka] - ex, dxy = es[s0][2][ra][ka + 1] - ez;  // Hy
+            const T dxz = es[s0][1][ra][ka + 1] - ey, dyz = es[s0][0][ra + 1][ka] - ex;  // Hz
+            const T fxh = coef[fhx + oi], fyh = coef[fhy + oj];
+            T v0 = R.h0 - dt * (dyx * fyh - dzx * fzh);
+            T v1 = R.h1 - dt * (dzy * fzh - dxy * fxh);
+            T v2 = R.h2 - dt * (dxz * fxh - dyz * fyh);
+            if (slabs) {
+                const SlabAt P = slab_at(A, K, oi, oj, k);
+                const bool wr = !pro;
+                if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, dyx / sty, wr));
+                if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, dzx / stz, wr));
+                if (P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, dxy / stx, wr));
+                if (P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, dzy / stz, wr));
+                if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, wr));
+                if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, wr));
+            }
+            if (injects) inject<T, 1>(A, K, t, oi, oj, k, 7, v0, v1, v2, T(0), T(0), T(0));
+            if (!pro) {
+                const int r = k * plane + orow;
+                A.hout[r] = v0;
+                A.hout[nv + r] = v1;
+                A.hout[2 * nv + r] = v2;
+            }
+            hs[0][wave + 1][lane + 1] = v0;
+            hs[1][wave + 1][lane + 1] = v1;
+            hs[2][wave + 1][lane + 1] = v2;
+        }
+        if (hw && !pro) {
+            // the row below the tile: Hz and Hx at (i0 + lane, j0 - 1)
+            if (hrow_ok) {
+                const int ka = lane + 1, ra = 0;
+                const T ex = es[s0][0][ra][ka], ey = es[s0][1][ra][ka], ez = es[s0][2][ra][ka];
+                const T dyx = es[s0][2][ra + 1][ka] - ez, dzx = es[s1][1][ra][ka] - ey;
+                const T dxz = es[s0][1][ra][ka + 1] - ey, dyz = es[s0][0][ra + 1][ka] - ex;
+                const T fxh = coef[fhx + oi], fyh = coef[fhy + hrj];
+                T v0 = R.r0 - dt * (dyx * fyh - dzx * fzh);
+                T v2 = R.r2 - dt * (dxz * fxh - dyz * fyh);
+                if (slabs) {
+                    const SlabAt P = slab_at(A, K, oi, hrj, k);
+                    if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, dyx / sty, false));
+                    if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, dzx / stz, false));
+                    if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, false));
+                    if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, false));
+                }
+                T v1 = T(0);
+                if (injects) inject<T, 1>(A, K, t, oi, hrj, k, 5, v0, v1, v2, T(0), T(0), T(0));
+                hs[0][0][lane + 1] = v0;
+                hs[2][0][lane + 1] = v2;
+            } else {
+                hs[0][0][lane + 1] = T(0);
+                hs[2][0][lane + 1] = T(0);
+            }
+            // the column left of the tile: Hz and Hy at (i0 - 1, j0 + lane)
+            if (hcol_ok) {
+                const int ka = 0, ra = lane + 1;
+                const T ex = es[s0][0][ra][ka], ey = es[s0][1][ra][ka], ez = es[s0][2][ra][ka];
+                const T dzy = es[s1][0][ra][ka] - ex, dxy = es[s0][2][ra][ka + 1] - ez;
+                const T dxz = es[s0][1][ra][ka + 1] - ey, dyz = es[s0][0][ra + 1][ka] - ex;
+                const T fxh = coef[fhx + hci], fyh = coef[fhy + hcj];
+                T v1 = R.l1 - dt * (dzy * fzh - dxy * fxh);
+                T v2 = R.l2 - dt * (dxz * fxh - dyz * fyh);
+                if (slabs) {
+                    const SlabAt P = slab_at(A, K, hci, hcj, k);
+                    if (P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, dxy / stx, false));
+                    if (P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, dzy / stz, false));
+                    if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, false));
+                    if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, false));
+                }
+                T v0 = T(0);
+                if (injects) inject<T, 1>(A, K, t, hci, hcj, k, 6, v0, v1, v2, T(0), T(0), T(0));
+                hs[1][lane + 1][0] = v1;
+                hs[2][lane + 1][0] = v2;
+            } else if (lane < TY) {
+                hs[1][lane + 1][0] = T(0);
+                hs[2][lane + 1][0] = T(0);
+            }
+        }
+        const uint8_t idc = R.id;
+        if (kk + 2 < kz1) load_h(R, kk + 2);
+        lds_barrier();
+        // 3. new E over the tile
+        if (own_ok && !pro) {
+            const int ka = lane + 1, ra = wave + 1;
+            const T hx = hs[0][ra][ka], hy = hs[1][ra][ka], hz = hs[2][ra][ka];
+            const T dyx = hz - hs[2][ra - 1][ka], dzx = hy - hp1;                 // Ex
+            const T dzy = hx - hp0, dxy = hz - hs[2][ra][ka - 1];                 // Ey
+            const T dxz = hy - hs[1][ra][ka - 1], dyz = hx - hs[0][ra - 1][ka];   // Ez
+            T ca0, ca1, ca2, cb0, cb1, cb2;
+            int g0, g1, g2;
+            const int r = k * plane + orow;
+            if (MODE == 0) {
+                const T *e = tab + 6 * idc;
+                ca0 = e[0]; cb0 = e[1]; ca1 = e[2]; cb1 = e[3]; ca2 = e[4]; cb2 = e[5];
+                g0 = grp[3 * idc]; g1 = grp[3 * idc + 1]; g2 = grp[3 * idc + 2];
+            } else {
+                ca0 = A.cacb[r]; ca1 = A.cacb[nv + r]; ca2 = A.cacb[2 * nv + r];
+                cb0 = A.cacb[3 * nv + r]; cb1 = A.cacb[4 * nv + r]; cb2 = A.cacb[5 * nv + r];
+                g0 = A.grp ? A.grp[r] : -1;
+                g1 = A.grp ? A.grp[nv + r] : -1;
+                g2 = A.grp ? A.grp[2 * nv + r] : -1;
+            }
+            const T fxe = coef[fex + oi], fye = coef[fey + oj], fze = coef[fez + k];
+            T v0 = ca0 * ec0 + cb0 * (dyx * fye - dzx * fze);
+            T v1 = ca1 * ec1 + cb1 * (dzy * fze - dxy * fxe);
+            T v2 = ca2 * ec2 + cb2 * (dxz * fxe - dyz * fye);
+            if (slabs) {
+                const SlabAt P = slab_at(A, K, oi, oj, k);
+                if (P.side[1] >= 0) v0 = v0 + cb0 * (T(1) * psi_at<0, 0, 1>(A, K, P, dyx / sty, true));
+                if (P.side[2] >= 0) v0 = v0 + cb0 * (T(-1) * psi_at<0, 0, 2>(A, K, P, dzx / stz, true));
+                if (P.side[0] >= 0) v1 = v1 + cb1 * (T(-1) * psi_at<0, 1, 0>(A, K, P, dxy / stx, true));
+                if (P.side[2] >= 0) v1 = v1 + cb1 * (T(1) * psi_at<0, 1, 2>(A, K, P, dzy / stz, true));
+                if (P.side[0] >= 0) v2 = v2 + cb2 * (T(1) * psi_at<0, 2, 0>(A, K, P, dxz / stx, true));
+                if (P.side[1] >= 0) v2 = v2 + cb2 * (T(-1) * psi_at<0, 2, 1>(A, K, P, dyz / sty, true));
+            }
+            if (injects) inject<T, 0>(A, K, t, oi, oj, k, 7, v0, v1, v2, cb0, cb1, cb2);
+            if (g0 >= 0) v0 = media_update(A, K, g0, 0, r, v0, cb0, ec0);
+            if (g1 >= 0) v1 = media_update(A, K, g1, 1, r, v1, cb1, ec1);
+            if (g2 >= 0) v2 = media_update(A, K, g2, 2, r, v2, cb2, ec2);
+            A.eout[r] = v0;
+            A.eout[nv + r] = v1;
+            A.eout[2 * nv + r] = v2;
+        }
+        if (kk + 2 < kz1) load_id(R, kk + 2);
+        if (own) {
+            hp0 = hs[0][wave + 1][lane + 1];
+            hp1 = hs[1][wave + 1][lane + 1];
+            ec0 = es[s1][0][wave + 1][lane + 1];
+            ec1 = es[s1][1][wave + 1][lane + 1];
+            ec2 = es[s1][2][wave + 1][lane + 1];
+        }
+    };
+
+    for (int kk = kfirst; kk < kz1; kk += 2) {
+        iteration(kk, RA);
+        if (kk + 1 < kz1) iteration(kk + 1, RB);
     }
 }
 

@@ -131,15 +131,15 @@ struct pxfdtd_resident {
     long long plane = 0, nval = 0, ncells = 0;
     int periodic[3] = {0, 0, 0}, low[3] = {0, 0, 0}, high[3] = {0, 0, 0};
     double dt = 0, step[3] = {0, 0, 0};
-    Buf field[2][6];
+    Buf fields[2];  // per copy: Ex, Ey, Ez, Hx, Hy, Hz, nval values each
     int mode = 0;
-    int table_stride = 256;
-    Buf ids[3], table, group_of, ca[3], cb[3], grp[3];
+    Buf ids, table, cacb, grpb, cold;
     bool has_grp = false;
-    Buf fe[3], fh[3];
-    Buf pb[3][2][2], pa[3][2][2];
+    Buf coef;  // fe, fh per axis, then the CPML's b and a
+    int fe_off[3] = {0, 0, 0}, fh_off[3] = {0, 0, 0}, cpml_off[3][2][2];
     Buf psi[2];
-    long long psi_off[2][3][3][2];
+    int psi_off[2][3][3][2];
+    template <typename T> T *field(int copy, int q) const { return fields[copy].as<T>() + (size_t)q * nval; }
     long long psi_len = 0;
     std::vector<SlabDesc> slabs;
     // media
@@ -252,11 +252,10 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
     const long long N = (long long)p->nx * p->ny * p->nz;
     const long long nval = r->nval;
     hipStream_t s = r->stream;
-    for (int c = 0; c < 2; c++)
-        for (int q = 0; q < 6; q++) {
-            TRY(r->field[c][q].alloc(nval * sizeof(T)));
-            HIPCHK(hipMemsetAsync(r->field[c][q].p, 0, nval * sizeof(T), s));
-        }
+    for (int c = 0; c < 2; c++) {
+        TRY(r->fields[c].alloc(6 * nval * sizeof(T)));
+        HIPCHK(hipMemsetAsync(r->fields[c].p, 0, 6 * nval * sizeof(T), s));
+    }
     // media membership per value: group + 1, 0 outside
     std::vector<int8_t> member;
     if (p->nmedia) {
@@ -272,83 +271,114 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
             }
         }
     }
-    // E's coefficients: uint8 ids if every component has at most 256 distinct
-    // (ca, cb, medium) triples, else whole arrays
-    std::vector<std::vector<uint8_t>> idv(3);
-    std::vector<double> tab(6 * 256, 0.0);
-    std::vector<int8_t> tgrp(3 * 256, -1);
+    // E's coefficients: one uint8 material id per cell if at most 256 distinct
+    // (ca, cb, medium) triples over the three components occur, else whole arrays
+    struct Key {
+        uint64_t v[6];
+        int8_t g[3];
+        bool operator<(const Key &o) const {
+            for (int q = 0; q < 6; q++)
+                if (v[q] != o.v[q]) return v[q] < o.v[q];
+            for (int q = 0; q < 3; q++)
+                if (g[q] != o.g[q]) return g[q] < o.g[q];
+            return false;
+        }
+        bool operator==(const Key &o) const { return !(*this < o) && !(o < *this); }
+    };
+    std::vector<uint8_t> idv(nval, 0);
+    std::vector<double> tab(256 * 6, 0.0);
+    std::vector<int8_t> tgrp(256 * 3, -1);
     bool table_ok = true;
-    for (int c = 0; c < 3 && table_ok; c++) {
-        std::map<std::tuple<uint64_t, uint64_t, int>, int> seen;
-        idv[c].resize(nval, 0);
+    {
+        std::map<Key, int> seen;
+        Key last;
+        int last_id = -1;
         for (long long q = 0; q < N && table_ok; q++) {
-            const double a = p->ca[c * N + q], b = p->cb[c * N + q];
-            const int g = member.empty() ? 0 : member[c * N + q];
-            uint64_t ab, bb;
-            memcpy(&ab, &a, 8);
-            memcpy(&bb, &b, 8);
-            auto key = std::make_tuple(ab, bb, g);
-            auto it = seen.find(key);
+            Key key;
+            for (int c = 0; c < 3; c++) {
+                memcpy(&key.v[2 * c], &p->ca[c * N + q], 8);
+                memcpy(&key.v[2 * c + 1], &p->cb[c * N + q], 8);
+                key.g[c] = member.empty() ? (int8_t)-1 : (int8_t)(member[c * N + q] - 1);
+            }
             int id;
-            if (it == seen.end()) {
-                id = (int)seen.size();
-                if (id >= 256) { table_ok = false; break; }
-                seen.emplace(key, id);
-                tab[(c * 256 + id) * 2] = a;
-                tab[(c * 256 + id) * 2 + 1] = b;
-                tgrp[c * 256 + id] = (int8_t)(g - 1);
+            if (last_id >= 0 && key == last) {
+                id = last_id;
             } else {
-                id = it->second;
+                auto it = seen.find(key);
+                if (it == seen.end()) {
+                    id = (int)seen.size();
+                    if (id >= 256) { table_ok = false; break; }
+                    seen.emplace(key, id);
+                    for (int c = 0; c < 3; c++) {
+                        tab[id * 6 + 2 * c] = p->ca[c * N + q];
+                        tab[id * 6 + 2 * c + 1] = p->cb[c * N + q];
+                        tgrp[id * 3 + c] = key.g[c];
+                    }
+                } else {
+                    id = it->second;
+                }
+                last = key;
+                last_id = id;
             }
             const long long i = q % p->nx, rest = q / p->nx;
-            idv[c][(rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i] = (uint8_t)id;
+            idv[(rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i] = (uint8_t)id;
         }
     }
     r->mode = table_ok ? 0 : 1;
     if (r->mode == 0) {
-        for (int c = 0; c < 3; c++) {
-            TRY(r->ids[c].alloc(nval));
-            HIPCHK(hipMemcpyAsync(r->ids[c].p, idv[c].data(), nval, hipMemcpyHostToDevice, s));
-            HIPCHK(hipStreamSynchronize(s));
+        TRY(r->ids.alloc(nval));
+        HIPCHK(hipMemcpyAsync(r->ids.p, idv.data(), nval, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        // the table, then the media groups as int8
+        std::vector<unsigned char> tb(256 * 6 * sizeof(T) + tgrp.size());
+        for (size_t q = 0; q < tab.size(); q++) {
+            const T v = (T)tab[q];
+            memcpy(tb.data() + q * sizeof(T), &v, sizeof(T));
         }
-        TRY(upload_table<T>(r->table, tab.data(), tab.size(), s));
-        TRY(r->group_of.alloc(tgrp.size()));
-        HIPCHK(hipMemcpyAsync(r->group_of.p, tgrp.data(), tgrp.size(), hipMemcpyHostToDevice, s));
+        memcpy(tb.data() + 256 * 6 * sizeof(T), tgrp.data(), tgrp.size());
+        TRY(r->table.alloc(tb.size()));
+        HIPCHK(hipMemcpyAsync(r->table.p, tb.data(), tb.size(), hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
     } else {
-        std::vector<T> a(nval, T(0)), b(nval, T(0));
-        std::vector<int8_t> g(nval, -1);
-        for (int c = 0; c < 3; c++) {
+        std::vector<T> ab(6 * nval, T(0));
+        std::vector<int8_t> g(3 * nval, -1);
+        for (int c = 0; c < 3; c++)
             for (long long q = 0; q < N; q++) {
                 const long long i = q % p->nx, rest = q / p->nx;
                 const long long d = (rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i;
-                a[d] = (T)p->ca[c * N + q];
-                b[d] = (T)p->cb[c * N + q];
-                if (!member.empty()) g[d] = (int8_t)(member[c * N + q] - 1);
+                ab[c * nval + d] = (T)p->ca[c * N + q];
+                ab[(3 + c) * nval + d] = (T)p->cb[c * N + q];
+                if (!member.empty()) g[c * nval + d] = (int8_t)(member[c * N + q] - 1);
             }
-            TRY(r->ca[c].alloc(nval * sizeof(T)));
-            TRY(r->cb[c].alloc(nval * sizeof(T)));
-            HIPCHK(hipMemcpyAsync(r->ca[c].p, a.data(), nval * sizeof(T), hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(r->cb[c].p, b.data(), nval * sizeof(T), hipMemcpyHostToDevice, s));
-            if (!member.empty()) {
-                TRY(r->grp[c].alloc(nval));
-                HIPCHK(hipMemcpyAsync(r->grp[c].p, g.data(), nval, hipMemcpyHostToDevice, s));
-            }
-            HIPCHK(hipStreamSynchronize(s));
+        TRY(r->cacb.alloc(ab.size() * sizeof(T)));
+        HIPCHK(hipMemcpyAsync(r->cacb.p, ab.data(), ab.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        if (!member.empty()) {
+            TRY(r->grpb.alloc(g.size()));
+            HIPCHK(hipMemcpyAsync(r->grpb.p, g.data(), g.size(), hipMemcpyHostToDevice, s));
         }
+        HIPCHK(hipStreamSynchronize(s));
         r->has_grp = !member.empty();
     }
     const int n[3] = {p->nx, p->ny, p->nz};
-    for (int a = 0; a < 3; a++) {
-        TRY(upload_table<T>(r->fe[a], p->factors_e[a], n[a], s));
-        TRY(upload_table<T>(r->fh[a], p->factors_h[a], n[a], s));
-        const int cnt[2] = {p->pml_low[a], p->pml_high[a]};
-        for (int sd = 0; sd < 2; sd++)
-            for (int f = 0; f < 2; f++)
-                if (cnt[sd]) {
-                    TRY(upload_table<T>(r->pb[a][sd][f], p->cpml_b[a][sd][f], cnt[sd], s));
-                    TRY(upload_table<T>(r->pa[a][sd][f], p->cpml_a[a][sd][f], cnt[sd], s));
+    {
+        std::vector<double> co;
+        for (int a = 0; a < 3; a++) {
+            r->fe_off[a] = (int)co.size();
+            co.insert(co.end(), p->factors_e[a], p->factors_e[a] + n[a]);
+            r->fh_off[a] = (int)co.size();
+            co.insert(co.end(), p->factors_h[a], p->factors_h[a] + n[a]);
+        }
+        for (int a = 0; a < 3; a++) {
+            const int cnt[2] = {p->pml_low[a], p->pml_high[a]};
+            for (int sd = 0; sd < 2; sd++)
+                for (int f = 0; f < 2; f++) {
+                    r->cpml_off[a][sd][f] = (int)co.size();
+                    if (!cnt[sd]) continue;
+                    co.insert(co.end(), p->cpml_b[a][sd][f], p->cpml_b[a][sd][f] + cnt[sd]);
+                    co.insert(co.end(), p->cpml_a[a][sd][f], p->cpml_a[a][sd][f] + cnt[sd]);
                 }
+        }
+        TRY(upload_table<T>(r->coef, co.data(), co.size(), s));
     }
     // psi: the reference's slab order (src/fdtd.rs:838-866)
     for (int F = 0; F < 2; F++)
@@ -366,12 +396,13 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
                     const int c = (a + cc) % 3;
                     const long long len = slab_volume(p, a, cnt[sd]);
                     r->slabs.push_back({F, c, a, sd, from[sd], cnt[sd], off, len});
-                    r->psi_off[F][c][a][sd] = off;
+                    r->psi_off[F][c][a][sd] = (int)off;
                     off += len;
                 }
         }
     }
     r->psi_len = off;
+    if (off >= (1LL << 31)) return px_fail(PXFDTD_ERR_UNSUPPORTED, "the CPML's psi is beyond 32-bit indexing");
     for (int q = 0; q < 2; q++) {
         TRY(r->psi[q].alloc(std::max<long long>(off, 1) * sizeof(T)));
         HIPCHK(hipMemsetAsync(r->psi[q].p, 0, std::max<long long>(off, 1) * sizeof(T), s));
@@ -423,8 +454,10 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
         r->step[a] = p->step[a];
     }
     r->dt = p->dt;
-    // tiling defaults: 4 rows per tile, chunks that give the chip many workgroups
-    r->ty = p->ny >= 4 ? 4 : (p->ny >= 2 ? 2 : 1);
+    if (r->nval * 2 >= (1LL << 31))
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "a grid of %lld values per component is beyond 32-bit indexing", r->nval);
+    // tiling defaults: 8 rows per tile, chunks that give the chip many workgroups
+    r->ty = p->ny >= 4 ? 4 : 2;
     {
         const long long tiles = (long long)((p->nx + 63) / 64) * ((p->ny + r->ty - 1) / r->ty);
         int kz = p->nz;
@@ -483,7 +516,7 @@ static int upload_fields(pxfdtd_resident *r, const double *e, const double *h, c
             const long long i = v % r->nx, rest = v / r->nx;
             buf[(rest / r->ny) * r->plane + (rest % r->ny) * r->pitch + i] = (T)src[v];
         }
-        HIPCHK(hipMemcpyAsync(r->field[r->p][q].p, buf.data(), r->nval * sizeof(T),
+        HIPCHK(hipMemcpyAsync(r->field<T>(r->p, q), buf.data(), r->nval * sizeof(T),
                               hipMemcpyHostToDevice, r->stream));
         HIPCHK(hipStreamSynchronize(r->stream));
     }
@@ -514,7 +547,7 @@ static int download_fields(pxfdtd_resident *r, double *e, double *h, double *psi
         double *dst = (q < 3 ? e : h);
         if (!dst) continue;
         dst += (q % 3) * N;
-        HIPCHK(hipMemcpyAsync(buf.data(), r->field[r->p][q].p, r->nval * sizeof(T),
+        HIPCHK(hipMemcpyAsync(buf.data(), r->field<T>(r->p, q), r->nval * sizeof(T),
                               hipMemcpyDeviceToHost, r->stream));
         HIPCHK(hipStreamSynchronize(r->stream));
         for (long long v = 0; v < N; v++) {
@@ -615,8 +648,8 @@ extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_i
 
 extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes) {
     if (rows) {
-        if (rows != 1 && rows != 2 && rows != 4 && rows != 8)
-            return px_fail(PXFDTD_ERR_INVALID, "rows per tile must be 1, 2, 4 or 8, got %d", rows);
+        if (rows != 2 && rows != 4 && rows != 8)
+            return px_fail(PXFDTD_ERR_INVALID, "rows per tile must be 2, 4 or 8, got %d", rows);
         r->ty = rows;
     }
     if (planes) {
@@ -644,7 +677,7 @@ static int build_taps(pxfdtd_resident *r, const std::vector<int64_t> &index,
     HIPCHK(hipMemcpyAsync(r->tap_which.p, which.data(), which.size(), hipMemcpyHostToDevice, r->stream));
     for (int c = 0; c < 2; c++) {
         void *ptrs[6];
-        for (int q = 0; q < 6; q++) ptrs[q] = r->field[c][q].p;
+        for (int q = 0; q < 6; q++) ptrs[q] = (char *)r->fields[c].p + (size_t)q * r->nval * r->esz;
         TRY(r->tap_fields[c].alloc(sizeof ptrs));
         HIPCHK(hipMemcpyAsync(r->tap_fields[c].p, ptrs, sizeof ptrs, hipMemcpyHostToDevice, r->stream));
     }
@@ -791,58 +824,69 @@ template <typename T>
 static StepArgs<T> args_of(const pxfdtd_resident *r) {
     StepArgs<T> A;
     memset(&A, 0, sizeof A);
-    A.nx = r->nx; A.ny = r->ny; A.nz = r->nz; A.pitch = r->pitch; A.plane = r->plane;
+    A.nx = r->nx; A.ny = r->ny; A.nz = r->nz; A.pitch = r->pitch; A.plane = (int)r->plane;
+    A.nval = (int)r->nval;
+    bool slabs = false;
     for (int a = 0; a < 3; a++) {
-        A.periodic[a] = r->periodic[a];
-        A.low[a] = r->low[a];
-        A.high[a] = r->high[a];
+        A.flags |= r->periodic[a] << a;
         A.step[a] = (T)r->step[a];
-        A.fe[a] = r->fe[a].as<T>();
-        A.fh[a] = r->fh[a].as<T>();
-        for (int s = 0; s < 2; s++)
-            for (int f = 0; f < 2; f++) {
-                A.pb[a][s][f] = r->pb[a][s][f].as<T>();
-                A.pa[a][s][f] = r->pa[a][s][f].as<T>();
-            }
+        slabs = slabs || r->low[a] || r->high[a];
     }
+    if (slabs) A.flags |= 8;
+    if (r->have_inj) A.flags |= 16;
+    A.coef = r->coef.as<T>();
     A.tiles_x = (r->nx + TX - 1) / TX;
     A.tiles_y = (r->ny + r->ty - 1) / r->ty;
     A.kz = r->kz;
-    A.chunks = (r->nz + r->kz - 1) / r->kz;
+    A.blocks = A.tiles_x * A.tiles_y * ((r->nz + r->kz - 1) / r->kz);
+    A.per_xcd = (A.blocks + 7) / 8;
     A.dt = (T)r->dt;
-    A.table_stride = r->table_stride;
-    for (int c = 0; c < 3; c++) {
-        A.ids[c] = r->ids[c].as<uint8_t>();
-        A.ca[c] = r->ca[c].as<T>();
-        A.cb[c] = r->cb[c].as<T>();
-        A.grp[c] = r->has_grp ? r->grp[c].as<int8_t>() : nullptr;
-    }
+    A.ids = r->ids.as<uint8_t>();
     A.table = r->table.as<T>();
-    A.group_of = r->group_of.as<int8_t>();
-    memcpy(A.psi_off, r->psi_off, sizeof A.psi_off);
-    A.inj_off = r->have_inj ? r->inj_off.as<int32_t>() : nullptr;
-    A.inj = r->inj.as<Injection>();
-    A.groups = r->groups.as<MediumGroup>();
-    A.ngroups = r->ngroups;
-    A.max_poles = r->max_poles;
-    A.m_before = r->m_before.as<T>();
-    A.m_cur = r->m_cur.as<T>();
-    A.m_prev = r->m_prev.as<T>();
-    A.mstride = r->nval;
+    A.cacb = r->cacb.as<T>();
+    A.grp = r->has_grp ? r->grpb.as<int8_t>() : nullptr;
     return A;
+}
+
+// the parameters slab cells, injections and media read, one set per copy parity
+template <typename T>
+static int upload_cold(pxfdtd_resident *r) {
+    StepCold<T> K[2];
+    memset(K, 0, sizeof K);
+    for (int p = 0; p < 2; p++) {
+        StepCold<T> &k = K[p];
+        for (int a = 0; a < 3; a++) {
+            k.low[a] = r->low[a];
+            k.high[a] = r->high[a];
+        }
+        memcpy(k.cpml_off, r->cpml_off, sizeof k.cpml_off);
+        memcpy(k.psi_off, r->psi_off, sizeof k.psi_off);
+        k.psi_in = r->psi[p].as<T>();
+        k.psi_out = r->psi[1 - p].as<T>();
+        k.inj_off = r->have_inj ? r->inj_off.as<int32_t>() : nullptr;
+        k.inj = r->inj.as<Injection>();
+        k.waves = r->waves.as<double>();
+        k.emitters = r->emitters;
+        k.groups = r->groups.as<MediumGroup>();
+        k.max_poles = r->max_poles;
+        k.m_before = r->m_before.as<T>();
+        k.m_cur = r->m_cur.as<T>();
+        k.m_prev = r->m_prev.as<T>();
+    }
+    TRY(r->cold.reserve(sizeof K));
+    HIPCHK(hipMemcpy(r->cold.p, K, sizeof K, hipMemcpyHostToDevice));
+    return PXFDTD_OK;
 }
 
 template <typename T, int MODE, int TY>
 static int launch_ty(const StepArgs<T> &A, hipStream_t s, size_t lds) {
-    const int blocks = A.tiles_x * A.tiles_y * A.chunks;
-    hipLaunchKernelGGL((fused_step<T, MODE, TY>), dim3(blocks), dim3(TX, TY + 2), lds, s, A);
+    hipLaunchKernelGGL((fused_step<T, MODE, TY>), dim3(8 * A.per_xcd), dim3(TX, TY + 1), lds, s, A);
     return PXFDTD_OK;
 }
 
 template <typename T, int MODE>
 static int launch_mode(const StepArgs<T> &A, int ty, hipStream_t s, size_t lds) {
     switch (ty) {
-    case 1: return launch_ty<T, MODE, 1>(A, s, lds);
     case 2: return launch_ty<T, MODE, 2>(A, s, lds);
     case 4: return launch_ty<T, MODE, 4>(A, s, lds);
     default: return launch_ty<T, MODE, 8>(A, s, lds);
@@ -877,7 +921,8 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
             std::swap(grown.bytes, r->probe_rec.bytes);
         }
     }
-    const size_t lds = r->mode == 0 ? (size_t)r->table_stride * (6 * sizeof(T) + 3) : 0;
+    const size_t lds = r->mode == 0 ? (size_t)256 * (6 * sizeof(T) + 3) : 0;
+    TRY(upload_cold<T>(r));
     int in_block = 0;
     long long block_first = 0;
     if (r->profile)
@@ -890,15 +935,12 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
     for (uint32_t n = 0; n < count; n++) {
         StepArgs<T> A = args_of<T>(r);
         const int p = r->p, q = 1 - p;
-        for (int c = 0; c < 3; c++) {
-            A.e_in[c] = r->field[p][c].as<T>();
-            A.h_in[c] = r->field[p][3 + c].as<T>();
-            A.e_out[c] = r->field[q][c].as<T>();
-            A.h_out[c] = r->field[q][3 + c].as<T>();
-        }
-        A.psi_in = r->psi[p].as<T>();
-        A.psi_out = r->psi[q].as<T>();
-        A.waves = r->waves.as<double>() + (size_t)n * r->emitters * 2;
+        A.ein = r->field<T>(p, 0);
+        A.hin = r->field<T>(p, 3);
+        A.eout = r->field<T>(q, 0);
+        A.hout = r->field<T>(q, 3);
+        A.cold = r->cold.as<StepCold<T>>() + p;
+        A.wave_step = (int)n;
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
         if (r->mode == 0)
             TRY((launch_mode<T, 0>(A, r->ty, s, lds)));

@@ -30,9 +30,13 @@ def pulse():
     return dict(kind=0, frequency=1.0, width=width, delay=6.0 * width)
 
 
-def busy_spec(two_d=False):
+def busy_spec(two_d=False, periodic=False):
     h = 0.05
-    if two_d:
+    if periodic:
+        # periodic along y and z at sizes no tile divides, a CPML and a wall along x
+        n = (37, 21, 19)
+        b = dict(periodic=(0, 1, 1), low=(4, 0, 0), high=(0, 0, 0), kappa=1.5, alpha=0.1)
+    elif two_d:
         n = (70, 45, 1)
         b = dict(periodic=(1, 0, 1), low=(0, 6, 0), high=(0, 6, 0), kappa=2.0, alpha=0.3)
     else:

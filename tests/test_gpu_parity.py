@@ -59,13 +59,13 @@ def same_bits(a, b):
                           np.asarray(b, dtype=np.float64).view(np.uint64))
 
 
-@pytest.mark.parametrize("two_d", [False, True], ids=["3d", "2d"])
+@pytest.mark.parametrize("shape", ["3d", "2d", "periodic"])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_device_is_the_oracle_bit_for_bit(gpu, two_d, prec):
+def test_device_is_the_oracle_bit_for_bit(gpu, shape, prec):
     import photonoxide_amd as P
     precision = P.Precision.F64 if prec == "f64" else P.Precision.F32
-    spec = busy_spec(two_d)
-    print(f"case busy two_d={two_d} {prec}", flush=True)
+    spec = busy_spec(shape == "2d", shape == "periodic")
+    print(f"case busy {shape} {prec}", flush=True)
     o, p = build_pair(spec, gpu, precision)
     o.run(80)
     p.run(80)
@@ -107,7 +107,7 @@ def test_f32_within_north_star_tolerance_of_f64_oracle(gpu):
     assert rel < 1e-5, rel
 
 
-@pytest.mark.parametrize("rows,planes", [(1, 1), (2, 3), (8, 19), (4, 5)])
+@pytest.mark.parametrize("rows,planes", [(2, 1), (2, 3), (8, 19), (4, 5)])
 def test_tiling_changes_no_bit(gpu, rows, planes):
     import photonoxide_amd as P
     spec = busy_spec(False)
