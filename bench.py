@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--plain", action="store_true",
                     help="the kernel alone: no source, no monitors (for characterization)")
     ap.add_argument("--pml", type=int, default=12)
+    ap.add_argument("--kernel", type=int, default=-1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: each box has one MI355X, and "
@@ -122,6 +123,8 @@ def main():
     s, g = mmi_problem(nx, ny, nz, 0.02, pml, plain=args.plain)
     gpu = P.Gpu(0)
     s.set_device(gpu, P.Precision.F32)
+    if args.kernel >= 0:
+        s.set_kernel(args.kernel)
     if args.rows or args.planes:
         s.set_tiling(args.rows, args.planes)
     res = s.resident()
@@ -156,12 +159,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else ""),
-                   "cells": cells, "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
+                   "cells": cells, "kernel": ["lds_tiles", "register_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
                    "monitors": "2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums",
                    "psi_values": psi},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": "fused_step<float,0,4>", "kernel_ms": round(kernel_ms, 5),
+                     "kernel": ["fused_step", "march_step"][s.kernel()], "kernel_ms": round(kernel_ms, 5),
                      "bytes_per_launch": bytes_step,
                      "gcell_per_s_kernel": round(cells / (kernel_ms * 1e-3) / 1e9, 3)},
     }

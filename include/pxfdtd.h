@@ -151,7 +151,12 @@ int pxfdtd_resident_read_probes(pxfdtd_resident *r, double *out, int64_t capacit
 /* the transform sums, FP64, series after series, each [f][k][j][i] complex */
 int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_t capacity);
 int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r);
-/* the tiling: rows per tile (1, 2, 4, 8) and planes per chunk; 0 keeps the current value */
+/* the step kernel: 0 tiles of 64 x rows exchanging H through LDS; 1 (the default) strips of
+ * 62 x rows per wavefront marching along z with no barrier */
+int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind);
+int pxfdtd_resident_get_kernel(const pxfdtd_resident *r);
+/* the tiling: rows per tile (kernel 0: 2, 4, 8; kernel 1: 1, 2, 4) and planes per chunk; 0
+ * keeps the current value */
 int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes);
 int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes);
 /* the coefficient storage chosen: 0 uint8 ids into a table, 1 full ca/cb arrays */

@@ -545,6 +545,218 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// The register-march kernel: no LDS exchange and no barrier in the plane loop.
+//
+// Each wavefront owns a strip of RW = 62 values in x by RY rows in y and marches along z.
+// Lane L holds x = i0 - 1 + L: lane 0 is the column left of the strip (its H is the i - 1
+// neighbour of lane 1's E update), lane 63 the column right of it (its E is the i + 1
+// neighbour of lane 62's H update); lanes 1..62 are the strip's own values. Each lane holds
+// RY + 2 rows of E (the row below the strip, the strip, the row above), computes H on RY + 1
+// rows (the row below and the strip) and E on the strip. Neighbours along x come from the
+// adjacent lane (a cross-lane shuffle), along y from the thread's own registers, along z
+// from the plane before (kept) and after (loaded). The waves of a workgroup are independent:
+// the chip stays busy from many waves in flight rather than from a synchronized pipeline.
+constexpr int RW = 62;
+
+template <typename T>
+__device__ __forceinline__ T from_next_lane(T v) { return __shfl_down(v, 1, 64); }
+template <typename T>
+__device__ __forceinline__ T from_prev_lane(T v) { return __shfl_up(v, 1, 64); }
+
+template <typename T, int MODE, int RY>
+__global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
+    constexpr int NR = RY + 2;  // rows of E held: j0 - 1 .. j0 + RY
+    __shared__ StepCold<T> K;
+    extern __shared__ unsigned char smem_raw[];
+    T *tab = reinterpret_cast<T *>(smem_raw);                 // [256][3][2]
+    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);  // [256][3]
+    const int tid = threadIdx.x;
+    if (MODE == 0) {
+        for (int q = tid; q < 256 * 6; q += 256) tab[q] = A.table[q];
+        const int8_t *g = reinterpret_cast<const int8_t *>(A.table + 256 * 6);
+        for (int q = tid; q < 256 * 3; q += 256) grp[q] = g[q];
+    }
+    {
+        const int *src = reinterpret_cast<const int *>(A.cold);
+        int *dst = reinterpret_cast<int *>(&K);
+        for (int q = tid; q < (int)(sizeof(StepCold<T>) / 4); q += 256) dst[q] = src[q];
+    }
+    __syncthreads();  // the only barrier: the parameters in LDS
+
+    const int lane = tid & 63;
+    // workgroups are dealt round-robin to the 8 XCDs: each XCD takes a contiguous run of
+    // strips, so neighbouring strips (which read each other's edges) share an L2
+    const int b = (int)(blockIdx.x % 8) * A.per_xcd + (int)(blockIdx.x / 8);
+    const int w = b * 4 + (tid >> 6);
+    if (w >= A.blocks) return;
+    const int tiles = A.tiles_x * A.tiles_y;
+    const int bz = w / tiles, t = w % tiles;
+    const int by = t / A.tiles_x, bx = t % A.tiles_x;
+    const int i0 = bx * RW, j0 = by * RY;
+    const int kz0 = bz * A.kz;
+    const int kz1 = min(A.nz, kz0 + A.kz);
+    const int nx = A.nx, ny = A.ny, nz = A.nz, pitch = A.pitch, plane = A.plane, nv = A.nval;
+    const int perx = A.flags & 1, pery = (A.flags >> 1) & 1, perz = (A.flags >> 2) & 1;
+    const bool slabs = (A.flags & 8) != 0, injects = (A.flags & 16) != 0;
+
+    const int i = i0 - 1 + lane;
+    const int xi = wrap(i, nx, perx);          // -1 beyond a wall
+    const bool own_x = lane >= 1 && lane <= RW && i < nx;
+    int yj[NR], off[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        yj[r] = wrap(j0 - 1 + r, ny, pery);
+        off[r] = (xi < 0 || yj[r] < 0) ? -1 : yj[r] * pitch + xi;
+    }
+    auto zplane = [&](int kk) -> int { return wrap(kk, nz, perz); };
+    auto ld = [&](const T *a, int kp, int o) -> T {
+        const bool ok = kp >= 0 && o >= 0;
+        const T v = a[ok ? kp * plane + o : 0];
+        return ok ? v : T(0);
+    };
+    const T *ein = A.ein, *hin = A.hin;
+    const T dt = A.dt;
+    const T stx = A.step[0], sty = A.step[1], stz = A.step[2];
+    const T *coef = A.coef;
+    const int fex = 0, fhx = nx, fey = 2 * nx, fhy = 2 * nx + ny, fez = 2 * (nx + ny),
+              fhz = 2 * (nx + ny) + nz;
+    const T fxh = coef[fhx + max(xi, 0)], fxe = coef[fex + max(xi, 0)];
+
+    const bool has_pro = kz0 > 0 || perz;
+    const int kfirst = has_pro ? kz0 - 1 : kz0;
+
+    T ek[NR][3];  // E of this plane
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int kp = zplane(kfirst);
+        ek[r][0] = ld(ein, kp, off[r]);
+        ek[r][1] = ld(ein + nv, kp, off[r]);
+        ek[r][2] = ld(ein + 2 * nv, kp, off[r]);
+    }
+    T hp[RY][2];  // new Hx, Hy of the plane below, on the strip's rows
+#pragma unroll
+    for (int r = 0; r < RY; r++) hp[r][0] = hp[r][1] = T(0);
+
+    for (int kk = kfirst; kk < kz1; kk++) {
+        const bool pro = kk < kz0;
+        const int k = zplane(kk);
+        const int k1 = zplane(kk + 1);
+        // E of the next plane, old H of this one and the material ids
+        T en[NR][3];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            en[r][0] = ld(ein, k1, off[r]);
+            en[r][1] = ld(ein + nv, k1, off[r]);
+            en[r][2] = ld(ein + 2 * nv, k1, off[r]);
+        }
+        T ho[RY + 1][3];
+#pragma unroll
+        for (int r = 0; r <= RY; r++) {
+            ho[r][0] = ld(hin, k, off[r]);
+            ho[r][1] = r == 0 ? T(0) : ld(hin + nv, k, off[r]);
+            ho[r][2] = ld(hin + 2 * nv, k, off[r]);
+        }
+        uint8_t id[RY];
+#pragma unroll
+        for (int r = 0; r < RY; r++)
+            id[r] = MODE == 0 ? A.ids[off[r + 1] >= 0 ? k * plane + off[r + 1] : 0] : 0;
+        const T fzh = coef[fhz + k], fze = coef[fez + k];
+        // new H on rows j0 - 1 .. j0 + RY - 1
+        T hn[RY + 1][3];
+#pragma unroll
+        for (int r = 0; r <= RY; r++) {
+            const T ey_n = from_next_lane(ek[r][1]);
+            const T ez_n = from_next_lane(ek[r][2]);
+            const int j = yj[r];
+            const bool ok = off[r] >= 0 && lane < 63;
+            const T fyh = coef[fhy + max(j, 0)];
+            const T dyx = ek[r + 1][2] - ek[r][2], dzx = en[r][1] - ek[r][1];  // Hx
+            const T dzy = en[r][0] - ek[r][0], dxy = ez_n - ek[r][2];          // Hy
+            const T dxz = ey_n - ek[r][1], dyz = ek[r + 1][0] - ek[r][0];      // Hz
+            T v0 = ho[r][0] - dt * (dyx * fyh - dzx * fzh);
+            T v1 = ho[r][1] - dt * (dzy * fzh - dxy * fxh);
+            T v2 = ho[r][2] - dt * (dxz * fxh - dyz * fyh);
+            if (slabs && ok) {
+                const SlabAt P = slab_at(A, K, xi, j, k);
+                const bool wr = !pro && r > 0 && own_x && j0 - 1 + r < ny;
+                if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, dyx / sty, wr));
+                if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, dzx / stz, wr));
+                if (r > 0 && P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, dxy / stx, wr));
+                if (r > 0 && P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, dzy / stz, wr));
+                if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, wr));
+                if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, wr));
+            }
+            if (injects && ok) inject<T, 1>(A, K, t, xi, j, k, r == 0 ? 5 : 7, v0, v1, v2, T(0), T(0), T(0));
+            hn[r][0] = ok ? v0 : T(0);
+            hn[r][1] = ok ? v1 : T(0);
+            hn[r][2] = ok ? v2 : T(0);
+        }
+        // new E on the strip
+#pragma unroll
+        for (int r = 1; r <= RY; r++) {
+            const T hy_p = from_prev_lane(hn[r][1]);
+            const T hz_p = from_prev_lane(hn[r][2]);
+            const bool own = own_x && j0 - 1 + r < ny;
+            if (!pro && own) {
+                const int o = k * plane + off[r];
+                A.hout[o] = hn[r][0];
+                A.hout[nv + o] = hn[r][1];
+                A.hout[2 * nv + o] = hn[r][2];
+                const int j = yj[r];
+                const T dyx = hn[r][2] - hn[r - 1][2], dzx = hn[r][1] - hp[r - 1][1];  // Ex
+                const T dzy = hn[r][0] - hp[r - 1][0], dxy = hn[r][2] - hz_p;          // Ey
+                const T dxz = hn[r][1] - hy_p, dyz = hn[r][0] - hn[r - 1][0];          // Ez
+                T ca0, ca1, ca2, cb0, cb1, cb2;
+                int g0, g1, g2;
+                if (MODE == 0) {
+                    const int q = id[r - 1];
+                    const T *e = tab + 6 * q;
+                    ca0 = e[0]; cb0 = e[1]; ca1 = e[2]; cb1 = e[3]; ca2 = e[4]; cb2 = e[5];
+                    g0 = grp[3 * q]; g1 = grp[3 * q + 1]; g2 = grp[3 * q + 2];
+                } else {
+                    ca0 = A.cacb[o]; ca1 = A.cacb[nv + o]; ca2 = A.cacb[2 * nv + o];
+                    cb0 = A.cacb[3 * nv + o]; cb1 = A.cacb[4 * nv + o]; cb2 = A.cacb[5 * nv + o];
+                    g0 = A.grp ? A.grp[o] : -1;
+                    g1 = A.grp ? A.grp[nv + o] : -1;
+                    g2 = A.grp ? A.grp[2 * nv + o] : -1;
+                }
+                const T fye = coef[fey + j];
+                T v0 = ca0 * ek[r][0] + cb0 * (dyx * fye - dzx * fze);
+                T v1 = ca1 * ek[r][1] + cb1 * (dzy * fze - dxy * fxe);
+                T v2 = ca2 * ek[r][2] + cb2 * (dxz * fxe - dyz * fye);
+                if (slabs) {
+                    const SlabAt P = slab_at(A, K, xi, j, k);
+                    if (P.side[1] >= 0) v0 = v0 + cb0 * (T(1) * psi_at<0, 0, 1>(A, K, P, dyx / sty, true));
+                    if (P.side[2] >= 0) v0 = v0 + cb0 * (T(-1) * psi_at<0, 0, 2>(A, K, P, dzx / stz, true));
+                    if (P.side[0] >= 0) v1 = v1 + cb1 * (T(-1) * psi_at<0, 1, 0>(A, K, P, dxy / stx, true));
+                    if (P.side[2] >= 0) v1 = v1 + cb1 * (T(1) * psi_at<0, 1, 2>(A, K, P, dzy / stz, true));
+                    if (P.side[0] >= 0) v2 = v2 + cb2 * (T(1) * psi_at<0, 2, 0>(A, K, P, dxz / stx, true));
+                    if (P.side[1] >= 0) v2 = v2 + cb2 * (T(-1) * psi_at<0, 2, 1>(A, K, P, dyz / sty, true));
+                }
+                if (injects) inject<T, 0>(A, K, t, xi, j, k, 7, v0, v1, v2, cb0, cb1, cb2);
+                if (g0 >= 0) v0 = media_update(A, K, g0, 0, o, v0, cb0, ek[r][0]);
+                if (g1 >= 0) v1 = media_update(A, K, g1, 1, o, v1, cb1, ek[r][1]);
+                if (g2 >= 0) v2 = media_update(A, K, g2, 2, o, v2, cb2, ek[r][2]);
+                A.eout[o] = v0;
+                A.eout[nv + o] = v1;
+                A.eout[2 * nv + o] = v2;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            hp[r][0] = hn[r + 1][0];
+            hp[r][1] = hn[r + 1][1];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            ek[r][0] = en[r][0];
+            ek[r][1] = en[r][1];
+            ek[r][2] = en[r][2];
+        }
+    }
+}
+
 // Copies this step's probe and transform values out of copy `src`: out[v] = field[index[v]].
 template <typename T>
 __global__ void tap_values(const T *const *fields /* Ex Ey Ez Hx Hy Hz */, const int64_t *index,

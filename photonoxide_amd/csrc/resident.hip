@@ -164,6 +164,7 @@ struct pxfdtd_resident {
     long long probe_rows = 0, probe_cap = 0;
     // stepping
     int p = 0;
+    int kernel = 1;  // 0: LDS tiles (64 x ty, barrier per plane); 1: register march (62 x ty)
     int ty = 4, kz = 16;
     double last_ms = 0;
     // per-launch timing of the fused kernel, when asked for
@@ -457,9 +458,9 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
     if (r->nval * 2 >= (1LL << 31))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "a grid of %lld values per component is beyond 32-bit indexing", r->nval);
     // tiling defaults: 8 rows per tile, chunks that give the chip many workgroups
-    r->ty = p->ny >= 4 ? 4 : 2;
+    r->ty = 2;
     {
-        const long long tiles = (long long)((p->nx + 63) / 64) * ((p->ny + r->ty - 1) / r->ty);
+        const long long tiles = (long long)((p->nx + RW - 1) / RW) * ((p->ny + r->ty - 1) / r->ty) / 4;
         int kz = p->nz;
         while (kz > 8 && tiles * ((p->nz + kz - 1) / kz) < 2048) kz = (kz + 1) / 2;
         r->kz = std::max(1, kz);
@@ -575,14 +576,56 @@ extern "C" int pxfdtd_resident_download(pxfdtd_resident *r, double *e, double *h
 // Each tile's list holds the injections on its own values and on its halo (the row below and
 // the column left of it, where H is recomputed), sorted by plane, each plane's in the
 // caller's order: a value's injections apply in that order.
+// The tile geometry of the kernel in use: own values [i0, i0 + tw) x [j0, j0 + th), tiles at
+// multiples of tw and th; H is also computed on the row below and the column left of the
+// tile (the LDS kernel: the row over the tile's columns and the column over its rows; the
+// register march: the whole rectangle [i0 - 1, i0 + tw) x [j0 - 1, j0 + th)).
+static void tile_shape(const pxfdtd_resident *r, int &tw, int &th) {
+    tw = r->kernel == 0 ? TX : RW;
+    th = r->ty;
+}
+
+static bool h_region_has(const pxfdtd_resident *r, int bx, int by, int i, int j) {
+    int tw, th;
+    tile_shape(r, tw, th);
+    const int i0 = bx * tw, j0 = by * th;
+    // the tile's own and halo coordinates, unwrapped, against i or i +- n (a periodic wrap)
+    auto in_x = [&](int lo, int hi) {  // [lo, hi)
+        for (int d : {0, r->nx, -r->nx}) {
+            if (d && !r->periodic[0]) continue;
+            const int ii = i + d;
+            if (ii >= lo && ii < hi) return true;
+        }
+        return false;
+    };
+    auto in_y = [&](int lo, int hi) {
+        for (int d : {0, r->ny, -r->ny}) {
+            if (d && !r->periodic[1]) continue;
+            const int jj = j + d;
+            if (jj >= lo && jj < hi) return true;
+        }
+        return false;
+    };
+    const int iw = std::min(i0 + tw, r->nx + (r->periodic[0] ? 1 : 0));
+    if (r->kernel == 1) return in_x(i0 - 1, iw) && in_y(j0 - 1, j0 + th);
+    const bool own = in_x(i0, i0 + tw) && in_y(j0, j0 + th);
+    const bool row = in_x(i0, i0 + tw) && in_y(j0 - 1, j0);
+    const bool col = in_x(i0 - 1, i0) && in_y(j0, j0 + th);
+    return own || row || col;
+}
+
+// Each tile's list holds the injections on its own values and, for H, on the values around
+// it where H is recomputed, sorted by plane, each plane's in the caller's order: a value's
+// injections apply in that order.
 static int build_injection_lists(pxfdtd_resident *r) {
     r->have_inj = !r->inj_host.empty();
     if (!r->have_inj) return PXFDTD_OK;
-    const int tx = (r->nx + TX - 1) / TX, tyn = (r->ny + r->ty - 1) / r->ty;
+    int tw, th;
+    tile_shape(r, tw, th);
+    const int tx = (r->nx + tw - 1) / tw, tyn = (r->ny + th - 1) / th;
     const long long tiles = (long long)tx * tyn;
     struct Item { long long tile; int k; long long seq; Injection in; };
     std::vector<Item> items;
-    const long long N = r->ncells;
     for (size_t q = 0; q < r->inj_host.size(); q++) {
         const pxfdtd_injection &s = r->inj_host[q];
         const long long v = s.index;
@@ -590,27 +633,26 @@ static int build_injection_lists(pxfdtd_resident *r) {
         Injection in;
         in.k = k; in.i = i; in.j = j; in.code = s.field * 4 + s.component; in.emitter = s.emitter;
         in.pad = 0; in.a_re = s.a_re; in.a_im = s.a_im;
-        (void)N;
-        const int bx = i / TX, by = j / r->ty;
-        items.push_back({(long long)by * tx + bx, k, (long long)q, in});
-        if (s.field == PXFDTD_H) {
-            // halo row of the tile above: j == j0' - 1
-            int jn = j + 1, byn = -1;
-            if (jn < r->ny && jn % r->ty == 0) byn = jn / r->ty;
-            else if (jn == r->ny && r->periodic[1]) byn = 0;
-            if (byn >= 0 && byn != by) items.push_back({(long long)byn * tx + bx, k, (long long)q, in});
-            // halo column of the tile to the right: i == i0' - 1
-            int in_ = i + 1, bxn = -1;
-            if (in_ < r->nx && in_ % TX == 0) bxn = in_ / TX;
-            else if (in_ == r->nx && r->periodic[0]) bxn = 0;
-            if (bxn >= 0 && bxn != bx) items.push_back({(long long)by * tx + bxn, k, (long long)q, in});
+        if (s.field == PXFDTD_E) {
+            items.push_back({(long long)(j / th) * tx + i / tw, k, (long long)q, in});
+            continue;
         }
+        int bxs[4] = {i / tw, (i + 1) / tw, 0, tx - 1}, bys[4] = {j / th, (j + 1) / th, 0, tyn - 1};
+        std::vector<long long> seen;
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 4; c++) {
+                const int bx = bxs[a], by = bys[c];
+                if (bx < 0 || bx >= tx || by < 0 || by >= tyn) continue;
+                const long long tile = (long long)by * tx + bx;
+                if (std::find(seen.begin(), seen.end(), tile) != seen.end()) continue;
+                if (!h_region_has(r, bx, by, i, j)) continue;
+                seen.push_back(tile);
+                items.push_back({tile, k, (long long)q, in});
+            }
     }
     std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
         return a.tile != b.tile ? a.tile < b.tile : (a.k != b.k ? a.k < b.k : a.seq < b.seq);
     });
-    // a tile's own value and its halo are different positions, so an entry never appears
-    // twice for one value within a tile
     std::vector<int32_t> off(tiles * (r->nz + 1), 0);
     std::vector<Injection> flat(items.size());
     size_t q = 0;
@@ -648,8 +690,11 @@ extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_i
 
 extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes) {
     if (rows) {
-        if (rows != 2 && rows != 4 && rows != 8)
-            return px_fail(PXFDTD_ERR_INVALID, "rows per tile must be 2, 4 or 8, got %d", rows);
+        const bool ok = r->kernel == 0 ? (rows == 2 || rows == 4 || rows == 8)
+                                       : (rows == 1 || rows == 2 || rows == 4);
+        if (!ok)
+            return px_fail(PXFDTD_ERR_INVALID, "rows per tile must be %s for this kernel, got %d",
+                           r->kernel == 0 ? "2, 4 or 8" : "1, 2 or 4", rows);
         r->ty = rows;
     }
     if (planes) {
@@ -659,6 +704,16 @@ extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int3
     HIPCHK(hipSetDevice(r->dev->ordinal));
     return build_injection_lists(r);
 }
+
+extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
+    if (kind != 0 && kind != 1) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0 or 1");
+    r->kernel = kind;
+    r->ty = kind == 0 ? 8 : 2;
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    return build_injection_lists(r);
+}
+
+extern "C" int pxfdtd_resident_get_kernel(const pxfdtd_resident *r) { return r->kernel; }
 
 extern "C" int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes) {
     *rows = r->ty;
@@ -835,11 +890,15 @@ static StepArgs<T> args_of(const pxfdtd_resident *r) {
     if (slabs) A.flags |= 8;
     if (r->have_inj) A.flags |= 16;
     A.coef = r->coef.as<T>();
-    A.tiles_x = (r->nx + TX - 1) / TX;
-    A.tiles_y = (r->ny + r->ty - 1) / r->ty;
+    int tw, th;
+    tile_shape(r, tw, th);
+    A.tiles_x = (r->nx + tw - 1) / tw;
+    A.tiles_y = (r->ny + th - 1) / th;
     A.kz = r->kz;
     A.blocks = A.tiles_x * A.tiles_y * ((r->nz + r->kz - 1) / r->kz);
-    A.per_xcd = (A.blocks + 7) / 8;
+    // the LDS kernel takes a tile per workgroup, the march a strip per wavefront (4 per group)
+    const int groups = r->kernel == 0 ? A.blocks : (A.blocks + 3) / 4;
+    A.per_xcd = (groups + 7) / 8;
     A.dt = (T)r->dt;
     A.ids = r->ids.as<uint8_t>();
     A.table = r->table.as<T>();
@@ -884,8 +943,21 @@ static int launch_ty(const StepArgs<T> &A, hipStream_t s, size_t lds) {
     return PXFDTD_OK;
 }
 
+template <typename T, int MODE, int RY>
+static int launch_march(const StepArgs<T> &A, hipStream_t s, size_t lds) {
+    hipLaunchKernelGGL((march_step<T, MODE, RY>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+    return PXFDTD_OK;
+}
+
 template <typename T, int MODE>
-static int launch_mode(const StepArgs<T> &A, int ty, hipStream_t s, size_t lds) {
+static int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, size_t lds) {
+    if (kernel == 1) {
+        switch (ty) {
+        case 1: return launch_march<T, MODE, 1>(A, s, lds);
+        case 2: return launch_march<T, MODE, 2>(A, s, lds);
+        default: return launch_march<T, MODE, 4>(A, s, lds);
+        }
+    }
     switch (ty) {
     case 2: return launch_ty<T, MODE, 2>(A, s, lds);
     case 4: return launch_ty<T, MODE, 4>(A, s, lds);
@@ -943,9 +1015,9 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         A.wave_step = (int)n;
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
         if (r->mode == 0)
-            TRY((launch_mode<T, 0>(A, r->ty, s, lds)));
+            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds)));
         else
-            TRY((launch_mode<T, 1>(A, r->ty, s, lds)));
+            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds)));
         HIPCHK(hipGetLastError());
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
         r->p = q;

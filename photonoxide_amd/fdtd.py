@@ -427,6 +427,16 @@ class Simulation:
             raise N.NativeError(N.ERR_UNSUPPORTED, "no device")
         N.check(N.lib.pxfdtd_resident_set_tiling(r, rows, planes))
 
+    def set_kernel(self, kind: int):
+        """0: tiles exchanging H through LDS; 1: barrier-free strips marching along z."""
+        r = self.resident()
+        if not r:
+            raise N.NativeError(N.ERR_UNSUPPORTED, "no device")
+        N.check(N.lib.pxfdtd_resident_set_kernel(r, kind))
+
+    def kernel(self) -> int:
+        return N.lib.pxfdtd_resident_get_kernel(self.resident())
+
     def tiling(self):
         r = self.resident()
         rows, planes = C.c_int32(), C.c_int32()

@@ -59,14 +59,16 @@ def same_bits(a, b):
                           np.asarray(b, dtype=np.float64).view(np.uint64))
 
 
+@pytest.mark.parametrize("kernel", [1, 0])
 @pytest.mark.parametrize("shape", ["3d", "2d", "periodic"])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_device_is_the_oracle_bit_for_bit(gpu, shape, prec):
+def test_device_is_the_oracle_bit_for_bit(gpu, shape, prec, kernel):
     import photonoxide_amd as P
     precision = P.Precision.F64 if prec == "f64" else P.Precision.F32
     spec = busy_spec(shape == "2d", shape == "periodic")
-    print(f"case busy {shape} {prec}", flush=True)
+    print(f"case busy {shape} {prec} kernel {kernel}", flush=True)
     o, p = build_pair(spec, gpu, precision)
+    p.set_kernel(kernel)
     o.run(80)
     p.run(80)
     fo, fp = o.fields(), product_fields(p)
@@ -107,8 +109,9 @@ def test_f32_within_north_star_tolerance_of_f64_oracle(gpu):
     assert rel < 1e-5, rel
 
 
-@pytest.mark.parametrize("rows,planes", [(2, 1), (2, 3), (8, 19), (4, 5)])
-def test_tiling_changes_no_bit(gpu, rows, planes):
+@pytest.mark.parametrize("kernel,rows,planes", [(0, 2, 1), (0, 2, 3), (0, 8, 19), (0, 4, 5),
+                                                (1, 1, 1), (1, 2, 3), (1, 4, 19), (1, 1, 7)])
+def test_tiling_changes_no_bit(gpu, kernel, rows, planes):
     import photonoxide_amd as P
     spec = busy_spec(False)
     ref = product(spec)
@@ -116,8 +119,9 @@ def test_tiling_changes_no_bit(gpu, rows, planes):
     ref.run(50)
     p = product(spec)
     p.set_device(gpu, P.Precision.F32)
+    p.set_kernel(kernel)
     p.set_tiling(rows, planes)
-    assert p.tiling() == (rows, planes)
+    assert p.tiling() == (rows, planes) and p.kernel() == kernel
     p.run(50)
     assert same_bits(product_fields(ref), product_fields(p))
 
