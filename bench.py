@@ -159,12 +159,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else ""),
-                   "cells": cells, "kernel": ["lds_tiles", "register_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
+                   "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch"][s.kernel()], "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
                    "monitors": "2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums",
                    "psi_values": psi},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": ["fused_step", "march_step"][s.kernel()], "kernel_ms": round(kernel_ms, 5),
+                     "kernel": ["fused_step", "march_step", "march_step"][s.kernel()], "kernel_ms": round(kernel_ms, 5),
                      "bytes_per_launch": bytes_step,
                      "gcell_per_s_kernel": round(cells / (kernel_ms * 1e-3) / 1e9, 3)},
     }

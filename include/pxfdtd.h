@@ -152,7 +152,8 @@ int pxfdtd_resident_read_probes(pxfdtd_resident *r, double *out, int64_t capacit
 int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_t capacity);
 int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r);
 /* the step kernel: 0 tiles of 64 x rows exchanging H through LDS; 1 (the default) strips of
- * 62 x rows per wavefront marching along z with no barrier */
+ * 62 x rows per wavefront marching along z with no barrier, loading a plane ahead; 2 the
+ * same without the plane-ahead loads */
 int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind);
 int pxfdtd_resident_get_kernel(const pxfdtd_resident *r);
 /* the tiling: rows per tile (kernel 0: 2, 4, 8; kernel 1: 1, 2, 4) and planes per chunk; 0

@@ -98,6 +98,8 @@ struct StepArgs {
     // kappa/step at the nodes (fe) and halves (fh) along x, y, z: fe_x, fh_x, fe_y, fh_y,
     // fe_z, fh_z, n_axis values each; then the CPML's b and a
     const T *coef;
+    int ncpml;                // CPML coefficients after the difference factors
+    const int4 *units;        // register march: the strips and plane chunks of this launch
     const StepCold<T> *cold;  // the one for this step's parity
     int wave_step;            // this step's row of cold->waves
 };
@@ -557,6 +559,68 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
 // adjacent lane (a cross-lane shuffle), along y from the thread's own registers, along z
 // from the plane before (kept) and after (loaded). The waves of a workgroup are independent:
 // the chip stays busy from many waves in flight rather than from a synchronized pipeline.
+
+// The CPML updates of one value's components in one half step, split in two: psi_load issues
+// every psi load of the value (from the top of the plane, with the field loads, so their
+// latencies overlap), psi_finish updates and stores them. Combos in the reference's order per
+// component (axis x -> y -> z): (Cx, y) (Cx, z) (Cy, x) (Cy, z) (Cz, x) (Cz, y). `cp` is the
+// CPML coefficient table staged in LDS (b then a, count each, per [axis][side][field] at
+// K.cpml_off minus `cbase`).
+template <int F, typename T>
+__device__ __forceinline__ void psi_load(const StepCold<T> &K, const SlabAt &P, int mask, T old[6]) {
+    constexpr int CC[6] = {0, 0, 1, 1, 2, 2}, WW[6] = {1, 2, 0, 2, 0, 1};
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        const int c = CC[n], w = WW[n];
+        const int side = P.side[w];
+        const bool on = side >= 0 && ((mask >> c) & 1);
+        const int off = (side == 1 ? K.psi_off[F][c][w][1] : K.psi_off[F][c][w][0]) + P.q[w];
+        old[n] = on ? K.psi_in[off] : T(0);
+    }
+}
+
+template <int F, typename T>
+__device__ __forceinline__ void psi_finish(const StepCold<T> &K, const T *cp, int cbase,
+                                           const SlabAt &P, int mask, const T old[6], T d_yx,
+                                           T d_zx, T d_xy, T d_zy, T d_xz, T d_yz, T stx, T sty,
+                                           T stz, bool write, T u[6]) {
+    constexpr int CC[6] = {0, 0, 1, 1, 2, 2}, WW[6] = {1, 2, 0, 2, 0, 1};
+    const T d[6] = {d_yx / sty, d_zx / stz, d_xy / stx, d_zy / stz, d_xz / stx, d_yz / sty};
+#pragma unroll
+    for (int n = 0; n < 6; n++) {
+        const int c = CC[n], w = WW[n];
+        const int side = P.side[w];
+        const bool on = side >= 0 && ((mask >> c) & 1);
+        const int co = (side == 1 ? K.cpml_off[w][1][F] : K.cpml_off[w][0][F]) - cbase;
+        const int cnt = side == 1 ? K.high[w] : K.low[w];
+        u[n] = on ? cp[co + P.s[w]] * old[n] + cp[co + cnt + P.s[w]] * d[n] : T(0);
+        if (write && on) {
+            const int off = (side == 1 ? K.psi_off[F][c][w][1] : K.psi_off[F][c][w][0]) + P.q[w];
+            K.psi_out[off] = u[n];
+        }
+    }
+}
+
+// Raw buffer access (CDNA's MUBUF): a 128-bit resource in scalar registers holds the base and
+// the size in bytes; a lane's 32-bit byte offset at or beyond the size reads 0. Wall
+// neighbours and planes beyond a wall then load as zero with no branch and no 64-bit address
+// arithmetic per load.
+constexpr uint32_t OUTSIDE = 0x80000000u;  // an offset beyond every array (each < 2 GiB)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
 constexpr int RW = 62;
 
 template <typename T>
@@ -564,10 +628,16 @@ __device__ __forceinline__ T from_next_lane(T v) { return __shfl_down(v, 1, 64);
 template <typename T>
 __device__ __forceinline__ T from_prev_lane(T v) { return __shfl_up(v, 1, 64); }
 
-template <typename T, int MODE, int RY>
+// SLABS: the strips and plane chunks that touch a CPML slab run in this kernel with the psi
+// updates compiled in; the others in one without them, lean in registers. PF: the loads of
+// the next plane are issued before this plane's arithmetic (more bytes in flight per wave,
+// at the cost of registers).
+template <typename T, int MODE, int RY, bool SLABS, bool PF>
 __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
     constexpr int NR = RY + 2;  // rows of E held: j0 - 1 .. j0 + RY
+    constexpr int CPML_MAX = 2048;
     __shared__ StepCold<T> K;
+    __shared__ T cp[SLABS ? CPML_MAX : 1];  // the CPML's b and a
     extern __shared__ unsigned char smem_raw[];
     T *tab = reinterpret_cast<T *>(smem_raw);                 // [256][3][2]
     int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);  // [256][3]
@@ -582,26 +652,32 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
         int *dst = reinterpret_cast<int *>(&K);
         for (int q = tid; q < (int)(sizeof(StepCold<T>) / 4); q += 256) dst[q] = src[q];
     }
+    const int cbase = 2 * (A.nx + A.ny + A.nz);
+    if (SLABS)
+        for (int q = tid; q < A.ncpml && q < CPML_MAX; q += 256) cp[q] = A.coef[cbase + q];
     __syncthreads();  // the only barrier: the parameters in LDS
 
     const int lane = tid & 63;
     // workgroups are dealt round-robin to the 8 XCDs: each XCD takes a contiguous run of
-    // strips, so neighbouring strips (which read each other's edges) share an L2
+    // units, so neighbouring strips (which read each other's edges) share an L2
     const int b = (int)(blockIdx.x % 8) * A.per_xcd + (int)(blockIdx.x / 8);
     const int w = b * 4 + (tid >> 6);
     if (w >= A.blocks) return;
-    const int tiles = A.tiles_x * A.tiles_y;
-    const int bz = w / tiles, t = w % tiles;
-    const int by = t / A.tiles_x, bx = t % A.tiles_x;
+    const int4 unit = A.units[w];  // strip x, strip y, first plane, end plane
+    const int bx = unit.x, by = unit.y, kz0 = unit.z, kz1 = unit.w;
+    const int t = by * A.tiles_x + bx;
     const int i0 = bx * RW, j0 = by * RY;
-    const int kz0 = bz * A.kz;
-    const int kz1 = min(A.nz, kz0 + A.kz);
     const int nx = A.nx, ny = A.ny, nz = A.nz, pitch = A.pitch, plane = A.plane, nv = A.nval;
     const int perx = A.flags & 1, pery = (A.flags >> 1) & 1, perz = (A.flags >> 2) & 1;
-    const bool slabs = (A.flags & 8) != 0, injects = (A.flags & 16) != 0;
+    // injections: only strips whose list holds any
+    bool injects = false;
+    if (A.flags & 16) {
+        const int32_t *io = K.inj_off + (long long)t * (nz + 1);
+        injects = io[nz] > io[0];
+    }
 
     const int i = i0 - 1 + lane;
-    const int xi = wrap(i, nx, perx);          // -1 beyond a wall
+    const int xi = wrap(i, nx, perx);  // -1 beyond a wall
     const bool own_x = lane >= 1 && lane <= RW && i < nx;
     int yj[NR], off[NR];
 #pragma unroll
@@ -610,12 +686,20 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
         off[r] = (xi < 0 || yj[r] < 0) ? -1 : yj[r] * pitch + xi;
     }
     auto zplane = [&](int kk) -> int { return wrap(kk, nz, perz); };
-    auto ld = [&](const T *a, int kp, int o) -> T {
-        const bool ok = kp >= 0 && o >= 0;
-        const T v = a[ok ? kp * plane + o : 0];
-        return ok ? v : T(0);
-    };
-    const T *ein = A.ein, *hin = A.hin;
+    // byte offsets within a plane, OUTSIDE beyond a wall; the plane's offset rides in soffset
+    uint32_t vo[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) vo[r] = off[r] >= 0 ? (uint32_t)off[r] * sizeof(T) : OUTSIDE;
+    const uint32_t cbytes = (uint32_t)nv * sizeof(T);
+    const __amdgpu_buffer_rsrc_t rex = rsrc(A.ein, cbytes), rey = rsrc(A.ein + nv, cbytes),
+                                 rez = rsrc(A.ein + 2 * nv, cbytes), rhx = rsrc(A.hin, cbytes),
+                                 rhy = rsrc(A.hin + nv, cbytes), rhz = rsrc(A.hin + 2 * nv, cbytes),
+                                 rid = rsrc(A.ids, MODE == 0 ? (uint32_t)nv : 0u),
+                                 rnone = rsrc(A.ein, 0u);
+    const uint32_t pbytes = (uint32_t)plane * sizeof(T);
+    // a plane beyond a wall reads through an empty resource: zeros
+    auto pick = [&](__amdgpu_buffer_rsrc_t r, int kp) { return kp >= 0 ? r : rnone; };
+    auto soff = [&](int kp) { return kp >= 0 ? (uint32_t)kp * pbytes : 0u; };
     const T dt = A.dt;
     const T stx = A.step[0], sty = A.step[1], stz = A.step[2];
     const T *coef = A.coef;
@@ -626,41 +710,82 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
     const bool has_pro = kz0 > 0 || perz;
     const int kfirst = has_pro ? kz0 - 1 : kz0;
 
-    T ek[NR][3];  // E of this plane
+    // what a plane kk loads: E of plane kk + 1, old H and the ids of plane kk
+    struct Loads {
+        T en[NR][3];
+        T ho[RY + 1][3];
+        uint8_t id[RY];
+    };
+    auto load = [&](Loads &L, int kk) {
+        const int k = zplane(kk), k1 = zplane(kk + 1);
+        const uint32_t s1 = soff(k1), s0 = soff(k);
+        const __amdgpu_buffer_rsrc_t ax = pick(rex, k1), ay = pick(rey, k1), az = pick(rez, k1);
+        const __amdgpu_buffer_rsrc_t hx = pick(rhx, k), hy = pick(rhy, k), hz = pick(rhz, k);
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
+        for (int r = 0; r < NR; r++) {
+            L.en[r][0] = bload<T>(ax, vo[r], s1);
+            L.en[r][1] = bload<T>(ay, vo[r], s1);
+            L.en[r][2] = bload<T>(az, vo[r], s1);
+        }
+#pragma unroll
+        for (int r = 0; r <= RY; r++) {
+            L.ho[r][0] = bload<T>(hx, vo[r], s0);
+            L.ho[r][1] = r == 0 ? T(0) : bload<T>(hy, vo[r], s0);
+            L.ho[r][2] = bload<T>(hz, vo[r], s0);
+        }
+        if (MODE == 0) {
+            const __amdgpu_buffer_rsrc_t ri = pick(rid, k);
+            const uint32_t si = k >= 0 ? (uint32_t)k * (uint32_t)plane : 0u;
+#pragma unroll
+            for (int r = 0; r < RY; r++)
+                L.id[r] = __builtin_amdgcn_raw_buffer_load_b8(
+                    ri, (int)(off[r + 1] >= 0 ? (uint32_t)off[r + 1] : OUTSIDE), (int)si, 0);
+        } else {
+#pragma unroll
+            for (int r = 0; r < RY; r++) L.id[r] = 0;
+        }
+    };
+
+    T ek[NR][3];  // E of this plane
+    {
         const int kp = zplane(kfirst);
-        ek[r][0] = ld(ein, kp, off[r]);
-        ek[r][1] = ld(ein + nv, kp, off[r]);
-        ek[r][2] = ld(ein + 2 * nv, kp, off[r]);
+        const uint32_t s0 = soff(kp);
+        const __amdgpu_buffer_rsrc_t ax = pick(rex, kp), ay = pick(rey, kp), az = pick(rez, kp);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            ek[r][0] = bload<T>(ax, vo[r], s0);
+            ek[r][1] = bload<T>(ay, vo[r], s0);
+            ek[r][2] = bload<T>(az, vo[r], s0);
+        }
     }
     T hp[RY][2];  // new Hx, Hy of the plane below, on the strip's rows
 #pragma unroll
     for (int r = 0; r < RY; r++) hp[r][0] = hp[r][1] = T(0);
+    Loads nxt;
+    if (PF) load(nxt, kfirst);
 
     for (int kk = kfirst; kk < kz1; kk++) {
         const bool pro = kk < kz0;
         const int k = zplane(kk);
-        const int k1 = zplane(kk + 1);
-        // E of the next plane, old H of this one and the material ids
-        T en[NR][3];
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            en[r][0] = ld(ein, k1, off[r]);
-            en[r][1] = ld(ein + nv, k1, off[r]);
-            en[r][2] = ld(ein + 2 * nv, k1, off[r]);
+        Loads L;
+        if (PF) {
+            L = nxt;
+            if (kk + 1 < kz1) load(nxt, kk + 1);
+        } else {
+            load(L, kk);
         }
-        T ho[RY + 1][3];
+        // the psi values this plane updates, loaded with the fields
+        SlabAt PH[RY + 1];
+        T psh[RY + 1][6], pse[RY][6];
+        if (SLABS) {
 #pragma unroll
-        for (int r = 0; r <= RY; r++) {
-            ho[r][0] = ld(hin, k, off[r]);
-            ho[r][1] = r == 0 ? T(0) : ld(hin + nv, k, off[r]);
-            ho[r][2] = ld(hin + 2 * nv, k, off[r]);
+            for (int r = 0; r <= RY; r++) {
+                PH[r] = slab_at(A, K, max(xi, 0), max(yj[r], 0), k);
+                const bool ok = off[r] >= 0 && lane < 63;
+                psi_load<1>(K, PH[r], ok ? (r == 0 ? 5 : 7) : 0, psh[r]);
+                if (r > 0) psi_load<0>(K, PH[r], own_x && j0 - 1 + r < ny ? 7 : 0, pse[r - 1]);
+            }
         }
-        uint8_t id[RY];
-#pragma unroll
-        for (int r = 0; r < RY; r++)
-            id[r] = MODE == 0 ? A.ids[off[r + 1] >= 0 ? k * plane + off[r + 1] : 0] : 0;
         const T fzh = coef[fhz + k], fze = coef[fez + k];
         // new H on rows j0 - 1 .. j0 + RY - 1
         T hn[RY + 1][3];
@@ -671,21 +796,25 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
             const int j = yj[r];
             const bool ok = off[r] >= 0 && lane < 63;
             const T fyh = coef[fhy + max(j, 0)];
-            const T dyx = ek[r + 1][2] - ek[r][2], dzx = en[r][1] - ek[r][1];  // Hx
-            const T dzy = en[r][0] - ek[r][0], dxy = ez_n - ek[r][2];          // Hy
-            const T dxz = ey_n - ek[r][1], dyz = ek[r + 1][0] - ek[r][0];      // Hz
-            T v0 = ho[r][0] - dt * (dyx * fyh - dzx * fzh);
-            T v1 = ho[r][1] - dt * (dzy * fzh - dxy * fxh);
-            T v2 = ho[r][2] - dt * (dxz * fxh - dyz * fyh);
-            if (slabs && ok) {
-                const SlabAt P = slab_at(A, K, xi, j, k);
+            const T dyx = ek[r + 1][2] - ek[r][2], dzx = L.en[r][1] - ek[r][1];  // Hx
+            const T dzy = L.en[r][0] - ek[r][0], dxy = ez_n - ek[r][2];          // Hy
+            const T dxz = ey_n - ek[r][1], dyz = ek[r + 1][0] - ek[r][0];        // Hz
+            T v0 = L.ho[r][0] - dt * (dyx * fyh - dzx * fzh);
+            T v1 = L.ho[r][1] - dt * (dzy * fzh - dxy * fxh);
+            T v2 = L.ho[r][2] - dt * (dxz * fxh - dyz * fyh);
+            if (SLABS) {
+                const SlabAt &P = PH[r];
+                const int mask = ok ? (r == 0 ? 5 : 7) : 0;
                 const bool wr = !pro && r > 0 && own_x && j0 - 1 + r < ny;
-                if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, dyx / sty, wr));
-                if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, dzx / stz, wr));
-                if (r > 0 && P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, dxy / stx, wr));
-                if (r > 0 && P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, dzy / stz, wr));
-                if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, wr));
-                if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, wr));
+                T u[6];
+                psi_finish<1>(K, cp, cbase, P, mask, psh[r], dyx, dzx, dxy, dzy, dxz, dyz, stx, sty,
+                              stz, wr, u);
+                if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * u[0]);
+                if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * u[1]);
+                if (r > 0 && P.side[0] >= 0) v1 = v1 - dt * (T(-1) * u[2]);
+                if (r > 0 && P.side[2] >= 0) v1 = v1 - dt * (T(1) * u[3]);
+                if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * u[4]);
+                if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * u[5]);
             }
             if (injects && ok) inject<T, 1>(A, K, t, xi, j, k, r == 0 ? 5 : 7, v0, v1, v2, T(0), T(0), T(0));
             hn[r][0] = ok ? v0 : T(0);
@@ -700,9 +829,10 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
             const bool own = own_x && j0 - 1 + r < ny;
             if (!pro && own) {
                 const int o = k * plane + off[r];
-                A.hout[o] = hn[r][0];
-                A.hout[nv + o] = hn[r][1];
-                A.hout[2 * nv + o] = hn[r][2];
+                const uint32_t uo = (uint32_t)o;
+                A.hout[uo] = hn[r][0];
+                A.hout[(uint32_t)nv + uo] = hn[r][1];
+                A.hout[2u * (uint32_t)nv + uo] = hn[r][2];
                 const int j = yj[r];
                 const T dyx = hn[r][2] - hn[r - 1][2], dzx = hn[r][1] - hp[r - 1][1];  // Ex
                 const T dzy = hn[r][0] - hp[r - 1][0], dxy = hn[r][2] - hz_p;          // Ey
@@ -710,7 +840,7 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
                 T ca0, ca1, ca2, cb0, cb1, cb2;
                 int g0, g1, g2;
                 if (MODE == 0) {
-                    const int q = id[r - 1];
+                    const int q = L.id[r - 1];
                     const T *e = tab + 6 * q;
                     ca0 = e[0]; cb0 = e[1]; ca1 = e[2]; cb1 = e[3]; ca2 = e[4]; cb2 = e[5];
                     g0 = grp[3 * q]; g1 = grp[3 * q + 1]; g2 = grp[3 * q + 2];
@@ -725,22 +855,25 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
                 T v0 = ca0 * ek[r][0] + cb0 * (dyx * fye - dzx * fze);
                 T v1 = ca1 * ek[r][1] + cb1 * (dzy * fze - dxy * fxe);
                 T v2 = ca2 * ek[r][2] + cb2 * (dxz * fxe - dyz * fye);
-                if (slabs) {
-                    const SlabAt P = slab_at(A, K, xi, j, k);
-                    if (P.side[1] >= 0) v0 = v0 + cb0 * (T(1) * psi_at<0, 0, 1>(A, K, P, dyx / sty, true));
-                    if (P.side[2] >= 0) v0 = v0 + cb0 * (T(-1) * psi_at<0, 0, 2>(A, K, P, dzx / stz, true));
-                    if (P.side[0] >= 0) v1 = v1 + cb1 * (T(-1) * psi_at<0, 1, 0>(A, K, P, dxy / stx, true));
-                    if (P.side[2] >= 0) v1 = v1 + cb1 * (T(1) * psi_at<0, 1, 2>(A, K, P, dzy / stz, true));
-                    if (P.side[0] >= 0) v2 = v2 + cb2 * (T(1) * psi_at<0, 2, 0>(A, K, P, dxz / stx, true));
-                    if (P.side[1] >= 0) v2 = v2 + cb2 * (T(-1) * psi_at<0, 2, 1>(A, K, P, dyz / sty, true));
+                if (SLABS) {
+                    const SlabAt &P = PH[r];
+                    T u[6];
+                    psi_finish<0>(K, cp, cbase, P, 7, pse[r - 1], dyx, dzx, dxy, dzy, dxz, dyz, stx,
+                                  sty, stz, true, u);
+                    if (P.side[1] >= 0) v0 = v0 + cb0 * (T(1) * u[0]);
+                    if (P.side[2] >= 0) v0 = v0 + cb0 * (T(-1) * u[1]);
+                    if (P.side[0] >= 0) v1 = v1 + cb1 * (T(-1) * u[2]);
+                    if (P.side[2] >= 0) v1 = v1 + cb1 * (T(1) * u[3]);
+                    if (P.side[0] >= 0) v2 = v2 + cb2 * (T(1) * u[4]);
+                    if (P.side[1] >= 0) v2 = v2 + cb2 * (T(-1) * u[5]);
                 }
                 if (injects) inject<T, 0>(A, K, t, xi, j, k, 7, v0, v1, v2, cb0, cb1, cb2);
                 if (g0 >= 0) v0 = media_update(A, K, g0, 0, o, v0, cb0, ek[r][0]);
                 if (g1 >= 0) v1 = media_update(A, K, g1, 1, o, v1, cb1, ek[r][1]);
                 if (g2 >= 0) v2 = media_update(A, K, g2, 2, o, v2, cb2, ek[r][2]);
-                A.eout[o] = v0;
-                A.eout[nv + o] = v1;
-                A.eout[2 * nv + o] = v2;
+                A.eout[uo] = v0;
+                A.eout[(uint32_t)nv + uo] = v1;
+                A.eout[2u * (uint32_t)nv + uo] = v2;
             }
         }
 #pragma unroll
@@ -750,9 +883,9 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
         }
 #pragma unroll
         for (int r = 0; r < NR; r++) {
-            ek[r][0] = en[r][0];
-            ek[r][1] = en[r][1];
-            ek[r][2] = en[r][2];
+            ek[r][0] = L.en[r][0];
+            ek[r][1] = L.en[r][1];
+            ek[r][2] = L.en[r][2];
         }
     }
 }

@@ -137,6 +137,7 @@ struct pxfdtd_resident {
     bool has_grp = false;
     Buf coef;  // fe, fh per axis, then the CPML's b and a
     int fe_off[3] = {0, 0, 0}, fh_off[3] = {0, 0, 0}, cpml_off[3][2][2];
+    int ncpml = 0;  // CPML coefficients in coef after the difference factors
     Buf psi[2];
     int psi_off[2][3][3][2];
     template <typename T> T *field(int copy, int q) const { return fields[copy].as<T>() + (size_t)q * nval; }
@@ -167,6 +168,11 @@ struct pxfdtd_resident {
     int kernel = 1;  // 0: LDS tiles (64 x ty, barrier per plane); 1: register march (62 x ty)
     int ty = 4, kz = 16;
     double last_ms = 0;
+    // register march: work units (strip x, strip y, first plane, end plane), those touching
+    // no CPML slab first, then those that do
+    Buf units;
+    int n_lean = 0, n_slab = 0;
+    bool units_dirty = true;
     // per-launch timing of the fused kernel, when asked for
     bool profile = false;
     std::vector<hipEvent_t> kev;
@@ -379,6 +385,9 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
                     co.insert(co.end(), p->cpml_a[a][sd][f], p->cpml_a[a][sd][f] + cnt[sd]);
                 }
         }
+        r->ncpml = (int)co.size() - 2 * (p->nx + p->ny + p->nz);
+        if (r->ncpml > 2048)
+            return px_fail(PXFDTD_ERR_UNSUPPORTED, "CPMLs this thick (%d coefficients) exceed the kernel's table", r->ncpml);
         TRY(upload_table<T>(r->coef, co.data(), co.size(), s));
     }
     // psi: the reference's slab order (src/fdtd.rs:838-866)
@@ -581,7 +590,7 @@ extern "C" int pxfdtd_resident_download(pxfdtd_resident *r, double *e, double *h
 // tile (the LDS kernel: the row over the tile's columns and the column over its rows; the
 // register march: the whole rectangle [i0 - 1, i0 + tw) x [j0 - 1, j0 + th)).
 static void tile_shape(const pxfdtd_resident *r, int &tw, int &th) {
-    tw = r->kernel == 0 ? TX : RW;
+    tw = r->kernel == 0 ? TX : RW;  // kernels 1 and 2 (the march, with and without prefetch)
     th = r->ty;
 }
 
@@ -607,7 +616,7 @@ static bool h_region_has(const pxfdtd_resident *r, int bx, int by, int i, int j)
         return false;
     };
     const int iw = std::min(i0 + tw, r->nx + (r->periodic[0] ? 1 : 0));
-    if (r->kernel == 1) return in_x(i0 - 1, iw) && in_y(j0 - 1, j0 + th);
+    if (r->kernel != 0) return in_x(i0 - 1, iw) && in_y(j0 - 1, j0 + th);
     const bool own = in_x(i0, i0 + tw) && in_y(j0, j0 + th);
     const bool row = in_x(i0, i0 + tw) && in_y(j0 - 1, j0);
     const bool col = in_x(i0 - 1, i0) && in_y(j0, j0 + th);
@@ -674,6 +683,48 @@ static int build_injection_lists(pxfdtd_resident *r) {
     return PXFDTD_OK;
 }
 
+// The register march's work: plane chunks of about kz planes whose edges fall at the CPML
+// slabs' along z (a chunk also reads the plane below it), crossed with the strips; a unit goes
+// to the slab kernel if any value it computes H or E on lies in a CPML slab.
+static int build_units(pxfdtd_resident *r) {
+    std::vector<int> cuts = {0};
+    const int lz = r->low[2], hz = r->high[2], nz = r->nz;
+    auto cut = [&](int z) {
+        if (z > cuts.back() && z < nz) cuts.push_back(z);
+    };
+    if (lz > 0) cut(lz + 1);
+    const int hi_start = hz > 0 ? nz - hz : nz;
+    for (int z = cuts.back() + r->kz; z < hi_start; z += r->kz) cut(z);
+    cut(hi_start);
+    cuts.push_back(nz);
+    const int tx = (r->nx + RW - 1) / RW, ty = (r->ny + r->ty - 1) / r->ty;
+    auto in_slab = [](int a, int b, int n, int lo, int hi) {  // [a, b) meets [0, lo) or [n-hi, n)
+        return (lo > 0 && a < lo) || (hi > 0 && b > n - hi);
+    };
+    std::vector<int4> lean, slab;
+    for (size_t c = 0; c + 1 < cuts.size(); c++) {
+        const int a = cuts[c], b = cuts[c + 1];
+        const bool zs = in_slab(a > 0 || r->periodic[2] ? a - 1 : a, b, nz, lz, hz) && !r->periodic[2];
+        for (int by = 0; by < ty; by++) {
+            const int j0 = by * r->ty;
+            const bool ys = !r->periodic[1] && in_slab(std::max(j0 - 1, 0), std::min(j0 + r->ty, r->ny), r->ny, r->low[1], r->high[1]);
+            for (int bx = 0; bx < tx; bx++) {
+                const int i0 = bx * RW;
+                const bool xs = !r->periodic[0] && in_slab(std::max(i0 - 1, 0), std::min(i0 + RW, r->nx), r->nx, r->low[0], r->high[0]);
+                const int4 u = make_int4(bx, by, a, b);
+                ((xs || ys || zs) ? slab : lean).push_back(u);
+            }
+        }
+    }
+    r->n_lean = (int)lean.size();
+    r->n_slab = (int)slab.size();
+    lean.insert(lean.end(), slab.begin(), slab.end());
+    TRY(r->units.alloc(std::max<size_t>(lean.size(), 1) * sizeof(int4)));
+    HIPCHK(hipMemcpy(r->units.p, lean.data(), lean.size() * sizeof(int4), hipMemcpyHostToDevice));
+    r->units_dirty = false;
+    return PXFDTD_OK;
+}
+
 extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_injection *inj,
                                               int64_t n, int32_t emitters) {
     for (int64_t q = 0; q < n; q++) {
@@ -691,7 +742,7 @@ extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_i
 extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes) {
     if (rows) {
         const bool ok = r->kernel == 0 ? (rows == 2 || rows == 4 || rows == 8)
-                                       : (rows == 1 || rows == 2 || rows == 4);
+                                       : (rows == 1 || rows == 2 || rows == 4);  // the march
         if (!ok)
             return px_fail(PXFDTD_ERR_INVALID, "rows per tile must be %s for this kernel, got %d",
                            r->kernel == 0 ? "2, 4 or 8" : "1, 2 or 4", rows);
@@ -701,14 +752,16 @@ extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int3
         if (planes < 1) return px_fail(PXFDTD_ERR_INVALID, "planes per chunk must be >= 1");
         r->kz = std::min(planes, r->nz);
     }
+    r->units_dirty = true;
     HIPCHK(hipSetDevice(r->dev->ordinal));
     return build_injection_lists(r);
 }
 
 extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
-    if (kind != 0 && kind != 1) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0 or 1");
+    if (kind < 0 || kind > 2) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0, 1 or 2");
     r->kernel = kind;
     r->ty = kind == 0 ? 8 : 2;
+    r->units_dirty = true;
     HIPCHK(hipSetDevice(r->dev->ordinal));
     return build_injection_lists(r);
 }
@@ -890,15 +943,19 @@ static StepArgs<T> args_of(const pxfdtd_resident *r) {
     if (slabs) A.flags |= 8;
     if (r->have_inj) A.flags |= 16;
     A.coef = r->coef.as<T>();
+    A.ncpml = r->ncpml;
     int tw, th;
     tile_shape(r, tw, th);
     A.tiles_x = (r->nx + tw - 1) / tw;
     A.tiles_y = (r->ny + th - 1) / th;
     A.kz = r->kz;
     A.blocks = A.tiles_x * A.tiles_y * ((r->nz + r->kz - 1) / r->kz);
-    // the LDS kernel takes a tile per workgroup, the march a strip per wavefront (4 per group)
-    const int groups = r->kernel == 0 ? A.blocks : (A.blocks + 3) / 4;
-    A.per_xcd = (groups + 7) / 8;
+    A.per_xcd = (A.blocks + 7) / 8;
+    if (r->kernel != 0) {  // the march's two unit lists: counts ride in blocks and per_xcd
+        A.units = r->units.as<int4>();
+        A.blocks = r->n_lean;
+        A.per_xcd = r->n_slab;
+    }
     A.dt = (T)r->dt;
     A.ids = r->ids.as<uint8_t>();
     A.table = r->table.as<T>();
@@ -943,9 +1000,22 @@ static int launch_ty(const StepArgs<T> &A, hipStream_t s, size_t lds) {
     return PXFDTD_OK;
 }
 
-template <typename T, int MODE, int RY>
-static int launch_march(const StepArgs<T> &A, hipStream_t s, size_t lds) {
-    hipLaunchKernelGGL((march_step<T, MODE, RY>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+template <typename T, int MODE, int RY, bool PF>
+static int launch_march(const StepArgs<T> &A0, hipStream_t s, size_t lds) {
+    // the units touching no slab, then those that do, each spread over the 8 XCDs
+    StepArgs<T> A = A0;
+    const int lean = A0.blocks, slab = A0.per_xcd;  // counts passed through blocks, per_xcd
+    if (lean) {
+        A.blocks = lean;
+        A.per_xcd = ((lean + 3) / 4 + 7) / 8;
+        hipLaunchKernelGGL((march_step<T, MODE, RY, false, PF>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+    }
+    if (slab) {
+        A.units = A0.units + lean;
+        A.blocks = slab;
+        A.per_xcd = ((slab + 3) / 4 + 7) / 8;
+        hipLaunchKernelGGL((march_step<T, MODE, RY, true, false>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+    }
     return PXFDTD_OK;
 }
 
@@ -953,9 +1023,16 @@ template <typename T, int MODE>
 static int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, size_t lds) {
     if (kernel == 1) {
         switch (ty) {
-        case 1: return launch_march<T, MODE, 1>(A, s, lds);
-        case 2: return launch_march<T, MODE, 2>(A, s, lds);
-        default: return launch_march<T, MODE, 4>(A, s, lds);
+        case 1: return launch_march<T, MODE, 1, true>(A, s, lds);
+        case 2: return launch_march<T, MODE, 2, true>(A, s, lds);
+        default: return launch_march<T, MODE, 4, true>(A, s, lds);
+        }
+    }
+    if (kernel == 2) {
+        switch (ty) {
+        case 1: return launch_march<T, MODE, 1, false>(A, s, lds);
+        case 2: return launch_march<T, MODE, 2, false>(A, s, lds);
+        default: return launch_march<T, MODE, 4, false>(A, s, lds);
         }
     }
     switch (ty) {
@@ -995,6 +1072,7 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
     }
     const size_t lds = r->mode == 0 ? (size_t)256 * (6 * sizeof(T) + 3) : 0;
     TRY(upload_cold<T>(r));
+    if (r->kernel != 0 && r->units_dirty) TRY(build_units(r));
     int in_block = 0;
     long long block_first = 0;
     if (r->profile)
