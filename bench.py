@@ -98,6 +98,26 @@ def cpu_baseline(nx, ny, nz, pml):
                       f"{nx}x{ny}x{nz} cells with a {pml}-cell CPML, best of 3 runs of {steps} steps"}
 
 
+def copy_rate(mib, steps, warmup):
+    """A plain copy of `mib` MiB between two device buffers: bytes read + written per second."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(warmup):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    return {"copy_mib": mib, "ms_per_copy": round(ms, 5), "bytes_per_copy": 2 * n * 4,
+            "gb_per_s": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,12 +130,18 @@ def main():
                     help="the kernel alone: no source, no monitors (for characterization)")
     ap.add_argument("--pml", type=int, default=12)
     ap.add_argument("--kernel", type=int, default=-1)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only time a plain device-to-device copy of this many MiB (the reference "
+                         "kernel for scaling the HBM byte counters and for the achievable copy rate)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: each box has one MI355X, and "
-                          "--gpus above 1 is not supported (the design shards a grid by z slabs; "
-                          "see DESIGN.md)"}))
+                          "--gpus above 1 is not supported (a grid would shard by z slabs with a "
+                          "one-plane halo exchange per half step)"}))
         sys.exit(2)
+    if args.copy_mib:
+        print(json.dumps(copy_rate(args.copy_mib, args.steps, args.warmup)), flush=True)
+        return
     import photonoxide_amd as P
     from photonoxide_amd import _native as N
 
@@ -159,12 +185,12 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else ""),
-                   "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch"][s.kernel()], "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
+                   "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch", "aligned_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
                    "monitors": "2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums",
                    "psi_values": psi},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": ["fused_step", "march_step", "march_step"][s.kernel()], "kernel_ms": round(kernel_ms, 5),
+                     "kernel": ["fused_step", "march_step", "march_step", "march64_step"][s.kernel()], "kernel_ms": round(kernel_ms, 5),
                      "bytes_per_launch": bytes_step,
                      "gcell_per_s_kernel": round(cells / (kernel_ms * 1e-3) / 1e9, 3)},
     }

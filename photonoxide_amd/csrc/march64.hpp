@@ -1,0 +1,483 @@
+// The aligned register march: the fused Yee + CPML step the time-stepper runs by default.
+//
+// Each wavefront owns a strip of 64 values in x (lane L holds x = i0 + L with i0 a multiple
+// of 64, so each row it loads or stores is two whole aligned 128-byte lines) by RY rows in y,
+// and marches along z over a chunk of planes. At each plane it computes H's new values on the
+// strip and on the row below it (the row's H is what E's back difference along y reads, and
+// is recomputed here rather than exchanged), then E's new values on the strip. Neighbours
+// along x move between lanes by DPP wave shifts; the two columns just outside the strip come
+// from gathers, a handful of lanes loading one value each:
+//   - E at x = i0 + 64 (Ey, Ez on the rows H is computed on): lane 63's i + 1 neighbour;
+//   - old H and E at x = i0 - 1: lane m < RY recomputes Hy, Hz at (i0 - 1, j0 + m), lane 0's
+//     i - 1 neighbour for E.
+// Along z each wave keeps the plane below's new H in registers. Only the strip's own values
+// are stored (through buffer stores whose offset is out of range elsewhere, so no lane
+// branches around them), and every value is computed with the same operations in the same
+// order as the reference's loops (src/fdtd/kernel/reference.rs:14-180), so any tiling gives
+// the same bits.
+//
+// CPML: a strip and chunk that meets a slab runs the SLABS variant, whose psi work sits behind
+// branches that are uniform over the wave: per plane for the z slabs, per row for the y slabs,
+// and for the x slabs per lane, fixed for the unit.
+#pragma once
+#include "kernels.hpp"
+
+namespace pxfdtd {
+
+constexpr int SW = 64;  // strip width: one value per lane
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp(float old, float v) {
+    return __builtin_bit_cast(float, dpp_i<CTRL>(__builtin_bit_cast(int, old), __builtin_bit_cast(int, v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double old, double v) {
+    const unsigned long long o = __builtin_bit_cast(unsigned long long, old);
+    const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)dpp_i<CTRL>((int)(unsigned)o, (int)(unsigned)x);
+    const unsigned hi = (unsigned)dpp_i<CTRL>((int)(unsigned)(o >> 32), (int)(unsigned)(x >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// lane L takes lane L + 1's v, lane 63 takes `fill` (DPP wave_shl:1, no write from beyond)
+template <typename T>
+__device__ __forceinline__ T next_lane(T v, T fill) { return dpp<0x130>(fill, v); }
+// lane L takes lane L - 1's v, lane 0 takes `fill` (DPP wave_shr:1)
+template <typename T>
+__device__ __forceinline__ T prev_lane(T v, T fill) { return dpp<0x138>(fill, v); }
+
+// lane l's value, the same for every lane
+__device__ __forceinline__ float lane_value(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const unsigned long long x = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename P>
+__device__ __forceinline__ P *uniform_ptr(P *p) {
+    const unsigned long long x = (unsigned long long)p;
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)x);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(x >> 32));
+    return (P *)(((unsigned long long)hi << 32) | lo);
+}
+
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ void bstore<float>(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+template <>
+__device__ __forceinline__ void bstore<double>(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+}
+
+// m on the grid: wrapped across a periodic axis (any distance), -1 beyond a wall
+__device__ __forceinline__ int wrap_any(int m, int n, int per) {
+    if (m >= 0 && m < n) return m;
+    if (!per) return -1;
+    const int q = m % n;
+    return q < 0 ? q + n : q;
+}
+
+template <typename T, int MODE, int RY, bool SLABS>
+__global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
+    constexpr int NR = RY + 2;  // rows of E held: j0 - 1 .. j0 + RY
+    constexpr int CPML_MAX = 2048;
+    __shared__ StepCold<T> K;
+    __shared__ T cp[SLABS ? CPML_MAX : 1];  // the CPML's b and a
+    extern __shared__ unsigned char smem_raw[];
+    T *tab = reinterpret_cast<T *>(smem_raw);                 // [256][3][2]
+    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);  // [256][3]
+    const int tid = threadIdx.x;
+    if (MODE == 0) {
+        for (int q = tid; q < 256 * 6; q += 256) tab[q] = A.table[q];
+        const int8_t *g = reinterpret_cast<const int8_t *>(A.table + 256 * 6);
+        for (int q = tid; q < 256 * 3; q += 256) grp[q] = g[q];
+    }
+    {
+        const int *src = reinterpret_cast<const int *>(A.cold);
+        int *dst = reinterpret_cast<int *>(&K);
+        for (int q = tid; q < (int)(sizeof(StepCold<T>) / 4); q += 256) dst[q] = src[q];
+    }
+    const int cbase = 2 * (A.nx + A.ny + A.nz);
+    if (SLABS)
+        for (int q = tid; q < A.ncpml && q < CPML_MAX; q += 256) cp[q] = A.coef[cbase + q];
+    __syncthreads();  // the only barrier: the parameters in LDS
+
+    const int lane = tid & 63;
+    // workgroups are dealt round-robin to the 8 XCDs: each XCD takes a contiguous run of
+    // units, so neighbouring strips (which read each other's edges) share an L2
+    const int b = (int)(blockIdx.x % 8) * A.per_xcd + (int)(blockIdx.x / 8);
+    const int w = uniform(b * 4 + (tid >> 6));
+    if (w >= A.blocks) return;
+    const int4 unit = A.units[w];  // strip x, strip y, first plane, end plane
+    const int bx = unit.x, by = unit.y, kz0 = unit.z, kz1 = unit.w;
+    const int t = by * A.tiles_x + bx;
+    const int i0 = bx * SW, j0 = by * RY;
+    const int nx = A.nx, ny = A.ny, nz = A.nz, pitch = A.pitch;
+    const int perx = A.flags & 1, pery = (A.flags >> 1) & 1, perz = (A.flags >> 2) & 1;
+    bool injects = false;
+    if (A.flags & 16) {
+        const int32_t *io = K.inj_off + (long long)t * (nz + 1);
+        injects = io[nz] > io[0];
+    }
+
+    // this lane's column, and the columns left and right of the strip (-1: beyond a wall)
+    const int i = i0 + lane;
+    const bool own_x = i < nx;
+    const int xi = own_x ? i : (i == nx && perx ? 0 : -1);
+    const int xl = wrap_any(i0 - 1, nx, perx);
+    const int xr = i0 + SW < nx ? i0 + SW : (i0 + SW == nx && perx ? 0 : -1);
+    int yj[NR];
+    uint32_t vo[NR], so[NR];  // byte offsets in a plane: loads (any cell) and stores (own)
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int jr = j0 - 1 + r;
+        yj[r] = wrap_any(jr, ny, pery);
+        vo[r] = (xi < 0 || yj[r] < 0) ? OUTSIDE : (uint32_t)(yj[r] * pitch + xi) * sizeof(T);
+        so[r] = (own_x && jr >= 0 && jr < ny) ? (uint32_t)(jr * pitch + i) * sizeof(T) : OUTSIDE;
+    }
+    // the gathers: lane m (< 32) takes (i0 - 1, j0 + m), lane 32 + m takes (i0 + 64, j0 - 1 + m)
+    const int m = lane & 31;
+    const int gyl = wrap_any(j0 + m, ny, pery), gyr = wrap_any(j0 - 1 + m, ny, pery);
+    const bool lok = lane < 32 && m <= RY && xl >= 0 && gyl >= 0;
+    const bool rok = lane >= 32 && m <= RY && xr >= 0 && gyr >= 0;
+    const uint32_t lof = lok ? (uint32_t)(gyl * pitch + xl) * sizeof(T) : OUTSIDE;
+    const uint32_t go_x = lof;                                        // Ex, m = 0..RY
+    const uint32_t go_yz = (lok && m < RY) ? lof
+                           : (rok ? (uint32_t)(gyr * pitch + xr) * sizeof(T) : OUTSIDE);
+    const uint32_t go_h = (lok && m < RY) ? lof : OUTSIDE;           // old Hy, Hz
+    const bool hok = lane < RY && xl >= 0 && gyl >= 0 && j0 + m < ny;  // a halo H this lane makes
+
+    const uint32_t cbytes = (uint32_t)A.nval * sizeof(T), pbytes = (uint32_t)A.plane * sizeof(T);
+    const __amdgpu_buffer_rsrc_t rE = rsrc(A.ein, 3 * cbytes), rH = rsrc(A.hin, 3 * cbytes),
+                                 wE = rsrc(A.eout, 3 * cbytes), wH = rsrc(A.hout, 3 * cbytes),
+                                 rid = rsrc(A.ids, MODE == 0 ? (uint32_t)A.nval : 0u),
+                                 rnone = rsrc(A.ein, 0u);
+    const T dt = A.dt;
+    const T stx = A.step[0], sty = A.step[1], stz = A.step[2];
+    const T *coef = A.coef;
+    const int fex = 0, fhx = nx, fey = 2 * nx, fhy = 2 * nx + ny, fez = 2 * (nx + ny),
+              fhz = 2 * (nx + ny) + nz;
+    const T fxh = coef[fhx + max(xi, 0)], fxe = coef[fex + max(xi, 0)];
+    const T fxh_l = coef[fhx + max(xl, 0)], fyh_l = coef[fhy + max(gyl, 0)];
+
+    // CPML geometry fixed for the unit: the x slab per lane (and at the column left of the
+    // strip), the y slab per row
+    int lx = 0, hx = 0, ly = 0, hy = 0, lz = 0, hz = 0;
+    int xs = -1, xcnt = 0, xsl = -1, xlcnt = 0, xls = 0;
+    T bxh = 0, axh = 0, bxe = 0, axe = 0, bxl = 0, axl = 0;
+    int ys[NR], yl[NR];
+    bool xany = false;
+    const T *psi_in = nullptr;
+    T *psi_out = nullptr;
+    if (SLABS) {
+        lx = uniform(K.low[0]); hx = uniform(K.high[0]);
+        ly = uniform(K.low[1]); hy = uniform(K.high[1]);
+        lz = uniform(K.low[2]); hz = uniform(K.high[2]);
+        psi_in = uniform_ptr(K.psi_in);
+        psi_out = uniform_ptr(K.psi_out);
+        xs = own_x ? side_of(xi, nx, lx, hx) : -1;
+        xcnt = xs == 1 ? hx : lx;
+        const int xsx = xi - (xs == 1 ? nx - hx : 0);
+        if (xs >= 0) {
+            const int ch = K.cpml_off[0][xs][1] - cbase, ce = K.cpml_off[0][xs][0] - cbase;
+            bxh = cp[ch + xsx]; axh = cp[ch + xcnt + xsx];
+            bxe = cp[ce + xsx]; axe = cp[ce + xcnt + xsx];
+        }
+        xany = (lx > 0 && i0 < lx) || (hx > 0 && min(i0 + SW, nx) > nx - hx);
+        xsl = xl >= 0 ? side_of(xl, nx, lx, hx) : -1;
+        xlcnt = xsl == 1 ? hx : lx;
+        xls = xl - (xsl == 1 ? nx - hx : 0);
+        if (xsl >= 0) {
+            const int ch = K.cpml_off[0][xsl][1] - cbase;
+            bxl = cp[ch + xls]; axl = cp[ch + xlcnt + xls];
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            ys[r] = yj[r] >= 0 ? side_of(yj[r], ny, ly, hy) : -1;
+            yl[r] = yj[r] - (ys[r] == 1 ? ny - hy : 0);
+        }
+    }
+    // psi offsets: x slab (k * ny + j) * count + s, y slab (k * count + s) * nx + i,
+    // z slab (s * ny + j) * nx + i, after the slab's base
+    auto xoff = [&](int F, int c, int k, int j) {
+        return K.psi_off[F][c][0][xs] + (k * ny + j) * xcnt + (xi - (xs == 1 ? nx - hx : 0));
+    };
+
+    auto zwrap = [&](int kk) { return wrap_any(kk, nz, perz); };
+    const bool has_pro = kz0 > 0 || perz;
+    const int kfirst = has_pro ? kz0 - 1 : kz0;
+
+    struct EPlane {
+        T e[NR][3];
+        T gx, gy, gz;  // gathered: left column Ex (m = 0..RY), Ey, Ez (m < RY); right Ey, Ez
+    };
+    auto load_e = [&](EPlane &P, int kk) {
+        const int kp = zwrap(kk);
+        const __amdgpu_buffer_rsrc_t rr = kp >= 0 ? rE : rnone;
+        const uint32_t s = kp >= 0 ? (uint32_t)kp * pbytes : 0u;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            P.e[r][0] = bload<T>(rr, vo[r], s);
+            P.e[r][1] = bload<T>(rr, vo[r], s + cbytes);
+            P.e[r][2] = bload<T>(rr, vo[r], s + 2 * cbytes);
+        }
+        P.gx = bload<T>(rr, go_x, s);
+        P.gy = bload<T>(rr, go_yz, s + cbytes);
+        P.gz = bload<T>(rr, go_yz, s + 2 * cbytes);
+    };
+
+    EPlane cur;
+    load_e(cur, kfirst);
+    T hp[RY][2];  // new Hx, Hy of the plane below, on the strip's rows
+#pragma unroll
+    for (int r = 0; r < RY; r++) hp[r][0] = hp[r][1] = T(0);
+
+    for (int kk = kfirst; kk < kz1; kk++) {
+        const bool pro = kk < kz0;
+        const int k = zwrap(kk);
+        const uint32_t sk = (uint32_t)k * pbytes;
+        EPlane nxt;
+        load_e(nxt, kk + 1);
+        T ho[RY + 1][3];
+#pragma unroll
+        for (int r = 0; r <= RY; r++) {
+            ho[r][0] = bload<T>(rH, vo[r], sk);
+            ho[r][1] = r == 0 ? T(0) : bload<T>(rH, vo[r], sk + cbytes);
+            ho[r][2] = bload<T>(rH, vo[r], sk + 2 * cbytes);
+        }
+        const T ghy = bload<T>(rH, go_h, sk + cbytes), ghz = bload<T>(rH, go_h, sk + 2 * cbytes);
+        uint8_t id[RY];
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            if (MODE == 0) {
+                const uint32_t o = so[r + 1] == OUTSIDE ? OUTSIDE : so[r + 1] / (uint32_t)sizeof(T);
+                id[r] = __builtin_amdgcn_raw_buffer_load_b8(rid, (int)o, (int)((uint32_t)k * (uint32_t)A.plane), 0);
+            } else {
+                id[r] = 0;
+            }
+        }
+        const T fzh = coef[fhz + k], fze = coef[fez + k];
+        int zs = -1, zsl = 0;
+        if (SLABS) {
+            zs = side_of(k, nz, lz, hz);
+            zsl = k - (zs == 1 ? nz - hz : 0);
+        }
+
+        // new H on rows j0 - 1 .. j0 + RY - 1 (the first only Hx, Hz, and not for the prologue)
+        T hn[RY + 1][3];
+#pragma unroll
+        for (int r = 0; r <= RY; r++) {
+            const T ey_n = next_lane(cur.e[r][1], lane_value(cur.gy, 32 + r));
+            const T ez_n = next_lane(cur.e[r][2], lane_value(cur.gz, 32 + r));
+            const bool ok = xi >= 0 && yj[r] >= 0;
+            const T fyh = coef[fhy + max(yj[r], 0)];
+            const T dyx = cur.e[r + 1][2] - cur.e[r][2], dzx = nxt.e[r][1] - cur.e[r][1];  // Hx
+            const T dzy = nxt.e[r][0] - cur.e[r][0], dxy = ez_n - cur.e[r][2];            // Hy
+            const T dxz = ey_n - cur.e[r][1], dyz = cur.e[r + 1][0] - cur.e[r][0];        // Hz
+            T v0 = ho[r][0] - dt * (dyx * fyh - dzx * fzh);
+            T v1 = ho[r][1] - dt * (dzy * fzh - dxy * fxh);
+            T v2 = ho[r][2] - dt * (dxz * fxh - dyz * fyh);
+            if (SLABS) {
+                // psi values exist on the grid's own cells; those of the strip's rows are
+                // written, the row below's only read
+                const bool cell = own_x && yj[r] >= 0 && !(pro && r == 0);
+                const bool wr = !pro && r > 0 && so[r] != OUTSIDE;
+                if (xany && cell && xs >= 0) {
+                    if (r > 0) {
+                        const int o = xoff(1, 1, k, yj[r]);
+                        const T u = bxh * psi_in[o] + axh * (dxy / stx);
+                        if (wr) psi_out[o] = u;
+                        v1 = v1 - dt * (T(-1) * u);
+                    }
+                    const int o = xoff(1, 2, k, yj[r]);
+                    const T u = bxh * psi_in[o] + axh * (dxz / stx);
+                    if (wr) psi_out[o] = u;
+                    v2 = v2 - dt * (T(1) * u);
+                }
+                if (ys[r] >= 0 && cell) {
+                    const int sd = ys[r], cnt = sd == 1 ? hy : ly;
+                    const int co = K.cpml_off[1][sd][1] - cbase;
+                    const T bb = cp[co + yl[r]], aa = cp[co + cnt + yl[r]];
+                    const int q = (k * cnt + yl[r]) * nx + xi;
+                    const int o0 = K.psi_off[1][0][1][sd] + q, o2 = K.psi_off[1][2][1][sd] + q;
+                    const T u0 = bb * psi_in[o0] + aa * (dyx / sty);
+                    const T u2 = bb * psi_in[o2] + aa * (dyz / sty);
+                    if (wr) {
+                        psi_out[o0] = u0;
+                        psi_out[o2] = u2;
+                    }
+                    v0 = v0 - dt * (T(1) * u0);
+                    v2 = v2 - dt * (T(-1) * u2);
+                }
+                if (zs >= 0 && cell) {
+                    const int cnt = zs == 1 ? hz : lz;
+                    const int co = K.cpml_off[2][zs][1] - cbase;
+                    const T bb = cp[co + zsl], aa = cp[co + cnt + zsl];
+                    const int q = (zsl * ny + yj[r]) * nx + xi;
+                    const int o0 = K.psi_off[1][0][2][zs] + q;
+                    const T u0 = bb * psi_in[o0] + aa * (dzx / stz);
+                    if (wr) psi_out[o0] = u0;
+                    v0 = v0 - dt * (T(-1) * u0);
+                    if (r > 0) {
+                        const int o1 = K.psi_off[1][1][2][zs] + q;
+                        const T u1 = bb * psi_in[o1] + aa * (dzy / stz);
+                        if (wr) psi_out[o1] = u1;
+                        v1 = v1 - dt * (T(1) * u1);
+                    }
+                }
+            }
+            if (injects && ok && !(pro && r == 0))
+                inject<T, 1>(A, K, t, xi, yj[r], k, r == 0 ? 5 : 7, v0, v1, v2, T(0), T(0), T(0));
+            hn[r][0] = ok ? v0 : T(0);
+            hn[r][1] = ok ? v1 : T(0);
+            hn[r][2] = ok ? v2 : T(0);
+        }
+
+        if (!pro) {
+            // Hy, Hz at (i0 - 1, j0 + m) on lane m < RY, from the gathered values and lane 0's
+            // own Ey, Ez of that row
+            T ey0 = T(0), ez0 = T(0);
+#pragma unroll
+            for (int q = 0; q < RY; q++) {
+                const T a = lane_value(cur.e[q + 1][1], 0), c = lane_value(cur.e[q + 1][2], 0);
+                ey0 = m == q ? a : ey0;
+                ez0 = m == q ? c : ez0;
+            }
+            const T exn = next_lane(cur.gx, T(0));  // Ex at (i0 - 1, j0 + m + 1)
+            const T dzy = nxt.gx - cur.gx, dxy = ez0 - cur.gz;
+            const T dxz = ey0 - cur.gy, dyz = exn - cur.gx;
+            T v1 = ghy - dt * (dzy * fzh - dxy * fxh_l);
+            T v2 = ghz - dt * (dxz * fxh_l - dyz * fyh_l);
+            if (SLABS && hok) {
+                if (xsl >= 0) {
+                    const int base = (k * ny + gyl) * xlcnt + xls;
+                    const T u1 = bxl * psi_in[K.psi_off[1][1][0][xsl] + base] + axl * (dxy / stx);
+                    const T u2 = bxl * psi_in[K.psi_off[1][2][0][xsl] + base] + axl * (dxz / stx);
+                    v1 = v1 - dt * (T(-1) * u1);
+                    v2 = v2 - dt * (T(1) * u2);
+                }
+                const int sd = side_of(gyl, ny, ly, hy);
+                if (sd >= 0) {
+                    const int cnt = sd == 1 ? hy : ly, s = gyl - (sd == 1 ? ny - hy : 0);
+                    const int co = K.cpml_off[1][sd][1] - cbase;
+                    const T u2 = cp[co + s] * psi_in[K.psi_off[1][2][1][sd] + (k * cnt + s) * nx + xl] +
+                                 cp[co + cnt + s] * (dyz / sty);
+                    v2 = v2 - dt * (T(-1) * u2);
+                }
+                if (zs >= 0) {
+                    const int cnt = zs == 1 ? hz : lz;
+                    const int co = K.cpml_off[2][zs][1] - cbase;
+                    const T u1 = cp[co + zsl] * psi_in[K.psi_off[1][1][2][zs] + (zsl * ny + gyl) * nx + xl] +
+                                 cp[co + cnt + zsl] * (dzy / stz);
+                    v1 = v1 - dt * (T(1) * u1);
+                }
+            }
+            T v0 = T(0);
+            if (injects && hok) inject<T, 1>(A, K, t, xl, gyl, k, 6, v0, v1, v2, T(0), T(0), T(0));
+            const T hy_h = hok ? v1 : T(0), hz_h = hok ? v2 : T(0);
+
+            // new E on the strip
+#pragma unroll
+            for (int r = 1; r <= RY; r++) {
+                const T hy_p = prev_lane(hn[r][1], lane_value(hy_h, r - 1));
+                const T hz_p = prev_lane(hn[r][2], lane_value(hz_h, r - 1));
+                const uint32_t st = so[r];
+                const bool own = st != OUTSIDE;
+                bstore<T>(hn[r][0], wH, st, sk);
+                bstore<T>(hn[r][1], wH, st, sk + cbytes);
+                bstore<T>(hn[r][2], wH, st, sk + 2 * cbytes);
+                const int j = yj[r];
+                const T dyx = hn[r][2] - hn[r - 1][2], dzx = hn[r][1] - hp[r - 1][1];  // Ex
+                const T dzy2 = hn[r][0] - hp[r - 1][0], dxy2 = hn[r][2] - hz_p;        // Ey
+                const T dxz2 = hn[r][1] - hy_p, dyz2 = hn[r][0] - hn[r - 1][0];        // Ez
+                T ca0, ca1, ca2, cb0, cb1, cb2;
+                int g0, g1, g2;
+                const long long o = (long long)k * A.plane + (long long)(st / (uint32_t)sizeof(T));
+                if (MODE == 0) {
+                    const int q = id[r - 1];
+                    const T *e = tab + 6 * q;
+                    ca0 = e[0]; cb0 = e[1]; ca1 = e[2]; cb1 = e[3]; ca2 = e[4]; cb2 = e[5];
+                    g0 = grp[3 * q]; g1 = grp[3 * q + 1]; g2 = grp[3 * q + 2];
+                } else {
+                    const long long nv = A.nval;
+                    ca0 = own ? A.cacb[o] : T(0);
+                    ca1 = own ? A.cacb[nv + o] : T(0);
+                    ca2 = own ? A.cacb[2 * nv + o] : T(0);
+                    cb0 = own ? A.cacb[3 * nv + o] : T(0);
+                    cb1 = own ? A.cacb[4 * nv + o] : T(0);
+                    cb2 = own ? A.cacb[5 * nv + o] : T(0);
+                    g0 = own && A.grp ? A.grp[o] : -1;
+                    g1 = own && A.grp ? A.grp[nv + o] : -1;
+                    g2 = own && A.grp ? A.grp[2 * nv + o] : -1;
+                }
+                const T fye = coef[fey + max(j, 0)];
+                T v0 = ca0 * cur.e[r][0] + cb0 * (dyx * fye - dzx * fze);
+                T w1 = ca1 * cur.e[r][1] + cb1 * (dzy2 * fze - dxy2 * fxe);
+                T w2 = ca2 * cur.e[r][2] + cb2 * (dxz2 * fxe - dyz2 * fye);
+                if (SLABS && own) {
+                    if (xany && xs >= 0) {
+                        const int o1 = xoff(0, 1, k, j), o2 = xoff(0, 2, k, j);
+                        const T u1 = bxe * psi_in[o1] + axe * (dxy2 / stx);
+                        const T u2 = bxe * psi_in[o2] + axe * (dxz2 / stx);
+                        psi_out[o1] = u1;
+                        psi_out[o2] = u2;
+                        w1 = w1 + cb1 * (T(-1) * u1);
+                        w2 = w2 + cb2 * (T(1) * u2);
+                    }
+                    if (ys[r] >= 0) {
+                        const int sd = ys[r], cnt = sd == 1 ? hy : ly;
+                        const int co = K.cpml_off[1][sd][0] - cbase;
+                        const T bb = cp[co + yl[r]], aa = cp[co + cnt + yl[r]];
+                        const int q = (k * cnt + yl[r]) * nx + xi;
+                        const int o0 = K.psi_off[0][0][1][sd] + q, o2 = K.psi_off[0][2][1][sd] + q;
+                        const T u0 = bb * psi_in[o0] + aa * (dyx / sty);
+                        const T u2 = bb * psi_in[o2] + aa * (dyz2 / sty);
+                        psi_out[o0] = u0;
+                        psi_out[o2] = u2;
+                        v0 = v0 + cb0 * (T(1) * u0);
+                        w2 = w2 + cb2 * (T(-1) * u2);
+                    }
+                    if (zs >= 0) {
+                        const int cnt = zs == 1 ? hz : lz;
+                        const int co = K.cpml_off[2][zs][0] - cbase;
+                        const T bb = cp[co + zsl], aa = cp[co + cnt + zsl];
+                        const int q = (zsl * ny + j) * nx + xi;
+                        const int o0 = K.psi_off[0][0][2][zs] + q, o1 = K.psi_off[0][1][2][zs] + q;
+                        const T u0 = bb * psi_in[o0] + aa * (dzx / stz);
+                        const T u1 = bb * psi_in[o1] + aa * (dzy2 / stz);
+                        psi_out[o0] = u0;
+                        psi_out[o1] = u1;
+                        v0 = v0 + cb0 * (T(-1) * u0);
+                        w1 = w1 + cb1 * (T(1) * u1);
+                    }
+                }
+                if (injects && own) inject<T, 0>(A, K, t, xi, j, k, 7, v0, w1, w2, cb0, cb1, cb2);
+                if (own && g0 >= 0) v0 = media_update(A, K, g0, 0, (int)o, v0, cb0, cur.e[r][0]);
+                if (own && g1 >= 0) w1 = media_update(A, K, g1, 1, (int)o, w1, cb1, cur.e[r][1]);
+                if (own && g2 >= 0) w2 = media_update(A, K, g2, 2, (int)o, w2, cb2, cur.e[r][2]);
+                bstore<T>(v0, wE, st, sk);
+                bstore<T>(w1, wE, st, sk + cbytes);
+                bstore<T>(w2, wE, st, sk + 2 * cbytes);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RY; r++) {
+            hp[r][0] = hn[r + 1][0];
+            hp[r][1] = hn[r + 1][1];
+        }
+        cur = nxt;
+    }
+}
+
+}  // namespace pxfdtd

@@ -17,6 +17,7 @@
 
 #include "../../include/pxfdtd.h"
 #include "kernels.hpp"
+#include "march64.hpp"
 
 using namespace pxfdtd;
 
@@ -590,7 +591,8 @@ extern "C" int pxfdtd_resident_download(pxfdtd_resident *r, double *e, double *h
 // tile (the LDS kernel: the row over the tile's columns and the column over its rows; the
 // register march: the whole rectangle [i0 - 1, i0 + tw) x [j0 - 1, j0 + th)).
 static void tile_shape(const pxfdtd_resident *r, int &tw, int &th) {
-    tw = r->kernel == 0 ? TX : RW;  // kernels 1 and 2 (the march, with and without prefetch)
+    // kernels 1 and 2: the 62-wide march (with and without prefetch); 3: the aligned march
+    tw = r->kernel == 0 ? TX : (r->kernel == 3 ? SW : RW);
     th = r->ty;
 }
 
@@ -616,7 +618,7 @@ static bool h_region_has(const pxfdtd_resident *r, int bx, int by, int i, int j)
         return false;
     };
     const int iw = std::min(i0 + tw, r->nx + (r->periodic[0] ? 1 : 0));
-    if (r->kernel != 0) return in_x(i0 - 1, iw) && in_y(j0 - 1, j0 + th);
+    if (r->kernel == 1 || r->kernel == 2) return in_x(i0 - 1, iw) && in_y(j0 - 1, j0 + th);
     const bool own = in_x(i0, i0 + tw) && in_y(j0, j0 + th);
     const bool row = in_x(i0, i0 + tw) && in_y(j0 - 1, j0);
     const bool col = in_x(i0 - 1, i0) && in_y(j0, j0 + th);
@@ -697,7 +699,9 @@ static int build_units(pxfdtd_resident *r) {
     for (int z = cuts.back() + r->kz; z < hi_start; z += r->kz) cut(z);
     cut(hi_start);
     cuts.push_back(nz);
-    const int tx = (r->nx + RW - 1) / RW, ty = (r->ny + r->ty - 1) / r->ty;
+    int tw, th;
+    tile_shape(r, tw, th);
+    const int tx = (r->nx + tw - 1) / tw, ty = (r->ny + th - 1) / th;
     auto in_slab = [](int a, int b, int n, int lo, int hi) {  // [a, b) meets [0, lo) or [n-hi, n)
         return (lo > 0 && a < lo) || (hi > 0 && b > n - hi);
     };
@@ -709,8 +713,8 @@ static int build_units(pxfdtd_resident *r) {
             const int j0 = by * r->ty;
             const bool ys = !r->periodic[1] && in_slab(std::max(j0 - 1, 0), std::min(j0 + r->ty, r->ny), r->ny, r->low[1], r->high[1]);
             for (int bx = 0; bx < tx; bx++) {
-                const int i0 = bx * RW;
-                const bool xs = !r->periodic[0] && in_slab(std::max(i0 - 1, 0), std::min(i0 + RW, r->nx), r->nx, r->low[0], r->high[0]);
+                const int i0 = bx * tw;
+                const bool xs = !r->periodic[0] && in_slab(std::max(i0 - 1, 0), std::min(i0 + tw, r->nx), r->nx, r->low[0], r->high[0]);
                 const int4 u = make_int4(bx, by, a, b);
                 ((xs || ys || zs) ? slab : lean).push_back(u);
             }
@@ -758,7 +762,9 @@ extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int3
 }
 
 extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
-    if (kind < 0 || kind > 2) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0, 1 or 2");
+    if (kind < 0 || kind > 3) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0, 1, 2 or 3");
+    if (kind == 3 && 3 * r->nval * (long long)r->esz >= (1LL << 31))
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the aligned march needs a field's three components within 2 GiB");
     r->kernel = kind;
     r->ty = kind == 0 ? 8 : 2;
     r->units_dirty = true;
@@ -1019,8 +1025,33 @@ static int launch_march(const StepArgs<T> &A0, hipStream_t s, size_t lds) {
     return PXFDTD_OK;
 }
 
+template <typename T, int MODE, int RY>
+static int launch_march64(const StepArgs<T> &A0, hipStream_t s, size_t lds) {
+    StepArgs<T> A = A0;
+    const int lean = A0.blocks, slab = A0.per_xcd;  // counts passed through blocks, per_xcd
+    if (lean) {
+        A.blocks = lean;
+        A.per_xcd = ((lean + 3) / 4 + 7) / 8;
+        hipLaunchKernelGGL((march64_step<T, MODE, RY, false>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+    }
+    if (slab) {
+        A.units = A0.units + lean;
+        A.blocks = slab;
+        A.per_xcd = ((slab + 3) / 4 + 7) / 8;
+        hipLaunchKernelGGL((march64_step<T, MODE, RY, true>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+    }
+    return PXFDTD_OK;
+}
+
 template <typename T, int MODE>
 static int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, size_t lds) {
+    if (kernel == 3) {
+        switch (ty) {
+        case 1: return launch_march64<T, MODE, 1>(A, s, lds);
+        case 2: return launch_march64<T, MODE, 2>(A, s, lds);
+        default: return launch_march64<T, MODE, 4>(A, s, lds);
+        }
+    }
     if (kernel == 1) {
         switch (ty) {
         case 1: return launch_march<T, MODE, 1, true>(A, s, lds);

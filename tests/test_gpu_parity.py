@@ -59,7 +59,7 @@ def same_bits(a, b):
                           np.asarray(b, dtype=np.float64).view(np.uint64))
 
 
-@pytest.mark.parametrize("kernel", [1, 2, 0])
+@pytest.mark.parametrize("kernel", [3, 1, 2, 0])
 @pytest.mark.parametrize("shape", ["3d", "2d", "periodic"])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_device_is_the_oracle_bit_for_bit(gpu, shape, prec, kernel):
@@ -110,7 +110,8 @@ def test_f32_within_north_star_tolerance_of_f64_oracle(gpu):
 
 
 @pytest.mark.parametrize("kernel,rows,planes", [(0, 2, 1), (0, 2, 3), (0, 8, 19), (0, 4, 5),
-                                                (1, 1, 1), (1, 2, 3), (1, 4, 19), (2, 1, 7), (2, 2, 2)])
+                                                (1, 1, 1), (1, 2, 3), (1, 4, 19), (2, 1, 7), (2, 2, 2),
+                                                (3, 1, 1), (3, 2, 3), (3, 4, 19), (3, 1, 7), (3, 2, 2)])
 def test_tiling_changes_no_bit(gpu, kernel, rows, planes):
     import photonoxide_amd as P
     spec = busy_spec(False)
