@@ -11,9 +11,12 @@ the geometry is grid-aligned, and the kernel's work does not depend on it).
 
 A step is one pass of the whole path over the grid: the fused Yee + CPML kernel, the monitors'
 taps and their FP64 accumulation. `value` is cells x steps / wall time of K steps with the fields
-resident in HBM. The roofline entry is for the fused kernel: algorithmic bytes per launch
-(E and H read and written, 48 B, plus 3 B of coefficient ids, plus 8 B per CPML psi value) over
-its average duration from HIP events on its stream.
+resident in HBM. The roofline entry is for the fused step (the march kernel's launches of one step): algorithmic
+bytes per step (E and H read and written, 48 B per cell, plus the coefficients read: 1 B per cell
+for a uint8 material id, 24 B for full ca, cb arrays, plus 8 B per CPML psi value, read and
+written) over the step's kernel time, from HIP events on the resident's stream around the
+launches. `traffic` is the HBM bytes per step that rocprofv3's FETCH_SIZE and WRITE_SIZE counted
+(profiles/traffic_march64_step_mmi_f32.json).
 """
 import argparse
 import json
@@ -78,7 +81,7 @@ def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12, plain=False):
     return s, g
 
 
-def algorithmic_bytes_per_step(nx, ny, nz, pml, coef_bytes=3):
+def algorithmic_bytes_per_step(nx, ny, nz, pml, coef_bytes=1):
     n = nx * ny * nz
     psi = 0
     for axis_n in (nx, ny, nz):
@@ -130,6 +133,9 @@ def main():
                     help="the kernel alone: no source, no monitors (for characterization)")
     ap.add_argument("--pml", type=int, default=12)
     ap.add_argument("--kernel", type=int, default=-1)
+    ap.add_argument("--launch", type=int, default=-1,
+                    help="the aligned march's launch mode: 0 three launches, 1 the slab one on a "
+                         "second stream, 2 one launch")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only time a plain device-to-device copy of this many MiB (the reference "
                          "kernel for scaling the HBM byte counters and for the achievable copy rate)")
@@ -154,6 +160,8 @@ def main():
     if args.rows or args.planes:
         s.set_tiling(args.rows, args.planes)
     res = s.resident()
+    if args.launch >= 0:
+        N.check(N.lib.pxfdtd_resident_set_concurrent(res, args.launch))
     s.run(args.warmup)
     N.check(N.lib.pxfdtd_resident_synchronize(res))
     N.check(N.lib.pxfdtd_resident_set_profiling(res, 1))
@@ -165,12 +173,16 @@ def main():
     cells = nx * ny * nz
     value = cells * args.steps / elapsed / 1e9
     mode = N.lib.pxfdtd_resident_coefficient_mode(res)
-    bytes_step, psi = algorithmic_bytes_per_step(nx, ny, nz, pml, 3 if mode == 0 else 24)
+    bytes_step, psi = algorithmic_bytes_per_step(nx, ny, nz, pml, 1 if mode == 0 else 24)
     achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_fused_step_mmi_f32.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_march64_step_mmi_f32.json")
     if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+        # the counted bytes hold for the workload and tiling they were counted on
+        t = json.load(open(tf))
+        if (not args.plain and pml == 12 and s.kernel() == t.get("kernel_kind")
+                and list(s.tiling()) == t.get("tiling")):
+            traffic = t.get("bytes_per_launch")
     out = {
         "metric": "3D FDTD Gcell-updates/s (% HBM roofline); batched-job p50/p99 latency",
         "value": round(value, 3),

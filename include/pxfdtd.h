@@ -151,13 +151,19 @@ int pxfdtd_resident_read_probes(pxfdtd_resident *r, double *out, int64_t capacit
 /* the transform sums, FP64, series after series, each [f][k][j][i] complex */
 int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_t capacity);
 int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r);
-/* the step kernel: 0 tiles of 64 x rows exchanging H through LDS; 1 (the default) strips of
- * 62 x rows per wavefront marching along z with no barrier, loading a plane ahead; 2 the
- * same without the plane-ahead loads */
+/* the step kernel: 0 tiles of 64 x rows exchanging H through LDS; 1 strips of 62 x rows per
+ * wavefront marching along z with no barrier, loading a plane ahead; 2 the same without the
+ * plane-ahead loads; 3 (the default where a field's three components fit in 2 GiB) aligned
+ * strips of 64 x rows, the values beside the strip gathered */
 int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind);
 int pxfdtd_resident_get_kernel(const pxfdtd_resident *r);
-/* the tiling: rows per tile (kernel 0: 2, 4, 8; kernel 1: 1, 2, 4) and planes per chunk; 0
- * keeps the current value */
+/* how the aligned march launches its units: 0 (the default) the units meeting no CPML slab,
+ * those meeting only the x slabs and the others as three launches on the resident's stream,
+ * each in a kernel variant compiled for just that psi work; 1 the same with the third on a
+ * second stream beside the others; 2 every unit in one launch of the full variant */
+int pxfdtd_resident_set_concurrent(pxfdtd_resident *r, int32_t mode);
+/* the tiling: rows per tile (kernel 0: 2, 4, 8; kernels 1 to 3: 1, 2, 4) and planes per
+ * chunk; 0 keeps the current value */
 int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes);
 int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes);
 /* the coefficient storage chosen: 0 uint8 ids into a table, 1 full ca/cb arrays */

@@ -110,6 +110,7 @@ SIGNATURES = {
     "pxfdtd_resident_get_tiling": (C.c_int, [vp, pi32, pi32]),
     "pxfdtd_resident_set_kernel": (C.c_int, [vp, C.c_int32]),
     "pxfdtd_resident_get_kernel": (C.c_int, [vp]),
+    "pxfdtd_resident_set_concurrent": (C.c_int, [vp, C.c_int32]),
     "pxfdtd_resident_coefficient_mode": (C.c_int32, [vp]),
     "pxfdtd_resident_last_kernel_ms": (C.c_double, [vp]),
     "pxfdtd_resident_synchronize": (C.c_int, [vp]),

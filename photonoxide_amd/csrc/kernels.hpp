@@ -72,6 +72,7 @@ struct StepCold {
     int max_poles;
     T *m_before;              // [3][nval]
     T *m_cur, *m_prev;        // [3][nval][max_poles]
+    int psi_len;              // psi values per copy
 };
 
 // The kernel's arguments: what every plane reads.

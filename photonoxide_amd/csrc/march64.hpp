@@ -16,9 +16,11 @@
 // order as the reference's loops (src/fdtd/kernel/reference.rs:14-180), so any tiling gives
 // the same bits.
 //
-// CPML: a strip and chunk that meets a slab runs the SLABS variant, whose psi work sits behind
-// branches that are uniform over the wave: per plane for the z slabs, per row for the y slabs,
-// and for the x slabs per lane, fixed for the unit.
+// CPML: the variant AX compiles in the psi work of the axes in its bit mask (1 x, 2 y, 4 z); a
+// strip and chunk runs in the variant of the slabs it meets. Every psi value a plane needs is
+// loaded at the top of the plane with the fields, behind branches that are uniform over the
+// wave (per plane for the z slabs, per row for the y slabs, per unit for the x slabs, whose
+// lanes are fixed), so the slab work adds no round trip to memory.
 #pragma once
 #include "kernels.hpp"
 
@@ -88,9 +90,10 @@ __device__ __forceinline__ int wrap_any(int m, int n, int per) {
     return q < 0 ? q + n : q;
 }
 
-template <typename T, int MODE, int RY, bool SLABS>
+template <typename T, int MODE, int RY, int AX>
 __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
     constexpr int NR = RY + 2;  // rows of E held: j0 - 1 .. j0 + RY
+    constexpr bool SLABS = AX != 0, XS = AX & 1, YS = AX & 2, ZS = AX & 4;
     constexpr int CPML_MAX = 2048;
     __shared__ StepCold<T> K;
     __shared__ T cp[SLABS ? CPML_MAX : 1];  // the CPML's b and a
@@ -172,27 +175,35 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
     const T fxh_l = coef[fhx + max(xl, 0)], fyh_l = coef[fhy + max(gyl, 0)];
 
     // CPML geometry fixed for the unit: the x slab per lane (and at the column left of the
-    // strip), the y slab per row
+    // strip), the y slab per row. psi moves through buffer loads and stores whose offset is
+    // out of range where a lane has no psi value, so no lane branches around them.
     int lx = 0, hx = 0, ly = 0, hy = 0, lz = 0, hz = 0;
-    int xs = -1, xcnt = 0, xsl = -1, xlcnt = 0, xls = 0;
+    int xs = -1, xcnt = 0, xsx = 0, xsl = -1, xlcnt = 0, xls = 0;
+    int xb[2][2] = {{0, 0}, {0, 0}};  // x slab psi base per [field][component y, z] at this lane
     T bxh = 0, axh = 0, bxe = 0, axe = 0, bxl = 0, axl = 0;
     int ys[NR], yl[NR];
     bool xany = false;
-    const T *psi_in = nullptr;
-    T *psi_out = nullptr;
+    __amdgpu_buffer_rsrc_t rPi = rnone, rPo = rnone;
+    const uint32_t vown = own_x ? (uint32_t)xi * sizeof(T) : OUTSIDE;
     if (SLABS) {
         lx = uniform(K.low[0]); hx = uniform(K.high[0]);
         ly = uniform(K.low[1]); hy = uniform(K.high[1]);
         lz = uniform(K.low[2]); hz = uniform(K.high[2]);
-        psi_in = uniform_ptr(K.psi_in);
-        psi_out = uniform_ptr(K.psi_out);
+        const uint32_t pb = (uint32_t)uniform(K.psi_len) * sizeof(T);
+        rPi = rsrc(uniform_ptr(K.psi_in), pb);
+        rPo = rsrc(uniform_ptr(K.psi_out), pb);
         xs = own_x ? side_of(xi, nx, lx, hx) : -1;
         xcnt = xs == 1 ? hx : lx;
-        const int xsx = xi - (xs == 1 ? nx - hx : 0);
+        xsx = xi - (xs == 1 ? nx - hx : 0);
         if (xs >= 0) {
             const int ch = K.cpml_off[0][xs][1] - cbase, ce = K.cpml_off[0][xs][0] - cbase;
             bxh = cp[ch + xsx]; axh = cp[ch + xcnt + xsx];
             bxe = cp[ce + xsx]; axe = cp[ce + xcnt + xsx];
+#pragma unroll
+            for (int F = 0; F < 2; F++) {
+                xb[F][0] = K.psi_off[F][1][0][xs] + xsx;
+                xb[F][1] = K.psi_off[F][2][0][xs] + xsx;
+            }
         }
         xany = (lx > 0 && i0 < lx) || (hx > 0 && min(i0 + SW, nx) > nx - hx);
         xsl = xl >= 0 ? side_of(xl, nx, lx, hx) : -1;
@@ -208,11 +219,6 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
             yl[r] = yj[r] - (ys[r] == 1 ? ny - hy : 0);
         }
     }
-    // psi offsets: x slab (k * ny + j) * count + s, y slab (k * count + s) * nx + i,
-    // z slab (s * ny + j) * nx + i, after the slab's base
-    auto xoff = [&](int F, int c, int k, int j) {
-        return K.psi_off[F][c][0][xs] + (k * ny + j) * xcnt + (xi - (xs == 1 ? nx - hx : 0));
-    };
 
     auto zwrap = [&](int kk) { return wrap_any(kk, nz, perz); };
     const bool has_pro = kz0 > 0 || perz;
@@ -268,13 +274,79 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
             }
         }
         const T fzh = coef[fhz + k], fze = coef[fez + k];
-        int zs = -1, zsl = 0;
+
+        // this plane's psi values, loaded with the fields: per [row][component] for H (rows
+        // j0 - 1 .. j0 + RY - 1) and E (the strip's rows), per axis; 0 where none
+        int zs = -1, zsl = 0, zcnt = 0;
+        T qx[RY + 1][2], qxe[RY][2], qy[RY + 1][2], qye[RY][2], qz[RY + 1][2], qze[RY][2], hq[4];
+        // x slab psi byte offset of (field F, component y or z) on row r; out of range if none
+        auto xo = [&](int F, int c, int r) -> uint32_t {
+            return (xs >= 0 && yj[r] >= 0) ? (uint32_t)(xb[F][c] + (k * ny + yj[r]) * xcnt) * sizeof(T) : OUTSIDE;
+        };
+        auto yo = [&](int F, int c, int r) -> uint32_t {  // y slab: the row's first value
+            const int cnt = ys[r] == 1 ? hy : ly;
+            return (uint32_t)(uniform(K.psi_off[F][c][1][ys[r]]) + (k * cnt + yl[r]) * nx) * sizeof(T);
+        };
+        auto zo = [&](int F, int c, int r) -> uint32_t {  // z slab: the row's first value
+            return (uint32_t)(uniform(K.psi_off[F][c][2][zs]) + (zsl * ny + yj[r]) * nx) * sizeof(T);
+        };
         if (SLABS) {
             zs = side_of(k, nz, lz, hz);
             zsl = k - (zs == 1 ? nz - hz : 0);
+            zcnt = zs == 1 ? hz : lz;
+#pragma unroll
+            for (int r = 0; r <= RY; r++) {
+                qx[r][0] = qx[r][1] = qy[r][0] = qy[r][1] = qz[r][0] = qz[r][1] = T(0);
+                if (r > 0) qxe[r - 1][0] = qxe[r - 1][1] = qye[r - 1][0] = qye[r - 1][1] = qze[r - 1][0] = qze[r - 1][1] = T(0);
+            }
+            hq[0] = hq[1] = hq[2] = hq[3] = T(0);
+            if (XS && xany) {
+#pragma unroll
+                for (int r = 0; r <= RY; r++) {
+                    if (r > 0) qx[r][0] = bload<T>(rPi, xo(1, 0, r), 0);
+                    qx[r][1] = bload<T>(rPi, xo(1, 1, r), 0);
+                    if (r > 0) {
+                        qxe[r - 1][0] = bload<T>(rPi, xo(0, 0, r), 0);
+                        qxe[r - 1][1] = bload<T>(rPi, xo(0, 1, r), 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r <= RY; r++) {
+                if (YS && ys[r] >= 0) {
+                    qy[r][0] = bload<T>(rPi, vown, yo(1, 0, r));
+                    qy[r][1] = bload<T>(rPi, vown, yo(1, 2, r));
+                    if (r > 0) {
+                        qye[r - 1][0] = bload<T>(rPi, vown, yo(0, 0, r));
+                        qye[r - 1][1] = bload<T>(rPi, vown, yo(0, 2, r));
+                    }
+                }
+                if (ZS && zs >= 0 && yj[r] >= 0) {
+                    qz[r][0] = bload<T>(rPi, vown, zo(1, 0, r));
+                    if (r > 0) {
+                        qz[r][1] = bload<T>(rPi, vown, zo(1, 1, r));
+                        qze[r - 1][0] = bload<T>(rPi, vown, zo(0, 0, r));
+                        qze[r - 1][1] = bload<T>(rPi, vown, zo(0, 1, r));
+                    }
+                }
+            }
+            if (!pro) {  // the column left of the strip: Hy, Hz on lane m < RY (read only)
+                if (XS && xsl >= 0) {
+                    const int q = (k * ny + gyl) * xlcnt + xls;
+                    hq[0] = bload<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][1][0][xsl]) + q) * sizeof(T) : OUTSIDE, 0);
+                    hq[1] = bload<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][2][0][xsl]) + q) * sizeof(T) : OUTSIDE, 0);
+                }
+                const int sd = YS && hok ? side_of(gyl, ny, ly, hy) : -1;
+                if (sd >= 0) {
+                    const int cnt = sd == 1 ? hy : ly, s = gyl - (sd == 1 ? ny - hy : 0);
+                    hq[2] = bload<T>(rPi, (uint32_t)(K.psi_off[1][2][1][sd] + (k * cnt + s) * nx + xl) * sizeof(T), 0);
+                }
+                if (ZS && zs >= 0)
+                    hq[3] = bload<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][1][2][zs]) + (zsl * ny + gyl) * nx + xl) * sizeof(T) : OUTSIDE, 0);
+            }
         }
 
-        // new H on rows j0 - 1 .. j0 + RY - 1 (the first only Hx, Hz, and not for the prologue)
+        // new H on rows j0 - 1 .. j0 + RY - 1 (the first only Hx, Hz)
         T hn[RY + 1][3];
 #pragma unroll
         for (int r = 0; r <= RY; r++) {
@@ -289,51 +361,42 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
             T v1 = ho[r][1] - dt * (dzy * fzh - dxy * fxh);
             T v2 = ho[r][2] - dt * (dxz * fxh - dyz * fyh);
             if (SLABS) {
-                // psi values exist on the grid's own cells; those of the strip's rows are
-                // written, the row below's only read
-                const bool cell = own_x && yj[r] >= 0 && !(pro && r == 0);
-                const bool wr = !pro && r > 0 && so[r] != OUTSIDE;
-                if (xany && cell && xs >= 0) {
+                // the strip's own rows write their psi; the row below's is only read
+                const bool wr = !pro && r > 0 && j0 - 1 + r < ny;
+                if (XS && xany) {
+                    const bool in = xs >= 0;
                     if (r > 0) {
-                        const int o = xoff(1, 1, k, yj[r]);
-                        const T u = bxh * psi_in[o] + axh * (dxy / stx);
-                        if (wr) psi_out[o] = u;
-                        v1 = v1 - dt * (T(-1) * u);
+                        const T u = bxh * qx[r][0] + axh * (dxy / stx);
+                        v1 = in ? v1 - dt * (T(-1) * u) : v1;
+                        if (wr) bstore<T>(u, rPo, xo(1, 0, r), 0);
                     }
-                    const int o = xoff(1, 2, k, yj[r]);
-                    const T u = bxh * psi_in[o] + axh * (dxz / stx);
-                    if (wr) psi_out[o] = u;
-                    v2 = v2 - dt * (T(1) * u);
+                    const T u = bxh * qx[r][1] + axh * (dxz / stx);
+                    v2 = in ? v2 - dt * (T(1) * u) : v2;
+                    if (wr) bstore<T>(u, rPo, xo(1, 1, r), 0);
                 }
-                if (ys[r] >= 0 && cell) {
-                    const int sd = ys[r], cnt = sd == 1 ? hy : ly;
-                    const int co = K.cpml_off[1][sd][1] - cbase;
+                if (YS && ys[r] >= 0) {
+                    const int cnt = ys[r] == 1 ? hy : ly;
+                    const int co = K.cpml_off[1][ys[r]][1] - cbase;
                     const T bb = cp[co + yl[r]], aa = cp[co + cnt + yl[r]];
-                    const int q = (k * cnt + yl[r]) * nx + xi;
-                    const int o0 = K.psi_off[1][0][1][sd] + q, o2 = K.psi_off[1][2][1][sd] + q;
-                    const T u0 = bb * psi_in[o0] + aa * (dyx / sty);
-                    const T u2 = bb * psi_in[o2] + aa * (dyz / sty);
-                    if (wr) {
-                        psi_out[o0] = u0;
-                        psi_out[o2] = u2;
-                    }
+                    const T u0 = bb * qy[r][0] + aa * (dyx / sty);
+                    const T u2 = bb * qy[r][1] + aa * (dyz / sty);
                     v0 = v0 - dt * (T(1) * u0);
                     v2 = v2 - dt * (T(-1) * u2);
+                    if (wr) {
+                        bstore<T>(u0, rPo, vown, yo(1, 0, r));
+                        bstore<T>(u2, rPo, vown, yo(1, 2, r));
+                    }
                 }
-                if (zs >= 0 && cell) {
-                    const int cnt = zs == 1 ? hz : lz;
+                if (ZS && zs >= 0 && yj[r] >= 0) {
                     const int co = K.cpml_off[2][zs][1] - cbase;
-                    const T bb = cp[co + zsl], aa = cp[co + cnt + zsl];
-                    const int q = (zsl * ny + yj[r]) * nx + xi;
-                    const int o0 = K.psi_off[1][0][2][zs] + q;
-                    const T u0 = bb * psi_in[o0] + aa * (dzx / stz);
-                    if (wr) psi_out[o0] = u0;
+                    const T bb = cp[co + zsl], aa = cp[co + zcnt + zsl];
+                    const T u0 = bb * qz[r][0] + aa * (dzx / stz);
                     v0 = v0 - dt * (T(-1) * u0);
+                    if (wr) bstore<T>(u0, rPo, vown, zo(1, 0, r));
                     if (r > 0) {
-                        const int o1 = K.psi_off[1][1][2][zs] + q;
-                        const T u1 = bb * psi_in[o1] + aa * (dzy / stz);
-                        if (wr) psi_out[o1] = u1;
+                        const T u1 = bb * qz[r][1] + aa * (dzy / stz);
                         v1 = v1 - dt * (T(1) * u1);
+                        if (wr) bstore<T>(u1, rPo, vown, zo(1, 1, r));
                     }
                 }
             }
@@ -359,28 +422,20 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
             const T dxz = ey0 - cur.gy, dyz = exn - cur.gx;
             T v1 = ghy - dt * (dzy * fzh - dxy * fxh_l);
             T v2 = ghz - dt * (dxz * fxh_l - dyz * fyh_l);
-            if (SLABS && hok) {
-                if (xsl >= 0) {
-                    const int base = (k * ny + gyl) * xlcnt + xls;
-                    const T u1 = bxl * psi_in[K.psi_off[1][1][0][xsl] + base] + axl * (dxy / stx);
-                    const T u2 = bxl * psi_in[K.psi_off[1][2][0][xsl] + base] + axl * (dxz / stx);
-                    v1 = v1 - dt * (T(-1) * u1);
-                    v2 = v2 - dt * (T(1) * u2);
+            if (SLABS) {
+                if (XS && xsl >= 0) {
+                    v1 = v1 - dt * (T(-1) * (bxl * hq[0] + axl * (dxy / stx)));
+                    v2 = v2 - dt * (T(1) * (bxl * hq[1] + axl * (dxz / stx)));
                 }
-                const int sd = side_of(gyl, ny, ly, hy);
+                const int sd = YS && hok ? side_of(gyl, ny, ly, hy) : -1;
                 if (sd >= 0) {
                     const int cnt = sd == 1 ? hy : ly, s = gyl - (sd == 1 ? ny - hy : 0);
                     const int co = K.cpml_off[1][sd][1] - cbase;
-                    const T u2 = cp[co + s] * psi_in[K.psi_off[1][2][1][sd] + (k * cnt + s) * nx + xl] +
-                                 cp[co + cnt + s] * (dyz / sty);
-                    v2 = v2 - dt * (T(-1) * u2);
+                    v2 = v2 - dt * (T(-1) * (cp[co + s] * hq[2] + cp[co + cnt + s] * (dyz / sty)));
                 }
-                if (zs >= 0) {
-                    const int cnt = zs == 1 ? hz : lz;
+                if (ZS && zs >= 0) {
                     const int co = K.cpml_off[2][zs][1] - cbase;
-                    const T u1 = cp[co + zsl] * psi_in[K.psi_off[1][1][2][zs] + (zsl * ny + gyl) * nx + xl] +
-                                 cp[co + cnt + zsl] * (dzy / stz);
-                    v1 = v1 - dt * (T(1) * u1);
+                    v1 = v1 - dt * (T(1) * (cp[co + zsl] * hq[3] + cp[co + zcnt + zsl] * (dzy / stz)));
                 }
             }
             T v0 = T(0);
@@ -425,41 +480,43 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
                 T v0 = ca0 * cur.e[r][0] + cb0 * (dyx * fye - dzx * fze);
                 T w1 = ca1 * cur.e[r][1] + cb1 * (dzy2 * fze - dxy2 * fxe);
                 T w2 = ca2 * cur.e[r][2] + cb2 * (dxz2 * fxe - dyz2 * fye);
-                if (SLABS && own) {
-                    if (xany && xs >= 0) {
-                        const int o1 = xoff(0, 1, k, j), o2 = xoff(0, 2, k, j);
-                        const T u1 = bxe * psi_in[o1] + axe * (dxy2 / stx);
-                        const T u2 = bxe * psi_in[o2] + axe * (dxz2 / stx);
-                        psi_out[o1] = u1;
-                        psi_out[o2] = u2;
-                        w1 = w1 + cb1 * (T(-1) * u1);
-                        w2 = w2 + cb2 * (T(1) * u2);
+                if (SLABS) {
+                    const bool wr = j0 - 1 + r < ny;
+                    if (XS && xany) {
+                        const bool in = xs >= 0;
+                        const T u1 = bxe * qxe[r - 1][0] + axe * (dxy2 / stx);
+                        const T u2 = bxe * qxe[r - 1][1] + axe * (dxz2 / stx);
+                        w1 = in ? w1 + cb1 * (T(-1) * u1) : w1;
+                        w2 = in ? w2 + cb2 * (T(1) * u2) : w2;
+                        if (wr) {
+                            bstore<T>(u1, rPo, xo(0, 0, r), 0);
+                            bstore<T>(u2, rPo, xo(0, 1, r), 0);
+                        }
                     }
-                    if (ys[r] >= 0) {
-                        const int sd = ys[r], cnt = sd == 1 ? hy : ly;
-                        const int co = K.cpml_off[1][sd][0] - cbase;
+                    if (YS && ys[r] >= 0) {
+                        const int cnt = ys[r] == 1 ? hy : ly;
+                        const int co = K.cpml_off[1][ys[r]][0] - cbase;
                         const T bb = cp[co + yl[r]], aa = cp[co + cnt + yl[r]];
-                        const int q = (k * cnt + yl[r]) * nx + xi;
-                        const int o0 = K.psi_off[0][0][1][sd] + q, o2 = K.psi_off[0][2][1][sd] + q;
-                        const T u0 = bb * psi_in[o0] + aa * (dyx / sty);
-                        const T u2 = bb * psi_in[o2] + aa * (dyz2 / sty);
-                        psi_out[o0] = u0;
-                        psi_out[o2] = u2;
+                        const T u0 = bb * qye[r - 1][0] + aa * (dyx / sty);
+                        const T u2 = bb * qye[r - 1][1] + aa * (dyz2 / sty);
                         v0 = v0 + cb0 * (T(1) * u0);
                         w2 = w2 + cb2 * (T(-1) * u2);
+                        if (wr) {
+                            bstore<T>(u0, rPo, vown, yo(0, 0, r));
+                            bstore<T>(u2, rPo, vown, yo(0, 2, r));
+                        }
                     }
-                    if (zs >= 0) {
-                        const int cnt = zs == 1 ? hz : lz;
+                    if (ZS && zs >= 0 && j >= 0) {
                         const int co = K.cpml_off[2][zs][0] - cbase;
-                        const T bb = cp[co + zsl], aa = cp[co + cnt + zsl];
-                        const int q = (zsl * ny + j) * nx + xi;
-                        const int o0 = K.psi_off[0][0][2][zs] + q, o1 = K.psi_off[0][1][2][zs] + q;
-                        const T u0 = bb * psi_in[o0] + aa * (dzx / stz);
-                        const T u1 = bb * psi_in[o1] + aa * (dzy2 / stz);
-                        psi_out[o0] = u0;
-                        psi_out[o1] = u1;
+                        const T bb = cp[co + zsl], aa = cp[co + zcnt + zsl];
+                        const T u0 = bb * qze[r - 1][0] + aa * (dzx / stz);
+                        const T u1 = bb * qze[r - 1][1] + aa * (dzy2 / stz);
                         v0 = v0 + cb0 * (T(-1) * u0);
                         w1 = w1 + cb1 * (T(1) * u1);
+                        if (wr) {
+                            bstore<T>(u0, rPo, vown, zo(0, 0, r));
+                            bstore<T>(u1, rPo, vown, zo(0, 1, r));
+                        }
                     }
                 }
                 if (injects && own) inject<T, 0>(A, K, t, xi, j, k, 7, v0, w1, w2, cb0, cb1, cb2);

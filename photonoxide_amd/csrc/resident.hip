@@ -125,6 +125,10 @@ struct SlabDesc {
 struct pxfdtd_resident {
     pxfdtd_device *dev = nullptr;
     hipStream_t stream = nullptr;
+    // a second stream: the CPML-slab launch runs beside the lean one, forked and joined by events
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    int concurrent = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int prec = PXFDTD_F32;
     size_t esz = 4;
@@ -166,13 +170,13 @@ struct pxfdtd_resident {
     long long probe_rows = 0, probe_cap = 0;
     // stepping
     int p = 0;
-    int kernel = 1;  // 0: LDS tiles (64 x ty, barrier per plane); 1: register march (62 x ty)
+    int kernel = 3;  // 0: LDS tiles (64 x ty); 1, 2: 62-wide march (with, without prefetch); 3: aligned march
     int ty = 4, kz = 16;
     double last_ms = 0;
     // register march: work units (strip x, strip y, first plane, end plane), those touching
     // no CPML slab first, then those that do
     Buf units;
-    int n_lean = 0, n_slab = 0;
+    int n_lean = 0, n_x = 0, n_slab = 0;  // the march's units: no slab, x slabs only, others
     bool units_dirty = true;
     // per-launch timing of the fused kernel, when asked for
     bool profile = false;
@@ -413,6 +417,7 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
         }
     }
     r->psi_len = off;
+    if (r->kernel == 3 && off * (long long)sizeof(T) >= (1LL << 31)) r->kernel = 1;  // psi beyond 32-bit offsets
     if (off >= (1LL << 31)) return px_fail(PXFDTD_ERR_UNSUPPORTED, "the CPML's psi is beyond 32-bit indexing");
     for (int q = 0; q < 2; q++) {
         TRY(r->psi[q].alloc(std::max<long long>(off, 1) * sizeof(T)));
@@ -467,18 +472,32 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
     r->dt = p->dt;
     if (r->nval * 2 >= (1LL << 31))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "a grid of %lld values per component is beyond 32-bit indexing", r->nval);
-    // tiling defaults: 8 rows per tile, chunks that give the chip many workgroups
+    // defaults: the aligned march where a field's three components fit its 32-bit buffer
+    // offsets (else the 62-wide one), 2 rows per strip, and plane chunks short enough that the
+    // chip sees several rounds of wavefronts (8 planes, fewer on small grids)
     r->ty = 2;
+    r->kernel = 3 * r->nval * (long long)r->esz < (1LL << 31) ? 3 : 1;
+    r->concurrent = r->kernel == 3 ? 1 : 0;
     {
-        const long long tiles = (long long)((p->nx + RW - 1) / RW) * ((p->ny + r->ty - 1) / r->ty) / 4;
-        int kz = p->nz;
-        while (kz > 8 && tiles * ((p->nz + kz - 1) / kz) < 2048) kz = (kz + 1) / 2;
+        const long long strips = (long long)((p->nx + (r->kernel == 3 ? SW : RW) - 1) / (r->kernel == 3 ? SW : RW)) *
+                                 ((p->ny + r->ty - 1) / r->ty);
+        int kz = std::min(p->nz, 8);
+        while (kz > 2 && strips * ((p->nz + kz - 1) / kz) < 4096) kz = (kz + 1) / 2;
         r->kz = std::max(1, kz);
     }
     hipError_t e = hipStreamCreate(&r->stream);
     if (e != hipSuccess) {
         delete r;
         return px_fail(PXFDTD_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    if (hipStreamCreate(&r->side) != hipSuccess || hipEventCreateWithFlags(&r->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&r->join_ev, hipEventDisableTiming) != hipSuccess) {
+        if (r->side) (void)hipStreamDestroy(r->side);
+        if (r->fork_ev) (void)hipEventDestroy(r->fork_ev);
+        if (r->join_ev) (void)hipEventDestroy(r->join_ev);
+        (void)hipStreamDestroy(r->stream);
+        delete r;
+        return px_fail(PXFDTD_ERR_DEVICE, "the side stream or its events could not be created");
     }
     (void)hipEventCreate(&r->ev0);
     (void)hipEventCreate(&r->ev1);
@@ -499,10 +518,14 @@ extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
     if (r->ev0) (void)hipEventDestroy(r->ev0);
     if (r->ev1) (void)hipEventDestroy(r->ev1);
     for (auto e : r->kev) (void)hipEventDestroy(e);
-    hipStream_t s = r->stream;
+    if (r->side) (void)hipStreamSynchronize(r->side);
+    if (r->fork_ev) (void)hipEventDestroy(r->fork_ev);
+    if (r->join_ev) (void)hipEventDestroy(r->join_ev);
+    hipStream_t s = r->stream, side = r->side;
     specs().erase(r);
     delete r;  // buffers free themselves
     if (s) (void)hipStreamDestroy(s);
+    if (side) (void)hipStreamDestroy(side);
 }
 
 extern "C" int64_t pxfdtd_resident_psi_len(const pxfdtd_resident *r) { return r->psi_len; }
@@ -705,23 +728,30 @@ static int build_units(pxfdtd_resident *r) {
     auto in_slab = [](int a, int b, int n, int lo, int hi) {  // [a, b) meets [0, lo) or [n-hi, n)
         return (lo > 0 && a < lo) || (hi > 0 && b > n - hi);
     };
-    std::vector<int4> lean, slab;
+    std::vector<int4> lean, xonly, slab;
     for (size_t c = 0; c + 1 < cuts.size(); c++) {
         const int a = cuts[c], b = cuts[c + 1];
         const bool zs = in_slab(a > 0 || r->periodic[2] ? a - 1 : a, b, nz, lz, hz) && !r->periodic[2];
         for (int by = 0; by < ty; by++) {
-            const int j0 = by * r->ty;
-            const bool ys = !r->periodic[1] && in_slab(std::max(j0 - 1, 0), std::min(j0 + r->ty, r->ny), r->ny, r->low[1], r->high[1]);
+            const int j0 = by * th;
+            const bool ys = !r->periodic[1] && in_slab(std::max(j0 - 1, 0), std::min(j0 + th, r->ny), r->ny, r->low[1], r->high[1]);
             for (int bx = 0; bx < tx; bx++) {
                 const int i0 = bx * tw;
                 const bool xs = !r->periodic[0] && in_slab(std::max(i0 - 1, 0), std::min(i0 + tw, r->nx), r->nx, r->low[0], r->high[0]);
                 const int4 u = make_int4(bx, by, a, b);
-                ((xs || ys || zs) ? slab : lean).push_back(u);
+                if (r->kernel == 3 && r->concurrent == 2)
+                    slab.push_back(u);  // one launch of every unit, in the grid's order
+                else if (r->kernel == 3 && xs && !ys && !zs)
+                    xonly.push_back(u);
+                else
+                    ((xs || ys || zs) ? slab : lean).push_back(u);
             }
         }
     }
     r->n_lean = (int)lean.size();
+    r->n_x = (int)xonly.size();
     r->n_slab = (int)slab.size();
+    lean.insert(lean.end(), xonly.begin(), xonly.end());
     lean.insert(lean.end(), slab.begin(), slab.end());
     TRY(r->units.alloc(std::max<size_t>(lean.size(), 1) * sizeof(int4)));
     HIPCHK(hipMemcpy(r->units.p, lean.data(), lean.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -763,7 +793,7 @@ extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int3
 
 extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
     if (kind < 0 || kind > 3) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0, 1, 2 or 3");
-    if (kind == 3 && 3 * r->nval * (long long)r->esz >= (1LL << 31))
+    if (kind == 3 && (3 * r->nval * (long long)r->esz >= (1LL << 31) || r->psi_len * (long long)r->esz >= (1LL << 31)))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the aligned march needs a field's three components within 2 GiB");
     r->kernel = kind;
     r->ty = kind == 0 ? 8 : 2;
@@ -773,6 +803,13 @@ extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
 }
 
 extern "C" int pxfdtd_resident_get_kernel(const pxfdtd_resident *r) { return r->kernel; }
+
+extern "C" int pxfdtd_resident_set_concurrent(pxfdtd_resident *r, int32_t mode) {
+    if (mode < 0 || mode > 2) return px_fail(PXFDTD_ERR_INVALID, "the launch mode must be 0, 1 or 2");
+    if (mode != r->concurrent) r->units_dirty = true;
+    r->concurrent = mode;
+    return PXFDTD_OK;
+}
 
 extern "C" int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes) {
     *rows = r->ty;
@@ -994,6 +1031,7 @@ static int upload_cold(pxfdtd_resident *r) {
         k.m_before = r->m_before.as<T>();
         k.m_cur = r->m_cur.as<T>();
         k.m_prev = r->m_prev.as<T>();
+        k.psi_len = (int)r->psi_len;
     }
     TRY(r->cold.reserve(sizeof K));
     HIPCHK(hipMemcpy(r->cold.p, K, sizeof K, hipMemcpyHostToDevice));
@@ -1025,31 +1063,57 @@ static int launch_march(const StepArgs<T> &A0, hipStream_t s, size_t lds) {
     return PXFDTD_OK;
 }
 
-template <typename T, int MODE, int RY>
-static int launch_march64(const StepArgs<T> &A0, hipStream_t s, size_t lds) {
+struct SideStream {
+    hipStream_t s;
+    hipEvent_t fork, join;
+};
+struct UnitCounts {
+    int lean, x, slab;  // units in the lists, in this order
+};
+
+// The units touching no CPML slab and those that do, as two launches: with a side stream the
+// slab launch (the longer per unit) runs there beside the lean one, so neither's tail idles
+// the chip; the main stream waits for both before the step's next work.
+template <typename T, int MODE, int RY, int AX>
+static void launch_units(const StepArgs<T> &A0, int first, int count, hipStream_t s, size_t lds) {
+    if (!count) return;
     StepArgs<T> A = A0;
-    const int lean = A0.blocks, slab = A0.per_xcd;  // counts passed through blocks, per_xcd
-    if (lean) {
-        A.blocks = lean;
-        A.per_xcd = ((lean + 3) / 4 + 7) / 8;
-        hipLaunchKernelGGL((march64_step<T, MODE, RY, false>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+    A.units = A0.units + first;
+    A.blocks = count;
+    A.per_xcd = ((count + 3) / 4 + 7) / 8;
+    hipLaunchKernelGGL((march64_step<T, MODE, RY, AX>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+}
+
+// The units touching no CPML slab, those touching only the x slabs and the others, each in the
+// variant that compiles in just that psi work. With a side stream the slab launch runs there
+// beside the others; the main stream waits for it before the step's next work.
+template <typename T, int MODE, int RY>
+static int launch_march64(const StepArgs<T> &A, hipStream_t s, size_t lds, const SideStream *side,
+                          const UnitCounts &n) {
+    const bool both = (n.lean || n.x) && n.slab && side;
+    hipStream_t ss = both ? side->s : s;
+    if (both) {
+        HIPCHK(hipEventRecord(side->fork, s));
+        HIPCHK(hipStreamWaitEvent(ss, side->fork, 0));
     }
-    if (slab) {
-        A.units = A0.units + lean;
-        A.blocks = slab;
-        A.per_xcd = ((slab + 3) / 4 + 7) / 8;
-        hipLaunchKernelGGL((march64_step<T, MODE, RY, true>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+    launch_units<T, MODE, RY, 7>(A, n.lean + n.x, n.slab, ss, lds);
+    launch_units<T, MODE, RY, 0>(A, 0, n.lean, s, lds);
+    launch_units<T, MODE, RY, 1>(A, n.lean, n.x, s, lds);
+    if (both) {
+        HIPCHK(hipEventRecord(side->join, ss));
+        HIPCHK(hipStreamWaitEvent(s, side->join, 0));
     }
     return PXFDTD_OK;
 }
 
 template <typename T, int MODE>
-static int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, size_t lds) {
+static int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, size_t lds,
+                       const SideStream *side, const UnitCounts &n) {
     if (kernel == 3) {
         switch (ty) {
-        case 1: return launch_march64<T, MODE, 1>(A, s, lds);
-        case 2: return launch_march64<T, MODE, 2>(A, s, lds);
-        default: return launch_march64<T, MODE, 4>(A, s, lds);
+        case 1: return launch_march64<T, MODE, 1>(A, s, lds, side, n);
+        case 2: return launch_march64<T, MODE, 2>(A, s, lds, side, n);
+        default: return launch_march64<T, MODE, 4>(A, s, lds, side, n);
         }
     }
     if (kernel == 1) {
@@ -1123,10 +1187,13 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         A.cold = r->cold.as<StepCold<T>>() + p;
         A.wave_step = (int)n;
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
+        const SideStream side = {r->side, r->fork_ev, r->join_ev};
+        const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
+        const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab};
         if (r->mode == 0)
-            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds)));
+            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc)));
         else
-            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds)));
+            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc)));
         HIPCHK(hipGetLastError());
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
         r->p = q;
