@@ -39,7 +39,8 @@ enum {
     PXFDTD_ERR_INVALID = 1,     /* a value out of range: Error::InvalidValue */
     PXFDTD_ERR_UNSUPPORTED = 2, /* what the device doesn't step: Error::Gpu */
     PXFDTD_ERR_MEMORY = 3,      /* it doesn't fit */
-    PXFDTD_ERR_DEVICE = 4       /* the HIP runtime failed */
+    PXFDTD_ERR_DEVICE = 4,      /* the HIP runtime failed */
+    PXFDTD_ERR_TIMEOUT = 5      /* a wait ended before its job did */
 };
 enum { PXFDTD_F32 = 0, PXFDTD_F64 = 1 };
 enum { PXFDTD_E = 0, PXFDTD_H = 1 };
@@ -176,8 +177,16 @@ int pxfdtd_resident_synchronize(pxfdtd_resident *r);
 int pxfdtd_resident_set_profiling(pxfdtd_resident *r, int32_t on);
 /* the fused step kernel's average duration over the last pxfdtd_resident_step call, ms */
 double pxfdtd_resident_kernel_ms(const pxfdtd_resident *r);
+/* The autotuner (no counterpart in the reference): times `trials` launches of the aligned
+ * march at each tiling of rows {1, 2, 4} (at most max_rows) x planes {4, 8, 16}, on the
+ * resident's current fields without taking a step (the parity stays, injections are left
+ * out), and keeps the fastest. Problems with dispersive media (whose state a trial would
+ * advance) and the other kernels keep their tiling. The tiling chosen and its ms per launch. */
+int pxfdtd_resident_autotune(pxfdtd_resident *r, int32_t trials, int32_t max_rows, int32_t *rows,
+                             int32_t *planes, double *best_ms);
 
 /* ------------------------------------------------------------ level 2: the host mirror */
+/* (the serving loop, which batches host-mirror simulations, follows the host mirror) */
 
 typedef struct pxfdtd_simulation pxfdtd_simulation;
 
@@ -261,6 +270,57 @@ int pxfdtd_simulation_probe(pxfdtd_simulation *s, int32_t n, double *out);
 int64_t pxfdtd_simulation_dft_len(pxfdtd_simulation *s, int32_t n, int32_t series);
 int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t series, double *out);
 pxfdtd_resident *pxfdtd_simulation_resident(pxfdtd_simulation *s);
+/* the device bytes the simulation needs in `precision` (fields in two copies, psi, coefficient
+ * tables, media state, monitor staging and FP64 sums): what admission by footprint counts */
+int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precision, int64_t *bytes);
+
+/* ------------------------------------------------------------ level 3: the serving loop */
+/* Continuous batching of jobs on one device (no counterpart in the reference; its nearest is
+ * the process-level farm, src/farm.rs:1-56). A job is a host-mirror simulation with no device,
+ * handed over with a step count; a loop thread admits jobs in arrival order while the admitted
+ * jobs' device footprint fits the budget and their number max_jobs, steps all admitted jobs
+ * together in segments of `segment` steps (one batched launch per kernel class and step,
+ * replayed from a captured hipGraph while the job set stays), and admits new arrivals at each
+ * segment boundary. A finished job's fields, psi and FP64 transform sums go back to its
+ * simulation, whose device is then none again, and its device memory is freed. Each job's
+ * results are the same bits as running it alone. Jobs step in f32; probes are not recorded
+ * (a job with probes is refused). The caller leaves a submitted simulation alone until its
+ * job is done, failed or cancelled. */
+typedef struct pxfdtd_server pxfdtd_server;
+typedef struct {
+    int64_t hbm_budget;  /* device bytes the admitted jobs may hold; 0: 90 % of the free HBM */
+    int32_t max_jobs;    /* jobs admitted at once; 0: 256 */
+    int32_t segment;     /* steps per batched segment; 0: 32 */
+    int32_t direct;      /* 1: launch each segment's kernels directly instead of a hipGraph */
+    int32_t autotune;    /* trial launches per tiling for the first job of each grid shape;
+                            0: the default tiling */
+} pxfdtd_server_config;
+enum {
+    PXFDTD_JOB_QUEUED = 0,
+    PXFDTD_JOB_RUNNING = 1,
+    PXFDTD_JOB_DONE = 2,
+    PXFDTD_JOB_FAILED = 3,
+    PXFDTD_JOB_CANCELLED = 4
+};
+typedef struct {
+    int64_t submitted, admitted, completed, failed;
+    int64_t segments, captures; /* segments stepped; hipGraphs captured (one per job set) */
+    double busy_seconds;        /* wall time of the segments on the device */
+    double cell_updates;        /* cells x steps over all jobs */
+    int64_t hbm_in_use;         /* the admitted jobs' footprint now */
+    int64_t peak_jobs;          /* the most jobs admitted at once */
+} pxfdtd_server_stats;
+int pxfdtd_server_create(pxfdtd_device *dev, const pxfdtd_server_config *cfg, pxfdtd_server **out);
+/* stops the loop: queued jobs are cancelled, running ones cancelled after their segment */
+void pxfdtd_server_destroy(pxfdtd_server *s);
+int pxfdtd_server_submit(pxfdtd_server *s, pxfdtd_simulation *sim, int64_t steps, int64_t *job);
+int pxfdtd_server_state(pxfdtd_server *s, int64_t job, int32_t *state);
+/* waits for the job (timeout_seconds <= 0: no limit); PXFDTD_OK once it is done */
+int pxfdtd_server_wait(pxfdtd_server *s, int64_t job, double timeout_seconds);
+/* seconds since the server started: submission, admission, completion */
+int pxfdtd_server_times(pxfdtd_server *s, int64_t job, double *submitted, double *started,
+                        double *finished);
+int pxfdtd_server_get_stats(pxfdtd_server *s, pxfdtd_server_stats *out);
 
 #ifdef __cplusplus
 }

@@ -9,7 +9,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpxfdtd.so")
 
-OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_MEMORY, ERR_DEVICE = range(5)
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_MEMORY, ERR_DEVICE, ERR_TIMEOUT = range(6)
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 F32, F64 = 0, 1
 E, H = 0, 1
 
@@ -57,6 +58,18 @@ class Probe(C.Structure):
 class Series(C.Structure):
     _fields_ = [("field", C.c_int32), ("component", C.c_int32), ("lo", i3), ("hi", i3),
                 ("monitor", C.c_int32)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("segment", C.c_int32),
+                ("direct", C.c_int32), ("autotune", C.c_int32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
+                ("failed", C.c_int64), ("segments", C.c_int64), ("captures", C.c_int64),
+                ("busy_seconds", C.c_double), ("cell_updates", C.c_double),
+                ("hbm_in_use", C.c_int64), ("peak_jobs", C.c_int64)]
 
 
 class Grid(C.Structure):
@@ -116,6 +129,14 @@ SIGNATURES = {
     "pxfdtd_resident_synchronize": (C.c_int, [vp]),
     "pxfdtd_resident_set_profiling": (C.c_int, [vp, C.c_int32]),
     "pxfdtd_resident_kernel_ms": (C.c_double, [vp]),
+    "pxfdtd_resident_autotune": (C.c_int, [vp, C.c_int32, C.c_int32, pi32, pi32, pd]),
+    "pxfdtd_server_create": (C.c_int, [vp, C.POINTER(ServerConfig), C.POINTER(vp)]),
+    "pxfdtd_server_destroy": (None, [vp]),
+    "pxfdtd_server_submit": (C.c_int, [vp, vp, C.c_int64, pi64]),
+    "pxfdtd_server_state": (C.c_int, [vp, C.c_int64, pi32]),
+    "pxfdtd_server_wait": (C.c_int, [vp, C.c_int64, C.c_double]),
+    "pxfdtd_server_times": (C.c_int, [vp, C.c_int64, pd, pd, pd]),
+    "pxfdtd_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
     "pxfdtd_simulation_new": (C.c_int, [C.POINTER(Grid), C.POINTER(Bounds), C.POINTER(Scene), C.c_double, C.POINTER(vp)]),
     "pxfdtd_simulation_new_eps": (C.c_int, [C.POINTER(Grid), C.POINTER(Bounds), pd, C.c_double, C.POINTER(vp)]),
     "pxfdtd_simulation_destroy": (None, [vp]),
@@ -140,6 +161,7 @@ SIGNATURES = {
     "pxfdtd_simulation_dft_len": (C.c_int64, [vp, C.c_int32, C.c_int32]),
     "pxfdtd_simulation_dft": (C.c_int, [vp, C.c_int32, C.c_int32, pd]),
     "pxfdtd_simulation_resident": (vp, [vp]),
+    "pxfdtd_simulation_footprint": (C.c_int, [vp, C.c_int32, pi64]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():
@@ -152,7 +174,7 @@ class NativeError(RuntimeError):
     """A status from the native library, with its class and message."""
 
     def __init__(self, code, message):
-        super().__init__(f"[{ {1: 'invalid', 2: 'unsupported', 3: 'memory', 4: 'device'}.get(code, code)}] {message}")
+        super().__init__(f"[{ {1: 'invalid', 2: 'unsupported', 3: 'memory', 4: 'device', 5: 'timeout'}.get(code, code)}] {message}")
         self.code = code
         self.message = message
 

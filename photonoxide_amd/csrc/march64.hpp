@@ -90,16 +90,18 @@ __device__ __forceinline__ int wrap_any(int m, int n, int per) {
     return q < 0 ? q + n : q;
 }
 
-template <typename T, int MODE, int RY, int AX>
-__global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
-    constexpr int NR = RY + 2;  // rows of E held: j0 - 1 .. j0 + RY
-    constexpr bool SLABS = AX != 0, XS = AX & 1, YS = AX & 2, ZS = AX & 4;
-    constexpr int CPML_MAX = 2048;
-    __shared__ StepCold<T> K;
-    __shared__ T cp[SLABS ? CPML_MAX : 1];  // the CPML's b and a
-    extern __shared__ unsigned char smem_raw[];
-    T *tab = reinterpret_cast<T *>(smem_raw);                 // [256][3][2]
-    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);  // [256][3]
+// What a workgroup stages in LDS before its wavefronts march: the cold parameters, the
+// material table (uint8 ids) and the CPML's b and a. All four waves of a workgroup belong to
+// one problem.
+template <typename T>
+struct MarchShared {
+    StepCold<T> K;
+    T cp[2048];  // the CPML's b and a (the SLABS variants use it)
+};
+
+template <typename T, int MODE, bool SLABS>
+__device__ __forceinline__ void march64_stage(const StepArgs<T> &A, MarchShared<T> &S, T *tab,
+                                              int8_t *grp) {
     const int tid = threadIdx.x;
     if (MODE == 0) {
         for (int q = tid; q < 256 * 6; q += 256) tab[q] = A.table[q];
@@ -108,21 +110,26 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
     }
     {
         const int *src = reinterpret_cast<const int *>(A.cold);
-        int *dst = reinterpret_cast<int *>(&K);
+        int *dst = reinterpret_cast<int *>(&S.K);
         for (int q = tid; q < (int)(sizeof(StepCold<T>) / 4); q += 256) dst[q] = src[q];
     }
     const int cbase = 2 * (A.nx + A.ny + A.nz);
     if (SLABS)
-        for (int q = tid; q < A.ncpml && q < CPML_MAX; q += 256) cp[q] = A.coef[cbase + q];
-    __syncthreads();  // the only barrier: the parameters in LDS
+        for (int q = tid; q < A.ncpml && q < 2048; q += 256) S.cp[q] = A.coef[cbase + q];
+}
 
+// One wavefront's unit of the march: strip bx, row strip by, planes [kz0, kz1).
+template <typename T, int MODE, int RY, int AX>
+__device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 unit,
+                                             const MarchShared<T> &S, const T *tab,
+                                             const int8_t *grp) {
+    constexpr int NR = RY + 2;  // rows of E held: j0 - 1 .. j0 + RY
+    constexpr bool SLABS = AX != 0, XS = AX & 1, YS = AX & 2, ZS = AX & 4;
+    const StepCold<T> &K = S.K;
+    const T *cp = S.cp;
+    const int tid = threadIdx.x;
     const int lane = tid & 63;
-    // workgroups are dealt round-robin to the 8 XCDs: each XCD takes a contiguous run of
-    // units, so neighbouring strips (which read each other's edges) share an L2
-    const int b = (int)(blockIdx.x % 8) * A.per_xcd + (int)(blockIdx.x / 8);
-    const int w = uniform(b * 4 + (tid >> 6));
-    if (w >= A.blocks) return;
-    const int4 unit = A.units[w];  // strip x, strip y, first plane, end plane
+    const int cbase = 2 * (A.nx + A.ny + A.nz);
     const int bx = unit.x, by = unit.y, kz0 = unit.z, kz1 = unit.w;
     const int t = by * A.tiles_x + bx;
     const int i0 = bx * SW, j0 = by * RY;
@@ -536,5 +543,22 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
         cur = nxt;
     }
 }
+
+template <typename T, int MODE, int RY, int AX>
+__global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
+    __shared__ MarchShared<T> S;
+    extern __shared__ unsigned char smem_raw[];
+    T *tab = reinterpret_cast<T *>(smem_raw);                 // [256][3][2]
+    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);  // [256][3]
+    march64_stage<T, MODE, AX != 0>(A, S, tab, grp);
+    __syncthreads();  // the only barrier: the parameters in LDS
+    // workgroups are dealt round-robin to the 8 XCDs: each XCD takes a contiguous run of
+    // units, so neighbouring strips (which read each other's edges) share an L2
+    const int b = (int)(blockIdx.x % 8) * A.per_xcd + (int)(blockIdx.x / 8);
+    const int w = uniform(b * 4 + (int)(threadIdx.x >> 6));
+    if (w >= A.blocks) return;
+    march64_unit<T, MODE, RY, AX>(A, A.units[w], S, tab, grp);
+}
+
 
 }  // namespace pxfdtd

@@ -161,6 +161,8 @@ struct pxfdtd_resident {
     int nprobes = 0;
     long long nvals = 0, ntaps = 0;
     Buf tap_index, tap_which, tap_fields[2];
+    std::vector<int64_t> tap_index_h;  // the same, on the host
+    std::vector<int8_t> tap_which_h;
     Buf staging;
     int block_steps = 32;
     Buf series_of, series_info, sums, phases;
@@ -176,6 +178,7 @@ struct pxfdtd_resident {
     // register march: work units (strip x, strip y, first plane, end plane), those touching
     // no CPML slab first, then those that do
     Buf units;
+    std::vector<int4> units_h;  // the same, on the host (the server batches them)
     int n_lean = 0, n_x = 0, n_slab = 0;  // the march's units: no slab, x slabs only, others
     bool units_dirty = true;
     // per-launch timing of the fused kernel, when asked for
@@ -753,6 +756,7 @@ static int build_units(pxfdtd_resident *r) {
     r->n_slab = (int)slab.size();
     lean.insert(lean.end(), xonly.begin(), xonly.end());
     lean.insert(lean.end(), slab.begin(), slab.end());
+    r->units_h = lean;
     TRY(r->units.alloc(std::max<size_t>(lean.size(), 1) * sizeof(int4)));
     HIPCHK(hipMemcpy(r->units.p, lean.data(), lean.size() * sizeof(int4), hipMemcpyHostToDevice));
     r->units_dirty = false;
@@ -821,6 +825,8 @@ extern "C" int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *row
 static int build_taps(pxfdtd_resident *r, const std::vector<int64_t> &index,
                       const std::vector<int8_t> &which) {
     r->ntaps = (long long)index.size();
+    r->tap_index_h = index;
+    r->tap_which_h = which;
     if (!r->ntaps) return PXFDTD_OK;
     TRY(r->tap_index.alloc(index.size() * 8));
     TRY(r->tap_which.alloc(which.size()));
@@ -1009,9 +1015,8 @@ static StepArgs<T> args_of(const pxfdtd_resident *r) {
 
 // the parameters slab cells, injections and media read, one set per copy parity
 template <typename T>
-static int upload_cold(pxfdtd_resident *r) {
-    StepCold<T> K[2];
-    memset(K, 0, sizeof K);
+static void fill_cold(const pxfdtd_resident *r, StepCold<T> K[2]) {
+    memset(K, 0, 2 * sizeof(StepCold<T>));
     for (int p = 0; p < 2; p++) {
         StepCold<T> &k = K[p];
         for (int a = 0; a < 3; a++) {
@@ -1033,6 +1038,12 @@ static int upload_cold(pxfdtd_resident *r) {
         k.m_prev = r->m_prev.as<T>();
         k.psi_len = (int)r->psi_len;
     }
+}
+
+template <typename T>
+static int upload_cold(pxfdtd_resident *r) {
+    StepCold<T> K[2];
+    fill_cold<T>(r, K);
     TRY(r->cold.reserve(sizeof K));
     HIPCHK(hipMemcpy(r->cold.p, K, sizeof K, hipMemcpyHostToDevice));
     return PXFDTD_OK;
@@ -1244,6 +1255,82 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
     return PXFDTD_OK;
 }
 
+// ------------------------------------------------------------------------------ tuning
+// `trials` launches of the step kernel that read copy p and write copy 1 - p without taking the
+// step (the parity stays, and psi ping-pongs as the fields do), so the state the next real step
+// reads is unchanged; injections are left out. Average ms per launch.
+template <typename T>
+static int trial_ms(pxfdtd_resident *r, int trials, double *ms) {
+    hipStream_t s = r->stream;
+    const size_t lds = r->mode == 0 ? (size_t)256 * (6 * sizeof(T) + 3) : 0;
+    TRY(upload_cold<T>(r));
+    if (r->kernel != 0 && r->units_dirty) TRY(build_units(r));
+    StepArgs<T> A = args_of<T>(r);
+    A.flags &= ~16;
+    const int p = r->p, q = 1 - p;
+    A.ein = r->field<T>(p, 0);
+    A.hin = r->field<T>(p, 3);
+    A.eout = r->field<T>(q, 0);
+    A.hout = r->field<T>(q, 3);
+    A.cold = r->cold.as<StepCold<T>>() + p;
+    A.wave_step = 0;
+    const SideStream side = {r->side, r->fork_ev, r->join_ev};
+    const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
+    const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab};
+    for (int n = 0; n <= trials; n++) {
+        if (n == 1) HIPCHK(hipEventRecord(r->ev0, s));
+        if (r->mode == 0)
+            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc)));
+        else
+            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc)));
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(r->ev1, s));
+    HIPCHK(hipEventSynchronize(r->ev1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, r->ev0, r->ev1));
+    *ms = trials ? t / trials : 0.0;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_autotune(pxfdtd_resident *r, int32_t trials, int32_t max_rows,
+                                        int32_t *rows, int32_t *planes, double *best_ms) {
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    *best_ms = 0.0;
+    // dispersive values update their state in place: a trial would advance it, so those keep
+    // the defaults; so do the kernels other than the aligned march
+    if (r->ngroups || r->kernel != 3 || trials < 1) {
+        *rows = r->ty;
+        *planes = r->kz;
+        return PXFDTD_OK;
+    }
+    std::vector<std::pair<int, int>> cand;
+    for (int ry : {1, 2, 4}) {
+        if (ry > std::max(1, (int)max_rows)) continue;
+        for (int kz : {4, 8, 16}) {
+            const int k = std::min(kz, r->nz);
+            if (std::find(cand.begin(), cand.end(), std::make_pair(ry, k)) == cand.end()) cand.push_back({ry, k});
+        }
+    }
+    double best = 1e300;
+    std::pair<int, int> pick = {r->ty, r->kz};
+    for (auto c : cand) {
+        TRY(pxfdtd_resident_set_tiling(r, c.first, c.second));
+        double ms = 0;
+        TRY(r->prec == PXFDTD_F64 ? trial_ms<double>(r, trials, &ms) : trial_ms<float>(r, trials, &ms));
+        if (ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    TRY(pxfdtd_resident_set_tiling(r, pick.first, pick.second));
+    *rows = pick.first;
+    *planes = pick.second;
+    *best_ms = best;
+    return PXFDTD_OK;
+}
+
 extern "C" int pxfdtd_resident_set_profiling(pxfdtd_resident *r, int32_t on) {
     r->profile = on != 0;
     return PXFDTD_OK;
@@ -1258,3 +1345,6 @@ extern "C" int pxfdtd_resident_step(pxfdtd_resident *r, uint32_t count, const do
     return r->prec == PXFDTD_F64 ? step_t<double>(r, count, waves, phases)
                                  : step_t<float>(r, count, waves, phases);
 }
+
+// ------------------------------------------------------------------ the serving loop
+#include "serving.hpp"

@@ -1,0 +1,660 @@
+// The serving loop: continuous batching of FDTD jobs on one device (pxfdtd_server_*).
+//
+// photonoxide has no counterpart (its nearest is the process-level farm, src/farm.rs:1-56):
+// this is the build's own. Jobs are host-mirror simulations handed over with a step count. A
+// loop thread admits them in arrival order while the jobs resident on the device fit the
+// server's HBM budget and job count, and steps every admitted job together in segments of K
+// steps. Each step of a segment is one launch per kernel class over the units of all jobs (the
+// aligned march's units, four per workgroup, grouped by job), one tap launch over all jobs'
+// monitored values, and per segment one FP64 transform accumulation over all of them. The
+// segment is captured once per job set as a hipGraph and replayed; a job that finishes mid
+// segment is skipped by the kernels from its last step on. Jobs admitted while others run join
+// at the next segment boundary; a finished job's results go back to its simulation and its
+// device memory is freed at once.
+//
+// Every cell is updated by the same device code as when the job runs alone (the march's unit
+// body, the taps, the step-ordered FP64 sums, no atomics), so a job's fields and spectra are
+// the same bits alone and in any mix. Tilings come from pxfdtd_resident_autotune on the first
+// job of each grid shape and are cached per shape.
+#pragma once
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+// the host mirror's side of the server (simulation.cpp)
+namespace pxfdtd_host {
+int sim_footprint(pxfdtd_simulation *s, int32_t precision, size_t *bytes);
+int sim_servable(const pxfdtd_simulation *s);
+int sim_attach(pxfdtd_simulation *s, pxfdtd_device *dev, int32_t precision, pxfdtd_resident **out);
+int sim_emitters(const pxfdtd_simulation *s);
+int64_t sim_steps(const pxfdtd_simulation *s);
+void sim_tables(const pxfdtd_simulation *s, int64_t n0, int count, int slots, double *waves, double *phases);
+void sim_advanced(pxfdtd_simulation *s, int64_t count);
+int sim_detach(pxfdtd_simulation *s);
+}  // namespace pxfdtd_host
+
+namespace pxfdtd {
+
+template <typename T>
+struct JobStep {
+    StepArgs<T> A[2];  // A[q]: reads copy q, writes copy 1 - q, the parameters of parity q
+};
+struct JobSeg {
+    int32_t active;  // steps the job takes this segment (0: none)
+    int32_t p0;      // the copy its first step reads
+};
+struct BatchBlock {
+    int32_t job, first;  // a workgroup's job and its first of four units
+};
+
+template <typename T, int MODE, int RY, int AX>
+__global__ __launch_bounds__(256) void march64_batch(const JobStep<T> *jobs, const JobSeg *seg,
+                                                     const BatchBlock *blocks, const int4 *units,
+                                                     int nblocks, int per_xcd, int s) {
+    __shared__ MarchShared<T> S;
+    extern __shared__ unsigned char smem_raw[];
+    T *tab = reinterpret_cast<T *>(smem_raw);
+    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);
+    const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (b >= nblocks) return;  // the whole workgroup
+    const BatchBlock B = blocks[b];
+    const JobSeg g = seg[B.job];
+    if (s >= g.active) return;  // the whole workgroup: its job is done
+    StepArgs<T> A = jobs[B.job].A[(g.p0 + s) & 1];
+    A.wave_step = s;
+    march64_stage<T, MODE, AX != 0>(A, S, tab, grp);
+    __syncthreads();
+    const int4 u = units[uniform(B.first + (int)(threadIdx.x >> 6))];
+    if (u.w <= u.z) return;  // a padding unit
+    march64_unit<T, MODE, RY, AX>(A, u, S, tab, grp);
+}
+
+template <typename T>
+struct TapJob {
+    const T *fields[2][6];  // copy, then Ex Ey Ez Hx Hy Hz
+    T *staging;             // [K][ntaps]
+    long long ntaps;
+};
+
+// after step s, every monitored value of every job that took it, from the copy it wrote
+template <typename T>
+__global__ void tap_batch(const TapJob<T> *tj, const JobSeg *seg, const int32_t *vjob,
+                          const int64_t *vlocal, const int64_t *vindex, const int8_t *vwhich,
+                          long long n, int s) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int j = vjob[v];
+    const JobSeg g = seg[j];
+    if (s >= g.active) return;
+    const int q = (g.p0 + s + 1) & 1;
+    const TapJob<T> &J = tj[j];
+    J.staging[(long long)s * J.ntaps + vlocal[v]] = J.fields[q][vwhich[v]][vindex[v]];
+}
+
+template <typename T>
+struct DftJob {
+    const T *staged;
+    long long row;
+    const int32_t *series_of;
+    const SeriesInfo *series;
+    const double *phases;  // [K][slots] complex, this segment's
+    int slots;
+    double *sums;
+};
+
+// the segment's FP64 transform sums of every job's values, each in step order as
+// dft_accumulate adds them (the same bits as a job stepped alone)
+template <typename T>
+__global__ void dft_batch(const DftJob<T> *dj, const JobSeg *seg, const int32_t *vjob,
+                          const int64_t *vlocal, long long n) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int j = vjob[v];
+    const int steps = seg[j].active;
+    if (!steps) return;
+    const DftJob<T> &D = dj[j];
+    const long long lv = vlocal[v];
+    const SeriesInfo S = D.series[D.series_of[lv]];
+    const long long local = lv - S.base_v;
+    for (int f = 0; f < S.nf; f++) {
+        const long long o = S.sums_off + (long long)f * S.volume + local;
+        double re = D.sums[2 * o], im = D.sums[2 * o + 1];
+        for (int st = 0; st < steps; st++) {
+            const double F = (double)D.staged[(long long)st * D.row + lv];
+            const double pr = D.phases[2 * ((long long)st * D.slots + S.slot + f)];
+            const double pi = D.phases[2 * ((long long)st * D.slots + S.slot + f) + 1];
+            re += F * pr - 0.0 * pi;
+            im += F * pi + 0.0 * pr;
+        }
+        D.sums[2 * o] = re;
+        D.sums[2 * o + 1] = im;
+    }
+}
+
+}  // namespace pxfdtd
+
+// ------------------------------------------------------------------------------ the server
+struct ServerJob {
+    int64_t id = 0;
+    pxfdtd_simulation *sim = nullptr;
+    int64_t steps = 0, left = 0;
+    int32_t state = PXFDTD_JOB_QUEUED;
+    double t_submit = 0, t_start = 0, t_end = 0;
+    size_t foot = 0;
+    pxfdtd_resident *r = nullptr;
+    int emitters = 0, slots = 0;
+    std::string error;
+    size_t wave_off = 0, phase_off = 0;  // this job's first double in the segment arenas
+};
+
+struct pxfdtd_server {
+    pxfdtd_device *dev = nullptr;
+    pxfdtd_server_config cfg;
+    hipStream_t stream = nullptr;
+    std::thread loop;
+    std::mutex mu;
+    std::condition_variable wake, changed;
+    std::deque<ServerJob *> queue;     // submitted, not admitted
+    std::vector<ServerJob *> running;  // admitted, not finished
+    std::map<int64_t, std::unique_ptr<ServerJob>> jobs;
+    int64_t next_id = 1;
+    bool stop = false;
+    size_t used = 0;
+    std::chrono::steady_clock::time_point t0;
+    std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> tuned;  // shape -> tiling
+    pxfdtd_server_stats stats;
+    // the tables of the job set last built, on the device
+    std::vector<ServerJob *> table;  // table entry -> job (finished ones stay, with no steps)
+    struct Class {
+        int mode, ry, ax;
+        int first, count;  // blocks
+    };
+    std::vector<Class> classes;
+    Buf d_jobs, d_colds, d_seg, d_blocks, d_units, d_waves, d_phases;
+    Buf d_tapjobs, d_tjob, d_tlocal, d_tindex, d_twhich, d_dftjobs, d_djob, d_dlocal;
+    long long ntaps = 0, nvals = 0;
+    double *h_waves = nullptr, *h_phases = nullptr;  // pinned
+    JobSeg *h_seg = nullptr;
+    size_t cap_waves = 0, cap_phases = 0, cap_seg = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static double server_now(const pxfdtd_server *S) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t0).count();
+}
+
+static void drop_graph(pxfdtd_server *S) {
+    if (S->exec) (void)hipGraphExecDestroy(S->exec);
+    if (S->graph) (void)hipGraphDestroy(S->graph);
+    S->exec = nullptr;
+    S->graph = nullptr;
+}
+
+template <typename T>
+static int pinned(T *&p, size_t &cap, size_t n) {
+    if (n <= cap) return PXFDTD_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    HIPCHK(hipHostMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+    cap = n;
+    return PXFDTD_OK;
+}
+
+// a job on the device: its resident, the aligned march and a tiling for its grid's shape
+static int server_admit(pxfdtd_server *S, ServerJob *J) {
+    TRY(pxfdtd_host::sim_attach(J->sim, S->dev, PXFDTD_F32, &J->r));
+    pxfdtd_resident *r = J->r;
+    if (r->kernel != 3) TRY(pxfdtd_resident_set_kernel(r, 3));
+    const auto key = std::make_tuple(r->nx, r->ny, r->nz, r->low[0] + 64 * r->low[1] + 4096 * r->low[2], r->mode);
+    auto it = S->tuned.find(key);
+    if (it != S->tuned.end()) {
+        TRY(pxfdtd_resident_set_tiling(r, it->second.first, it->second.second));
+    } else {
+        int32_t rows = 2, planes = 8;
+        double ms = 0;
+        if (S->cfg.autotune > 0) {
+            TRY(pxfdtd_resident_autotune(r, S->cfg.autotune, 2, &rows, &planes, &ms));
+        } else {
+            TRY(pxfdtd_resident_set_tiling(r, 2, std::min(8, r->nz)));
+            rows = 2;
+            planes = std::min(8, r->nz);
+        }
+        S->tuned[key] = {rows, planes};
+    }
+    if (r->ty > 2) TRY(pxfdtd_resident_set_tiling(r, 2, 0));
+    TRY(build_units(r));
+    J->emitters = std::max(1, pxfdtd_host::sim_emitters(J->sim));
+    J->slots = std::max(1, (int)r->slots);
+    return PXFDTD_OK;
+}
+
+template <int MODE, int RY, int AX>
+static void launch_batch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, int s, size_t lds) {
+    const int per_xcd = (c.count + 7) / 8;
+    hipLaunchKernelGGL((march64_batch<float, MODE, RY, AX>), dim3(8 * per_xcd), dim3(256), lds, S->stream,
+                       S->d_jobs.as<JobStep<float>>(), S->d_seg.as<JobSeg>(),
+                       S->d_blocks.as<BatchBlock>() + c.first, S->d_units.as<int4>(), c.count, per_xcd, s);
+}
+
+static void launch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, int s) {
+    const size_t lds = c.mode == 0 ? (size_t)256 * (6 * sizeof(float) + 3) : 0;
+#define PX_CLASS(M, R, X) \
+    if (c.mode == M && c.ry == R && c.ax == X) return launch_batch_class<M, R, X>(S, c, s, lds);
+    PX_CLASS(0, 1, 0) PX_CLASS(0, 1, 1) PX_CLASS(0, 1, 7) PX_CLASS(0, 2, 0) PX_CLASS(0, 2, 1) PX_CLASS(0, 2, 7)
+    PX_CLASS(1, 1, 0) PX_CLASS(1, 1, 1) PX_CLASS(1, 1, 7) PX_CLASS(1, 2, 0) PX_CLASS(1, 2, 1) PX_CLASS(1, 2, 7)
+#undef PX_CLASS
+}
+
+// one segment's launches on the server's stream (captured into the graph, or run directly)
+static int launch_segment(pxfdtd_server *S) {
+    const int K = S->cfg.segment;
+    for (int s = 0; s < K; s++) {
+        for (const auto &c : S->classes) launch_class(S, c, s);
+        if (S->ntaps) {
+            const long long blocks = (S->ntaps + 255) / 256;
+            hipLaunchKernelGGL((tap_batch<float>), dim3((unsigned)blocks), dim3(256), 0, S->stream,
+                               S->d_tapjobs.as<TapJob<float>>(), S->d_seg.as<JobSeg>(), S->d_tjob.as<int32_t>(),
+                               S->d_tlocal.as<int64_t>(), S->d_tindex.as<int64_t>(), S->d_twhich.as<int8_t>(),
+                               S->ntaps, s);
+        }
+    }
+    if (S->nvals) {
+        const long long blocks = (S->nvals + 255) / 256;
+        hipLaunchKernelGGL((dft_batch<float>), dim3((unsigned)blocks), dim3(256), 0, S->stream,
+                           S->d_dftjobs.as<DftJob<float>>(), S->d_seg.as<JobSeg>(), S->d_djob.as<int32_t>(),
+                           S->d_dlocal.as<int64_t>(), S->nvals);
+    }
+    HIPCHK(hipGetLastError());
+    return PXFDTD_OK;
+}
+
+template <typename V>
+static int upload_vec(Buf &b, const std::vector<V> &v, hipStream_t s) {
+    TRY(b.reserve(std::max<size_t>(v.size(), 1) * sizeof(V)));
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice, s));
+    return PXFDTD_OK;
+}
+
+// the device tables of the running jobs, and the segment's graph
+static int server_rebuild(pxfdtd_server *S) {
+    drop_graph(S);
+    HIPCHK(hipStreamSynchronize(S->stream));
+    S->table = S->running;
+    const int K = S->cfg.segment;
+    const int nj = (int)S->table.size();
+    std::vector<JobStep<float>> steps(nj);
+    std::vector<StepCold<float>> colds(2 * nj);
+    std::vector<TapJob<float>> tapjobs(nj);
+    std::vector<DftJob<float>> dftjobs(nj);
+    std::vector<int32_t> tjob, djob;
+    std::vector<int64_t> tlocal, tindex, dlocal;
+    std::vector<int8_t> twhich;
+    std::map<std::tuple<int, int, int>, std::vector<BatchBlock>> blocks;
+    std::vector<int4> units;
+    size_t wv = 0, ph = 0;
+    for (int j = 0; j < nj; j++) {
+        ServerJob *J = S->table[j];
+        J->wave_off = wv;
+        J->phase_off = ph;
+        wv += (size_t)K * J->emitters * 2;
+        ph += (size_t)K * J->slots * 2;
+    }
+    TRY(S->d_waves.reserve(std::max<size_t>(wv, 1) * 8));
+    TRY(S->d_phases.reserve(std::max<size_t>(ph, 1) * 8));
+    TRY(S->d_colds.reserve(std::max<size_t>(colds.size(), 1) * sizeof(StepCold<float>)));
+    for (int j = 0; j < nj; j++) {
+        ServerJob *J = S->table[j];
+        pxfdtd_resident *r = J->r;
+        if (r->units_dirty) TRY(build_units(r));
+        fill_cold<float>(r, &colds[2 * j]);
+        for (int q = 0; q < 2; q++) {
+            colds[2 * j + q].waves = S->d_waves.as<double>() + J->wave_off;
+            StepArgs<float> A = args_of<float>(r);
+            A.ein = r->field<float>(q, 0);
+            A.hin = r->field<float>(q, 3);
+            A.eout = r->field<float>(1 - q, 0);
+            A.hout = r->field<float>(1 - q, 3);
+            A.cold = S->d_colds.as<StepCold<float>>() + 2 * j + q;
+            A.units = nullptr;
+            A.wave_step = 0;
+            steps[j].A[q] = A;
+        }
+        // the units, four per workgroup, by class: no slab, x slabs only, the others
+        const int counts[3] = {r->n_lean, r->n_x, r->n_slab}, axes[3] = {0, 1, 7};
+        int at = 0;
+        for (int c = 0; c < 3; c++) {
+            auto &bl = blocks[std::make_tuple(r->mode, r->ty, axes[c])];
+            for (int u = 0; u < counts[c]; u += 4) {
+                bl.push_back({j, (int32_t)units.size()});
+                for (int w = 0; w < 4; w++)
+                    units.push_back(u + w < counts[c] ? r->units_h[at + u + w] : make_int4(0, 0, 0, 0));
+            }
+            at += counts[c];
+        }
+        // taps and transform values
+        TapJob<float> &tj = tapjobs[j];
+        for (int q = 0; q < 2; q++)
+            for (int f = 0; f < 6; f++) tj.fields[q][f] = r->field<float>(q, f);
+        tj.staging = r->staging.as<float>();
+        tj.ntaps = r->ntaps;
+        for (long long v = 0; v < r->ntaps; v++) {
+            tjob.push_back(j);
+            tlocal.push_back(v);
+            tindex.push_back(r->tap_index_h[v]);
+            twhich.push_back(r->tap_which_h[v]);
+        }
+        DftJob<float> &dj = dftjobs[j];
+        dj.staged = r->staging.as<float>() + r->nprobes;
+        dj.row = r->ntaps;
+        dj.series_of = r->series_of.as<int32_t>();
+        dj.series = r->series_info.as<SeriesInfo>();
+        dj.phases = S->d_phases.as<double>() + J->phase_off;
+        dj.slots = J->slots;
+        dj.sums = r->sums.as<double>();
+        for (long long v = 0; v < r->nvals; v++) {
+            djob.push_back(j);
+            dlocal.push_back(v);
+        }
+        // the staging must hold a whole segment
+        if ((long long)r->staging.bytes < (long long)K * r->ntaps * 4) {
+            TRY(r->staging.alloc((size_t)K * r->ntaps * 4));
+            tj.staging = r->staging.as<float>();
+            dj.staged = r->staging.as<float>() + r->nprobes;
+        }
+    }
+    std::vector<BatchBlock> all;
+    S->classes.clear();
+    for (auto &kv : blocks) {
+        if (kv.second.empty()) continue;
+        S->classes.push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), (int)all.size(),
+                              (int)kv.second.size()});
+        all.insert(all.end(), kv.second.begin(), kv.second.end());
+    }
+    hipStream_t st = S->stream;
+    TRY(upload_vec(S->d_jobs, steps, st));
+    if (!colds.empty()) HIPCHK(hipMemcpyAsync(S->d_colds.p, colds.data(), colds.size() * sizeof(StepCold<float>), hipMemcpyHostToDevice, st));
+    TRY(upload_vec(S->d_blocks, all, st));
+    TRY(upload_vec(S->d_units, units, st));
+    TRY(upload_vec(S->d_tapjobs, tapjobs, st));
+    TRY(upload_vec(S->d_tjob, tjob, st));
+    TRY(upload_vec(S->d_tlocal, tlocal, st));
+    TRY(upload_vec(S->d_tindex, tindex, st));
+    TRY(upload_vec(S->d_twhich, twhich, st));
+    TRY(upload_vec(S->d_dftjobs, dftjobs, st));
+    TRY(upload_vec(S->d_djob, djob, st));
+    TRY(upload_vec(S->d_dlocal, dlocal, st));
+    TRY(S->d_seg.reserve(std::max<size_t>(nj, 1) * sizeof(JobSeg)));
+    S->ntaps = (long long)tjob.size();
+    S->nvals = (long long)djob.size();
+    TRY(pinned(S->h_waves, S->cap_waves, wv));
+    TRY(pinned(S->h_phases, S->cap_phases, ph));
+    TRY(pinned(S->h_seg, S->cap_seg, (size_t)nj));
+    HIPCHK(hipStreamSynchronize(st));
+    if (!S->cfg.direct) {
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_segment(S);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc != PXFDTD_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        HIPCHK(e);
+        S->graph = g;
+        HIPCHK(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
+        S->stats.captures++;
+    }
+    return PXFDTD_OK;
+}
+
+// one segment: every running job's waves and phases, then the batched steps
+static int server_segment(pxfdtd_server *S) {
+    const int K = S->cfg.segment;
+    const int nj = (int)S->table.size();
+    double cells = 0;
+    for (int j = 0; j < nj; j++) {
+        ServerJob *J = S->table[j];
+        const bool on = J->state == PXFDTD_JOB_RUNNING && J->left > 0;
+        const int a = on ? (int)std::min<int64_t>(K, J->left) : 0;
+        S->h_seg[j] = {a, on ? J->r->p : 0};
+        if (a)
+            pxfdtd_host::sim_tables(J->sim, pxfdtd_host::sim_steps(J->sim), a, J->slots, S->h_waves + J->wave_off,
+                                    S->h_phases + J->phase_off);
+        if (a) cells += (double)a * J->r->ncells;
+    }
+    hipStream_t st = S->stream;
+    if (S->cap_waves) HIPCHK(hipMemcpyAsync(S->d_waves.p, S->h_waves, S->cap_waves * 8, hipMemcpyHostToDevice, st));
+    if (S->cap_phases) HIPCHK(hipMemcpyAsync(S->d_phases.p, S->h_phases, S->cap_phases * 8, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(S->d_seg.p, S->h_seg, (size_t)nj * sizeof(JobSeg), hipMemcpyHostToDevice, st));
+    const auto t = std::chrono::steady_clock::now();
+    if (S->exec)
+        HIPCHK(hipGraphLaunch(S->exec, st));
+    else
+        TRY(launch_segment(S));
+    HIPCHK(hipStreamSynchronize(st));
+    S->stats.busy_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
+    S->stats.segments++;
+    S->stats.cell_updates += cells;
+    for (int j = 0; j < nj; j++) {
+        ServerJob *J = S->table[j];
+        const int a = S->h_seg[j].active;
+        if (!a) continue;
+        J->r->p = (S->h_seg[j].p0 + a) & 1;
+        J->left -= a;
+        pxfdtd_host::sim_advanced(J->sim, a);
+    }
+    return PXFDTD_OK;
+}
+
+static void server_finish(pxfdtd_server *S, ServerJob *J, int32_t state, const std::string &why) {
+    J->state = state;
+    J->error = why;
+    J->t_end = server_now(S);
+    if (state == PXFDTD_JOB_DONE) S->stats.completed++;
+    if (state == PXFDTD_JOB_FAILED) S->stats.failed++;
+    S->changed.notify_all();
+}
+
+static void server_loop(pxfdtd_server *S) {
+    (void)hipSetDevice(S->dev->ordinal);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        // admit in arrival order while the budget and the job count allow
+        bool admitted = false;
+        while (!S->stop && !S->queue.empty() && (int)S->running.size() < S->cfg.max_jobs) {
+            ServerJob *J = S->queue.front();
+            if (S->used + J->foot > (size_t)S->cfg.hbm_budget) break;
+            S->queue.pop_front();
+            J->state = PXFDTD_JOB_RUNNING;
+            J->t_start = server_now(S);
+            lk.unlock();
+            const int rc = server_admit(S, J);
+            const std::string why = rc ? pxfdtd_last_error() : "";
+            if (rc) (void)pxfdtd_host::sim_detach(J->sim);
+            lk.lock();
+            if (rc) {
+                server_finish(S, J, PXFDTD_JOB_FAILED, why);
+                continue;
+            }
+            S->running.push_back(J);
+            S->used += J->foot;
+            S->stats.admitted++;
+            S->stats.hbm_in_use = (int64_t)S->used;
+            S->stats.peak_jobs = std::max<int64_t>(S->stats.peak_jobs, (int64_t)S->running.size());
+            admitted = true;
+        }
+        if (S->running.empty()) {
+            if (S->stop) break;
+            S->wake.wait(lk);
+            continue;
+        }
+        if (S->stop) {
+            for (ServerJob *J : S->running) {
+                lk.unlock();
+                (void)pxfdtd_host::sim_detach(J->sim);
+                lk.lock();
+                server_finish(S, J, PXFDTD_JOB_CANCELLED, "the server stopped");
+            }
+            S->running.clear();
+            break;
+        }
+        lk.unlock();
+        int rc = PXFDTD_OK;
+        if (admitted) rc = server_rebuild(S);
+        if (rc == PXFDTD_OK) rc = server_segment(S);
+        const std::string why = rc ? pxfdtd_last_error() : "";
+        lk.lock();
+        // retire finished jobs (or every job, after a device error): results to the host and
+        // their device memory freed; the tables keep their entries, with no steps, until the
+        // next admission rebuilds them
+        for (auto it = S->running.begin(); it != S->running.end();) {
+            ServerJob *J = *it;
+            if (rc == PXFDTD_OK && J->left > 0) {
+                ++it;
+                continue;
+            }
+            lk.unlock();
+            const int rc2 = pxfdtd_host::sim_detach(J->sim);
+            const std::string why2 = rc2 ? pxfdtd_last_error() : "";
+            lk.lock();
+            J->r = nullptr;
+            S->used -= J->foot;
+            if (rc != PXFDTD_OK)
+                server_finish(S, J, PXFDTD_JOB_FAILED, why);
+            else if (rc2 != PXFDTD_OK)
+                server_finish(S, J, PXFDTD_JOB_FAILED, why2);
+            else
+                server_finish(S, J, PXFDTD_JOB_DONE, "");
+            it = S->running.erase(it);
+        }
+        S->stats.hbm_in_use = (int64_t)S->used;
+        if (rc != PXFDTD_OK) {
+            S->table.clear();
+            lk.unlock();
+            drop_graph(S);
+            lk.lock();
+        }
+    }
+}
+
+extern "C" int pxfdtd_server_create(pxfdtd_device *dev, const pxfdtd_server_config *cfg, pxfdtd_server **out) {
+    HIPCHK(hipSetDevice(dev->ordinal));
+    auto S = std::make_unique<pxfdtd_server>();
+    S->dev = dev;
+    std::memset(&S->cfg, 0, sizeof S->cfg);
+    if (cfg) S->cfg = *cfg;
+    std::memset(&S->stats, 0, sizeof S->stats);
+    if (S->cfg.segment < 0 || S->cfg.max_jobs < 0 || S->cfg.hbm_budget < 0 || S->cfg.autotune < 0)
+        return px_fail(PXFDTD_ERR_INVALID, "a server setting is negative");
+    if (!S->cfg.segment) S->cfg.segment = 32;
+    if (S->cfg.segment > 4096) return px_fail(PXFDTD_ERR_INVALID, "segments of at most 4096 steps");
+    if (!S->cfg.max_jobs) S->cfg.max_jobs = 256;
+    if (!S->cfg.hbm_budget) {
+        size_t free_b = 0, total = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total));
+        S->cfg.hbm_budget = (int64_t)(0.9 * (double)free_b);
+    }
+    HIPCHK(hipStreamCreate(&S->stream));
+    S->t0 = std::chrono::steady_clock::now();
+    pxfdtd_server *raw = S.release();
+    raw->loop = std::thread(server_loop, raw);
+    *out = raw;
+    return PXFDTD_OK;
+}
+
+extern "C" void pxfdtd_server_destroy(pxfdtd_server *S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        for (ServerJob *J : S->queue) server_finish(S, J, PXFDTD_JOB_CANCELLED, "the server stopped");
+        S->queue.clear();
+    }
+    S->wake.notify_all();
+    if (S->loop.joinable()) S->loop.join();
+    drop_graph(S);
+    if (S->h_waves) (void)hipHostFree(S->h_waves);
+    if (S->h_phases) (void)hipHostFree(S->h_phases);
+    if (S->h_seg) (void)hipHostFree(S->h_seg);
+    hipStream_t st = S->stream;
+    delete S;  // device tables free themselves
+    if (st) (void)hipStreamDestroy(st);
+}
+
+extern "C" int pxfdtd_server_submit(pxfdtd_server *S, pxfdtd_simulation *sim, int64_t steps, int64_t *job) {
+    if (steps < 0) return px_fail(PXFDTD_ERR_INVALID, "a negative step count");
+    if (pxfdtd_simulation_resident(sim))
+        return px_fail(PXFDTD_ERR_INVALID, "hand the server a simulation with no device: it sets one on admission");
+    TRY(pxfdtd_host::sim_servable(sim));
+    size_t foot = 0;
+    TRY(pxfdtd_host::sim_footprint(sim, PXFDTD_F32, &foot));
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) return px_fail(PXFDTD_ERR_INVALID, "the server is stopping");
+    if (foot > (size_t)S->cfg.hbm_budget)
+        return px_fail(PXFDTD_ERR_MEMORY, "the job needs %zu bytes on the device, more than the server's budget of %lld",
+                       foot, (long long)S->cfg.hbm_budget);
+    for (auto &kv : S->jobs)
+        if (kv.second->sim == sim && kv.second->state <= PXFDTD_JOB_RUNNING)
+            return px_fail(PXFDTD_ERR_INVALID, "the simulation is already a queued or running job");
+    auto J = std::make_unique<ServerJob>();
+    J->id = S->next_id++;
+    J->sim = sim;
+    J->steps = J->left = steps;
+    J->foot = foot;
+    J->t_submit = server_now(S);
+    ServerJob *raw = J.get();
+    S->jobs[raw->id] = std::move(J);
+    S->queue.push_back(raw);
+    S->stats.submitted++;
+    *job = raw->id;
+    S->wake.notify_all();
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_server_state(pxfdtd_server *S, int64_t job, int32_t *state) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return px_fail(PXFDTD_ERR_INVALID, "no job %lld", (long long)job);
+    *state = it->second->state;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_server_wait(pxfdtd_server *S, int64_t job, double timeout_seconds) {
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return px_fail(PXFDTD_ERR_INVALID, "no job %lld", (long long)job);
+    ServerJob *J = it->second.get();
+    auto done = [&] { return J->state >= PXFDTD_JOB_DONE; };
+    if (timeout_seconds > 0) {
+        if (!S->changed.wait_for(lk, std::chrono::duration<double>(timeout_seconds), done))
+            return px_fail(PXFDTD_ERR_TIMEOUT, "job %lld is not done after %g s", (long long)job, timeout_seconds);
+    } else {
+        S->changed.wait(lk, done);
+    }
+    if (J->state != PXFDTD_JOB_DONE)
+        return px_fail(J->state == PXFDTD_JOB_FAILED ? PXFDTD_ERR_DEVICE : PXFDTD_ERR_INVALID, "job %lld %s: %s",
+                       (long long)job, J->state == PXFDTD_JOB_FAILED ? "failed" : "was cancelled", J->error.c_str());
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_server_times(pxfdtd_server *S, int64_t job, double *submitted, double *started,
+                                   double *finished) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return px_fail(PXFDTD_ERR_INVALID, "no job %lld", (long long)job);
+    *submitted = it->second->t_submit;
+    *started = it->second->t_start;
+    *finished = it->second->t_end;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_server_get_stats(pxfdtd_server *S, pxfdtd_server_stats *out) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->stats;
+    return PXFDTD_OK;
+}

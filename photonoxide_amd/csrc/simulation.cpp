@@ -710,51 +710,57 @@ static int sync_host(pxfdtd_simulation *s) {
     return PXFDTD_OK;
 }
 
+// the device problem of the simulation (what Resident::new reads); `media` holds the media
+// records p points into
+static void make_problem(const pxfdtd_simulation *s, int precision, pxfdtd_problem &p,
+                         std::vector<pxfdtd_medium> &media) {
+    std::memset(&p, 0, sizeof p);
+    p.nx = s->n[0];
+    p.ny = s->n[1];
+    p.nz = s->n[2];
+    p.precision = precision;
+    p.dt = s->dt;
+    p.step[0] = s->g.dx;
+    p.step[1] = s->g.dy;
+    p.step[2] = s->g.dz;
+    for (int a = 0; a < 3; a++) {
+        p.periodic[a] = s->b.periodic[a];
+        p.pml_low[a] = s->b.periodic[a] ? 0 : s->b.low[a];
+        p.pml_high[a] = s->b.periodic[a] ? 0 : s->b.high[a];
+        p.factors_e[a] = s->fe[a].data();
+        p.factors_h[a] = s->fh[a].data();
+        for (int sd = 0; sd < 2; sd++)
+            for (int f = 0; f < 2; f++) {
+                p.cpml_b[a][sd][f] = s->cpml_b[a][sd][f].empty() ? nullptr : s->cpml_b[a][sd][f].data();
+                p.cpml_a[a][sd][f] = s->cpml_a[a][sd][f].empty() ? nullptr : s->cpml_a[a][sd][f].data();
+            }
+    }
+    p.ca = s->ca.data();
+    p.cb = s->cb.data();
+    media.clear();
+    for (size_t g = 0; g < s->groups.size(); g++) {
+        const Group &G = s->groups[g];
+        pxfdtd_medium m;
+        std::memset(&m, 0, sizeof m);
+        m.poles = G.poles;
+        m.chi = G.chi;
+        std::memcpy(m.coef, G.coef, sizeof m.coef);
+        m.count = (int64_t)G.r.size();
+        m.component = G.comp.data();
+        m.index = G.r.data();
+        // media with no terms left are plain values
+        if (m.count && m.poles) media.push_back(m);
+    }
+    p.nmedia = (int32_t)media.size();
+    p.media = media.data();
+}
+
 static int ensure_resident(pxfdtd_simulation *s) {
     if (!s->dev) return px_fail(PXFDTD_ERR_UNSUPPORTED, "no device: set one with pxfdtd_simulation_set_device");
     if (!s->res) {
         pxfdtd_problem p;
-        std::memset(&p, 0, sizeof p);
-        p.nx = s->n[0];
-        p.ny = s->n[1];
-        p.nz = s->n[2];
-        p.precision = s->precision;
-        p.dt = s->dt;
-        p.step[0] = s->g.dx;
-        p.step[1] = s->g.dy;
-        p.step[2] = s->g.dz;
-        for (int a = 0; a < 3; a++) {
-            p.periodic[a] = s->b.periodic[a];
-            p.pml_low[a] = s->b.periodic[a] ? 0 : s->b.low[a];
-            p.pml_high[a] = s->b.periodic[a] ? 0 : s->b.high[a];
-            p.factors_e[a] = s->fe[a].data();
-            p.factors_h[a] = s->fh[a].data();
-            for (int sd = 0; sd < 2; sd++)
-                for (int f = 0; f < 2; f++) {
-                    p.cpml_b[a][sd][f] = s->cpml_b[a][sd][f].empty() ? nullptr : s->cpml_b[a][sd][f].data();
-                    p.cpml_a[a][sd][f] = s->cpml_a[a][sd][f].empty() ? nullptr : s->cpml_a[a][sd][f].data();
-                }
-        }
-        p.ca = s->ca.data();
-        p.cb = s->cb.data();
-        std::vector<pxfdtd_medium> media(s->groups.size());
-        for (size_t g = 0; g < s->groups.size(); g++) {
-            const Group &G = s->groups[g];
-            pxfdtd_medium &m = media[g];
-            std::memset(&m, 0, sizeof m);
-            m.poles = G.poles;
-            m.chi = G.chi;
-            std::memcpy(m.coef, G.coef, sizeof m.coef);
-            m.count = (int64_t)G.r.size();
-            m.component = G.comp.data();
-            m.index = G.r.data();
-        }
-        // media with no terms left are plain values
         std::vector<pxfdtd_medium> live;
-        for (auto &m : media)
-            if (m.count && m.poles) live.push_back(m);
-        p.nmedia = (int32_t)live.size();
-        p.media = live.data();
+        make_problem(s, s->precision, p, live);
         TRY(pxfdtd_resident_create(s->dev, &p, &s->res));
         s->host_newer = true;
         s->inj_dirty = s->taps_dirty = true;
@@ -839,6 +845,50 @@ static int read_probes(pxfdtd_simulation *s) {
 
 // Simulation::run on the device (src/fdtd.rs:1142-1158, resident.rs:430-539): waveforms and
 // phases in FP64 on the host, in chunks of steps
+// each emitter's w and each transform phase for `m` steps from step n0, in FP64 as the
+// reference's host computes them (resident.rs:797-843): waves [m][max(1, emitters)][2] at the
+// field's time (H at n dt, E at (n + 1/2) dt), phases [m][max(1, slots)][2], e^(i w t) dt with E
+// at (n + 1) dt and H at (n + 1/2) dt
+static void fill_tables(const pxfdtd_simulation *s, int64_t n0, int64_t m, int slots, double *waves,
+                        double *phases) {
+    const int ew = std::max(1, s->emitters), sw = std::max(1, slots);
+    for (int64_t q = 0; q < m; q++) {
+        const long long n = n0 + q;
+        const double t = (double)n * s->dt;
+        const double half = t + 0.5 * s->dt;
+        int em = 0;
+        double *w = waves + (size_t)q * ew * 2;
+        w[0] = w[1] = 0.0;
+        for (auto &so : s->sources) {
+            w[2 * em] = wave_at(so.w, so.field == PXFDTD_H ? t : half);
+            w[2 * em + 1] = 0.0;
+            em++;
+        }
+        for (auto &cu : s->currents) {
+            const C64 v = wave_complex_at(cu.w, cu.field == PXFDTD_H ? t : half);
+            w[2 * em] = v.re;
+            w[2 * em + 1] = v.im;
+            em++;
+        }
+        const double te = (double)(n + 1) * s->dt;
+        const double th = te - 0.5 * s->dt;
+        double *ph = phases + (size_t)q * sw * 2;
+        ph[0] = ph[1] = 0.0;
+        int slot = 0;
+        for (auto &d : s->dfts) {
+            for (int f2 = 0; f2 < 2; f2++)
+                for (double fr : d.freqs) {
+                    const C64 p = pxfdtd::pcexp({0.0, TAU * fr * (f2 == 0 ? te : th)});
+                    ph[2 * slot] = p.re * s->dt;
+                    ph[2 * slot + 1] = p.im * s->dt;
+                    slot++;
+                }
+        }
+    }
+}
+
+// Simulation::run on the device (src/fdtd.rs:1142-1158, resident.rs:430-539): waveforms and
+// phases in FP64 on the host, in chunks of steps
 extern "C" int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count) {
     if (count < 0) return px_fail(PXFDTD_ERR_INVALID, "a negative step count");
     TRY(ensure_resident(s));
@@ -849,37 +899,7 @@ extern "C" int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count) {
         const int64_t m = std::min(chunk, count - done);
         waves.assign((size_t)m * std::max(1, s->emitters) * 2, 0.0);
         phases.assign((size_t)m * std::max(1, slots) * 2, 0.0);
-        for (int64_t q = 0; q < m; q++) {
-            const long long n = s->steps + done + q;
-            const double t = (double)n * s->dt;
-            const double half = t + 0.5 * s->dt;
-            int em = 0;
-            double *w = waves.data() + (size_t)q * std::max(1, s->emitters) * 2;
-            for (auto &so : s->sources) {
-                w[2 * em] = wave_at(so.w, so.field == PXFDTD_H ? t : half);
-                w[2 * em + 1] = 0.0;
-                em++;
-            }
-            for (auto &cu : s->currents) {
-                const C64 v = wave_complex_at(cu.w, cu.field == PXFDTD_H ? t : half);
-                w[2 * em] = v.re;
-                w[2 * em + 1] = v.im;
-                em++;
-            }
-            const double te = (double)(n + 1) * s->dt;
-            const double th = te - 0.5 * s->dt;
-            double *ph = phases.data() + (size_t)q * std::max(1, slots) * 2;
-            int slot = 0;
-            for (auto &d : s->dfts) {
-                for (int f2 = 0; f2 < 2; f2++)
-                    for (double fr : d.freqs) {
-                        const C64 p = pxfdtd::pcexp({0.0, TAU * fr * (f2 == 0 ? te : th)});
-                        ph[2 * slot] = p.re * s->dt;
-                        ph[2 * slot + 1] = p.im * s->dt;
-                        slot++;
-                    }
-            }
-        }
+        fill_tables(s, s->steps + done, m, slots, waves.data(), phases.data());
         TRY(pxfdtd_resident_step(s->res, (uint32_t)m, waves.data(), phases.data()));
         done += m;
         TRY(read_probes(s));
@@ -950,5 +970,71 @@ extern "C" int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t se
         at += series * len;
         for (size_t q = 0; q < len; q++) out[q] += sums[at + q];
     }
+    return PXFDTD_OK;
+}
+
+// ---------------------------------------------------------------- for the serving loop
+// (resident.hip's server drives simulations through these; not part of the C ABI)
+namespace pxfdtd_host {
+
+int sim_footprint(pxfdtd_simulation *s, int32_t precision, size_t *bytes) {
+    TRY(sync_host(s));
+    pxfdtd_problem p;
+    std::vector<pxfdtd_medium> media;
+    make_problem(s, precision, p, media);
+    TRY(pxfdtd_resident_footprint(&p, bytes));
+    // taps, staging and FP64 sums of the transform monitors
+    long long vals = 0, sums = 0;
+    for (auto &d : s->dfts) {
+        long long v = 1;
+        for (int a = 0; a < 3; a++) v *= d.hi[a] - d.lo[a];
+        vals += 6 * v;
+        sums += 6 * v * (long long)d.freqs.size();
+    }
+    const size_t esz = precision == PXFDTD_F64 ? 8 : 4;
+    *bytes += (size_t)vals * (9 + 32 * esz + 4) + (size_t)sums * 16;
+    return PXFDTD_OK;
+}
+
+int sim_servable(const pxfdtd_simulation *s) {
+    if (!s->probes.empty())
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the server records transform monitors, not probes: a job with probes runs alone");
+    return PXFDTD_OK;
+}
+
+int sim_attach(pxfdtd_simulation *s, pxfdtd_device *dev, int32_t precision, pxfdtd_resident **out) {
+    TRY(pxfdtd_simulation_set_device(s, dev, precision));
+    *out = s->res;
+    return PXFDTD_OK;
+}
+
+int sim_emitters(const pxfdtd_simulation *s) { return s->emitters; }
+int64_t sim_steps(const pxfdtd_simulation *s) { return s->steps; }
+
+void sim_tables(const pxfdtd_simulation *s, int64_t n0, int count, int slots, double *waves, double *phases) {
+    fill_tables(s, n0, count, slots, waves, phases);
+}
+
+void sim_advanced(pxfdtd_simulation *s, int64_t count) {
+    s->steps += count;
+    if (count) s->device_newer = true;
+}
+
+// the results to the host and the device state freed: the simulation has no device again
+int sim_detach(pxfdtd_simulation *s) {
+    TRY(sync_host(s));
+    drop_resident(s);
+    s->dev = nullptr;
+    return PXFDTD_OK;
+}
+
+}  // namespace pxfdtd_host
+
+extern "C" int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precision, int64_t *bytes) {
+    if (precision != PXFDTD_F32 && precision != PXFDTD_F64)
+        return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");
+    size_t b = 0;
+    TRY(pxfdtd_host::sim_footprint(s, precision, &b));
+    *bytes = (int64_t)b;
     return PXFDTD_OK;
 }

@@ -421,6 +421,24 @@ class Simulation:
     def resident(self):
         return N.lib.pxfdtd_simulation_resident(self._h)
 
+    def footprint(self, precision: "Precision" = None) -> int:
+        """Device bytes this simulation needs (what the server's admission counts)."""
+        b = C.c_int64()
+        p = Precision.F32 if precision is None else precision
+        N.check(N.lib.pxfdtd_simulation_footprint(self._h, int(p), C.byref(b)))
+        return b.value
+
+    def autotune(self, trials: int = 3, max_rows: int = 4):
+        """Times the aligned march's tilings on the current fields without stepping and keeps
+        the fastest: (rows, planes, ms per step kernel)."""
+        r = self.resident()
+        if not r:
+            raise N.NativeError(N.ERR_UNSUPPORTED, "no device")
+        rows, planes, ms = C.c_int32(), C.c_int32(), C.c_double()
+        N.check(N.lib.pxfdtd_resident_autotune(r, int(trials), int(max_rows), C.byref(rows),
+                                               C.byref(planes), C.byref(ms)))
+        return rows.value, planes.value, ms.value
+
     def set_tiling(self, rows=0, planes=0):
         r = self.resident()
         if not r:
