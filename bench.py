@@ -121,6 +121,79 @@ def copy_rate(mib, steps, warmup):
             "gb_per_s": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
 
 
+def serving(gpu, n2d, n3d, rate, seed=2, autotune=2, direct=False):
+    """The batched-serving workload: a seeded Poisson stream of 2D (U[128,1024]^2) and 3D
+    (U[64,256]^3) design jobs, 1000-4000 steps, 5-frequency monitors, f32. Each job first runs
+    alone on the plain path (one at a time: the sequential throughput), then the stream goes
+    through the server (the batched throughput and each job's latency, arrival to results on
+    the host), and every served spectrum is compared with its job's alone run."""
+    import threading
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import serving_jobs as SJ
+    import photonoxide_amd as P
+    from photonoxide_amd.serving import Server
+    stream = SJ.stream(n2d, n3d, seed, rate=rate)
+    specs = [SJ.job_spec(*n, st, sd) for (_, n, st, sd) in stream]
+    cells = [sp["n"][0] * sp["n"][1] * sp["n"][2] * sp["steps"] for sp in specs]
+    # one job at a time on the plain path
+    alone, t_seq = [], 0.0
+    parts = np.zeros(3)  # set-up on the device, stepping, results back
+    for sp in specs:
+        sim = SJ.product_job(sp)
+        t = time.perf_counter()
+        sim.set_device(gpu, P.Precision.F32)
+        t1 = time.perf_counter()
+        sim.run(sp["steps"])
+        t2 = time.perf_counter()
+        out = SJ.spectra(sim)
+        t3 = time.perf_counter()
+        parts += (t1 - t, t2 - t1, t3 - t2)
+        t_seq += t3 - t
+        alone.append(out)
+        del sim
+    # the stream through the server
+    sims = [SJ.product_job(sp) for sp in specs]
+    srv = Server(gpu, autotune=autotune, direct=direct)
+    ids = [None] * len(specs)
+    t0 = time.perf_counter()
+
+    def arrive():
+        for q, (at, _, _, _) in enumerate(stream):
+            d = at - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            ids[q] = srv.submit(sims[q], specs[q]["steps"])
+    th = threading.Thread(target=arrive)
+    th.start()
+    th.join()
+    lat = []
+    for q in range(len(specs)):
+        srv.wait(ids[q], timeout=600)
+        sub, _, fin = srv.times(ids[q])
+        lat.append(fin - sub)
+    span = max(srv.times(j)[2] for j in ids) - min(srv.times(j)[0] for j in ids)
+    st = srv.stats()
+    srv.close()
+    same = all(np.array_equal(SJ.spectra(s).view(np.uint8), a.view(np.uint8)) for s, a in zip(sims, alone))
+    worst = max(float(np.abs(SJ.spectra(s) - a).max() / max(np.abs(a).max(), 1e-300)) for s, a in zip(sims, alone))
+    total = float(sum(cells))
+    lat = np.array(lat)
+    return {
+        "metric": "batched-job serving throughput (Gcell-updates/s) and p50/p99 latency",
+        "value": round(total / span / 1e9, 3), "unit": "Gcell-updates/s", "higher_is_better": True,
+        "sequential": round(total / t_seq / 1e9, 3),
+        "speedup_vs_sequential": round(t_seq / span, 2),
+        "p50_s": round(float(np.percentile(lat, 50)), 4), "p99_s": round(float(np.percentile(lat, 99)), 4),
+        "jobs": {"2d": n2d, "3d": n3d, "poisson_rate_per_s": rate, "seed": seed},
+        "cell_updates": total, "served_span_s": round(span, 3), "sequential_s": round(t_seq, 3),
+        "sequential_parts_s": {"to_device": round(parts[0], 3), "stepping": round(parts[1], 3),
+                               "results_back": round(parts[2], 3)},
+        "identical_to_alone": bool(same), "max_rel_spectral_diff_vs_alone": worst,
+        "dtype": "f32", "data": "synthetic",
+        "server": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +212,12 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only time a plain device-to-device copy of this many MiB (the reference "
                          "kernel for scaling the HBM byte counters and for the achievable copy rate)")
+    ap.add_argument("--serving", choices=["small", "full"], default=None,
+                    help="the batched-serving workload instead: small is 64 2D + 4 3D jobs, full "
+                         "256 2D + 16 3D")
+    ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
+    ap.add_argument("--direct", action="store_true",
+                    help="serving: launch each segment's kernels directly instead of a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: each box has one MI355X, and "
@@ -147,6 +226,11 @@ def main():
         sys.exit(2)
     if args.copy_mib:
         print(json.dumps(copy_rate(args.copy_mib, args.steps, args.warmup)), flush=True)
+        return
+    if args.serving:
+        import photonoxide_amd as P
+        n2d, n3d = (256, 16) if args.serving == "full" else (64, 4)
+        print(json.dumps(serving(P.Gpu(0), n2d, n3d, args.rate, direct=args.direct)), flush=True)
         return
     import photonoxide_amd as P
     from photonoxide_amd import _native as N

@@ -155,7 +155,9 @@ int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r);
 /* the step kernel: 0 tiles of 64 x rows exchanging H through LDS; 1 strips of 62 x rows per
  * wavefront marching along z with no barrier, loading a plane ahead; 2 the same without the
  * plane-ahead loads; 3 (the default where a field's three components fit in 2 GiB) aligned
- * strips of 64 x rows, the values beside the strip gathered */
+ * strips of 64 x rows, the values beside the strip gathered; 4 (the default for a grid one cell
+ * thick with z periodic) the 2D march: strips of 64 marching along y, stepping only the field
+ * set (TE: Ex, Ey, Hz; TM: Ez, Hx, Hy) that holds values or sources */
 int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind);
 int pxfdtd_resident_get_kernel(const pxfdtd_resident *r);
 /* how the aligned march launches its units: 0 (the default) the units meeting no CPML slab,
@@ -163,8 +165,8 @@ int pxfdtd_resident_get_kernel(const pxfdtd_resident *r);
  * each in a kernel variant compiled for just that psi work; 1 the same with the third on a
  * second stream beside the others; 2 every unit in one launch of the full variant */
 int pxfdtd_resident_set_concurrent(pxfdtd_resident *r, int32_t mode);
-/* the tiling: rows per tile (kernel 0: 2, 4, 8; kernels 1 to 3: 1, 2, 4) and planes per
- * chunk; 0 keeps the current value */
+/* the tiling: rows per tile (kernel 0: 2, 4, 8; kernels 1 to 3: 1, 2, 4; kernel 4: 1) and
+ * planes per chunk (kernel 4: rows per chunk); 0 keeps the current value */
 int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes);
 int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes);
 /* the coefficient storage chosen: 0 uint8 ids into a table, 1 full ca/cb arrays */
@@ -178,7 +180,8 @@ int pxfdtd_resident_set_profiling(pxfdtd_resident *r, int32_t on);
 /* the fused step kernel's average duration over the last pxfdtd_resident_step call, ms */
 double pxfdtd_resident_kernel_ms(const pxfdtd_resident *r);
 /* The autotuner (no counterpart in the reference): times `trials` launches of the aligned
- * march at each tiling of rows {1, 2, 4} (at most max_rows) x planes {4, 8, 16}, on the
+ * march at each tiling of rows {1, 2, 4} (at most max_rows) x planes {4, 8, 16} (the 2D
+ * march: chunks of 2 to 32 rows), on the
  * resident's current fields without taking a step (the parity stays, injections are left
  * out), and keeps the fastest. Problems with dispersive media (whose state a trial would
  * advance) and the other kernels keep their tiling. The tiling chosen and its ms per launch. */
@@ -292,8 +295,8 @@ typedef struct {
     int32_t max_jobs;    /* jobs admitted at once; 0: 256 */
     int32_t segment;     /* steps per batched segment; 0: 32 */
     int32_t direct;      /* 1: launch each segment's kernels directly instead of a hipGraph */
-    int32_t autotune;    /* trial launches per tiling for the first job of each grid shape;
-                            0: the default tiling */
+    int32_t autotune;    /* trial launches per tiling for the first job of each class of grid
+                            shape (2D or 3D, each extent's power of two); 0: default tiling */
 } pxfdtd_server_config;
 enum {
     PXFDTD_JOB_QUEUED = 0,
@@ -309,6 +312,9 @@ typedef struct {
     double cell_updates;        /* cells x steps over all jobs */
     int64_t hbm_in_use;         /* the admitted jobs' footprint now */
     int64_t peak_jobs;          /* the most jobs admitted at once */
+    double admit_seconds;       /* host time putting jobs on the device (tables, uploads, tuning) */
+    double retire_seconds;      /* host time bringing finished jobs' results back */
+    double rebuild_seconds;     /* host time building the batched tables and capturing graphs */
 } pxfdtd_server_stats;
 int pxfdtd_server_create(pxfdtd_device *dev, const pxfdtd_server_config *cfg, pxfdtd_server **out);
 /* stops the loop: queued jobs are cancelled, running ones cancelled after their segment */

@@ -69,7 +69,9 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
                 ("failed", C.c_int64), ("segments", C.c_int64), ("captures", C.c_int64),
                 ("busy_seconds", C.c_double), ("cell_updates", C.c_double),
-                ("hbm_in_use", C.c_int64), ("peak_jobs", C.c_int64)]
+                ("hbm_in_use", C.c_int64), ("peak_jobs", C.c_int64),
+                ("admit_seconds", C.c_double), ("retire_seconds", C.c_double),
+                ("rebuild_seconds", C.c_double)]
 
 
 class Grid(C.Structure):

@@ -18,6 +18,7 @@
 #include "../../include/pxfdtd.h"
 #include "kernels.hpp"
 #include "march64.hpp"
+#include "march2d.hpp"
 
 using namespace pxfdtd;
 
@@ -172,7 +173,9 @@ struct pxfdtd_resident {
     long long probe_rows = 0, probe_cap = 0;
     // stepping
     int p = 0;
-    int kernel = 3;  // 0: LDS tiles (64 x ty); 1, 2: 62-wide march (with, without prefetch); 3: aligned march
+    int kernel = 3;  // 0: LDS tiles (64 x ty); 1, 2: 62-wide march (with, without prefetch); 3: aligned march; 4: 2D march
+    // the 2D march's field sets with values or sources: 1 TE (Ex, Ey, Hz), 2 TM (Ez, Hx, Hy)
+    int fields_set = 3, inj_set = 0;
     int ty = 4, kz = 16;
     double last_ms = 0;
     // register march: work units (strip x, strip y, first plane, end plane), those touching
@@ -227,6 +230,16 @@ static int check_problem(const pxfdtd_problem *p) {
     return PXFDTD_OK;
 }
 
+// The 2D march's rows per chunk alone on the device: a wavefront pays one memory round trip
+// per row, so a small grid takes short chunks (more wavefronts in flight, at the cost of the
+// row below each chunk recomputed); a large one takes up to 16.
+static int default_rows_2d(int nx, int ny) {
+    const long long strips = (nx + SW - 1) / SW;
+    int rows = 16;
+    while (rows > 2 && strips * ((ny + rows - 1) / rows) < 8192) rows /= 2;
+    return std::max(1, std::min(ny, rows));
+}
+
 static int pitch_of(int nx) { return (nx + 63) / 64 * 64; }
 
 extern "C" int pxfdtd_resident_footprint(const pxfdtd_problem *p, size_t *bytes) {
@@ -243,6 +256,42 @@ extern "C" int pxfdtd_resident_footprint(const pxfdtd_problem *p, size_t *bytes)
     if (maxp) b += 3 * nval * esz * (1 + 2 * (size_t)maxp);
     *bytes = b + (1u << 20);
     return PXFDTD_OK;
+}
+
+// a component between the host layout ((k ny + j) nx + i) and the device's pitched one, in
+// parallel over rows
+template <typename T>
+static void to_pitched(const pxfdtd_resident *r, const double *src, T *dst) {
+    const int nx = r->nx, ny = r->ny, nz = r->nz;
+    const long long rows = (long long)ny * nz;
+#pragma omp parallel for schedule(static) if (r->ncells > (1 << 18))
+    for (long long row = 0; row < rows; row++) {
+        const long long k = row / ny, j = row % ny;
+        const double *s = src + row * nx;
+        T *d = dst + k * r->plane + j * r->pitch;
+        for (int i = 0; i < nx; i++) d[i] = (T)s[i];
+    }
+}
+
+template <typename T>
+static void from_pitched(const pxfdtd_resident *r, const T *src, double *dst) {
+    const int nx = r->nx, ny = r->ny, nz = r->nz;
+    const long long rows = (long long)ny * nz;
+#pragma omp parallel for schedule(static) if (r->ncells > (1 << 18))
+    for (long long row = 0; row < rows; row++) {
+        const long long k = row / ny, j = row % ny;
+        const T *s = src + k * r->plane + j * r->pitch;
+        double *d = dst + row * nx;
+        for (int i = 0; i < nx; i++) d[i] = (double)s[i];
+    }
+}
+
+// whether any of n values is not zero
+static bool any_nonzero(const double *v, long long n) {
+    int found = 0;
+#pragma omp parallel for schedule(static) reduction(| : found) if (n > (1 << 20))
+    for (long long q = 0; q < n; q++) found |= v[q] != 0.0;
+    return found != 0;
 }
 
 template <typename T>
@@ -357,14 +406,16 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
     } else {
         std::vector<T> ab(6 * nval, T(0));
         std::vector<int8_t> g(3 * nval, -1);
-        for (int c = 0; c < 3; c++)
-            for (long long q = 0; q < N; q++) {
-                const long long i = q % p->nx, rest = q / p->nx;
-                const long long d = (rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i;
-                ab[c * nval + d] = (T)p->ca[c * N + q];
-                ab[(3 + c) * nval + d] = (T)p->cb[c * N + q];
-                if (!member.empty()) g[c * nval + d] = (int8_t)(member[c * N + q] - 1);
-            }
+        for (int c = 0; c < 3; c++) {
+            to_pitched<T>(r, p->ca + c * N, ab.data() + c * nval);
+            to_pitched<T>(r, p->cb + c * N, ab.data() + (3 + c) * nval);
+            if (!member.empty())
+                for (long long q = 0; q < N; q++) {
+                    const long long i = q % p->nx, rest = q / p->nx;
+                    g[c * nval + (rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i] =
+                        (int8_t)(member[c * N + q] - 1);
+                }
+        }
         TRY(r->cacb.alloc(ab.size() * sizeof(T)));
         HIPCHK(hipMemcpyAsync(r->cacb.p, ab.data(), ab.size() * sizeof(T), hipMemcpyHostToDevice, s));
         if (!member.empty()) {
@@ -420,7 +471,7 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
         }
     }
     r->psi_len = off;
-    if (r->kernel == 3 && off * (long long)sizeof(T) >= (1LL << 31)) r->kernel = 1;  // psi beyond 32-bit offsets
+    if (r->kernel >= 3 && off * (long long)sizeof(T) >= (1LL << 31)) r->kernel = 1;  // psi beyond 32-bit offsets
     if (off >= (1LL << 31)) return px_fail(PXFDTD_ERR_UNSUPPORTED, "the CPML's psi is beyond 32-bit indexing");
     for (int q = 0; q < 2; q++) {
         TRY(r->psi[q].alloc(std::max<long long>(off, 1) * sizeof(T)));
@@ -480,6 +531,7 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
     // chip sees several rounds of wavefronts (8 planes, fewer on small grids)
     r->ty = 2;
     r->kernel = 3 * r->nval * (long long)r->esz < (1LL << 31) ? 3 : 1;
+    if (r->kernel == 3 && p->nz == 1 && p->periodic[2]) r->kernel = 4;  // a 2D grid
     r->concurrent = r->kernel == 3 ? 1 : 0;
     {
         const long long strips = (long long)((p->nx + (r->kernel == 3 ? SW : RW) - 1) / (r->kernel == 3 ? SW : RW)) *
@@ -487,6 +539,10 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
         int kz = std::min(p->nz, 8);
         while (kz > 2 && strips * ((p->nz + kz - 1) / kz) < 4096) kz = (kz + 1) / 2;
         r->kz = std::max(1, kz);
+        if (r->kernel == 4) {
+            r->ty = 1;
+            r->kz = default_rows_2d(p->nx, p->ny);
+        }
     }
     hipError_t e = hipStreamCreate(&r->stream);
     if (e != hipSuccess) {
@@ -544,15 +600,37 @@ extern "C" int pxfdtd_resident_synchronize(pxfdtd_resident *r) {
 template <typename T>
 static int upload_fields(pxfdtd_resident *r, const double *e, const double *h, const double *psi) {
     const long long N = r->ncells;
+    // the other copy and its psi start at zero: a field set the 2D march leaves out is then
+    // zero in both copies
+    HIPCHK(hipMemsetAsync(r->fields[1 - r->p].p, 0, r->fields[1 - r->p].bytes, r->stream));
+    if (r->psi_len) HIPCHK(hipMemsetAsync(r->psi[1 - r->p].p, 0, r->psi[1 - r->p].bytes, r->stream));
+    // which sets hold values: TE is Ex, Ey, Hz (field*3 + comp 0, 1, 5), TM is Ez, Hx, Hy
+    int set = 0;
+    for (int q = 0; q < 6; q++) {
+        const double *src = (q < 3 ? e : h);
+        if (!src) continue;
+        src += (q % 3) * N;
+        const int bit = (q == 0 || q == 1 || q == 5) ? 1 : 2;
+        if (!(set & bit) && any_nonzero(src, N)) set |= bit;
+    }
+    if (psi)
+        for (const SlabDesc &d : r->slabs) {
+            const int q = d.field * 3 + d.comp;
+            const int bit = (q == 0 || q == 1 || q == 5) ? 1 : 2;
+            if (set & bit) continue;
+            for (long long v = 0; v < d.len; v++)
+                if (psi[d.offset + v] != 0.0) {
+                    set |= bit;
+                    break;
+                }
+        }
+    r->fields_set = set;
     std::vector<T> buf(r->nval, T(0));
     for (int q = 0; q < 6; q++) {
         const double *src = (q < 3 ? e : h);
         if (!src) continue;
         src += (q % 3) * N;
-        for (long long v = 0; v < N; v++) {
-            const long long i = v % r->nx, rest = v / r->nx;
-            buf[(rest / r->ny) * r->plane + (rest % r->ny) * r->pitch + i] = (T)src[v];
-        }
+        to_pitched<T>(r, src, buf.data());
         HIPCHK(hipMemcpyAsync(r->field<T>(r->p, q), buf.data(), r->nval * sizeof(T),
                               hipMemcpyHostToDevice, r->stream));
         HIPCHK(hipStreamSynchronize(r->stream));
@@ -587,10 +665,7 @@ static int download_fields(pxfdtd_resident *r, double *e, double *h, double *psi
         HIPCHK(hipMemcpyAsync(buf.data(), r->field<T>(r->p, q), r->nval * sizeof(T),
                               hipMemcpyDeviceToHost, r->stream));
         HIPCHK(hipStreamSynchronize(r->stream));
-        for (long long v = 0; v < N; v++) {
-            const long long i = v % r->nx, rest = v / r->nx;
-            dst[v] = (double)buf[(rest / r->ny) * r->plane + (rest % r->ny) * r->pitch + i];
-        }
+        from_pitched<T>(r, buf.data(), dst);
     }
     if (psi && r->psi_len) {
         std::vector<T> ps(r->psi_len);
@@ -618,8 +693,8 @@ extern "C" int pxfdtd_resident_download(pxfdtd_resident *r, double *e, double *h
 // register march: the whole rectangle [i0 - 1, i0 + tw) x [j0 - 1, j0 + th)).
 static void tile_shape(const pxfdtd_resident *r, int &tw, int &th) {
     // kernels 1 and 2: the 62-wide march (with and without prefetch); 3: the aligned march
-    tw = r->kernel == 0 ? TX : (r->kernel == 3 ? SW : RW);
-    th = r->ty;
+    tw = r->kernel == 0 ? TX : (r->kernel >= 3 ? SW : RW);
+    th = r->kernel == 4 ? r->ny : r->ty;  // the 2D march: a strip's lists run over all rows
 }
 
 static bool h_region_has(const pxfdtd_resident *r, int bx, int by, int i, int j) {
@@ -654,9 +729,62 @@ static bool h_region_has(const pxfdtd_resident *r, int bx, int by, int i, int j)
 // Each tile's list holds the injections on its own values and, for H, on the values around
 // it where H is recomputed, sorted by plane, each plane's in the caller's order: a value's
 // injections apply in that order.
+// The 2D march's lists: per strip of 64 columns, per row. A strip holds the injections on its
+// own columns and, for H, on the column left of it (where it recomputes H).
+static int build_injection_lists_2d(pxfdtd_resident *r) {
+    const int tx = (r->nx + SW - 1) / SW, ny = r->ny;
+    struct Item { int strip, j; long long seq; Injection in; };
+    std::vector<Item> items;
+    for (size_t q = 0; q < r->inj_host.size(); q++) {
+        const pxfdtd_injection &s = r->inj_host[q];
+        const long long v = s.index;
+        const int i = (int)(v % r->nx), j = (int)((v / r->nx) % r->ny);
+        Injection in;
+        in.k = 0; in.i = i; in.j = j; in.code = s.field * 4 + s.component; in.emitter = s.emitter;
+        in.pad = 0; in.a_re = s.a_re; in.a_im = s.a_im;
+        items.push_back({i / SW, j, (long long)q, in});
+        if (s.field == PXFDTD_H) {
+            // the strip whose left column this is: i + 1 starts it (or column 0 across a
+            // periodic side, for i = nx - 1)
+            int right = -1;
+            if ((i + 1) % SW == 0 && i + 1 < r->nx) right = (i + 1) / SW;
+            if (i == r->nx - 1 && r->periodic[0]) right = 0;
+            if (right >= 0 && right != i / SW) items.push_back({right, j, (long long)q, in});
+        }
+    }
+    std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+        return a.strip != b.strip ? a.strip < b.strip : (a.j != b.j ? a.j < b.j : a.seq < b.seq);
+    });
+    std::vector<int32_t> off((size_t)tx * (ny + 1), 0);
+    std::vector<Injection> flat(items.size());
+    size_t q = 0;
+    for (int t = 0; t < tx; t++)
+        for (int j = 0; j <= ny; j++) {
+            off[(size_t)t * (ny + 1) + j] = (int32_t)q;
+            if (j == ny) break;
+            while (q < items.size() && items[q].strip == t && items[q].j == j) {
+                flat[q] = items[q].in;
+                q++;
+            }
+        }
+    TRY(r->inj_off.alloc(off.size() * sizeof(int32_t)));
+    TRY(r->inj.alloc(std::max<size_t>(flat.size(), 1) * sizeof(Injection)));
+    HIPCHK(hipMemcpyAsync(r->inj_off.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, r->stream));
+    if (!flat.empty())
+        HIPCHK(hipMemcpyAsync(r->inj.p, flat.data(), flat.size() * sizeof(Injection), hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return PXFDTD_OK;
+}
+
 static int build_injection_lists(pxfdtd_resident *r) {
     r->have_inj = !r->inj_host.empty();
+    r->inj_set = 0;
+    for (const auto &s : r->inj_host) {
+        const int q = s.field * 3 + s.component;
+        r->inj_set |= (q == 0 || q == 1 || q == 5) ? 1 : 2;
+    }
     if (!r->have_inj) return PXFDTD_OK;
+    if (r->kernel == 4) return build_injection_lists_2d(r);
     int tw, th;
     tile_shape(r, tw, th);
     const int tx = (r->nx + tw - 1) / tw, tyn = (r->ny + th - 1) / th;
@@ -714,7 +842,54 @@ static int build_injection_lists(pxfdtd_resident *r) {
 // The register march's work: plane chunks of about kz planes whose edges fall at the CPML
 // slabs' along z (a chunk also reads the plane below it), crossed with the strips; a unit goes
 // to the slab kernel if any value it computes H or E on lies in a CPML slab.
+// The 2D march's work: strips crossed with chunks of about kz rows whose edges fall at the y
+// slabs' (a chunk also reads the row below it); a unit meeting a CPML slab goes to the slab
+// variant.
+static int build_units_2d(pxfdtd_resident *r) {
+    std::vector<int> cuts = {0};
+    const int ly = r->periodic[1] ? 0 : r->low[1], hy = r->periodic[1] ? 0 : r->high[1], ny = r->ny;
+    auto cut = [&](int y) {
+        if (y > cuts.back() && y < ny) cuts.push_back(y);
+    };
+    if (ly > 0) cut(ly + 1);
+    const int hi_start = hy > 0 ? ny - hy : ny;
+    for (int y = cuts.back() + r->kz; y < hi_start; y += r->kz) cut(y);
+    cut(hi_start);
+    for (int y = cuts.back() + r->kz; y < ny; y += r->kz) cut(y);
+    cuts.push_back(ny);
+    const int tx = (r->nx + SW - 1) / SW;
+    auto in_slab = [](int a, int b, int n, int lo, int hi) {  // [a, b) meets [0, lo) or [n-hi, n)
+        return (lo > 0 && a < lo) || (hi > 0 && b > n - hi);
+    };
+    std::vector<int4> lean, slab;
+    for (size_t c = 0; c + 1 < cuts.size(); c++) {
+        const int a = cuts[c], b = cuts[c + 1];
+        const bool ys = in_slab(a > 0 || r->periodic[1] ? a - 1 : a, b, ny, ly, hy);
+        for (int bx = 0; bx < tx; bx++) {
+            const int i0 = bx * SW;
+            const bool xs = !r->periodic[0] && in_slab(std::max(i0 - 1, 0), std::min(i0 + SW, r->nx), r->nx, r->low[0], r->high[0]);
+            ((xs || ys) ? slab : lean).push_back(make_int4(bx, 0, a, b));
+        }
+    }
+    // a grid too small to fill the chip several times over steps in one launch (the slab
+    // variant's psi work is behind uniform branches): its step time is one launch's latency
+    if ((long long)(lean.size() + slab.size()) < 4LL * 4096) {
+        slab.insert(slab.begin(), lean.begin(), lean.end());
+        lean.clear();
+    }
+    r->n_lean = (int)lean.size();
+    r->n_x = 0;
+    r->n_slab = (int)slab.size();
+    lean.insert(lean.end(), slab.begin(), slab.end());
+    r->units_h = lean;
+    TRY(r->units.alloc(std::max<size_t>(lean.size(), 1) * sizeof(int4)));
+    HIPCHK(hipMemcpy(r->units.p, lean.data(), lean.size() * sizeof(int4), hipMemcpyHostToDevice));
+    r->units_dirty = false;
+    return PXFDTD_OK;
+}
+
 static int build_units(pxfdtd_resident *r) {
+    if (r->kernel == 4) return build_units_2d(r);
     std::vector<int> cuts = {0};
     const int lz = r->low[2], hz = r->high[2], nz = r->nz;
     auto cut = [&](int z) {
@@ -780,7 +955,8 @@ extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_i
 extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes) {
     if (rows) {
         const bool ok = r->kernel == 0 ? (rows == 2 || rows == 4 || rows == 8)
-                                       : (rows == 1 || rows == 2 || rows == 4);  // the march
+                        : r->kernel == 4 ? rows == 1  // the 2D march: one row at a time
+                                         : (rows == 1 || rows == 2 || rows == 4);  // the march
         if (!ok)
             return px_fail(PXFDTD_ERR_INVALID, "rows per tile must be %s for this kernel, got %d",
                            r->kernel == 0 ? "2, 4 or 8" : "1, 2 or 4", rows);
@@ -788,7 +964,7 @@ extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int3
     }
     if (planes) {
         if (planes < 1) return px_fail(PXFDTD_ERR_INVALID, "planes per chunk must be >= 1");
-        r->kz = std::min(planes, r->nz);
+        r->kz = std::min(planes, r->kernel == 4 ? r->ny : r->nz);
     }
     r->units_dirty = true;
     HIPCHK(hipSetDevice(r->dev->ordinal));
@@ -796,11 +972,14 @@ extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int3
 }
 
 extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
-    if (kind < 0 || kind > 3) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0, 1, 2 or 3");
-    if (kind == 3 && (3 * r->nval * (long long)r->esz >= (1LL << 31) || r->psi_len * (long long)r->esz >= (1LL << 31)))
+    if (kind < 0 || kind > 4) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0 to 4");
+    if (kind == 4 && !(r->nz == 1 && r->periodic[2]))
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the 2D march steps grids one cell thick with z periodic");
+    if (kind >= 3 && (3 * r->nval * (long long)r->esz >= (1LL << 31) || r->psi_len * (long long)r->esz >= (1LL << 31)))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the aligned march needs a field's three components within 2 GiB");
     r->kernel = kind;
-    r->ty = kind == 0 ? 8 : 2;
+    r->ty = kind == 0 ? 8 : (kind == 4 ? 1 : 2);
+    if (kind == 4) r->kz = default_rows_2d(r->nx, r->ny);
     r->units_dirty = true;
     HIPCHK(hipSetDevice(r->dev->ordinal));
     return build_injection_lists(r);
@@ -1078,6 +1257,12 @@ struct SideStream {
     hipStream_t s;
     hipEvent_t fork, join;
 };
+
+// the 2D march's field sets to step: those with values or sources (TE if none has either)
+static int step_set(const pxfdtd_resident *r) {
+    const int set = r->fields_set | r->inj_set;
+    return set ? set : 1;
+}
 struct UnitCounts {
     int lean, x, slab;  // units in the lists, in this order
 };
@@ -1117,9 +1302,33 @@ static int launch_march64(const StepArgs<T> &A, hipStream_t s, size_t lds, const
     return PXFDTD_OK;
 }
 
+template <typename T, int MODE, int AX, int SET>
+static void launch_units_2d(const StepArgs<T> &A0, int first, int count, hipStream_t s, size_t lds) {
+    if (!count) return;
+    StepArgs<T> A = A0;
+    A.units = A0.units + first;
+    A.blocks = count;
+    A.per_xcd = ((count + 3) / 4 + 7) / 8;
+    hipLaunchKernelGGL((march2d_step<T, MODE, AX, SET>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+}
+
+template <typename T, int MODE, int SET>
+static int launch_march2d(const StepArgs<T> &A, hipStream_t s, size_t lds, const UnitCounts &n) {
+    launch_units_2d<T, MODE, 3, SET>(A, n.lean, n.slab, s, lds);
+    launch_units_2d<T, MODE, 0, SET>(A, 0, n.lean, s, lds);
+    return PXFDTD_OK;
+}
+
 template <typename T, int MODE>
 static int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, size_t lds,
-                       const SideStream *side, const UnitCounts &n) {
+                       const SideStream *side, const UnitCounts &n, int set = 3) {
+    if (kernel == 4) {
+        switch (set) {
+        case 1: return launch_march2d<T, MODE, 1>(A, s, lds, n);
+        case 2: return launch_march2d<T, MODE, 2>(A, s, lds, n);
+        default: return launch_march2d<T, MODE, 3>(A, s, lds, n);
+        }
+    }
     if (kernel == 3) {
         switch (ty) {
         case 1: return launch_march64<T, MODE, 1>(A, s, lds, side, n);
@@ -1201,10 +1410,11 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         const SideStream side = {r->side, r->fork_ev, r->join_ev};
         const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
         const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab};
+        const int set = step_set(r);
         if (r->mode == 0)
-            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc)));
+            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         else
-            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc)));
+            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         HIPCHK(hipGetLastError());
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
         r->p = q;
@@ -1277,12 +1487,13 @@ static int trial_ms(pxfdtd_resident *r, int trials, double *ms) {
     const SideStream side = {r->side, r->fork_ev, r->join_ev};
     const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
     const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab};
+    const int set = step_set(r);
     for (int n = 0; n <= trials; n++) {
         if (n == 1) HIPCHK(hipEventRecord(r->ev0, s));
         if (r->mode == 0)
-            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc)));
+            TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         else
-            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc)));
+            TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipEventRecord(r->ev1, s));
@@ -1300,14 +1511,20 @@ extern "C" int pxfdtd_resident_autotune(pxfdtd_resident *r, int32_t trials, int3
     *best_ms = 0.0;
     // dispersive values update their state in place: a trial would advance it, so those keep
     // the defaults; so do the kernels other than the aligned march
-    if (r->ngroups || r->kernel != 3 || trials < 1) {
+    if (r->ngroups || (r->kernel != 3 && r->kernel != 4) || trials < 1) {
         *rows = r->ty;
         *planes = r->kz;
         return PXFDTD_OK;
     }
     std::vector<std::pair<int, int>> cand;
+    if (r->kernel == 4) {  // the 2D march: rows per chunk
+        for (int kz : {2, 4, 8, 16, 32}) {
+            const int k = std::min(kz, r->ny);
+            if (std::find(cand.begin(), cand.end(), std::make_pair(1, k)) == cand.end()) cand.push_back({1, k});
+        }
+    }
     for (int ry : {1, 2, 4}) {
-        if (ry > std::max(1, (int)max_rows)) continue;
+        if (r->kernel != 3 || ry > std::max(1, (int)max_rows)) continue;
         for (int kz : {4, 8, 16}) {
             const int k = std::min(kz, r->nz);
             if (std::find(cand.begin(), cand.end(), std::make_pair(ry, k)) == cand.end()) cand.push_back({ry, k});

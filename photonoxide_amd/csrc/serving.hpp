@@ -15,7 +15,8 @@
 // Every cell is updated by the same device code as when the job runs alone (the march's unit
 // body, the taps, the step-ordered FP64 sums, no atomics), so a job's fields and spectra are
 // the same bits alone and in any mix. Tilings come from pxfdtd_resident_autotune on the first
-// job of each grid shape and are cached per shape.
+// job of each class of grid shape (2D or 3D, each extent's power of two) and are cached per
+// class.
 #pragma once
 #include <chrono>
 #include <condition_variable>
@@ -34,6 +35,8 @@ int64_t sim_steps(const pxfdtd_simulation *s);
 void sim_tables(const pxfdtd_simulation *s, int64_t n0, int count, int slots, double *waves, double *phases);
 void sim_advanced(pxfdtd_simulation *s, int64_t count);
 int sim_detach(pxfdtd_simulation *s);
+int sim_fetch(pxfdtd_simulation *s);
+void sim_release(pxfdtd_simulation *s);
 }  // namespace pxfdtd_host
 
 namespace pxfdtd {
@@ -70,6 +73,28 @@ __global__ __launch_bounds__(256) void march64_batch(const JobStep<T> *jobs, con
     const int4 u = units[uniform(B.first + (int)(threadIdx.x >> 6))];
     if (u.w <= u.z) return;  // a padding unit
     march64_unit<T, MODE, RY, AX>(A, u, S, tab, grp);
+}
+
+template <typename T, int MODE, int AX, int SET>
+__global__ __launch_bounds__(256) void march2d_batch(const JobStep<T> *jobs, const JobSeg *seg,
+                                                     const BatchBlock *blocks, const int4 *units,
+                                                     int nblocks, int per_xcd, int s) {
+    __shared__ MarchShared<T> S;
+    extern __shared__ unsigned char smem_raw[];
+    T *tab = reinterpret_cast<T *>(smem_raw);
+    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);
+    const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (b >= nblocks) return;
+    const BatchBlock B = blocks[b];
+    const JobSeg g = seg[B.job];
+    if (s >= g.active) return;
+    StepArgs<T> A = jobs[B.job].A[(g.p0 + s) & 1];
+    A.wave_step = s;
+    march64_stage<T, MODE, AX != 0>(A, S, tab, grp);
+    __syncthreads();
+    const int4 u = units[uniform(B.first + (int)(threadIdx.x >> 6))];
+    if (u.w <= u.z) return;
+    march2d_unit<T, MODE, AX, SET>(A, u, S, tab, grp);
 }
 
 template <typename T>
@@ -169,8 +194,8 @@ struct pxfdtd_server {
     // the tables of the job set last built, on the device
     std::vector<ServerJob *> table;  // table entry -> job (finished ones stay, with no steps)
     struct Class {
-        int mode, ry, ax;
-        int first, count;  // blocks
+        int kind, mode, v, ax;  // v: rows per strip (the aligned march) or field set (2D)
+        int first, count;       // blocks
     };
     std::vector<Class> classes;
     Buf d_jobs, d_colds, d_seg, d_blocks, d_units, d_waves, d_phases;
@@ -181,6 +206,9 @@ struct pxfdtd_server {
     size_t cap_waves = 0, cap_phases = 0, cap_seg = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::chrono::steady_clock::time_point seg_t0;  // the segment in flight
+    double seg_cells = 0;
+    int64_t captures = 0;  // read into stats under the lock
 };
 
 static double server_now(const pxfdtd_server *S) {
@@ -209,10 +237,17 @@ static int pinned(T *&p, size_t &cap, size_t n) {
 static int server_admit(pxfdtd_server *S, ServerJob *J) {
     TRY(pxfdtd_host::sim_attach(J->sim, S->dev, PXFDTD_F32, &J->r));
     pxfdtd_resident *r = J->r;
-    if (r->kernel != 3) TRY(pxfdtd_resident_set_kernel(r, 3));
-    const auto key = std::make_tuple(r->nx, r->ny, r->nz, r->low[0] + 64 * r->low[1] + 4096 * r->low[2], r->mode);
+    if (r->kernel != 3 && r->kernel != 4) TRY(pxfdtd_resident_set_kernel(r, 3));
+    // tilings are cached per class of shape: 2D or 3D, each extent's power of two, the
+    // coefficient storage
+    auto lg = [](int n) { int b = 0; while ((2 << b) <= n) b++; return b; };
+    const auto key = std::make_tuple(r->nz == 1 ? 0 : 1, lg(r->nx), lg(r->ny), lg(r->nz), r->mode);
     auto it = S->tuned.find(key);
-    if (it != S->tuned.end()) {
+    if (r->kernel == 4) {
+        // 2D jobs: in a batch the device is full, so the longest chunks (the least recomputed
+        // rows) win, whatever suits the job alone
+        TRY(pxfdtd_resident_set_tiling(r, 1, std::min(r->ny, 16)));
+    } else if (it != S->tuned.end()) {
         TRY(pxfdtd_resident_set_tiling(r, it->second.first, it->second.second));
     } else {
         int32_t rows = 2, planes = 8;
@@ -220,13 +255,12 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
         if (S->cfg.autotune > 0) {
             TRY(pxfdtd_resident_autotune(r, S->cfg.autotune, 2, &rows, &planes, &ms));
         } else {
-            TRY(pxfdtd_resident_set_tiling(r, 2, std::min(8, r->nz)));
-            rows = 2;
-            planes = std::min(8, r->nz);
+            rows = r->ty;
+            planes = r->kz;
         }
         S->tuned[key] = {rows, planes};
     }
-    if (r->ty > 2) TRY(pxfdtd_resident_set_tiling(r, 2, 0));
+    if (r->kernel == 3 && r->ty > 2) TRY(pxfdtd_resident_set_tiling(r, 2, 0));
     TRY(build_units(r));
     J->emitters = std::max(1, pxfdtd_host::sim_emitters(J->sim));
     J->slots = std::max(1, (int)r->slots);
@@ -241,13 +275,26 @@ static void launch_batch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, 
                        S->d_blocks.as<BatchBlock>() + c.first, S->d_units.as<int4>(), c.count, per_xcd, s);
 }
 
+template <int MODE, int AX, int SET>
+static void launch_batch_class_2d(pxfdtd_server *S, const pxfdtd_server::Class &c, int s, size_t lds) {
+    const int per_xcd = (c.count + 7) / 8;
+    hipLaunchKernelGGL((march2d_batch<float, MODE, AX, SET>), dim3(8 * per_xcd), dim3(256), lds, S->stream,
+                       S->d_jobs.as<JobStep<float>>(), S->d_seg.as<JobSeg>(),
+                       S->d_blocks.as<BatchBlock>() + c.first, S->d_units.as<int4>(), c.count, per_xcd, s);
+}
+
 static void launch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, int s) {
     const size_t lds = c.mode == 0 ? (size_t)256 * (6 * sizeof(float) + 3) : 0;
 #define PX_CLASS(M, R, X) \
-    if (c.mode == M && c.ry == R && c.ax == X) return launch_batch_class<M, R, X>(S, c, s, lds);
+    if (c.kind == 3 && c.mode == M && c.v == R && c.ax == X) return launch_batch_class<M, R, X>(S, c, s, lds);
     PX_CLASS(0, 1, 0) PX_CLASS(0, 1, 1) PX_CLASS(0, 1, 7) PX_CLASS(0, 2, 0) PX_CLASS(0, 2, 1) PX_CLASS(0, 2, 7)
     PX_CLASS(1, 1, 0) PX_CLASS(1, 1, 1) PX_CLASS(1, 1, 7) PX_CLASS(1, 2, 0) PX_CLASS(1, 2, 1) PX_CLASS(1, 2, 7)
 #undef PX_CLASS
+#define PX_CLASS2(M, X, E) \
+    if (c.kind == 4 && c.mode == M && c.ax == X && c.v == E) return launch_batch_class_2d<M, X, E>(S, c, s, lds);
+    PX_CLASS2(0, 0, 1) PX_CLASS2(0, 0, 2) PX_CLASS2(0, 0, 3) PX_CLASS2(0, 3, 1) PX_CLASS2(0, 3, 2) PX_CLASS2(0, 3, 3)
+    PX_CLASS2(1, 0, 1) PX_CLASS2(1, 0, 2) PX_CLASS2(1, 0, 3) PX_CLASS2(1, 3, 1) PX_CLASS2(1, 3, 2) PX_CLASS2(1, 3, 3)
+#undef PX_CLASS2
 }
 
 // one segment's launches on the server's stream (captured into the graph, or run directly)
@@ -294,7 +341,7 @@ static int server_rebuild(pxfdtd_server *S) {
     std::vector<int32_t> tjob, djob;
     std::vector<int64_t> tlocal, tindex, dlocal;
     std::vector<int8_t> twhich;
-    std::map<std::tuple<int, int, int>, std::vector<BatchBlock>> blocks;
+    std::map<std::tuple<int, int, int, int>, std::vector<BatchBlock>> blocks;
     std::vector<int4> units;
     size_t wv = 0, ph = 0;
     for (int j = 0; j < nj; j++) {
@@ -324,11 +371,15 @@ static int server_rebuild(pxfdtd_server *S) {
             A.wave_step = 0;
             steps[j].A[q] = A;
         }
-        // the units, four per workgroup, by class: no slab, x slabs only, the others
-        const int counts[3] = {r->n_lean, r->n_x, r->n_slab}, axes[3] = {0, 1, 7};
+        // the units, four per workgroup, by class: no slab, x slabs only, the others (the 2D
+        // march: no slab, the others)
+        const bool two_d = r->kernel == 4;
+        const int counts[3] = {r->n_lean, r->n_x, r->n_slab};
+        const int axes[3] = {0, 1, two_d ? 3 : 7};
+        const int v = two_d ? step_set(r) : r->ty;
         int at = 0;
         for (int c = 0; c < 3; c++) {
-            auto &bl = blocks[std::make_tuple(r->mode, r->ty, axes[c])];
+            auto &bl = blocks[std::make_tuple(r->kernel, r->mode, v, axes[c])];
             for (int u = 0; u < counts[c]; u += 4) {
                 bl.push_back({j, (int32_t)units.size()});
                 for (int w = 0; w < 4; w++)
@@ -371,8 +422,8 @@ static int server_rebuild(pxfdtd_server *S) {
     S->classes.clear();
     for (auto &kv : blocks) {
         if (kv.second.empty()) continue;
-        S->classes.push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first), (int)all.size(),
-                              (int)kv.second.size()});
+        S->classes.push_back({std::get<0>(kv.first), std::get<1>(kv.first), std::get<2>(kv.first),
+                              std::get<3>(kv.first), (int)all.size(), (int)kv.second.size()});
         all.insert(all.end(), kv.second.begin(), kv.second.end());
     }
     hipStream_t st = S->stream;
@@ -407,39 +458,45 @@ static int server_rebuild(pxfdtd_server *S) {
         HIPCHK(e);
         S->graph = g;
         HIPCHK(hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0));
-        S->stats.captures++;
+        S->captures++;
     }
     return PXFDTD_OK;
 }
 
-// one segment: every running job's waves and phases, then the batched steps
-static int server_segment(pxfdtd_server *S) {
+// one segment: every running job's waves and phases, then the batched steps, launched on the
+// server's stream without waiting (server_segment_end waits)
+static int server_segment_begin(pxfdtd_server *S) {
     const int K = S->cfg.segment;
     const int nj = (int)S->table.size();
     double cells = 0;
     for (int j = 0; j < nj; j++) {
         ServerJob *J = S->table[j];
-        const bool on = J->state == PXFDTD_JOB_RUNNING && J->left > 0;
+        const bool on = J->state == PXFDTD_JOB_RUNNING && J->left > 0 && J->r;
         const int a = on ? (int)std::min<int64_t>(K, J->left) : 0;
         S->h_seg[j] = {a, on ? J->r->p : 0};
-        if (a)
+        if (a) {
             pxfdtd_host::sim_tables(J->sim, pxfdtd_host::sim_steps(J->sim), a, J->slots, S->h_waves + J->wave_off,
                                     S->h_phases + J->phase_off);
-        if (a) cells += (double)a * J->r->ncells;
+            cells += (double)a * J->r->ncells;
+        }
     }
     hipStream_t st = S->stream;
     if (S->cap_waves) HIPCHK(hipMemcpyAsync(S->d_waves.p, S->h_waves, S->cap_waves * 8, hipMemcpyHostToDevice, st));
     if (S->cap_phases) HIPCHK(hipMemcpyAsync(S->d_phases.p, S->h_phases, S->cap_phases * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(S->d_seg.p, S->h_seg, (size_t)nj * sizeof(JobSeg), hipMemcpyHostToDevice, st));
-    const auto t = std::chrono::steady_clock::now();
+    S->seg_t0 = std::chrono::steady_clock::now();
+    S->seg_cells = cells;
     if (S->exec)
         HIPCHK(hipGraphLaunch(S->exec, st));
     else
         TRY(launch_segment(S));
-    HIPCHK(hipStreamSynchronize(st));
-    S->stats.busy_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count();
-    S->stats.segments++;
-    S->stats.cell_updates += cells;
+    return PXFDTD_OK;
+}
+
+static int server_segment_end(pxfdtd_server *S, double *busy) {
+    HIPCHK(hipStreamSynchronize(S->stream));
+    *busy = std::chrono::duration<double>(std::chrono::steady_clock::now() - S->seg_t0).count();
+    const int nj = (int)S->table.size();
     for (int j = 0; j < nj; j++) {
         ServerJob *J = S->table[j];
         const int a = S->h_seg[j].active;
@@ -460,85 +517,136 @@ static void server_finish(pxfdtd_server *S, ServerJob *J, int32_t state, const s
     S->changed.notify_all();
 }
 
+// The loop. Each turn launches a segment of the running jobs and, while the device steps it,
+// puts the newly admitted jobs on the device and brings back the results of the jobs that
+// finished in the previous segment; after the segment it frees those jobs' memory. Newly
+// admitted jobs join at the next turn (the tables and the graph are rebuilt then).
 static void server_loop(pxfdtd_server *S) {
     (void)hipSetDevice(S->dev->ordinal);
+    std::vector<ServerJob *> fetch;  // finished in the last segment, results not yet back
+    bool dirty = false;
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        // admit in arrival order while the budget and the job count allow
-        bool admitted = false;
-        while (!S->stop && !S->queue.empty() && (int)S->running.size() < S->cfg.max_jobs) {
+        // admissions: in arrival order while the budget and the job count allow
+        std::vector<ServerJob *> admit;
+        while (!S->stop && !S->queue.empty() &&
+               (int)(S->running.size() + admit.size() + fetch.size()) < S->cfg.max_jobs) {
             ServerJob *J = S->queue.front();
             if (S->used + J->foot > (size_t)S->cfg.hbm_budget) break;
             S->queue.pop_front();
             J->state = PXFDTD_JOB_RUNNING;
             J->t_start = server_now(S);
-            lk.unlock();
-            const int rc = server_admit(S, J);
-            const std::string why = rc ? pxfdtd_last_error() : "";
-            if (rc) (void)pxfdtd_host::sim_detach(J->sim);
-            lk.lock();
-            if (rc) {
-                server_finish(S, J, PXFDTD_JOB_FAILED, why);
-                continue;
-            }
-            S->running.push_back(J);
             S->used += J->foot;
-            S->stats.admitted++;
-            S->stats.hbm_in_use = (int64_t)S->used;
-            S->stats.peak_jobs = std::max<int64_t>(S->stats.peak_jobs, (int64_t)S->running.size());
-            admitted = true;
+            admit.push_back(J);
         }
-        if (S->running.empty()) {
-            if (S->stop) break;
-            S->wake.wait(lk);
-            continue;
-        }
-        if (S->stop) {
+        if (S->stop && admit.empty() && fetch.empty()) {
             for (ServerJob *J : S->running) {
                 lk.unlock();
                 (void)pxfdtd_host::sim_detach(J->sim);
                 lk.lock();
+                S->used -= J->foot;
                 server_finish(S, J, PXFDTD_JOB_CANCELLED, "the server stopped");
             }
             S->running.clear();
             break;
         }
+        if (S->running.empty() && admit.empty() && fetch.empty()) {
+            S->wake.wait(lk);
+            continue;
+        }
         lk.unlock();
         int rc = PXFDTD_OK;
-        if (admitted) rc = server_rebuild(S);
-        if (rc == PXFDTD_OK) rc = server_segment(S);
+        bool launched = false;
+        double t_rebuild = 0, t_admit = 0, t_retire = 0, t_busy = 0;
+        if (!S->running.empty() && !S->stop) {
+            if (dirty) {
+                const double tb = server_now(S);
+                rc = server_rebuild(S);
+                t_rebuild = server_now(S) - tb;
+                dirty = false;
+            }
+            if (rc == PXFDTD_OK) rc = server_segment_begin(S);
+            launched = rc == PXFDTD_OK;
+        }
         const std::string why = rc ? pxfdtd_last_error() : "";
+        // while the segment runs: new jobs onto the device, finished ones' results back
+        std::vector<std::pair<ServerJob *, std::string>> failed;
+        std::vector<ServerJob *> ready;
+        const double ta = server_now(S);
+        for (ServerJob *J : admit) {
+            const int rc2 = server_admit(S, J);
+            if (rc2 != PXFDTD_OK) {
+                failed.push_back({J, pxfdtd_last_error()});
+                (void)pxfdtd_host::sim_detach(J->sim);
+            } else {
+                ready.push_back(J);
+            }
+        }
+        t_admit = server_now(S) - ta;
+        const double tf = server_now(S);
+        std::vector<std::pair<ServerJob *, std::string>> fetched;
+        for (ServerJob *J : fetch) {
+            const int rc3 = pxfdtd_host::sim_fetch(J->sim);
+            fetched.push_back({J, rc3 ? pxfdtd_last_error() : ""});
+        }
+        int rc4 = PXFDTD_OK;
+        if (launched) rc4 = server_segment_end(S, &t_busy);
+        const std::string why4 = rc4 ? pxfdtd_last_error() : "";
+        for (auto &f : fetched) pxfdtd_host::sim_release(f.first->sim);
+        t_retire = server_now(S) - tf;
         lk.lock();
-        // retire finished jobs (or every job, after a device error): results to the host and
-        // their device memory freed; the tables keep their entries, with no steps, until the
-        // next admission rebuilds them
+        S->stats.rebuild_seconds += t_rebuild;
+        S->stats.admit_seconds += t_admit;
+        S->stats.retire_seconds += t_retire;
+        if (launched && rc4 == PXFDTD_OK) {
+            S->stats.busy_seconds += t_busy;
+            S->stats.segments++;
+            S->stats.cell_updates += S->seg_cells;
+        }
+        for (auto &f : failed) {
+            S->used -= f.first->foot;
+            server_finish(S, f.first, PXFDTD_JOB_FAILED, f.second);
+        }
+        for (auto &f : fetched) {
+            f.first->r = nullptr;
+            S->used -= f.first->foot;
+            server_finish(S, f.first, f.second.empty() ? PXFDTD_JOB_DONE : PXFDTD_JOB_FAILED, f.second);
+        }
+        fetch.clear();
+        const bool device_error = rc != PXFDTD_OK || rc4 != PXFDTD_OK;
+        // the jobs this segment finished (or all of them, after a device error) go to fetch
         for (auto it = S->running.begin(); it != S->running.end();) {
             ServerJob *J = *it;
-            if (rc == PXFDTD_OK && J->left > 0) {
+            if (!device_error && J->left > 0) {
                 ++it;
                 continue;
             }
-            lk.unlock();
-            const int rc2 = pxfdtd_host::sim_detach(J->sim);
-            const std::string why2 = rc2 ? pxfdtd_last_error() : "";
-            lk.lock();
-            J->r = nullptr;
-            S->used -= J->foot;
-            if (rc != PXFDTD_OK)
-                server_finish(S, J, PXFDTD_JOB_FAILED, why);
-            else if (rc2 != PXFDTD_OK)
-                server_finish(S, J, PXFDTD_JOB_FAILED, why2);
-            else
-                server_finish(S, J, PXFDTD_JOB_DONE, "");
+            if (device_error) {
+                lk.unlock();
+                (void)pxfdtd_host::sim_detach(J->sim);
+                lk.lock();
+                J->r = nullptr;
+                S->used -= J->foot;
+                server_finish(S, J, PXFDTD_JOB_FAILED, rc ? why : why4);
+            } else {
+                fetch.push_back(J);
+            }
             it = S->running.erase(it);
         }
-        S->stats.hbm_in_use = (int64_t)S->used;
-        if (rc != PXFDTD_OK) {
+        if (device_error) {
             S->table.clear();
             lk.unlock();
             drop_graph(S);
             lk.lock();
         }
+        for (ServerJob *J : ready) {
+            S->running.push_back(J);
+            S->stats.admitted++;
+            dirty = true;
+        }
+        S->stats.hbm_in_use = (int64_t)S->used;
+        S->stats.captures = S->captures;
+        S->stats.peak_jobs = std::max<int64_t>(S->stats.peak_jobs, (int64_t)S->running.size());
     }
 }
 

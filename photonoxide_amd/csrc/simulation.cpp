@@ -1028,6 +1028,15 @@ int sim_detach(pxfdtd_simulation *s) {
     return PXFDTD_OK;
 }
 
+// the two halves of sim_detach: the results to the host (device memory untouched, so it may
+// run while other work is on the device), then the device state freed
+int sim_fetch(pxfdtd_simulation *s) { return sync_host(s); }
+
+void sim_release(pxfdtd_simulation *s) {
+    drop_resident(s);
+    s->dev = nullptr;
+}
+
 }  // namespace pxfdtd_host
 
 extern "C" int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precision, int64_t *bytes) {

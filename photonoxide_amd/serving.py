@@ -24,7 +24,8 @@ class Server:
     hbm_budget: device bytes the admitted jobs may hold (0: 90 % of the free HBM);
     max_jobs: jobs admitted at once (0: 256); segment: steps per batched segment (0: 32);
     direct: launch each segment's kernels directly instead of replaying a hipGraph;
-    autotune: trial launches per tiling for the first job of each grid shape (0: default tiling).
+    autotune: trial launches per tiling for the first job of each class of grid shape (2D or 3D,
+    each extent's power of two; 0: the default tiling).
     """
 
     def __init__(self, gpu: Gpu, hbm_budget: int = 0, max_jobs: int = 0, segment: int = 0,
