@@ -328,6 +328,36 @@ int pxfdtd_server_times(pxfdtd_server *s, int64_t job, double *submitted, double
                         double *finished);
 int pxfdtd_server_get_stats(pxfdtd_server *s, pxfdtd_server_stats *out);
 
+/* ------------------------------------------------------- level 4: the reverse-time adjoint */
+/* The gradient of J = |S|^2, S = sum over a transform monitor's values of m(r) times its FP64
+ * sum of E's component c at one frequency, with respect to cb of chosen E values (for a
+ * lossless value cb = dt / eps, so dJ/deps = -dJ/dcb cb^2 / dt). It is the exact transpose of
+ * the device's time stepping (fields and CPML), run backwards from the last step with the state
+ * recomputed from checkpoints taken every `checkpoint` steps in HBM. photonoxide's adjoint is
+ * frequency-domain with no checkpointing (src/fdtd/adjoint.rs:43-46,496-569); this one has no
+ * counterpart. The simulation has a device and has not run; the design values lie outside the
+ * CPML slabs, carry no E source, and are lossless and not walls; dispersive media are refused. */
+typedef struct pxfdtd_adjoint pxfdtd_adjoint;
+typedef struct {
+    int32_t monitor;          /* the transform monitor (pxfdtd_simulation_add_dft's number) */
+    int32_t component;        /* E's component the objective weighs */
+    int32_t frequency;        /* the index of the monitor's frequency */
+    const double *weights;    /* m over the monitor's box, [k][j][i] complex */
+    int64_t ndesign;
+    const int32_t *design_component;
+    const int64_t *design_index;  /* (k ny + j) nx + i */
+    int32_t checkpoint;       /* steps between checkpoints */
+} pxfdtd_adjoint_spec;
+int pxfdtd_adjoint_create(pxfdtd_simulation *sim, const pxfdtd_adjoint_spec *spec,
+                          pxfdtd_adjoint **out);
+/* runs `steps` steps from step 0 with checkpoints; the objective J */
+int pxfdtd_adjoint_forward(pxfdtd_adjoint *a, int64_t steps, double *objective);
+/* dJ/dcb per design value (FP64); the simulation is left as the forward run left it */
+int pxfdtd_adjoint_backward(pxfdtd_adjoint *a, double *grad_cb);
+int pxfdtd_adjoint_info(const pxfdtd_adjoint *a, int64_t *checkpoints, int64_t *checkpoint_bytes,
+                        int64_t *recomputed_steps);
+void pxfdtd_adjoint_destroy(pxfdtd_adjoint *a);
+
 #ifdef __cplusplus
 }
 #endif

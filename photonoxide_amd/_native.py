@@ -65,6 +65,13 @@ class ServerConfig(C.Structure):
                 ("direct", C.c_int32), ("autotune", C.c_int32)]
 
 
+class AdjointSpec(C.Structure):
+    _fields_ = [("monitor", C.c_int32), ("component", C.c_int32), ("frequency", C.c_int32),
+                ("weights", C.POINTER(C.c_double)), ("ndesign", C.c_int64),
+                ("design_component", C.POINTER(C.c_int32)), ("design_index", C.POINTER(C.c_int64)),
+                ("checkpoint", C.c_int32)]
+
+
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("completed", C.c_int64),
                 ("failed", C.c_int64), ("segments", C.c_int64), ("captures", C.c_int64),
@@ -139,6 +146,11 @@ SIGNATURES = {
     "pxfdtd_server_wait": (C.c_int, [vp, C.c_int64, C.c_double]),
     "pxfdtd_server_times": (C.c_int, [vp, C.c_int64, pd, pd, pd]),
     "pxfdtd_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+    "pxfdtd_adjoint_create": (C.c_int, [vp, C.POINTER(AdjointSpec), C.POINTER(vp)]),
+    "pxfdtd_adjoint_forward": (C.c_int, [vp, C.c_int64, pd]),
+    "pxfdtd_adjoint_backward": (C.c_int, [vp, pd]),
+    "pxfdtd_adjoint_info": (C.c_int, [vp, pi64, pi64, pi64]),
+    "pxfdtd_adjoint_destroy": (None, [vp]),
     "pxfdtd_simulation_new": (C.c_int, [C.POINTER(Grid), C.POINTER(Bounds), C.POINTER(Scene), C.c_double, C.POINTER(vp)]),
     "pxfdtd_simulation_new_eps": (C.c_int, [C.POINTER(Grid), C.POINTER(Bounds), pd, C.c_double, C.POINTER(vp)]),
     "pxfdtd_simulation_destroy": (None, [vp]),

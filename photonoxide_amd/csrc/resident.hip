@@ -184,6 +184,7 @@ struct pxfdtd_resident {
     std::vector<int4> units_h;  // the same, on the host (the server batches them)
     int n_lean = 0, n_x = 0, n_slab = 0;  // the march's units: no slab, x slabs only, others
     bool units_dirty = true;
+    bool quiet = false;  // steps record no probe or transform values (the adjoint's recomputation)
     // per-launch timing of the fused kernel, when asked for
     bool profile = false;
     std::vector<hipEvent_t> kev;
@@ -1418,7 +1419,7 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         HIPCHK(hipGetLastError());
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
         r->p = q;
-        if (r->ntaps) {
+        if (r->ntaps && !r->quiet) {
             const int threads = 256;
             const long long blocks = (r->ntaps + threads - 1) / threads;
             hipLaunchKernelGGL((tap_values<T>), dim3((unsigned)blocks), dim3(threads), 0, s,
@@ -1565,3 +1566,7 @@ extern "C" int pxfdtd_resident_step(pxfdtd_resident *r, uint32_t count, const do
 
 // ------------------------------------------------------------------ the serving loop
 #include "serving.hpp"
+
+// ------------------------------------------------------------- the reverse-time adjoint
+#include "adjoint.hpp"
+#include "adjoint_host.hpp"

@@ -1009,6 +1009,39 @@ int sim_attach(pxfdtd_simulation *s, pxfdtd_device *dev, int32_t precision, pxfd
 }
 
 int sim_emitters(const pxfdtd_simulation *s) { return s->emitters; }
+
+void sim_set_steps(pxfdtd_simulation *s, int64_t steps) { s->steps = steps; }
+
+double sim_dft_frequency(const pxfdtd_simulation *s, int monitor, int f) {
+    if (monitor < 0 || monitor >= (int)s->dfts.size() || f < 0 || f >= (int)s->dfts[monitor].freqs.size()) return -1.0;
+    return s->dfts[monitor].freqs[f];
+}
+
+// the monitor's box, lo inclusive and hi exclusive
+int sim_dft_box(const pxfdtd_simulation *s, int monitor, int lo[3], int hi[3]) {
+    if (monitor < 0 || monitor >= (int)s->dfts.size()) return px_fail(PXFDTD_ERR_INVALID, "no transform monitor %d", monitor);
+    for (int a = 0; a < 3; a++) {
+        lo[a] = s->dfts[monitor].lo[a];
+        hi[a] = s->dfts[monitor].hi[a];
+    }
+    return PXFDTD_OK;
+}
+
+bool sim_has_media(const pxfdtd_simulation *s) { return !s->groups.empty(); }
+
+// the E sources' values (component, index) and whether any E current or source sits there
+bool sim_e_source_at(const pxfdtd_simulation *s, int comp, long long r) {
+    for (auto &so : s->sources)
+        if (so.field == PXFDTD_E && so.comp == comp && so.r == r) return true;
+    for (auto &cu : s->currents)
+        if (cu.field == PXFDTD_E)
+            for (size_t v = 0; v < cu.r.size(); v++)
+                if (cu.comps[v] == comp && cu.r[v] == r) return true;
+    return false;
+}
+
+const double *sim_ca(const pxfdtd_simulation *s) { return s->ca.data(); }
+const double *sim_cb(const pxfdtd_simulation *s) { return s->cb.data(); }
 int64_t sim_steps(const pxfdtd_simulation *s) { return s->steps; }
 
 void sim_tables(const pxfdtd_simulation *s, int64_t n0, int count, int slots, double *waves, double *phases) {
