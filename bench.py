@@ -194,6 +194,64 @@ def serving(gpu, n2d, n3d, rate, seed=2, autotune=2, direct=False):
     }
 
 
+def adjoint_workload(gpu, fd_cells=16, steps=6000, checkpoint=64):
+    """The adjoint-gradient workload: 256 x 192 x 96 cells of 20 nm, a 500 x 220 nm Si strip
+    in SiO2 along x crossing a 2 x 2 um x 220 nm design block (100 x 100 x 11 cells) whose density
+    is seeded U(0, 1) then Gaussian-filtered (sigma 2 cells), a pulse at 1.55 um, the objective
+    |overlap of the output plane's E_y transform with the input profile|^2, 6000 steps with a
+    checkpoint every 64. The f32 gradient (forward with checkpoints, then the reverse-time
+    adjoint with recomputation) is timed; central finite differences of f64 runs at 16 random
+    design cells check it, beside the f64 adjoint's gradient."""
+    from scipy.ndimage import gaussian_filter
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from adjoint_problem import DesignProblem
+    import photonoxide_amd as P
+    width = math.sqrt(2.0 * math.log(2.0)) / (math.pi * (1 / 1.45 - 1 / 1.65))
+    kw = dict(n=(256, 192, 96), pml=12, design=((78, 178), (46, 146), (42, 53)), steps=steps,
+              checkpoint=checkpoint, core=(0.5, 0.22), pulse=(width, 6.0 * width))
+    p32 = DesignProblem(gpu, precision=P.Precision.F32, **kw)
+    rng = np.random.default_rng(4)
+    rho = gaussian_filter(rng.random(p32.shape), 2.0)
+    t0 = time.perf_counter()
+    sim, adj, st = p32.build(rho)
+    t1 = time.perf_counter()
+    j32 = adj.forward(st)
+    t2 = time.perf_counter()
+    g32 = p32.chain(adj.grad_eps(adj.backward()))
+    t3 = time.perf_counter()
+    info = adj.info()
+    cells = 256 * 192 * 96
+    p64 = DesignProblem(gpu, precision=P.Precision.F64, **kw)
+    sim64, adj64, _ = p64.build(rho)
+    j64 = adj64.forward(st)
+    g64 = p64.chain(adj64.grad_eps(adj64.backward()))
+    del sim64, adj64
+    rng2 = np.random.default_rng(5)
+    picks = [tuple(int(rng2.integers(0, s)) for s in p64.shape) for _ in range(fd_cells)]
+    d = 1e-3
+    rel32, rel64 = [], []
+    for c in picks:
+        hi, lo = rho.copy(), rho.copy()
+        hi[c] += d
+        lo[c] -= d
+        fd = (p64.objective(hi) - p64.objective(lo)) / (2 * d)
+        rel32.append(abs(g32[c] - fd) / abs(fd))
+        rel64.append(abs(g64[c] - fd) / abs(fd))
+    return {
+        "metric": "reverse-time adjoint gradient: time and agreement with finite differences",
+        "config": {"workload": "adjoint_256x192x96_dx20nm_design100x100x11_6000steps_K64", "cells": cells},
+        "objective_f32": j32, "objective_f64": j64,
+        "forward_s": round(t2 - t1, 3), "backward_s": round(t3 - t2, 3), "setup_s": round(t1 - t0, 3),
+        "forward_gcell_per_s": round(cells * st / (t2 - t1) / 1e9, 2),
+        "checkpoints": info["checkpoints"], "checkpoint_gb": round(info["checkpoint_bytes"] / 1e9, 2),
+        "recomputed_steps": info["recomputed_steps"],
+        "fd_cells": fd_cells, "fd_step": d,
+        "max_rel_err_f32_adjoint_vs_fd_f64": float(max(rel32)),
+        "max_rel_err_f64_adjoint_vs_fd_f64": float(max(rel64)),
+        "dtype": "f32", "data": "synthetic",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +274,8 @@ def main():
                     help="the batched-serving workload instead: small is 64 2D + 4 3D jobs, full "
                          "256 2D + 16 3D")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
+    ap.add_argument("--adjoint", action="store_true",
+                    help="the adjoint-gradient workload instead (with its finite-difference check)")
     ap.add_argument("--direct", action="store_true",
                     help="serving: launch each segment's kernels directly instead of a hipGraph")
     args = ap.parse_args()
@@ -226,6 +286,10 @@ def main():
         sys.exit(2)
     if args.copy_mib:
         print(json.dumps(copy_rate(args.copy_mib, args.steps, args.warmup)), flush=True)
+        return
+    if args.adjoint:
+        import photonoxide_amd as P
+        print(json.dumps(adjoint_workload(P.Gpu(0))), flush=True)
         return
     if args.serving:
         import photonoxide_amd as P
