@@ -74,6 +74,22 @@ typedef struct {
     double *before, *now, *cur, *prev;
 } group_t;
 
+/* a plane wave's auxiliary grid and its couplings onto the box's surface; [0] E's, [1] H's */
+typedef struct {
+    long g[3];
+    double steps[3];
+    long base, pad, len;
+    double *e[3], *h[3], *ca, *cb, *da, *db;
+    int nsheet, sheet_c[6];
+    long sheet_i[6];
+    double sheet_j[6];
+    or_waveform w;
+    long non[2];
+    int *lc[2], *lf[2];
+    long *lr[2], *li[2];
+    double *lw[2];
+} plane_t;
+
 typedef struct or_sim or_sim;
 static void probe_push(probe_t *p, double v);
 static void dft_phases(dft_t *d, double t, double dt);
@@ -113,7 +129,13 @@ struct or_sim {
     group_t groups[MAX_GROUPS];
     int32_t *member[3]; /* each E value's media group, -1 outside */
     double *views[6];   /* f64 copies of an f32 run's fields */
+    int nplanes;
+    plane_t *planes;
 };
+
+static void pw_step_h(plane_t *P);
+static void pw_step_e(plane_t *P, double half);
+static void pw_free(plane_t *P);
 
 #define R double
 #define SFX(x) x##_64
@@ -515,8 +537,12 @@ void or_sim_free(or_sim *s) {
         free(gr->comp); free(gr->r); free(gr->scale); free(gr->poles);
         free(gr->before); free(gr->now); free(gr->cur); free(gr->prev);
     }
+    for (int p = 0; p < s->nplanes; p++) pw_free(&s->planes[p]);
+    free(s->planes);
     free(s);
 }
+
+#include "planewave.inc"
 
 /* src/fdtd.rs:949-992 */
 int or_sim_conductivity(or_sim *s, const or_scene *sigma) {

@@ -17,6 +17,11 @@
  *   - DFT: the sum by hand from the probes (src/fdtd/monitors_checks.rs:35-71);
  *   - ADE media: a uniform dispersive box against a non-dispersive one where the poles
  *     vanish, and the steady-state leapfrog permittivity (src/fdtd/media.rs:148-159);
+ *   - total-field/scattered-field plane waves (src/fdtd/sources.rs:742-1040): the box leaks
+ *     only round-off at the reference's five angles (src/fdtd/tests.rs:252-265, bar 1e-12),
+ *     a planted coupling error leaks, the reference's refusals (tests.rs:350-400), and a slab's
+ *     reflection against Airy's formula at the reference's own measured errors
+ *     (src/validation.rs:2848-2856);
  *   - f32 stepping: the same arithmetic rounded to f32 (src/fdtd/kernel.rs `Real for f32`),
  *     checked against the f64 run to the reference's own growth law 2e-8 per step
  *     (src/fdtd/kernel_tests.rs:235-258).
@@ -103,6 +108,21 @@ int or_sim_add_dft(or_sim *s, const int32_t low[3], const int32_t high[3], int n
 /* poles [np][4]: kind (0 Lorentz: strength, resonance, damping; 1 Drude: plasma, damping) */
 int or_sim_add_medium(or_sim *s, double eps_inf, int np, const double *poles,
                       const or_scene *inside);
+/* a total-field/scattered-field plane wave (src/fdtd/sources.rs:742-1040) on the box of nodes
+ * low to high, travelling along the integer direction, E along the polarization's part across
+ * it, in a background of permittivity eps; before the run starts */
+int or_sim_add_plane_wave(or_sim *s, const int low[3], const int high[3], const long direction[3],
+                          const double polarization[3], double eps, const or_waveform *w);
+/* plane p's couplings onto field (0 E, 1 H): returns their count; with arrays, fills
+ * component, value index, weight, incident component and auxiliary sample per coupling */
+long or_sim_plane_links(const or_sim *s, int p, int field, int32_t *comp, int64_t *r, double *w,
+                        int32_t *from, int64_t *sample);
+/* plane p's auxiliary grid length; with out, copies its field (0 E, 1 H) component c */
+long or_sim_plane_aux(const or_sim *s, int p, int field, int c, double *out);
+/* multiplies coupling m of plane p onto field by factor (a planted mistake, for tests) */
+void or_sim_plane_scale_link(or_sim *s, int p, int field, long m, double factor);
+/* plane p's incident field (0 E, 1 H) component comp at the value (i, j, k) now */
+double or_sim_plane_incident(const or_sim *s, int p, int field, int comp, int i, int j, int k);
 /* from now on step in f32 (the tables rounded once, the fields rounded now) */
 void or_sim_use_f32(or_sim *s);
 void or_sim_run(or_sim *s, long count);

@@ -62,6 +62,11 @@ for name, res, args in [
     ("or_sim_add_probe", C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("or_sim_add_dft", C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int, pd]),
     ("or_sim_add_medium", C.c_int, [vp, C.c_double, C.c_int, pd, C.POINTER(Scene)]),
+    ("or_sim_add_plane_wave", C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long), pd, C.c_double, C.POINTER(Wave)]),
+    ("or_sim_plane_links", C.c_long, [vp, C.c_int, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int64), pd, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    ("or_sim_plane_aux", C.c_long, [vp, C.c_int, C.c_int, C.c_int, pd]),
+    ("or_sim_plane_scale_link", None, [vp, C.c_int, C.c_int, C.c_long, C.c_double]),
+    ("or_sim_plane_incident", C.c_double, [vp] + [C.c_int] * 6),
     ("or_sim_use_f32", None, [vp]),
     ("or_sim_run", None, [vp, C.c_long]),
     ("or_sim_field", pd, [vp, C.c_int, C.c_int]),
@@ -204,6 +209,36 @@ class Sim:
 
     def energy(self):
         return lib.or_sim_electric_energy(self.h)
+
+    def add_plane_wave(self, low, high, direction, polarization, eps, wave):
+        if lib.or_sim_add_plane_wave(self.h, (C.c_int * 3)(*low), (C.c_int * 3)(*high),
+                                     (C.c_long * 3)(*direction), d3(*polarization), eps,
+                                     C.byref(wave)):
+            raise ValueError(lib.or_last_error().decode())
+
+    def plane_links(self, p, field):
+        """(component, value index, weight, incident component, sample) of plane p's couplings
+        onto field (0 E, 1 H)."""
+        n = lib.or_sim_plane_links(self.h, p, field, None, None, None, None, None)
+        c, r, f = np.zeros(n, np.int32), np.zeros(n, np.int64), np.zeros(n, np.int32)
+        w, i = np.zeros(n), np.zeros(n, np.int64)
+        lib.or_sim_plane_links(self.h, p, field, c.ctypes.data_as(C.POINTER(C.c_int32)),
+                               r.ctypes.data_as(C.POINTER(C.c_int64)), w.ctypes.data_as(pd),
+                               f.ctypes.data_as(C.POINTER(C.c_int32)),
+                               i.ctypes.data_as(C.POINTER(C.c_int64)))
+        return c, r, w, f, i
+
+    def plane_scale_link(self, p, field, m, factor):
+        lib.or_sim_plane_scale_link(self.h, p, field, m, factor)
+
+    def incident(self, p, field, comp, at):
+        return lib.or_sim_plane_incident(self.h, p, field, comp, *at)
+
+    def plane_aux(self, p, field, comp):
+        n = lib.or_sim_plane_aux(self.h, p, field, comp, None)
+        out = np.zeros(n)
+        lib.or_sim_plane_aux(self.h, p, field, comp, out.ctypes.data_as(pd))
+        return out
 
 
 def wave(kind=0, frequency=1.0, width=0.6, delay=1.8, amp=1 + 0j, ramp=0.0):

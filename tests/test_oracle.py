@@ -206,3 +206,143 @@ def test_lorentz_medium_reaches_the_leapfrog_permittivity():
     # the grid's wavenumber: sin^2(k h/2)/h^2 = eps sin^2(w dt/2)/dt^2
     k_grid = 2 / h * math.asin(math.sqrt(eps.real) * math.sin(w * dt / 2) * h / dt)
     assert abs(k_measured - k_grid) < 2e-3 * k_grid, (k_measured, k_grid)
+
+
+def _pulse(bandwidth):
+    """Waveform::pulse (src/fdtd.rs:300-312): width sqrt(2 ln 2)/(pi B), delay 6 widths."""
+    width = math.sqrt(2 * math.log(2)) / (math.pi * bandwidth)
+    return O.wave(0, 1.0, width, 6 * width)
+
+
+def tfsf_leakage(direction, polarization, three_d):
+    """src/fdtd/checks.rs:511-563: the largest field in the scattered region over a pulse's run,
+    relative to the largest in the box, vacuum, nothing inside."""
+    h = 0.05
+    if three_d:
+        n, low, high, cp = (36, 36, 36), (12, 12, 12), (24, 24, 24), 8
+        b = O.bounds(low=(cp,) * 3, high=(cp,) * 3)
+    else:
+        n, low, high, cp = (80, 80, 1), (25, 25, 0), (55, 55, 1), 10
+        b = O.bounds(periodic=(0, 0, 1), low=(cp, cp, 0), high=(cp, cp, 0))
+    grid = (*n, h, h, h, -n[0] * h / 2, -n[1] * h / 2, -n[2] * h / 2)
+    s = O.Sim(grid, b, O.scene(1.0), 0.9)
+    s.add_plane_wave(low, high, direction, polarization, 1.0, _pulse(0.6))
+    i = np.arange(n[0] * n[1] * n[2])
+    at = [i % n[0], (i // n[0]) % n[1], i // (n[0] * n[1])]
+    inside = {}
+    for f in (0, 1):
+        for c in range(3):
+            m = np.ones(i.size, bool)
+            for a in range(3):
+                if n[a] == 1:
+                    continue
+                t = 2 * at[a] + ((c == a) if f == 0 else (c != a))
+                m &= (t >= 2 * low[a]) & (t <= 2 * high[a])
+            inside[f, c] = m
+    leak = largest = 0.0
+    for _ in range(int(12.0 / s.dt())):
+        s.run(1)
+        for (f, c), m in inside.items():
+            v = np.abs(s.field(f, c))
+            largest = max(largest, v[m].max())
+            leak = max(leak, v[~m].max())
+    return leak / largest
+
+
+@pytest.mark.parametrize("direction,polarization,three_d", [
+    ((1, 0, 0), (0, 0, 1), False), ((2, 1, 0), (0, 0, 1), False), ((1, -3, 0), (1, 0, 0), False),
+    ((0, 0, 1), (1, 0, 0), True), ((2, 1, 1), (1, -1, 0), True)])
+def test_plane_waves_box_leaks_only_round_off(direction, polarization, three_d):
+    """src/fdtd/tests.rs:252-265: at every angle of the reference's test, < 1e-12 (it measured
+    1.3e-15). A wrong sign, weight or sample in any coupling leaks O(1)."""
+    leak = tfsf_leakage(direction, polarization, three_d)
+    assert leak < 1e-12, leak
+
+
+def tfsf_leak_2d(plant=None):
+    n, h, cp = (80, 80, 1), 0.05, 10
+    s = O.Sim((*n, h, h, h, 0, 0, 0), O.bounds(periodic=(0, 0, 1), low=(cp, cp, 0), high=(cp, cp, 0)),
+              O.scene(1.0), 0.9)
+    s.add_plane_wave((25, 25, 0), (55, 55, 1), (2, 1, 0), (0, 0, 1), 1.0, _pulse(0.6))
+    if plant:
+        field, factor = plant
+        c, r, w, f, q = s.plane_links(0, field)
+        # a coupling that carries the incident wave: onto Ez (from Hx, Hy), or from Ez
+        m = int(np.flatnonzero(c == 2 if field == 0 else f == 2)[len(c) // 7])
+        s.plane_scale_link(0, field, m, factor)
+    leak = largest = 0.0
+    for _ in range(int(8.0 / s.dt()) // 20):
+        s.run(20)
+        ez = np.abs(s.field(0, 2)).reshape(80, 80)
+        largest = max(largest, ez[26:55, 26:55].max())
+        leak = max(leak, ez[:, :24].max())
+    return leak / largest
+
+
+@pytest.mark.parametrize("plant", [(0, 1.001), (1, -1.0)])
+def test_plane_wave_leak_check_sees_a_planted_mistake(plant):
+    """A coupling's weight off by 1e-3 (onto E) or of the wrong sign (onto H), planted in the
+    oracle: the scattered region then holds far more than round-off."""
+    assert tfsf_leak_2d() < 1e-12
+    assert tfsf_leak_2d(plant) > 1e-6
+
+
+def test_plane_wave_refuses_what_it_cannot_launch():
+    """src/fdtd/tests.rs:350-400: a face in or against the CPML, the box not whole along a
+    periodic axis, a direction along it, a box across a block, the polarization along the
+    direction, a zero direction."""
+    n = (40, 40, 1)
+    sc = O.scene(1.0, [(0, (0.4 + 1e-9, -INF, -INF), (INF, INF, INF), 4.0)])
+    grid = (*n, 0.05, 0.05, 0.05, -1.0, -1.0, -0.025)
+    b = O.bounds(periodic=(0, 0, 1), low=(6, 6, 0), high=(6, 6, 0))
+    ok = O.Sim(grid, b, sc, 0.9)
+    ok.add_plane_wave((10, 10, 0), (25, 25, 1), (2, 1, 0), (0, 0, 1), 1.0, _pulse(0.5))
+    for low, high, d, p in [((5, 10, 0), (25, 25, 1), (1, 0, 0), (0, 0, 1)),
+                            ((6, 10, 0), (25, 25, 1), (1, 0, 0), (0, 0, 1)),
+                            ((10, 10, 0), (25, 25, 0), (1, 0, 0), (0, 0, 1)),
+                            ((10, 10, 0), (25, 25, 1), (1, 0, 1), (0, 0, 1)),
+                            ((10, 10, 0), (30, 25, 1), (1, 0, 0), (0, 0, 1)),
+                            ((10, 10, 0), (25, 25, 1), (1, 0, 0), (1, 0, 0)),
+                            ((10, 10, 0), (25, 25, 1), (0, 0, 0), (0, 0, 1))]:
+        s = O.Sim(grid, b, sc, 0.9)
+        with pytest.raises(ValueError):
+            s.add_plane_wave(low, high, d, p, 1.0, _pulse(0.5))
+
+
+def tfsf_slab(h):
+    """src/fdtd/checks.rs:565-650: a slab of eps 4, 0.3 um thick, in a box at normal incidence;
+    the largest difference of |DFT(E_s)/DFT(E_i)|^2 from Airy's formula at 0.8, 1, 1.2."""
+    thick, eps_slab = 0.3, 4.0
+    cells = lambda x: int(round(x / h))
+    cp = cells(0.5)
+    nz = 2 * cp + cells(2.4)
+    low, high = cp + cells(0.4), nz - cp - cells(0.4)
+    za = (low + cells(0.5)) * h
+    sc = O.scene(1.0, [(0, (-INF, -INF, za - 1e-12), (INF, INF, za + thick - 1e-12), eps_slab)])
+    s = O.Sim((1, 1, nz, h, h, h, 0, 0, 0), O.bounds(periodic=(1, 1, 0), low=(0, 0, cp), high=(0, 0, cp)),
+              sc, 0.9)
+    s.add_plane_wave((0, 0, low), (1, 1, high), (0, 0, 1), (1, 0, 0), 1.0, _pulse(0.8))
+    front, inside = low - cells(0.2), (0, 0, low + 2)
+    fr = np.array([0.8, 1.0, 1.2])
+    sca, inc = np.zeros(3, complex), np.zeros(3, complex)
+    dt = s.dt()
+    for n in range(int(30.0 / dt)):
+        s.run(1)
+        w = np.exp(2j * math.pi * fr * (n + 1) * dt)
+        sca += s.field(0, 0)[front] * w
+        inc += s.incident(0, 0, 0, inside) * w
+    n2 = math.sqrt(eps_slab)
+    r12 = (1 - n2) / (1 + n2)
+    ph = np.exp(2j * n2 * 2 * math.pi * fr * thick)
+    r = r12 * (1 - ph) / (1 - r12 * r12 * ph)
+    return float(np.max(np.abs(np.abs(sca / inc) ** 2 - np.abs(r) ** 2)))
+
+
+def test_plane_wave_on_a_slab_reflects_as_airy_says_to_second_order():
+    """src/fdtd/tests.rs:267-276 and src/validation.rs:2848-2856: the error falls by more than 3
+    per halving of the cells and is < 1e-3 at 5 nm (the reference measured 5.4e-3, 1.3e-3,
+    3.3e-4 at 20, 10, 5 nm)."""
+    e = [tfsf_slab(h) for h in (0.02, 0.01, 0.005)]
+    assert e[1] < e[0] / 3 and e[2] < e[1] / 3, e
+    assert e[2] < 1e-3, e
+    assert e == pytest.approx([5.4e-3, 1.3e-3, 3.3e-4], rel=0.05)
