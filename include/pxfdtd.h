@@ -152,6 +152,47 @@ int pxfdtd_resident_read_probes(pxfdtd_resident *r, double *out, int64_t capacit
 /* the transform sums, FP64, series after series, each [f][k][j][i] complex */
 int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_t capacity);
 int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r);
+/* sets the transform sums the device goes on adding to (a run continued from sums held
+ * elsewhere: each sum stays one running FP64 sum in step order, as the reference's) */
+int pxfdtd_resident_write_sums(pxfdtd_resident *r, const double *in, int64_t count);
+/* A total-field/scattered-field plane wave's auxiliary 1D grid (PlaneWaveRun,
+ * src/fdtd/sources.rs:742-766): the incident wave on samples 0 .. len - 1 along
+ * l = sum_a g_a twice_a (twice_a: twice a value's position along axis a in steps). Each step,
+ * before the grid's H: H~ <- da H~ - db curl E; after the grid's E: E <- ca E + cb curl H~, then
+ * E_c[i] -= cb[i] j w for each sheet entry (c, i, j), w the emitter's value that step
+ * (Waveform::at at t + dt/2); both over samples pad .. len - pad - 1, with differences
+ * (f[i + g_a] - f[i - g_a]) / step_a (sources.rs:993-1039). Always FP64. */
+typedef struct {
+    int32_t g[3];
+    int64_t len, pad;
+    const double *ca, *cb, *da, *db;  /* len each */
+    int32_t nsheet;                   /* at most 6 */
+    int32_t sheet_component[6];
+    int64_t sheet_sample[6];
+    double sheet_j[6];
+    int32_t emitter;
+} pxfdtd_plane_wave;
+
+/* A coupling across a plane wave's box (PlaneWaveRun::correct, sources.rs:971-990): after the
+ * field's sources and currents, H -= dt weight F or E -= cb weight F, F the wave's incident
+ * field now (E's for H's update, H~'s for E's) of component from_component at `sample`. */
+typedef struct {
+    int32_t field, component;
+    int64_t index;
+    double weight;
+    int32_t plane, from_component;
+    int64_t sample;
+} pxfdtd_coupling;
+
+/* The plane waves and their boxes' couplings (Simulation::add_plane_wave's run,
+ * sources.rs:716-721; the auxiliary fields start at zero). Emitters index the waves given to
+ * pxfdtd_resident_step, as pxfdtd_resident_set_injections set them. */
+int pxfdtd_resident_set_plane_waves(pxfdtd_resident *r, const pxfdtd_plane_wave *waves,
+                                    int32_t n, const pxfdtd_coupling *couplings, int64_t count);
+/* plane wave n's auxiliary fields [3][len], E and H~ (either may be NULL) */
+int pxfdtd_resident_plane_state(pxfdtd_resident *r, int32_t n, double *e, double *h);
+int pxfdtd_resident_set_plane_state(pxfdtd_resident *r, int32_t n, const double *e,
+                                    const double *h);
 /* the step kernel: 0 tiles of 64 x rows exchanging H through LDS; 1 strips of 62 x rows per
  * wavefront marching along z with no barrier, loading a plane ahead; 2 the same without the
  * plane-ahead loads; 3 (the default where a field's three components fit in 2 GiB) aligned
@@ -255,6 +296,18 @@ int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low[3], const 
 /* poles [np][4]: (0, strength, resonance, damping) Lorentz or (1, plasma, damping, 0) Drude */
 int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf, int32_t np,
                                  const double *poles, const pxfdtd_scene *inside);
+/* Simulation::add_plane_wave (src/fdtd/sources.rs:701-721): a plane wave of `w` on the
+ * total-field/scattered-field box of nodes low .. high, travelling along the integer direction,
+ * E along the polarization's part across it, in a uniform background of permittivity eps; its
+ * number (for pxfdtd_simulation_incident) in *number. The same refusals as the reference's. */
+int pxfdtd_simulation_add_plane_wave(pxfdtd_simulation *s, const int32_t low[3],
+                                     const int32_t high[3], const int64_t direction[3],
+                                     const double polarization[3], double eps,
+                                     const pxfdtd_waveform *w, int32_t *number);
+/* Simulation::incident (sources.rs:723-737): plane wave n's incident field now at a value of
+ * the grid (E at the current time, H~ half a step behind); 0 beyond its auxiliary grid */
+int pxfdtd_simulation_incident(pxfdtd_simulation *s, int32_t n, int32_t field, int32_t comp,
+                               int32_t i, int32_t j, int32_t k, double *out);
 /* Steps on `dev` in `precision` from now on (Simulation::set_device); NULL: back to none */
 int pxfdtd_simulation_set_device(pxfdtd_simulation *s, pxfdtd_device *dev, int32_t precision);
 int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count);

@@ -110,7 +110,7 @@ int or_sim_add_medium(or_sim *s, double eps_inf, int np, const double *poles,
                       const or_scene *inside);
 /* a total-field/scattered-field plane wave (src/fdtd/sources.rs:742-1040) on the box of nodes
  * low to high, travelling along the integer direction, E along the polarization's part across
- * it, in a background of permittivity eps; before the run starts */
+ * it, in a background of permittivity eps; its auxiliary grid starts at rest now */
 int or_sim_add_plane_wave(or_sim *s, const int low[3], const int high[3], const long direction[3],
                           const double polarization[3], double eps, const or_waveform *w);
 /* plane p's couplings onto field (0 E, 1 H): returns their count; with arrays, fills

@@ -107,6 +107,18 @@ class Wave(C.Structure):
                 ("ramp", C.c_double)]
 
 
+class PlaneWaveAux(C.Structure):
+    _fields_ = [("g", C.c_int32 * 3), ("len", C.c_int64), ("pad", C.c_int64), ("ca", pd), ("cb", pd),
+                ("da", pd), ("db", pd), ("nsheet", C.c_int32), ("sheet_component", C.c_int32 * 6),
+                ("sheet_sample", C.c_int64 * 6), ("sheet_j", C.c_double * 6), ("emitter", C.c_int32)]
+
+
+class Coupling(C.Structure):
+    _fields_ = [("field", C.c_int32), ("component", C.c_int32), ("index", C.c_int64),
+                ("weight", C.c_double), ("plane", C.c_int32), ("from_component", C.c_int32),
+                ("sample", C.c_int64)]
+
+
 vp = C.c_void_p
 SIGNATURES = {
     "pxfdtd_last_error": (C.c_char_p, []),
@@ -128,6 +140,10 @@ SIGNATURES = {
     "pxfdtd_resident_read_probes": (C.c_int, [vp, pd, C.c_int64, pi64]),
     "pxfdtd_resident_read_sums": (C.c_int, [vp, pd, C.c_int64]),
     "pxfdtd_resident_sums_len": (C.c_int64, [vp]),
+    "pxfdtd_resident_write_sums": (C.c_int, [vp, pd, C.c_int64]),
+    "pxfdtd_resident_set_plane_waves": (C.c_int, [vp, C.POINTER(PlaneWaveAux), C.c_int32, C.POINTER(Coupling), C.c_int64]),
+    "pxfdtd_resident_plane_state": (C.c_int, [vp, C.c_int32, pd, pd]),
+    "pxfdtd_resident_set_plane_state": (C.c_int, [vp, C.c_int32, pd, pd]),
     "pxfdtd_resident_set_tiling": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "pxfdtd_resident_get_tiling": (C.c_int, [vp, pi32, pi32]),
     "pxfdtd_resident_set_kernel": (C.c_int, [vp, C.c_int32]),
@@ -159,6 +175,8 @@ SIGNATURES = {
     "pxfdtd_simulation_add_source": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Wave)]),
     "pxfdtd_simulation_add_current": (C.c_int, [vp, C.c_int32, C.c_int64, pi32, pi64, pd, C.POINTER(Wave)]),
     "pxfdtd_simulation_add_dipole": (C.c_int, [vp, C.c_int32, C.c_int32, pd, C.c_double, C.c_double, C.POINTER(Wave)]),
+    "pxfdtd_simulation_add_plane_wave": (C.c_int, [vp, pi32, pi32, pi64, pd, C.c_double, C.POINTER(Wave), pi32]),
+    "pxfdtd_simulation_incident": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pd]),
     "pxfdtd_simulation_add_probe": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pi32]),
     "pxfdtd_simulation_add_dft": (C.c_int, [vp, pi32, pi32, C.c_int32, pd, pi32]),
     "pxfdtd_simulation_add_medium": (C.c_int, [vp, C.c_double, C.c_int32, pd, C.POINTER(Scene)]),

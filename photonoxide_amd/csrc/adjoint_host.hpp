@@ -17,6 +17,7 @@ void sim_set_steps(pxfdtd_simulation *s, int64_t steps);
 double sim_dft_frequency(const pxfdtd_simulation *s, int monitor, int f);
 int sim_dft_box(const pxfdtd_simulation *s, int monitor, int lo[3], int hi[3]);
 bool sim_has_media(const pxfdtd_simulation *s);
+bool sim_has_planes(const pxfdtd_simulation *s);
 bool sim_e_source_at(const pxfdtd_simulation *s, int comp, long long r);
 const double *sim_ca(const pxfdtd_simulation *s);
 const double *sim_cb(const pxfdtd_simulation *s);
@@ -79,6 +80,8 @@ extern "C" int pxfdtd_adjoint_create(pxfdtd_simulation *sim, const pxfdtd_adjoin
         return px_fail(PXFDTD_ERR_INVALID, "the adjoint starts from a simulation that has not run");
     if (pxfdtd_host::sim_has_media(sim))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the adjoint does not step dispersive media");
+    if (pxfdtd_host::sim_has_planes(sim))
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the adjoint does not checkpoint plane waves' auxiliary grids");
     if (spec->checkpoint < 1) return px_fail(PXFDTD_ERR_INVALID, "checkpoints every K >= 1 steps");
     if (spec->component < 0 || spec->component > 2) return px_fail(PXFDTD_ERR_INVALID, "no E component %d", spec->component);
     if (pxfdtd_host::sim_dft_frequency(sim, spec->monitor, spec->frequency) <= 0)
@@ -173,6 +176,8 @@ static int adj_forward_t(pxfdtd_adjoint *a, int64_t steps) {
 extern "C" int pxfdtd_adjoint_forward(pxfdtd_adjoint *a, int64_t steps, double *objective) {
     if (steps < 1) return px_fail(PXFDTD_ERR_INVALID, "the forward run needs steps");
     if (a->forward_done) return px_fail(PXFDTD_ERR_INVALID, "the forward run has been done");
+    if (pxfdtd_host::sim_has_planes(a->sim))
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the adjoint does not checkpoint plane waves' auxiliary grids");
     HIPCHK(hipSetDevice(a->r->dev->ordinal));
     TRY(a->r->prec == PXFDTD_F64 ? adj_forward_t<double>(a, steps) : adj_forward_t<float>(a, steps));
     a->steps = steps;

@@ -73,6 +73,9 @@ struct StepCold {
     T *m_before;              // [3][nval]
     T *m_cur, *m_prev;        // [3][nval][max_poles]
     int psi_len;              // psi values per copy
+    // plane waves' auxiliary fields, FP64: an injection with emitter < 0 is a coupling of a
+    // plane wave's box and reads its incident field at aux[-1 - emitter]
+    const double *aux;
 };
 
 // The kernel's arguments: what every plane reads.
@@ -164,6 +167,43 @@ __device__ __forceinline__ T psi_at(const StepArgs<T> &A, const StepCold<T> &K, 
     return v;
 }
 
+// One injection on component c of a value: a source value, H -= dt Re(a w) or
+// E -= cb Re(a w) (src/fdtd.rs:1346-1395), or a plane wave's coupling, H -= (dt weight) F or
+// E -= (cb weight) F with F its incident field (src/fdtd/sources.rs:971-990), both in the
+// reference's association. Selects, not an indexed array: the three values stay in registers.
+template <typename T, int FIELD>
+__device__ __forceinline__ void apply_injection(const StepArgs<T> &A, const StepCold<T> &K,
+                                                const Injection &in, int c, T &v0, T &v1, T &v2,
+                                                T cb0, T cb1, T cb2) {
+    if (in.emitter < 0) {
+        const T f = T(K.aux[-1LL - in.emitter]);
+        const T wt = T(in.a_re);
+        if (FIELD == 1) {
+            const T d = (A.dt * wt) * f;
+            v0 = c == 0 ? v0 - d : v0;
+            v1 = c == 1 ? v1 - d : v1;
+            v2 = c == 2 ? v2 - d : v2;
+        } else {
+            v0 = c == 0 ? v0 - (cb0 * wt) * f : v0;
+            v1 = c == 1 ? v1 - (cb1 * wt) * f : v1;
+            v2 = c == 2 ? v2 - (cb2 * wt) * f : v2;
+        }
+        return;
+    }
+    const double *w = K.waves + 2 * ((long long)A.wave_step * K.emitters + in.emitter);
+    const T amount = T(in.a_re * w[0] - in.a_im * w[1]);
+    if (FIELD == 1) {
+        const T d = A.dt * amount;
+        v0 = c == 0 ? v0 - d : v0;
+        v1 = c == 1 ? v1 - d : v1;
+        v2 = c == 2 ? v2 - d : v2;
+    } else {
+        v0 = c == 0 ? v0 - cb0 * amount : v0;
+        v1 = c == 1 ? v1 - cb1 * amount : v1;
+        v2 = c == 2 ? v2 - cb2 * amount : v2;
+    }
+}
+
 // the injections of plane k on the value (field, i, j) in `mask`, in the tile's list order
 template <typename T, int FIELD>
 __device__ __forceinline__ void inject(const StepArgs<T> &A, const StepCold<T> &K, int tile, int i,
@@ -175,20 +215,7 @@ __device__ __forceinline__ void inject(const StepArgs<T> &A, const StepCold<T> &
         const Injection in = K.inj[q];
         const int c = in.code & 3;
         if ((in.code >> 2) != FIELD || in.i != i || in.j != j || !((mask >> c) & 1)) continue;
-        const double *w = K.waves + 2 * ((long long)A.wave_step * K.emitters + in.emitter);
-        const double wr = w[0], wi = w[1];
-        const T amount = T(in.a_re * wr - in.a_im * wi);
-        // selects, not an indexed array: the three values stay in registers
-        if (FIELD == 1) {
-            const T d = A.dt * amount;
-            v0 = c == 0 ? v0 - d : v0;
-            v1 = c == 1 ? v1 - d : v1;
-            v2 = c == 2 ? v2 - d : v2;
-        } else {
-            v0 = c == 0 ? v0 - cb0 * amount : v0;
-            v1 = c == 1 ? v1 - cb1 * amount : v1;
-            v2 = c == 2 ? v2 - cb2 * amount : v2;
-        }
+        apply_injection<T, FIELD>(A, K, in, c, v0, v1, v2, cb0, cb1, cb2);
     }
 }
 
@@ -892,6 +919,69 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
 }
 
 // Copies this step's probe and transform values out of copy `src`: out[v] = field[index[v]].
+// ------------------------------------------------------------------ plane waves
+// A plane wave's auxiliary 1D grid on the device: fields at aux + field_off, E's three
+// components then H~'s, len each; its ca, cb, da, db at coef + coef_off, len each.
+struct PlaneDesc {
+    int g[3];
+    int emitter;
+    long long len, pad, field_off, coef_off;
+    double step[3];
+    int nsheet;
+    int sheet_c[6];
+    long long sheet_i[6];
+    double sheet_j[6];
+};
+
+// (f[i + g_a] - f[i - g_a]) / step_a, exactly 0 where g_a = 0 (src/fdtd/sources.rs:993-1003)
+__device__ __forceinline__ double aux_difference(const PlaneDesc &d, const double *f, int a,
+                                                 long long i) {
+    const int o = d.g[a];
+    if (o == 0) return 0.0;
+    return (f[i + o] - f[i - o]) / d.step[a];
+}
+
+// H~ <- da H~ - db curl E over every plane wave's samples pad .. len - pad - 1
+// (sources.rs:1011-1021); one thread per sample, blockIdx.y the plane wave
+__global__ __launch_bounds__(256) void aux_step_h(const PlaneDesc *planes, double *aux,
+                                                  const double *coef) {
+    const PlaneDesc &d = planes[blockIdx.y];
+    const long long i = d.pad + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.len - d.pad) return;
+    const double *e = aux + d.field_off;
+    double *h = aux + d.field_off + 3 * d.len;
+    const double *da = coef + d.coef_off + 2 * d.len, *db = coef + d.coef_off + 3 * d.len;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int a = (c + 1) % 3, b = (c + 2) % 3;
+        const double curl = aux_difference(d, e + b * d.len, a, i) - aux_difference(d, e + a * d.len, b, i);
+        h[c * d.len + i] = da[i] * h[c * d.len + i] - db[i] * curl;
+    }
+}
+
+// E <- ca E + cb curl H~, then the sheet E_c[i] -= cb[i] j w (sources.rs:1024-1039), w the
+// plane wave's emitter value this step
+__global__ __launch_bounds__(256) void aux_step_e(const PlaneDesc *planes, double *aux,
+                                                  const double *coef, const double *waves,
+                                                  int emitters, int wave_step) {
+    const PlaneDesc &d = planes[blockIdx.y];
+    const long long i = d.pad + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.len - d.pad) return;
+    double *e = aux + d.field_off;
+    const double *h = aux + d.field_off + 3 * d.len;
+    const double *ca = coef + d.coef_off, *cb = coef + d.coef_off + d.len;
+    const double w = waves[2 * ((long long)wave_step * emitters + d.emitter)];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int a = (c + 1) % 3, b = (c + 2) % 3;
+        const double curl = aux_difference(d, h + b * d.len, a, i) - aux_difference(d, h + a * d.len, b, i);
+        double v = ca[i] * e[c * d.len + i] + cb[i] * curl;
+        for (int q = 0; q < d.nsheet; q++)
+            if (d.sheet_c[q] == c && d.sheet_i[q] == i) v -= cb[i] * d.sheet_j[q] * w;
+        e[c * d.len + i] = v;
+    }
+}
+
 template <typename T>
 __global__ void tap_values(const T *const *fields /* Ex Ey Ez Hx Hy Hz */, const int64_t *index,
                            const int8_t *which, long long n, T *out) {

@@ -29,18 +29,7 @@ __device__ __forceinline__ void inject2d(const StepArgs<T> &A, const StepCold<T>
         const Injection in = K.inj[q];
         const int c = in.code & 3;
         if ((in.code >> 2) != FIELD || in.i != i || !((mask >> c) & 1)) continue;
-        const double *w = K.waves + 2 * ((long long)A.wave_step * K.emitters + in.emitter);
-        const T amount = T(in.a_re * w[0] - in.a_im * w[1]);
-        if (FIELD == 1) {
-            const T d = A.dt * amount;
-            v0 = c == 0 ? v0 - d : v0;
-            v1 = c == 1 ? v1 - d : v1;
-            v2 = c == 2 ? v2 - d : v2;
-        } else {
-            v0 = c == 0 ? v0 - cb0 * amount : v0;
-            v1 = c == 1 ? v1 - cb1 * amount : v1;
-            v2 = c == 2 ? v2 - cb2 * amount : v2;
-        }
+        apply_injection<T, FIELD>(A, K, in, c, v0, v1, v2, cb0, cb1, cb2);
     }
 }
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -154,6 +155,14 @@ struct pxfdtd_resident {
     int ngroups = 0, max_poles = 0;
     // injections
     std::vector<pxfdtd_injection> inj_host;
+    // plane waves: their auxiliary grids (FP64, E then H~ per wave), coefficients, and the
+    // couplings of their boxes as injections whose emitter < 0 names the incident value
+    // aux[-1 - emitter]; a coupling applies after every source on its value
+    std::vector<PlaneDesc> planes_h;
+    std::vector<pxfdtd_injection> coup_host;
+    std::vector<char> coup_live;  // whether a coupling's incident component can be nonzero
+    Buf plane_desc, aux, aux_coef;
+    long long aux_len = 0, aux_max = 0;
     int emitters = 0;
     Buf inj_off, inj;
     bool have_inj = false;
@@ -732,12 +741,20 @@ static bool h_region_has(const pxfdtd_resident *r, int bx, int by, int i, int j)
 // injections apply in that order.
 // The 2D march's lists: per strip of 64 columns, per row. A strip holds the injections on its
 // own columns and, for H, on the column left of it (where it recomputes H).
+// the sources' injections, then the plane waves' couplings: the order they apply in on a value
+static std::vector<pxfdtd_injection> every_injection(const pxfdtd_resident *r) {
+    std::vector<pxfdtd_injection> all = r->inj_host;
+    all.insert(all.end(), r->coup_host.begin(), r->coup_host.end());
+    return all;
+}
+
 static int build_injection_lists_2d(pxfdtd_resident *r) {
     const int tx = (r->nx + SW - 1) / SW, ny = r->ny;
     struct Item { int strip, j; long long seq; Injection in; };
     std::vector<Item> items;
-    for (size_t q = 0; q < r->inj_host.size(); q++) {
-        const pxfdtd_injection &s = r->inj_host[q];
+    const std::vector<pxfdtd_injection> all = every_injection(r);
+    for (size_t q = 0; q < all.size(); q++) {
+        const pxfdtd_injection &s = all[q];
         const long long v = s.index;
         const int i = (int)(v % r->nx), j = (int)((v / r->nx) % r->ny);
         Injection in;
@@ -778,12 +795,16 @@ static int build_injection_lists_2d(pxfdtd_resident *r) {
 }
 
 static int build_injection_lists(pxfdtd_resident *r) {
-    r->have_inj = !r->inj_host.empty();
+    r->have_inj = !r->inj_host.empty() || !r->coup_host.empty();
     r->inj_set = 0;
-    for (const auto &s : r->inj_host) {
+    auto mark = [&](const pxfdtd_injection &s) {
         const int q = s.field * 3 + s.component;
         r->inj_set |= (q == 0 || q == 1 || q == 5) ? 1 : 2;
-    }
+    };
+    for (const auto &s : r->inj_host) mark(s);
+    // a coupling whose incident component stays zero changes no value of a field set at rest
+    for (size_t q = 0; q < r->coup_host.size(); q++)
+        if (r->coup_live[q]) mark(r->coup_host[q]);
     if (!r->have_inj) return PXFDTD_OK;
     if (r->kernel == 4) return build_injection_lists_2d(r);
     int tw, th;
@@ -792,8 +813,9 @@ static int build_injection_lists(pxfdtd_resident *r) {
     const long long tiles = (long long)tx * tyn;
     struct Item { long long tile; int k; long long seq; Injection in; };
     std::vector<Item> items;
-    for (size_t q = 0; q < r->inj_host.size(); q++) {
-        const pxfdtd_injection &s = r->inj_host[q];
+    const std::vector<pxfdtd_injection> all = every_injection(r);
+    for (size_t q = 0; q < all.size(); q++) {
+        const pxfdtd_injection &s = all[q];
         const long long v = s.index;
         const int i = (int)(v % r->nx), j = (int)((v / r->nx) % r->ny), k = (int)(v / ((long long)r->nx * r->ny));
         Injection in;
@@ -951,6 +973,139 @@ extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_i
     r->inj_host.assign(inj, inj + n);
     r->emitters = emitters;
     return build_injection_lists(r);
+}
+
+// Which of a plane wave's auxiliary components can ever be nonzero: E's across the
+// polarization, and what the curls reach from them (a difference along an axis the direction
+// has no part on is exactly zero).
+static void live_components(const pxfdtd_plane_wave &w, bool live[2][3]) {
+    for (int c = 0; c < 3; c++) live[1][c] = false;
+    for (int c = 0; c < 3; c++) {
+        live[0][c] = false;
+        for (int q = 0; q < w.nsheet; q++) live[0][c] = live[0][c] || w.sheet_component[q] == c;
+    }
+    for (bool grew = true; grew;) {
+        grew = false;
+        for (int f = 0; f < 2; f++)
+            for (int c = 0; c < 3; c++) {
+                const int a = (c + 1) % 3, b = (c + 2) % 3;
+                const bool reach = (w.g[a] != 0 && live[1 - f][b]) || (w.g[b] != 0 && live[1 - f][a]);
+                if (reach && !live[f][c]) live[f][c] = grew = true;
+            }
+    }
+}
+
+extern "C" int pxfdtd_resident_set_plane_waves(pxfdtd_resident *r, const pxfdtd_plane_wave *waves,
+                                               int32_t n, const pxfdtd_coupling *couplings,
+                                               int64_t count) {
+    if (n < 0 || count < 0 || (n && !waves) || (count && !couplings))
+        return px_fail(PXFDTD_ERR_INVALID, "plane waves: bad counts or arrays");
+    std::vector<PlaneDesc> desc((size_t)n);
+    long long fields = 0, coefs = 0, longest = 0;
+    for (int q = 0; q < n; q++) {
+        const pxfdtd_plane_wave &w = waves[q];
+        long long reach = 0;
+        for (int a = 0; a < 3; a++) reach = std::max<long long>(reach, std::llabs((long long)w.g[a]));
+        if (w.len <= 0 || w.pad < reach || 2 * w.pad >= w.len || !w.ca || !w.cb || !w.da || !w.db ||
+            w.nsheet < 0 || w.nsheet > 6 || w.emitter < 0 || w.emitter >= std::max(r->emitters, 1))
+            return px_fail(PXFDTD_ERR_INVALID, "plane wave %d: its grid, pad, sheet or emitter is out of range", q);
+        PlaneDesc &d = desc[q];
+        memset(&d, 0, sizeof d);
+        for (int a = 0; a < 3; a++) {
+            d.g[a] = w.g[a];
+            d.step[a] = r->step[a];
+        }
+        d.emitter = w.emitter;
+        d.len = w.len;
+        d.pad = w.pad;
+        d.field_off = fields;
+        d.coef_off = coefs;
+        d.nsheet = w.nsheet;
+        for (int s2 = 0; s2 < w.nsheet; s2++) {
+            if (w.sheet_component[s2] < 0 || w.sheet_component[s2] > 2 || w.sheet_sample[s2] < w.pad ||
+                w.sheet_sample[s2] >= w.len - w.pad)
+                return px_fail(PXFDTD_ERR_INVALID, "plane wave %d: sheet entry %d is out of range", q, s2);
+            d.sheet_c[s2] = w.sheet_component[s2];
+            d.sheet_i[s2] = w.sheet_sample[s2];
+            d.sheet_j[s2] = w.sheet_j[s2];
+        }
+        fields += 6 * w.len;
+        coefs += 4 * w.len;
+        longest = std::max<long long>(longest, w.len);
+    }
+    if (fields >= (1LL << 31))
+        return px_fail(PXFDTD_ERR_INVALID, "the plane waves' auxiliary grids are too long");
+    std::vector<pxfdtd_injection> coup((size_t)count);
+    std::vector<char> live((size_t)count);
+    std::vector<std::array<std::array<bool, 3>, 2>> lives((size_t)n);
+    for (int q = 0; q < n; q++) {
+        bool l[2][3];
+        live_components(waves[q], l);
+        for (int f = 0; f < 2; f++)
+            for (int c = 0; c < 3; c++) lives[q][f][c] = l[f][c];
+    }
+    for (int64_t q = 0; q < count; q++) {
+        const pxfdtd_coupling &c = couplings[q];
+        if (c.field < 0 || c.field > 1 || c.component < 0 || c.component > 2 || c.index < 0 ||
+            c.index >= r->ncells || c.plane < 0 || c.plane >= n || c.from_component < 0 ||
+            c.from_component > 2 || c.sample < 0 || c.sample >= waves[c.plane].len)
+            return px_fail(PXFDTD_ERR_INVALID, "coupling %lld is out of range", (long long)q);
+        const PlaneDesc &d = desc[c.plane];
+        // E's update reads the incident H~, H's the incident E
+        const int from_field = c.field == PXFDTD_E ? 1 : 0;
+        const long long at = d.field_off + (long long)(from_field * 3 + c.from_component) * d.len + c.sample;
+        coup[q] = {c.field, c.component, c.index, c.weight, 0.0, (int32_t)(-1 - at)};
+        live[q] = lives[c.plane][from_field][c.from_component];
+    }
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    TRY(r->plane_desc.alloc(std::max<size_t>(desc.size(), 1) * sizeof(PlaneDesc)));
+    TRY(r->aux.alloc(std::max<long long>(fields, 1) * sizeof(double)));
+    TRY(r->aux_coef.alloc(std::max<long long>(coefs, 1) * sizeof(double)));
+    if (n) {
+        HIPCHK(hipMemcpy(r->plane_desc.p, desc.data(), desc.size() * sizeof(PlaneDesc), hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(r->aux.p, 0, fields * sizeof(double)));
+        for (int q = 0; q < n; q++) {
+            const pxfdtd_plane_wave &w = waves[q];
+            const double *src[4] = {w.ca, w.cb, w.da, w.db};
+            for (int t = 0; t < 4; t++)
+                HIPCHK(hipMemcpy(r->aux_coef.as<double>() + desc[q].coef_off + t * w.len, src[t],
+                                 w.len * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    r->planes_h = desc;
+    r->aux_len = fields;
+    r->aux_max = longest;
+    r->coup_host = coup;
+    r->coup_live = live;
+    return build_injection_lists(r);
+}
+
+static int plane_segment(pxfdtd_resident *r, int32_t n, long long &off, long long &len) {
+    if (n < 0 || n >= (int32_t)r->planes_h.size()) return px_fail(PXFDTD_ERR_INVALID, "no plane wave %d", n);
+    off = r->planes_h[n].field_off;
+    len = r->planes_h[n].len;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_plane_state(pxfdtd_resident *r, int32_t n, double *e, double *h) {
+    long long off, len;
+    TRY(plane_segment(r, n, off, len));
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    if (e) HIPCHK(hipMemcpy(e, r->aux.as<double>() + off, 3 * len * sizeof(double), hipMemcpyDeviceToHost));
+    if (h) HIPCHK(hipMemcpy(h, r->aux.as<double>() + off + 3 * len, 3 * len * sizeof(double), hipMemcpyDeviceToHost));
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_resident_set_plane_state(pxfdtd_resident *r, int32_t n, const double *e,
+                                               const double *h) {
+    long long off, len;
+    TRY(plane_segment(r, n, off, len));
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    if (e) HIPCHK(hipMemcpy(r->aux.as<double>() + off, e, 3 * len * sizeof(double), hipMemcpyHostToDevice));
+    if (h) HIPCHK(hipMemcpy(r->aux.as<double>() + off + 3 * len, h, 3 * len * sizeof(double), hipMemcpyHostToDevice));
+    return PXFDTD_OK;
 }
 
 extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes) {
@@ -1133,6 +1288,17 @@ extern "C" int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_
     return PXFDTD_OK;
 }
 
+extern "C" int pxfdtd_resident_write_sums(pxfdtd_resident *r, const double *in, int64_t count) {
+    if (count != r->sums_len)
+        return px_fail(PXFDTD_ERR_INVALID, "the sums are %lld values, not %lld", (long long)r->sums_len,
+                       (long long)count);
+    HIPCHK(hipSetDevice(r->dev->ordinal));
+    if (r->sums_len)
+        HIPCHK(hipMemcpyAsync(r->sums.p, in, r->sums_len * 8, hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return PXFDTD_OK;
+}
+
 template <typename T>
 static int read_probes_t(pxfdtd_resident *r, double *out, int64_t capacity, int64_t *steps) {
     *steps = r->probe_rows;
@@ -1217,6 +1383,7 @@ static void fill_cold(const pxfdtd_resident *r, StepCold<T> K[2]) {
         k.m_cur = r->m_cur.as<T>();
         k.m_prev = r->m_prev.as<T>();
         k.psi_len = (int)r->psi_len;
+        k.aux = r->aux.as<double>();
     }
 }
 
@@ -1361,7 +1528,7 @@ static int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, 
 template <typename T>
 static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const double *phases) {
     hipStream_t s = r->stream;
-    if (r->emitters && r->have_inj) {
+    if (r->emitters && (r->have_inj || !r->planes_h.empty())) {
         if (!waves) return px_fail(PXFDTD_ERR_INVALID, "the injections need their waves");
         const size_t b = (size_t)count * r->emitters * 2 * 8;
         TRY(r->waves.reserve(b));
@@ -1412,12 +1579,24 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
         const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab};
         const int set = step_set(r);
+        const dim3 aux_grid((unsigned)((r->aux_max + 255) / 256), (unsigned)r->planes_h.size());
+        // the plane waves' H~ to t + dt/2 before the grid's step reads it (E's couplings);
+        // their E stays at t for H's couplings until after the step
+        if (!r->planes_h.empty())
+            hipLaunchKernelGGL(aux_step_h, aux_grid, dim3(256), 0, s, r->plane_desc.as<PlaneDesc>(),
+                               r->aux.as<double>(), r->aux_coef.as<double>());
         if (r->mode == 0)
             TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         else
             TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         HIPCHK(hipGetLastError());
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
+        if (!r->planes_h.empty()) {
+            hipLaunchKernelGGL(aux_step_e, aux_grid, dim3(256), 0, s, r->plane_desc.as<PlaneDesc>(),
+                               r->aux.as<double>(), r->aux_coef.as<double>(), r->waves.as<double>(),
+                               r->emitters, (int)n);
+            HIPCHK(hipGetLastError());
+        }
         r->p = q;
         if (r->ntaps && !r->quiet) {
             const int threads = 256;

@@ -52,6 +52,21 @@ struct CurrentRec {
     pxfdtd_waveform w;
 };
 
+// A plane wave's run (src/fdtd/sources.rs:742-766): its auxiliary 1D grid, whose state lives
+// on the device while it steps there, and its box's couplings.
+struct PlaneRec {
+    int64_t g[3];
+    long long base, pad, len;
+    std::vector<double> ca, cb, da, db;
+    std::vector<double> e, h;  // [3][len] each
+    int nsheet = 0;
+    int32_t sheet_c[6];
+    int64_t sheet_i[6];
+    double sheet_j[6];
+    pxfdtd_waveform w;
+    std::vector<pxfdtd_coupling> couplings;
+};
+
 struct ProbeRec {
     int field, comp;
     long long r;
@@ -168,6 +183,7 @@ struct pxfdtd_simulation {
     std::vector<double> cpml_b[3][2][2], cpml_a[3][2][2];
     std::vector<SourceRec> sources;
     std::vector<CurrentRec> currents;
+    std::vector<PlaneRec> planes;
     std::vector<ProbeRec> probes;
     std::vector<DftRec> dfts;
     std::vector<Group> groups;
@@ -179,6 +195,8 @@ struct pxfdtd_simulation {
     bool device_newer = false;  // the device's fields are newer than the host's
     bool host_newer = false;    // the host's fields changed since the last upload
     bool inj_dirty = true, taps_dirty = true;
+    int planes_on_device = 0;      // the plane waves the resident steps
+    bool planes_device_newer = false;  // their auxiliary fields there are newer than the host's
     int emitters = 0;
 
     void e_position(int comp, long long r, double p[3]) const {
@@ -372,16 +390,17 @@ extern "C" void pxfdtd_simulation_destroy(pxfdtd_simulation *s) {
 
 static void drop_resident(pxfdtd_simulation *s);
 static int sync_host(pxfdtd_simulation *s);
+static int pull_planes(pxfdtd_simulation *s);
 
-// the sums the device has taken since its series were set, added to the host's; the device's
-// restart from zero when its series are set next
+// the device's transform sums to the host: the device went on from the host's, so they are the
+// whole running sums; when its series are set next, it goes on from these again
 static int bank_sums(pxfdtd_simulation *s) {
     if (!s->res || s->taps_dirty || s->dfts.empty()) return PXFDTD_OK;
     std::vector<double> sums(pxfdtd_resident_sums_len(s->res));
     TRY(pxfdtd_resident_read_sums(s->res, sums.data(), (int64_t)sums.size()));
     size_t at = 0;
     for (auto &d : s->dfts)
-        for (auto &v : d.acc) v += sums[at++];
+        for (auto &v : d.acc) v = sums[at++];
     s->taps_dirty = true;
     return PXFDTD_OK;
 }
@@ -561,6 +580,231 @@ extern "C" int pxfdtd_simulation_add_dipole(pxfdtd_simulation *s, int32_t field,
     return PXFDTD_OK;
 }
 
+// ------------------------------------------------------------------------ plane waves
+
+// twice a value's position along `axis` in steps from node 0: E_c is halfway along c, H~_c
+// halfway along the other two (src/fdtd/sources.rs:155-161)
+static long long twice(int field, int comp, int axis, long long m) {
+    const bool half = field == PXFDTD_E ? comp == axis : comp != axis;
+    return 2 * m + (half ? 1 : 0);
+}
+
+static int64_t gcd64(int64_t a, int64_t b) { return b == 0 ? (a < 0 ? -a : a) : gcd64(b, a % b); }
+
+static long long mod2(long long v) { return ((v % 2) + 2) % 2; }
+
+// Simulation::add_plane_wave: PlaneWaveRun::new (src/fdtd/sources.rs:774-950) with the box's
+// couplings from Simulation::links (:321-418) and the surface check Simulation::uniform
+// (:511-522)
+extern "C" int pxfdtd_simulation_add_plane_wave(pxfdtd_simulation *s, const int32_t low[3],
+                                                const int32_t high[3], const int64_t direction[3],
+                                                const double polarization[3], double eps,
+                                                const pxfdtd_waveform *w, int32_t *number) {
+    TRY(check_waveform(w));
+    int64_t g[3] = {direction[0], direction[1], direction[2]};
+    const int64_t divisor = gcd64(gcd64(g[0], g[1]), g[2]);
+    if (divisor == 0) return px_fail(PXFDTD_ERR_INVALID, "a plane wave's direction can't be zero");
+    for (auto &v : g) v /= divisor;
+    if (!(std::isfinite(eps) && eps > 0.0))
+        return px_fail(PXFDTD_ERR_INVALID, "the background's permittivity must be positive, got %g", eps);
+    bool faces[3];
+    for (int a = 0; a < 3; a++) {
+        const int n = s->n[a];
+        faces[a] = !s->b.periodic[a];
+        if (!faces[a]) {
+            if (low[a] != 0 || high[a] != n || g[a] != 0)
+                return px_fail(PXFDTD_ERR_INVALID,
+                               "along the periodic axis %d the box must be whole (0 to %d) and the direction "
+                               "zero, got %d to %d and %lld", a, n, low[a], high[a], (long long)g[a]);
+        } else if (low[a] < s->b.low[a] + 1 || high[a] + s->b.high[a] + 1 > n || low[a] >= high[a]) {
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "along axis %d the box's faces, nodes %d and %d, must be in order and a cell clear "
+                           "of the CPMLs (%d and %d cells) and the ends of %d cells",
+                           a, low[a], high[a], s->b.low[a], s->b.high[a], n);
+        }
+    }
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    double gv[3], norm2 = 0.0;
+    for (int a = 0; a < 3; a++) {
+        gv[a] = (double)g[a] / steps[a];
+        norm2 += gv[a] * gv[a];
+    }
+    const double norm = std::sqrt(norm2);
+    double khat[3], pt[3], along = 0.0, size2 = 0.0;
+    for (int a = 0; a < 3; a++) khat[a] = gv[a] / norm;
+    for (int a = 0; a < 3; a++) along += polarization[a] * khat[a];
+    for (int a = 0; a < 3; a++) pt[a] = polarization[a] - along * khat[a];
+    for (int a = 0; a < 3; a++) size2 += pt[a] * pt[a];
+    const double size = std::sqrt(size2);
+    if (!(std::isfinite(size) && size > 1e-12))
+        return px_fail(PXFDTD_ERR_INVALID, "the polarization has nothing across the direction");
+    for (auto &v : pt) v /= size;
+    TRY(sync_host(s));  // the waves already stepping keep their state
+
+    // the couplings: each value near the box whose update reads a value on the other side
+    auto total = [&](int field, int comp, const long long at[3]) {
+        for (int a = 0; a < 3; a++) {
+            if (!faces[a]) continue;
+            const long long t = twice(field, comp, a, at[a]);
+            if (t < 2LL * low[a] || t > 2LL * high[a]) return false;
+        }
+        return true;
+    };
+    auto neighbour = [&](const long long at[3], int axis, bool forward, long long out[3]) {
+        for (int a = 0; a < 3; a++) out[a] = at[a];
+        const long long n = s->n[axis];
+        if (forward) {
+            if (at[axis] + 1 < n) out[axis]++;
+            else if (s->b.periodic[axis]) out[axis] = 0;
+            else return false;
+        } else if (at[axis] > 0) {
+            out[axis]--;
+        } else if (s->b.periodic[axis]) {
+            out[axis] = n - 1;
+        } else {
+            return false;
+        }
+        return true;
+    };
+    auto index_of = [&](const long long at[3]) { return (at[2] * s->n[1] + at[1]) * s->n[0] + at[0]; };
+    struct Link { int target, comp; long long r; double weight; int from_comp; long long from[3]; };
+    std::vector<Link> links;
+    long long lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = faces[a] ? low[a] - 1 : 0;
+        hi[a] = faces[a] ? high[a] + 2 : s->n[a];
+    }
+    const long long N = s->N;
+    for (long long k = lo[2]; k < hi[2]; k++)
+        for (long long j = lo[1]; j < hi[1]; j++)
+            for (long long i = lo[0]; i < hi[0]; i++) {
+                const long long at[3] = {i, j, k};
+                const long long r = index_of(at);
+                for (int c = 0; c < 3; c++) {
+                    const int a = (c + 1) % 3, b = (c + 2) % 3;
+                    long long nb[3];
+                    // E_c: curl = D_a H_b - D_b H_a with back differences
+                    if (s->cb[c * N + r] != 0.0) {
+                        const bool in = total(PXFDTD_E, c, at);
+                        const double fa = s->fe[a][at[a]], fb = s->fe[b][at[b]];
+                        auto take = [&](int fc, const long long *from, double coef) {
+                            if (from && total(PXFDTD_H, fc, from) != in)
+                                links.push_back({PXFDTD_E, c, r, (in ? -1.0 : 1.0) * coef, fc, {from[0], from[1], from[2]}});
+                        };
+                        take(b, at, fa);
+                        take(b, neighbour(at, a, false, nb) ? nb : nullptr, -fa);
+                        take(a, at, -fb);
+                        take(a, neighbour(at, b, false, nb) ? nb : nullptr, fb);
+                    }
+                    // H~_c: curl = D_a E_b - D_b E_a with forward differences; held E gives none
+                    const bool in = total(PXFDTD_H, c, at);
+                    const double fa = s->fh[a][at[a]], fb = s->fh[b][at[b]];
+                    auto take = [&](int fc, const long long *from, double coef) {
+                        if (!from || s->cb[fc * N + index_of(from)] == 0.0) return;
+                        if (total(PXFDTD_E, fc, from) != in)
+                            links.push_back({PXFDTD_H, c, r, (in ? 1.0 : -1.0) * coef, fc, {from[0], from[1], from[2]}});
+                    };
+                    take(b, neighbour(at, a, true, nb) ? nb : nullptr, fa);
+                    take(b, at, -fa);
+                    take(a, neighbour(at, b, true, nb) ? nb : nullptr, -fb);
+                    take(a, at, fb);
+                }
+            }
+    // every E on the surface in a lossless medium of eps, not dispersive
+    const double expected = s->dt / eps;
+    for (const Link &l : links) {
+        const int c = l.target == PXFDTD_E ? l.comp : l.from_comp;
+        const long long r = l.target == PXFDTD_E ? l.r : index_of(l.from);
+        const bool media = !s->member.empty() && s->member[c * N + r] >= 0;
+        if (!(std::fabs(s->cb[c * N + r] - expected) <= 1e-12 * expected && s->ca[c * N + r] == 1.0 && !media))
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "the medium on the box's surface must be uniform, with the permittivity %g and no "
+                           "conductivity", eps);
+    }
+    // l over the box and a cell around it: its extremes are at the corners
+    long long least = 0, most = 0, stride = 0;
+    for (int a = 0; a < 3; a++) {
+        const long long t0 = faces[a] ? 2LL * low[a] - 2 : 0, t1 = faces[a] ? 2LL * high[a] + 2 : 0;
+        least += std::min(g[a] * t0, g[a] * t1);
+        most += std::max(g[a] * t0, g[a] * t1);
+        stride = std::max<long long>(stride, g[a] < 0 ? -g[a] : g[a]);
+    }
+    PlaneRec P;
+    P.pad = stride;
+    const long long absorber = 200 * stride;
+    const long long sheet_at = least - 4 * stride - 4;
+    P.base = sheet_at - (absorber + 4 * stride + P.pad);
+    P.base -= mod2(P.base);
+    const long long top = most + 4 * stride + absorber + P.pad;
+    P.len = top - P.base + 1;
+    // a matched absorber at each end: cubic sigma for E, sigma/eps for H~
+    const double depth = (double)absorber * (1.0 / (2.0 * norm));
+    const double sqrt_eps = std::sqrt(eps);
+    const double sigma_max = 50.0 * sqrt_eps / depth;
+    const double dt = s->dt;
+    P.ca.assign(P.len, 1.0);
+    P.cb.assign(P.len, dt / eps);
+    P.da.assign(P.len, 1.0);
+    P.db.assign(P.len, dt);
+    const long long edge = P.pad + absorber;
+    for (long long i = 0; i < P.len; i++) {
+        double into;
+        if (i < edge) into = (double)(edge - i);
+        else if (i + edge >= P.len) into = (double)(i + edge + 1 - P.len);
+        else continue;
+        const double sigma = sigma_max * pxfdtd::ppowi(std::min(into / (double)absorber, 1.0), 3);
+        const double half_e = sigma * dt / (2.0 * eps);
+        P.ca[i] = (1.0 - half_e) / (1.0 + half_e);
+        P.cb[i] = dt / eps / (1.0 + half_e);
+        const double half_h = sigma / eps * dt / 2.0;
+        P.da[i] = (1.0 - half_h) / (1.0 + half_h);
+        P.db[i] = dt / (1.0 + half_h);
+    }
+    // the sheet: J = -2 n |G| p on the samples where each component lives, halved over the two
+    // neighbours where the sheet's l has the other parity
+    for (int c = 0; c < 3; c++) {
+        if (pt[c] == 0.0) continue;
+        const double jc = -2.0 * sqrt_eps * norm * pt[c];
+        if (mod2(sheet_at - g[c]) == 0) {
+            P.sheet_c[P.nsheet] = c; P.sheet_i[P.nsheet] = sheet_at - P.base; P.sheet_j[P.nsheet++] = jc;
+        } else {
+            P.sheet_c[P.nsheet] = c; P.sheet_i[P.nsheet] = sheet_at - 1 - P.base; P.sheet_j[P.nsheet++] = 0.5 * jc;
+            P.sheet_c[P.nsheet] = c; P.sheet_i[P.nsheet] = sheet_at + 1 - P.base; P.sheet_j[P.nsheet++] = 0.5 * jc;
+        }
+    }
+    for (int a = 0; a < 3; a++) P.g[a] = g[a];
+    P.w = *w;
+    P.e.assign(3 * P.len, 0.0);
+    P.h.assign(3 * P.len, 0.0);
+    for (const Link &l : links) {
+        const int from_field = l.target == PXFDTD_E ? PXFDTD_H : PXFDTD_E;
+        long long ell = 0;
+        for (int a = 0; a < 3; a++) ell += g[a] * twice(from_field, l.from_comp, a, l.from[a]);
+        const long long q = ell - P.base;
+        if (q < 0 || q >= P.len) return px_fail(PXFDTD_ERR_INVALID, "the auxiliary grid does not span the box");
+        P.couplings.push_back({l.target, l.comp, l.r, l.weight, 0, l.from_comp, q});
+    }
+    s->planes.push_back(std::move(P));
+    s->inj_dirty = true;
+    if (number) *number = (int32_t)s->planes.size() - 1;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_incident(pxfdtd_simulation *s, int32_t n, int32_t field, int32_t comp,
+                                          int32_t i, int32_t j, int32_t k, double *out) {
+    if (n < 0 || n >= (int32_t)s->planes.size()) return px_fail(PXFDTD_ERR_INVALID, "no plane wave %d", n);
+    if (field < 0 || field > 1 || comp < 0 || comp > 2) return px_fail(PXFDTD_ERR_INVALID, "a field and a component");
+    TRY(inside(s, i, j, k));
+    TRY(pull_planes(s));
+    const PlaneRec &P = s->planes[n];
+    const long long at[3] = {i, j, k};
+    long long ell = 0;
+    for (int a = 0; a < 3; a++) ell += P.g[a] * twice(field, comp, a, at[a]);
+    const long long q = ell - P.base;
+    *out = (q < 0 || q >= P.len) ? 0.0 : (field == PXFDTD_E ? P.e : P.h)[comp * P.len + q];
+    return PXFDTD_OK;
+}
+
 extern "C" int pxfdtd_simulation_add_probe(pxfdtd_simulation *s, int32_t field, int32_t comp,
                                            int32_t i, int32_t j, int32_t k, int32_t *number) {
     TRY(inside(s, i, j, k));
@@ -692,12 +936,24 @@ static void drop_resident(pxfdtd_simulation *s) {
         pxfdtd_resident_destroy(s->res);
         s->res = nullptr;
     }
+    s->planes_on_device = 0;
+    s->planes_device_newer = false;
     s->inj_dirty = s->taps_dirty = true;
     s->host_newer = true;
 }
 
+// the plane waves' auxiliary fields back on the host
+static int pull_planes(pxfdtd_simulation *s) {
+    if (!s->res || !s->planes_device_newer) return PXFDTD_OK;
+    for (int q = 0; q < s->planes_on_device; q++)
+        TRY(pxfdtd_resident_plane_state(s->res, q, s->planes[q].e.data(), s->planes[q].h.data()));
+    s->planes_device_newer = false;
+    return PXFDTD_OK;
+}
+
 // the device's fields, psi and transform sums back on the host
 static int sync_host(pxfdtd_simulation *s) {
+    TRY(pull_planes(s));
     if (!s->res || !s->device_newer) return PXFDTD_OK;
     std::vector<double> psi(pxfdtd_resident_psi_len(s->res));
     TRY(pxfdtd_resident_download(s->res, s->e.data(), s->h.data(), psi.data()));
@@ -781,8 +1037,46 @@ static int ensure_resident(pxfdtd_simulation *s) {
                 inj.push_back({cu.field, cu.comps[v], cu.r[v], cu.amps[v].re, cu.amps[v].im, em});
             em++;
         }
+        const int first_plane = em;
+        em += (int)s->planes.size();
         s->emitters = em;
         TRY(pxfdtd_resident_set_injections(s->res, inj.data(), (int64_t)inj.size(), std::max(em, 1)));
+        if (!s->planes.empty() || s->planes_on_device) {
+            // the waves the device steps keep their state: fetched before the tables change
+            TRY(pull_planes(s));
+            std::vector<pxfdtd_plane_wave> pw;
+            std::vector<pxfdtd_coupling> cp;
+            for (size_t q = 0; q < s->planes.size(); q++) {
+                const PlaneRec &P = s->planes[q];
+                pxfdtd_plane_wave w;
+                std::memset(&w, 0, sizeof w);
+                for (int a = 0; a < 3; a++) w.g[a] = (int32_t)P.g[a];
+                w.len = P.len;
+                w.pad = P.pad;
+                w.ca = P.ca.data();
+                w.cb = P.cb.data();
+                w.da = P.da.data();
+                w.db = P.db.data();
+                w.nsheet = P.nsheet;
+                for (int t = 0; t < P.nsheet; t++) {
+                    w.sheet_component[t] = P.sheet_c[t];
+                    w.sheet_sample[t] = P.sheet_i[t];
+                    w.sheet_j[t] = P.sheet_j[t];
+                }
+                w.emitter = first_plane + (int)q;
+                pw.push_back(w);
+                for (pxfdtd_coupling c : P.couplings) {
+                    c.plane = (int32_t)q;
+                    cp.push_back(c);
+                }
+            }
+            TRY(pxfdtd_resident_set_plane_waves(s->res, pw.data(), (int32_t)pw.size(), cp.data(),
+                                                (int64_t)cp.size()));
+            for (size_t q = 0; q < s->planes.size(); q++)
+                TRY(pxfdtd_resident_set_plane_state(s->res, (int32_t)q, s->planes[q].e.data(),
+                                                    s->planes[q].h.data()));
+            s->planes_on_device = (int)s->planes.size();
+        }
         s->inj_dirty = false;
     }
     if (s->taps_dirty) {
@@ -807,6 +1101,12 @@ static int ensure_resident(pxfdtd_simulation *s) {
             }
         }
         TRY(pxfdtd_resident_set_series(s->res, se.data(), (int32_t)se.size(), nf.data(), (int32_t)nf.size()));
+        // the sums go on from the host's (one running sum each, in step order)
+        std::vector<double> acc;
+        for (auto &d : s->dfts) acc.insert(acc.end(), d.acc.begin(), d.acc.end());
+        bool any = false;
+        for (double v : acc) any = any || v != 0.0 || std::signbit(v);
+        if (any) TRY(pxfdtd_resident_write_sums(s->res, acc.data(), (int64_t)acc.size()));
         s->taps_dirty = false;
     }
     return PXFDTD_OK;
@@ -870,6 +1170,11 @@ static void fill_tables(const pxfdtd_simulation *s, int64_t n0, int64_t m, int s
             w[2 * em + 1] = v.im;
             em++;
         }
+        for (auto &pl : s->planes) {  // the sheet's w at t + dt/2 (sources.rs:1035)
+            w[2 * em] = wave_at(pl.w, half);
+            w[2 * em + 1] = 0.0;
+            em++;
+        }
         const double te = (double)(n + 1) * s->dt;
         const double th = te - 0.5 * s->dt;
         double *ph = phases + (size_t)q * sw * 2;
@@ -906,6 +1211,7 @@ extern "C" int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count) {
     }
     s->steps += count;
     if (count) s->device_newer = true;
+    if (count && s->planes_on_device) s->planes_device_newer = true;
     return PXFDTD_OK;
 }
 
@@ -960,15 +1266,15 @@ extern "C" int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t se
         return px_fail(PXFDTD_ERR_INVALID, "no transform %d series %d", n, series);
     const DftRec &d = s->dfts[n];
     const size_t len = d.acc.size() / 6;
-    std::memcpy(out, d.acc.data() + series * len, len * sizeof(double));
     if (s->res && !s->taps_dirty) {
-        // add what the device has summed since its series were set
+        // the device's sums, which went on from the host's
         std::vector<double> sums(pxfdtd_resident_sums_len(s->res));
         TRY(pxfdtd_resident_read_sums(s->res, sums.data(), (int64_t)sums.size()));
         size_t at = 0;
         for (int32_t m = 0; m < n; m++) at += s->dfts[m].acc.size();
-        at += series * len;
-        for (size_t q = 0; q < len; q++) out[q] += sums[at + q];
+        std::memcpy(out, sums.data() + at + series * len, len * sizeof(double));
+    } else {
+        std::memcpy(out, d.acc.data() + series * len, len * sizeof(double));
     }
     return PXFDTD_OK;
 }
@@ -999,6 +1305,8 @@ int sim_footprint(pxfdtd_simulation *s, int32_t precision, size_t *bytes) {
 int sim_servable(const pxfdtd_simulation *s) {
     if (!s->probes.empty())
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the server records transform monitors, not probes: a job with probes runs alone");
+    if (!s->planes.empty())
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "a job with a plane wave runs alone: its auxiliary grid steps outside the batched kernels");
     return PXFDTD_OK;
 }
 
@@ -1028,6 +1336,7 @@ int sim_dft_box(const pxfdtd_simulation *s, int monitor, int lo[3], int hi[3]) {
 }
 
 bool sim_has_media(const pxfdtd_simulation *s) { return !s->groups.empty(); }
+bool sim_has_planes(const pxfdtd_simulation *s) { return !s->planes.empty(); }
 
 // the E sources' values (component, index) and whether any E current or source sits there
 bool sim_e_source_at(const pxfdtd_simulation *s, int comp, long long r) {

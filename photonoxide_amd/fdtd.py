@@ -193,6 +193,19 @@ class Dipole:
 
 
 @dataclass
+class PlaneWave:
+    """A plane wave on a total-field/scattered-field box (src/fdtd/sources.rs PlaneWave): the
+    box of nodes low to high, the integer direction, E along the polarization's part across it,
+    in a uniform background of permittivity eps."""
+    low: tuple
+    high: tuple
+    direction: tuple
+    polarization: tuple
+    eps: float
+    waveform: Waveform
+
+
+@dataclass
 class Box:
     """An open box lo < p < hi (inf allowed) with a diagonal permittivity (or one value)."""
     lo: tuple
@@ -342,6 +355,21 @@ class Simulation:
         N.check(N.lib.pxfdtd_simulation_add_dipole(self._h, int(dipole.field), int(dipole.component),
                                                    N.dptr(pos), a.real, a.imag,
                                                    C.byref(dipole.waveform.native())))
+
+    def add_plane_wave(self, wave: PlaneWave) -> int:
+        """Simulation::add_plane_wave (src/fdtd/sources.rs:701-721): its number."""
+        n = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_add_plane_wave(
+            self._h, (C.c_int32 * 3)(*wave.low), (C.c_int32 * 3)(*wave.high),
+            (C.c_int64 * 3)(*wave.direction), (C.c_double * 3)(*wave.polarization), float(wave.eps),
+            C.byref(wave.waveform.native()), C.byref(n)))
+        return n.value
+
+    def incident(self, n: int, field, component, at) -> float:
+        """Plane wave n's incident field now at a value (Simulation::incident, sources.rs:723-737)."""
+        v = C.c_double()
+        N.check(N.lib.pxfdtd_simulation_incident(self._h, n, int(field), int(component), *at, C.byref(v)))
+        return v.value
 
     def add_probe(self, field, component, at) -> int:
         n = C.c_int32()

@@ -89,6 +89,8 @@ def oracle(spec, with_fields=True):
     for f, vals, w in spec.get("currents", []):
         s.add_current(f, [v[0] for v in vals], [index(spec["grid"], v[1]) for v in vals],
                       [v[2] for v in vals], O.wave(**{k: v for k, v in w.items() if k != "kind"}, kind=w["kind"]))
+    for lo, hi, d, pol, eps, w in spec.get("planes", []):
+        s.add_plane_wave(lo, hi, d, pol, eps, O.wave(**{k: v for k, v in w.items() if k != "kind"}, kind=w["kind"]))
     for f, c, at in spec.get("probes", []):
         s.add_probe(f, c, at)
     if spec.get("dft"):
@@ -128,12 +130,54 @@ def product(spec, with_fields=True):
     for f, vals, w in spec.get("currents", []):
         s.add_current(P.Current(P.Field(f), [(P.Axis(v[0]), v[1], v[2]) for v in vals],
                                 P.Waveform(w["kind"], w["frequency"], w["width"], w["delay"])))
+    for lo, hi, d, pol, eps, w in spec.get("planes", []):
+        s.add_plane_wave(P.PlaneWave(lo, hi, d, pol, eps, P.Waveform(w["kind"], w["frequency"], w["width"], w["delay"])))
     for f, c, at in spec.get("probes", []):
         s.add_probe(f, c, at)
     if spec.get("dft"):
         lo, hi, fr = spec["dft"]
         s.add_dft(lo, hi, fr)
     return s
+
+
+def tfsf_spec(two_d=False, direction=None, polarization=None, scatterer=True, second=False):
+    """A plane wave on a box in vacuum around a dielectric scatterer (eps 4) and a lossy one,
+    CPMLs with kappa and alpha, a point source outside the box, probes in and out of it and a
+    transform box across its face; optionally a second plane wave on the same box."""
+    h = 0.05
+    if two_d:
+        n = (83, 70, 1)
+        b = dict(periodic=(0, 0, 1), low=(8, 7, 0), high=(9, 7, 0), kappa=2.0, alpha=0.2)
+        low, high = (20, 16, 0), (62, 52, 1)
+        direction = direction or (2, 1, 0)
+        polarization = polarization or (0, 0, 1)
+    else:
+        n = (45, 38, 33)
+        b = dict(periodic=(0, 0, 0), low=(6, 5, 5), high=(6, 6, 5), kappa=2.0, alpha=0.2)
+        low, high = (12, 11, 11), (33, 27, 22)
+        direction = direction or (2, 1, 1)
+        polarization = polarization or (1, -1, 0)
+    grid = (n[0], n[1], n[2], h, h, h, -n[0] * h / 2, -n[1] * h / 2, -n[2] * h / 2)
+    nx, ny, nz = n
+    mid = (nx // 2, ny // 2, nz // 2)
+    width = math.sqrt(2.0 * math.log(2.0)) / (math.pi * 1.5)  # Waveform::pulse(1.2, 1.5)
+    pw = dict(kind=0, frequency=1.2, width=width, delay=6.0 * width)
+    spec = dict(
+        grid=grid, bounds=b, courant=0.9,
+        eps=(1.0, [(0, (-0.25, -0.2, -INF if two_d else -0.15), (0.2, 0.25, INF if two_d else 0.15), 4.0)]
+             if scatterer else []),
+        sigma=(0.0, [(0, (0.25, -0.3, -INF if two_d else -0.1), (0.4, -0.1, INF if two_d else 0.1), 1.5)])
+        if scatterer else None,
+        sources=[(0, 2, (4 + b["low"][0], 3 + b["low"][1], 0 if two_d else nz // 2), pulse())],
+        planes=[(low, high, direction, polarization, 1.0, pw)],
+        probes=[(f, c, at) for at in [mid, (low[0] - 2, mid[1], mid[2]), (high[0] + 1, high[1], mid[2])]
+                for (f, c) in [(0, 2), (1, 1), (0, 0)]],
+        dft=((low[0] - 3, mid[1] - 4, 0 if two_d else mid[2] - 2), (low[0] + 3, mid[1] + 4, 0 if two_d else mid[2] + 2),
+             [0.9, 1.2]),
+    )
+    if second:  # another wave on the same box, along another direction
+        spec["planes"].append((low, high, (1, -3, 0), (0, 0, 1) if two_d else (1, 0, 1), 1.0, pulse()))
+    return spec
 
 
 def product_fields(s):

@@ -75,3 +75,28 @@ def test_errors_are_statuses():
         s.run(1)
     with pytest.raises(NativeError, match="Nyquist"):
         s.add_dft((0, 0, 0), (1, 1, 1), [1e6])
+
+
+def test_plane_wave_refusals_are_the_references():
+    """src/fdtd/tests.rs:350-400, each refused by the host mirror as by the oracle: a face in or
+    against the CPML, the box not whole along a periodic axis, a direction along it, a box
+    across a block, the polarization along the direction, a zero direction."""
+    import photonoxide_amd as P
+    from photonoxide_amd._native import NativeError
+    g = P.Grid3d(40, 40, 1, 0.05, 0.05, 0.05, -1.0, -1.0, -0.025)
+    b = P.Boundaries(P.Pml(6, 6), P.Pml(6, 6), P.Periodic())
+    block = P.Scene(1.0, [P.Box((0.4 + 1e-9, -INF, -INF), (INF, INF, INF), 4.0)])
+    pulse = P.Waveform.pulse(1.0, 0.5)
+    s = P.Simulation.new(g, block, b, 0.9)
+    assert s.add_plane_wave(P.PlaneWave((10, 10, 0), (25, 25, 1), (2, 1, 0), (0, 0, 1), 1.0, pulse)) == 0
+    for low, high, d, pol in [((5, 10, 0), (25, 25, 1), (1, 0, 0), (0, 0, 1)),
+                              ((6, 10, 0), (25, 25, 1), (1, 0, 0), (0, 0, 1)),
+                              ((10, 10, 0), (25, 25, 0), (1, 0, 0), (0, 0, 1)),
+                              ((10, 10, 0), (25, 25, 1), (1, 0, 1), (0, 0, 1)),
+                              ((10, 10, 0), (30, 25, 1), (1, 0, 0), (0, 0, 1)),
+                              ((10, 10, 0), (25, 25, 1), (1, 0, 0), (1, 0, 0)),
+                              ((10, 10, 0), (25, 25, 1), (0, 0, 0), (0, 0, 1))]:
+        with pytest.raises(NativeError):
+            s.add_plane_wave(P.PlaneWave(low, high, d, pol, 1.0, pulse))
+    # the incident field before any step: at rest
+    assert s.incident(0, 0, 2, (15, 15, 0)) == 0.0
