@@ -40,7 +40,8 @@ enum {
     PXFDTD_ERR_UNSUPPORTED = 2, /* what the device doesn't step: Error::Gpu */
     PXFDTD_ERR_MEMORY = 3,      /* it doesn't fit */
     PXFDTD_ERR_DEVICE = 4,      /* the HIP runtime failed */
-    PXFDTD_ERR_TIMEOUT = 5      /* a wait ended before its job did */
+    PXFDTD_ERR_TIMEOUT = 5,     /* a wait ended before its job did */
+    PXFDTD_ERR_GROWING = 6      /* the fields grew again after the sources ended: Error::Growing */
 };
 enum { PXFDTD_F32 = 0, PXFDTD_F64 = 1 };
 enum { PXFDTD_E = 0, PXFDTD_H = 1 };
@@ -296,6 +297,21 @@ int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low[3], const 
 /* poles [np][4]: (0, strength, resonance, damping) Lorentz or (1, plasma, damping, 0) Drude */
 int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf, int32_t np,
                                  const double *poles, const pxfdtd_scene *inside);
+/* Simulation::add_flux (src/fdtd/monitors.rs:470-491): the flux towards +axis through the plane
+ * halfway between nodes `plane` and plane + 1, within window[w] = {lo, hi} (the half-planes
+ * after nodes lo and hi) along the w-th of the axes after `axis` (lo < 0: the whole axis), at
+ * nf frequencies (c/um) from the next step on; its number in *number */
+int pxfdtd_simulation_add_flux(pxfdtd_simulation *s, int32_t axis, int32_t plane,
+                               const int32_t window[2][2], int32_t nf, const double *freqs,
+                               int32_t *number);
+/* Simulation::add_flux_box (monitors.rs:493-528): the flux out of the box between the
+ * half-planes after nodes planes[a][0] and planes[a][1] along each axis a */
+int pxfdtd_simulation_add_flux_box(pxfdtd_simulation *s, const int32_t planes[3][2], int32_t nf,
+                                   const double *freqs, int32_t *number);
+/* Simulation::flux (monitors.rs:530-547): flux monitor n's flux at each of its frequencies,
+ * from the FP64 transform sums (out holds pxfdtd_simulation_flux_frequencies values) */
+int pxfdtd_simulation_flux(pxfdtd_simulation *s, int32_t n, double *out);
+int32_t pxfdtd_simulation_flux_frequencies(const pxfdtd_simulation *s, int32_t n);
 /* Simulation::add_plane_wave (src/fdtd/sources.rs:701-721): a plane wave of `w` on the
  * total-field/scattered-field box of nodes low .. high, travelling along the integer direction,
  * E along the polarization's part across it, in a uniform background of permittivity eps; its
@@ -311,6 +327,16 @@ int pxfdtd_simulation_incident(pxfdtd_simulation *s, int32_t n, int32_t field, i
 /* Steps on `dev` in `precision` from now on (Simulation::set_device); NULL: back to none */
 int pxfdtd_simulation_set_device(pxfdtd_simulation *s, pxfdtd_device *dev, int32_t precision);
 int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count);
+/* Simulation::run_until (src/fdtd.rs:1160-1167): steps until E's time reaches `time` (um/c) */
+int pxfdtd_simulation_run_until(pxfdtd_simulation *s, double time);
+/* Simulation::run_until_decayed (src/fdtd/monitors.rs:740-841): runs in blocks of `interval`
+ * until, over a whole block, |F|^2 at each probe stayed below `fraction` of its largest, or the
+ * time reaches `limit`; *decayed says which. After the sources end (none continuous) it watches
+ * for the fields growing again and stops with PXFDTD_ERR_GROWING (the factor in *growth_factor,
+ * which may be NULL) */
+int pxfdtd_simulation_run_until_decayed(pxfdtd_simulation *s, const int32_t *probes, int32_t n,
+                                        double fraction, double interval, double limit,
+                                        int32_t *decayed, double *growth_factor);
 double pxfdtd_simulation_dt(const pxfdtd_simulation *s);
 int64_t pxfdtd_simulation_steps(const pxfdtd_simulation *s);
 int64_t pxfdtd_simulation_cells(const pxfdtd_simulation *s);

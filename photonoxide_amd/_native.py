@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpxfdtd.so")
 
-OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_MEMORY, ERR_DEVICE, ERR_TIMEOUT = range(6)
+OK, ERR_INVALID, ERR_UNSUPPORTED, ERR_MEMORY, ERR_DEVICE, ERR_TIMEOUT, ERR_GROWING = range(7)
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = range(5)
 F32, F64 = 0, 1
 E, H = 0, 1
@@ -175,6 +175,10 @@ SIGNATURES = {
     "pxfdtd_simulation_add_source": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.POINTER(Wave)]),
     "pxfdtd_simulation_add_current": (C.c_int, [vp, C.c_int32, C.c_int64, pi32, pi64, pd, C.POINTER(Wave)]),
     "pxfdtd_simulation_add_dipole": (C.c_int, [vp, C.c_int32, C.c_int32, pd, C.c_double, C.c_double, C.POINTER(Wave)]),
+    "pxfdtd_simulation_add_flux": (C.c_int, [vp, C.c_int32, C.c_int32, C.POINTER(C.c_int32 * 2), C.c_int32, pd, pi32]),
+    "pxfdtd_simulation_add_flux_box": (C.c_int, [vp, C.POINTER(C.c_int32 * 2), C.c_int32, pd, pi32]),
+    "pxfdtd_simulation_flux": (C.c_int, [vp, C.c_int32, pd]),
+    "pxfdtd_simulation_flux_frequencies": (C.c_int32, [vp, C.c_int32]),
     "pxfdtd_simulation_add_plane_wave": (C.c_int, [vp, pi32, pi32, pi64, pd, C.c_double, C.POINTER(Wave), pi32]),
     "pxfdtd_simulation_incident": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pd]),
     "pxfdtd_simulation_add_probe": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pi32]),
@@ -182,6 +186,8 @@ SIGNATURES = {
     "pxfdtd_simulation_add_medium": (C.c_int, [vp, C.c_double, C.c_int32, pd, C.POINTER(Scene)]),
     "pxfdtd_simulation_set_device": (C.c_int, [vp, vp, C.c_int32]),
     "pxfdtd_simulation_run": (C.c_int, [vp, C.c_int64]),
+    "pxfdtd_simulation_run_until": (C.c_int, [vp, C.c_double]),
+    "pxfdtd_simulation_run_until_decayed": (C.c_int, [vp, pi32, C.c_int32, C.c_double, C.c_double, C.c_double, pi32, pd]),
     "pxfdtd_simulation_dt": (C.c_double, [vp]),
     "pxfdtd_simulation_steps": (C.c_int64, [vp]),
     "pxfdtd_simulation_cells": (C.c_int64, [vp]),
@@ -206,7 +212,7 @@ class NativeError(RuntimeError):
     """A status from the native library, with its class and message."""
 
     def __init__(self, code, message):
-        super().__init__(f"[{ {1: 'invalid', 2: 'unsupported', 3: 'memory', 4: 'device', 5: 'timeout'}.get(code, code)}] {message}")
+        super().__init__(f"[{ {1: 'invalid', 2: 'unsupported', 3: 'memory', 4: 'device', 5: 'timeout', 6: 'growing'}.get(code, code)}] {message}")
         self.code = code
         self.message = message
 

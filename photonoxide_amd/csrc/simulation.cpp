@@ -8,6 +8,7 @@
 // Each set-up computation follows the reference's operation order (cited per function), so the
 // tables it hands the device are the reference's bits.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -77,6 +78,22 @@ struct DftRec {
     std::vector<double> freqs;
     int lo[3], hi[3];
     std::vector<double> acc;  // host-held sums of earlier device layouts, 6 series concatenated
+};
+
+// A flux plane (src/fdtd/monitors.rs:210-350): halfway between nodes `plane` and plane + 1
+// along `axis`, within a window between half-planes along each of the other two axes (or
+// whole); its transforms live in a transform monitor of its own over the plane's nodes either
+// side, taken with the flux's sign (-1 for a box's low faces).
+struct FluxFace {
+    int axis, plane;
+    bool windowed[2];
+    int wlo[2], whi[2];
+    double sign;
+    int dft;  // its transform monitor, an index into the simulation's
+};
+
+struct FluxRec {
+    std::vector<FluxFace> faces;
 };
 
 struct Group {
@@ -186,6 +203,8 @@ struct pxfdtd_simulation {
     std::vector<PlaneRec> planes;
     std::vector<ProbeRec> probes;
     std::vector<DftRec> dfts;
+    std::vector<int> shown;  // the caller's transform monitors: their indices in dfts
+    std::vector<FluxRec> fluxes;
     std::vector<Group> groups;
     std::vector<int32_t> member;  // [3][N] group or -1, when media exist
     // the device
@@ -817,13 +836,18 @@ extern "C" int pxfdtd_simulation_add_probe(pxfdtd_simulation *s, int32_t field, 
 }
 
 // src/fdtd/monitors.rs:395-450
-extern "C" int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low[3],
-                                         const int32_t high[3], int32_t nf, const double *freqs,
-                                         int32_t *number) {
+static int check_frequencies(const pxfdtd_simulation *s, int32_t nf, const double *freqs) {
     if (nf <= 0) return px_fail(PXFDTD_ERR_INVALID, "a monitor needs at least one frequency");
     for (int f = 0; f < nf; f++)
         if (!(freqs[f] > 0) || TAU * freqs[f] * s->dt >= 3.141592653589793)
             return px_fail(PXFDTD_ERR_INVALID, "%g c/um is beyond the time step's Nyquist frequency", freqs[f]);
+    return PXFDTD_OK;
+}
+
+// a transform monitor over nodes low..high (inclusive); its index in dfts
+static int add_transform(pxfdtd_simulation *s, const int32_t low[3], const int32_t high[3],
+                         int32_t nf, const double *freqs, int *index) {
+    TRY(check_frequencies(s, nf, freqs));
     TRY(inside(s, high[0], high[1], high[2]));
     for (int a = 0; a < 3; a++)
         if (low[a] < 0 || low[a] > high[a])
@@ -840,8 +864,114 @@ extern "C" int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low
     // sums so far on the device are kept on the host before the device's layout changes
     TRY(bank_sums(s));
     s->dfts.push_back(std::move(d));
-    *number = (int32_t)s->dfts.size() - 1;
+    *index = (int)s->dfts.size() - 1;
     s->taps_dirty = true;
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low[3],
+                                         const int32_t high[3], int32_t nf, const double *freqs,
+                                         int32_t *number) {
+    int index;
+    TRY(add_transform(s, low, high, nf, freqs, &index));
+    s->shown.push_back(index);
+    *number = (int32_t)s->shown.size() - 1;
+    return PXFDTD_OK;
+}
+
+// FluxPlane::range: the indices along the window's axis slot w whose values it reaches
+static void flux_range(const pxfdtd_simulation *s, const FluxFace &F, int w, int &from, int &to) {
+    const int along = (F.axis + 1 + w) % 3;
+    if (F.windowed[w]) {
+        from = F.wlo[w];
+        to = F.whi[w] + 1;
+    } else {
+        from = 0;
+        to = s->n[along];
+    }
+}
+
+// FluxPlane::weight: a value's share of the window, halfway after node m (`half`) or on it
+static double flux_weight(const FluxFace &F, int w, int m, bool half) {
+    if (!F.windowed[w]) return 1.0;
+    const int lo = F.wlo[w], hi = F.whi[w];
+    if (half) return (lo < m && m < hi) ? 1.0 : ((m == lo || m == hi) ? 0.5 : 0.0);
+    return (lo < m && m <= hi) ? 1.0 : 0.0;
+}
+
+// FluxPlane::check and the face's transform monitor (FluxPlane::series, monitors.rs:267-316)
+static int add_face(pxfdtd_simulation *s, FluxFace &F, int32_t nf, const double *freqs) {
+    const int a = F.axis;
+    if (F.plane < 0 || F.plane + 1 >= s->n[a])
+        return px_fail(PXFDTD_ERR_INVALID, "a flux plane after node %d along axis %d needs node %d on the grid",
+                       F.plane, a, F.plane + 1);
+    int32_t low[3], high[3];
+    low[a] = F.plane;
+    high[a] = F.plane + 1;
+    for (int w = 0; w < 2; w++) {
+        const int along = (a + 1 + w) % 3;
+        if (F.windowed[w] && !(F.wlo[w] >= 0 && F.wlo[w] < F.whi[w] && F.whi[w] < s->n[along]))
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "a flux plane's window along axis %d must run from one half-plane to a later one on "
+                           "the grid, got %d to %d of %d", along, F.wlo[w], F.whi[w], s->n[along]);
+        int from, to;
+        flux_range(s, F, w, from, to);
+        low[along] = from;
+        high[along] = to - 1;
+    }
+    return add_transform(s, low, high, nf, freqs, &F.dft);
+}
+
+// Simulation::add_flux (src/fdtd/monitors.rs:470-491): window[w] = {lo, hi} along the w-th of
+// the axes after `axis`, or lo < 0 for the whole axis
+extern "C" int pxfdtd_simulation_add_flux(pxfdtd_simulation *s, int32_t axis, int32_t plane,
+                                          const int32_t window[2][2], int32_t nf, const double *freqs,
+                                          int32_t *number) {
+    if (axis < 0 || axis > 2) return px_fail(PXFDTD_ERR_INVALID, "no axis %d", axis);
+    TRY(check_frequencies(s, nf, freqs));
+    FluxFace F;
+    F.axis = axis;
+    F.plane = plane;
+    F.sign = 1.0;
+    for (int w = 0; w < 2; w++) {
+        F.windowed[w] = window && window[w][0] >= 0;
+        F.wlo[w] = F.windowed[w] ? window[w][0] : 0;
+        F.whi[w] = F.windowed[w] ? window[w][1] : 0;
+    }
+    TRY(add_face(s, F, nf, freqs));
+    s->fluxes.push_back({{F}});
+    *number = (int32_t)s->fluxes.size() - 1;
+    return PXFDTD_OK;
+}
+
+// Simulation::add_flux_box (monitors.rs:493-528): the six faces of the box between the
+// half-planes after nodes planes[a][0] and planes[a][1] along each axis a, outwards
+extern "C" int pxfdtd_simulation_add_flux_box(pxfdtd_simulation *s, const int32_t planes[3][2],
+                                              int32_t nf, const double *freqs, int32_t *number) {
+    TRY(check_frequencies(s, nf, freqs));
+    FluxRec R;
+    for (int a = 0; a < 3; a++) {
+        const int lo = planes[a][0], hi = planes[a][1];
+        if (lo >= hi)
+            return px_fail(PXFDTD_ERR_INVALID, "a flux box's planes along axis %d must be from low to high, got %d and %d",
+                           a, lo, hi);
+        for (int side = 0; side < 2; side++) {
+            FluxFace F;
+            F.axis = a;
+            F.plane = side ? hi : lo;
+            F.sign = side ? 1.0 : -1.0;
+            for (int w = 0; w < 2; w++) {
+                const int along = (a + 1 + w) % 3;
+                F.windowed[w] = true;
+                F.wlo[w] = planes[along][0];
+                F.whi[w] = planes[along][1];
+            }
+            TRY(add_face(s, F, nf, freqs));
+            R.faces.push_back(F);
+        }
+    }
+    s->fluxes.push_back(std::move(R));
+    *number = (int32_t)s->fluxes.size() - 1;
     return PXFDTD_OK;
 }
 
@@ -1215,6 +1345,110 @@ extern "C" int pxfdtd_simulation_run(pxfdtd_simulation *s, int64_t count) {
     return PXFDTD_OK;
 }
 
+// Simulation::run_until (src/fdtd.rs:1160-1167): steps until E's time reaches `time`
+extern "C" int pxfdtd_simulation_run_until(pxfdtd_simulation *s, double time) {
+    if (std::isnan(time)) return px_fail(PXFDTD_ERR_INVALID, "a time to run until must be a number");
+    int64_t count = 0;
+    while ((double)(s->steps + count) * s->dt < time) count++;
+    return pxfdtd_simulation_run(s, count);
+}
+
+namespace {
+
+// Growth (src/fdtd/monitors.rs:665-717): rising at each of the last `needed` checks, to a
+// thousand times the level before the first of them
+struct Growth {
+    int needed;
+    double start = 0.0, last = 0.0;
+    int rises = 0;
+    explicit Growth(int n) : needed(n) {}
+    // for checks every `check` after a pulse that ends at `pulse_end`
+    static Growth after(double pulse_end, double check) {
+        return Growth(std::max((int)std::ceil(pulse_end / check) + 1, 4));
+    }
+    // the factor the fields grew by, or 0 while they aren't growing
+    double take(double level) {
+        if (last > 0.0 && level > last) {
+            if (rises == 0) start = last;
+            rises++;
+        } else {
+            rises = 0;
+        }
+        last = level;
+        return (rises >= needed && level >= 1e3 * start) ? level / start : 0.0;
+    }
+};
+
+}  // namespace
+
+// when the last source ends (monitors.rs:719-738): a pulse 6 widths after its peak; false for
+// a continuous wave, which never ends
+static bool sources_end(const pxfdtd_simulation *s, double &end) {
+    end = 0.0;
+    auto take = [&](const pxfdtd_waveform &w) {
+        if (w.kind == 2) return false;
+        end = std::max(end, w.delay + 6.0 * w.width);
+        return true;
+    };
+    for (auto &so : s->sources)
+        if (!take(so.w)) return false;
+    for (auto &cu : s->currents)
+        if (!take(cu.w)) return false;
+    for (auto &pl : s->planes)
+        if (!take(pl.w)) return false;
+    return true;
+}
+
+// Simulation::run_until_decayed (monitors.rs:740-841)
+extern "C" int pxfdtd_simulation_run_until_decayed(pxfdtd_simulation *s, const int32_t *probes, int32_t n,
+                                                   double fraction, double interval, double limit,
+                                                   int32_t *decayed, double *growth_factor) {
+    if (n <= 0 || !probes || !(fraction > 0.0 && fraction < 1.0) || !(std::isfinite(interval) && interval > 0.0) ||
+        !std::isfinite(limit))
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "running until the fields decay needs probes, a fraction in (0, 1), a positive interval and a "
+                       "finite limit, got %d probes, %g, %g and %g", n, fraction, interval, limit);
+    for (int q = 0; q < n; q++)
+        if (probes[q] < 0 || probes[q] >= (int32_t)s->probes.size())
+            return px_fail(PXFDTD_ERR_INVALID, "no probe %d", probes[q]);
+    *decayed = 0;
+    if (growth_factor) *growth_factor = 0.0;
+    double end;
+    const bool ends = sources_end(s, end);
+    Growth growth = Growth::after(ends ? end : 0.0, interval);
+    while ((double)s->steps * s->dt < limit) {
+        const size_t from = s->probes[probes[0]].values.size();
+        const double began = (double)s->steps * s->dt;
+        TRY(pxfdtd_simulation_run_until(s, std::min(began + interval, limit)));
+        bool all = true;
+        double level = 0.0;
+        for (int q = 0; q < n; q++) {
+            const std::vector<double> &v = s->probes[probes[q]].values;
+            double peak = 0.0, recent = 0.0;
+            for (size_t t = 0; t < v.size(); t++) {
+                peak = std::max(peak, v[t] * v[t]);
+                if (t >= std::min(from, v.size())) recent = std::max(recent, v[t] * v[t]);
+            }
+            all = all && peak > 0.0 && recent <= fraction * peak;
+            level = std::max(level, recent);
+        }
+        if (all) {
+            *decayed = 1;
+            return PXFDTD_OK;
+        }
+        if (ends && began >= end) {
+            const double factor = growth.take(level);
+            if (factor > 0.0) {
+                if (growth_factor) *growth_factor = factor;
+                return px_fail(PXFDTD_ERR_GROWING,
+                               "the fields grew %g times over %d checks to t = %g um/c: a wave of the grid amplified "
+                               "in a CPML, which a finer grid removes", factor, growth.needed, (double)s->steps * s->dt);
+            }
+        }
+    }
+    return PXFDTD_OK;
+}
+
 extern "C" double pxfdtd_simulation_dt(const pxfdtd_simulation *s) { return s->dt; }
 extern "C" int64_t pxfdtd_simulation_steps(const pxfdtd_simulation *s) { return s->steps; }
 extern "C" int64_t pxfdtd_simulation_cells(const pxfdtd_simulation *s) { return s->N; }
@@ -1257,13 +1491,13 @@ extern "C" int pxfdtd_simulation_probe(pxfdtd_simulation *s, int32_t n, double *
 }
 
 extern "C" int64_t pxfdtd_simulation_dft_len(pxfdtd_simulation *s, int32_t n, int32_t series) {
-    if (n < 0 || n >= (int32_t)s->dfts.size() || series < 0 || series > 5) return -1;
-    return (int64_t)s->dfts[n].acc.size() / 6;
+    if (n < 0 || n >= (int32_t)s->shown.size() || series < 0 || series > 5) return -1;
+    return (int64_t)s->dfts[s->shown[n]].acc.size() / 6;
 }
 
-extern "C" int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t series, double *out) {
-    if (n < 0 || n >= (int32_t)s->dfts.size() || series < 0 || series > 5)
-        return px_fail(PXFDTD_ERR_INVALID, "no transform %d series %d", n, series);
+// the sums of transform monitor `index` (an index into dfts), series `series`, into out
+static int transform_sums(pxfdtd_simulation *s, int index, int series, double *out) {
+    const int n = index;
     const DftRec &d = s->dfts[n];
     const size_t len = d.acc.size() / 6;
     if (s->res && !s->taps_dirty) {
@@ -1277,6 +1511,79 @@ extern "C" int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t se
         std::memcpy(out, d.acc.data() + series * len, len * sizeof(double));
     }
     return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_simulation_dft(pxfdtd_simulation *s, int32_t n, int32_t series, double *out) {
+    if (n < 0 || n >= (int32_t)s->shown.size() || series < 0 || series > 5)
+        return px_fail(PXFDTD_ERR_INVALID, "no transform %d series %d", n, series);
+    return transform_sums(s, s->shown[n], series, out);
+}
+
+// FluxPlane::flux (monitors.rs:318-350): S.n = 1/2 Re(E_b H~_c* - E_c H~_b*) over the plane, E
+// the mean of the nodes either side; summed in the reference's order
+static double face_flux(pxfdtd_simulation *s, const FluxFace &F, const std::vector<double> series[6], int f) {
+    const DftRec &d = s->dfts[F.dft];
+    const int a = F.axis, b = (a + 1) % 3, c = (a + 2) % 3;
+    const int ex = d.hi[0] - d.lo[0], ey = d.hi[1] - d.lo[1], ez = d.hi[2] - d.lo[2];
+    auto value = [&](int field, int comp, const int at[3]) {
+        const std::vector<double> &v = series[field * 3 + comp];
+        const long long q = (((long long)f * ez + (at[2] - d.lo[2])) * ey + (at[1] - d.lo[1])) * ex + (at[0] - d.lo[0]);
+        return C64{v[2 * q], v[2 * q + 1]};
+    };
+    int bfrom, bto, cfrom, cto;
+    flux_range(s, F, 0, bfrom, bto);
+    flux_range(s, F, 1, cfrom, cto);
+    double total = 0.0;
+    const int comps[2] = {b, c}, hs[2] = {c, b};
+    const double signs[2] = {0.5, -0.5};
+    for (int q = 0; q < 2; q++) {
+        const int comp = comps[q], h = hs[q];
+        for (int v = cfrom; v < cto; v++)
+            for (int u = bfrom; u < bto; u++) {
+                const double weight = flux_weight(F, 0, u, comp == b) * flux_weight(F, 1, v, comp == c);
+                if (weight == 0.0) continue;
+                int at[3];
+                at[a] = F.plane;
+                at[b] = u;
+                at[c] = v;
+                const C64 low = value(PXFDTD_E, comp, at);
+                const C64 hv = value(PXFDTD_H, h, at);
+                at[a] = F.plane + 1;
+                const C64 high = value(PXFDTD_E, comp, at);
+                const double mre = 0.5 * (low.re + high.re), mim = 0.5 * (low.im + high.im);
+                // Re(m conj(hv)) as num-complex multiplies: m.re hv.re - m.im (-hv.im)
+                total += weight * signs[q] * (mre * hv.re - mim * -hv.im);
+            }
+    }
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    return total * steps[b] * steps[c];
+}
+
+// Simulation::flux (monitors.rs:530-547): per frequency, the faces' signed fluxes summed
+extern "C" int pxfdtd_simulation_flux(pxfdtd_simulation *s, int32_t n, double *out) {
+    if (n < 0 || n >= (int32_t)s->fluxes.size()) return px_fail(PXFDTD_ERR_INVALID, "no flux monitor %d", n);
+    const FluxRec &R = s->fluxes[n];
+    const int nf = (int)s->dfts[R.faces[0].dft].freqs.size();
+    std::vector<std::array<std::vector<double>, 6>> sums(R.faces.size());
+    for (size_t q = 0; q < R.faces.size(); q++) {
+        const DftRec &d = s->dfts[R.faces[q].dft];
+        for (int t = 0; t < 6; t++) {
+            sums[q][t].resize(d.acc.size() / 6);
+            TRY(transform_sums(s, R.faces[q].dft, t, sums[q][t].data()));
+        }
+    }
+    for (int f = 0; f < nf; f++) {
+        // a sum of f64s from -0.0, as Rust's Sum for f64 folds (Rust 1.83 on)
+        double total = -0.0;
+        for (size_t q = 0; q < R.faces.size(); q++) total += R.faces[q].sign * face_flux(s, R.faces[q], sums[q].data(), f);
+        out[f] = total;
+    }
+    return PXFDTD_OK;
+}
+
+extern "C" int32_t pxfdtd_simulation_flux_frequencies(const pxfdtd_simulation *s, int32_t n) {
+    if (n < 0 || n >= (int32_t)s->fluxes.size()) return -1;
+    return (int32_t)s->dfts[s->fluxes[n].faces[0].dft].freqs.size();
 }
 
 // ---------------------------------------------------------------- for the serving loop
@@ -1321,16 +1628,20 @@ int sim_emitters(const pxfdtd_simulation *s) { return s->emitters; }
 void sim_set_steps(pxfdtd_simulation *s, int64_t steps) { s->steps = steps; }
 
 double sim_dft_frequency(const pxfdtd_simulation *s, int monitor, int f) {
-    if (monitor < 0 || monitor >= (int)s->dfts.size() || f < 0 || f >= (int)s->dfts[monitor].freqs.size()) return -1.0;
-    return s->dfts[monitor].freqs[f];
+    if (monitor < 0 || monitor >= (int)s->shown.size()) return -1.0;
+    const DftRec &d = s->dfts[s->shown[monitor]];
+    if (f < 0 || f >= (int)d.freqs.size()) return -1.0;
+    return d.freqs[f];
 }
 
-// the monitor's box, lo inclusive and hi exclusive
+// the caller's transform monitor's box, lo inclusive and hi exclusive, and its index among
+// all the simulation's transform monitors (the device's series order)
 int sim_dft_box(const pxfdtd_simulation *s, int monitor, int lo[3], int hi[3]) {
-    if (monitor < 0 || monitor >= (int)s->dfts.size()) return px_fail(PXFDTD_ERR_INVALID, "no transform monitor %d", monitor);
+    if (monitor < 0 || monitor >= (int)s->shown.size()) return px_fail(PXFDTD_ERR_INVALID, "no transform monitor %d", monitor);
+    const DftRec &d = s->dfts[s->shown[monitor]];
     for (int a = 0; a < 3; a++) {
-        lo[a] = s->dfts[monitor].lo[a];
-        hi[a] = s->dfts[monitor].hi[a];
+        lo[a] = d.lo[a];
+        hi[a] = d.hi[a];
     }
     return PXFDTD_OK;
 }

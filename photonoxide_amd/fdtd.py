@@ -356,6 +356,35 @@ class Simulation:
                                                    N.dptr(pos), a.real, a.imag,
                                                    C.byref(dipole.waveform.native())))
 
+    def add_flux(self, axis, plane: int, frequencies, window=(None, None)) -> int:
+        """Simulation::add_flux (src/fdtd/monitors.rs:470-491): the flux towards +axis through
+        the plane halfway after node `plane`; window[w] = (lo, hi) half-planes along the w-th
+        of the axes after `axis`, or None for the whole axis."""
+        win = (C.c_int32 * 2 * 2)(*[(C.c_int32 * 2)(*(w if w is not None else (-1, -1))) for w in window])
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        n = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_add_flux(self._h, int(axis), int(plane), win, len(f), N.dptr(f),
+                                                 C.byref(n)))
+        return n.value
+
+    def add_flux_box(self, planes, frequencies) -> int:
+        """Simulation::add_flux_box (monitors.rs:493-528): the flux out of the box between the
+        half-planes after nodes planes[a][0] and planes[a][1] along each axis a."""
+        pl = (C.c_int32 * 2 * 3)(*[(C.c_int32 * 2)(*p) for p in planes])
+        f = np.ascontiguousarray(frequencies, dtype=np.float64)
+        n = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_add_flux_box(self._h, pl, len(f), N.dptr(f), C.byref(n)))
+        return n.value
+
+    def flux(self, n: int) -> np.ndarray:
+        """Simulation::flux (monitors.rs:530-547): per frequency."""
+        m = N.lib.pxfdtd_simulation_flux_frequencies(self._h, n)
+        if m < 0:
+            raise IndexError(f"no flux monitor {n}")
+        out = np.zeros(m)
+        N.check(N.lib.pxfdtd_simulation_flux(self._h, n, N.dptr(out)))
+        return out
+
     def add_plane_wave(self, wave: PlaneWave) -> int:
         """Simulation::add_plane_wave (src/fdtd/sources.rs:701-721): its number."""
         n = C.c_int32()
@@ -414,6 +443,21 @@ class Simulation:
 
     def run(self, count: int):
         N.check(N.lib.pxfdtd_simulation_run(self._h, int(count)))
+
+    def run_until(self, time: float):
+        """Steps until E's time reaches `time` (src/fdtd.rs:1160-1167)."""
+        N.check(N.lib.pxfdtd_simulation_run_until(self._h, float(time)))
+
+    def run_until_decayed(self, probes, fraction: float, interval: float, limit: float) -> bool:
+        """Simulation::run_until_decayed (src/fdtd/monitors.rs:740-841): whether the fields at
+        the probes decayed below `fraction` of their peak over a block of `interval` before
+        `limit`; raises NativeError (code ERR_GROWING) if they grow again after the sources end."""
+        pr = np.ascontiguousarray(probes, dtype=np.int32)
+        d = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_run_until_decayed(self._h, pr.ctypes.data_as(N.pi32), len(pr),
+                                                          float(fraction), float(interval), float(limit),
+                                                          C.byref(d), None))
+        return bool(d.value)
 
     def dt(self) -> float:
         return N.lib.pxfdtd_simulation_dt(self._h)

@@ -150,6 +150,9 @@ class Sim:
     def dt(self):
         return lib.or_sim_dt(self.h)
 
+    def steps(self):
+        return lib.or_sim_steps(self.h)
+
     def run(self, n):
         lib.or_sim_run(self.h, n)
 

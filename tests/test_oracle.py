@@ -346,3 +346,133 @@ def test_plane_wave_on_a_slab_reflects_as_airy_says_to_second_order():
     assert e[1] < e[0] / 3 and e[2] < e[1] / 3, e
     assert e[2] < 1e-3, e
     assert e == pytest.approx([5.4e-3, 1.3e-3, 3.3e-4], rel=0.05)
+
+
+def box_balance_problem(inside, plant=None):
+    """src/fdtd/monitors_checks.rs:188-245 on the oracle: a walled 16 x 14 x 12 box, lossy
+    (sigma 3) but where the flux box's values are, a block of eps 4, an E_y pulse in or out of the
+    flux box; the flux box and its six faces as planes, run to t = 200. Returns (net, largest)
+    over the two frequencies."""
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import flux as F
+    n, h = (16, 14, 12), 0.05
+    planes = [(5, 10), (4, 9), (3, 8)]
+    x0 = [-m * h / 2 for m in n]
+    sig = O.scene(3.0, [(0, tuple(x0[a] + planes[a][0] * h - 2e-9 for a in range(3)),
+                         tuple(x0[a] + (planes[a][1] + 1) * h + 2e-9 for a in range(3)), 0.0)])
+    s = O.Sim((*n, h, h, h, *x0), O.bounds(), O.scene(1.0, [(0, (-0.08, -0.06, -0.05), (0.08, 0.06, 0.05), 4.0)]), 0.9)
+    s.conductivity(sig)
+    f0 = 1.0 / 1.55
+    width = math.sqrt(2 * math.log(2)) / (math.pi * 0.3)
+    s.add_source(0, 1, (8, 6, 5) if inside else (2, 6, 5), O.wave(0, f0, width, 6 * width))
+    freqs = [f0, f0 + 0.15]
+    faces = F.box_faces(planes)
+    if plant:
+        plant(faces)
+    dfts = []
+    for face in faces:
+        lo, hi = face.box(n)
+        dfts.append((s.add_dft(lo, hi, freqs), lo))
+    count = 0
+    while count * s.dt() < 200.0:
+        count += 1
+    s.run(count)
+
+    def value_of(q):
+        m, lo = dfts[q]
+        series = {(fd, c): s.dft(m, fd, c) for fd in (0, 1) for c in range(3)}
+        return lambda fd, c, at, f: series[fd, c][f, at[2] - lo[2], at[1] - lo[1], at[0] - lo[0]]
+
+    values = [value_of(q) for q in range(6)]
+    net = max(abs(F.total_flux(faces, n, (h, h, h), values, f)) for f in range(2))
+    largest = max(abs(face.flux(n, (h, h, h), values[q], f)) for q, face in enumerate(faces) for f in range(2))
+    return net, largest
+
+
+def test_flux_box_balances_in_a_closed_lossless_box():
+    """src/fdtd/monitors_tests.rs:18-25: no net flux leaves a lossless box with no source in it;
+    with the source inside, what leaves is what it radiates."""
+    net, largest = box_balance_problem(False)
+    assert largest > 0 and net < 1e-12 * largest, (net, largest)
+    net, largest = box_balance_problem(True)
+    assert net > 0.1 * largest, (net, largest)
+
+
+def test_flux_balance_sees_a_planted_mistake():
+    """The balance is sensitive: an edge value's half weight taken whole, or a face's sign
+    flipped, planted in the oracle's flux, leaves a net flux far above round-off."""
+    def whole_edges(faces):
+        for f in faces:
+            f.weight = (lambda w, m, half, f=f: 1.0 if f.window[w] is None else
+                        (1.0 if f.window[w][0] <= m <= f.window[w][1] and half else
+                         (1.0 if f.window[w][0] < m <= f.window[w][1] else 0.0)))
+
+    def flipped(faces):
+        faces[2].sign = -faces[2].sign
+
+    for plant in (whole_edges, flipped):
+        net, largest = box_balance_problem(False, plant)
+        assert net > 1e-6 * largest, (plant.__name__, net, largest)
+
+
+def _control():
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+    import control
+    return control
+
+
+def test_growth_fires_only_after_enough_rises_to_a_thousand_times():
+    """Growth (src/fdtd/monitors.rs:679-717) on hand-made levels."""
+    K = _control()
+    assert K.Growth.after(10.0, 1.0).needed == 11 and K.Growth.after(0.5, 1.0).needed == 4
+    g = K.Growth(4)
+    levels = [1.0, 10.0, 100.0, 1000.0, 10000.0]
+    out = [g.check(v) for v in levels]
+    assert out[:4] == [None] * 4 and out[4] == 1e4  # 4 rises from 1.0, reaching 1e4 >= 1e3
+    g = K.Growth(4)
+    out = [g.check(v) for v in [1.0, 2.0, 3.0, 4.0, 5.0, 6.0]]
+    assert out == [None] * 6  # rising, but not a thousand times
+    g = K.Growth(2)
+    out = [g.check(v) for v in [1.0, 100.0, 50.0, 5000.0, 6e4]]
+    assert out == [None, None, None, None, 6e4 / 50.0]  # the fall restarts the count
+
+
+def _open_box(cw=False):
+    n, h, cp = (40, 36, 1), 0.05, 8
+    s = O.Sim((*n, h, h, h, 0, 0, 0), O.bounds(periodic=(0, 0, 1), low=(cp, cp, 0), high=(cp, cp, 0)),
+              O.scene(1.0, [(0, (0.8, 0.7, -INF), (1.2, 1.1, INF), 3.0)]), 0.9)
+    w = O.wave(2, 1.0, 0.0, 0.0, 1.0, ramp=2.0) if cw else O.wave(0, 1.0, 0.3, 1.8)
+    s.add_source(0, 2, (14, 15, 0), w)
+    p = [s.add_probe(0, 2, (25, 20, 0)), s.add_probe(1, 0, (20, 25, 0))]
+    return s, p, [(2, 0.0, 0.0)] if cw else [(0, 0.3, 1.8)]
+
+
+def test_run_until_stops_at_the_first_step_past_the_time():
+    K = _control()
+    s, _, _ = _open_box()
+    K.run_until(s, 1.0)
+    n = s.steps()
+    assert n * s.dt() >= 1.0 and (n - 1) * s.dt() < 1.0
+    K.run_until(s, 0.5)  # already past: no step
+    assert s.steps() == n
+
+
+def test_run_until_decayed_on_a_pulse_and_a_continuous_wave():
+    """A pulse in an open box decays: the last block's values all stay under the fraction of
+    their peak and the block before's do not. A continuous wave never does: the run ends at the
+    limit."""
+    K = _control()
+    s, p, w = _open_box()
+    assert K.run_until_decayed(s, p, 1e-4, 0.5, 40.0, w)
+    t = s.steps() * s.dt()
+    assert t < 40.0
+    for n in p:
+        v = np.asarray(s.probe(n)) ** 2
+        last = int(round(0.5 / s.dt()))
+        assert v[-last + 1:].max() <= 1e-4 * v.max()
+    s, p, w = _open_box(cw=True)
+    assert not K.run_until_decayed(s, p, 1e-4, 1.0, 8.0, w)
+    assert s.steps() * s.dt() >= 8.0 and (s.steps() - 1) * s.dt() < 8.0
