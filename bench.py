@@ -352,7 +352,9 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                      "kernel": ["fused_step", "march_step", "march_step", "march64_step"][s.kernel()], "kernel_ms": round(kernel_ms, 5),
                      "bytes_per_launch": bytes_step,
-                     "gcell_per_s_kernel": round(cells / (kernel_ms * 1e-3) / 1e9, 3)},
+                     "gcell_per_s_kernel": round(cells / (kernel_ms * 1e-3) / 1e9, 3),
+                     # the reference's own byte model (two-pass, f32 ca/cb arrays), for comparison
+                     "reference_bytes_per_step": s.bytes_per_step(P.Precision.F32)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, ny, nz, pml)

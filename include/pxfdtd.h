@@ -355,6 +355,9 @@ pxfdtd_resident *pxfdtd_simulation_resident(pxfdtd_simulation *s);
 /* the device bytes the simulation needs in `precision` (fields in two copies, psi, coefficient
  * tables, media state, monitor staging and FP64 sums): what admission by footprint counts */
 int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precision, int64_t *bytes);
+/* Yee::bytes_per_step (src/fdtd/kernel/yee.rs:141-153), the reference's byte model of one step:
+ * (24 N + 6 per E psi value + 5 per H psi value) values of the precision's size */
+int pxfdtd_simulation_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes);
 
 /* ------------------------------------------------------------ level 3: the serving loop */
 /* Continuous batching of jobs on one device (no counterpart in the reference; its nearest is

@@ -1700,3 +1700,14 @@ extern "C" int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precisi
     *bytes = (int64_t)b;
     return PXFDTD_OK;
 }
+
+// Yee::bytes_per_step (src/fdtd/kernel/yee.rs:141-153): the reference's byte model of a step,
+// (24 N + 6 psi_E + 5 psi_H) values of the precision's size
+extern "C" int pxfdtd_simulation_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes) {
+    if (precision != PXFDTD_F32 && precision != PXFDTD_F64)
+        return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");
+    long long slabs = 0;
+    for (const Slab &sl : s->slabs) slabs += (long long)sl.psi.size() * (sl.field == PXFDTD_E ? 6 : 5);
+    *bytes = (int64_t)((s->N * (9 + 15) + slabs) * (precision == PXFDTD_F32 ? 4 : 8));
+    return PXFDTD_OK;
+}

@@ -444,6 +444,13 @@ class Simulation:
     def run(self, count: int):
         N.check(N.lib.pxfdtd_simulation_run(self._h, int(count)))
 
+    def bytes_per_step(self, precision=None) -> int:
+        """The reference's byte model of a step (Yee::bytes_per_step, src/fdtd/kernel/yee.rs:141-153)."""
+        b = C.c_int64()
+        N.check(N.lib.pxfdtd_simulation_bytes_per_step(self._h, int(Precision.F32 if precision is None else precision),
+                                                       C.byref(b)))
+        return b.value
+
     def run_until(self, time: float):
         """Steps until E's time reaches `time` (src/fdtd.rs:1160-1167)."""
         N.check(N.lib.pxfdtd_simulation_run_until(self._h, float(time)))

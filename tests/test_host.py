@@ -100,3 +100,25 @@ def test_plane_wave_refusals_are_the_references():
             s.add_plane_wave(P.PlaneWave(low, high, d, pol, 1.0, pulse))
     # the incident field before any step: at rest
     assert s.incident(0, 0, 2, (15, 15, 0)) == 0.0
+
+
+def test_bytes_per_step_is_the_references_model():
+    """Yee::bytes_per_step (src/fdtd/kernel/yee.rs:141-153): (24 N + 6 psi_E + 5 psi_H) values,
+    the psi counts from the oracle's slabs."""
+    import ctypes as C
+    import photonoxide_amd as P
+    from fixtures import product
+    spec = busy_spec(False)
+    o = oracle(spec, with_fields=False)
+    p = product(spec, with_fields=False)
+    psi = {0: 0, 1: 0}
+    for q in range(O.lib.or_sim_slab_count(o.h)):
+        info = (C.c_int32 * 5)()
+        b, a, s = O.pd(), O.pd(), O.pd()
+        n = C.c_int64()
+        O.lib.or_sim_slab(o.h, q, info, C.byref(b), C.byref(a), C.byref(s), C.byref(n))
+        psi[info[0]] += n.value
+    cells = spec["grid"][0] * spec["grid"][1] * spec["grid"][2]
+    want = 24 * cells + 6 * psi[0] + 5 * psi[1]
+    assert p.bytes_per_step(P.Precision.F32) == 4 * want
+    assert p.bytes_per_step(P.Precision.F64) == 8 * want
