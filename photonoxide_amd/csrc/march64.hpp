@@ -180,6 +180,14 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
               fhz = 2 * (nx + ny) + nz;
     const T fxh = coef[fhx + max(xi, 0)], fxe = coef[fex + max(xi, 0)];
     const T fxh_l = coef[fhx + max(xl, 0)], fyh_l = coef[fhy + max(gyl, 0)];
+    // the rows' difference factors, fixed for the unit: in registers, not reloaded each plane
+    T fyh_r[RY + 1], fye_r[RY + 1];
+#pragma unroll
+    for (int r = 0; r <= RY; r++) {
+        const int jr = wrap_any(j0 - 1 + r, ny, pery);
+        fyh_r[r] = coef[fhy + max(jr, 0)];
+        fye_r[r] = coef[fey + max(jr, 0)];
+    }
 
     // CPML geometry fixed for the unit: the x slab per lane (and at the column left of the
     // strip), the y slab per row. psi moves through buffer loads and stores whose offset is
@@ -250,37 +258,61 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
         P.gz = bload<T>(rr, go_yz, s + 2 * cbytes);
     };
 
-    EPlane cur;
-    load_e(cur, kfirst);
-    T hp[RY][2];  // new Hx, Hy of the plane below, on the strip's rows
-#pragma unroll
-    for (int r = 0; r < RY; r++) hp[r][0] = hp[r][1] = T(0);
-
-    for (int kk = kfirst; kk < kz1; kk++) {
-        const bool pro = kk < kz0;
-        const int k = zwrap(kk);
-        const uint32_t sk = (uint32_t)k * pbytes;
-        EPlane nxt;
-        load_e(nxt, kk + 1);
+    // old H of a plane (own rows j0 - 1 .. j0 + RY - 1, the left column's Hy, Hz) and the
+    // material ids of the strip's rows
+    struct HPlane {
         T ho[RY + 1][3];
+        T ghy, ghz;
+        T fzh, fze;  // the plane's difference factors along z
+        uint8_t id[RY];
+    };
+    auto load_h = [&](HPlane &P, int kk) {
+        const int kp = zwrap(kk);
+        const __amdgpu_buffer_rsrc_t rr = kp >= 0 ? rH : rnone;
+        const __amdgpu_buffer_rsrc_t ri = kp >= 0 ? rid : rnone;
+        const uint32_t s = kp >= 0 ? (uint32_t)kp * pbytes : 0u;
 #pragma unroll
         for (int r = 0; r <= RY; r++) {
-            ho[r][0] = bload<T>(rH, vo[r], sk);
-            ho[r][1] = r == 0 ? T(0) : bload<T>(rH, vo[r], sk + cbytes);
-            ho[r][2] = bload<T>(rH, vo[r], sk + 2 * cbytes);
+            P.ho[r][0] = bload<T>(rr, vo[r], s);
+            P.ho[r][1] = r == 0 ? T(0) : bload<T>(rr, vo[r], s + cbytes);
+            P.ho[r][2] = bload<T>(rr, vo[r], s + 2 * cbytes);
         }
-        const T ghy = bload<T>(rH, go_h, sk + cbytes), ghz = bload<T>(rH, go_h, sk + 2 * cbytes);
-        uint8_t id[RY];
+        P.ghy = bload<T>(rr, go_h, s + cbytes);
+        P.ghz = bload<T>(rr, go_h, s + 2 * cbytes);
+        P.fzh = coef[fhz + max(kp, 0)];
+        P.fze = coef[fez + max(kp, 0)];
 #pragma unroll
         for (int r = 0; r < RY; r++) {
             if (MODE == 0) {
                 const uint32_t o = so[r + 1] == OUTSIDE ? OUTSIDE : so[r + 1] / (uint32_t)sizeof(T);
-                id[r] = __builtin_amdgcn_raw_buffer_load_b8(rid, (int)o, (int)((uint32_t)k * (uint32_t)A.plane), 0);
+                P.id[r] = __builtin_amdgcn_raw_buffer_load_b8(ri, (int)o, (int)(kp >= 0 ? (uint32_t)kp * (uint32_t)A.plane : 0u), 0);
             } else {
-                id[r] = 0;
+                P.id[r] = 0;
             }
         }
-        const T fzh = coef[fhz + k], fze = coef[fez + k];
+    };
+
+    // The lean variant is software pipelined by a plane: the loads of E two planes up and of
+    // old H one plane up are issued before the arithmetic of this plane, whose own were issued
+    // a plane before. The slab variants, whose psi values take the registers that would need,
+    // issue E one plane up and old H of this plane at the top of the plane.
+    constexpr bool DEEP = AX == 0;
+    T hp[RY][2];  // new Hx, Hy of the plane below, on the strip's rows
+#pragma unroll
+    for (int r = 0; r < RY; r++) hp[r][0] = hp[r][1] = T(0);
+
+    // the arithmetic of plane kk from E of it and the plane above and old H of it
+    // the arithmetic of plane kk; `ahead` issues loads for a later plane once this plane's psi
+    // loads are issued, so that waiting for these does not wait for those
+    auto step_plane = [&](const int kk, const EPlane &cur, const EPlane &nxt, const HPlane &hcur,
+                          auto &&ahead) {
+        const bool pro = kk < kz0;
+        const int k = zwrap(kk);
+        const uint32_t sk = (uint32_t)k * pbytes;
+        const T (&ho)[RY + 1][3] = hcur.ho;
+        const T ghy = hcur.ghy, ghz = hcur.ghz;
+        const uint8_t (&id)[RY] = hcur.id;
+        const T fzh = hcur.fzh, fze = hcur.fze;
 
         // this plane's psi values, loaded with the fields: per [row][component] for H (rows
         // j0 - 1 .. j0 + RY - 1) and E (the strip's rows), per axis; 0 where none
@@ -353,6 +385,8 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
             }
         }
 
+        ahead();
+
         // new H on rows j0 - 1 .. j0 + RY - 1 (the first only Hx, Hz)
         T hn[RY + 1][3];
 #pragma unroll
@@ -360,7 +394,7 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
             const T ey_n = next_lane(cur.e[r][1], lane_value(cur.gy, 32 + r));
             const T ez_n = next_lane(cur.e[r][2], lane_value(cur.gz, 32 + r));
             const bool ok = xi >= 0 && yj[r] >= 0;
-            const T fyh = coef[fhy + max(yj[r], 0)];
+            const T fyh = fyh_r[r];
             const T dyx = cur.e[r + 1][2] - cur.e[r][2], dzx = nxt.e[r][1] - cur.e[r][1];  // Hx
             const T dzy = nxt.e[r][0] - cur.e[r][0], dxy = ez_n - cur.e[r][2];            // Hy
             const T dxz = ey_n - cur.e[r][1], dyz = cur.e[r + 1][0] - cur.e[r][0];        // Hz
@@ -483,7 +517,7 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                     g1 = own && A.grp ? A.grp[nv + o] : -1;
                     g2 = own && A.grp ? A.grp[2 * nv + o] : -1;
                 }
-                const T fye = coef[fey + max(j, 0)];
+                const T fye = fye_r[r];
                 T v0 = ca0 * cur.e[r][0] + cb0 * (dyx * fye - dzx * fze);
                 T w1 = ca1 * cur.e[r][1] + cb1 * (dzy2 * fze - dxy2 * fxe);
                 T w2 = ca2 * cur.e[r][2] + cb2 * (dxz2 * fxe - dyz2 * fye);
@@ -540,7 +574,37 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
             hp[r][0] = hn[r + 1][0];
             hp[r][1] = hn[r + 1][1];
         }
-        cur = nxt;
+    };
+
+    if (DEEP) {
+        EPlane cur, nxt;
+        HPlane hcur;
+        load_e(cur, kfirst);
+        load_e(nxt, kfirst + 1);
+        load_h(hcur, kfirst);
+        for (int kk = kfirst; kk < kz1; kk++) {
+            EPlane nxt2;
+            HPlane hnxt;
+            load_e(nxt2, kk + 2);
+            load_h(hnxt, kk + 1);
+            step_plane(kk, cur, nxt, hcur, [] {});
+            cur = nxt;
+            nxt = nxt2;
+            hcur = hnxt;
+        }
+    } else {
+        // old H a plane ahead, issued after the plane's psi loads
+        EPlane cur, nxt;
+        HPlane hcur;
+        load_e(cur, kfirst);
+        load_h(hcur, kfirst);
+        for (int kk = kfirst; kk < kz1; kk++) {
+            load_e(nxt, kk + 1);
+            HPlane hnxt;
+            step_plane(kk, cur, nxt, hcur, [&] { load_h(hnxt, kk + 1); });
+            cur = nxt;
+            hcur = hnxt;
+        }
     }
 }
 
