@@ -81,6 +81,58 @@ def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12, plain=False):
     return s, g
 
 
+# Si: eps_inf and two Lorentz terms; Au: a Drude term and two Lorentz terms, each as
+# ("lorentz", strength, resonance, damping) or ("drude", plasma, damping), frequencies in c/um
+SI_MEDIUM = (7.0, [("lorentz", 3.2, 3.9, 0.0), ("lorentz", 1.8, 2.2, 0.01)])
+AU_MEDIUM = (1.5, [("drude", 7.25, 0.053), ("lorentz", 1.2, 2.6, 0.5), ("lorentz", 2.4, 3.4, 1.1)])
+
+
+def dispersive_problem(nx=512, ny=512, nz=256, h=0.015, pml=16, seed=1):
+    """The dispersive 3D workload: 512 x 512 x 256 cells of 15 nm in SiO2 (n = 1.444), 40 Si
+    pillars (2-pole Lorentz medium, full height, radius U[80, 200] nm) and 10 Au blocks (Drude +
+    2 Lorentz, sides U[100, 300] nm) at seeded positions, a 16-cell CPML on every face, Courant
+    0.9, a broadband pulse (1.2-2.0 um) as an E_y current sheet across the grid near its low x
+    face, and a 5-frequency transform plane near the high x face."""
+    import photonoxide_amd as P
+    x0, y0, z0 = -nx * h / 2, -ny * h / 2, -nz * h / 2
+    g = P.Grid3d(nx, ny, nz, h, h, h, x0, y0, z0)
+    s = P.Simulation.new(g, P.Scene(1.444**2), P.Boundaries.cpml_all(pml), 0.9)
+    rng = np.random.default_rng(seed)
+    lo_x, hi_x = x0 + (pml + 20) * h, -x0 - (pml + 20) * h
+    lo_y, hi_y = y0 + (pml + 4) * h, -y0 - (pml + 4) * h
+    pillars = []
+    for _ in range(40):
+        r = rng.uniform(0.08, 0.2)
+        c = (rng.uniform(lo_x + r, hi_x - r), rng.uniform(lo_y + r, hi_y - r))
+        pillars.append(P.Cylinder(c, r, (-np.inf, np.inf), 1.0))
+    blocks = []
+    for _ in range(10):
+        w = rng.uniform(0.1, 0.3) / 2
+        c = (rng.uniform(lo_x + w, hi_x - w), rng.uniform(lo_y + w, hi_y - w),
+             rng.uniform(z0 + (pml + 4) * h + w, -z0 - (pml + 4) * h - w))
+        blocks.append(P.Box(tuple(q - w for q in c), tuple(q + w for q in c), 1.0))
+    s.with_medium(SI_MEDIUM[0], SI_MEDIUM[1], P.Scene(0.0, pillars))
+    s.with_medium(AU_MEDIUM[0], AU_MEDIUM[1], P.Scene(0.0, blocks))
+    wave = P.Waveform.pulse((1 / 1.2 + 1 / 2.0) / 2, 1 / 1.2 - 1 / 2.0)
+    i_src = pml + 8
+    jj, kk = np.meshgrid(np.arange(pml, ny - pml), np.arange(pml, nz - pml))
+    idx = ((kk * ny + jj) * nx + i_src).ravel()
+    s.add_current_indices(P.Field.E, np.full(idx.size, 1), idx, np.ones(idx.size, complex), wave)
+    i_mon = nx - pml - 8
+    s.add_dft((i_mon, pml, pml), (i_mon, ny - pml - 1, nz - pml - 1), 1 / np.linspace(1.2, 2.0, 5))
+    return s, g
+
+
+def ade_bytes_per_step(sim):
+    """The media state each step reads and writes, f32: E at the previous step (4 B read, 4 B
+    written) and each term's two currents (8 B read, 8 B written) per value in a medium."""
+    import ctypes
+    from photonoxide_amd import _native as N
+    b = ctypes.c_int64()
+    N.check(N.lib.pxfdtd_simulation_media_bytes_per_step(sim._h, 0, ctypes.byref(b)))
+    return int(b.value)
+
+
 def algorithmic_bytes_per_step(nx, ny, nz, pml, coef_bytes=1):
     n = nx * ny * nz
     psi = 0
@@ -274,6 +326,9 @@ def main():
                     help="the batched-serving workload instead: small is 64 2D + 4 3D jobs, full "
                          "256 2D + 16 3D")
     ap.add_argument("--rate", type=float, default=200.0, help="serving arrivals per second")
+    ap.add_argument("--workload", choices=["mmi", "dispersive"], default="mmi",
+                    help="mmi: the 1x2 MMI, 384 x 256 x 192 (the default); dispersive: 512 x 512 x "
+                         "256 with Lorentz/Drude media")
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint-gradient workload instead (with its finite-difference check)")
     ap.add_argument("--direct", action="store_true",
@@ -299,8 +354,14 @@ def main():
     import photonoxide_amd as P
     from photonoxide_amd import _native as N
 
-    nx, ny, nz, pml = 384, 256, 192, args.pml
-    s, g = mmi_problem(nx, ny, nz, 0.02, pml, plain=args.plain)
+    if args.workload == "dispersive":
+        nx, ny, nz, pml = 512, 512, 256, 16
+        t_setup = time.perf_counter()
+        s, g = dispersive_problem(nx, ny, nz, 0.015, pml)
+        t_setup = time.perf_counter() - t_setup
+    else:
+        nx, ny, nz, pml = 384, 256, 192, args.pml
+        s, g = mmi_problem(nx, ny, nz, 0.02, pml, plain=args.plain)
     gpu = P.Gpu(0)
     s.set_device(gpu, P.Precision.F32)
     if args.kernel >= 0:
@@ -322,10 +383,12 @@ def main():
     value = cells * args.steps / elapsed / 1e9
     mode = N.lib.pxfdtd_resident_coefficient_mode(res)
     bytes_step, psi = algorithmic_bytes_per_step(nx, ny, nz, pml, 1 if mode == 0 else 24)
+    media_bytes = ade_bytes_per_step(s) if args.workload == "dispersive" else 0
+    bytes_step += media_bytes
     achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_march64_step_mmi_f32.json")
-    if os.path.exists(tf):
+    if os.path.exists(tf) and args.workload == "mmi":
         # the counted bytes hold for the workload and tiling they were counted on
         t = json.load(open(tf))
         if (not args.plain and pml == 12 and s.kernel() == t.get("kernel_kind")
@@ -344,9 +407,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else ""),
+        "config": {"workload": (f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else "")
+                                if args.workload == "mmi" else
+                                "dispersive_512x512x256_dx15nm_40si_lorentz2_10au_drude_lorentz2_cpml16_f32"),
                    "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch", "aligned_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
-                   "monitors": "2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums",
+                   "monitors": ("2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums"
+                                if args.workload == "mmi" else
+                                "a plane of 6 components x 480x224 values x 5 frequencies, FP64 sums"),
                    "psi_values": psi},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
@@ -356,8 +423,13 @@ def main():
                      # the reference's own byte model (two-pass, f32 ca/cb arrays), for comparison
                      "reference_bytes_per_step": s.bytes_per_step(P.Precision.F32)},
     }
+    if media_bytes:
+        out["roofline"]["media_bytes_per_step"] = media_bytes
+        out["config"]["setup_s"] = round(t_setup, 2)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nx, ny, nz, pml)
+        if media_bytes:
+            out["cpu_baseline"]["sample"] += " (Yee + CPML only: the restated CPU kernel has no media)"
     print(json.dumps(out), flush=True)
 
 

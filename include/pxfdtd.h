@@ -358,6 +358,10 @@ int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precision, int64_t
 /* Yee::bytes_per_step (src/fdtd/kernel/yee.rs:141-153), the reference's byte model of one step:
  * (24 N + 6 per E psi value + 5 per H psi value) values of the precision's size */
 int pxfdtd_simulation_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes);
+/* The dispersive media's share of a step's algorithmic bytes (no counterpart in the reference,
+ * whose byte model leaves media out): per value in a medium, E at the previous step read and
+ * written and each term's two currents read and written, values of the precision's size */
+int pxfdtd_simulation_media_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes);
 
 /* ------------------------------------------------------------ level 3: the serving loop */
 /* Continuous batching of jobs on one device (no counterpart in the reference; its nearest is

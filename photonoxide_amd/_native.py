@@ -201,6 +201,7 @@ SIGNATURES = {
     "pxfdtd_simulation_resident": (vp, [vp]),
     "pxfdtd_simulation_footprint": (C.c_int, [vp, C.c_int32, pi64]),
     "pxfdtd_simulation_bytes_per_step": (C.c_int, [vp, C.c_int32, pi64]),
+    "pxfdtd_simulation_media_bytes_per_step": (C.c_int, [vp, C.c_int32, pi64]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

@@ -84,7 +84,8 @@ struct StepArgs {
     int nx, ny, nz, pitch;
     int plane;                // pitch * ny
     int nval;                 // values per component: a field's components are nval apart
-    int flags;                // bit a: axis a periodic; bit 3: CPML slabs; bit 4: injections
+    int flags;                // bit a: axis a periodic; bit 3: CPML slabs; bit 4: injections;
+                              // bit 5: media in media_pass after the step kernel
     int tiles_x, tiles_y, kz;
     int blocks;               // tiles_x * tiles_y * chunks
     int per_xcd;              // blocks dealt to each XCD, contiguous in the tile order
@@ -253,6 +254,24 @@ __device__ __forceinline__ T media_update(const StepArgs<T> &A, const StepCold<T
     return next;
 }
 
+// The dispersive values' update as a pass of its own after a step kernel that left it out
+// (flags bit 5): the step kernel stored each value's curl update with its sources, E^n is
+// still in the input copy, and this adds the media term and advances the state with
+// media_update, the arithmetic the step kernels apply inline (src/fdtd/media.rs:670-705). One
+// thread per value in a medium: `vals` holds c * nval + r, `vgrp` its group, `vcb` its cb.
+template <typename T>
+__global__ __launch_bounds__(256) void media_pass(StepArgs<T> A, const uint32_t *vals,
+                                                  const int8_t *vgrp, const T *vcb, long long count) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= count) return;
+    const StepCold<T> &K = *A.cold;
+    const uint32_t s = vals[e];
+    const uint32_t nv = (uint32_t)A.nval;
+    const int c = s >= 2 * nv ? 2 : (s >= nv ? 1 : 0);
+    const int r = (int)(s - (uint32_t)c * nv);
+    A.eout[s] = media_update(A, K, vgrp[e], c, r, A.eout[s], vcb[e], A.ein[s]);
+}
+
 // What a thread loads one or two planes ahead: E of a plane (its own value and a halo
 // value), old H of another (own, and the halo row's Hx, Hz and column's Hy, Hz) and the
 // material id.
@@ -284,7 +303,7 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
     if (MODE == 0) {
         for (int q = tid; q < 256 * 6; q += NT) tab[q] = A.table[q];
         const int8_t *g = reinterpret_cast<const int8_t *>(A.table + 256 * 6);
-        for (int q = tid; q < 256 * 3; q += NT) grp[q] = g[q];
+        for (int q = tid; q < 256 * 3; q += NT) grp[q] = (A.flags & 32) ? int8_t(-1) : g[q];
     }
     {
         const int *src = reinterpret_cast<const int *>(A.cold);
@@ -673,7 +692,7 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
     if (MODE == 0) {
         for (int q = tid; q < 256 * 6; q += 256) tab[q] = A.table[q];
         const int8_t *g = reinterpret_cast<const int8_t *>(A.table + 256 * 6);
-        for (int q = tid; q < 256 * 3; q += 256) grp[q] = g[q];
+        for (int q = tid; q < 256 * 3; q += 256) grp[q] = (A.flags & 32) ? int8_t(-1) : g[q];
     }
     {
         const int *src = reinterpret_cast<const int *>(A.cold);

@@ -106,7 +106,7 @@ __device__ __forceinline__ void march64_stage(const StepArgs<T> &A, MarchShared<
     if (MODE == 0) {
         for (int q = tid; q < 256 * 6; q += 256) tab[q] = A.table[q];
         const int8_t *g = reinterpret_cast<const int8_t *>(A.table + 256 * 6);
-        for (int q = tid; q < 256 * 3; q += 256) grp[q] = g[q];
+        for (int q = tid; q < 256 * 3; q += 256) grp[q] = (A.flags & 32) ? int8_t(-1) : g[q];
     }
     {
         const int *src = reinterpret_cast<const int *>(A.cold);

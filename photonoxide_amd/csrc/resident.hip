@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -153,6 +154,10 @@ struct pxfdtd_resident {
     // media
     Buf groups, m_before, m_cur, m_prev;
     int ngroups = 0, max_poles = 0;
+    // the values in a medium, updated by media_pass after the step kernel: c * nval + r, the
+    // group, cb
+    Buf mvals, mgrp, mcb;
+    long long nmvals = 0;
     // injections
     std::vector<pxfdtd_injection> inj_host;
     // plane waves: their auxiliary grids (FP64, E then H~ per wave), coefficients, and the
@@ -507,6 +512,29 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
         HIPCHK(hipMemsetAsync(r->m_before.p, 0, r->m_before.bytes, s));
         HIPCHK(hipMemsetAsync(r->m_cur.p, 0, r->m_cur.bytes, s));
         HIPCHK(hipMemsetAsync(r->m_prev.p, 0, r->m_prev.bytes, s));
+        if (3 * nval < (1LL << 32)) {
+            std::vector<uint32_t> vals;
+            std::vector<int8_t> vg;
+            std::vector<T> vcb;
+            for (int c = 0; c < 3; c++)
+                for (long long q = 0; q < N; q++) {
+                    if (!member[c * N + q]) continue;
+                    const long long i = q % p->nx, rest = q / p->nx;
+                    vals.push_back((uint32_t)(c * nval + (rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i));
+                    vg.push_back((int8_t)(member[c * N + q] - 1));
+                    vcb.push_back((T)p->cb[c * N + q]);
+                }
+            r->nmvals = (long long)vals.size();
+            if (r->nmvals) {
+                TRY(r->mvals.alloc(vals.size() * sizeof(uint32_t)));
+                TRY(r->mgrp.alloc(vg.size()));
+                TRY(r->mcb.alloc(vcb.size() * sizeof(T)));
+                HIPCHK(hipMemcpyAsync(r->mvals.p, vals.data(), vals.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(r->mgrp.p, vg.data(), vg.size(), hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(r->mcb.p, vcb.data(), vcb.size() * sizeof(T), hipMemcpyHostToDevice, s));
+                HIPCHK(hipStreamSynchronize(s));
+            }
+        }
     }
     HIPCHK(hipStreamSynchronize(s));
     return PXFDTD_OK;
@@ -918,10 +946,23 @@ static int build_units(pxfdtd_resident *r) {
     auto cut = [&](int z) {
         if (z > cuts.back() && z < nz) cuts.push_back(z);
     };
-    if (lz > 0) cut(lz + 1);
+    // the z slabs' chunks (their units run the slower CPML variant) at most kzs planes long,
+    // split evenly
+    const char *ev = getenv("PXFDTD_KZ_SLAB");
+    const int kzs = ev ? atoi(ev) : 0;
+    auto split = [&](int a, int b) {
+        if (kzs <= 0 || b - a <= kzs) return;
+        const int parts = (b - a + kzs - 1) / kzs;
+        for (int q = 1; q < parts; q++) cut(a + (b - a) * q / parts);
+    };
+    if (lz > 0) {
+        split(0, lz + 1);
+        cut(lz + 1);
+    }
     const int hi_start = hz > 0 ? nz - hz : nz;
     for (int z = cuts.back() + r->kz; z < hi_start; z += r->kz) cut(z);
     cut(hi_start);
+    if (hz > 0) split(hi_start, nz);
     cuts.push_back(nz);
     int tw, th;
     tile_shape(r, tw, th);
@@ -1574,6 +1615,10 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         A.hout = r->field<T>(q, 3);
         A.cold = r->cold.as<StepCold<T>>() + p;
         A.wave_step = (int)n;
+        if (r->nmvals) {  // the media after the step kernel, in a pass of their own
+            A.flags |= 32;
+            A.grp = nullptr;
+        }
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
         const SideStream side = {r->side, r->fork_ev, r->join_ev};
         const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
@@ -1590,6 +1635,11 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         else
             TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         HIPCHK(hipGetLastError());
+        if (r->nmvals) {
+            hipLaunchKernelGGL((media_pass<T>), dim3((unsigned)((r->nmvals + 255) / 256)), dim3(256), 0, s, A,
+                               r->mvals.as<uint32_t>(), r->mgrp.as<int8_t>(), r->mcb.as<T>(), r->nmvals);
+            HIPCHK(hipGetLastError());
+        }
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
         if (!r->planes_h.empty()) {
             hipLaunchKernelGGL(aux_step_e, aux_grid, dim3(256), 0, s, r->plane_desc.as<PlaneDesc>(),

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <omp.h>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -386,6 +387,13 @@ extern "C" int pxfdtd_simulation_new(const pxfdtd_grid *g, const pxfdtd_boundari
     if (!eps) return px_fail(PXFDTD_ERR_INVALID, "a scene is required");
     const double hs[3] = {g->dx, g->dy, g->dz};
     const pxfdtd_grid gg = *g;
+    if (eps->n == 0) {
+        // a scene of its background alone: every value's average is the same arithmetic on the
+        // same samples, so it is computed once per component
+        const double o[3] = {gg.x0, gg.y0, gg.z0};
+        const double u[3] = {averaged(eps, 0, o, hs), averaged(eps, 1, o, hs), averaged(eps, 2, o, hs)};
+        return make(g, b, courant, [&](int c, long long) { return u[c]; }, out);
+    }
     return make(g, b, courant, [&](int c, long long r) {
         const int i = (int)(r % gg.nx), j = (int)((r / gg.nx) % gg.ny), k = (int)(r / ((long long)gg.nx * gg.ny));
         double p[3] = {gg.x0 + i * gg.dx, gg.y0 + j * gg.dy, gg.z0 + k * gg.dz};
@@ -1026,24 +1034,65 @@ extern "C" int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf
     G.chi = gsum / (4.0 * dt);
     const long long N = s->N;
     std::vector<double> sig;
-    for (int c = 0; c < 3; c++)
-        for (long long r = 0; r < N; r++) {
-            if (s->cb[c * N + r] == 0.0) continue;
-            double p[3];
-            s->e_position(c, r, p);
-            if (scene_value(inside_, 0, p[0], p[1], p[2]) == 0.0) continue;
-            double old = 0.0;
-            if (!s->member.empty() && s->member[c * N + r] >= 0) old = s->groups[s->member[c * N + r]].chi;
-            const double sigma = (1.0 - s->ca[c * N + r]) / s->cb[c * N + r] - old;
-            G.comp.push_back(c);
-            G.r.push_back(r);
-            sig.push_back(sigma > 0.0 ? sigma : 0.0);
+    // the values inside, in component-then-index order: each thread scans a contiguous range
+    // and the ranges are joined in order
+    for (int c = 0; c < 3; c++) {
+        std::vector<std::vector<long long>> found;
+        std::vector<std::vector<double>> fsig;
+#pragma omp parallel
+        {
+#pragma omp single
+            {
+                found.resize((size_t)omp_get_num_threads());
+                fsig.resize(found.size());
+            }
+            const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+            const long long a = N * t / nt, z = N * (t + 1) / nt;
+            for (long long r = a; r < z; r++) {
+                if (s->cb[c * N + r] == 0.0) continue;
+                double p[3];
+                s->e_position(c, r, p);
+                if (scene_value(inside_, 0, p[0], p[1], p[2]) == 0.0) continue;
+                double old = 0.0;
+                if (!s->member.empty() && s->member[c * N + r] >= 0) old = s->groups[s->member[c * N + r]].chi;
+                const double sigma = (1.0 - s->ca[c * N + r]) / s->cb[c * N + r] - old;
+                found[t].push_back(r);
+                fsig[t].push_back(sigma > 0.0 ? sigma : 0.0);
+            }
         }
+        for (size_t t = 0; t < found.size(); t++) {
+            G.r.insert(G.r.end(), found[t].begin(), found[t].end());
+            G.comp.insert(G.comp.end(), found[t].size(), c);
+            sig.insert(sig.end(), fsig[t].begin(), fsig[t].end());
+        }
+    }
     const double e = eps_inf;
+    // the values taken from earlier media leave those, each group compacted once
+    std::vector<char> dirty(s->groups.size(), 0);
+    if (!s->member.empty())
+        for (size_t v = 0; v < G.r.size(); v++) {
+            int &m = s->member[G.comp[v] * N + G.r[v]];
+            if (m >= 0) {
+                dirty[(size_t)m] = 1;
+                m = -1;
+            }
+        }
+    for (size_t g = 0; g < dirty.size(); g++) {
+        if (!dirty[g]) continue;
+        Group &O = s->groups[g];
+        size_t w = 0;
+        for (size_t v = 0; v < O.r.size(); v++) {
+            if (s->member[O.comp[v] * N + O.r[v]] != (int)g) continue;
+            O.comp[w] = O.comp[v];
+            O.r[w] = O.r[v];
+            w++;
+        }
+        O.comp.resize(w);
+        O.r.resize(w);
+    }
     for (size_t v = 0; v < G.r.size(); v++) {
         const int c = G.comp[v];
         const long long r = G.r[v];
-        media_remove(s, c, r);
         const double nu = 1.0 + dt / e * (sig[v] / 2.0 + G.chi);
         s->ca[c * N + r] = (1.0 - dt * sig[v] / (2.0 * e)) / nu;
         s->cb[c * N + r] = dt / e / nu;
@@ -1703,6 +1752,15 @@ extern "C" int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precisi
 
 // Yee::bytes_per_step (src/fdtd/kernel/yee.rs:141-153): the reference's byte model of a step,
 // (24 N + 6 psi_E + 5 psi_H) values of the precision's size
+extern "C" int pxfdtd_simulation_media_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes) {
+    if (precision != PXFDTD_F32 && precision != PXFDTD_F64)
+        return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");
+    long long v = 0;
+    for (const Group &G : s->groups) v += (long long)G.r.size() * (2 + 4 * G.poles);
+    *bytes = (int64_t)(v * (precision == PXFDTD_F32 ? 4 : 8));
+    return PXFDTD_OK;
+}
+
 extern "C" int pxfdtd_simulation_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes) {
     if (precision != PXFDTD_F32 && precision != PXFDTD_F64)
         return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");

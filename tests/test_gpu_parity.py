@@ -83,14 +83,41 @@ def test_device_is_the_oracle_bit_for_bit(gpu, shape, prec, kernel):
             assert same_bits(o.dft(0, f, c).view(np.float64), p.dft(0, f, c).view(np.float64)), (f, c)
 
 
+@pytest.mark.parametrize("kernel", [3, 1, 0])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_dispersive_media_bit_for_bit(gpu, prec):
+def test_dispersive_media_bit_for_bit(gpu, prec, kernel):
+    """The media's update runs as a pass of its own after every step kernel."""
     import photonoxide_amd as P
     precision = P.Precision.F64 if prec == "f64" else P.Precision.F32
-    print(f"case dispersive {prec}", flush=True)
+    print(f"case dispersive {prec} kernel {kernel}", flush=True)
     o, p = build_pair(dispersive_spec(), gpu, precision)
+    p.set_kernel(kernel)
     o.run(120)
-    p.run(120)
+    p.run(57)
+    p.run(63)
+    fo, fp = o.fields(), product_fields(p)
+    assert same_bits(fo, fp), np.abs(fo - fp).max()
+    for n in range(len(dispersive_spec()["probes"])):
+        assert same_bits(o.probe(n), p.probe(n)), n
+
+
+@pytest.mark.parametrize("kernel", [4, 3])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dispersive_media_2d_bit_for_bit(gpu, prec, kernel):
+    import photonoxide_amd as P
+    precision = P.Precision.F64 if prec == "f64" else P.Precision.F32
+    spec = dict(busy_spec(True))
+    spec["media"] = [
+        (2.0, [("lorentz", 1.2, 1.6, 0.05), ("lorentz", 0.4, 2.5, 0.0)],
+         (0.0, [(0, (-0.8, -0.3, -INF), (-0.1, 0.3, INF), 1.0)])),
+        (1.0, [("drude", 2.0, 0.1), ("lorentz", 0.5, 1.9, 0.1)],
+         (0.0, [(1, (0.5, 0.1, -INF), (0.35, 0, INF), 1.0)])),
+    ]
+    print(f"case dispersive 2d {prec} kernel {kernel}", flush=True)
+    o, p = build_pair(spec, gpu, precision)
+    p.set_kernel(kernel)
+    o.run(150)
+    p.run(150)
     fo, fp = o.fields(), product_fields(p)
     assert same_bits(fo, fp), np.abs(fo - fp).max()
 
