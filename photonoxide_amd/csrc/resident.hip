@@ -10,7 +10,6 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -946,23 +945,10 @@ static int build_units(pxfdtd_resident *r) {
     auto cut = [&](int z) {
         if (z > cuts.back() && z < nz) cuts.push_back(z);
     };
-    // the z slabs' chunks (their units run the slower CPML variant) at most kzs planes long,
-    // split evenly
-    const char *ev = getenv("PXFDTD_KZ_SLAB");
-    const int kzs = ev ? atoi(ev) : 0;
-    auto split = [&](int a, int b) {
-        if (kzs <= 0 || b - a <= kzs) return;
-        const int parts = (b - a + kzs - 1) / kzs;
-        for (int q = 1; q < parts; q++) cut(a + (b - a) * q / parts);
-    };
-    if (lz > 0) {
-        split(0, lz + 1);
-        cut(lz + 1);
-    }
+    if (lz > 0) cut(lz + 1);
     const int hi_start = hz > 0 ? nz - hz : nz;
     for (int z = cuts.back() + r->kz; z < hi_start; z += r->kz) cut(z);
     cut(hi_start);
-    if (hz > 0) split(hi_start, nz);
     cuts.push_back(nz);
     int tw, th;
     tile_shape(r, tw, th);
