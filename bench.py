@@ -382,7 +382,7 @@ def main():
     cells = nx * ny * nz
     value = cells * args.steps / elapsed / 1e9
     mode = N.lib.pxfdtd_resident_coefficient_mode(res)
-    bytes_step, psi = algorithmic_bytes_per_step(nx, ny, nz, pml, 1 if mode == 0 else 24)
+    bytes_step, psi = algorithmic_bytes_per_step(nx, ny, nz, pml, {0: 1, 1: 24, 2: 12}[mode])
     media_bytes = ade_bytes_per_step(s) if args.workload == "dispersive" else 0
     bytes_step += media_bytes
     achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
@@ -410,7 +410,7 @@ def main():
         "config": {"workload": (f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else "")
                                 if args.workload == "mmi" else
                                 "dispersive_512x512x256_dx15nm_40si_lorentz2_10au_drude_lorentz2_cpml16_f32"),
-                   "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch", "aligned_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": "uint8 ids" if mode == 0 else "arrays",
+                   "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch", "aligned_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": {0: "uint8 ids", 1: "ca, cb arrays", 2: "cb arrays"}[mode],
                    "monitors": ("2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums"
                                 if args.workload == "mmi" else
                                 "a plane of 6 components x 480x224 values x 5 frequencies, FP64 sums"),

@@ -211,7 +211,8 @@ int pxfdtd_resident_set_concurrent(pxfdtd_resident *r, int32_t mode);
  * planes per chunk (kernel 4: rows per chunk); 0 keeps the current value */
 int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes);
 int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes);
-/* the coefficient storage chosen: 0 uint8 ids into a table, 1 full ca/cb arrays */
+/* the coefficient storage chosen: 0 uint8 ids into a table, 1 full ca/cb arrays, 2 cb arrays
+ * alone (a lossless problem without media, each ca 0 where its cb is 0 and 1 elsewhere) */
 int32_t pxfdtd_resident_coefficient_mode(const pxfdtd_resident *r);
 /* the average time per step of the last pxfdtd_resident_step call (all its kernels), ms, from
  * HIP events on the resident's stream */

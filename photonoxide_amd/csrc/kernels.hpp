@@ -95,7 +95,9 @@ struct StepArgs {
     // [256][3][2] (ca, cb per component) then the media groups [256][3] as int8
     const uint8_t *ids;
     const T *table;
-    // mode 1: ca then cb, [2][3][nval], and the media group per value [3][nval] (or null)
+    // mode 1: ca then cb, [2][3][nval], and the media group per value [3][nval] (or null);
+    // mode 2 (a lossless problem without media): cb alone, [3][nval], ca being 0 where cb is 0
+    // (walls, conductors) and 1 elsewhere
     const T *cacb;
     const int8_t *grp;
     T dt;
@@ -253,6 +255,11 @@ __device__ __forceinline__ T media_update(const StepArgs<T> &A, const StepCold<T
     K.m_before[s] = eold;
     return next;
 }
+
+// ca of a lossless problem's value from its cb (coefficient mode 2): 0 where cb is 0 (walls and
+// conductors zero both, src/fdtd.rs:738-756, 995-1010), 1 elsewhere
+template <typename T>
+__device__ __forceinline__ T lossless_ca(T cb) { return cb == T(0) ? T(0) : T(1); }
 
 // The dispersive values' update as a pass of its own after a step kernel that left it out
 // (flags bit 5): the step kernel stored each value's curl update with its sources, E^n is
@@ -550,6 +557,10 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
                 const T *e = tab + 6 * idc;
                 ca0 = e[0]; cb0 = e[1]; ca1 = e[2]; cb1 = e[3]; ca2 = e[4]; cb2 = e[5];
                 g0 = grp[3 * idc]; g1 = grp[3 * idc + 1]; g2 = grp[3 * idc + 2];
+            } else if (MODE == 2) {
+                cb0 = A.cacb[r]; cb1 = A.cacb[nv + r]; cb2 = A.cacb[2 * nv + r];
+                ca0 = lossless_ca(cb0); ca1 = lossless_ca(cb1); ca2 = lossless_ca(cb2);
+                g0 = g1 = g2 = -1;
             } else {
                 ca0 = A.cacb[r]; ca1 = A.cacb[nv + r]; ca2 = A.cacb[2 * nv + r];
                 cb0 = A.cacb[3 * nv + r]; cb1 = A.cacb[4 * nv + r]; cb2 = A.cacb[5 * nv + r];
@@ -891,6 +902,10 @@ __global__ __launch_bounds__(256) void march_step(StepArgs<T> A) {
                     const T *e = tab + 6 * q;
                     ca0 = e[0]; cb0 = e[1]; ca1 = e[2]; cb1 = e[3]; ca2 = e[4]; cb2 = e[5];
                     g0 = grp[3 * q]; g1 = grp[3 * q + 1]; g2 = grp[3 * q + 2];
+                } else if (MODE == 2) {
+                    cb0 = A.cacb[o]; cb1 = A.cacb[nv + o]; cb2 = A.cacb[2 * nv + o];
+                    ca0 = lossless_ca(cb0); ca1 = lossless_ca(cb1); ca2 = lossless_ca(cb2);
+                    g0 = g1 = g2 = -1;
                 } else {
                     ca0 = A.cacb[o]; ca1 = A.cacb[nv + o]; ca2 = A.cacb[2 * nv + o];
                     cb0 = A.cacb[3 * nv + o]; cb1 = A.cacb[4 * nv + o]; cb2 = A.cacb[5 * nv + o];

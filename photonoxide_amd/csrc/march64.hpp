@@ -505,6 +505,13 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                     const T *e = tab + 6 * q;
                     ca0 = e[0]; cb0 = e[1]; ca1 = e[2]; cb1 = e[3]; ca2 = e[4]; cb2 = e[5];
                     g0 = grp[3 * q]; g1 = grp[3 * q + 1]; g2 = grp[3 * q + 2];
+                } else if (MODE == 2) {
+                    const long long nv = A.nval;
+                    cb0 = own ? A.cacb[o] : T(0);
+                    cb1 = own ? A.cacb[nv + o] : T(0);
+                    cb2 = own ? A.cacb[2 * nv + o] : T(0);
+                    ca0 = lossless_ca(cb0); ca1 = lossless_ca(cb1); ca2 = lossless_ca(cb2);
+                    g0 = g1 = g2 = -1;
                 } else {
                     const long long nv = A.nval;
                     ca0 = own ? A.cacb[o] : T(0);

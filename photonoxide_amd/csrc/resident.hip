@@ -402,8 +402,28 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
             idv[(rest / p->ny) * r->plane + (rest % p->ny) * r->pitch + i] = (uint8_t)id;
         }
     }
-    r->mode = table_ok ? 0 : 1;
-    if (r->mode == 0) {
+    // beyond 256 triples, a lossless problem without media stores cb alone: each ca is 0 where
+    // its cb is 0 and 1 elsewhere (the kernels derive it), bit for bit
+    bool lossless = !table_ok && member.empty();
+    if (lossless) {
+        const uint64_t one = 0x3ff0000000000000ull;
+        int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+        for (long long q = 0; q < 3 * N; q++) {
+            uint64_t a;
+            memcpy(&a, &p->ca[q], 8);
+            bad |= a != (p->cb[q] == 0.0 ? 0ull : one);
+        }
+        lossless = bad == 0;
+    }
+    r->mode = table_ok ? 0 : (lossless ? 2 : 1);
+    if (r->mode == 2) {
+        std::vector<T> cbv(3 * nval, T(0));
+        for (int c = 0; c < 3; c++) to_pitched<T>(r, p->cb + c * N, cbv.data() + c * nval);
+        TRY(r->cacb.alloc(cbv.size() * sizeof(T)));
+        HIPCHK(hipMemcpyAsync(r->cacb.p, cbv.data(), cbv.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    } else if (r->mode == 0) {
         TRY(r->ids.alloc(nval));
         HIPCHK(hipMemcpyAsync(r->ids.p, idv.data(), nval, hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -1618,6 +1638,8 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
                                r->aux.as<double>(), r->aux_coef.as<double>());
         if (r->mode == 0)
             TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
+        else if (r->mode == 2)
+            TRY((launch_mode<T, 2>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         else
             TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         HIPCHK(hipGetLastError());
@@ -1708,6 +1730,8 @@ static int trial_ms(pxfdtd_resident *r, int trials, double *ms) {
         if (n == 1) HIPCHK(hipEventRecord(r->ev0, s));
         if (r->mode == 0)
             TRY((launch_mode<T, 0>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
+        else if (r->mode == 2)
+            TRY((launch_mode<T, 2>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         else
             TRY((launch_mode<T, 1>(A, r->kernel, r->ty, s, lds, sp, uc, set)));
         HIPCHK(hipGetLastError());

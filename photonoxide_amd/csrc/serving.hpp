@@ -289,11 +289,13 @@ static void launch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, int s)
     if (c.kind == 3 && c.mode == M && c.v == R && c.ax == X) return launch_batch_class<M, R, X>(S, c, s, lds);
     PX_CLASS(0, 1, 0) PX_CLASS(0, 1, 1) PX_CLASS(0, 1, 7) PX_CLASS(0, 2, 0) PX_CLASS(0, 2, 1) PX_CLASS(0, 2, 7)
     PX_CLASS(1, 1, 0) PX_CLASS(1, 1, 1) PX_CLASS(1, 1, 7) PX_CLASS(1, 2, 0) PX_CLASS(1, 2, 1) PX_CLASS(1, 2, 7)
+    PX_CLASS(2, 1, 0) PX_CLASS(2, 1, 1) PX_CLASS(2, 1, 7) PX_CLASS(2, 2, 0) PX_CLASS(2, 2, 1) PX_CLASS(2, 2, 7)
 #undef PX_CLASS
 #define PX_CLASS2(M, X, E) \
     if (c.kind == 4 && c.mode == M && c.ax == X && c.v == E) return launch_batch_class_2d<M, X, E>(S, c, s, lds);
     PX_CLASS2(0, 0, 1) PX_CLASS2(0, 0, 2) PX_CLASS2(0, 0, 3) PX_CLASS2(0, 3, 1) PX_CLASS2(0, 3, 2) PX_CLASS2(0, 3, 3)
     PX_CLASS2(1, 0, 1) PX_CLASS2(1, 0, 2) PX_CLASS2(1, 0, 3) PX_CLASS2(1, 3, 1) PX_CLASS2(1, 3, 2) PX_CLASS2(1, 3, 3)
+    PX_CLASS2(2, 0, 1) PX_CLASS2(2, 0, 2) PX_CLASS2(2, 0, 3) PX_CLASS2(2, 3, 1) PX_CLASS2(2, 3, 2) PX_CLASS2(2, 3, 3)
 #undef PX_CLASS2
 }
 

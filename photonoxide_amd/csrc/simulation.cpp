@@ -1750,8 +1750,8 @@ extern "C" int pxfdtd_simulation_footprint(pxfdtd_simulation *s, int32_t precisi
     return PXFDTD_OK;
 }
 
-// Yee::bytes_per_step (src/fdtd/kernel/yee.rs:141-153): the reference's byte model of a step,
-// (24 N + 6 psi_E + 5 psi_H) values of the precision's size
+// the media's share of a step's algorithmic bytes: per value in a medium, E at the previous
+// step and each term's two currents, each read and written
 extern "C" int pxfdtd_simulation_media_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes) {
     if (precision != PXFDTD_F32 && precision != PXFDTD_F64)
         return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");
@@ -1761,6 +1761,8 @@ extern "C" int pxfdtd_simulation_media_bytes_per_step(const pxfdtd_simulation *s
     return PXFDTD_OK;
 }
 
+// Yee::bytes_per_step (src/fdtd/kernel/yee.rs:141-153): the reference's byte model of a step,
+// (24 N + 6 psi_E + 5 psi_H) values of the precision's size
 extern "C" int pxfdtd_simulation_bytes_per_step(const pxfdtd_simulation *s, int32_t precision, int64_t *bytes) {
     if (precision != PXFDTD_F32 && precision != PXFDTD_F64)
         return px_fail(PXFDTD_ERR_INVALID, "precision must be PXFDTD_F32 or PXFDTD_F64");

@@ -59,6 +59,19 @@ def busy_spec(two_d=False, periodic=False):
     return spec
 
 
+def smooth_eps_spec(two_d=False, lossy=False, seed=7):
+    """The busy problem with a smoothed random permittivity per E value (far more than 256
+    distinct coefficients), lossless or with the busy problem's conductivity."""
+    spec = dict(busy_spec(two_d))
+    nx, ny, nz = spec["grid"][:3]
+    rng = np.random.default_rng(seed)
+    eps = 1.0 + 4.0 * rng.random((3, nz, ny, nx))
+    spec["eps3n"] = eps
+    if not lossy:
+        spec["sigma"] = None
+    return spec
+
+
 def initial_fields(n):
     """0.01 noise in E and H, as the reference's check problem starts."""
     r = np.arange(n)
@@ -72,7 +85,10 @@ def index(grid, at):
 
 
 def oracle(spec, with_fields=True):
-    s = O.Sim(spec["grid"], O.bounds(**spec["bounds"]), O.scene(*spec["eps"]), spec["courant"])
+    if spec.get("eps3n") is not None:  # a permittivity per E value instead of a scene
+        s = O.Sim(spec["grid"], O.bounds(**spec["bounds"]), courant=spec["courant"], eps3n=spec["eps3n"])
+    else:
+        s = O.Sim(spec["grid"], O.bounds(**spec["bounds"]), O.scene(*spec["eps"]), spec["courant"])
     if spec.get("sigma"):
         s.conductivity(O.scene(*spec["sigma"]))
     n = s.n
@@ -114,7 +130,10 @@ def product(spec, with_fields=True):
         return P.Scene(bg, [P.Box(lo, hi, v) if k == 0 else P.Cylinder((lo[0], lo[1]), hi[0], (lo[2], hi[2]), v)
                             for k, lo, hi, v in shapes])
 
-    s = P.Simulation.new(g, to_scene(spec["eps"]), bd, spec["courant"])
+    if spec.get("eps3n") is not None:
+        s = P.Simulation.with_eps(g, spec["eps3n"], bd, spec["courant"])
+    else:
+        s = P.Simulation.new(g, to_scene(spec["eps"]), bd, spec["courant"])
     if spec.get("sigma"):
         s.with_conductivity(to_scene(spec["sigma"]))
     if with_fields:

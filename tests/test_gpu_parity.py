@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from fixtures import INF, busy_spec, oracle, product, product_fields
+from fixtures import INF, busy_spec, oracle, product, product_fields, smooth_eps_spec
 
 pytestmark = pytest.mark.gpu
 
@@ -98,6 +98,31 @@ def test_dispersive_media_bit_for_bit(gpu, prec, kernel):
     fo, fp = o.fields(), product_fields(p)
     assert same_bits(fo, fp), np.abs(fo - fp).max()
     for n in range(len(dispersive_spec()["probes"])):
+        assert same_bits(o.probe(n), p.probe(n)), n
+
+
+@pytest.mark.parametrize("kernel", [3, 4, 1, 0])
+@pytest.mark.parametrize("lossy", [False, True])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_coefficient_arrays_bit_for_bit(gpu, prec, lossy, kernel):
+    """Beyond 256 distinct coefficients: full ca, cb arrays (mode 1) for a lossy problem, cb
+    alone (mode 2) for a lossless one."""
+    import photonoxide_amd as P
+    from photonoxide_amd import _native as N
+    if kernel == 4:
+        spec = smooth_eps_spec(True, lossy)
+    else:
+        spec = smooth_eps_spec(False, lossy)
+    precision = P.Precision.F64 if prec == "f64" else P.Precision.F32
+    print(f"case coefficient arrays {prec} lossy {lossy} kernel {kernel}", flush=True)
+    o, p = build_pair(spec, gpu, precision)
+    p.set_kernel(kernel)
+    assert N.lib.pxfdtd_resident_coefficient_mode(p.resident()) == (1 if lossy else 2)
+    o.run(80)
+    p.run(80)
+    fo, fp = o.fields(), product_fields(p)
+    assert same_bits(fo, fp), np.abs(fo - fp).max()
+    for n in range(len(spec["probes"])):
         assert same_bits(o.probe(n), p.probe(n)), n
 
 
