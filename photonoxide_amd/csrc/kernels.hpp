@@ -678,6 +678,18 @@ template <>
 __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
+// a load of a value read once per step (CPML psi): non-temporal (cache policy NT), so it does
+// not displace the field rows the neighbouring strips read again
+template <typename T>
+__device__ __forceinline__ T bload_once(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ float bload_once<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 2));
+}
+template <>
+__device__ __forceinline__ double bload_once<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 2));
+}
 
 constexpr int RW = 62;
 

@@ -164,23 +164,23 @@ __device__ __forceinline__ void march2d_unit(const StepArgs<T> &A, const int4 un
         T hq[3] = {T(0), T(0), T(0)};
         if (SLABS) {
             if (xs >= 0 || xsl >= 0) {  // (Cy, x) and (Cz, x)
-                if (TM) qxh[0] = bload<T>(rPi, xo(1, 1, j), 0);
-                if (TE) qxh[1] = bload<T>(rPi, xo(1, 2, j), 0);
-                if (TE) qxe[0] = bload<T>(rPi, xo(0, 1, j), 0);
-                if (TM) qxe[1] = bload<T>(rPi, xo(0, 2, j), 0);
+                if (TM) qxh[0] = bload_once<T>(rPi, xo(1, 1, j), 0);
+                if (TE) qxh[1] = bload_once<T>(rPi, xo(1, 2, j), 0);
+                if (TE) qxe[0] = bload_once<T>(rPi, xo(0, 1, j), 0);
+                if (TM) qxe[1] = bload_once<T>(rPi, xo(0, 2, j), 0);
                 if (xsl >= 0 && !pro) {
                     const uint32_t q = lane == 0 ? (uint32_t)(j * xlcnt + xls) : OUTSIDE;
-                    if (TM) hq[0] = bload<T>(rPi, q == OUTSIDE ? OUTSIDE : (uint32_t)(uniform(K.psi_off[1][1][0][xsl]) + q) * sizeof(T), 0);
-                    if (TE) hq[1] = bload<T>(rPi, q == OUTSIDE ? OUTSIDE : (uint32_t)(uniform(K.psi_off[1][2][0][xsl]) + q) * sizeof(T), 0);
+                    if (TM) hq[0] = bload_once<T>(rPi, q == OUTSIDE ? OUTSIDE : (uint32_t)(uniform(K.psi_off[1][1][0][xsl]) + q) * sizeof(T), 0);
+                    if (TE) hq[1] = bload_once<T>(rPi, q == OUTSIDE ? OUTSIDE : (uint32_t)(uniform(K.psi_off[1][2][0][xsl]) + q) * sizeof(T), 0);
                 }
             }
             if (ys >= 0) {  // (Cx, y) and (Cz, y)
-                if (TM) qyh[0] = bload<T>(rPi, vown, yo(1, 0, ys, yl));
-                if (TE) qyh[1] = bload<T>(rPi, vown, yo(1, 2, ys, yl));
-                if (TE) qye[0] = bload<T>(rPi, vown, yo(0, 0, ys, yl));
-                if (TM) qye[1] = bload<T>(rPi, vown, yo(0, 2, ys, yl));
+                if (TM) qyh[0] = bload_once<T>(rPi, vown, yo(1, 0, ys, yl));
+                if (TE) qyh[1] = bload_once<T>(rPi, vown, yo(1, 2, ys, yl));
+                if (TE) qye[0] = bload_once<T>(rPi, vown, yo(0, 0, ys, yl));
+                if (TM) qye[1] = bload_once<T>(rPi, vown, yo(0, 2, ys, yl));
                 if (TE && !pro && xl >= 0)
-                    hq[2] = bload<T>(rPi, lane == 0 ? (uint32_t)xl * sizeof(T) : OUTSIDE, yo(1, 2, ys, yl));
+                    hq[2] = bload_once<T>(rPi, lane == 0 ? (uint32_t)xl * sizeof(T) : OUTSIDE, yo(1, 2, ys, yl));
             }
         }
         const bool ok = xi >= 0;

@@ -71,15 +71,17 @@ __device__ __forceinline__ P *uniform_ptr(P *p) {
 }
 
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+// stores of the step's new fields and psi, which the next step reads a whole grid later:
+// non-temporal (cache policy NT), so they do not displace rows still to be read this step
 template <typename T>
 __device__ __forceinline__ void bstore(T v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
 template <>
 __device__ __forceinline__ void bstore<float>(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 2);
 }
 template <>
 __device__ __forceinline__ void bstore<double>(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 2);
 }
 
 // m on the grid: wrapped across a periodic axis (any distance), -1 beyond a wall
@@ -342,46 +344,46 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
             if (XS && xany) {
 #pragma unroll
                 for (int r = 0; r <= RY; r++) {
-                    if (r > 0) qx[r][0] = bload<T>(rPi, xo(1, 0, r), 0);
-                    qx[r][1] = bload<T>(rPi, xo(1, 1, r), 0);
+                    if (r > 0) qx[r][0] = bload_once<T>(rPi, xo(1, 0, r), 0);
+                    qx[r][1] = bload_once<T>(rPi, xo(1, 1, r), 0);
                     if (r > 0) {
-                        qxe[r - 1][0] = bload<T>(rPi, xo(0, 0, r), 0);
-                        qxe[r - 1][1] = bload<T>(rPi, xo(0, 1, r), 0);
+                        qxe[r - 1][0] = bload_once<T>(rPi, xo(0, 0, r), 0);
+                        qxe[r - 1][1] = bload_once<T>(rPi, xo(0, 1, r), 0);
                     }
                 }
             }
 #pragma unroll
             for (int r = 0; r <= RY; r++) {
                 if (YS && ys[r] >= 0) {
-                    qy[r][0] = bload<T>(rPi, vown, yo(1, 0, r));
-                    qy[r][1] = bload<T>(rPi, vown, yo(1, 2, r));
+                    qy[r][0] = bload_once<T>(rPi, vown, yo(1, 0, r));
+                    qy[r][1] = bload_once<T>(rPi, vown, yo(1, 2, r));
                     if (r > 0) {
-                        qye[r - 1][0] = bload<T>(rPi, vown, yo(0, 0, r));
-                        qye[r - 1][1] = bload<T>(rPi, vown, yo(0, 2, r));
+                        qye[r - 1][0] = bload_once<T>(rPi, vown, yo(0, 0, r));
+                        qye[r - 1][1] = bload_once<T>(rPi, vown, yo(0, 2, r));
                     }
                 }
                 if (ZS && zs >= 0 && yj[r] >= 0) {
-                    qz[r][0] = bload<T>(rPi, vown, zo(1, 0, r));
+                    qz[r][0] = bload_once<T>(rPi, vown, zo(1, 0, r));
                     if (r > 0) {
-                        qz[r][1] = bload<T>(rPi, vown, zo(1, 1, r));
-                        qze[r - 1][0] = bload<T>(rPi, vown, zo(0, 0, r));
-                        qze[r - 1][1] = bload<T>(rPi, vown, zo(0, 1, r));
+                        qz[r][1] = bload_once<T>(rPi, vown, zo(1, 1, r));
+                        qze[r - 1][0] = bload_once<T>(rPi, vown, zo(0, 0, r));
+                        qze[r - 1][1] = bload_once<T>(rPi, vown, zo(0, 1, r));
                     }
                 }
             }
             if (!pro) {  // the column left of the strip: Hy, Hz on lane m < RY (read only)
                 if (XS && xsl >= 0) {
                     const int q = (k * ny + gyl) * xlcnt + xls;
-                    hq[0] = bload<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][1][0][xsl]) + q) * sizeof(T) : OUTSIDE, 0);
-                    hq[1] = bload<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][2][0][xsl]) + q) * sizeof(T) : OUTSIDE, 0);
+                    hq[0] = bload_once<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][1][0][xsl]) + q) * sizeof(T) : OUTSIDE, 0);
+                    hq[1] = bload_once<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][2][0][xsl]) + q) * sizeof(T) : OUTSIDE, 0);
                 }
                 const int sd = YS && hok ? side_of(gyl, ny, ly, hy) : -1;
                 if (sd >= 0) {
                     const int cnt = sd == 1 ? hy : ly, s = gyl - (sd == 1 ? ny - hy : 0);
-                    hq[2] = bload<T>(rPi, (uint32_t)(K.psi_off[1][2][1][sd] + (k * cnt + s) * nx + xl) * sizeof(T), 0);
+                    hq[2] = bload_once<T>(rPi, (uint32_t)(K.psi_off[1][2][1][sd] + (k * cnt + s) * nx + xl) * sizeof(T), 0);
                 }
                 if (ZS && zs >= 0)
-                    hq[3] = bload<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][1][2][zs]) + (zsl * ny + gyl) * nx + xl) * sizeof(T) : OUTSIDE, 0);
+                    hq[3] = bload_once<T>(rPi, hok ? (uint32_t)(uniform(K.psi_off[1][1][2][zs]) + (zsl * ny + gyl) * nx + xl) * sizeof(T) : OUTSIDE, 0);
             }
         }
 
