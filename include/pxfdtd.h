@@ -313,6 +313,31 @@ int pxfdtd_simulation_add_flux_box(pxfdtd_simulation *s, const int32_t planes[3]
  * from the FP64 transform sums (out holds pxfdtd_simulation_flux_frequencies values) */
 int pxfdtd_simulation_flux(pxfdtd_simulation *s, int32_t n, double *out);
 int32_t pxfdtd_simulation_flux_frequencies(const pxfdtd_simulation *s, int32_t n);
+/* A port's mode for Simulation::add_mode_source (src/fdtd/sources.rs:550-596), given by its
+ * profile as the reference's PortMode3d holds it (src/fdfd/three/ports.rs:46-60): the axis the
+ * waveguide runs along (0 x, 1 y, 2 z), the plane of nodes it was solved on, the cells along
+ * the plane's two axes b = (axis + 1) % 3 and c = (axis + 2) % 3, the step along the axis and
+ * the wavenumber k0 it was solved at, its propagation constant beta (rad/um), E_b then E_c on
+ * the plane (2 x size[0] x size[1] complex as re, im; b fastest) and E_a on the half-plane
+ * after it (size[0] x size[1] complex), going forward at unit amplitude. The profile comes
+ * from a mode solver: the host mirror holds none (the reference's is FDFD's PortMode3d). */
+typedef struct {
+    int32_t axis, plane;
+    int32_t size[2];
+    double step, k0;
+    double beta[2];
+    const double *tangential;
+    const double *normal;
+} pxfdtd_port_mode;
+/* The mode launched going `direction` (0 forward, 1 backward) along its axis by
+ * total-field/scattered-field on its plane, at the waveform's carrier: the currents of the
+ * surface's couplings (src/fdtd/sources.rs:321-418, 419-453) for the mode's E and the grid's
+ * curl of it, H~ = curl E / (i k~). The same refusals as the reference's: a plane not this
+ * grid's (size or step), a periodic axis, the plane and the next not clear of the CPMLs and
+ * the ends, a waveform with no carrier, beyond the Nyquist frequency, or whose leapfrog
+ * wavenumber is not k0 to 1e-9. */
+int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxfdtd_port_mode *mode,
+                                      int32_t direction, const pxfdtd_waveform *w);
 /* Simulation::add_plane_wave (src/fdtd/sources.rs:701-721): a plane wave of `w` on the
  * total-field/scattered-field box of nodes low .. high, travelling along the integer direction,
  * E along the polarization's part across it, in a uniform background of permittivity eps; its

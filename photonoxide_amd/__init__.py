@@ -5,7 +5,7 @@ The compute path is hand-written HIP for gfx950 in ``csrc/`` behind a C ABI
 ``fdtd`` API in :mod:`photonoxide_amd.fdtd`.
 """
 from .fdtd import (Axis, Box, Boundaries, Cpml, Current, Cylinder, Dipole, Field, Gpu, Grid3d,
-                   Periodic, PlaneWave, Pml, Precision, Scene, Simulation, Source, Waveform)
+                   Periodic, PlaneWave, Pml, PortMode, Precision, Scene, Simulation, Source, Waveform)
 
 __all__ = ["Axis", "Box", "Boundaries", "Cpml", "Current", "Cylinder", "Dipole", "Field", "Gpu",
-           "Grid3d", "Periodic", "PlaneWave", "Pml", "Precision", "Scene", "Simulation", "Source", "Waveform"]
+           "Grid3d", "Periodic", "PlaneWave", "Pml", "PortMode", "Precision", "Scene", "Simulation", "Source", "Waveform"]

@@ -620,6 +620,195 @@ static int64_t gcd64(int64_t a, int64_t b) { return b == 0 ? (a < 0 ? -a : a) : 
 
 static long long mod2(long long v) { return ((v % 2) + 2) % 2; }
 
+// The couplings across a total-field/scattered-field surface among the values in the region
+// [lo, hi) (src/fdtd/sources.rs:321-418): each value whose update reads a value on the other
+// side of the surface, `total(field, component, at)` saying which side each value is on, with
+// the weight that value's difference carries. Values E is held at zero on (cb = 0: walls,
+// conductors) neither take nor give a correction.
+struct Link { int target, comp; long long r; double weight; int from_comp; long long from[3]; };
+
+template <typename Total>
+static std::vector<Link> collect_links(const pxfdtd_simulation *s, const long long lo[3], const long long hi[3],
+                                       const Total &total) {
+    auto neighbour = [&](const long long at[3], int axis, bool forward, long long out[3]) {
+        for (int a = 0; a < 3; a++) out[a] = at[a];
+        const long long n = s->n[axis];
+        if (forward) {
+            if (at[axis] + 1 < n) out[axis]++;
+            else if (s->b.periodic[axis]) out[axis] = 0;
+            else return false;
+        } else if (at[axis] > 0) {
+            out[axis]--;
+        } else if (s->b.periodic[axis]) {
+            out[axis] = n - 1;
+        } else {
+            return false;
+        }
+        return true;
+    };
+    auto index_of = [&](const long long at[3]) { return (at[2] * s->n[1] + at[1]) * s->n[0] + at[0]; };
+    std::vector<Link> links;
+    const long long N = s->N;
+    for (long long k = lo[2]; k < hi[2]; k++)
+        for (long long j = lo[1]; j < hi[1]; j++)
+            for (long long i = lo[0]; i < hi[0]; i++) {
+                const long long at[3] = {i, j, k};
+                const long long r = index_of(at);
+                for (int c = 0; c < 3; c++) {
+                    const int a = (c + 1) % 3, b = (c + 2) % 3;
+                    long long nb[3];
+                    // E_c: curl = D_a H_b - D_b H_a with back differences
+                    if (s->cb[c * N + r] != 0.0) {
+                        const bool in = total(PXFDTD_E, c, at);
+                        const double fa = s->fe[a][at[a]], fb = s->fe[b][at[b]];
+                        auto take = [&](int fc, const long long *from, double coef) {
+                            if (from && total(PXFDTD_H, fc, from) != in)
+                                links.push_back({PXFDTD_E, c, r, (in ? -1.0 : 1.0) * coef, fc, {from[0], from[1], from[2]}});
+                        };
+                        take(b, at, fa);
+                        take(b, neighbour(at, a, false, nb) ? nb : nullptr, -fa);
+                        take(a, at, -fb);
+                        take(a, neighbour(at, b, false, nb) ? nb : nullptr, fb);
+                    }
+                    // H~_c: curl = D_a E_b - D_b E_a with forward differences; held E gives none
+                    const bool in = total(PXFDTD_H, c, at);
+                    const double fa = s->fh[a][at[a]], fb = s->fh[b][at[b]];
+                    auto take = [&](int fc, const long long *from, double coef) {
+                        if (!from || s->cb[fc * N + index_of(from)] == 0.0) return;
+                        if (total(PXFDTD_E, fc, from) != in)
+                            links.push_back({PXFDTD_H, c, r, (in ? 1.0 : -1.0) * coef, fc, {from[0], from[1], from[2]}});
+                    };
+                    take(b, neighbour(at, a, true, nb) ? nb : nullptr, fa);
+                    take(b, at, -fa);
+                    take(a, neighbour(at, b, true, nb) ? nb : nullptr, -fb);
+                    take(a, at, fb);
+                }
+            }
+    return links;
+}
+
+// num-complex's product and quotient, operation for operation
+static C64 cmul(C64 a, C64 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+static C64 cdiv(C64 a, C64 b) {
+    const double n = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n};
+}
+
+// Simulation::add_mode_source (src/fdtd/sources.rs:550-596): the mode's E from
+// PortMode3d::value_at (src/fdfd/three/ports.rs:165-184), H~ from it by h_of (sources.rs:457-
+// 483), the couplings of the planes around the mode's by links (:321-418), their currents by
+// phasor_surface (:419-453), after carrier_wavenumber's checks (:485-508)
+extern "C" int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxfdtd_port_mode *mode,
+                                                 int32_t direction, const pxfdtd_waveform *w) {
+    TRY(check_waveform(w));
+    if (!mode || direction < 0 || direction > 1)
+        return px_fail(PXFDTD_ERR_INVALID, "a mode source needs a mode and a direction (0 forward, 1 backward)");
+    const int axis = mode->axis;
+    if (axis < 0 || axis > 2) return px_fail(PXFDTD_ERR_INVALID, "a mode's axis must be 0, 1 or 2");
+    const int ab = (axis + 1) % 3, ac = (axis + 2) % 3;
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    if (mode->size[0] != s->n[ab] || mode->size[1] != s->n[ac] || mode->step != steps[axis])
+        return px_fail(PXFDTD_ERR_INVALID, "the mode's plane must be this grid's: solve it with FDFD on the same grid");
+    if (s->b.periodic[axis])
+        return px_fail(PXFDTD_ERR_INVALID, "a mode source normal to axis %d needs ends along it, not periodic ones", axis);
+    const long long n = s->n[axis], p = mode->plane;
+    const long long low = s->b.low[axis], high = s->b.high[axis];
+    if (p < low + 1 || p + 2 + high > n)
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the mode's plane %lld and the next must be clear of the CPMLs (%lld and %lld cells) and "
+                       "the ends of %lld cells along axis %d", p, low, high, n, axis);
+    const long long plane_values = (long long)mode->size[0] * mode->size[1];
+    if (!mode->tangential || !mode->normal) return px_fail(PXFDTD_ERR_INVALID, "a mode needs its profile");
+    for (long long q = 0; q < 4 * plane_values; q++)
+        if (!std::isfinite(mode->tangential[q])) return px_fail(PXFDTD_ERR_INVALID, "a mode's profile must be finite");
+    for (long long q = 0; q < 2 * plane_values; q++)
+        if (!std::isfinite(mode->normal[q])) return px_fail(PXFDTD_ERR_INVALID, "a mode's profile must be finite");
+    if (!std::isfinite(mode->beta[0]) || !std::isfinite(mode->beta[1]))
+        return px_fail(PXFDTD_ERR_INVALID, "a mode's propagation constant must be finite");
+    // carrier_wavenumber: the leapfrog's wavenumber at the carrier, which must be the mode's k0
+    if (w->kind == 1)
+        return px_fail(PXFDTD_ERR_INVALID, "the mode needs a waveform with a carrier (a Gaussian or a continuous wave)");
+    const double omega = TAU * w->frequency;
+    if (omega * s->dt >= M_PI)
+        return px_fail(PXFDTD_ERR_INVALID, "%g c/um is beyond the time step's Nyquist frequency", w->frequency);
+    double sn = 0.0, cs = 0.0;
+    pxfdtd::psincos(omega * s->dt / 2.0, sn, cs);
+    const double tilde = 2.0 / s->dt * sn;
+    if (std::fabs(mode->k0 - tilde) > 1e-9 * tilde)
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the mode was solved at k0 = %.17g rad/um, but the waveform's carrier runs at the leapfrog's "
+                       "%.17g: solve it at the leapfrog's wavelength", mode->k0, tilde);
+    const double k0 = tilde;
+    const bool forward = direction == 0;
+
+    // PortMode3d::e and value_at
+    auto profile = [&](int comp, long long u, long long v) -> C64 {
+        const long long k = v * mode->size[0] + u;
+        if (comp == axis) return {mode->normal[2 * k], mode->normal[2 * k + 1]};
+        const long long o = comp == ab ? k : plane_values + k;
+        return {mode->tangential[2 * o], mode->tangential[2 * o + 1]};
+    };
+    C64 i_beta = cmul({0.0, 1.0}, {mode->beta[0], mode->beta[1]});
+    i_beta = {i_beta.re * mode->step, i_beta.im * mode->step};
+    auto e_of = [&](int comp, const long long at[3]) -> C64 {
+        const double st = (double)at[axis] - (double)p;
+        const C64 ev = profile(comp, at[ab], at[ac]);
+        if (forward) return cmul(ev, pxfdtd::pcexp({i_beta.re * st, i_beta.im * st}));
+        const C64 neg = {-i_beta.re, -i_beta.im};
+        const C64 phase = pxfdtd::pcexp({neg.re * st, neg.im * st});
+        if (comp == axis) return cmul(cmul({-ev.re, -ev.im}, pxfdtd::pcexp(neg)), phase);
+        return cmul(ev, phase);
+    };
+    // h_of: curl E / (i k0), forward differences, zero beyond a wall
+    auto h_of = [&](int comp, const long long at[3]) -> C64 {
+        const int a = (comp + 1) % 3, b = (comp + 2) % 3;
+        auto next = [&](int ax, int f) -> C64 {
+            long long m[3] = {at[0], at[1], at[2]};
+            if (at[ax] + 1 < s->n[ax]) m[ax]++;
+            else if (s->b.periodic[ax]) m[ax] = 0;
+            else return {0.0, 0.0};
+            return e_of(f, m);
+        };
+        const double fa = s->fh[a][at[a]], fb = s->fh[b][at[b]];
+        const C64 da = next(a, b), eb = e_of(b, at), db = next(b, a), ea = e_of(a, at);
+        const C64 t1 = {(da.re - eb.re) * fa, (da.im - eb.im) * fa};
+        const C64 t2 = {(db.re - ea.re) * fb, (db.im - ea.im) * fb};
+        return cdiv({t1.re - t2.re, t1.im - t2.im}, {0.0, k0});
+    };
+    auto total = [&](int field, int comp, const long long at[3]) {
+        const long long t = twice(field, comp, axis, at[axis]);
+        return forward ? t >= 2 * p : t <= 2 * (p + 1);
+    };
+    long long lo[3] = {0, 0, 0}, hi[3] = {s->n[0], s->n[1], s->n[2]};
+    lo[axis] = p - 1;
+    hi[axis] = std::min(p + 3, n);
+    const std::vector<Link> links = collect_links(s, lo, hi, total);
+    // phasor_surface: per target field, the links' sums per (component, value) in that order,
+    // zeros dropped
+    for (int field : {PXFDTD_E, PXFDTD_H}) {
+        std::map<std::pair<int, long long>, C64> values;
+        for (const Link &l : links) {
+            if (l.target != field) continue;
+            const C64 inc = l.target == PXFDTD_E ? h_of(l.from_comp, l.from) : e_of(l.from_comp, l.from);
+            C64 &acc = values[{l.comp, l.r}];
+            acc.re += l.weight * inc.re;
+            acc.im += l.weight * inc.im;
+        }
+        CurrentRec c;
+        c.field = field;
+        c.w = *w;
+        for (auto &kv : values) {
+            if (kv.second.re == 0.0 && kv.second.im == 0.0) continue;
+            c.comps.push_back(kv.first.first);
+            c.r.push_back(kv.first.second);
+            c.amps.push_back(kv.second);
+        }
+        if (!c.r.empty()) s->currents.push_back(std::move(c));
+    }
+    s->inj_dirty = true;
+    return PXFDTD_OK;
+}
+
 // Simulation::add_plane_wave: PlaneWaveRun::new (src/fdtd/sources.rs:774-950) with the box's
 // couplings from Simulation::links (:321-418) and the surface check Simulation::uniform
 // (:511-522)
@@ -677,66 +866,14 @@ extern "C" int pxfdtd_simulation_add_plane_wave(pxfdtd_simulation *s, const int3
         }
         return true;
     };
-    auto neighbour = [&](const long long at[3], int axis, bool forward, long long out[3]) {
-        for (int a = 0; a < 3; a++) out[a] = at[a];
-        const long long n = s->n[axis];
-        if (forward) {
-            if (at[axis] + 1 < n) out[axis]++;
-            else if (s->b.periodic[axis]) out[axis] = 0;
-            else return false;
-        } else if (at[axis] > 0) {
-            out[axis]--;
-        } else if (s->b.periodic[axis]) {
-            out[axis] = n - 1;
-        } else {
-            return false;
-        }
-        return true;
-    };
-    auto index_of = [&](const long long at[3]) { return (at[2] * s->n[1] + at[1]) * s->n[0] + at[0]; };
-    struct Link { int target, comp; long long r; double weight; int from_comp; long long from[3]; };
-    std::vector<Link> links;
     long long lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = faces[a] ? low[a] - 1 : 0;
         hi[a] = faces[a] ? high[a] + 2 : s->n[a];
     }
     const long long N = s->N;
-    for (long long k = lo[2]; k < hi[2]; k++)
-        for (long long j = lo[1]; j < hi[1]; j++)
-            for (long long i = lo[0]; i < hi[0]; i++) {
-                const long long at[3] = {i, j, k};
-                const long long r = index_of(at);
-                for (int c = 0; c < 3; c++) {
-                    const int a = (c + 1) % 3, b = (c + 2) % 3;
-                    long long nb[3];
-                    // E_c: curl = D_a H_b - D_b H_a with back differences
-                    if (s->cb[c * N + r] != 0.0) {
-                        const bool in = total(PXFDTD_E, c, at);
-                        const double fa = s->fe[a][at[a]], fb = s->fe[b][at[b]];
-                        auto take = [&](int fc, const long long *from, double coef) {
-                            if (from && total(PXFDTD_H, fc, from) != in)
-                                links.push_back({PXFDTD_E, c, r, (in ? -1.0 : 1.0) * coef, fc, {from[0], from[1], from[2]}});
-                        };
-                        take(b, at, fa);
-                        take(b, neighbour(at, a, false, nb) ? nb : nullptr, -fa);
-                        take(a, at, -fb);
-                        take(a, neighbour(at, b, false, nb) ? nb : nullptr, fb);
-                    }
-                    // H~_c: curl = D_a E_b - D_b E_a with forward differences; held E gives none
-                    const bool in = total(PXFDTD_H, c, at);
-                    const double fa = s->fh[a][at[a]], fb = s->fh[b][at[b]];
-                    auto take = [&](int fc, const long long *from, double coef) {
-                        if (!from || s->cb[fc * N + index_of(from)] == 0.0) return;
-                        if (total(PXFDTD_E, fc, from) != in)
-                            links.push_back({PXFDTD_H, c, r, (in ? 1.0 : -1.0) * coef, fc, {from[0], from[1], from[2]}});
-                    };
-                    take(b, neighbour(at, a, true, nb) ? nb : nullptr, fa);
-                    take(b, at, -fa);
-                    take(a, neighbour(at, b, true, nb) ? nb : nullptr, -fb);
-                    take(a, at, fb);
-                }
-            }
+    auto index_of = [&](const long long at[3]) { return (at[2] * s->n[1] + at[1]) * s->n[0] + at[0]; };
+    const std::vector<Link> links = collect_links(s, lo, hi, total);
     // every E on the surface in a lossless medium of eps, not dispersive
     const double expected = s->dt / eps;
     for (const Link &l : links) {

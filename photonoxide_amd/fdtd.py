@@ -206,6 +206,23 @@ class PlaneWave:
 
 
 @dataclass
+class PortMode:
+    """A port's mode as the reference's PortMode3d holds it (src/fdfd/three/ports.rs:46-60), from
+    a mode solver: the waveguide's axis, the plane of nodes it was solved on, the step along the
+    axis and the wavenumber k0 it was solved at, its propagation constant beta (rad/um), E on
+    the plane across the axis as `tangential[2][size_c][size_b]` (E_b then E_c, b = (axis + 1)
+    % 3, c = (axis + 2) % 3) and E_a on the half-plane after it as `normal[size_c][size_b]`,
+    going forward at unit amplitude."""
+    axis: int
+    plane: int
+    step: float
+    k0: float
+    beta: complex
+    tangential: np.ndarray
+    normal: np.ndarray
+
+
+@dataclass
 class Box:
     """An open box lo < p < hi (inf allowed) with a diagonal permittivity (or one value)."""
     lo: tuple
@@ -393,6 +410,27 @@ class Simulation:
             (C.c_int64 * 3)(*wave.direction), (C.c_double * 3)(*wave.polarization), float(wave.eps),
             C.byref(wave.waveform.native()), C.byref(n)))
         return n.value
+
+    def add_mode_source(self, mode: PortMode, direction: int, waveform: Waveform):
+        """Simulation::add_mode_source (src/fdtd/sources.rs:550-596): the mode launched going
+        `direction` (0 forward, 1 backward) along its axis by total-field/scattered-field on its
+        plane, at the waveform's carrier, as fixed currents."""
+        tan = np.ascontiguousarray(mode.tangential, dtype=np.complex128)
+        nor = np.ascontiguousarray(mode.normal, dtype=np.complex128)
+        if tan.ndim != 3 or tan.shape[0] != 2 or nor.shape != tan.shape[1:]:
+            raise ValueError("a mode's tangential E is [2][size_c][size_b] and its normal E [size_c][size_b]")
+        tv = tan.view(np.float64).ravel()
+        nv = nor.view(np.float64).ravel()
+        m = N.PortMode(int(mode.axis), int(mode.plane), (C.c_int32 * 2)(tan.shape[2], tan.shape[1]),
+                       float(mode.step), float(mode.k0), (C.c_double * 2)(complex(mode.beta).real, complex(mode.beta).imag),
+                       N.dptr(tv), N.dptr(nv))
+        N.check(N.lib.pxfdtd_simulation_add_mode_source(self._h, C.byref(m), int(direction),
+                                                        C.byref(waveform.native())))
+
+    def leapfrog_wavenumber(self, frequency: float) -> float:
+        """The wavenumber a source at `frequency` settles to on the grid, (2/dt) sin(w dt/2)
+        (src/fdtd/sources.rs:171-173), the k0 a mode for it is solved at."""
+        return 2.0 / self.dt() * math.sin(2.0 * math.pi * frequency * self.dt() / 2.0)
 
     def incident(self, n: int, field, component, at) -> float:
         """Plane wave n's incident field now at a value (Simulation::incident, sources.rs:723-737)."""
