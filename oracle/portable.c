@@ -9,8 +9,8 @@
  *     polynomial R(r^2) of degree 5, result scaled through the exponent bits;
  *   - sincos: musl libm sincos.c (the `libm` crate 0.2.16 is its Rust port), i.e. fdlibm's
  *     kernels __sin/__cos on [-pi/4, pi/4] after the medium-size Cody-Waite reduction
- *     __rem_pio2 in three rounds (arguments above 2^20 pi/2 are refused: no waveform or
- *     phase of a run reaches them);
+ *     __rem_pio2 in three rounds, and above 2^20 pi/2 fdlibm's k_rem_pio2 (Payne-Hanek with
+ *     2/pi in 24-bit chunks);
  *   - powi: square and multiply over the binary digits of |n|, then 1/x^|n| for n < 0.
  * Compiled with -ffp-contract=off: every + - * / is one IEEE 754 rounding, as in Rust.
  * Pinned by: the reference's FDTD canary hash (tests/canary.rs:1007-1062), which runs every
@@ -131,6 +131,140 @@ static int rem_pio2_medium(double x, uint32_t ix, double *y0, double *y1) {
     return n;
 }
 
+/* The reduction of a large argument (fdlibm's k_rem_pio2.c, Sun Microsystems 1993, as musl's
+ * __rem_pio2_large and Rust's libm rem_pio2_large call it with prec 1): x = sum tx[i] 2^(e0 -
+ * 24 i) with 24-bit integer tx, times 2/pi in 24-bit chunks, the integer part mod 8 and the
+ * fraction as y0 + y1 (times pi/2 in 24-bit pieces).
+ * ipio2: 2/pi = sum ipio2[i] 2^(-24 (i + 1)); pio2_pieces: pi/2 = sum of 24-bit pieces at
+ * 2^0, 2^-24, ... Both are the exact digits of the constants (written out with mpmath). */
+static const int32_t ipio2[66] = {
+    0xA2F983, 0x6E4E44, 0x1529FC, 0x2757D1, 0xF534DD, 0xC0DB62, 0x95993C, 0x439041, 0xFE5163,
+    0xABDEBB, 0xC561B7, 0x246E3A, 0x424DD2, 0xE00649, 0x2EEA09, 0xD1921C, 0xFE1DEB, 0x1CB129,
+    0xA73EE8, 0x8235F5, 0x2EBB44, 0x84E99C, 0x7026B4, 0x5F7E41, 0x3991D6, 0x398353, 0x39F49C,
+    0x845F8B, 0xBDF928, 0x3B1FF8, 0x97FFDE, 0x05980F, 0xEF2F11, 0x8B5A0A, 0x6D1F6D, 0x367ECF,
+    0x27CB09, 0xB74F46, 0x3F669E, 0x5FEA2D, 0x7527BA, 0xC7EBE5, 0xF17B3D, 0x0739F7, 0x8A5292,
+    0xEA6BFB, 0x5FB11F, 0x8D5D08, 0x560330, 0x46FC7B, 0x6BABF0, 0xCFBC20, 0x9AF436, 0x1DA9E3,
+    0x91615E, 0xE61B08, 0x659985, 0x5F14A0, 0x68408D, 0xFFD880, 0x4D7327, 0x310606, 0x1556CA,
+    0x73A8C9, 0x60E27B, 0xC08C6B};
+static const double pio2_pieces[8] = {
+    1.57079625129699707031e+00, 7.54978941586159635335e-08, 5.39030252995776476554e-15,
+    3.28200341580791294123e-22, 1.27065575308067607349e-29, 1.22933308981111328932e-36,
+    2.73370053816464559624e-44, 2.16741683877804819444e-51};
+
+static int rem_pio2_large(const double *x, double *y, int e0, int nx) {
+    const double two24 = 16777216.0, twon24 = 5.9604644775390625e-08;
+    const int jk = 4, jp = 4; /* prec 1: init_jk[1] */
+    int32_t iq[20];
+    double f[20], fq[20], q[20];
+    int jx = nx - 1, jv = (e0 - 3) / 24, i, j, k, m, n, jz, carry, ih;
+    double z, fw;
+    if (jv < 0) jv = 0;
+    int q0 = e0 - 24 * (jv + 1);
+    j = jv - jx;
+    m = jx + jk;
+    for (i = 0; i <= m; i++, j++) f[i] = j < 0 ? 0.0 : (double)ipio2[j];
+    for (i = 0; i <= jk; i++) {
+        for (j = 0, fw = 0.0; j <= jx; j++) fw += x[j] * f[jx + i - j];
+        q[i] = fw;
+    }
+    jz = jk;
+    for (;;) {
+        /* distill q[] into iq[] reversingly */
+        for (i = 0, j = jz, z = q[jz]; j > 0; i++, j--) {
+            fw = (double)(int32_t)(twon24 * z);
+            iq[i] = (int32_t)(z - two24 * fw);
+            z = q[j - 1] + fw;
+        }
+        z = scalbn(z, q0);
+        z -= 8.0 * floor(z * 0.125);
+        n = (int32_t)z;
+        z -= (double)n;
+        ih = 0;
+        if (q0 > 0) {
+            i = iq[jz - 1] >> (24 - q0);
+            n += i;
+            iq[jz - 1] -= i << (24 - q0);
+            ih = iq[jz - 1] >> (23 - q0);
+        } else if (q0 == 0) {
+            ih = iq[jz - 1] >> 23;
+        } else if (z >= 0.5) {
+            ih = 2;
+        }
+        if (ih > 0) { /* q > 0.5 */
+            n += 1;
+            carry = 0;
+            for (i = 0; i < jz; i++) {
+                j = iq[i];
+                if (carry == 0) {
+                    if (j != 0) {
+                        carry = 1;
+                        iq[i] = 0x1000000 - j;
+                    }
+                } else {
+                    iq[i] = 0xffffff - j;
+                }
+            }
+            if (q0 == 1) iq[jz - 1] &= 0x7fffff;
+            else if (q0 == 2) iq[jz - 1] &= 0x3fffff;
+            if (ih == 2) {
+                z = 1.0 - z;
+                if (carry != 0) z -= scalbn(1.0, q0);
+            }
+        }
+        /* a zero fraction: more terms of 2/pi, unless it is exact */
+        if (z == 0.0) {
+            j = 0;
+            for (i = jz - 1; i >= jk; i--) j |= iq[i];
+            if (j == 0) {
+                for (k = 1; iq[jk - k] == 0; k++) {}
+                for (i = jz + 1; i <= jz + k; i++) {
+                    f[jx + i] = (double)ipio2[jv + i];
+                    for (j = 0, fw = 0.0; j <= jx; j++) fw += x[j] * f[jx + i - j];
+                    q[i] = fw;
+                }
+                jz += k;
+                continue;
+            }
+        }
+        break;
+    }
+    if (z == 0.0) {
+        jz -= 1;
+        q0 -= 24;
+        while (iq[jz] == 0) {
+            jz--;
+            q0 -= 24;
+        }
+    } else {
+        z = scalbn(z, -q0);
+        if (z >= two24) {
+            fw = (double)(int32_t)(twon24 * z);
+            iq[jz] = (int32_t)(z - two24 * fw);
+            jz += 1;
+            q0 += 24;
+            iq[jz] = (int32_t)fw;
+        } else {
+            iq[jz] = (int32_t)z;
+        }
+    }
+    fw = scalbn(1.0, q0);
+    for (i = jz; i >= 0; i--) {
+        q[i] = fw * (double)iq[i];
+        fw *= twon24;
+    }
+    for (i = jz; i >= 0; i--) {
+        for (fw = 0.0, k = 0; k <= jp && k <= jz - i; k++) fw += pio2_pieces[k] * q[i + k];
+        fq[jz - i] = fw;
+    }
+    fw = 0.0;
+    for (i = jz; i >= 0; i--) fw += fq[i];
+    y[0] = ih == 0 ? fw : -fw;
+    fw = fq[0] - fw;
+    for (i = 1; i <= jz; i++) fw += fq[i];
+    y[1] = ih == 0 ? fw : -fw;
+    return n & 7;
+}
+
 static int rem_pio2(double x, double *y0, double *y1) {
     const double pio2_1 = 1.57079632673412561417e+00, pio2_1t = 6.07710050650619224932e-11;
     uint64_t u = bits_of(x);
@@ -176,8 +310,26 @@ static int rem_pio2(double x, double *y0, double *y1) {
         }
     }
     if (ix < 0x413921fbu) return rem_pio2_medium(x, ix, y0, y1);
-    fprintf(stderr, "oracle: sincos argument %g beyond 2^20 pi/2 is not supported\n", x);
-    abort();
+    /* |x| >= 2^20 pi/2: x's significand as three 24-bit pieces of an integer scaled by 2^e0 */
+    double tx[3], ty[2];
+    uint64_t v = (u & (((uint64_t)1 << 52) - 1)) | ((uint64_t)(0x3ff + 23) << 52);
+    double z = of_bits(v);
+    int i;
+    for (i = 0; i < 2; i++) {
+        tx[i] = (double)(int32_t)z;
+        z = (z - tx[i]) * 16777216.0;
+    }
+    tx[i] = z;
+    while (tx[i] == 0.0) i--;
+    int n = rem_pio2_large(tx, ty, (int)(ix >> 20) - (0x3ff + 23), i + 1);
+    if (sign) {
+        *y0 = -ty[0];
+        *y1 = -ty[1];
+        return -n;
+    }
+    *y0 = ty[0];
+    *y1 = ty[1];
+    return n;
 }
 
 void or_sincos(double x, double *s, double *c) {

@@ -1,6 +1,8 @@
 """The oracle pinned against the reference's golden vectors and against what the mathematics
 fixes (CPU only)."""
 import math
+import os
+import struct
 
 import numpy as np
 import pytest
@@ -59,6 +61,95 @@ def test_portable_exp_and_sincos():
     assert s.value == 0.0 and c.value == 1.0
     # fdlibm's exp(1) is one ulp above e's nearest double, as fdlibm is known to give
     assert O.lib.or_exp(0.0) == 1.0 and O.lib.or_exp(1.0) == 2.7182818284590455
+
+
+# src/portable/tests.rs:357-548: the reference's grid of arguments (special values, the
+# subnormals' edge, 600 from a fixed LCG over five scales) and its hashes of each portable
+# function's bits over it (h = rotl(h, 7) ^ bits, NaN as the canonical quiet NaN), taken on
+# x86-64 with libm 0.2.16. Pairs are (grid[i], grid[n - 1 - i]).
+GRID_HASHES = {"exp": 0x413FBC04627E9685, "sin_cos.0": 0xDA525217FA01FBF0,
+               "sin_cos.1": 0xDA5C8775269B6B93, "powi": 0x0193356F37480A41,
+               "c_exp": 0x9FC0EEDECCB4CA6E}
+
+
+def reference_grid():
+    xs = [0.0, -0.0, math.inf, -math.inf, math.nan, 2.2250738585072014e-308, struct.unpack("<d", struct.pack("<Q", 1))[0],
+          -struct.unpack("<d", struct.pack("<Q", 3))[0], 1.7976931348623157e308, -1.7976931348623157e308,
+          1.0, -1.0, 0.5, 2.0, math.pi, math.pi / 2, 1e22, -709.5, 710.0, 26.5]
+    s = 0x0822_2026_1113_0001
+    for scale in (1.0, 4.0, 30.0, 800.0, 1e6):
+        for _ in range(120):
+            s = (s * 6_364_136_223_846_793_005 + 1_442_695_040_888_963_407) % (1 << 64)
+            xs.append(((s >> 11) / 9_007_199_254_740_992.0 - 0.5) * 2.0 * scale)
+    return xs
+
+
+def grid_fold(h, v):
+    b = 0x7FF8000000000000 if math.isnan(v) else struct.unpack("<Q", struct.pack("<d", v))[0]
+    return (((h << 7) | (h >> 57)) & ((1 << 64) - 1)) ^ b
+
+
+def test_portable_bits_are_the_references_over_its_grid():
+    """exp, sin_cos (both parts), powi and complex exp give the reference's bits at every
+    argument of its grid, large ones (1e22, the largest double: Payne-Hanek) and special ones
+    (inf, NaN, subnormals) included: the hashes are the reference's own."""
+    xs = reference_grid()
+    pairs = list(zip(xs, reversed(xs)))
+    s, c = O.C.c_double(), O.C.c_double()
+
+    def sincos(x):
+        O.lib.or_sincos(x, O.C.byref(s), O.C.byref(c))
+        return s.value, c.value
+
+    def as_i32(v):  # Rust's saturating `as i32`
+        return 0 if math.isnan(v) else int(max(min(v, 2**31 - 1), -2**31))
+
+    got = {"exp": 0, "sin_cos.0": 0, "sin_cos.1": 0, "powi": 0, "c_exp": 0}
+    for x in xs:
+        got["exp"] = grid_fold(got["exp"], O.lib.or_exp(x))
+        sv, cv = sincos(x)
+        got["sin_cos.0"] = grid_fold(got["sin_cos.0"], sv)
+        got["sin_cos.1"] = grid_fold(got["sin_cos.1"], cv)
+    for a, b in pairs:
+        n = as_i32(b * 7.0)
+        got["powi"] = grid_fold(got["powi"], O.lib.or_powi(a, int(math.copysign(abs(n) % 64, n))))
+        r = O.lib.or_exp(a)  # portable::complex::exp: exp(re) (cos im, sin im), im 0 kept
+        if b == 0.0:
+            w = (r, b)
+        else:
+            sv, cv = sincos(b)
+            w = (r * cv, r * sv)
+        got["c_exp"] = grid_fold(grid_fold(got["c_exp"], w[0]), w[1])
+    for name, want in GRID_HASHES.items():
+        assert got[name] == want, (name, hex(got[name]), hex(want))
+
+
+def test_portable_within_the_references_bounds_of_its_golden_values():
+    """tests/data/canary/portable.txt (copied as tests/golden/portable_exp_sin_cos.txt): the
+    correctly rounded exp, sin and cos at the reference's 48 arguments each (the largest known
+    errors of OpenLibm and musl among them, and arguments to 1e300), held to the reference's
+    bounds (src/portable/tests.rs:200-215: 0.949, 0.831, 0.834 ulps, so one binary64 step)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "portable_exp_sin_cos.txt")
+    s, c = O.C.c_double(), O.C.c_double()
+    seen = {"exp": 0, "sin": 0, "cos": 0}
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        f = line.split()
+        name, a, r = f[0], int(f[1], 16), int(f[5], 16)
+        x = struct.unpack("<d", struct.pack("<Q", a))[0]
+        if name == "exp":
+            got = O.lib.or_exp(x)
+        else:
+            O.lib.or_sincos(x, O.C.byref(s), O.C.byref(c))
+            got = s.value if name == "sin" else c.value
+        gb = struct.unpack("<q", struct.pack("<d", got))[0]
+        wb = struct.unpack("<q", struct.pack("<d", struct.unpack("<d", struct.pack("<Q", r))[0]))[0]
+        # binary64 numbers between them (same sign: the ordered integer distance)
+        assert (gb < 0) == (wb < 0) or got == 0.0, (name, x, got)
+        assert abs(gb - wb) <= 1, (name, x, got, hex(r))
+        seen[name] += 1
+    assert seen == {"exp": 48, "sin": 48, "cos": 48}
 
 
 def test_plane_wave_dispersion_is_exact():
