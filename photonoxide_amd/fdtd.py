@@ -92,6 +92,13 @@ class Periodic:
 @dataclass(frozen=True)
 class Cpml:
     """The convolutional PML's parameters (src/fdtd.rs:84-110): FDFD's R = 1e-8, order 3."""
+
+    @staticmethod
+    def sigma_optimal(order: float, eps: float, step: float) -> float:
+        """Taflove's sigma_opt = (m + 1) / (150 pi sqrt(eps) step) in this build's units (sigma
+        times eta0), src/fdtd.rs:115-117."""
+        return (order + 1.0) * 376.730313668 / (150.0 * math.pi) / (math.sqrt(eps) * step)
+
     reflection: float = 1e-8
     sigma: Optional[float] = None
     order: float = 3.0
@@ -337,6 +344,24 @@ class Simulation:
 
     def conductor(self, component, at):
         N.check(N.lib.pxfdtd_simulation_conductor(self._h, int(component), *at))
+
+    def plate(self, normal, node: int, first: range, second: range):
+        """A perfectly conducting plate on the plane of nodes `node` normal to `normal`, over
+        cells `first` and `second` along the other two axes (Simulation::plate,
+        src/fdtd.rs:1013-1047): E along each in-plane axis zeroed on the cells along it and on
+        the nodes (edges included) along the other, within the grid."""
+        a, b = (int(normal) + 1) % 3, (int(normal) + 2) % 3
+        n = (self.grid.nx, self.grid.ny, self.grid.nz)
+
+        def put(component, ra, rb):
+            for ia in ra:
+                for ib in rb:
+                    at = [0, 0, 0]
+                    at[int(normal)], at[a], at[b] = node, ia, ib
+                    if at[0] < n[0] and at[1] < n[1] and at[2] < n[2]:
+                        self.conductor(component, tuple(at))
+        put(a, range(first.start, max(first.stop - 1, 0) + 1), range(second.start, second.stop + 1))
+        put(b, range(first.start, first.stop + 1), range(second.start, max(second.stop - 1, 0) + 1))
 
     def add_source(self, source: Source):
         N.check(N.lib.pxfdtd_simulation_add_source(self._h, int(source.field), int(source.component),
