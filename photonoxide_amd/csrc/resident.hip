@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -206,6 +207,7 @@ struct pxfdtd_resident {
 
 struct TapSpec;
 static std::map<const pxfdtd_resident *, TapSpec> &specs();
+static std::mutex &specs_mutex();
 
 // ---------------------------------------------------------------------- problem checks
 static long long slab_volume(const pxfdtd_problem *p, int axis, int count) {
@@ -638,7 +640,10 @@ extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
     if (r->fork_ev) (void)hipEventDestroy(r->fork_ev);
     if (r->join_ev) (void)hipEventDestroy(r->join_ev);
     hipStream_t s = r->stream, side = r->side;
-    specs().erase(r);
+    {
+        std::lock_guard<std::mutex> g(specs_mutex());
+        specs().erase(r);
+    }
     delete r;  // buffers free themselves
     if (s) (void)hipStreamDestroy(s);
     if (side) (void)hipStreamDestroy(side);
@@ -1281,10 +1286,20 @@ static int rebuild_taps(pxfdtd_resident *r, const TapSpec &t) {
     return PXFDTD_OK;
 }
 
-// the resident keeps its tap specification to rebuild when either part changes
+// the resident keeps its tap specification to rebuild when either part changes; residents are
+// made and freed on several host threads at once (the server's admissions), so the registry is
+// locked (an entry, once made, stays where it is until its resident goes)
+static std::mutex &specs_mutex() {
+    static std::mutex m;
+    return m;
+}
 static std::map<const pxfdtd_resident *, TapSpec> &specs() {
     static std::map<const pxfdtd_resident *, TapSpec> m;
     return m;
+}
+static TapSpec &spec_of(const pxfdtd_resident *r) {
+    std::lock_guard<std::mutex> g(specs_mutex());
+    return specs()[r];
 }
 
 extern "C" int pxfdtd_resident_set_probes(pxfdtd_resident *r, const pxfdtd_probe *probes, int32_t n) {
@@ -1294,7 +1309,7 @@ extern "C" int pxfdtd_resident_set_probes(pxfdtd_resident *r, const pxfdtd_probe
             return px_fail(PXFDTD_ERR_INVALID, "probe %d is out of range", q);
     HIPCHK(hipSetDevice(r->dev->ordinal));
     HIPCHK(hipStreamSynchronize(r->stream));
-    TapSpec &t = specs()[r];
+    TapSpec &t = spec_of(r);
     t.probes.assign(probes, probes + n);
     r->probe_rows = 0;
     return rebuild_taps(r, t);
@@ -1316,7 +1331,7 @@ extern "C" int pxfdtd_resident_set_series(pxfdtd_resident *r, const pxfdtd_serie
         if (monitor_nf[m] < 1) return px_fail(PXFDTD_ERR_INVALID, "a monitor needs frequencies");
     HIPCHK(hipSetDevice(r->dev->ordinal));
     HIPCHK(hipStreamSynchronize(r->stream));
-    TapSpec &t = specs()[r];
+    TapSpec &t = spec_of(r);
     t.series.assign(series, series + n);
     t.monitor_nf.assign(monitor_nf, monitor_nf + monitors);
     return rebuild_taps(r, t);

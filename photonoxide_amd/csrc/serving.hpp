@@ -190,6 +190,7 @@ struct pxfdtd_server {
     size_t used = 0;
     std::chrono::steady_clock::time_point t0;
     std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> tuned;  // shape -> tiling
+    std::mutex tuned_mu;  // admissions run on several host threads
     pxfdtd_server_stats stats;
     // the tables of the job set last built, on the device
     std::vector<ServerJob *> table;  // table entry -> job (finished ones stay, with no steps)
@@ -242,13 +243,20 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
     // coefficient storage
     auto lg = [](int n) { int b = 0; while ((2 << b) <= n) b++; return b; };
     const auto key = std::make_tuple(r->nz == 1 ? 0 : 1, lg(r->nx), lg(r->ny), lg(r->nz), r->mode);
-    auto it = S->tuned.find(key);
+    std::pair<int, int> known = {0, 0};
+    bool have = false;
+    {
+        std::lock_guard<std::mutex> g(S->tuned_mu);
+        auto it = S->tuned.find(key);
+        have = it != S->tuned.end();
+        if (have) known = it->second;
+    }
     if (r->kernel == 4) {
         // 2D jobs: in a batch the device is full, so the longest chunks (the least recomputed
         // rows) win, whatever suits the job alone
         TRY(pxfdtd_resident_set_tiling(r, 1, std::min(r->ny, 16)));
-    } else if (it != S->tuned.end()) {
-        TRY(pxfdtd_resident_set_tiling(r, it->second.first, it->second.second));
+    } else if (have) {
+        TRY(pxfdtd_resident_set_tiling(r, known.first, known.second));
     } else {
         int32_t rows = 2, planes = 8;
         double ms = 0;
@@ -258,6 +266,7 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
             rows = r->ty;
             planes = r->kz;
         }
+        std::lock_guard<std::mutex> g(S->tuned_mu);
         S->tuned[key] = {rows, planes};
     }
     if (r->kernel == 3 && r->ty > 2) TRY(pxfdtd_resident_set_tiling(r, 2, 0));
@@ -471,6 +480,9 @@ static int server_segment_begin(pxfdtd_server *S) {
     const int K = S->cfg.segment;
     const int nj = (int)S->table.size();
     double cells = 0;
+    // every job's waves and phases for the segment (host FP64, each job's own slice of the
+    // tables): on up to 8 host threads, the device waiting on them
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : cells) num_threads(8) if (nj > 8)
     for (int j = 0; j < nj; j++) {
         ServerJob *J = S->table[j];
         const bool on = J->state == PXFDTD_JOB_RUNNING && J->left > 0 && J->r;
@@ -572,24 +584,40 @@ static void server_loop(pxfdtd_server *S) {
         }
         const std::string why = rc ? pxfdtd_last_error() : "";
         // while the segment runs: new jobs onto the device, finished ones' results back
+        // (each job's set-up and results are its own: up to 8 host threads take them at once,
+        // on the jobs' own streams)
         std::vector<std::pair<ServerJob *, std::string>> failed;
         std::vector<ServerJob *> ready;
         const double ta = server_now(S);
-        for (ServerJob *J : admit) {
-            const int rc2 = server_admit(S, J);
-            if (rc2 != PXFDTD_OK) {
-                failed.push_back({J, pxfdtd_last_error()});
-                (void)pxfdtd_host::sim_detach(J->sim);
-            } else {
-                ready.push_back(J);
+        {
+            const int na = (int)admit.size();
+            std::vector<int> arc(na, PXFDTD_OK);
+            std::vector<std::string> aerr(na);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(std::max(1, std::min(8, na))) if (na > 1)
+            for (int q = 0; q < na; q++) {
+                (void)hipSetDevice(S->dev->ordinal);
+                arc[q] = server_admit(S, admit[q]);
+                if (arc[q] != PXFDTD_OK) {
+                    aerr[q] = pxfdtd_last_error();
+                    (void)pxfdtd_host::sim_detach(admit[q]->sim);
+                }
+            }
+            for (int q = 0; q < na; q++) {
+                if (arc[q] != PXFDTD_OK) failed.push_back({admit[q], aerr[q]});
+                else ready.push_back(admit[q]);
             }
         }
         t_admit = server_now(S) - ta;
         const double tf = server_now(S);
-        std::vector<std::pair<ServerJob *, std::string>> fetched;
-        for (ServerJob *J : fetch) {
-            const int rc3 = pxfdtd_host::sim_fetch(J->sim);
-            fetched.push_back({J, rc3 ? pxfdtd_last_error() : ""});
+        std::vector<std::pair<ServerJob *, std::string>> fetched(fetch.size());
+        {
+            const int nf = (int)fetch.size();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(std::max(1, std::min(8, nf))) if (nf > 1)
+            for (int q = 0; q < nf; q++) {
+                (void)hipSetDevice(S->dev->ordinal);
+                const int rc3 = pxfdtd_host::sim_fetch(fetch[q]->sim);
+                fetched[q] = {fetch[q], rc3 ? pxfdtd_last_error() : ""};
+            }
         }
         int rc4 = PXFDTD_OK;
         if (launched) rc4 = server_segment_end(S, &t_busy);
