@@ -963,6 +963,9 @@ static int build_units_2d(pxfdtd_resident *r) {
     return PXFDTD_OK;
 }
 
+// lean units march this many z chunks (MMI step 0.338 -> 0.336 ms, the dispersive grid's
+// 1.469 -> 1.454 ms; 4 chunks: 0.343 and 1.433)
+constexpr int LEAN_CHUNKS = 2;
 static int build_units(pxfdtd_resident *r) {
     if (r->kernel == 4) return build_units_2d(r);
     std::vector<int> cuts = {0};
@@ -1000,6 +1003,29 @@ static int build_units(pxfdtd_resident *r) {
                     ((xs || ys || zs) ? slab : lean).push_back(u);
             }
         }
+    }
+    // the lean units march LEAN_CHUNKS chunks at a time: their waves are the shortest, and a
+    // longer march loads fewer prologue planes
+    if (r->kernel == 3 && LEAN_CHUNKS > 1 && !lean.empty()) {
+        std::sort(lean.begin(), lean.end(), [](const int4 &p, const int4 &q) {
+            return std::make_tuple(p.y, p.x, p.z) < std::make_tuple(q.y, q.x, q.z);
+        });
+        std::vector<int4> merged;
+        for (size_t q = 0; q < lean.size();) {
+            int4 u = lean[q];
+            size_t k = q + 1;
+            while (k < lean.size() && k - q < (size_t)LEAN_CHUNKS && lean[k].x == u.x && lean[k].y == u.y &&
+                   lean[k].z == u.w) {
+                u.w = lean[k].w;
+                k++;
+            }
+            merged.push_back(u);
+            q = k;
+        }
+        std::sort(merged.begin(), merged.end(), [](const int4 &p, const int4 &q) {
+            return std::make_tuple(p.z, p.y, p.x) < std::make_tuple(q.z, q.y, q.x);
+        });
+        lean = merged;
     }
     r->n_lean = (int)lean.size();
     r->n_x = (int)xonly.size();
