@@ -9,12 +9,24 @@
 The 2000 steps end before the pulse's peak (its delay is 6 widths = 26.9 um/c, the run
 14.1 um/c), so the spectra are those of the pulse's leading edge, which is what the workload
 defines.
+
+The power transmission |S21|^2 between two planes across the whole guide (the flux through
+one near the far end over the flux through one just past the line source, Simulation::flux,
+src/fdtd/monitors.rs:318-350) needs the pulse to have passed both: the same problem run for
+12000 steps, the device in f32 against the FP64 oracle (its transforms and oracle/flux.py), the
+error stated per wavelength and held to 1e-4.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 
 from configs import te_strip_2d
 from fixtures import oracle, product, product_fields
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import flux as F  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -76,4 +88,51 @@ def test_te_strip_400x400_f32_against_the_fp64_oracle(gpu, oracles):
     for q in range(ref.shape[0]):
         err = np.linalg.norm(got[q] - ref[q]) / np.linalg.norm(ref[q])
         print(f"spectrum at {lam[q]:.3f} um: relative L2 error vs FP64 oracle {err:.3e}")
+        assert err < 1e-4, (q, err)
+
+
+LONG = 12000
+
+
+def transmission_planes(spec):
+    """The planes across the guide between the CPMLs (the whole z): just past the source and
+    near the far end."""
+    n = spec["grid"][0]
+    h = spec["grid"][3]
+    pml = spec["bounds"]["low"][1]
+    i_in = int(round((-2.0 - spec["grid"][6]) / h))
+    i_out = int(round((3.0 - spec["grid"][6]) / h)) - 2
+    window = ((pml, n - pml - 2), None)
+    return [F.Face(0, i_in, window), F.Face(0, i_out, window)], (n, n, 1)
+
+
+def test_te_strip_transmission_f32_against_the_fp64_oracle(gpu):
+    import photonoxide_amd as P
+    spec = te_strip_2d()
+    faces, n = transmission_planes(spec)
+    freqs = spec["dft"][2]
+    h = spec["grid"][3]
+    o = oracle(spec, with_fields=False)
+    boxes = [f.box(n) for f in faces]
+    ms = [o.add_dft(lo, hi, freqs) for lo, hi in boxes]
+    p = product(spec, with_fields=False)
+    planes = [p.add_flux(P.Axis.X, f.plane, freqs, window=f.window) for f in faces]
+    p.set_device(gpu, P.Precision.F32)
+    print(f"case te_strip_2d transmission, {LONG} steps", flush=True)
+    o.run(LONG)
+    p.run(LONG)
+
+    def value_fn(m, lo):
+        series = {(fd, c): o.dft(m, fd, c) for fd in (0, 1) for c in range(3)}
+        return lambda fd, c, at, f: series[fd, c][f, at[2] - lo[2], at[1] - lo[1], at[0] - lo[0]]
+    values = [value_fn(m, lo) for m, (lo, _) in zip(ms, boxes)]
+    lam = 1 / np.asarray(freqs)
+    for q in range(len(freqs)):
+        want = [faces[k].flux(n, (h, h, h), values[k], q) for k in range(2)]
+        got = [p.flux(planes[k])[q] for k in range(2)]
+        t_want, t_got = want[1] / want[0], got[1] / got[0]
+        err = abs(t_got - t_want) / abs(t_want)
+        print(f"|S21|^2 at {lam[q]:.3f} um: {t_got:.6f} (FP64 oracle {t_want:.6f}), relative error {err:.2e}; "
+              f"fluxes in {got[0]:.4e} out {got[1]:.4e}")
+        assert want[0] > 0 and 0.5 < t_want < 1.0
         assert err < 1e-4, (q, err)
