@@ -338,6 +338,31 @@ typedef struct {
  * wavenumber is not k0 to 1e-9. */
 int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxfdtd_port_mode *mode,
                                       int32_t direction, const pxfdtd_waveform *w);
+/* GaussianBeam (src/fdtd/sources.rs:104-128): the plane's normal axis and node, the direction
+ * (0 forward, 1 backward), where the beam's axis crosses the plane along the other two axes
+ * (um), the waist w0 (the field's 1/e radius at the focus, um), the focus's distance from the
+ * plane along the beam (um), the axis among the other two it tilts towards and the tilt
+ * (radians, |angle| < pi/2), E's direction (0 perpendicular to the plane of incidence, 1
+ * parallel) and the medium's relative permittivity on the plane. */
+typedef struct {
+    int32_t axis, plane, direction;
+    double centre[2];
+    double waist, focus;
+    int32_t tilt;
+    double angle;
+    int32_t polarization;
+    double eps;
+} pxfdtd_gaussian_beam;
+/* Simulation::add_beam (src/fdtd/sources.rs:618-699): the paraxial beam's E on the plane
+ * (beam_profile, :1043-1108), decomposed into the grid's own plane waves over the plane's
+ * window (PlaneSpectrum, :1110-1300: each wave's normal wavenumber from the grid's dispersion
+ * at the leapfrog wavenumber, its normal E making it divergence-free on the grid), launched one
+ * way by total-field/scattered-field at the waveform's carrier as fixed currents. The same
+ * refusals as the reference's (the plane in a CPML or at an end, a periodic axis, a tilt along
+ * the axis or not across a varying axis, |angle| >= pi/2, a non-positive waist or
+ * permittivity, no carrier, a medium on the plane not uniform with eps). */
+int pxfdtd_simulation_add_beam(pxfdtd_simulation *s, const pxfdtd_gaussian_beam *beam,
+                               const pxfdtd_waveform *w);
 /* Simulation::add_plane_wave (src/fdtd/sources.rs:701-721): a plane wave of `w` on the
  * total-field/scattered-field box of nodes low .. high, travelling along the integer direction,
  * E along the polarization's part across it, in a uniform background of permittivity eps; its

@@ -4,8 +4,8 @@ The compute path is hand-written HIP for gfx950 in ``csrc/`` behind a C ABI
 (``include/pxfdtd.h``); this package binds it with ctypes and mirrors the reference's
 ``fdtd`` API in :mod:`photonoxide_amd.fdtd`.
 """
-from .fdtd import (Axis, Box, Boundaries, Cpml, Current, Cylinder, Dipole, Field, Gpu, Grid3d,
+from .fdtd import (Axis, Box, Boundaries, Cpml, Current, Cylinder, Dipole, Field, GaussianBeam, Gpu, Grid3d,
                    Periodic, PlaneWave, Pml, PortMode, Precision, Scene, Simulation, Source, Waveform)
 
-__all__ = ["Axis", "Box", "Boundaries", "Cpml", "Current", "Cylinder", "Dipole", "Field", "Gpu",
+__all__ = ["Axis", "Box", "Boundaries", "Cpml", "Current", "Cylinder", "Dipole", "Field", "GaussianBeam", "Gpu",
            "Grid3d", "Periodic", "PlaneWave", "Pml", "PortMode", "Precision", "Scene", "Simulation", "Source", "Waveform"]

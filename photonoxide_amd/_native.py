@@ -113,6 +113,13 @@ class PortMode(C.Structure):
                 ("tangential", C.POINTER(C.c_double)), ("normal", C.POINTER(C.c_double))]
 
 
+class GaussianBeam(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("plane", C.c_int32), ("direction", C.c_int32),
+                ("centre", C.c_double * 2), ("waist", C.c_double), ("focus", C.c_double),
+                ("tilt", C.c_int32), ("angle", C.c_double), ("polarization", C.c_int32),
+                ("eps", C.c_double)]
+
+
 class PlaneWaveAux(C.Structure):
     _fields_ = [("g", C.c_int32 * 3), ("len", C.c_int64), ("pad", C.c_int64), ("ca", pd), ("cb", pd),
                 ("da", pd), ("db", pd), ("nsheet", C.c_int32), ("sheet_component", C.c_int32 * 6),
@@ -187,6 +194,7 @@ SIGNATURES = {
     "pxfdtd_simulation_flux_frequencies": (C.c_int32, [vp, C.c_int32]),
     "pxfdtd_simulation_add_plane_wave": (C.c_int, [vp, pi32, pi32, pi64, pd, C.c_double, C.POINTER(Wave), pi32]),
     "pxfdtd_simulation_add_mode_source": (C.c_int, [vp, C.POINTER(PortMode), C.c_int32, C.POINTER(Wave)]),
+    "pxfdtd_simulation_add_beam": (C.c_int, [vp, C.POINTER(GaussianBeam), C.POINTER(Wave)]),
     "pxfdtd_simulation_incident": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pd]),
     "pxfdtd_simulation_add_probe": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pi32]),
     "pxfdtd_simulation_add_dft": (C.c_int, [vp, pi32, pi32, C.c_int32, pd, pi32]),

@@ -10,12 +10,14 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <complex>
 #include <omp.h>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pxfdtd.h"
@@ -687,11 +689,75 @@ static std::vector<Link> collect_links(const pxfdtd_simulation *s, const long lo
     return links;
 }
 
+// Simulation::uniform (src/fdtd/sources.rs:511-522): every E value on the couplings' surface in
+// a lossless medium of permittivity eps, not dispersive
+static bool surface_uniform(const pxfdtd_simulation *s, const std::vector<Link> &links, double eps) {
+    const double expected = s->dt / eps;
+    const long long N = s->N;
+    for (const Link &l : links) {
+        const int c = l.target == PXFDTD_E ? l.comp : l.from_comp;
+        const long long r = l.target == PXFDTD_E ? l.r : (l.from[2] * s->n[1] + l.from[1]) * s->n[0] + l.from[0];
+        const bool media = !s->member.empty() && s->member[c * N + r] >= 0;
+        if (!(std::fabs(s->cb[c * N + r] - expected) <= 1e-12 * expected && s->ca[c * N + r] == 1.0 && !media))
+            return false;
+    }
+    return true;
+}
+
 // num-complex's product and quotient, operation for operation
 static C64 cmul(C64 a, C64 b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 static C64 cdiv(C64 a, C64 b) {
     const double n = b.re * b.re + b.im * b.im;
     return {(a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n};
+}
+
+// Simulation::phasor_surface (src/fdtd/sources.rs:419-453): per target field, E then H, the
+// couplings' sums per (component, value) in that order, zeros dropped, added as currents with
+// the waveform. E targets read the incident H~ (`h_of`), H targets the incident E (`e_of`).
+template <typename EOf, typename HOf>
+static void add_phasor_surface(pxfdtd_simulation *s, const std::vector<Link> &links, const EOf &e_of,
+                               const HOf &h_of, const pxfdtd_waveform *w) {
+    for (int field : {PXFDTD_E, PXFDTD_H}) {
+        std::map<std::pair<int, long long>, C64> values;
+        for (const Link &l : links) {
+            if (l.target != field) continue;
+            const C64 inc = l.target == PXFDTD_E ? h_of(l.from_comp, l.from) : e_of(l.from_comp, l.from);
+            C64 &acc = values[{l.comp, l.r}];
+            acc.re += l.weight * inc.re;
+            acc.im += l.weight * inc.im;
+        }
+        CurrentRec c;
+        c.field = field;
+        c.w = *w;
+        for (auto &kv : values) {
+            if (kv.second.re == 0.0 && kv.second.im == 0.0) continue;
+            c.comps.push_back(kv.first.first);
+            c.r.push_back(kv.first.second);
+            c.amps.push_back(kv.second);
+        }
+        if (!c.r.empty()) s->currents.push_back(std::move(c));
+    }
+    s->inj_dirty = true;
+}
+
+// carrier_wavenumber (src/fdtd/sources.rs:485-508): the leapfrog's wavenumber at the waveform's
+// carrier, (2 / dt) sin(w dt / 2); with `k0` given, it must be that to 1e-9
+static int carrier_wavenumber(const pxfdtd_simulation *s, const pxfdtd_waveform *w, const double *k0,
+                              const char *what, double *out) {
+    if (w->kind == 1)
+        return px_fail(PXFDTD_ERR_INVALID, "%s needs a waveform with a carrier (a Gaussian or a continuous wave)", what);
+    const double omega = TAU * w->frequency;
+    if (omega * s->dt >= M_PI)
+        return px_fail(PXFDTD_ERR_INVALID, "%g c/um is beyond the time step's Nyquist frequency", w->frequency);
+    double sn = 0.0, cs = 0.0;
+    pxfdtd::psincos(omega * s->dt / 2.0, sn, cs);
+    const double tilde = 2.0 / s->dt * sn;
+    if (k0 && std::fabs(*k0 - tilde) > 1e-9 * tilde)
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "%s was solved at k0 = %.17g rad/um, but the waveform's carrier runs at the leapfrog's "
+                       "%.17g: solve it at the leapfrog's wavelength", what, *k0, tilde);
+    *out = tilde;
+    return PXFDTD_OK;
 }
 
 // Simulation::add_mode_source (src/fdtd/sources.rs:550-596): the mode's E from
@@ -725,20 +791,8 @@ extern "C" int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxf
         if (!std::isfinite(mode->normal[q])) return px_fail(PXFDTD_ERR_INVALID, "a mode's profile must be finite");
     if (!std::isfinite(mode->beta[0]) || !std::isfinite(mode->beta[1]))
         return px_fail(PXFDTD_ERR_INVALID, "a mode's propagation constant must be finite");
-    // carrier_wavenumber: the leapfrog's wavenumber at the carrier, which must be the mode's k0
-    if (w->kind == 1)
-        return px_fail(PXFDTD_ERR_INVALID, "the mode needs a waveform with a carrier (a Gaussian or a continuous wave)");
-    const double omega = TAU * w->frequency;
-    if (omega * s->dt >= M_PI)
-        return px_fail(PXFDTD_ERR_INVALID, "%g c/um is beyond the time step's Nyquist frequency", w->frequency);
-    double sn = 0.0, cs = 0.0;
-    pxfdtd::psincos(omega * s->dt / 2.0, sn, cs);
-    const double tilde = 2.0 / s->dt * sn;
-    if (std::fabs(mode->k0 - tilde) > 1e-9 * tilde)
-        return px_fail(PXFDTD_ERR_INVALID,
-                       "the mode was solved at k0 = %.17g rad/um, but the waveform's carrier runs at the leapfrog's "
-                       "%.17g: solve it at the leapfrog's wavelength", mode->k0, tilde);
-    const double k0 = tilde;
+    double k0 = 0.0;
+    TRY(carrier_wavenumber(s, w, &mode->k0, "the mode", &k0));
     const bool forward = direction == 0;
 
     // PortMode3d::e and value_at
@@ -783,29 +837,257 @@ extern "C" int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxf
     lo[axis] = p - 1;
     hi[axis] = std::min(p + 3, n);
     const std::vector<Link> links = collect_links(s, lo, hi, total);
-    // phasor_surface: per target field, the links' sums per (component, value) in that order,
-    // zeros dropped
-    for (int field : {PXFDTD_E, PXFDTD_H}) {
-        std::map<std::pair<int, long long>, C64> values;
-        for (const Link &l : links) {
-            if (l.target != field) continue;
-            const C64 inc = l.target == PXFDTD_E ? h_of(l.from_comp, l.from) : e_of(l.from_comp, l.from);
-            C64 &acc = values[{l.comp, l.r}];
-            acc.re += l.weight * inc.re;
-            acc.im += l.weight * inc.im;
-        }
-        CurrentRec c;
-        c.field = field;
-        c.w = *w;
-        for (auto &kv : values) {
-            if (kv.second.re == 0.0 && kv.second.im == 0.0) continue;
-            c.comps.push_back(kv.first.first);
-            c.r.push_back(kv.first.second);
-            c.amps.push_back(kv.second);
-        }
-        if (!c.r.empty()) s->currents.push_back(std::move(c));
+    add_phasor_surface(s, links, e_of, h_of, w);
+    return PXFDTD_OK;
+}
+
+// Simulation::add_beam (src/fdtd/sources.rs:618-699) with beam_profile (:1043-1108) and
+// PlaneSpectrum (:1110-1350). Complex sqrt, asin and abs come from the C++ library (the
+// reference's are its portable ones); every other operation is the reference's.
+extern "C" int pxfdtd_simulation_add_beam(pxfdtd_simulation *s, const pxfdtd_gaussian_beam *beam,
+                                          const pxfdtd_waveform *w) {
+    using cd = std::complex<double>;
+    TRY(check_waveform(w));
+    if (!beam) return px_fail(PXFDTD_ERR_INVALID, "a beam is required");
+    const int axis = beam->axis, tilt = beam->tilt;
+    if (axis < 0 || axis > 2 || tilt < 0 || tilt > 2 || beam->direction < 0 || beam->direction > 1 ||
+        beam->polarization < 0 || beam->polarization > 1)
+        return px_fail(PXFDTD_ERR_INVALID, "a beam's axis, tilt, direction or polarization is out of range");
+    auto ok = [](double v) { return std::isfinite(v) && v > 0.0; };
+    if (!ok(beam->waist) || !ok(beam->eps) || !std::isfinite(beam->focus))
+        return px_fail(PXFDTD_ERR_INVALID, "a beam needs a positive waist and permittivity and a finite focus");
+    bool varies[3];
+    for (int a = 0; a < 3; a++) varies[a] = !(s->n[a] == 1 && s->b.periodic[a]);
+    if (tilt == axis || std::isnan(beam->angle) || std::fabs(beam->angle) >= M_PI / 2 ||
+        (beam->angle != 0.0 && !varies[tilt]))
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "a beam tilts towards an axis across its plane along which the field varies, by less than 90 degrees");
+    if (s->b.periodic[axis])
+        return px_fail(PXFDTD_ERR_INVALID, "a beam normal to axis %d needs ends along it, not periodic ones", axis);
+    const long long n = s->n[axis], p = beam->plane;
+    const long long low = s->b.low[axis], high = s->b.high[axis];
+    if (p < low + 1 || p + 2 + high > n)
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the beam's plane %lld and the next must be clear of the CPMLs (%lld and %lld cells) and "
+                       "the ends of %lld cells along axis %d", p, low, high, n, axis);
+    double k0 = 0.0;
+    TRY(carrier_wavenumber(s, w, nullptr, "a beam", &k0));
+    const int ab = (axis + 1) % 3, ac = (axis + 2) % 3;
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    const double origin[3] = {s->g.x0, s->g.y0, s->g.z0};
+    long long wlo[2], whi[2];
+    for (int q = 0; q < 2; q++) {
+        const int t = q == 0 ? ab : ac;
+        wlo[q] = s->b.periodic[t] ? 0 : s->b.low[t];
+        whi[q] = s->b.periodic[t] ? s->n[t] : s->n[t] - s->b.high[t];
     }
-    s->inj_dirty = true;
+    const bool forward = beam->direction == 0;
+    auto node = [&](int a, long long m) { return origin[a] + (double)m * steps[a]; };
+
+    // beam_profile: E of the paraxial beam at a point
+    const double sign = forward ? 1.0 : -1.0;
+    double sa = 0.0, ca = 0.0;
+    pxfdtd::psincos(beam->angle, sa, ca);
+    double u[3] = {0.0, 0.0, 0.0};
+    u[axis] = sign * ca;
+    u[tilt] = sa;
+    double crossing[3] = {0.0, 0.0, 0.0};
+    crossing[axis] = node(axis, p);
+    crossing[ab] = beam->centre[0];
+    crossing[ac] = beam->centre[1];
+    double waist[3];
+    for (int q = 0; q < 3; q++) waist[q] = crossing[q] + beam->focus * u[q];
+    const int s_axis = 3 - axis - tilt;
+    double sv[3] = {0.0, 0.0, 0.0};
+    sv[s_axis] = 1.0;
+    double pv[3] = {sv[1] * u[2] - sv[2] * u[1], sv[2] * u[0] - sv[0] * u[2], sv[0] * u[1] - sv[1] * u[0]};
+    if (pv[tilt] < 0.0 || (pv[tilt] == 0.0 && pv[axis] < 0.0))
+        for (auto &v : pv) v = -v;
+    const double *ev = beam->polarization == 0 ? sv : pv;
+    const double kk = std::sqrt(beam->eps) * k0;
+    const double rayleigh = kk * beam->waist * beam->waist / 2.0;
+    const int transverse = (int)varies[ab] + (int)varies[ac];
+    auto profile = [&](const double r[3], int comp) -> C64 {
+        double rho[3] = {0.0, 0.0, 0.0};
+        for (int q = 0; q < 3; q++)
+            if (varies[q]) rho[q] = r[q] - waist[q];
+        double z = 0.0;
+        for (int q = 0; q < 3; q++) z += rho[q] * u[q];
+        double r2 = 0.0;
+        for (int q = 0; q < 3; q++) r2 += rho[q] * rho[q];
+        r2 -= z * z;
+        const C64 qq = {z, -rayleigh};
+        const C64 ratio = cdiv({0.0, -rayleigh}, qq);
+        C64 envelope = ratio;
+        if (transverse < 2) {
+            const cd root = std::sqrt(cd(ratio.re, ratio.im));
+            envelope = {root.real(), root.imag()};
+        }
+        // i (k r2 / (2 q) + k z)
+        const C64 t1 = cdiv({kk * r2, 0.0}, {2.0 * qq.re, 2.0 * qq.im});
+        const C64 inner = {t1.re + kk * z, t1.im};
+        const C64 phase = cmul({0.0, 1.0}, inner);
+        const C64 value = cmul(envelope, pxfdtd::pcexp(phase));
+        return {value.re * ev[comp], value.im * ev[comp]};
+    };
+
+    // PlaneSpectrum::new
+    const long long nw[2] = {whi[0] - wlo[0], whi[1] - wlo[1]};
+    const double st[2] = {steps[ab], steps[ac]};
+    std::vector<double> kt[2];
+    for (int q = 0; q < 2; q++) {
+        const long long lowm = -((nw[q] - 1) / 2);
+        for (long long m = 0; m < nw[q]; m++) kt[q].push_back(TAU * (double)(lowm + m) / ((double)nw[q] * st[q]));
+    }
+    auto positions = [&](int field, int comp, std::vector<double> x[2]) {
+        for (int q = 0; q < 2; q++) {
+            const int t = q == 0 ? ab : ac;
+            const bool half = field == PXFDTD_E ? comp == t : comp != t;
+            x[q].clear();
+            for (long long m = wlo[q]; m < whi[q]; m++) x[q].push_back(node(t, m) + (half ? 0.5 * steps[t] : 0.0));
+        }
+    };
+    // the separable sum over the other index, from samples to waves (sgn -1) or back (+1)
+    auto transform = [&](const std::vector<C64> &values, const std::vector<double> x[2], double sgn) {
+        const bool to_waves = sgn < 0.0;
+        std::vector<C64> tab[2];
+        for (int q = 0; q < 2; q++) {
+            const long long m = nw[q];
+            tab[q].resize((size_t)(m * m));
+            for (long long a = 0; a < m; a++)
+                for (long long b = 0; b < m; b++) {
+                    const long long wave = to_waves ? a : b, sample = to_waves ? b : a;
+                    tab[q][a * m + b] = pxfdtd::pcexp({0.0, sgn * kt[q][wave] * x[q][sample]});
+                }
+        }
+        const long long n0 = nw[0], n1 = nw[1];
+        std::vector<C64> along(values.size()), out(values.size());
+        for (long long v = 0; v < n1; v++)
+            for (long long pp = 0; pp < n0; pp++) {
+                C64 sum = {0.0, 0.0};
+                for (long long q = 0; q < n0; q++) {
+                    const C64 t = cmul(values[v * n0 + q], tab[0][pp * n0 + q]);
+                    sum.re += t.re;
+                    sum.im += t.im;
+                }
+                along[v * n0 + pp] = sum;
+            }
+        for (long long pp = 0; pp < n1; pp++)
+            for (long long uu = 0; uu < n0; uu++) {
+                C64 sum = {0.0, 0.0};
+                for (long long q = 0; q < n1; q++) {
+                    const C64 t = cmul(along[q * n0 + uu], tab[1][pp * n1 + q]);
+                    sum.re += t.re;
+                    sum.im += t.im;
+                }
+                out[pp * n0 + uu] = sum;
+            }
+        return out;
+    };
+    const size_t nwaves = (size_t)(nw[0] * nw[1]);
+    std::vector<C64> coefficients[2];
+    for (int q = 0; q < 2; q++) {
+        const int comp = q == 0 ? ab : ac;
+        std::vector<double> x[2];
+        positions(PXFDTD_E, comp, x);
+        std::vector<C64> samples(nwaves);
+        for (long long v = 0; v < nw[1]; v++)
+            for (long long uu = 0; uu < nw[0]; uu++) {
+                long long at[3];
+                at[axis] = p;
+                at[ab] = wlo[0] + uu;
+                at[ac] = wlo[1] + v;
+                double pos[3] = {node(0, at[0]), node(1, at[1]), node(2, at[2])};
+                pos[comp] += 0.5 * steps[comp];
+                samples[v * nw[0] + uu] = profile(pos, comp);
+            }
+        coefficients[q] = transform(samples, x, -1.0);
+        const double scale = 1.0 / (double)(nw[0] * nw[1]);
+        for (auto &c : coefficients[q]) c = {c.re * scale, c.im * scale};
+    }
+    const double step_a = steps[axis];
+    std::vector<C64> ka(nwaves);
+    std::vector<std::array<C64, 3>> amplitude(nwaves), big_k(nwaves);
+    for (long long mc = 0; mc < nw[1]; mc++)
+        for (long long mb = 0; mb < nw[0]; mb++) {
+            double s0, c0, s1, c1;
+            pxfdtd::psincos(kt[0][mb] * st[0] / 2.0, s0, c0);
+            pxfdtd::psincos(kt[1][mc] * st[1] / 2.0, s1, c1);
+            const double kb = 2.0 / st[0] * s0, kc = 2.0 / st[1] * s1;
+            const double square = beam->eps * k0 * k0 - kb * kb - kc * kc;
+            C64 normal = square >= 0.0 ? C64{std::sqrt(square), 0.0} : C64{0.0, std::sqrt(-square)};
+            normal = {normal.re * sign, normal.im * sign};
+            const cd as = std::asin(cd(normal.re * step_a / 2.0, normal.im * step_a / 2.0));
+            const C64 k_normal = {2.0 / step_a * as.real(), 2.0 / step_a * as.imag()};
+            const size_t wv = (size_t)(mc * nw[0] + mb);
+            const C64 eb = coefficients[0][wv], ec = coefficients[1][wv];
+            std::array<C64, 3> vec = {C64{0.0, 0.0}, C64{0.0, 0.0}, C64{0.0, 0.0}};
+            std::array<C64, 3> kvec = {C64{0.0, 0.0}, C64{0.0, 0.0}, C64{0.0, 0.0}};
+            kvec[axis] = normal;
+            kvec[ab] = {kb, 0.0};
+            kvec[ac] = {kc, 0.0};
+            if (std::abs(cd(normal.re, normal.im)) > 1e-12 * k0) {
+                vec[ab] = eb;
+                vec[ac] = ec;
+                const C64 num = {kb * eb.re + kc * ec.re, kb * eb.im + kc * ec.im};
+                vec[axis] = cdiv({-num.re, -num.im}, normal);
+            }
+            ka[wv] = k_normal;
+            amplitude[wv] = vec;
+            big_k[wv] = kvec;
+        }
+
+    // the couplings of the planes around the beam's over the window
+    auto total = [&](int field, int comp, const long long at[3]) {
+        const long long t = twice(field, comp, axis, at[axis]);
+        return forward ? t >= 2 * p : t <= 2 * (p + 1);
+    };
+    long long lo[3], hi[3];
+    lo[axis] = p - 1;
+    hi[axis] = std::min(p + 3, n);
+    lo[ab] = wlo[0];
+    hi[ab] = whi[0];
+    lo[ac] = wlo[1];
+    hi[ac] = whi[1];
+    const std::vector<Link> links = collect_links(s, lo, hi, total);
+    if (!surface_uniform(s, links, beam->eps))
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the medium where the beam enters must be uniform, with its permittivity %g and no conductivity",
+                       beam->eps);
+    // PlaneSpectrum::planes and synthesize: each plane of values the couplings read
+    std::map<std::tuple<int, int, long long>, std::vector<C64>> planes;
+    for (const Link &l : links) {
+        const int ff = l.target == PXFDTD_E ? PXFDTD_H : PXFDTD_E;
+        const auto key = std::make_tuple(ff, l.from_comp, l.from[axis]);
+        if (planes.count(key)) continue;
+        const double offset = (double)(twice(ff, l.from_comp, axis, l.from[axis]) - 2 * p) * step_a / 2.0;
+        std::vector<C64> co(nwaves);
+        for (size_t wv = 0; wv < nwaves; wv++) {
+            C64 value;
+            if (ff == PXFDTD_E) {
+                value = amplitude[wv][l.from_comp];
+            } else {
+                const int x = (l.from_comp + 1) % 3, y = (l.from_comp + 2) % 3;
+                const C64 t1 = cmul(big_k[wv][x], amplitude[wv][y]), t2 = cmul(big_k[wv][y], amplitude[wv][x]);
+                value = {(t1.re - t2.re) / k0, (t1.im - t2.im) / k0};
+            }
+            const C64 ik = cmul({0.0, 1.0}, ka[wv]);
+            co[wv] = cmul(value, pxfdtd::pcexp({ik.re * offset, ik.im * offset}));
+        }
+        std::vector<double> x[2];
+        positions(ff, l.from_comp, x);
+        planes[key] = transform(co, x, 1.0);
+    }
+    auto lookup = [&](int field, int comp, const long long at[3]) -> C64 {
+        const long long uu = at[ab], v = at[ac];
+        if (uu < wlo[0] || uu >= whi[0] || v < wlo[1] || v >= whi[1]) return {0.0, 0.0};
+        auto it = planes.find(std::make_tuple(field, comp, at[axis]));
+        if (it == planes.end()) return {0.0, 0.0};
+        return it->second[(size_t)((v - wlo[1]) * nw[0] + (uu - wlo[0]))];
+    };
+    auto e_of = [&](int comp, const long long at[3]) { return lookup(PXFDTD_E, comp, at); };
+    auto h_of = [&](int comp, const long long at[3]) { return lookup(PXFDTD_H, comp, at); };
+    add_phasor_surface(s, links, e_of, h_of, w);
     return PXFDTD_OK;
 }
 
@@ -875,16 +1157,12 @@ extern "C" int pxfdtd_simulation_add_plane_wave(pxfdtd_simulation *s, const int3
     auto index_of = [&](const long long at[3]) { return (at[2] * s->n[1] + at[1]) * s->n[0] + at[0]; };
     const std::vector<Link> links = collect_links(s, lo, hi, total);
     // every E on the surface in a lossless medium of eps, not dispersive
-    const double expected = s->dt / eps;
-    for (const Link &l : links) {
-        const int c = l.target == PXFDTD_E ? l.comp : l.from_comp;
-        const long long r = l.target == PXFDTD_E ? l.r : index_of(l.from);
-        const bool media = !s->member.empty() && s->member[c * N + r] >= 0;
-        if (!(std::fabs(s->cb[c * N + r] - expected) <= 1e-12 * expected && s->ca[c * N + r] == 1.0 && !media))
-            return px_fail(PXFDTD_ERR_INVALID,
-                           "the medium on the box's surface must be uniform, with the permittivity %g and no "
-                           "conductivity", eps);
-    }
+    if (!surface_uniform(s, links, eps))
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the medium on the box's surface must be uniform, with the permittivity %g and no "
+                       "conductivity", eps);
+    (void)index_of;
+    (void)N;
     // l over the box and a cell around it: its extremes are at the corners
     long long least = 0, most = 0, stride = 0;
     for (int a = 0; a < 3; a++) {

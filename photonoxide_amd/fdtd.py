@@ -230,6 +230,31 @@ class PortMode:
 
 
 @dataclass
+class GaussianBeam:
+    """A Gaussian beam (src/fdtd/sources.rs:104-128): launched from the plane of nodes `plane`
+    normal to `axis`, going `direction` (0 forward, 1 backward), its axis crossing the plane at
+    `centre` (um, along the other two axes), waist w0 (um) at `focus` um along the beam, tilted
+    by `angle` radians towards `tilt`, E perpendicular (0) or parallel (1) to the plane of
+    incidence, in a medium of permittivity `eps` on the plane."""
+    axis: int
+    plane: int
+    waist: float
+    focus: float = 0.0
+    centre: tuple = (0.0, 0.0)
+    direction: int = 0
+    tilt: int = -1
+    angle: float = 0.0
+    polarization: int = 0
+    eps: float = 1.0
+
+    def native(self) -> N.GaussianBeam:
+        tilt = (int(self.axis) + 1) % 3 if self.tilt < 0 else int(self.tilt)
+        return N.GaussianBeam(int(self.axis), int(self.plane), int(self.direction),
+                              (C.c_double * 2)(*self.centre), float(self.waist), float(self.focus), tilt,
+                              float(self.angle), int(self.polarization), float(self.eps))
+
+
+@dataclass
 class Box:
     """An open box lo < p < hi (inf allowed) with a diagonal permittivity (or one value)."""
     lo: tuple
@@ -451,6 +476,11 @@ class Simulation:
                        N.dptr(tv), N.dptr(nv))
         N.check(N.lib.pxfdtd_simulation_add_mode_source(self._h, C.byref(m), int(direction),
                                                         C.byref(waveform.native())))
+
+    def add_beam(self, beam: GaussianBeam, waveform: Waveform):
+        """Simulation::add_beam (src/fdtd/sources.rs:618-699): the beam launched one way from its
+        plane by total-field/scattered-field at the waveform's carrier, as fixed currents."""
+        N.check(N.lib.pxfdtd_simulation_add_beam(self._h, C.byref(beam.native()), C.byref(waveform.native())))
 
     def leapfrog_wavenumber(self, frequency: float) -> float:
         """The wavenumber a source at `frequency` settles to on the grid, (2/dt) sin(w dt/2)
