@@ -252,8 +252,9 @@ def adjoint_workload(gpu, fd_cells=16, steps=6000, checkpoint=64):
     is seeded U(0, 1) then Gaussian-filtered (sigma 2 cells), a pulse at 1.55 um, the objective
     |overlap of the output plane's E_y transform with the input profile|^2, 6000 steps with a
     checkpoint every 64. The f32 gradient (forward with checkpoints, then the reverse-time
-    adjoint with recomputation) is timed; central finite differences of f64 runs at 16 random
-    design cells check it, beside the f64 adjoint's gradient."""
+    adjoint with recomputation) is timed; central finite differences of f64 runs at fd_cells
+    random design cells check it, beside the f64 adjoint's gradient (none with fd_cells 0: the
+    timed part alone, for a profile)."""
     from scipy.ndimage import gaussian_filter
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from adjoint_problem import DesignProblem
@@ -273,15 +274,18 @@ def adjoint_workload(gpu, fd_cells=16, steps=6000, checkpoint=64):
     t3 = time.perf_counter()
     info = adj.info()
     cells = 256 * 192 * 96
-    p64 = DesignProblem(gpu, precision=P.Precision.F64, **kw)
-    sim64, adj64, _ = p64.build(rho)
-    j64 = adj64.forward(st)
-    g64 = p64.chain(adj64.grad_eps(adj64.backward()))
-    del sim64, adj64
-    rng2 = np.random.default_rng(5)
-    picks = [tuple(int(rng2.integers(0, s)) for s in p64.shape) for _ in range(fd_cells)]
+    j64 = None
     d = 1e-3
     rel32, rel64 = [], []
+    picks = []
+    if fd_cells:
+        p64 = DesignProblem(gpu, precision=P.Precision.F64, **kw)
+        sim64, adj64, _ = p64.build(rho)
+        j64 = adj64.forward(st)
+        g64 = p64.chain(adj64.grad_eps(adj64.backward()))
+        del sim64, adj64
+        rng2 = np.random.default_rng(5)
+        picks = [tuple(int(rng2.integers(0, s)) for s in p64.shape) for _ in range(fd_cells)]
     for c in picks:
         hi, lo = rho.copy(), rho.copy()
         hi[c] += d
@@ -298,8 +302,8 @@ def adjoint_workload(gpu, fd_cells=16, steps=6000, checkpoint=64):
         "checkpoints": info["checkpoints"], "checkpoint_gb": round(info["checkpoint_bytes"] / 1e9, 2),
         "recomputed_steps": info["recomputed_steps"],
         "fd_cells": fd_cells, "fd_step": d,
-        "max_rel_err_f32_adjoint_vs_fd_f64": float(max(rel32)),
-        "max_rel_err_f64_adjoint_vs_fd_f64": float(max(rel64)),
+        "max_rel_err_f32_adjoint_vs_fd_f64": float(max(rel32)) if rel32 else None,
+        "max_rel_err_f64_adjoint_vs_fd_f64": float(max(rel64)) if rel64 else None,
         "dtype": "f32", "data": "synthetic",
     }
 
@@ -331,6 +335,8 @@ def main():
                          "256 with Lorentz/Drude media")
     ap.add_argument("--adjoint", action="store_true",
                     help="the adjoint-gradient workload instead (with its finite-difference check)")
+    ap.add_argument("--fd-cells", type=int, default=16,
+                    help="adjoint: design cells checked by finite differences (0: none)")
     ap.add_argument("--direct", action="store_true",
                     help="serving: launch each segment's kernels directly instead of a hipGraph")
     args = ap.parse_args()
@@ -344,7 +350,7 @@ def main():
         return
     if args.adjoint:
         import photonoxide_amd as P
-        print(json.dumps(adjoint_workload(P.Gpu(0))), flush=True)
+        print(json.dumps(adjoint_workload(P.Gpu(0), fd_cells=args.fd_cells)), flush=True)
         return
     if args.serving:
         import photonoxide_amd as P

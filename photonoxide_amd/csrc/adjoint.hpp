@@ -43,8 +43,8 @@ struct AdjArgs {
     const T *coef;  // the forward's factors and CPML b, a
     const T *ca, *cb;  // [3][nval], pitched
     T *eb, *hb;        // Ebar, Hbar [3][nval]
-    T *pb;             // psibar, the forward's psi layout
-    T *q;              // R1a / R2a's q, the same layout
+    const T *pb_in;    // psibar before the step, the forward's psi layout
+    T *pb_out;         // psibar after it (the other of two buffers)
 };
 
 __device__ __forceinline__ int adj_wrap(int m, int n, int per) {
@@ -53,151 +53,214 @@ __device__ __forceinline__ int adj_wrap(int m, int n, int per) {
     return m < 0 ? m + n : m - n;
 }
 
-// the index of q (field F, component c, axis w) at cell (i, j, k), or -1 outside its slab
-template <int F, typename T>
-__device__ __forceinline__ long long adj_qidx(const AdjArgs<T> &A, int c, int w, int i, int j, int k) {
-    const AdjGrid &g = A.g;
-    const int n[3] = {g.nx, g.ny, g.nz};
-    const int m[3] = {i, j, k};
-    if (i < 0 || j < 0 || k < 0) return -1;
-    const int side = side_of(m[w], n[w], g.low[w], g.high[w]);
-    if (side < 0) return -1;
-    const int base = g.psi_off[F][c][w][side];
-    if (base < 0) return -1;
-    const int cnt = side == 1 ? g.high[w] : g.low[w];
-    const int s = m[w] - (side == 1 ? n[w] - g.high[w] : 0);
-    long long q;
-    if (w == 0)
-        q = ((long long)k * g.ny + j) * cnt + s;
-    else if (w == 1)
-        q = ((long long)k * cnt + s) * g.nx + i;
-    else
-        q = ((long long)s * g.ny + j) * g.nx + i;
-    return base + q;
+// One of two launch-wide values by a lane's side. Both are taken into scalar registers first:
+// a plain select of two argument fields compiles into a per-lane load from the argument segment
+// at a selected address, which adds a memory round trip in front of the load it indexes.
+__device__ __forceinline__ int adj_by_side(int side, int low, int high) {
+    const int l = __builtin_amdgcn_readfirstlane(low), h = __builtin_amdgcn_readfirstlane(high);
+    return side == 1 ? h : l;
 }
 
-// R1a and R2a: one thread per psi value of the slabs of field F (table of slabs by offset)
-struct AdjSlab {
-    int field, comp, axis, side, from, count;
-    long long offset, len;
+// Where a point sits among axis w's CPML slabs: the side (-1 outside both, or past a wall), its
+// coordinate within the slab, and its index within one component's slab (the forward's layout:
+// along x [k][j][s], along y [k][s][i], along z [s][j][i]). m is the point, m[w] < 0 past a wall.
+struct AdjPt {
+    int side, s, q;
 };
 
-template <typename T, int F>
-__global__ void adj_psi(AdjArgs<T> A, const AdjSlab *slabs, int nslabs, long long total) {
-    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= total) return;
-    int sl = -1;
-    for (int q = 0; q < nslabs; q++)
-        if (v >= slabs[q].offset && v < slabs[q].offset + slabs[q].len) sl = q;
-    if (sl < 0 || slabs[sl].field != F) return;
-    const AdjSlab S = slabs[sl];
-    const AdjGrid &g = A.g;
-    const long long lv = v - S.offset;
-    // the slab's box: the grid with axis w reduced to count, i fastest
-    const int bn[3] = {S.axis == 0 ? S.count : g.nx, S.axis == 1 ? S.count : g.ny, S.axis == 2 ? S.count : g.nz};
-    int m[3];
-    m[0] = (int)(lv % bn[0]);
-    m[1] = (int)((lv / bn[0]) % bn[1]);
-    m[2] = (int)(lv / ((long long)bn[0] * bn[1]));
-    const int s = m[S.axis];
-    m[S.axis] += S.from;
-    const long long cell = (long long)m[2] * g.plane + (long long)m[1] * g.pitch + m[0];
-    const int c = S.comp, w = S.axis;
-    const int a = (c + 1) % 3;
-    const T sign = w == a ? T(1) : T(-1);
-    const int co = g.cpml_off[w][S.side][F];
-    const T b = A.coef[co + s], aa = A.coef[co + S.count + s];
-    T tot;
-    if (F == 0)
-        tot = A.pb[v] + sign * A.cb[c * g.nval + cell] * A.eb[c * g.nval + cell];
+__device__ __forceinline__ AdjPt adj_pt(const AdjGrid &g, int w, const int m[3]) {
+    AdjPt p;
+    const int n = w == 0 ? g.nx : (w == 1 ? g.ny : g.nz);
+    p.side = m[w] < 0 ? -1 : side_of(m[w], n, g.low[w], g.high[w]);
+    const int cnt = adj_by_side(p.side, g.low[w], g.high[w]);
+    p.s = m[w] - (p.side == 1 ? n - g.high[w] : 0);
+    if (w == 0)
+        p.q = (m[2] * g.ny + m[1]) * cnt + p.s;
+    else if (w == 1)
+        p.q = (m[2] * cnt + p.s) * g.nx + m[0];
     else
-        tot = A.pb[v] - A.dt * sign * A.hb[c * g.nval + cell];
-    A.q[v] = aa * tot / A.step[w];
-    A.pb[v] = b * tot;
+        p.q = (p.s * g.ny + m[1]) * g.nx + m[0];
+    return p;
 }
 
-// R1b: Hbar += the transposes of F4's differences and psi_E's, one thread per cell
-template <typename T>
-__global__ void adj_h(AdjArgs<T> A) {
-    const AdjGrid &g = A.g;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.plane * g.nz) return;
-    const int i = (int)(t % g.pitch), j = (int)((t / g.pitch) % g.ny), k = (int)(t / g.plane);
-    if (i >= g.nx) return;
-    const int m[3] = {i, j, k};
-    const int n[3] = {g.nx, g.ny, g.nz};
-    // g_cw at a cell: cb_c fe_w Ebar_c, zero past the high wall
-    auto gv = [&](int c, int w, int d) -> T {  // at the cell + d e_w (d = 0 or 1)
-        int mm[3] = {m[0], m[1], m[2]};
-        mm[w] = adj_wrap(mm[w] + d, n[w], g.per[w]);
-        if (mm[w] < 0) return T(0);
-        const long long cell = (long long)mm[2] * g.plane + (long long)mm[1] * g.pitch + mm[0];
-        return A.cb[c * g.nval + cell] * A.coef[g.fe_off[w] + mm[w]] * A.eb[c * g.nval + cell];
-    };
-    auto qv = [&](int c, int w, int d) -> T {
-        int mm[3] = {m[0], m[1], m[2]};
-        mm[w] = adj_wrap(mm[w] + d, n[w], g.per[w]);
-        if (mm[w] < 0) return T(0);
-        const long long q = adj_qidx<0>(A, c, w, mm[0], mm[1], mm[2]);
-        return q >= 0 ? A.q[q] : T(0);
-    };
-    const long long own = (long long)k * g.plane + (long long)j * g.pitch + i;
-    // Hbar_o for o = x, y, z: the E components and axes that read H_o (c's others (a, b):
-    // E_c reads H_b along a with +1 and H_a along b with -1)
-    for (int o = 0; o < 3; o++) {
-        T acc = T(0);
-        for (int c = 0; c < 3; c++) {
-            if (c == o) continue;
-            const int a = (c + 1) % 3, b = (c + 2) % 3;
-            // E_c reads H_o along w: o == b -> w = a, sign +1; o == a -> w = b, sign -1
-            const int w = o == b ? a : b;
-            const T s = o == b ? T(1) : T(-1);
-            acc = acc - s * (gv(c, w, 1) - gv(c, w, 0));
-            acc = acc - (qv(c, w, 1) - qv(c, w, 0));
-        }
-        A.hb[o * g.nval + own] += acc;
-    }
+// The psibar index of component c's value along axis w at point p, -1 where it has none.
+template <int F>
+__device__ __forceinline__ int adj_pidx(const AdjGrid &g, int c, int w, const AdjPt &p) {
+    if (p.side < 0) return -1;
+    const int base = adj_by_side(p.side, g.psi_off[F][c][w][0], g.psi_off[F][c][w][1]);
+    return base < 0 ? -1 : base + p.q;
 }
 
-// R2b: Ebar = ca Ebar + the transposes of F2's differences and psi_H's, one thread per cell
+// Axis w's CPML b and a of field F at point p (the table's first entry, unused, outside the
+// slabs, so that the load needs no branch).
+template <int F, typename T>
+__device__ __forceinline__ void adj_ba(const AdjArgs<T> &A, int w, const AdjPt &p, T &b, T &a) {
+    const int co = p.side < 0 ? 0 : adj_by_side(p.side, A.g.cpml_off[w][0][F], A.g.cpml_off[w][1][F]) + p.s;
+    const int cnt = p.side < 0 ? 0 : adj_by_side(p.side, A.g.low[w], A.g.high[w]);
+    b = A.coef[co];
+    a = A.coef[co + cnt];
+}
+
+// The launch shape of adj_h and adj_e: 64 x 4 threads over (i, j), one block row per k.
+constexpr int ADJ_BX = 64, ADJ_BY = 4;
+
+// R1a + R1b: Hbar += the transposes of F4's differences and psi_E's, one thread per cell. The
+// thread updates its own cell's psibar_E values (R1a: psi~ = psibar + s cb Ebar, psibar <- b
+// psi~ into the step's output buffer, q = a psi~ / step_w) and recomputes the q of the cell +
+// e_w from the step's input buffer, so no other thread's output is read. Every load is issued
+// before the arithmetic, with indices that stay in bounds where a term is zero, so that a wave
+// waits on memory once rather than once per branch.
 template <typename T>
-__global__ void adj_e(AdjArgs<T> A) {
+__global__ __launch_bounds__(ADJ_BX *ADJ_BY) void adj_h(AdjArgs<T> A) {
     const AdjGrid &g = A.g;
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= g.plane * g.nz) return;
-    const int i = (int)(t % g.pitch), j = (int)((t / g.pitch) % g.ny), k = (int)(t / g.plane);
-    if (i >= g.nx) return;
+    const int i = blockIdx.x * ADJ_BX + threadIdx.x, j = blockIdx.y * ADJ_BY + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
     const int m[3] = {i, j, k};
     const int n[3] = {g.nx, g.ny, g.nz};
-    // h_cw at a cell: fh_w Hbar_c, zero before the low wall
-    auto hv = [&](int c, int w, int d) -> T {  // at the cell - d e_w
-        int mm[3] = {m[0], m[1], m[2]};
-        mm[w] = adj_wrap(mm[w] - d, n[w], g.per[w]);
-        if (mm[w] < 0) return T(0);
-        const long long cell = (long long)mm[2] * g.plane + (long long)mm[1] * g.pitch + mm[0];
-        return A.coef[g.fh_off[w] + mm[w]] * A.hb[c * g.nval + cell];
-    };
-    auto qv = [&](int c, int w, int d) -> T {
-        int mm[3] = {m[0], m[1], m[2]};
-        mm[w] = adj_wrap(mm[w] - d, n[w], g.per[w]);
-        if (mm[w] < 0) return T(0);
-        const long long q = adj_qidx<1>(A, c, w, mm[0], mm[1], mm[2]);
-        return q >= 0 ? A.q[q] : T(0);
-    };
     const long long own = (long long)k * g.plane + (long long)j * g.pitch + i;
-    for (int o = 0; o < 3; o++) {
-        T acc = A.ca[o * g.nval + own] * A.eb[o * g.nval + own];
-        for (int c = 0; c < 3; c++) {
-            if (c == o) continue;
-            const int a = (c + 1) % 3, b = (c + 2) % 3;
-            // H_c reads E_b along a with +1 and E_a along b with -1
-            const int w = o == b ? a : b;
-            const T s = o == b ? T(1) : T(-1);
-            acc = acc + A.dt * s * (hv(c, w, 0) - hv(c, w, 1));
-            acc = acc - (qv(c, w, 0) - qv(c, w, 1));
-        }
-        A.eb[o * g.nval + own] = acc;
+    const long long stride[3] = {1, g.pitch, g.plane};
+    // the cell + e_w: its coordinate along w (-1 past a high wall, where its loads read the cell
+    // itself and its terms are zero), index and slab positions
+    int up[3], upc[3];
+    long long nb[3];
+    AdjPt p[2][3];
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        up[w] = adj_wrap(m[w] + 1, n[w], g.per[w]);
+        upc[w] = up[w] < 0 ? m[w] : up[w];
+        nb[w] = own + (long long)(upc[w] - m[w]) * stride[w];
+        int mm[3] = {i, j, k};
+        p[0][w] = adj_pt(g, w, mm);
+        mm[w] = up[w];
+        p[1][w] = adj_pt(g, w, mm);
     }
+    // E_c reads H_b along a (t = 0, s = +1) and H_a along b (t = 1, s = -1): w = (c + 1 + t) % 3
+    T cb0[3], eb0[3], cb1[3][2], eb1[3][2], pb[2][3][2], fe[2][3], bq[2][3], aq[2][3];
+    int pi[2][3][2];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        cb0[c] = A.cb[c * g.nval + own];
+        eb0[c] = A.eb[c * g.nval + own];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int w = (c + 1 + t) % 3;
+            cb1[c][t] = A.cb[c * g.nval + nb[w]];
+            eb1[c][t] = A.eb[c * g.nval + nb[w]];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                pi[d][c][t] = adj_pidx<0>(g, c, w, p[d][w]);
+                pb[d][c][t] = A.pb_in[pi[d][c][t] < 0 ? 0 : pi[d][c][t]];
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        fe[0][w] = A.coef[g.fe_off[w] + m[w]];
+        fe[1][w] = A.coef[g.fe_off[w] + upc[w]];
+        adj_ba<0>(A, w, p[0][w], bq[0][w], aq[0][w]);
+        adj_ba<0>(A, w, p[1][w], bq[1][w], aq[1][w]);
+    }
+    // g_cw = cb_c fe_w Ebar_c; Hbar_o gets -s (g_cw(+e_w) - g_cw) - (q_cw(+e_w) - q_cw)
+    T acc[3] = {T(0), T(0), T(0)};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int w = (c + 1 + t) % 3, o = (c + 2 - t) % 3;
+            const T s = t == 0 ? T(1) : T(-1);
+            const T g1 = up[w] >= 0 ? cb1[c][t] * fe[1][w] * eb1[c][t] : T(0);
+            T q1 = T(0);
+            if (pi[1][c][t] >= 0) q1 = aq[1][w] * (pb[1][c][t] + s * cb1[c][t] * eb1[c][t]) / A.step[w];
+            const T g0 = cb0[c] * fe[0][w] * eb0[c];
+            T q0 = T(0);
+            if (pi[0][c][t] >= 0) {
+                const T tot = pb[0][c][t] + s * cb0[c] * eb0[c];
+                A.pb_out[pi[0][c][t]] = bq[0][w] * tot;
+                q0 = aq[0][w] * tot / A.step[w];
+            }
+            acc[o] = acc[o] - s * (g1 - g0);
+            acc[o] = acc[o] - (q1 - q0);
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < 3; o++) A.hb[o * g.nval + own] += acc[o];
+}
+
+// R2a + R2b: Ebar = ca Ebar + the transposes of F2's differences and psi_H's, one thread per
+// cell; psibar_H as psibar_E in adj_h (R2a: psi~ = psibar - dt s Hbar), with the cell - e_w
+// for the neighbour.
+template <typename T>
+__global__ __launch_bounds__(ADJ_BX *ADJ_BY) void adj_e(AdjArgs<T> A) {
+    const AdjGrid &g = A.g;
+    const int i = blockIdx.x * ADJ_BX + threadIdx.x, j = blockIdx.y * ADJ_BY + threadIdx.y, k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+    const int m[3] = {i, j, k};
+    const int n[3] = {g.nx, g.ny, g.nz};
+    const long long own = (long long)k * g.plane + (long long)j * g.pitch + i;
+    const long long stride[3] = {1, g.pitch, g.plane};
+    int down[3], downc[3];
+    long long nb[3];
+    AdjPt p[2][3];
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        down[w] = adj_wrap(m[w] - 1, n[w], g.per[w]);
+        downc[w] = down[w] < 0 ? m[w] : down[w];
+        nb[w] = own + (long long)(downc[w] - m[w]) * stride[w];
+        int mm[3] = {i, j, k};
+        p[0][w] = adj_pt(g, w, mm);
+        mm[w] = down[w];
+        p[1][w] = adj_pt(g, w, mm);
+    }
+    // H_c reads E_b along a (t = 0, s = +1) and E_a along b (t = 1, s = -1)
+    T acc[3], hb0[3], hb1[3][2], pb[2][3][2], fh[2][3], bq[2][3], aq[2][3];
+    int pi[2][3][2];
+#pragma unroll
+    for (int o = 0; o < 3; o++) acc[o] = A.ca[o * g.nval + own] * A.eb[o * g.nval + own];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        hb0[c] = A.hb[c * g.nval + own];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int w = (c + 1 + t) % 3;
+            hb1[c][t] = A.hb[c * g.nval + nb[w]];
+#pragma unroll
+            for (int d = 0; d < 2; d++) {
+                pi[d][c][t] = adj_pidx<1>(g, c, w, p[d][w]);
+                pb[d][c][t] = A.pb_in[pi[d][c][t] < 0 ? 0 : pi[d][c][t]];
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < 3; w++) {
+        fh[0][w] = A.coef[g.fh_off[w] + m[w]];
+        fh[1][w] = A.coef[g.fh_off[w] + downc[w]];
+        adj_ba<1>(A, w, p[0][w], bq[0][w], aq[0][w]);
+        adj_ba<1>(A, w, p[1][w], bq[1][w], aq[1][w]);
+    }
+    // h_cw = fh_w Hbar_c, zero before a low wall; Ebar_o gets dt s (h_cw - h_cw(-e_w)) -
+    // (q_cw - q_cw(-e_w))
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int w = (c + 1 + t) % 3, o = (c + 2 - t) % 3;
+            const T s = t == 0 ? T(1) : T(-1);
+            const T h0 = fh[0][w] * hb0[c];
+            T q0 = T(0);
+            if (pi[0][c][t] >= 0) {
+                const T tot = pb[0][c][t] - A.dt * s * hb0[c];
+                A.pb_out[pi[0][c][t]] = bq[0][w] * tot;
+                q0 = aq[0][w] * tot / A.step[w];
+            }
+            const T h1 = down[w] >= 0 ? fh[1][w] * hb1[c][t] : T(0);
+            T q1 = T(0);
+            if (pi[1][c][t] >= 0) q1 = aq[1][w] * (pb[1][c][t] - A.dt * s * hb1[c][t]) / A.step[w];
+            acc[o] = acc[o] + A.dt * s * (h0 - h1);
+            acc[o] = acc[o] - (q0 - q1);
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < 3; o++) A.eb[o * g.nval + own] = acc[o];
 }
 
 // the objective's derivative at the monitor's values: Ebar += 2 Re(w p), w = conj(S) m

@@ -9,7 +9,8 @@
 // with t = (n + 1) dt, the phase the transform added there. Block by block from the last, the
 // state is restored from its checkpoint and the block's steps recomputed (recording nothing but
 // the term cb multiplies at each design value), then the adjoint steps run backwards through
-// the block: seed, gradient, R1a, R1b, R2a, R2b (adjoint.hpp). dJ/dcb is summed in FP64.
+// the block: seed, gradient, then R1a with R1b in one kernel and R2a with R2b in another
+// (adjoint.hpp), psibar kept in two buffers that swap each step. dJ/dcb is summed in FP64.
 #pragma once
 
 namespace pxfdtd_host {
@@ -37,7 +38,7 @@ struct pxfdtd_adjoint {
     double S_re = 0, S_im = 0;
     std::vector<Buf> ckpt;
     size_t ckpt_bytes = 0;
-    Buf ca, cb, eb, hb, pb, q, slabs, term, grad, midx, mw, didx, dcomp, dcell;
+    Buf ca, cb, eb, hb, pb, term, grad, midx, mw, didx, dcomp, dcell;  // pb: two psibar buffers
     int64_t recomputed = 0;
 };
 
@@ -65,12 +66,15 @@ static AdjArgs<T> adj_args(pxfdtd_adjoint *a) {
     A.cb = a->cb.as<T>();
     A.eb = a->eb.as<T>();
     A.hb = a->hb.as<T>();
-    A.pb = a->pb.as<T>();
-    A.q = a->q.as<T>();
     return A;
 }
 
 static unsigned adj_blocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+// adj_h and adj_e's grid: (i, j) in blocks of ADJ_BX x ADJ_BY, k one block row each
+static dim3 adj_cell_grid(const pxfdtd_resident *r) {
+    return dim3((unsigned)((r->nx + ADJ_BX - 1) / ADJ_BX), (unsigned)((r->ny + ADJ_BY - 1) / ADJ_BY), (unsigned)r->nz);
+}
 
 extern "C" int pxfdtd_adjoint_create(pxfdtd_simulation *sim, const pxfdtd_adjoint_spec *spec,
                                      pxfdtd_adjoint **out) {
@@ -83,6 +87,8 @@ extern "C" int pxfdtd_adjoint_create(pxfdtd_simulation *sim, const pxfdtd_adjoin
     if (pxfdtd_host::sim_has_planes(sim))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the adjoint does not checkpoint plane waves' auxiliary grids");
     if (spec->checkpoint < 1) return px_fail(PXFDTD_ERR_INVALID, "checkpoints every K >= 1 steps");
+    if (r->nz > 65535 || (r->ny + ADJ_BY - 1) / ADJ_BY > 65535)
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the adjoint steps at most 65535 z planes and %d y rows", 65535 * ADJ_BY);
     if (spec->component < 0 || spec->component > 2) return px_fail(PXFDTD_ERR_INVALID, "no E component %d", spec->component);
     if (pxfdtd_host::sim_dft_frequency(sim, spec->monitor, spec->frequency) <= 0)
         return px_fail(PXFDTD_ERR_INVALID, "no frequency %d of transform monitor %d", spec->frequency, spec->monitor);
@@ -222,15 +228,11 @@ static int adj_backward_t(pxfdtd_adjoint *a, double *grad_out) {
     }
     TRY(a->eb.alloc(3 * nval * sizeof(T)));
     TRY(a->hb.alloc(3 * nval * sizeof(T)));
-    TRY(a->pb.alloc(std::max<long long>(r->psi_len, 1) * sizeof(T)));
-    TRY(a->q.alloc(std::max<long long>(r->psi_len, 1) * sizeof(T)));
+    const long long plen = std::max<long long>(r->psi_len, 1);
+    TRY(a->pb.alloc(2 * plen * sizeof(T)));
     HIPCHK(hipMemsetAsync(a->eb.p, 0, a->eb.bytes, st));
     HIPCHK(hipMemsetAsync(a->hb.p, 0, a->hb.bytes, st));
     HIPCHK(hipMemsetAsync(a->pb.p, 0, a->pb.bytes, st));
-    HIPCHK(hipMemsetAsync(a->q.p, 0, a->q.bytes, st));
-    std::vector<AdjSlab> sl;
-    for (const SlabDesc &d : r->slabs) sl.push_back({d.field, d.comp, d.axis, d.side, d.from, d.count, d.offset, d.len});
-    TRY(upload_vec(a->slabs, sl, st));
     // the seed's weights: w = conj(S) m
     std::vector<double> w(2 * M);
     for (long long v = 0; v < M; v++) {
@@ -254,7 +256,8 @@ static int adj_backward_t(pxfdtd_adjoint *a, double *grad_out) {
     const int em = std::max(1, pxfdtd_host::sim_emitters(sim));
     std::vector<double> waves((size_t)em * 2), phases((size_t)slots * 2);
     const int64_t blocks = (a->steps + K - 1) / K;
-    const long long cells = r->plane * r->nz;
+    const dim3 cgrid = adj_cell_grid(r), cblock(ADJ_BX, ADJ_BY);
+    int cur = 0;  // which psibar buffer holds the state
     r->quiet = true;
     int rc = PXFDTD_OK;
     for (int64_t b = blocks - 1; b >= 0 && rc == PXFDTD_OK; b--) {
@@ -283,14 +286,11 @@ static int adj_backward_t(pxfdtd_adjoint *a, double *grad_out) {
             if (D)
                 hipLaunchKernelGGL((adj_grad<T>), dim3(adj_blocks(D)), dim3(256), 0, st, A.eb, a->didx.as<int64_t>(),
                                    a->term.as<T>() + qn * D, D, a->grad.as<double>());
-            if (r->psi_len)
-                hipLaunchKernelGGL((adj_psi<T, 0>), dim3(adj_blocks(r->psi_len)), dim3(256), 0, st, A,
-                                   a->slabs.as<AdjSlab>(), (int)sl.size(), (long long)r->psi_len);
-            hipLaunchKernelGGL((adj_h<T>), dim3(adj_blocks(cells)), dim3(256), 0, st, A);
-            if (r->psi_len)
-                hipLaunchKernelGGL((adj_psi<T, 1>), dim3(adj_blocks(r->psi_len)), dim3(256), 0, st, A,
-                                   a->slabs.as<AdjSlab>(), (int)sl.size(), (long long)r->psi_len);
-            hipLaunchKernelGGL((adj_e<T>), dim3(adj_blocks(cells)), dim3(256), 0, st, A);
+            A.pb_in = a->pb.as<T>() + cur * plen;
+            A.pb_out = a->pb.as<T>() + (cur ^ 1) * plen;
+            hipLaunchKernelGGL((adj_h<T>), cgrid, cblock, 0, st, A);
+            hipLaunchKernelGGL((adj_e<T>), cgrid, cblock, 0, st, A);
+            cur ^= 1;
         }
         const hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = px_fail(PXFDTD_ERR_DEVICE, "the adjoint steps: %s", hipGetErrorString(e));
