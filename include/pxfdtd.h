@@ -338,6 +338,24 @@ typedef struct {
  * wavenumber is not k0 to 1e-9. */
 int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxfdtd_port_mode *mode,
                                       int32_t direction, const pxfdtd_waveform *w);
+/* Simulation::add_mode_monitor (src/fdtd/monitors.rs:569-611): monitors the amplitudes of
+ * `count` modes from the next step on, each by the FP64 transform of E over its plane and the
+ * next at the frequency whose leapfrog wavenumber is its k0 (Simulation::leapfrog_frequency,
+ * :560-567). Refused as the reference refuses: no mode, a plane not this grid's (size or step)
+ * or its next plane off the grid, a k0 beyond the Nyquist frequency. Mode monitors are
+ * numbered on their own, from 0; the number in *number. */
+int pxfdtd_simulation_add_mode_monitor(pxfdtd_simulation *s, const pxfdtd_port_mode *modes,
+                                       int32_t count, int32_t *number);
+/* The modes in mode monitor n, -1 for no monitor n. */
+int32_t pxfdtd_simulation_mode_count(const pxfdtd_simulation *s, int32_t n);
+/* Simulation::mode_frequencies (monitors.rs:653-663): each mode's frequency (c/um) in out. */
+int pxfdtd_simulation_mode_frequencies(const pxfdtd_simulation *s, int32_t n, double *out);
+/* Simulation::mode_amplitudes (monitors.rs:613-640): each mode's forward and backward
+ * amplitudes in the transform, out[4 m ..] = forward re, im, backward re, im: FDFD's Lorentz
+ * form across the mode's plane and the next (src/fdfd/three/ports.rs:253-287, 637-651), with
+ * the PMLs graded as the CPMLs are, in the transform's units (divide by the waveform's
+ * spectrum for FDFD's amplitudes). */
+int pxfdtd_simulation_mode_amplitudes(pxfdtd_simulation *s, int32_t n, double *out);
 /* GaussianBeam (src/fdtd/sources.rs:104-128): the plane's normal axis and node, the direction
  * (0 forward, 1 backward), where the beam's axis crosses the plane along the other two axes
  * (um), the waist w0 (the field's 1/e radius at the focus, um), the focus's distance from the

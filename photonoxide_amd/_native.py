@@ -194,6 +194,10 @@ SIGNATURES = {
     "pxfdtd_simulation_flux_frequencies": (C.c_int32, [vp, C.c_int32]),
     "pxfdtd_simulation_add_plane_wave": (C.c_int, [vp, pi32, pi32, pi64, pd, C.c_double, C.POINTER(Wave), pi32]),
     "pxfdtd_simulation_add_mode_source": (C.c_int, [vp, C.POINTER(PortMode), C.c_int32, C.POINTER(Wave)]),
+    "pxfdtd_simulation_add_mode_monitor": (C.c_int, [vp, C.POINTER(PortMode), C.c_int32, C.POINTER(C.c_int32)]),
+    "pxfdtd_simulation_mode_count": (C.c_int32, [vp, C.c_int32]),
+    "pxfdtd_simulation_mode_frequencies": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_double)]),
+    "pxfdtd_simulation_mode_amplitudes": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_double)]),
     "pxfdtd_simulation_add_beam": (C.c_int, [vp, C.POINTER(GaussianBeam), C.POINTER(Wave)]),
     "pxfdtd_simulation_incident": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pd]),
     "pxfdtd_simulation_add_probe": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pi32]),

@@ -99,6 +99,25 @@ struct FluxRec {
     std::vector<FluxFace> faces;
 };
 
+// A mode monitor (Simulation::add_mode_monitor, src/fdtd/monitors.rs:569-611): per mode, a copy
+// of the mode and the transform monitor of E over its plane and the next, at the frequency
+// whose leapfrog wavenumber is the mode's k0
+struct ModeEntry {
+    int32_t axis, plane, size[2];
+    double step, k0, beta[2];
+    std::vector<double> tangential, normal;  // as pxfdtd_port_mode holds them
+    double frequency;
+    int dft;  // an index into the simulation's transform monitors
+
+    pxfdtd_port_mode view() const {
+        return {axis, plane, {size[0], size[1]}, step, k0, {beta[0], beta[1]}, tangential.data(), normal.data()};
+    }
+};
+
+struct ModeMonRec {
+    std::vector<ModeEntry> modes;
+};
+
 struct Group {
     std::vector<int32_t> comp;
     std::vector<int64_t> r;
@@ -208,6 +227,7 @@ struct pxfdtd_simulation {
     std::vector<DftRec> dfts;
     std::vector<int> shown;  // the caller's transform monitors: their indices in dfts
     std::vector<FluxRec> fluxes;
+    std::vector<ModeMonRec> modemons;
     std::vector<Group> groups;
     std::vector<int32_t> member;  // [3][N] group or -1, when media exist
     // the device
@@ -760,6 +780,31 @@ static int carrier_wavenumber(const pxfdtd_simulation *s, const pxfdtd_waveform 
     return PXFDTD_OK;
 }
 
+// PortMode3d::e and value_at (src/fdfd/three/ports.rs:130-184): the mode's E component `comp`
+// at the 3D index `at`, going forward (or backward) at unit amplitude on its plane: the profile
+// carried along the axis as e^(+-i beta step) per step, the normal component going backward
+// negated and moved half a step (-e_a e^(-i beta step)).
+static C64 mode_value(const pxfdtd_port_mode *mode, int comp, const long long at[3], bool forward) {
+    const int axis = mode->axis, ab = (axis + 1) % 3, ac = (axis + 2) % 3;
+    const long long plane_values = (long long)mode->size[0] * mode->size[1];
+    const long long k = at[ac] * mode->size[0] + at[ab];
+    C64 ev;
+    if (comp == axis) {
+        ev = {mode->normal[2 * k], mode->normal[2 * k + 1]};
+    } else {
+        const long long o = comp == ab ? k : plane_values + k;
+        ev = {mode->tangential[2 * o], mode->tangential[2 * o + 1]};
+    }
+    C64 i_beta = cmul({0.0, 1.0}, {mode->beta[0], mode->beta[1]});
+    i_beta = {i_beta.re * mode->step, i_beta.im * mode->step};
+    const double st = (double)at[axis] - (double)mode->plane;
+    if (forward) return cmul(ev, pxfdtd::pcexp({i_beta.re * st, i_beta.im * st}));
+    const C64 neg = {-i_beta.re, -i_beta.im};
+    const C64 phase = pxfdtd::pcexp({neg.re * st, neg.im * st});
+    if (comp == axis) return cmul(cmul({-ev.re, -ev.im}, pxfdtd::pcexp(neg)), phase);
+    return cmul(ev, phase);
+}
+
 // Simulation::add_mode_source (src/fdtd/sources.rs:550-596): the mode's E from
 // PortMode3d::value_at (src/fdfd/three/ports.rs:165-184), H~ from it by h_of (sources.rs:457-
 // 483), the couplings of the planes around the mode's by links (:321-418), their currents by
@@ -795,24 +840,7 @@ extern "C" int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxf
     TRY(carrier_wavenumber(s, w, &mode->k0, "the mode", &k0));
     const bool forward = direction == 0;
 
-    // PortMode3d::e and value_at
-    auto profile = [&](int comp, long long u, long long v) -> C64 {
-        const long long k = v * mode->size[0] + u;
-        if (comp == axis) return {mode->normal[2 * k], mode->normal[2 * k + 1]};
-        const long long o = comp == ab ? k : plane_values + k;
-        return {mode->tangential[2 * o], mode->tangential[2 * o + 1]};
-    };
-    C64 i_beta = cmul({0.0, 1.0}, {mode->beta[0], mode->beta[1]});
-    i_beta = {i_beta.re * mode->step, i_beta.im * mode->step};
-    auto e_of = [&](int comp, const long long at[3]) -> C64 {
-        const double st = (double)at[axis] - (double)p;
-        const C64 ev = profile(comp, at[ab], at[ac]);
-        if (forward) return cmul(ev, pxfdtd::pcexp({i_beta.re * st, i_beta.im * st}));
-        const C64 neg = {-i_beta.re, -i_beta.im};
-        const C64 phase = pxfdtd::pcexp({neg.re * st, neg.im * st});
-        if (comp == axis) return cmul(cmul({-ev.re, -ev.im}, pxfdtd::pcexp(neg)), phase);
-        return cmul(ev, phase);
-    };
+    auto e_of = [&](int comp, const long long at[3]) -> C64 { return mode_value(mode, comp, at, forward); };
     // h_of: curl E / (i k0), forward differences, zero beyond a wall
     auto h_of = [&](int comp, const long long at[3]) -> C64 {
         const int a = (comp + 1) % 3, b = (comp + 2) % 3;
@@ -2048,6 +2076,318 @@ extern "C" int pxfdtd_simulation_flux(pxfdtd_simulation *s, int32_t n, double *o
 extern "C" int32_t pxfdtd_simulation_flux_frequencies(const pxfdtd_simulation *s, int32_t n) {
     if (n < 0 || n >= (int32_t)s->fluxes.size()) return -1;
     return (int32_t)s->dfts[s->fluxes[n].faces[0].dft].freqs.size();
+}
+
+// ---------------------------------------------------------------- mode monitors
+
+// FDFD's lattice for the grid's boundaries at k0 (Lattice::new, src/fdfd/three.rs:276-308, with
+// the grid's boundaries as fdfd_boundaries gives them, src/fdtd/monitors.rs:642-651): a CPML of
+// low and high cells is a PML of as many (0 cells a wall) graded with the CPML's reflection and
+// order and no real stretch, a periodic axis Bloch with k = 0. Its stretches at the nodes and
+// halfway after them (graded, src/fdfd/mod.rs:219-242, with the C library's log and pow where
+// the reference has its portable ones); shift, fixed, neighbour, curl_e, curl_h and
+// curl_curl_into (three.rs:310-418) and volume (src/fdfd/three/ports.rs:228-240). Values are
+// numbered as Grid3d::index numbers them: component, then k, j, i.
+struct ModeLattice {
+    int n[3];
+    long long cells;
+    double h[3];
+    bool periodic[3];
+    double k0;
+    std::vector<C64> nodes[3], halves[3];
+
+    ModeLattice(const pxfdtd_simulation *s, double k0_) : k0(k0_) {
+        const double origin[3] = {s->g.x0, s->g.y0, s->g.z0};
+        const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+        cells = s->N;
+        for (int a = 0; a < 3; a++) {
+            n[a] = s->n[a];
+            h[a] = steps[a];
+            periodic[a] = s->b.periodic[a] != 0;
+            const int low = periodic[a] ? 0 : s->b.low[a], high = periodic[a] ? 0 : s->b.high[a];
+            const double span0 = origin[a] + 0.0 * h[a], span1 = origin[a] + (double)n[a] * h[a];
+            for (int off = 0; off < 2; off++) {
+                std::vector<C64> &out = off == 0 ? nodes[a] : halves[a];
+                out.resize(n[a]);
+                for (int m = 0; m < n[a]; m++) {
+                    const double pos = origin[a] + (double)m * h[a] + (off == 0 ? 0.0 : 0.5) * h[a];
+                    const C64 st = graded(pos, span0, span1, low, high, h[a], s->b.reflection, s->b.order);
+                    out[m] = {st.re + 0.0 * st.im, st.im};
+                }
+            }
+        }
+    }
+
+    C64 graded(double pos, double span0, double span1, int low, int high, double step, double R,
+               double order) const {
+        auto profile = [&](double depth, double d) -> C64 {
+            const double sigma = (order + 1.0) * (-std::log(R)) / (2.0 * k0 * d);
+            double q = depth / d;
+            q = q < 0.0 ? 0.0 : (q > 1.0 ? 1.0 : q);
+            return {1.0, sigma * std::pow(q, order)};
+        };
+        const double dl = (double)low * step, dh = (double)high * step;
+        if (dl > 0.0 && pos < span0 + dl) return profile(span0 + dl - pos, dl);
+        if (dh > 0.0 && pos > span1 - dh) return profile(pos - (span1 - dh), dh);
+        return {1.0, 0.0};
+    }
+
+    long long index(int c, const int at[3]) const {
+        return (long long)c * cells + ((long long)at[2] * n[1] + at[1]) * n[0] + at[0];
+    }
+    void at(long long r, int &c, int a[3]) const {
+        c = (int)(r / cells);
+        const long long rest = r % cells;
+        a[0] = (int)(rest % n[0]);
+        a[1] = (int)((rest / n[0]) % n[1]);
+        a[2] = (int)(rest / ((long long)n[0] * n[1]));
+    }
+    // m + step along axis a and the phase there; false beyond a wall
+    bool shift(int a, int m, int step, int &to, C64 &phase) const {
+        to = m + step;
+        if (to >= 0 && to < n[a]) {
+            phase = {1.0, 0.0};
+            return true;
+        }
+        if (!periodic[a]) return false;
+        const double turns = to < 0 ? -1.0 : 1.0, period = (double)n[a] * h[a];
+        const double angle = turns * 0.0 * period;
+        phase = {1.0 * std::cos(angle), 1.0 * std::sin(angle)};
+        to = ((to % n[a]) + n[a]) % n[a];
+        return true;
+    }
+    bool fixed(long long r) const {
+        int c, a[3];
+        at(r, c, a);
+        for (int ax = 0; ax < 3; ax++)
+            if (ax != c && a[ax] == 0 && !periodic[ax]) return true;
+        return false;
+    }
+    bool neighbour(int comp, const int a[3], int ax, int step, long long &r, C64 &phase) const {
+        int to;
+        if (!shift(ax, a[ax], step, to, phase)) return false;
+        int m[3] = {a[0], a[1], a[2]};
+        m[ax] = to;
+        r = index(comp, m);
+        return true;
+    }
+    // num-complex's real over complex: a (c - i d) / (c^2 + d^2)
+    static C64 rdiv(double x, C64 b) {
+        const double nn = b.re * b.re + b.im * b.im;
+        return {x * b.re / nn, -x * b.im / nn};
+    }
+    // curl E at H's value r, curl H at E's value r: (value, coefficient) in the reference's order
+    int curl_e(long long r, long long *cols, C64 *ws) const {
+        int a, m[3], k = 0;
+        at(r, a, m);
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        const int alongs[2] = {b, c}, fields[2] = {c, b};
+        const double signs[2] = {1.0, -1.0};
+        for (int t = 0; t < 2; t++) {
+            const int along = alongs[t], field = fields[t];
+            const C64 st = halves[along][m[along]];
+            const C64 w = rdiv(signs[t], {st.re * h[along], st.im * h[along]});
+            long long col;
+            C64 phase;
+            if (neighbour(field, m, along, 1, col, phase)) {
+                cols[k] = col;
+                ws[k++] = cmul(w, phase);
+            }
+            cols[k] = index(field, m);
+            ws[k++] = {-w.re, -w.im};
+        }
+        return k;
+    }
+    int curl_h(long long r, long long *cols, C64 *ws) const {
+        int a, m[3], k = 0;
+        at(r, a, m);
+        const int b = (a + 1) % 3, c = (a + 2) % 3;
+        const int alongs[2] = {b, c}, fields[2] = {c, b};
+        const double signs[2] = {1.0, -1.0};
+        for (int t = 0; t < 2; t++) {
+            const int along = alongs[t], field = fields[t];
+            const C64 st = nodes[along][m[along]];
+            const C64 w = rdiv(signs[t], {st.re * h[along], st.im * h[along]});
+            cols[k] = index(field, m);
+            ws[k++] = w;
+            long long col;
+            C64 phase;
+            if (neighbour(field, m, along, -1, col, phase)) {
+                cols[k] = col;
+                ws[k++] = cmul({-w.re, -w.im}, phase);
+            }
+        }
+        return k;
+    }
+    // row r of -curl curl: up to 16 entries, none for a value fixed at zero
+    int curl_curl(long long r, long long *cols, C64 *ws) const {
+        if (fixed(r)) return 0;
+        long long fc[4], ec[4];
+        C64 fw[4], ew[4];
+        const int nf = curl_h(r, fc, fw);
+        int k = 0;
+        for (int q = 0; q < nf; q++) {
+            const int ne = curl_e(fc[q], ec, ew);
+            for (int t = 0; t < ne; t++) {
+                if (fixed(ec[t])) continue;
+                cols[k] = ec[t];
+                ws[k++] = cmul({-fw[q].re, -fw[q].im}, ew[t]);
+            }
+        }
+        return k;
+    }
+    C64 volume(long long r) const {
+        int c, m[3];
+        at(r, c, m);
+        C64 v = {1.0, 0.0};
+        for (int a = 0; a < 3; a++) v = cmul(v, a == c ? halves[a][m[a]] : nodes[a][m[a]]);
+        return v;
+    }
+    // Lattice::lorentz (src/fdfd/three/ports.rs:253-287): N(u, w) across the cut between node
+    // `plane` and the next along `axis`
+    template <typename U, typename W>
+    C64 lorentz(int axis, int plane, const U &u, const W &w) const {
+        const int b = (axis + 1) % 3, c = (axis + 2) % 3;
+        C64 total = {0.0, 0.0};
+        long long cols[16];
+        C64 ws[16];
+        for (int v = 0; v < n[c]; v++)
+            for (int uu = 0; uu < n[b]; uu++)
+                for (int comp : {b, c, axis}) {
+                    int m[3];
+                    m[axis] = plane;
+                    m[b] = uu;
+                    m[c] = v;
+                    const long long r = index(comp, m);
+                    const C64 ur = u(r), wr = w(r);
+                    const int k = curl_curl(r, cols, ws);
+                    C64 sum = {0.0, 0.0};
+                    for (int q = 0; q < k; q++) {
+                        int to, sat[3];
+                        at(cols[q], to, sat);
+                        if (to != axis && sat[axis] == plane + 1) {
+                            const C64 t1 = cmul(wr, u(cols[q])), t2 = cmul(ur, w(cols[q]));
+                            const C64 d = cmul(ws[q], {t1.re - t2.re, t1.im - t2.im});
+                            sum = {sum.re + d.re, sum.im + d.im};
+                        }
+                    }
+                    const C64 add = cmul(volume(r), sum);
+                    total = {total.re + add.re, total.im + add.im};
+                }
+        const double vol = h[0] * h[1] * h[2];
+        return cdiv({total.re * vol, total.im * vol}, {0.0, 4.0 * k0});
+    }
+};
+
+// Simulation::leapfrog_frequency (src/fdtd/monitors.rs:560-567): the frequency whose leapfrog
+// wavenumber is k0, (2 / dt) asin(k0 dt / 2) / 2 pi (the C library's asin), or false beyond the
+// Nyquist frequency
+static bool leapfrog_frequency(const pxfdtd_simulation *s, double k0, double *f) {
+    const double x = 0.5 * k0 * s->dt;
+    if (!(x >= 0.0 && x < 1.0)) return false;
+    *f = 2.0 / s->dt * std::asin(x) / TAU;
+    return true;
+}
+
+// Simulation::add_mode_monitor (src/fdtd/monitors.rs:580-611)
+extern "C" int pxfdtd_simulation_add_mode_monitor(pxfdtd_simulation *s, const pxfdtd_port_mode *modes,
+                                                  int32_t count, int32_t *number) {
+    if (!modes || count < 1) return px_fail(PXFDTD_ERR_INVALID, "a mode monitor needs at least one mode");
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    ModeMonRec rec;
+    for (int32_t q = 0; q < count; q++) {
+        const pxfdtd_port_mode &m = modes[q];
+        if (m.axis < 0 || m.axis > 2) return px_fail(PXFDTD_ERR_INVALID, "a mode's axis must be 0, 1 or 2");
+        const int axis = m.axis, b = (axis + 1) % 3, c = (axis + 2) % 3;
+        if (m.size[0] != s->n[b] || m.size[1] != s->n[c] || m.step != steps[axis] || m.plane < 0 ||
+            m.plane + 1 >= s->n[axis])
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "a mode's plane and the next must be this grid's: solve it with FDFD on the same grid");
+        if (!m.tangential || !m.normal) return px_fail(PXFDTD_ERR_INVALID, "a mode needs its profile");
+        ModeEntry e;
+        if (!leapfrog_frequency(s, m.k0, &e.frequency))
+            return px_fail(PXFDTD_ERR_INVALID, "the mode's k0 = %.17g rad/um is beyond the time step's Nyquist frequency",
+                           m.k0);
+        e.axis = m.axis;
+        e.plane = m.plane;
+        e.size[0] = m.size[0];
+        e.size[1] = m.size[1];
+        e.step = m.step;
+        e.k0 = m.k0;
+        e.beta[0] = m.beta[0];
+        e.beta[1] = m.beta[1];
+        const size_t pv = (size_t)m.size[0] * m.size[1];
+        e.tangential.assign(m.tangential, m.tangential + 4 * pv);
+        e.normal.assign(m.normal, m.normal + 2 * pv);
+        rec.modes.push_back(std::move(e));
+    }
+    // the checks passed: the transforms (E on the plane and the next, all of the other two axes)
+    for (ModeEntry &e : rec.modes) {
+        int32_t low[3] = {0, 0, 0}, high[3] = {s->n[0] - 1, s->n[1] - 1, s->n[2] - 1};
+        low[e.axis] = e.plane;
+        high[e.axis] = e.plane + 1;
+        TRY(add_transform(s, low, high, 1, &e.frequency, &e.dft));
+    }
+    s->modemons.push_back(std::move(rec));
+    *number = (int32_t)s->modemons.size() - 1;
+    return PXFDTD_OK;
+}
+
+extern "C" int32_t pxfdtd_simulation_mode_count(const pxfdtd_simulation *s, int32_t n) {
+    if (n < 0 || n >= (int32_t)s->modemons.size()) return -1;
+    return (int32_t)s->modemons[n].modes.size();
+}
+
+extern "C" int pxfdtd_simulation_mode_frequencies(const pxfdtd_simulation *s, int32_t n, double *out) {
+    if (n < 0 || n >= (int32_t)s->modemons.size()) return px_fail(PXFDTD_ERR_INVALID, "no mode monitor %d", n);
+    for (size_t q = 0; q < s->modemons[n].modes.size(); q++) out[q] = s->modemons[n].modes[q].frequency;
+    return PXFDTD_OK;
+}
+
+// Simulation::mode_amplitudes (src/fdtd/monitors.rs:623-640) by mode_amplitudes_of and
+// Lattice::mode_amplitudes (src/fdfd/three/ports.rs:637-651, 755-767): forward N(u, g) and
+// backward -N(u, f), u the transforms of E, f and g the mode going forward and backward
+extern "C" int pxfdtd_simulation_mode_amplitudes(pxfdtd_simulation *s, int32_t n, double *out) {
+    if (n < 0 || n >= (int32_t)s->modemons.size()) return px_fail(PXFDTD_ERR_INVALID, "no mode monitor %d", n);
+    const ModeMonRec &R = s->modemons[n];
+    for (size_t q = 0; q < R.modes.size(); q++) {
+        const ModeEntry &e = R.modes[q];
+        const pxfdtd_port_mode mode = e.view();
+        const DftRec &d = s->dfts[e.dft];
+        const size_t len = d.acc.size() / 6;
+        std::vector<double> series[3];
+        for (int c = 0; c < 3; c++) {
+            series[c].resize(len);
+            TRY(transform_sums(s, e.dft, PXFDTD_E * 3 + c, series[c].data()));
+        }
+        const int ex = d.hi[0] - d.lo[0], ey = d.hi[1] - d.lo[1];
+        const ModeLattice L(s, e.k0);
+        auto u = [&](long long r) -> C64 {
+            int c, a[3];
+            L.at(r, c, a);
+            const long long v = ((long long)(a[2] - d.lo[2]) * ey + (a[1] - d.lo[1])) * ex + (a[0] - d.lo[0]);
+            return {series[c][2 * v], series[c][2 * v + 1]};
+        };
+        auto f = [&](long long r) -> C64 {
+            int c, a[3];
+            L.at(r, c, a);
+            const long long at3[3] = {a[0], a[1], a[2]};
+            return mode_value(&mode, c, at3, true);
+        };
+        auto g = [&](long long r) -> C64 {
+            int c, a[3];
+            L.at(r, c, a);
+            const long long at3[3] = {a[0], a[1], a[2]};
+            return mode_value(&mode, c, at3, false);
+        };
+        const C64 forward = L.lorentz(e.axis, e.plane, u, g);
+        const C64 back = L.lorentz(e.axis, e.plane, u, f);
+        out[4 * q + 0] = forward.re;
+        out[4 * q + 1] = forward.im;
+        out[4 * q + 2] = -back.re;
+        out[4 * q + 3] = -back.im;
+    }
+    return PXFDTD_OK;
 }
 
 // ---------------------------------------------------------------- for the serving loop

@@ -229,6 +229,22 @@ class PortMode:
     normal: np.ndarray
 
 
+def _port_mode(mode: PortMode):
+    """The C ABI's pxfdtd_port_mode for `mode`, and the arrays it points into (kept alive by
+    the caller while the struct is in use)."""
+    tan = np.ascontiguousarray(mode.tangential, dtype=np.complex128)
+    nor = np.ascontiguousarray(mode.normal, dtype=np.complex128)
+    if tan.ndim != 3 or tan.shape[0] != 2 or nor.shape != tan.shape[1:]:
+        raise ValueError("a mode's tangential E is [2][size_c][size_b] and its normal E [size_c][size_b]")
+    tv = tan.view(np.float64).ravel()
+    nv = nor.view(np.float64).ravel()
+    beta = complex(mode.beta)
+    m = N.PortMode(int(mode.axis), int(mode.plane), (C.c_int32 * 2)(tan.shape[2], tan.shape[1]),
+                   float(mode.step), float(mode.k0), (C.c_double * 2)(beta.real, beta.imag),
+                   N.dptr(tv), N.dptr(nv))
+    return m, (tv, nv)
+
+
 @dataclass
 class GaussianBeam:
     """A Gaussian beam (src/fdtd/sources.rs:104-128): launched from the plane of nodes `plane`
@@ -465,17 +481,45 @@ class Simulation:
         """Simulation::add_mode_source (src/fdtd/sources.rs:550-596): the mode launched going
         `direction` (0 forward, 1 backward) along its axis by total-field/scattered-field on its
         plane, at the waveform's carrier, as fixed currents."""
-        tan = np.ascontiguousarray(mode.tangential, dtype=np.complex128)
-        nor = np.ascontiguousarray(mode.normal, dtype=np.complex128)
-        if tan.ndim != 3 or tan.shape[0] != 2 or nor.shape != tan.shape[1:]:
-            raise ValueError("a mode's tangential E is [2][size_c][size_b] and its normal E [size_c][size_b]")
-        tv = tan.view(np.float64).ravel()
-        nv = nor.view(np.float64).ravel()
-        m = N.PortMode(int(mode.axis), int(mode.plane), (C.c_int32 * 2)(tan.shape[2], tan.shape[1]),
-                       float(mode.step), float(mode.k0), (C.c_double * 2)(complex(mode.beta).real, complex(mode.beta).imag),
-                       N.dptr(tv), N.dptr(nv))
+        m, keep = _port_mode(mode)
         N.check(N.lib.pxfdtd_simulation_add_mode_source(self._h, C.byref(m), int(direction),
                                                         C.byref(waveform.native())))
+        del keep
+
+    def add_mode_monitor(self, modes) -> int:
+        """Simulation::add_mode_monitor (src/fdtd/monitors.rs:569-611): the amplitudes of
+        `modes` (each solved at the leapfrog wavenumber of the frequency it is wanted at) from
+        the next step on, by the transform of E over each mode's plane and the next. Its number
+        for mode_amplitudes; mode monitors are numbered on their own."""
+        modes = list(modes)
+        if not modes:
+            raise ValueError("a mode monitor needs at least one mode")
+        packed = [_port_mode(m) for m in modes]
+        arr = (N.PortMode * len(packed))(*[p[0] for p in packed])
+        n = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_add_mode_monitor(self._h, arr, len(packed), C.byref(n)))
+        del packed
+        return n.value
+
+    def mode_amplitudes(self, n: int):
+        """Simulation::mode_amplitudes (src/fdtd/monitors.rs:613-640): (forward, backward) of
+        each of monitor n's modes, in the transform's units."""
+        count = N.lib.pxfdtd_simulation_mode_count(self._h, n)
+        if count < 0:
+            raise IndexError(f"no mode monitor {n}")
+        out = np.zeros(4 * count)
+        N.check(N.lib.pxfdtd_simulation_mode_amplitudes(self._h, n, N.dptr(out)))
+        return [(complex(out[4 * q], out[4 * q + 1]), complex(out[4 * q + 2], out[4 * q + 3]))
+                for q in range(count)]
+
+    def mode_frequencies(self, n: int):
+        """Simulation::mode_frequencies (src/fdtd/monitors.rs:653-663): each mode's frequency."""
+        count = N.lib.pxfdtd_simulation_mode_count(self._h, n)
+        if count < 0:
+            raise IndexError(f"no mode monitor {n}")
+        out = np.zeros(count)
+        N.check(N.lib.pxfdtd_simulation_mode_frequencies(self._h, n, N.dptr(out)))
+        return [float(v) for v in out]
 
     def add_beam(self, beam: GaussianBeam, waveform: Waveform):
         """Simulation::add_beam (src/fdtd/sources.rs:618-699): the beam launched one way from its
