@@ -346,6 +346,23 @@ int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxfdtd_port_mo
  * numbered on their own, from 0; the number in *number. */
 int pxfdtd_simulation_add_mode_monitor(pxfdtd_simulation *s, const pxfdtd_port_mode *modes,
                                        int32_t count, int32_t *number);
+/* The permittivity each value of E sees, [3][N] into out: the per-component averages
+ * (Simulation::diagonal, src/fdtd.rs:619-653) or the values given to pxfdtd_simulation_new_eps. */
+int pxfdtd_simulation_permittivity(const pxfdtd_simulation *s, double *out);
+/* Lattice::port_modes (src/fdfd/three/ports.rs:408-594): up to `count` modes of the guide
+ * crossing node plane `plane` normal to `axis`, within window[w] = {lo, hi} (nodes; lo < 0 for
+ * the whole axis) along the w-th of the axes after `axis`, on the simulation's grid and
+ * permittivity at the leapfrog wavenumber of `frequency` (c/um): the eigenvalues nearest
+ * k0^2 max(eps, 1) of the plane's transverse problem (found in *found), each normalized to
+ * N(f, g) = 1 with its largest tangential value's real part positive, propagating modes first
+ * by |Re beta|. Out: beta [count][2], tangential [count][2][size_c][size_b] complex and normal
+ * [count][size_c][size_b] complex as pxfdtd_port_mode holds them, and each mode's power (the
+ * scheme's flux at unit amplitude; 1 for a real mode). The plane and the next must be clear of
+ * the CPMLs as a mode source's; a periodic axis across must be whole in the window. */
+int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, int32_t plane,
+                                 const int32_t window[2][2], double frequency, int32_t count,
+                                 int32_t *found, double *beta, double *tangential, double *normal,
+                                 double *power);
 /* The modes in mode monitor n, -1 for no monitor n. */
 int32_t pxfdtd_simulation_mode_count(const pxfdtd_simulation *s, int32_t n);
 /* Simulation::mode_frequencies (monitors.rs:653-663): each mode's frequency (c/um) in out. */

@@ -22,6 +22,7 @@
 
 #include "../../include/pxfdtd.h"
 #include "portable.hpp"
+#include "eigen.hpp"
 
 using pxfdtd::C64;
 
@@ -217,6 +218,7 @@ struct pxfdtd_simulation {
     double dt;
     long long steps = 0;
     std::vector<double> e, h, ca, cb;
+    std::vector<double> eps;  // [3][N] the permittivity each value of E sees (the port-mode solver's)
     std::vector<double> kn[3], kh[3], fe[3], fh[3];
     std::vector<Slab> slabs;
     std::vector<double> cpml_b[3][2][2], cpml_a[3][2][2];
@@ -374,6 +376,7 @@ int make(const pxfdtd_grid *g, const pxfdtd_boundaries *b, double courant, F eps
     s->h.assign(3 * N, 0.0);
     s->ca.assign(3 * N, 1.0);
     s->cb.assign(3 * N, 0.0);
+    s->eps.assign(3 * N, 0.0);
     int bad = 0;
     for (int c = 0; c < 3; c++) {
         // each value's average alone: the same bits on any number of threads
@@ -381,6 +384,7 @@ int make(const pxfdtd_grid *g, const pxfdtd_boundaries *b, double courant, F eps
         for (long long r = 0; r < N; r++) {
             const double v = eps(c, r);
             if (!(std::isfinite(v) && v > 0)) bad |= 1;
+            s->eps[c * N + r] = v;
             s->cb[c * N + r] = s->dt / v;
         }
     }
@@ -2236,6 +2240,51 @@ struct ModeLattice {
         }
         return k;
     }
+    // a node (numbered as E_x's value there) on a wall behind a PML, where a potential is zero
+    bool node_fixed(long long q) const {
+        int c, a[3];
+        at(q, c, a);
+        for (int ax = 0; ax < 3; ax++)
+            if (a[ax] == 0 && !periodic[ax]) return true;
+        return false;
+    }
+    // the divergence at node q (Lattice::divergence, three.rs:454-466) and the gradient at E's
+    // value r (Lattice::gradient, :490-506): (value or node, coefficient); node_fixed :445-451
+    int divergence(long long q, long long *cols, C64 *ws) const {
+        int c, m[3], k = 0;
+        at(q, c, m);
+        for (int a = 0; a < 3; a++) {
+            const C64 st = nodes[a][m[a]];
+            const C64 w = rdiv(1.0, {st.re * h[a], st.im * h[a]});
+            cols[k] = index(a, m);
+            ws[k++] = w;
+            long long col;
+            C64 phase;
+            if (neighbour(a, m, a, -1, col, phase)) {
+                cols[k] = col;
+                ws[k++] = cmul({-w.re, -w.im}, phase);
+            }
+        }
+        return k;
+    }
+    int gradient(long long r, long long *cols, C64 *ws) const {
+        int a, m[3], k = 0;
+        at(r, a, m);
+        const C64 st = halves[a][m[a]];
+        const C64 w = rdiv(1.0, {st.re * h[a], st.im * h[a]});
+        long long q;
+        C64 phase;
+        if (neighbour(0, m, a, 1, q, phase) && !node_fixed(q)) {
+            cols[k] = q;
+            ws[k++] = cmul(w, phase);
+        }
+        const long long here = index(0, m);
+        if (!node_fixed(here)) {
+            cols[k] = here;
+            ws[k++] = {-w.re, -w.im};
+        }
+        return k;
+    }
     C64 volume(long long r) const {
         int c, m[3];
         at(r, c, m);
@@ -2387,6 +2436,296 @@ extern "C" int pxfdtd_simulation_mode_amplitudes(pxfdtd_simulation *s, int32_t n
         out[4 * q + 2] = -back.re;
         out[4 * q + 3] = -back.im;
     }
+    return PXFDTD_OK;
+}
+
+// the permittivity each value of E sees, [3][N] (Simulation::diagonal's averages, or those given)
+extern "C" int pxfdtd_simulation_permittivity(const pxfdtd_simulation *s, double *out) {
+    std::memcpy(out, s->eps.data(), s->eps.size() * sizeof(double));
+    return PXFDTD_OK;
+}
+
+// Lattice::flux_with and h_with (src/fdfd/three.rs:1138-1171): the scheme's flux through the
+// half-plane after node `plane` of the field whose E value r is e(r), H~ = curl E / (i k0)
+template <typename E>
+static double lattice_flux(const ModeLattice &L, int axis, int plane, const E &e) {
+    const int b = (axis + 1) % 3, c = (axis + 2) % 3;
+    double total = 0.0;
+    const int comps[2] = {b, c}, hs[2] = {c, b};
+    const double signs[2] = {0.5, -0.5};
+    long long cols[4];
+    C64 ws[4];
+    for (int t = 0; t < 2; t++)
+        for (int v = 0; v < L.n[c]; v++)
+            for (int u = 0; u < L.n[b]; u++) {
+                int m[3];
+                m[axis] = plane;
+                m[b] = u;
+                m[c] = v;
+                const C64 low = e(L.index(comps[t], m));
+                m[axis] = plane + 1;
+                const C64 high = e(L.index(comps[t], m));
+                m[axis] = plane;
+                const int k = L.curl_e(L.index(hs[t], m), cols, ws);
+                C64 curl = {0.0, 0.0};
+                for (int q = 0; q < k; q++) {
+                    const C64 p = cmul(ws[q], e(cols[q]));
+                    curl = {curl.re + p.re, curl.im + p.im};
+                }
+                const C64 hv = cdiv(curl, {0.0, L.k0});
+                const C64 mean = {0.5 * (low.re + high.re), 0.5 * (low.im + high.im)};
+                total += signs[t] * (mean.re * hv.re - mean.im * -hv.im);
+            }
+    return total * L.h[b] * L.h[c];
+}
+
+// Lattice::port_modes (src/fdfd/three/ports.rs:408-594) on the simulation's grid and
+// permittivity, at the leapfrog wavenumber of `frequency`. The unknowns are E across the plane
+// inside the window; the rows are k0^2 eps E_t - curl_t curl_t E_t + grad_t(eps_a^-1 div_t(eps
+// E_t)); the eigenvalues nearest k0^2 max(eps, 1) are beta_d^2, beta = (2/h) asin(beta_d h / 2)
+// (complex asin from the C++ library). E_a = i div_t(eps E_t) / (beta_d eps_a) e^(i beta h / 2)
+// on the half-plane after; each mode normalized to N(f, g) = 1 with its largest tangential
+// value's real part positive, going the way its power goes (propagating) or decays; sorted
+// propagating first by |Re beta|, then the others by Im beta. The plane must be clear of the
+// CPMLs as a mode source's (the reference's job solves on planes cut out of the grid there).
+extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, int32_t plane,
+                                            const int32_t window[2][2], double frequency, int32_t count,
+                                            int32_t *found, double *beta_out, double *tangential_out,
+                                            double *normal_out, double *power_out) {
+    *found = 0;
+    if (axis < 0 || axis > 2) return px_fail(PXFDTD_ERR_INVALID, "a port's axis must be 0, 1 or 2");
+    if (count < 1) return px_fail(PXFDTD_ERR_INVALID, "ask for one mode or more");
+    const int b = (axis + 1) % 3, c = (axis + 2) % 3;
+    if (s->b.periodic[axis])
+        return px_fail(PXFDTD_ERR_INVALID, "a port normal to axis %d needs ends along it, not periodic ones", axis);
+    const long long na = s->n[axis], low = s->b.low[axis], high = s->b.high[axis];
+    if (plane < low + 1 || plane + 2 + high > na)
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the port's plane %d and the next must be clear of the CPMLs (%lld and %lld cells) and the "
+                       "ends of %lld cells along axis %d", plane, low, high, na, axis);
+    if (!(frequency > 0) || TAU * frequency * s->dt >= M_PI)
+        return px_fail(PXFDTD_ERR_INVALID, "%g c/um is beyond the time step's Nyquist frequency", frequency);
+    int wlo[2], whi[2];
+    const int others[2] = {b, c};
+    for (int w = 0; w < 2; w++) {
+        const int along = others[w], cells = s->n[along];
+        const bool whole = !window || window[w][0] < 0;
+        wlo[w] = whole ? 0 : window[w][0];
+        whi[w] = whole ? cells : window[w][1];
+        if (s->b.periodic[along]) {
+            if (wlo[w] != 0 || whi[w] != cells)
+                return px_fail(PXFDTD_ERR_INVALID, "a periodic axis %d must be whole in the port's window", along);
+        } else if (whi[w] > cells || wlo[w] < 0 || whi[w] - wlo[w] < 3) {
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "the window along axis %d, cells %d to %d, must have 3 cells or more on the grid's %d",
+                           along, wlo[w], whi[w], cells);
+        }
+    }
+    double sn = 0.0, cs = 0.0;
+    pxfdtd::psincos(TAU * frequency * s->dt / 2.0, sn, cs);
+    const double k0 = 2.0 / s->dt * sn, k2 = k0 * k0;
+    const ModeLattice L(s, k0);
+    const int size[2] = {s->n[b], s->n[c]};
+    const long long per = (long long)size[0] * size[1];
+    // the window's values: tangential on the plane, normal on the half-plane after it
+    auto inside = [&](long long r) {
+        int comp, m[3];
+        L.at(r, comp, m);
+        if (m[axis] != plane) return false;
+        for (int w = 0; w < 2; w++) {
+            const int along = others[w];
+            if (s->b.periodic[along]) continue;
+            const int mm = m[along];
+            const bool half = comp == along;
+            if (half ? !(mm >= wlo[w] && mm < whi[w]) : !(mm > wlo[w] && mm < whi[w])) return false;
+        }
+        return true;
+    };
+    // the unknowns, numbered (v, u, component) so that the matrix's band is about 2 size[0]
+    std::vector<long long> local((size_t)(2 * per), -1), unknowns;
+    for (int v = 0; v < size[1]; v++)
+        for (int u = 0; u < size[0]; u++)
+            for (int t = 0; t < 2; t++) {
+                int m[3];
+                m[axis] = plane;
+                m[b] = u;
+                m[c] = v;
+                const long long r = L.index(others[t], m);
+                if (inside(r) && !L.fixed(r)) {
+                    local[(size_t)(t * per + (long long)v * size[0] + u)] = (long long)unknowns.size();
+                    unknowns.push_back(r);
+                }
+            }
+    if (unknowns.empty()) return px_fail(PXFDTD_ERR_INVALID, "the window holds no field");
+    auto number = [&](long long r) -> long long {
+        int comp, m[3];
+        L.at(r, comp, m);
+        if (comp == axis || m[axis] != plane) return -1;
+        const int t = comp == c ? 1 : 0;
+        return local[(size_t)(t * per + (long long)m[c] * size[0] + m[b])];
+    };
+    const long long N = s->N;
+    auto eps_of = [&](long long r) { return s->eps[(size_t)r]; };  // r = component * N + cell
+    auto normal_index = [&](long long q) {
+        int comp, m[3];
+        L.at(q, comp, m);
+        return L.index(axis, m);
+    };
+    std::vector<std::tuple<long long, long long, pxeig::cd>> entries;
+    double top = 1.0;
+    long long cols[16], cols2[16];
+    C64 ws[16], ws2[16];
+    for (size_t row = 0; row < unknowns.size(); row++) {
+        const long long r = unknowns[row];
+        top = std::max(top, eps_of(r));
+        entries.emplace_back((long long)row, (long long)row, pxeig::cd(k2 * eps_of(r), 0.0));
+        const int nf = L.curl_h(r, cols, ws);
+        for (int q = 0; q < nf; q++) {
+            int fc, fm[3];
+            L.at(cols[q], fc, fm);
+            if (fc != axis) continue;
+            const int ne = L.curl_e(cols[q], cols2, ws2);
+            for (int t = 0; t < ne; t++) {
+                const long long k = number(cols2[t]);
+                if (k < 0) continue;
+                const C64 v = cmul({-ws[q].re, -ws[q].im}, ws2[t]);
+                entries.emplace_back((long long)row, k, pxeig::cd(v.re, v.im));
+            }
+        }
+        const int ng = L.gradient(r, cols, ws);
+        for (int q = 0; q < ng; q++) {
+            const long long nq = normal_index(cols[q]);
+            if (!inside(nq)) continue;
+            const double inverse = 1.0 / eps_of(nq);
+            const int nd = L.divergence(cols[q], cols2, ws2);
+            for (int t = 0; t < nd; t++) {
+                const long long k = number(cols2[t]);
+                if (k < 0) continue;
+                const C64 v = cmul(ws[q], ws2[t]);
+                const double f = eps_of(cols2[t]) * inverse;
+                entries.emplace_back((long long)row, k, pxeig::cd(v.re * f, v.im * f));
+            }
+        }
+    }
+    (void)N;
+    std::string why;
+    const std::vector<pxeig::Pair> pairs =
+        pxeig::nearest((long long)unknowns.size(), entries, pxeig::cd(k2 * top, 0.0), count, 1e-10, &why);
+    if (pairs.empty()) return px_fail(PXFDTD_ERR_INVALID, "the port's modes: %s", why.c_str());
+    const double hstep = L.h[axis];
+    struct Built {
+        C64 beta;
+        std::vector<double> tangential, normal;
+        double power;
+    };
+    auto propagates = [](C64 bt) { return std::fabs(bt.im) < std::fabs(bt.re); };
+    auto build = [&](const pxeig::Pair &pair, C64 beta_d) -> Built {
+        Built m;
+        const std::complex<double> bd(beta_d.re, beta_d.im);
+        const std::complex<double> beta = 2.0 / hstep * std::asin(bd * hstep / 2.0);
+        m.beta = {beta.real(), beta.imag()};
+        m.tangential.assign((size_t)(4 * per), 0.0);
+        m.normal.assign((size_t)(2 * per), 0.0);
+        auto at_plane = [&](long long r) -> C64 {
+            const long long k = number(r);
+            if (k < 0) return {0.0, 0.0};
+            return {pair.vector[k].real(), pair.vector[k].imag()};
+        };
+        for (long long t = 0; t < 2 * per; t++) {
+            const long long k = local[(size_t)t];
+            if (k < 0) continue;
+            m.tangential[2 * t] = pair.vector[k].real();
+            m.tangential[2 * t + 1] = pair.vector[k].imag();
+        }
+        const C64 half = pxfdtd::pcexp(cmul({0.0, 0.5}, {m.beta.re * hstep, m.beta.im * hstep}));
+        for (int v = 0; v < size[1]; v++)
+            for (int u = 0; u < size[0]; u++) {
+                int mm[3];
+                mm[axis] = plane;
+                mm[b] = u;
+                mm[c] = v;
+                const long long r = L.index(axis, mm);
+                if (!inside(r) || L.fixed(r)) continue;
+                const int nd = L.divergence(L.index(0, mm), cols, ws);
+                C64 div = {0.0, 0.0};
+                for (int t = 0; t < nd; t++) {
+                    int cc, cm[3];
+                    L.at(cols[t], cc, cm);
+                    if (cc == axis) continue;
+                    const C64 e = at_plane(cols[t]);
+                    const double ep = eps_of(cols[t]);
+                    const C64 p = cmul({ws[t].re * ep, ws[t].im * ep}, e);
+                    div = {div.re + p.re, div.im + p.im};
+                }
+                const C64 val = cmul(cdiv(cmul({0.0, 1.0}, div), {beta_d.re * eps_of(r), beta_d.im * eps_of(r)}), half);
+                const long long o = (long long)v * size[0] + u;
+                m.normal[2 * o] = val.re;
+                m.normal[2 * o + 1] = val.im;
+            }
+        pxfdtd_port_mode view = {axis, plane, {size[0], size[1]}, hstep, k0, {m.beta.re, m.beta.im},
+                                 m.tangential.data(), m.normal.data()};
+        auto fwd = [&](long long r) -> C64 {
+            int cc, a[3];
+            L.at(r, cc, a);
+            const long long at3[3] = {a[0], a[1], a[2]};
+            return mode_value(&view, cc, at3, true);
+        };
+        auto bwd = [&](long long r) -> C64 {
+            int cc, a[3];
+            L.at(r, cc, a);
+            const long long at3[3] = {a[0], a[1], a[2]};
+            return mode_value(&view, cc, at3, false);
+        };
+        const C64 norm = L.lorentz(axis, plane, fwd, bwd);
+        const std::complex<double> sq = std::sqrt(std::complex<double>(norm.re, norm.im));
+        std::complex<double> scale = 1.0 / sq;
+        std::complex<double> peak(1.0, 0.0);
+        double best = -1.0;
+        for (long long t = 0; t < 2 * per; t++) {
+            const std::complex<double> x(m.tangential[2 * t], m.tangential[2 * t + 1]);
+            if (std::abs(x) >= best) {
+                best = std::abs(x);
+                peak = x;
+            }
+        }
+        if ((peak * scale).real() < 0.0) scale = -scale;
+        for (long long t = 0; t < 2 * per; t++) {
+            const std::complex<double> x = std::complex<double>(m.tangential[2 * t], m.tangential[2 * t + 1]) * scale;
+            m.tangential[2 * t] = x.real();
+            m.tangential[2 * t + 1] = x.imag();
+        }
+        for (long long t = 0; t < per; t++) {
+            const std::complex<double> x = std::complex<double>(m.normal[2 * t], m.normal[2 * t + 1]) * scale;
+            m.normal[2 * t] = x.real();
+            m.normal[2 * t + 1] = x.imag();
+        }
+        m.power = lattice_flux(L, axis, plane, fwd);
+        return m;
+    };
+    std::vector<Built> modes;
+    for (const pxeig::Pair &pair : pairs) {
+        const std::complex<double> root = std::sqrt(pair.value);
+        C64 bd = {root.real(), root.imag()};
+        const bool prop = propagates(bd);
+        if ((!prop && bd.im < 0.0) || (prop && bd.re < 0.0)) bd = {-bd.re, -bd.im};
+        Built m = build(pair, bd);
+        if (prop && m.power < 0.0) m = build(pair, {-bd.re, -bd.im});
+        modes.push_back(std::move(m));
+    }
+    std::stable_sort(modes.begin(), modes.end(), [&](const Built &p, const Built &q) {
+        const bool ep = !propagates(p.beta), eq = !propagates(q.beta);
+        if (ep != eq) return !ep;
+        if (std::fabs(p.beta.re) != std::fabs(q.beta.re)) return std::fabs(p.beta.re) > std::fabs(q.beta.re);
+        return p.beta.im < q.beta.im;
+    });
+    for (size_t q = 0; q < modes.size(); q++) {
+        beta_out[2 * q] = modes[q].beta.re;
+        beta_out[2 * q + 1] = modes[q].beta.im;
+        std::memcpy(tangential_out + q * 4 * per, modes[q].tangential.data(), (size_t)(4 * per) * sizeof(double));
+        std::memcpy(normal_out + q * 2 * per, modes[q].normal.data(), (size_t)(2 * per) * sizeof(double));
+        power_out[q] = modes[q].power;
+    }
+    *found = (int32_t)modes.size();
     return PXFDTD_OK;
 }
 

@@ -219,7 +219,8 @@ class PortMode:
     axis and the wavenumber k0 it was solved at, its propagation constant beta (rad/um), E on
     the plane across the axis as `tangential[2][size_c][size_b]` (E_b then E_c, b = (axis + 1)
     % 3, c = (axis + 2) % 3) and E_a on the half-plane after it as `normal[size_c][size_b]`,
-    going forward at unit amplitude."""
+    going forward at unit amplitude; `power`, the scheme's flux at unit amplitude, when a solver
+    gave it."""
     axis: int
     plane: int
     step: float
@@ -227,6 +228,7 @@ class PortMode:
     beta: complex
     tangential: np.ndarray
     normal: np.ndarray
+    power: Optional[float] = None
 
 
 def _port_mode(mode: PortMode):
@@ -485,6 +487,42 @@ class Simulation:
         N.check(N.lib.pxfdtd_simulation_add_mode_source(self._h, C.byref(m), int(direction),
                                                         C.byref(waveform.native())))
         del keep
+
+    def permittivity(self) -> np.ndarray:
+        """The permittivity each value of E sees: [3][nz][ny][nx]."""
+        g = self.grid
+        out = np.zeros(3 * g.nx * g.ny * g.nz)
+        N.check(N.lib.pxfdtd_simulation_permittivity(self._h, N.dptr(out)))
+        return out.reshape(3, g.nz, g.ny, g.nx)
+
+    def port_modes(self, axis: int, plane: int, frequency: float, count: int = 1, window=(None, None)):
+        """Lattice::port_modes (src/fdfd/three/ports.rs:408-594) on this grid and permittivity:
+        up to `count` modes of the guide crossing node plane `plane` normal to `axis`, at the
+        leapfrog wavenumber of `frequency` (the k0 a mode source at that frequency needs),
+        within window[w] = (lo, hi) nodes along the w-th of the axes after `axis` (None: whole);
+        propagating modes first by |Re beta|, each normalized to N(f, g) = 1."""
+        g = self.grid
+        n = (g.nx, g.ny, g.nz)
+        nb, nc = n[(axis + 1) % 3], n[(axis + 2) % 3]
+        per = nb * nc
+        win = (C.c_int32 * 2 * 2)(*[(C.c_int32 * 2)(*(w if w is not None else (-1, -1))) for w in window])
+        beta = np.zeros(2 * count)
+        tan = np.zeros(count * 4 * per)
+        nor = np.zeros(count * 2 * per)
+        power = np.zeros(count)
+        found = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_port_modes(self._h, int(axis), int(plane), C.cast(win, C.c_void_p),
+                                                   float(frequency), int(count), C.byref(found), N.dptr(beta),
+                                                   N.dptr(tan), N.dptr(nor), N.dptr(power)))
+        k0 = self.leapfrog_wavenumber(frequency)
+        step = (g.dx, g.dy, g.dz)[axis]
+        out = []
+        for q in range(found.value):
+            t = tan[q * 4 * per:(q + 1) * 4 * per].view(np.complex128).reshape(2, nc, nb).copy()
+            m = nor[q * 2 * per:(q + 1) * 2 * per].view(np.complex128).reshape(nc, nb).copy()
+            out.append(PortMode(int(axis), int(plane), step, k0, complex(beta[2 * q], beta[2 * q + 1]), t, m,
+                                float(power[q])))
+        return out
 
     def add_mode_monitor(self, modes) -> int:
         """Simulation::add_mode_monitor (src/fdtd/monitors.rs:569-611): the amplitudes of

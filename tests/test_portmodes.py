@@ -1,0 +1,183 @@
+"""The port-mode solver (Lattice::port_modes, src/fdfd/three/ports.rs:408-594) on the host mirror,
+held to the reference's own port tests (src/cases/fdfd/three/port_tests.rs) and its mode-source
+check (src/fdtd/checks.rs:742-826, tests.rs:285-296).
+
+CPU:
+- in a uniform guide periodic across, the fundamental's beta is the grid's own dispersion,
+  (2/h sin(beta h / 2))^2 = eps k0^2, to 1e-13 (E_y and E_z are degenerate there: one start
+  vector finds one mode of the pair);
+- in the reference's walled strip (port_tests.rs:38-70), the mode extended as e^(i beta x)
+  solves every row of the 3D system k0^2 eps E - curl curl E clear of the PMLs to 1e-13 of the
+  rows' scale (port_tests.rs:71-100; the rows from the oracle's restatement of the lattice);
+- a real mode normalized by its Lorentz form carries unit power to 1e-12, its tangential E is
+  real and its normal E imaginary half a step on (port_tests.rs:102-133);
+- the refusals.
+
+GPU: the reference's open guide (40 x 36 x 40 cells of 50 nm, CPMLs of 8, 64 steps a period of
+1.55 um), its fundamental along z launched forward from plane 13 by a continuous wave: the
+backward power behind (plane 10's backward amplitude squared times its power) is below 1e-5 of
+the forward flux through plane 26, and that flux is the mode's power at the source's amplitude
+to 5e-4 (the reference's bounds); the mode monitor's forward amplitude there, over the
+transform's own weight, is 1 to 5e-4 as well.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import modemonitor as MM  # noqa: E402
+
+H = 0.05
+SILICON, OXIDE = 3.48, 1.444
+
+
+def sincos(x):
+    s, c = O.C.c_double(), O.C.c_double()
+    O.lib.or_sincos(x, O.C.byref(s), O.C.byref(c))
+    return s.value, c.value
+
+
+def oracle_mode(m):
+    return dict(axis=m.axis, plane=m.plane, step=m.step, beta=(complex(m.beta).real, complex(m.beta).imag),
+                tangential=[[[(float(v.real), float(v.imag)) for v in row] for row in comp] for comp in m.tangential],
+                normal=[[(float(v.real), float(v.imag)) for v in row] for row in m.normal])
+
+
+def test_a_uniform_guides_mode_has_the_grids_dispersion():
+    import photonoxide_amd as P
+    g = P.Grid3d(40, 6, 4, H, H, H)
+    s = P.Simulation.new(g, P.Scene(1.5 ** 2), P.Boundaries(P.Pml(8, 8), P.Periodic(), P.Periodic()), 0.9)
+    k0 = s.leapfrog_wavenumber(1.0)
+    exact = 2.0 / H * math.asin(H * 1.5 * k0 / 2.0)
+    modes = s.port_modes(0, 20, 1.0, count=1)
+    print([m.beta for m in modes], exact)
+    assert abs(modes[0].beta - exact) < 1e-13 * exact
+
+
+def walled_strip():
+    """port_tests.rs:38-70: a 0.5 x 0.22 um silicon strip along x in oxide, 14 x 16 x 12 cells of
+    50 nm centred across, PMLs of 4 along x, walls across."""
+    import photonoxide_amd as P
+    n = (14, 16, 12)
+    g = P.Grid3d(*n, H, H, H, 0.0, -n[1] * H / 2, -n[2] * H / 2)
+    s = P.Simulation.new(g, P.Scene(OXIDE ** 2, [P.Box((-np.inf, -0.25, -0.11), (np.inf, 0.25, 0.11), SILICON ** 2)]),
+                         P.Boundaries(P.Pml(4, 4), P.Pml(0, 0), P.Pml(0, 0)), 0.9)
+    return s, n
+
+
+def test_a_port_mode_solves_every_row_of_the_3d_system():
+    s, n = walled_strip()
+    mode = s.port_modes(0, 6, 1 / 1.55, count=1)[0]
+    print(f"beta {mode.beta}, n_eff {mode.beta / mode.k0}")
+    eps = s.permittivity().reshape(3, -1)
+    lat = MM.Lattice(n, (H,) * 3, (0.0, -n[1] * H / 2, -n[2] * H / 2), (0, 0, 0), (4, 0, 0), (4, 0, 0), 1e-8, 3.0,
+                     mode.k0)
+    om = oracle_mode(mode)
+    cache = {}
+
+    def field(r):
+        if r not in cache:
+            c, at = lat.at(r)
+            cache[r] = complex(*MM.mode_value(om, c, at, True, O.lib.or_exp, sincos))
+        return cache[r]
+    worst, largest = 0.0, 0.0
+    for c in range(3):
+        for k in range(n[2]):
+            for j in range(n[1]):
+                for i in range(5, 9):
+                    r = lat.index(c, (i, j, k))
+                    if lat.fixed(r):
+                        applied, scale = field(r), abs(field(r))
+                    else:
+                        diag = mode.k0 ** 2 * eps[c][r - c * lat.cells] * field(r)
+                        terms = [diag] + [complex(*a) * field(q) for q, a in lat.curl_curl(r)]
+                        applied, scale = sum(terms), sum(abs(t) for t in terms)
+                    worst = max(worst, abs(applied))
+                    largest = max(largest, scale)
+    print(f"worst row {worst:.3e} of {largest:.3e}")
+    assert worst < 1e-13 * largest
+
+
+def test_a_real_mode_carries_unit_power_and_is_real():
+    s, n = walled_strip()
+    modes = s.port_modes(0, 6, 1 / 1.55, count=3)
+    real = [m for m in modes if abs(m.beta.imag) < 1e-9 * abs(m.beta)]
+    print([(m.beta / m.k0, m.power) for m in modes])
+    assert len(real) >= 2
+    assert all(abs(modes[q].beta.real) >= abs(modes[q + 1].beta.real) for q in range(len(modes) - 1))
+    for m in real:
+        assert abs(m.power - 1.0) < 1e-12, m.power
+        half = np.exp(-0.5j * H * m.beta)
+        largest = max(np.abs(m.tangential).max(), np.abs(m.normal).max())
+        wrong = max(np.abs(m.tangential.imag).max(), np.abs((m.normal * half).real).max())
+        assert wrong < 1e-12 * largest, (wrong, largest)
+
+
+def test_host_mirror_refuses_ports_that_dont_fit():
+    """ports.rs:530-580 (check_port) and the mode source's clearance of the CPMLs."""
+    import photonoxide_amd as P
+    s, n = walled_strip()
+    bad = [dict(axis=0, plane=4), dict(axis=0, plane=9), dict(axis=1, plane=0),
+           dict(axis=0, plane=6, window=((0, 2), None)), dict(axis=0, plane=6, window=((0, 17), None)),
+           dict(axis=0, plane=6, count=0), dict(axis=0, plane=6, frequency=1e3)]
+    for b in bad:
+        kw = dict(frequency=1 / 1.55)
+        kw.update(b)
+        with pytest.raises(Exception):
+            s.port_modes(**kw)
+    g = P.Grid3d(20, 6, 4, H, H, H)
+    p = P.Simulation.new(g, P.Scene(2.0), P.Boundaries(P.Pml(4, 4), P.Periodic(), P.Periodic()), 0.9)
+    with pytest.raises(Exception):
+        p.port_modes(0, 8, 1.0, window=((1, 5), None))  # periodic across: the window must be whole
+    with pytest.raises(Exception):
+        p.port_modes(1, 3, 1.0)  # periodic along
+
+
+# ------------------------------------------------------------------------------ the device
+
+def open_guide():
+    """checks.rs:742-770: eps 12 in a 0.4 x 0.3 um core along z in eps 2.1, 40 x 36 x 40 cells
+    of 50 nm centred on it, CPMLs of 8, 64 steps a period of 1.55 um."""
+    import photonoxide_amd as P
+    n = (40, 36, 40)
+    g = P.Grid3d(*n, H, H, H, -n[0] * H / 2, -n[1] * H / 2, -n[2] * H / 2)
+    courant = 1.55 / 64 * math.sqrt(3.0 / (H * H))
+    s = P.Simulation.new(g, P.Scene(2.1, [P.Box((-0.2, -0.15, -np.inf), (0.2, 0.15, np.inf), 12.0)]),
+                         P.Boundaries.cpml_all(8), courant)
+    return s, n
+
+
+@pytest.mark.gpu
+def test_a_mode_source_launches_its_mode_one_way_with_its_power():
+    import photonoxide_amd as P
+    print("case port mode launched one way", flush=True)
+    s, n = open_guide()
+    f = 1 / 1.55
+    launch = s.port_modes(2, 13, f)[0]
+    behind = s.port_modes(2, 10, f)[0]
+    ahead = s.port_modes(2, 26, f)[0]
+    print(f"n_eff {launch.beta / launch.k0}, power {launch.power}")
+    s.add_mode_source(launch, 0, P.Waveform.continuous(f, ramp=4.0 * 1.55))
+    s.set_device(P.Gpu(0), P.Precision.F64)
+    spp = 64
+    s.run(40 * spp)
+    mon = s.add_mode_monitor([behind, ahead])
+    freq = s.mode_frequencies(mon)[1]
+    flux = s.add_flux(2, 26, [freq])
+    periods = 4
+    s.run(periods * spp)
+    (_, back), (fwd, _) = s.mode_amplitudes(mon)
+    forward = s.flux(flux)[0]
+    weight = 0.5 * periods * spp * s.dt()  # the transform of a unit steady wave over whole periods
+    backward_share = abs(back) ** 2 * behind.power / forward
+    forward_share = forward / (weight ** 2 * ahead.power)
+    print(f"backward {backward_share:.2e} of forward, forward {forward_share:.6f} of the mode's power, "
+          f"forward amplitude {abs(fwd) / weight:.6f}")
+    assert backward_share < 1e-5
+    assert abs(forward_share - 1.0) < 5e-4
+    assert abs(abs(fwd) / weight - 1.0) < 5e-4
