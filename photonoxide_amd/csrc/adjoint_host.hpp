@@ -254,7 +254,7 @@ static int adj_backward_t(pxfdtd_adjoint *a, double *grad_out) {
     const double omega = 2.0 * 3.141592653589793 * pxfdtd_host::sim_dft_frequency(sim, a->monitor, a->frequency);
     const int slots = std::max(1, (int)r->slots);
     const int em = std::max(1, pxfdtd_host::sim_emitters(sim));
-    std::vector<double> waves((size_t)em * 2), phases((size_t)slots * 2);
+    std::vector<double> waves((size_t)em * 2 * K), phases((size_t)slots * 2 * K);
     const int64_t blocks = (a->steps + K - 1) / K;
     const dim3 cgrid = adj_cell_grid(r), cblock(ADJ_BX, ADJ_BY);
     int cur = 0;  // which psibar buffer holds the state
@@ -264,16 +264,17 @@ static int adj_backward_t(pxfdtd_adjoint *a, double *grad_out) {
         const int64_t n0 = b * K, m = std::min<int64_t>(K, a->steps - n0);
         rc = adj_restore<T>(a, (size_t)b);
         if (rc) break;
-        // recompute the block, recording the term cb multiplies at the design values
-        for (int64_t qn = 0; qn < m && rc == PXFDTD_OK; qn++) {
-            pxfdtd_host::sim_tables(sim, n0 + qn, 1, slots, waves.data(), phases.data());
-            rc = pxfdtd_resident_step(r, 1, waves.data(), phases.data());
-            if (rc) break;
-            if (D)
-                hipLaunchKernelGGL((adj_term<T>), dim3(adj_blocks(D)), dim3(256), 0, st, A.g, A.coef,
+        // recompute the block in one call, recording after each step the term cb multiplies at
+        // the design values (from the step's new H)
+        pxfdtd_host::sim_tables(sim, n0, (int)m, slots, waves.data(), phases.data());
+        if (D)
+            r->after_step = [&](hipStream_t on, uint32_t qn) {
+                hipLaunchKernelGGL((adj_term<T>), dim3(adj_blocks(D)), dim3(256), 0, on, A.g, A.coef,
                                    r->field<T>(r->p, 3), a->dcell.as<int64_t>(), a->dcomp.as<int8_t>(), D,
-                                   a->term.as<T>() + qn * D);
-        }
+                                   a->term.as<T>() + (int64_t)qn * D);
+            };
+        rc = pxfdtd_resident_step(r, (uint32_t)m, waves.data(), phases.data());
+        r->after_step = nullptr;
         a->recomputed += m;
         if (rc) break;
         // the adjoint steps backwards through the block

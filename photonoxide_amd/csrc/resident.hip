@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -199,6 +200,9 @@ struct pxfdtd_resident {
     int n_lean = 0, n_x = 0, n_slab = 0;  // the march's units: no slab, x slabs only, others
     bool units_dirty = true;
     bool quiet = false;  // steps record no probe or transform values (the adjoint's recomputation)
+    // launched on the stream after each step's kernels, with the step's number in the call (the
+    // adjoint's record of each recomputed step); none when empty
+    std::function<void(hipStream_t, uint32_t)> after_step;
     // per-launch timing of the fused kernel, when asked for
     bool profile = false;
     std::vector<hipEvent_t> kev;
@@ -1697,6 +1701,10 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
             HIPCHK(hipGetLastError());
         }
         r->p = q;
+        if (r->after_step) {
+            r->after_step(s, n);
+            HIPCHK(hipGetLastError());
+        }
         if (r->ntaps && !r->quiet) {
             const int threads = 256;
             const long long blocks = (r->ntaps + threads - 1) / threads;
