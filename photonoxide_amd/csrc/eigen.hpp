@@ -55,28 +55,51 @@ struct BandLU {
             if (p != i)
                 for (long long c = i; c <= cend; c++) std::swap(at(i, c), at(p, c));
             const cd d = at(i, i);
+            const long long len = cend - i;  // columns i + 1 .. cend
+            const double *pivot_row = reinterpret_cast<const double *>(&at(i, i + 1));
+            // the rows below the pivot are independent: in parallel when there is enough of them
+#pragma omp parallel for schedule(static) if ((last - i) * len > 65536)
             for (long long r = i + 1; r <= last; r++) {
                 const cd m = at(r, i) / d;
                 l[(size_t)i * kl + (r - i - 1)] = m;
                 at(r, i) = 0.0;
                 if (m == cd(0.0, 0.0)) continue;
-                for (long long c = i + 1; c <= cend; c++) at(r, c) -= m * at(i, c);
+                // row r -= m row i, the complex products written out (no library call per value)
+                const double mr = m.real(), mi = m.imag();
+                double *row = reinterpret_cast<double *>(&at(r, i + 1));
+                for (long long c = 0; c < len; c++) {
+                    const double ar = pivot_row[2 * c], ai = pivot_row[2 * c + 1];
+                    row[2 * c] -= mr * ar - mi * ai;
+                    row[2 * c + 1] -= mr * ai + mi * ar;
+                }
             }
         }
         return true;
     }
 
     void solve(std::vector<cd> &b) {
+        double *x = reinterpret_cast<double *>(b.data());
         for (long long i = 0; i < n; i++) {
             if (piv[i] != i) std::swap(b[i], b[piv[i]]);
             const long long last = std::min(n - 1, i + kl);
-            for (long long r = i + 1; r <= last; r++) b[r] -= l[(size_t)i * kl + (r - i - 1)] * b[i];
+            const double br = x[2 * i], bi = x[2 * i + 1];
+            const double *li = reinterpret_cast<const double *>(&l[(size_t)i * kl]);
+            for (long long r = i + 1; r <= last; r++) {
+                const double mr = li[2 * (r - i - 1)], mi = li[2 * (r - i - 1) + 1];
+                x[2 * r] -= mr * br - mi * bi;
+                x[2 * r + 1] -= mr * bi + mi * br;
+            }
         }
         for (long long i = n - 1; i >= 0; i--) {
             const long long cend = std::min(n - 1, i + kl + ku);
-            cd s = b[i];
-            for (long long c = i + 1; c <= cend; c++) s -= at(i, c) * b[c];
-            b[i] = s / at(i, i);
+            double sr = x[2 * i], si = x[2 * i + 1];
+            const double *row = reinterpret_cast<const double *>(&at(i, i));
+            for (long long c = 1; c <= cend - i; c++) {
+                const double ar = row[2 * c], ai = row[2 * c + 1], vr = x[2 * (i + c)], vi = x[2 * (i + c) + 1];
+                sr -= ar * vr - ai * vi;
+                si -= ar * vi + ai * vr;
+            }
+            b[i] = cd(sr, si) / at(i, i);
         }
     }
 };
@@ -192,6 +215,14 @@ inline std::vector<Pair> nearest(long long n, const std::vector<std::tuple<long 
         ku = std::max<long long>(ku, -d);
     }
     for (long long i = 0; i < n; i++) shifted.emplace_back(i, i, -shift);
+    // the band's factorization takes about n kl (kl + ku) complex products and n (2 kl + ku)
+    // values: past about a minute of host work (1e11 products; 5e10 take 32 s on 8 threads) or
+    // 4 GB, refuse rather than grind
+    if ((double)n * kl * (double)(kl + ku) > 1e11 || (double)n * (2.0 * kl + ku + 1) * 16.0 > 4e9) {
+        *why = "the plane's problem is too large for the banded solve (n = " + std::to_string(n) + ", band " +
+               std::to_string(kl) + "): give a window around the guide";
+        return out;
+    }
     BandLU lu;
     if (!lu.factor(n, kl, ku, shifted)) {
         *why = "the shifted matrix is singular";
@@ -218,10 +249,19 @@ inline std::vector<Pair> nearest(long long n, const std::vector<std::tuple<long 
         lu.solve(w);
         for (int pass = 0; pass < 2; pass++)
             for (int i = 0; i <= j; i++) {
-                cd dot = 0.0;
-                for (long long q = 0; q < n; q++) dot += std::conj(V[i][q]) * w[q];
-                H(i, j) += dot;
-                for (long long q = 0; q < n; q++) w[q] -= dot * V[i][q];
+                const double *vi = reinterpret_cast<const double *>(V[i].data());
+                double *wv = reinterpret_cast<double *>(w.data());
+                double dr = 0.0, di = 0.0;  // conj(v) . w
+                for (long long q = 0; q < n; q++) {
+                    dr += vi[2 * q] * wv[2 * q] + vi[2 * q + 1] * wv[2 * q + 1];
+                    di += vi[2 * q] * wv[2 * q + 1] - vi[2 * q + 1] * wv[2 * q];
+                }
+                H(i, j) += cd(dr, di);
+                for (long long q = 0; q < n; q++) {
+                    const double ar = vi[2 * q], ai = vi[2 * q + 1];
+                    wv[2 * q] -= dr * ar - di * ai;
+                    wv[2 * q + 1] -= dr * ai + di * ar;
+                }
             }
         double wn = 0.0;
         for (const cd &x : w) wn += std::norm(x);
