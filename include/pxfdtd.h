@@ -363,6 +363,14 @@ int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, int32_t pla
                                  const int32_t window[2][2], double frequency, int32_t count,
                                  int32_t *found, double *beta, double *tangential, double *normal,
                                  double *power);
+/* harmonic_inversion (src/fdtd/harmonic.rs:62-184): the terms sum_k d_k e^(-i w_k t) of a
+ * signal of n complex samples (re, im pairs) every dt (um/c) whose frequencies lie in [low,
+ * high] (c/um), by Mandelshtam and Taylor's filter diagonalization, sorted by frequency: each
+ * as (frequency, decay rate, amplitude re, amplitude im, error) in out, up to `capacity`, the
+ * number found in *found. Refused: dt not positive, high not above low or at or past the
+ * Nyquist frequency, fewer than 8 samples or one not finite. */
+int pxfdtd_harmonic_inversion(const double *signal, int64_t n, double dt, double low, double high,
+                              int32_t capacity, int32_t *found, double *out);
 /* The modes in mode monitor n, -1 for no monitor n. */
 int32_t pxfdtd_simulation_mode_count(const pxfdtd_simulation *s, int32_t n);
 /* Simulation::mode_frequencies (monitors.rs:653-663): each mode's frequency (c/um) in out. */

@@ -5,7 +5,9 @@ The compute path is hand-written HIP for gfx950 in ``csrc/`` behind a C ABI
 ``fdtd`` API in :mod:`photonoxide_amd.fdtd`.
 """
 from .fdtd import (Axis, Box, Boundaries, Cpml, Current, Cylinder, Dipole, Field, GaussianBeam, Gpu, Grid3d,
-                   Periodic, PlaneWave, Pml, PortMode, Precision, Scene, Simulation, Source, Waveform)
+                   Periodic, PlaneWave, Pml, PortMode, Precision, Resonance, Scene, Simulation, Source, Waveform,
+                   harmonic_inversion)
 
 __all__ = ["Axis", "Box", "Boundaries", "Cpml", "Current", "Cylinder", "Dipole", "Field", "GaussianBeam", "Gpu",
-           "Grid3d", "Periodic", "PlaneWave", "Pml", "PortMode", "Precision", "Scene", "Simulation", "Source", "Waveform"]
+           "Grid3d", "Periodic", "PlaneWave", "Pml", "PortMode", "Precision", "Resonance", "Scene", "Simulation", "Source",
+           "Waveform", "harmonic_inversion"]

@@ -23,6 +23,7 @@
 #include "../../include/pxfdtd.h"
 #include "portable.hpp"
 #include "eigen.hpp"
+#include "harmonic.hpp"
 
 using pxfdtd::C64;
 
@@ -2435,6 +2436,28 @@ extern "C" int pxfdtd_simulation_mode_amplitudes(pxfdtd_simulation *s, int32_t n
         out[4 * q + 1] = forward.im;
         out[4 * q + 2] = -back.re;
         out[4 * q + 3] = -back.im;
+    }
+    return PXFDTD_OK;
+}
+
+// harmonic_inversion (src/fdtd/harmonic.rs:62-184): up to `capacity` resonances into out as
+// (frequency, decay, amplitude re, amplitude im, error), all found in *found
+extern "C" int pxfdtd_harmonic_inversion(const double *signal, int64_t n, double dt, double low, double high,
+                                         int32_t capacity, int32_t *found, double *out) {
+    *found = 0;
+    if (n < 0 || (n > 0 && !signal)) return px_fail(PXFDTD_ERR_INVALID, "harmonic inversion needs a signal");
+    std::vector<std::complex<double>> c((size_t)n);
+    for (int64_t q = 0; q < n; q++) c[(size_t)q] = {signal[2 * q], signal[2 * q + 1]};
+    std::vector<pxharm::Resonance> r;
+    std::string why;
+    if (!pxharm::harmonic_inversion(c, dt, low, high, r, &why)) return px_fail(PXFDTD_ERR_INVALID, "%s", why.c_str());
+    *found = (int32_t)r.size();
+    for (int32_t q = 0; q < std::min<int32_t>(capacity, (int32_t)r.size()); q++) {
+        out[5 * q] = r[q].frequency;
+        out[5 * q + 1] = r[q].decay;
+        out[5 * q + 2] = r[q].amplitude.real();
+        out[5 * q + 3] = r[q].amplitude.imag();
+        out[5 * q + 4] = r[q].error;
     }
     return PXFDTD_OK;
 }

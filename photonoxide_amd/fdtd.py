@@ -231,6 +231,39 @@ class PortMode:
     power: Optional[float] = None
 
 
+@dataclass
+class Resonance:
+    """One term of a signal sum_k d_k e^(-i w_k t) (src/fdtd/harmonic.rs:24-46): Re w / 2 pi
+    (c/um), the decay rate -Im w (per um/c), the amplitude d at the first sample, and the
+    filter diagonalization's error (near round-off for a term the signal holds)."""
+    frequency: float
+    decay: float
+    amplitude: complex
+    error: float
+
+    def q(self) -> float:
+        """Q = w / (2 gamma)."""
+        return math.pi * self.frequency / self.decay
+
+
+def harmonic_inversion(signal, dt: float, low: float, high: float):
+    """harmonic_inversion (src/fdtd/harmonic.rs:62-184): the terms of `signal` (sampled every
+    `dt`) with frequencies in [low, high], by filter diagonalization, sorted by frequency."""
+    c = np.ascontiguousarray(np.asarray(signal, dtype=np.complex128))
+    flat = c.view(np.float64).ravel()
+    capacity = 64
+    while True:
+        out = np.zeros(5 * capacity)
+        found = C.c_int32()
+        N.check(N.lib.pxfdtd_harmonic_inversion(N.dptr(flat), len(c), float(dt), float(low), float(high),
+                                                capacity, C.byref(found), N.dptr(out)))
+        if found.value <= capacity:
+            break
+        capacity = found.value
+    return [Resonance(float(out[5 * q]), float(out[5 * q + 1]), complex(out[5 * q + 2], out[5 * q + 3]),
+                      float(out[5 * q + 4])) for q in range(found.value)]
+
+
 def _port_mode(mode: PortMode):
     """The C ABI's pxfdtd_port_mode for `mode`, and the arrays it points into (kept alive by
     the caller while the struct is in use)."""
