@@ -10,6 +10,13 @@ CPU (synthetic signals, closed forms):
   amplitude A e^(i phi) / 2 to 1e-9, Q = w / 2g to 1e-7 (:97-124);
 - the refusals (:126-132).
 
+GPU: a slab of eps 4, 0.5 um thick, in vacuum along z (one cell across, periodic; 1 um of
+vacuum and a 30-cell CPML each side, Courant 0.5), E_x kicked inside it and recorded until it
+decays below 1e-28 of its peak: its second resonance against the continuum's, w = (2 pi + i ln
+r) / (n L) (1.0 c/um, Q = 2.86), converges at second order (the errors of Re w and of the decay
+rate fall 4 times, to 0.1, from 20 to 10 nm cells) to below 1e-3 and 1e-2
+(monitors_checks.rs:395-484, monitors_tests.rs:38-45).
+
 GPU: a lossy 2D cavity (24 x 18 cells of 50 nm between walls, eps 2, sigma 0.05, E_z kicked
 by a pulse), its probe's 4000 steps after the pulse: at least 4 resonances with error below
 1e-8, each e^(-i w dt) within 1e-11 of a root of the leapfrog's own characteristic equation
@@ -103,3 +110,39 @@ def test_a_lossy_cavitys_resonances_are_the_leapfrogs_own():
     print(f"{len(found)} found, {len(kept)} kept, worst distance to the leapfrog's roots {worst:.2e}")
     assert len(kept) >= 4
     assert worst < 1e-11
+
+
+def slab_resonance(h):
+    """monitors_checks.rs:403-484: the relative errors of Re w and of the decay rate."""
+    import photonoxide_amd as P
+    n_index, thickness, gap, cpml = 2.0, 0.5, 1.0, 30
+    cells = int(round((thickness + 2.0 * gap) / h)) + 2 * cpml
+    g = P.Grid3d(1, 1, cells, h, h, h, 0.0, 0.0, -cells * h / 2)
+    centre = 0.5 * h  # the faces halfway between nodes
+    s = P.Simulation.new(g, P.Scene(1.0, [P.Box((-np.inf, -np.inf, centre - thickness / 2),
+                                                (np.inf, np.inf, centre + thickness / 2), n_index ** 2)]),
+                         P.Boundaries(P.Periodic(), P.Periodic(), P.Pml(cpml, cpml)), 0.5)
+    pulse = P.Waveform.pulse(1.0, 1.0)
+    middle = cells // 2
+    s.add_source(P.Source(P.Field.E, P.Axis.X, (0, 0, middle - int(round(0.1 / h))), pulse))
+    p = s.add_probe(P.Field.E, P.Axis.X, (0, 0, middle + int(round(0.07 / h))))
+    s.set_device(P.Gpu(0), P.Precision.F64)
+    s.run_until(pulse.delay + 6.0 * pulse.width)
+    start = len(s.probe(p))
+    assert s.run_until_decayed([p], 1e-28, 2.0, 200.0), "the slab's field didn't decay"
+    found = P.harmonic_inversion(s.probe(p)[start:].astype(complex), s.dt(), 0.75, 1.25)
+    r = (n_index - 1.0) / (n_index + 1.0)
+    omega = complex(2.0 * np.pi, math.log(r)) / (n_index * thickness)
+    best = min((x for x in found if x.error < 1e-6), key=lambda x: abs(x.frequency - 1.0))
+    return (abs(2 * np.pi * best.frequency - omega.real) / omega.real,
+            abs(best.decay + omega.imag) / -omega.imag)
+
+
+@pytest.mark.gpu
+def test_a_slabs_resonance_and_q_converge_to_the_continuums_at_second_order():
+    print("case slab resonance", flush=True)
+    coarse, fine = slab_resonance(0.02), slab_resonance(0.01)
+    print(f"20 nm: {coarse}, 10 nm: {fine}, ratios {coarse[0] / fine[0]:.3f} {coarse[1] / fine[1]:.3f}")
+    for c, f in zip(coarse, fine):
+        assert abs(c / f - 4.0) < 0.1
+    assert fine[0] < 1e-3 and fine[1] < 1e-2
