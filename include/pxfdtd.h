@@ -371,6 +371,13 @@ int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, int32_t pla
  * Nyquist frequency, fewer than 8 samples or one not finite. */
 int pxfdtd_harmonic_inversion(const double *signal, int64_t n, double dt, double low, double high,
                               int32_t capacity, int32_t *found, double *out);
+/* Lattice::mode_amplitude_weights (src/fdfd/three/ports.rs:293-345): the mode monitor's
+ * amplitude of `mode` going `direction` (0 forward, 1 backward) as weights on E's transform:
+ * out [3][box] complex, the box the mode's plane and the next along its axis and the whole grid
+ * across, [c][k][j][i] box-local; the amplitude is the sum of weight times transform (an
+ * adjoint's objective, with component -1). */
+int pxfdtd_simulation_mode_weights(pxfdtd_simulation *s, const pxfdtd_port_mode *mode, int32_t direction,
+                                   double *out);
 /* The modes in mode monitor n, -1 for no monitor n. */
 int32_t pxfdtd_simulation_mode_count(const pxfdtd_simulation *s, int32_t n);
 /* Simulation::mode_frequencies (monitors.rs:653-663): each mode's frequency (c/um) in out. */
@@ -520,7 +527,8 @@ int pxfdtd_server_get_stats(pxfdtd_server *s, pxfdtd_server_stats *out);
 typedef struct pxfdtd_adjoint pxfdtd_adjoint;
 typedef struct {
     int32_t monitor;          /* the transform monitor (pxfdtd_simulation_add_dft's number) */
-    int32_t component;        /* E's component the objective weighs */
+    int32_t component;        /* E's component the objective weighs; -1 all three, the weights
+                                 then [c][k][j][i] */
     int32_t frequency;        /* the index of the monitor's frequency */
     const double *weights;    /* m over the monitor's box, [k][j][i] complex */
     int64_t ndesign;

@@ -196,6 +196,7 @@ SIGNATURES = {
     "pxfdtd_simulation_add_mode_source": (C.c_int, [vp, C.POINTER(PortMode), C.c_int32, C.POINTER(Wave)]),
     "pxfdtd_simulation_add_mode_monitor": (C.c_int, [vp, C.POINTER(PortMode), C.c_int32, C.POINTER(C.c_int32)]),
     "pxfdtd_simulation_mode_count": (C.c_int32, [vp, C.c_int32]),
+    "pxfdtd_simulation_mode_weights": (C.c_int, [vp, C.POINTER(PortMode), C.c_int32, C.POINTER(C.c_double)]),
     "pxfdtd_harmonic_inversion": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.c_double, C.c_double, C.c_double,
                                             C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pxfdtd_simulation_permittivity": (C.c_int, [vp, C.POINTER(C.c_double)]),

@@ -20,8 +20,9 @@ from .fdtd import Precision, Simulation
 class Adjoint:
     """The adjoint of `sim` (which has a device and has not run).
 
-    monitor, component, frequency: the transform monitor, E's component and the frequency index
-    the objective weighs; weights: m over the monitor's box, complex [k][j][i]; design: (component,
+    monitor, component, frequency: the transform monitor, E's component (-1: all three) and the
+    frequency index the objective weighs; weights: m over the monitor's box, complex [k][j][i]
+    ([c][k][j][i] for all three, as Simulation.mode_weights gives a mode's); design: (component,
     flat index (k ny + j) nx + i) of each design value; checkpoint: steps between checkpoints.
     """
 

@@ -89,7 +89,7 @@ extern "C" int pxfdtd_adjoint_create(pxfdtd_simulation *sim, const pxfdtd_adjoin
     if (spec->checkpoint < 1) return px_fail(PXFDTD_ERR_INVALID, "checkpoints every K >= 1 steps");
     if (r->nz > 65535 || (r->ny + ADJ_BY - 1) / ADJ_BY > 65535)
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the adjoint steps at most 65535 z planes and %d y rows", 65535 * ADJ_BY);
-    if (spec->component < 0 || spec->component > 2) return px_fail(PXFDTD_ERR_INVALID, "no E component %d", spec->component);
+    if (spec->component < -1 || spec->component > 2) return px_fail(PXFDTD_ERR_INVALID, "no E component %d", spec->component);
     if (pxfdtd_host::sim_dft_frequency(sim, spec->monitor, spec->frequency) <= 0)
         return px_fail(PXFDTD_ERR_INVALID, "no frequency %d of transform monitor %d", spec->frequency, spec->monitor);
     HIPCHK(hipSetDevice(r->dev->ordinal));
@@ -103,11 +103,14 @@ extern "C" int pxfdtd_adjoint_create(pxfdtd_simulation *sim, const pxfdtd_adjoin
     int lo[3], hi[3];
     TRY(pxfdtd_host::sim_dft_box(sim, spec->monitor, lo, hi));
     const long long vol = (long long)(hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
-    a->weights.assign(spec->weights, spec->weights + 2 * vol);
-    for (int k = lo[2]; k < hi[2]; k++)
-        for (int j = lo[1]; j < hi[1]; j++)
-            for (int i = lo[0]; i < hi[0]; i++)
-                a->mon_index.push_back(spec->component * r->nval + (long long)k * r->plane + (long long)j * r->pitch + i);
+    // one component's weights [k][j][i], or (component -1) all three's [c][k][j][i]
+    const int c0 = spec->component < 0 ? 0 : spec->component, c1 = spec->component < 0 ? 2 : spec->component;
+    a->weights.assign(spec->weights, spec->weights + 2 * vol * (c1 - c0 + 1));
+    for (int c = c0; c <= c1; c++)
+        for (int k = lo[2]; k < hi[2]; k++)
+            for (int j = lo[1]; j < hi[1]; j++)
+                for (int i = lo[0]; i < hi[0]; i++)
+                    a->mon_index.push_back(c * r->nval + (long long)k * r->plane + (long long)j * r->pitch + i);
     const long long N = r->ncells;
     const double *ca = pxfdtd_host::sim_ca(sim), *cb = pxfdtd_host::sim_cb(sim);
     for (int64_t d = 0; d < spec->ndesign; d++) {
@@ -188,17 +191,21 @@ extern "C" int pxfdtd_adjoint_forward(pxfdtd_adjoint *a, int64_t steps, double *
     TRY(a->r->prec == PXFDTD_F64 ? adj_forward_t<double>(a, steps) : adj_forward_t<float>(a, steps));
     a->steps = steps;
     a->forward_done = true;
-    // S = sum m(r) * transform of E_c at frequency f
-    const int64_t len = pxfdtd_simulation_dft_len(a->sim, a->monitor, a->component);
-    std::vector<double> sums((size_t)len);
-    TRY(pxfdtd_simulation_dft(a->sim, a->monitor, a->component, sums.data()));
-    const long long vol = (long long)a->mon_index.size();
-    const double *s = sums.data() + 2 * (size_t)a->frequency * vol;
+    // S = sum m(r) * transform of E_c at frequency f, over one component or all three
+    const int c0 = a->component < 0 ? 0 : a->component, c1 = a->component < 0 ? 2 : a->component;
+    const long long vol = (long long)a->mon_index.size() / (c1 - c0 + 1);
     double re = 0, im = 0;
-    for (long long v = 0; v < vol; v++) {
-        const double mr = a->weights[2 * v], mi = a->weights[2 * v + 1];
-        re += mr * s[2 * v] - mi * s[2 * v + 1];
-        im += mr * s[2 * v + 1] + mi * s[2 * v];
+    for (int c = c0; c <= c1; c++) {
+        const int64_t len = pxfdtd_simulation_dft_len(a->sim, a->monitor, c);
+        std::vector<double> sums((size_t)len);
+        TRY(pxfdtd_simulation_dft(a->sim, a->monitor, c, sums.data()));
+        const double *s = sums.data() + 2 * (size_t)a->frequency * vol;
+        const double *w = a->weights.data() + 2 * (size_t)(c - c0) * vol;
+        for (long long v = 0; v < vol; v++) {
+            const double mr = w[2 * v], mi = w[2 * v + 1];
+            re += mr * s[2 * v] - mi * s[2 * v + 1];
+            im += mr * s[2 * v + 1] + mi * s[2 * v];
+        }
     }
     a->S_re = re;
     a->S_im = im;

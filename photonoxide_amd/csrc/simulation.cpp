@@ -2462,6 +2462,70 @@ extern "C" int pxfdtd_harmonic_inversion(const double *signal, int64_t n, double
     return PXFDTD_OK;
 }
 
+// Lattice::mode_amplitude_weights and lorentz_weights (src/fdfd/three/ports.rs:293-345): the
+// amplitude of `mode` going `direction` as weights on E's transform over the mode's plane and
+// the next, out [3][box] complex with the box [plane, plane + 1] along the axis and whole
+// across ([c][k][j][i], box-local): the amplitude is sum w_r u_r. Each coupling adds
+// f a w_r to u_s's weight and -f a w_s to u_r's, f = dx dy dz / (4 i k0) V_r.
+extern "C" int pxfdtd_simulation_mode_weights(pxfdtd_simulation *s, const pxfdtd_port_mode *mode, int32_t direction,
+                                              double *out) {
+    if (!mode || mode->axis < 0 || mode->axis > 2 || direction < 0 || direction > 1)
+        return px_fail(PXFDTD_ERR_INVALID, "mode weights need a mode and a direction (0 forward, 1 backward)");
+    const int axis = mode->axis, b = (axis + 1) % 3, c = (axis + 2) % 3;
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    if (mode->size[0] != s->n[b] || mode->size[1] != s->n[c] || mode->step != steps[axis] || mode->plane < 0 ||
+        mode->plane + 1 >= s->n[axis])
+        return px_fail(PXFDTD_ERR_INVALID, "a mode's plane and the next must be this grid's");
+    const ModeLattice L(s, mode->k0);
+    int bl[3] = {0, 0, 0}, bn[3] = {s->n[0], s->n[1], s->n[2]};
+    bl[axis] = mode->plane;
+    bn[axis] = 2;
+    const long long vol = (long long)bn[0] * bn[1] * bn[2];
+    std::fill(out, out + 6 * vol, 0.0);
+    auto slot = [&](long long r) -> double * {
+        int comp, m[3];
+        L.at(r, comp, m);
+        const long long v = ((long long)(m[2] - bl[2]) * bn[1] + (m[1] - bl[1])) * bn[0] + (m[0] - bl[0]);
+        return out + 2 * (comp * vol + v);
+    };
+    const bool forward = direction == 0;
+    auto w = [&](long long r) -> C64 {  // the mode going the other way
+        int comp, m[3];
+        L.at(r, comp, m);
+        const long long at3[3] = {m[0], m[1], m[2]};
+        return mode_value(mode, comp, at3, !forward);
+    };
+    const C64 scale = ModeLattice::rdiv(s->g.dx * s->g.dy * s->g.dz, {0.0, 4.0 * L.k0});
+    const double sign = forward ? 1.0 : -1.0;
+    long long cols[16];
+    C64 ws[16];
+    for (int v = 0; v < L.n[c]; v++)
+        for (int uu = 0; uu < L.n[b]; uu++)
+            for (int comp : {b, c, axis}) {
+                int m[3];
+                m[axis] = mode->plane;
+                m[b] = uu;
+                m[c] = v;
+                const long long r = L.index(comp, m);
+                const C64 wr = w(r);
+                const int k = L.curl_curl(r, cols, ws);
+                const C64 factor = cmul(scale, L.volume(r));
+                for (int q = 0; q < k; q++) {
+                    int to, sat[3];
+                    L.at(cols[q], to, sat);
+                    if (to == axis || sat[axis] != mode->plane + 1) continue;
+                    const C64 fa = cmul(factor, ws[q]);
+                    const C64 add = cmul(fa, wr), sub = cmul(fa, w(cols[q]));
+                    double *os = slot(cols[q]), *orr = slot(r);
+                    os[0] += sign * add.re;
+                    os[1] += sign * add.im;
+                    orr[0] -= sign * sub.re;
+                    orr[1] -= sign * sub.im;
+                }
+            }
+    return PXFDTD_OK;
+}
+
 // the permittivity each value of E sees, [3][N] (Simulation::diagonal's averages, or those given)
 extern "C" int pxfdtd_simulation_permittivity(const pxfdtd_simulation *s, double *out) {
     std::memcpy(out, s->eps.data(), s->eps.size() * sizeof(double));

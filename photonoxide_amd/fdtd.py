@@ -572,6 +572,26 @@ class Simulation:
         del packed
         return n.value
 
+    def mode_box(self, mode: PortMode):
+        """The nodes (low, high) of the box a mode's amplitude reads: its plane and the next along
+        its axis, the whole grid across (for add_dft and mode_weights)."""
+        g = self.grid
+        lo, hi = [0, 0, 0], [g.nx - 1, g.ny - 1, g.nz - 1]
+        lo[mode.axis], hi[mode.axis] = mode.plane, mode.plane + 1
+        return tuple(lo), tuple(hi)
+
+    def mode_weights(self, mode: PortMode, direction: int = 0) -> np.ndarray:
+        """Lattice::mode_amplitude_weights (src/fdfd/three/ports.rs:293-345): the amplitude of
+        `mode` going `direction` as weights on E's transform over mode_box(mode), [3][k][j][i]
+        complex: the amplitude is sum(weights * transform)."""
+        lo, hi = self.mode_box(mode)
+        shape = (3,) + tuple(h - l + 1 for l, h in zip(lo[::-1], hi[::-1]))
+        out = np.zeros(2 * int(np.prod(shape)))
+        m, keep = _port_mode(mode)
+        N.check(N.lib.pxfdtd_simulation_mode_weights(self._h, C.byref(m), int(direction), N.dptr(out)))
+        del keep
+        return out.view(np.complex128).reshape(shape)
+
     def mode_amplitudes(self, n: int):
         """Simulation::mode_amplitudes (src/fdtd/monitors.rs:613-640): (forward, backward) of
         each of monitor n's modes, in the transform's units."""

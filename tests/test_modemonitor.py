@@ -259,3 +259,91 @@ def test_an_eigenmode_reads_back_one_way_with_its_flux():
     assert abs(fwd) > 0
     assert abs(bwd) < 1e-5 * abs(fwd)
     assert abs(abs(fwd) ** 2 / nfg - power) < 1e-10 * power
+
+
+# ----------------------------------------------- a mode's amplitude as weights (an objective)
+
+def test_mode_weights_are_the_forms_coefficients():
+    """Lattice::mode_amplitude_weights (ports.rs:293-345): sum(weights * u) is the oracle
+    restatement's N(u, g) (forward) and -N(u, f) (backward) for a random field u over the mode's
+    two planes and a random complex mode, in a box with CPMLs across (the stretches in play),
+    to round-off (the sums run in another order)."""
+    import photonoxide_amd as P
+    n = (16, 12, 10)
+    g = P.Grid3d(*n, H_STEP, H_STEP, H_STEP)
+    s = P.Simulation.new(g, P.Scene(2.0), P.Boundaries(P.Pml(3, 3), P.Pml(3, 3), P.Pml(2, 2)), 0.9)
+    rng = np.random.default_rng(5)
+    k0 = s.leapfrog_wavenumber(1.3)
+    tan = rng.standard_normal((2, n[2], n[1])) + 1j * rng.standard_normal((2, n[2], n[1]))
+    nor = rng.standard_normal((n[2], n[1])) + 1j * rng.standard_normal((n[2], n[1]))
+    mode = P.PortMode(0, 7, H_STEP, k0, 1.4 * k0 + 0.02j, tan, nor)
+    lo, hi = s.mode_box(mode)
+    u = rng.standard_normal((3, n[2], n[1], 2)) + 1j * rng.standard_normal((3, n[2], n[1], 2))
+
+    def field(c, at):
+        v = u[c, at[2] - lo[2], at[1] - lo[1], at[0] - lo[0]]
+        return (float(v.real), float(v.imag))
+    lat = MM.Lattice(n, (H_STEP,) * 3, (0.0, 0.0, 0.0), (0, 0, 0), (3, 3, 2), (3, 3, 2), 1e-8, 3.0, k0)
+    of, ob = MM.mode_amplitudes(lat, oracle_mode(mode), field, EXP, sincos)
+    for direction, expected in ((0, complex(*of)), (1, complex(*ob))):
+        w = s.mode_weights(mode, direction)
+        got = np.sum(w * u)
+        scale = np.sum(np.abs(w) * np.abs(u))
+        print(f"direction {direction}: weights {got}, form {expected}, difference {abs(got - expected) / scale:.2e} of the terms")
+        assert abs(got - expected) < 1e-14 * scale
+
+
+@pytest.mark.gpu
+def test_the_adjoint_differentiates_a_modes_transmission():
+    """The adjoint with a mode's weights as its objective (component -1): J = |a|^2, a the strip
+    guide's fundamental (this build's port_modes) going forward on the output plane. J is the mode
+    monitor's |a|^2 of a plain run, and dJ/drho agrees with central differences of it in FP64
+    within 1e-3 at random design cells."""
+    import torch
+    from adjoint_problem import DesignProblem
+    from photonoxide_amd.adjoint import Adjoint, objective
+    import photonoxide_amd as P
+    print("case adjoint of a mode's transmission", flush=True)
+
+    class ModeProblem(DesignProblem):
+        def mode(self, s):
+            i_mon = self.n[0] - self.pml - 3
+            return s.port_modes(0, i_mon, 1 / 1.55)[0]
+
+        def build(self, rho):
+            s = self.simulation(rho)
+            m = self.mode(s)
+            freq = s.mode_frequencies(s.add_mode_monitor([m]))[0]
+            lo, hi = s.mode_box(m)
+            d = s.add_dft(lo, hi, [freq])
+            adj = Adjoint(s, monitor=d, component=-1, frequency=0, weights=s.mode_weights(m, 0),
+                          design=self.design_values(), checkpoint=self.checkpoint)
+            return s, adj, self.steps
+
+        def objective(self, rho):
+            s = self.simulation(rho)
+            mon = s.add_mode_monitor([self.mode(s)])
+            s.run(self.steps)
+            return float(abs(s.mode_amplitudes(mon)[0][0]) ** 2)
+
+    p = ModeProblem(P.Gpu(0))
+    rng = np.random.default_rng(8)
+    rho = rng.random(p.shape)
+    r = torch.tensor(rho, dtype=torch.float64, requires_grad=True)
+    j = objective(r, p)
+    j.backward()
+    g = r.grad.numpy()
+    plain = p.objective(rho)
+    print(f"J {float(j):.12e}, plain run's |a|^2 {plain:.12e}")
+    assert abs(float(j) - plain) < 1e-10 * plain
+    d = 1e-3
+    worst = 0.0
+    for _ in range(4):
+        c = tuple(rng.integers(0, q) for q in p.shape)
+        hi, lo = rho.copy(), rho.copy()
+        hi[c] += d
+        lo[c] -= d
+        fd = (p.objective(hi) - p.objective(lo)) / (2 * d)
+        worst = max(worst, abs(g[c] - fd) / abs(fd))
+        print(f"cell {c}: adjoint {g[c]:.9e}, central differences {fd:.9e}")
+    assert worst < 1e-3, worst
