@@ -231,14 +231,19 @@ __device__ __forceinline__ T media_update(const StepArgs<T> &A, const StepCold<T
     const long long s = (long long)c * A.nval + r;
     const int mp = K.max_poles;
     T *m_cur = K.m_cur, *m_prev = K.m_prev;
-    const T before = K.m_before[s];
-    T rest = T(G.chi) * before;
+    // the state's loads in a loop of their own, so that they issue together
     T cur[4], prv[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         if (q >= G.poles) break;
         cur[q] = m_cur[s * mp + q];
         prv[q] = m_prev[s * mp + q];
+    }
+    const T before = K.m_before[s];
+    T rest = T(G.chi) * before;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (q >= G.poles) break;
         const T alpha = T(G.coef[q][0]), xi = T(G.coef[q][1]);
         rest = rest - T(0.5) * ((alpha + T(1)) * cur[q] + xi * prv[q]);
     }
