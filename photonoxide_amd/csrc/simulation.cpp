@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -219,7 +220,9 @@ struct pxfdtd_simulation {
     double dt;
     long long steps = 0;
     std::vector<double> e, h, ca, cb;
-    std::vector<double> eps;  // [3][N] the permittivity each value of E sees (the port-mode solver's)
+    // [3][N] the permittivity each value of E sees (the port-mode solver's); allocated without
+    // a fill, the parallel loop that averages it touching it first
+    std::unique_ptr<double[]> eps;
     std::vector<double> kn[3], kh[3], fe[3], fh[3];
     std::vector<Slab> slabs;
     std::vector<double> cpml_b[3][2][2], cpml_a[3][2][2];
@@ -377,7 +380,7 @@ int make(const pxfdtd_grid *g, const pxfdtd_boundaries *b, double courant, F eps
     s->h.assign(3 * N, 0.0);
     s->ca.assign(3 * N, 1.0);
     s->cb.assign(3 * N, 0.0);
-    s->eps.assign(3 * N, 0.0);
+    s->eps.reset(new double[(size_t)(3 * N)]);
     int bad = 0;
     for (int c = 0; c < 3; c++) {
         // each value's average alone: the same bits on any number of threads
@@ -2528,7 +2531,7 @@ extern "C" int pxfdtd_simulation_mode_weights(pxfdtd_simulation *s, const pxfdtd
 
 // the permittivity each value of E sees, [3][N] (Simulation::diagonal's averages, or those given)
 extern "C" int pxfdtd_simulation_permittivity(const pxfdtd_simulation *s, double *out) {
-    std::memcpy(out, s->eps.data(), s->eps.size() * sizeof(double));
+    std::memcpy(out, s->eps.get(), (size_t)(3 * s->N) * sizeof(double));
     return PXFDTD_OK;
 }
 
