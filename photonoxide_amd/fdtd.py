@@ -521,6 +521,22 @@ class Simulation:
                                                         C.byref(waveform.native())))
         del keep
 
+    def port_mode(self, axis: int, plane: int, frequency: float, te: bool = True, window=(None, None)) -> PortMode:
+        """The fundamental mode of one polarization, as the reference's FDTD job picks it
+        (src/job/fdtd.rs:2097-2135): of the 4 modes port_modes gives, the first whose E lies
+        mostly (te) or not (tm) along the in-plane axis across the guide (y for a port normal
+        to x, else x), by the share of |E_t|^2."""
+        modes = self.port_modes(axis, plane, frequency, count=4, window=window)
+        b, c = (axis + 1) % 3, (axis + 2) % 3
+        across = 1 if axis == 0 else 0
+        for m in modes:
+            e = np.abs(m.tangential) ** 2  # [t][v][u], t = 0 along b, 1 along c
+            along = e[0].sum() if b == across else e[1].sum()
+            share = along / max(e.sum(), np.finfo(float).tiny)
+            if (share > 0.5) == te:
+                return m
+        raise ValueError(f"no {'TE' if te else 'TM'} mode guides on the plane at node {plane} along axis {axis}")
+
     def permittivity(self) -> np.ndarray:
         """The permittivity each value of E sees: [3][nz][ny][nx]."""
         g = self.grid

@@ -118,6 +118,21 @@ def test_a_real_mode_carries_unit_power_and_is_real():
         assert wrong < 1e-12 * largest, (wrong, largest)
 
 
+def test_the_jobs_te_and_tm_picks():
+    """src/job/fdtd.rs:2108-2135: TE has its E mostly across the guide (y for a port along x),
+    TM mostly along z; in the walled strip (wider than tall) the TE fundamental has the higher
+    index."""
+    s, n = walled_strip()
+    te = s.port_mode(0, 6, 1 / 1.55, te=True)
+    tm = s.port_mode(0, 6, 1 / 1.55, te=False)
+    ey, ez = (np.abs(te.tangential) ** 2).sum(axis=(1, 2))
+    print(f"TE n_eff {te.beta / te.k0}, TM n_eff {tm.beta / tm.k0}")
+    assert ey > ez
+    ey, ez = (np.abs(tm.tangential) ** 2).sum(axis=(1, 2))
+    assert ez > ey
+    assert te.beta.real > tm.beta.real
+
+
 def test_host_mirror_refuses_ports_that_dont_fit():
     """ports.rs:530-580 (check_port) and the mode source's clearance of the CPMLs."""
     import photonoxide_amd as P
