@@ -358,11 +358,14 @@ int pxfdtd_simulation_permittivity(const pxfdtd_simulation *s, double *out);
  * by |Re beta|. Out: beta [count][2], tangential [count][2][size_c][size_b] complex and normal
  * [count][size_c][size_b] complex as pxfdtd_port_mode holds them, and each mode's power (the
  * scheme's flux at unit amplitude; 1 for a real mode). The plane and the next must be clear of
- * the CPMLs as a mode source's; a periodic axis across must be whole in the window. */
+ * the CPMLs as a mode source's; a periodic axis across must be whole in the window. With
+ * `lossy` nonzero the permittivity is the leapfrog's at the frequency where a conductivity was
+ * given, eps + i sigma cos(w dt / 2) / k0 (the reference's lossy check, src/fdtd/checks.rs:689-
+ * 740); otherwise the real permittivity, as the reference's job solves it. */
 int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, int32_t plane,
                                  const int32_t window[2][2], double frequency, int32_t count,
-                                 int32_t *found, double *beta, double *tangential, double *normal,
-                                 double *power);
+                                 int32_t lossy, int32_t *found, double *beta, double *tangential,
+                                 double *normal, double *power);
 /* harmonic_inversion (src/fdtd/harmonic.rs:62-184): the terms sum_k d_k e^(-i w_k t) of a
  * signal of n complex samples (re, im pairs) every dt (um/c) whose frequencies lie in [low,
  * high] (c/um), by Mandelshtam and Taylor's filter diagonalization, sorted by frequency: each

@@ -201,7 +201,7 @@ SIGNATURES = {
                                             C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "pxfdtd_simulation_permittivity": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "pxfdtd_simulation_port_modes": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_void_p, C.c_double, C.c_int32,
-                                               C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                               C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pxfdtd_simulation_mode_frequencies": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_double)]),
     "pxfdtd_simulation_mode_amplitudes": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_double)]),

@@ -223,6 +223,7 @@ struct pxfdtd_simulation {
     // [3][N] the permittivity each value of E sees (the port-mode solver's); allocated without
     // a fill, the parallel loop that averages it touching it first
     std::unique_ptr<double[]> eps;
+    std::vector<double> sigma;  // [3][N] the conductivity given, when one was (lossy port modes)
     std::vector<double> kn[3], kh[3], fe[3], fh[3];
     std::vector<Slab> slabs;
     std::vector<double> cpml_b[3][2][2], cpml_a[3][2][2];
@@ -481,6 +482,8 @@ extern "C" int pxfdtd_simulation_conductivity(pxfdtd_simulation *s, const pxfdtd
             const double half = sg * cb / 2.0;
             s->ca[c * N + r] = (1.0 - half) / (1.0 + half);
             cb /= 1.0 + half;
+            if (s->sigma.empty()) s->sigma.assign((size_t)(3 * N), 0.0);
+            s->sigma[(size_t)(c * N + r)] = sg;
         }
     drop_resident(s);
     return PXFDTD_OK;
@@ -2580,8 +2583,8 @@ static double lattice_flux(const ModeLattice &L, int axis, int plane, const E &e
 // CPMLs as a mode source's (the reference's job solves on planes cut out of the grid there).
 extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, int32_t plane,
                                             const int32_t window[2][2], double frequency, int32_t count,
-                                            int32_t *found, double *beta_out, double *tangential_out,
-                                            double *normal_out, double *power_out) {
+                                            int32_t lossy, int32_t *found, double *beta_out,
+                                            double *tangential_out, double *normal_out, double *power_out) {
     *found = 0;
     if (axis < 0 || axis > 2) return px_fail(PXFDTD_ERR_INVALID, "a port's axis must be 0, 1 or 2");
     if (count < 1) return px_fail(PXFDTD_ERR_INVALID, "ask for one mode or more");
@@ -2655,7 +2658,13 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
         return local[(size_t)(t * per + (long long)m[c] * size[0] + m[b])];
     };
     const long long N = s->N;
-    auto eps_of = [&](long long r) { return s->eps[(size_t)r]; };  // r = component * N + cell
+    // the permittivity the leapfrog sees at the frequency: eps, or with `lossy` eps + i sigma
+    // cos(w dt / 2) / k0 (the conductivity's semi-implicit average; r = component * N + cell)
+    const double loss_factor = cs / k0;
+    auto eps_of = [&](long long r) -> C64 {
+        const double im = lossy && !s->sigma.empty() ? s->sigma[(size_t)r] * loss_factor : 0.0;
+        return {s->eps[(size_t)r], im};
+    };
     auto normal_index = [&](long long q) {
         int comp, m[3];
         L.at(q, comp, m);
@@ -2667,8 +2676,9 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
     C64 ws[16], ws2[16];
     for (size_t row = 0; row < unknowns.size(); row++) {
         const long long r = unknowns[row];
-        top = std::max(top, eps_of(r));
-        entries.emplace_back((long long)row, (long long)row, pxeig::cd(k2 * eps_of(r), 0.0));
+        const C64 er = eps_of(r);
+        top = std::max(top, er.re);
+        entries.emplace_back((long long)row, (long long)row, pxeig::cd(k2 * er.re, k2 * er.im));
         const int nf = L.curl_h(r, cols, ws);
         for (int q = 0; q < nf; q++) {
             int fc, fm[3];
@@ -2686,14 +2696,13 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
         for (int q = 0; q < ng; q++) {
             const long long nq = normal_index(cols[q]);
             if (!inside(nq)) continue;
-            const double inverse = 1.0 / eps_of(nq);
+            const C64 inverse = ModeLattice::rdiv(1.0, eps_of(nq));
             const int nd = L.divergence(cols[q], cols2, ws2);
             for (int t = 0; t < nd; t++) {
                 const long long k = number(cols2[t]);
                 if (k < 0) continue;
-                const C64 v = cmul(ws[q], ws2[t]);
-                const double f = eps_of(cols2[t]) * inverse;
-                entries.emplace_back((long long)row, k, pxeig::cd(v.re * f, v.im * f));
+                const C64 v = cmul(cmul(cmul(ws[q], ws2[t]), eps_of(cols2[t])), inverse);
+                entries.emplace_back((long long)row, k, pxeig::cd(v.re, v.im));
             }
         }
     }
@@ -2742,12 +2751,10 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
                     int cc, cm[3];
                     L.at(cols[t], cc, cm);
                     if (cc == axis) continue;
-                    const C64 e = at_plane(cols[t]);
-                    const double ep = eps_of(cols[t]);
-                    const C64 p = cmul({ws[t].re * ep, ws[t].im * ep}, e);
+                    const C64 p = cmul(cmul(ws[t], eps_of(cols[t])), at_plane(cols[t]));
                     div = {div.re + p.re, div.im + p.im};
                 }
-                const C64 val = cmul(cdiv(cmul({0.0, 1.0}, div), {beta_d.re * eps_of(r), beta_d.im * eps_of(r)}), half);
+                const C64 val = cmul(cdiv(cmul({0.0, 1.0}, div), cmul(beta_d, eps_of(r))), half);
                 const long long o = (long long)v * size[0] + u;
                 m.normal[2 * o] = val.re;
                 m.normal[2 * o + 1] = val.im;

@@ -544,12 +544,14 @@ class Simulation:
         N.check(N.lib.pxfdtd_simulation_permittivity(self._h, N.dptr(out)))
         return out.reshape(3, g.nz, g.ny, g.nx)
 
-    def port_modes(self, axis: int, plane: int, frequency: float, count: int = 1, window=(None, None)):
+    def port_modes(self, axis: int, plane: int, frequency: float, count: int = 1, window=(None, None),
+                   lossy: bool = False):
         """Lattice::port_modes (src/fdfd/three/ports.rs:408-594) on this grid and permittivity:
         up to `count` modes of the guide crossing node plane `plane` normal to `axis`, at the
         leapfrog wavenumber of `frequency` (the k0 a mode source at that frequency needs),
         within window[w] = (lo, hi) nodes along the w-th of the axes after `axis` (None: whole);
-        propagating modes first by |Re beta|, each normalized to N(f, g) = 1."""
+        propagating modes first by |Re beta|, each normalized to N(f, g) = 1; `lossy`: with the
+        conductivity as the leapfrog sees it, eps + i sigma cos(w dt / 2) / k0."""
         g = self.grid
         n = (g.nx, g.ny, g.nz)
         nb, nc = n[(axis + 1) % 3], n[(axis + 2) % 3]
@@ -561,7 +563,8 @@ class Simulation:
         power = np.zeros(count)
         found = C.c_int32()
         N.check(N.lib.pxfdtd_simulation_port_modes(self._h, int(axis), int(plane), C.cast(win, C.c_void_p),
-                                                   float(frequency), int(count), C.byref(found), N.dptr(beta),
+                                                   float(frequency), int(count), int(bool(lossy)), C.byref(found),
+                                                   N.dptr(beta),
                                                    N.dptr(tan), N.dptr(nor), N.dptr(power)))
         k0 = self.leapfrog_wavenumber(frequency)
         step = (g.dx, g.dy, g.dz)[axis]

@@ -196,3 +196,76 @@ def test_a_mode_source_launches_its_mode_one_way_with_its_power():
     assert backward_share < 1e-5
     assert abs(forward_share - 1.0) < 5e-4
     assert abs(abs(fwd) / weight - 1.0) < 5e-4
+
+
+@pytest.mark.gpu
+def test_a_mode_source_in_a_lossy_box_is_fdfds_mode_source():
+    """checks.rs:689-740, tests.rs:279-283: the guide's fundamental (this build's port_modes with
+    the leapfrog's loss, eps + i sigma cos(w dt / 2) / k0) launched forward from plane 6 by a
+    continuous wave in a closed box of a lossy medium (sigma = 4/um everywhere; 16 x 14 x 24 cells
+    of 50 nm, 64 steps a period of 1.55 um), its steady amplitude after 90 periods (the
+    transform over 4 more, times 2 / (4 x 64 dt)) against FDFD's field for FDFD's own mode source
+    of the same mode (Lattice::mode_source, ports.rs:594-634, restated here; A = k0^2 eps -
+    curl curl from the oracle's lattice, walls all round, solved by scipy's sparse LU): within
+    1e-10 of the largest field, the reference's bound."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    import photonoxide_amd as P
+    print("case mode source in a lossy box against FDFD", flush=True)
+    n, spp, sigma = (16, 14, 24), 64, 4.0
+    g = P.Grid3d(*n, H, H, H, -n[0] * H / 2, -n[1] * H / 2, -n[2] * H / 2)
+    courant = 1.55 / spp * math.sqrt(3.0 / (H * H))
+    walls = P.Pml(0, 0)
+    s = P.Simulation.new(g, P.Scene(2.1, [P.Box((-0.2, -0.15, -np.inf), (0.2, 0.15, np.inf), 12.0)]),
+                         P.Boundaries(walls, walls, walls), courant)
+    s.with_conductivity(P.Scene(sigma))
+    f = 1 / 1.55
+    mode = s.port_modes(2, 6, f, lossy=True)[0]
+    k0 = mode.k0
+    dt = s.dt()
+    s.add_mode_source(mode, 0, P.Waveform.continuous(f, ramp=4.0 * 1.55))
+    s.set_device(P.Gpu(0), P.Precision.F64)
+    s.run(90 * spp)
+    d = s.add_dft((0, 0, 0), (n[0] - 1, n[1] - 1, n[2] - 1), [f])
+    s.run(4 * spp)
+    amp = np.concatenate([s.dft(d, P.Field.E, c)[0].ravel() for c in range(3)]) * (2.0 / (4 * spp * dt))
+    # FDFD: A x = b on the same lattice
+    lat = MM.Lattice(n, (H,) * 3, (g.x0, g.y0, g.z0), (0, 0, 0), (0, 0, 0), (0, 0, 0), 1e-8, 3.0, k0)
+    loss = sigma * math.cos(2 * math.pi * f * dt / 2.0) / k0
+    eps = s.permittivity().ravel() + 1j * loss
+    total = 3 * lat.cells
+    rows, cols, vals = [], [], []
+    for r in range(total):
+        if lat.fixed(r):
+            rows.append(r), cols.append(r), vals.append(1.0)
+            continue
+        rows.append(r), cols.append(r), vals.append(k0 * k0 * eps[r])
+        for q, a in lat.curl_curl(r):
+            rows.append(r), cols.append(q), vals.append(complex(*a))
+    A = sp.csc_matrix((vals, (rows, cols)), shape=(total, total))
+    om = oracle_mode(mode)
+    p = mode.plane
+
+    def scattered(r):
+        c, at = lat.at(r)
+        return 2 * at[2] + (1 if c == 2 else 0) < 2 * p
+    b = np.zeros(total, complex)
+    for m in range(p - 1, p + 3):
+        for v in range(n[1]):
+            for u in range(n[0]):
+                for c in range(3):
+                    r = lat.index(c, (u, v, m))
+                    qr = scattered(r)
+                    acc = 0j
+                    for q, a in lat.curl_curl(r):
+                        if scattered(q) != qr:
+                            cq, at = lat.at(q)
+                            val = complex(*MM.mode_value(om, cq, at, True, O.lib.or_exp, sincos))
+                            acc += (1.0 if qr else -1.0) * complex(*a) * val
+                    b[r] += acc
+    x = spl.spsolve(A, b)
+    largest = np.abs(x).max()
+    worst = np.abs(x - amp).max()
+    print(f"n_eff {mode.beta / k0}, largest field {largest:.4e}, worst difference {worst:.3e}, "
+          f"relative {worst / largest:.2e}")
+    assert worst < 1e-10 * largest
