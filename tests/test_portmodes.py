@@ -207,7 +207,10 @@ def test_a_mode_source_in_a_lossy_box_is_fdfds_mode_source():
     transform over 4 more, times 2 / (4 x 64 dt)) against FDFD's field for FDFD's own mode source
     of the same mode (Lattice::mode_source, ports.rs:594-634, restated here; A = k0^2 eps -
     curl curl from the oracle's lattice, walls all round, solved by scipy's sparse LU): within
-    1e-10 of the largest field, the reference's bound."""
+    1e-10 of the largest field, the reference's bound. The mode monitor's amplitudes over the
+    same periods (the same scale), for the mode behind on plane 4 and ahead on plane 16, are
+    FDFD's mode_amplitudes of FDFD's field (the oracle's restatement of the form) within 1e-9 of
+    the largest (monitors_checks.rs:255-310, there with a pulse over its analytic spectrum)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spl
     import photonoxide_amd as P
@@ -221,14 +224,18 @@ def test_a_mode_source_in_a_lossy_box_is_fdfds_mode_source():
     s.with_conductivity(P.Scene(sigma))
     f = 1 / 1.55
     mode = s.port_modes(2, 6, f, lossy=True)[0]
+    behind = s.port_modes(2, 4, f, lossy=True)[0]
+    ahead = s.port_modes(2, 16, f, lossy=True)[0]
     k0 = mode.k0
     dt = s.dt()
     s.add_mode_source(mode, 0, P.Waveform.continuous(f, ramp=4.0 * 1.55))
     s.set_device(P.Gpu(0), P.Precision.F64)
     s.run(90 * spp)
     d = s.add_dft((0, 0, 0), (n[0] - 1, n[1] - 1, n[2] - 1), [f])
+    mon = s.add_mode_monitor([behind, ahead])
     s.run(4 * spp)
     amp = np.concatenate([s.dft(d, P.Field.E, c)[0].ravel() for c in range(3)]) * (2.0 / (4 * spp * dt))
+    measured = [(a * (2.0 / (4 * spp * dt)), bk * (2.0 / (4 * spp * dt))) for a, bk in s.mode_amplitudes(mon)]
     # FDFD: A x = b on the same lattice
     lat = MM.Lattice(n, (H,) * 3, (g.x0, g.y0, g.z0), (0, 0, 0), (0, 0, 0), (0, 0, 0), 1e-8, 3.0, k0)
     loss = sigma * math.cos(2 * math.pi * f * dt / 2.0) / k0
@@ -269,3 +276,13 @@ def test_a_mode_source_in_a_lossy_box_is_fdfds_mode_source():
     print(f"n_eff {mode.beta / k0}, largest field {largest:.4e}, worst difference {worst:.3e}, "
           f"relative {worst / largest:.2e}")
     assert worst < 1e-10 * largest
+
+    def fdfd_field(c, at):
+        v = x[lat.index(c, at)]
+        return (float(v.real), float(v.imag))
+    expected = [MM.mode_amplitudes(lat, oracle_mode(m), fdfd_field, O.lib.or_exp, sincos) for m in (behind, ahead)]
+    expected = [(complex(*a), complex(*bk)) for a, bk in expected]
+    top = max(max(abs(a), abs(bk)) for a, bk in expected)
+    err = max(max(abs(e[0] - m[0]), abs(e[1] - m[1])) for e, m in zip(expected, measured))
+    print(f"mode amplitudes: FDFD {expected}, monitor {measured}, worst {err / top:.2e} of the largest")
+    assert err < 1e-9 * top
