@@ -1201,7 +1201,6 @@ extern "C" int pxfdtd_simulation_add_plane_wave(pxfdtd_simulation *s, const int3
                        "the medium on the box's surface must be uniform, with the permittivity %g and no "
                        "conductivity", eps);
     (void)index_of;
-    (void)N;
     // l over the box and a cell around it: its extremes are at the corners
     long long least = 0, most = 0, stride = 0;
     for (int a = 0; a < 3; a++) {
@@ -2657,7 +2656,6 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
         const int t = comp == c ? 1 : 0;
         return local[(size_t)(t * per + (long long)m[c] * size[0] + m[b])];
     };
-    const long long N = s->N;
     // the permittivity the leapfrog sees at the frequency: eps, or with `lossy` eps + i sigma
     // cos(w dt / 2) / k0 (the conductivity's semi-implicit average; r = component * N + cell)
     const double loss_factor = cs / k0;
@@ -2706,7 +2704,6 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
             }
         }
     }
-    (void)N;
     std::string why;
     const std::vector<pxeig::Pair> pairs =
         pxeig::nearest((long long)unknowns.size(), entries, pxeig::cd(k2 * top, 0.0), count, 1e-10, &why);
