@@ -994,7 +994,7 @@ __device__ __forceinline__ double aux_difference(const PlaneDesc &d, const doubl
 
 // H~ <- da H~ - db curl E over every plane wave's samples pad .. len - pad - 1
 // (sources.rs:1011-1021); one thread per sample, blockIdx.y the plane wave
-__global__ __launch_bounds__(256) void aux_step_h(const PlaneDesc *planes, double *aux,
+static __global__ __launch_bounds__(256) void aux_step_h(const PlaneDesc *planes, double *aux,
                                                   const double *coef) {
     const PlaneDesc &d = planes[blockIdx.y];
     const long long i = d.pad + (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1012,7 +1012,7 @@ __global__ __launch_bounds__(256) void aux_step_h(const PlaneDesc *planes, doubl
 
 // E <- ca E + cb curl H~, then the sheet E_c[i] -= cb[i] j w (sources.rs:1024-1039), w the
 // plane wave's emitter value this step
-__global__ __launch_bounds__(256) void aux_step_e(const PlaneDesc *planes, double *aux,
+static __global__ __launch_bounds__(256) void aux_step_e(const PlaneDesc *planes, double *aux,
                                                   const double *coef, const double *waves,
                                                   int emitters, int wave_step) {
     const PlaneDesc &d = planes[blockIdx.y];

@@ -25,6 +25,8 @@
 #include <mutex>
 #include <thread>
 
+#include "batch_launch.hpp"
+
 // the host mirror's side of the server (simulation.cpp)
 namespace pxfdtd_host {
 int sim_footprint(pxfdtd_simulation *s, int32_t precision, size_t *bytes);
@@ -40,62 +42,6 @@ void sim_release(pxfdtd_simulation *s);
 }  // namespace pxfdtd_host
 
 namespace pxfdtd {
-
-template <typename T>
-struct JobStep {
-    StepArgs<T> A[2];  // A[q]: reads copy q, writes copy 1 - q, the parameters of parity q
-};
-struct JobSeg {
-    int32_t active;  // steps the job takes this segment (0: none)
-    int32_t p0;      // the copy its first step reads
-};
-struct BatchBlock {
-    int32_t job, first;  // a workgroup's job and its first of four units
-};
-
-template <typename T, int MODE, int RY, int AX>
-__global__ __launch_bounds__(256) void march64_batch(const JobStep<T> *jobs, const JobSeg *seg,
-                                                     const BatchBlock *blocks, const int4 *units,
-                                                     int nblocks, int per_xcd, int s) {
-    __shared__ MarchShared<T> S;
-    extern __shared__ unsigned char smem_raw[];
-    T *tab = reinterpret_cast<T *>(smem_raw);
-    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);
-    const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (b >= nblocks) return;  // the whole workgroup
-    const BatchBlock B = blocks[b];
-    const JobSeg g = seg[B.job];
-    if (s >= g.active) return;  // the whole workgroup: its job is done
-    StepArgs<T> A = jobs[B.job].A[(g.p0 + s) & 1];
-    A.wave_step = s;
-    march64_stage<T, MODE, AX != 0>(A, S, tab, grp);
-    __syncthreads();
-    const int4 u = units[uniform(B.first + (int)(threadIdx.x >> 6))];
-    if (u.w <= u.z) return;  // a padding unit
-    march64_unit<T, MODE, RY, AX>(A, u, S, tab, grp);
-}
-
-template <typename T, int MODE, int AX, int SET>
-__global__ __launch_bounds__(256) void march2d_batch(const JobStep<T> *jobs, const JobSeg *seg,
-                                                     const BatchBlock *blocks, const int4 *units,
-                                                     int nblocks, int per_xcd, int s) {
-    __shared__ MarchShared<T> S;
-    extern __shared__ unsigned char smem_raw[];
-    T *tab = reinterpret_cast<T *>(smem_raw);
-    int8_t *grp = reinterpret_cast<int8_t *>(tab + 256 * 6);
-    const int b = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
-    if (b >= nblocks) return;
-    const BatchBlock B = blocks[b];
-    const JobSeg g = seg[B.job];
-    if (s >= g.active) return;
-    StepArgs<T> A = jobs[B.job].A[(g.p0 + s) & 1];
-    A.wave_step = s;
-    march64_stage<T, MODE, AX != 0>(A, S, tab, grp);
-    __syncthreads();
-    const int4 u = units[uniform(B.first + (int)(threadIdx.x >> 6))];
-    if (u.w <= u.z) return;
-    march2d_unit<T, MODE, AX, SET>(A, u, S, tab, grp);
-}
 
 template <typename T>
 struct TapJob {
@@ -276,36 +222,9 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
     return PXFDTD_OK;
 }
 
-template <int MODE, int RY, int AX>
-static void launch_batch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, int s, size_t lds) {
-    const int per_xcd = (c.count + 7) / 8;
-    hipLaunchKernelGGL((march64_batch<float, MODE, RY, AX>), dim3(8 * per_xcd), dim3(256), lds, S->stream,
-                       S->d_jobs.as<JobStep<float>>(), S->d_seg.as<JobSeg>(),
-                       S->d_blocks.as<BatchBlock>() + c.first, S->d_units.as<int4>(), c.count, per_xcd, s);
-}
-
-template <int MODE, int AX, int SET>
-static void launch_batch_class_2d(pxfdtd_server *S, const pxfdtd_server::Class &c, int s, size_t lds) {
-    const int per_xcd = (c.count + 7) / 8;
-    hipLaunchKernelGGL((march2d_batch<float, MODE, AX, SET>), dim3(8 * per_xcd), dim3(256), lds, S->stream,
-                       S->d_jobs.as<JobStep<float>>(), S->d_seg.as<JobSeg>(),
-                       S->d_blocks.as<BatchBlock>() + c.first, S->d_units.as<int4>(), c.count, per_xcd, s);
-}
-
 static void launch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, int s) {
-    const size_t lds = c.mode == 0 ? (size_t)256 * (6 * sizeof(float) + 3) : 0;
-#define PX_CLASS(M, R, X) \
-    if (c.kind == 3 && c.mode == M && c.v == R && c.ax == X) return launch_batch_class<M, R, X>(S, c, s, lds);
-    PX_CLASS(0, 1, 0) PX_CLASS(0, 1, 1) PX_CLASS(0, 1, 7) PX_CLASS(0, 2, 0) PX_CLASS(0, 2, 1) PX_CLASS(0, 2, 7)
-    PX_CLASS(1, 1, 0) PX_CLASS(1, 1, 1) PX_CLASS(1, 1, 7) PX_CLASS(1, 2, 0) PX_CLASS(1, 2, 1) PX_CLASS(1, 2, 7)
-    PX_CLASS(2, 1, 0) PX_CLASS(2, 1, 1) PX_CLASS(2, 1, 7) PX_CLASS(2, 2, 0) PX_CLASS(2, 2, 1) PX_CLASS(2, 2, 7)
-#undef PX_CLASS
-#define PX_CLASS2(M, X, E) \
-    if (c.kind == 4 && c.mode == M && c.ax == X && c.v == E) return launch_batch_class_2d<M, X, E>(S, c, s, lds);
-    PX_CLASS2(0, 0, 1) PX_CLASS2(0, 0, 2) PX_CLASS2(0, 0, 3) PX_CLASS2(0, 3, 1) PX_CLASS2(0, 3, 2) PX_CLASS2(0, 3, 3)
-    PX_CLASS2(1, 0, 1) PX_CLASS2(1, 0, 2) PX_CLASS2(1, 0, 3) PX_CLASS2(1, 3, 1) PX_CLASS2(1, 3, 2) PX_CLASS2(1, 3, 3)
-    PX_CLASS2(2, 0, 1) PX_CLASS2(2, 0, 2) PX_CLASS2(2, 0, 3) PX_CLASS2(2, 3, 1) PX_CLASS2(2, 3, 2) PX_CLASS2(2, 3, 3)
-#undef PX_CLASS2
+    launch_batch_class(c.kind, c.mode, c.v, c.ax, S->d_jobs.as<JobStep<float>>(), S->d_seg.as<JobSeg>(),
+                       S->d_blocks.as<BatchBlock>() + c.first, S->d_units.as<int4>(), c.count, s, S->stream);
 }
 
 // one segment's launches on the server's stream (captured into the graph, or run directly)

@@ -1,0 +1,7 @@
+// The step kernels in f32, coefficient mode 2.
+#include "step_launch_impl.hpp"
+
+namespace pxfdtd {
+template int launch_mode<float, 2>(const StepArgs<float> &, int, int, hipStream_t, size_t, const SideStream *,
+                                  const UnitCounts &, int);
+}  // namespace pxfdtd

@@ -1,0 +1,7 @@
+// The step kernels in f64, coefficient mode 2.
+#include "step_launch_impl.hpp"
+
+namespace pxfdtd {
+template int launch_mode<double, 2>(const StepArgs<double> &, int, int, hipStream_t, size_t, const SideStream *,
+                                  const UnitCounts &, int);
+}  // namespace pxfdtd
