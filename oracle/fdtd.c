@@ -345,9 +345,9 @@ static void build_cpml(or_sim *s) {
         const double d_ = (double)(cells) * h;                                                \
         double q_ = (depth) / (double)(cells);                                                \
         q_ = q_ < 0 ? 0 : (q_ > 1 ? 1 : q_);                                                  \
-        const double x_ = pow(q_, cp->order);                                                 \
+        const double x_ = or_pow(q_, cp->order);                                                 \
         const double smax_ = cp->has_sigma ? cp->sigma                                        \
-                                           : (cp->order + 1.0) * (-log(cp->reflection)) /     \
+                                           : (cp->order + 1.0) * (-or_ln(cp->reflection)) /   \
                                                  (2.0 * d_);                                  \
         sig = smax_ * x_;                                                                     \
         kap = 1.0 + (cp->kappa - 1.0) * x_;                                                   \

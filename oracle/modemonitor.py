@@ -17,9 +17,9 @@ reference, as plain Python, with the reference's operations in the reference's o
 
 Complex numbers are (re, im) pairs with num-complex's product and quotient, a real over a
 complex as num-complex divides it (a (c - i d) / (c^2 + d^2)), a product of stretches folded
-from 1; ln and powf are the C library's (math.log, math.pow), where the reference uses its
-portable ones: the stretches can differ from the reference's in their last bits. complex exp is
-the reference's portable one from the oracle's C exp and sincos, which the caller passes.
+from 1; ln and powf are the reference's portable ones (src/portable/mod.rs:141-220, :299-301),
+the oracle's C restatements or_ln and or_pow, which the caller installs with use_portable; complex
+exp is the reference's portable one from the oracle's C exp and sincos, which the caller passes.
 
 Only tests/ use this, as the checker of the host mirror's amplitudes. What pins it, beyond its
 own formula (tests/test_modemonitor.py):
@@ -63,11 +63,21 @@ def cscale(a, x):
     return (a[0] * x, a[1] * x)
 
 
+_PORTABLE = {}
+
+
+def use_portable(ln, powf):
+    """The portable ln and powf the stretches are graded with (the oracle's or_ln, or_pow)."""
+    _PORTABLE["ln"], _PORTABLE["powf"] = ln, powf
+
+
 def graded(pos, span, layers, h, k0, reflection, order):
     """graded: the stretch at pos on an axis spanning span with layers cells of PML."""
+    ln, powf = _PORTABLE["ln"], _PORTABLE["powf"]
+
     def profile(depth, d):
-        sigma = (order + 1.0) * (-math.log(reflection)) / (2.0 * k0 * d)
-        return (1.0, sigma * math.pow(min(max(depth / d, 0.0), 1.0), order))
+        sigma = (order + 1.0) * (-ln(reflection)) / (2.0 * k0 * d)
+        return (1.0, sigma * powf(min(max(depth / d, 0.0), 1.0), order))
     dl, dh = layers[0] * h, layers[1] * h
     if dl > 0.0 and pos < span[0] + dl:
         return profile(span[0] + dl - pos, dl)

@@ -46,6 +46,8 @@ void or_sincos(double x, double *s, double *c);
 double or_sin(double x);
 double or_cos(double x);
 double or_powi(double x, int n);
+double or_ln(double x);
+double or_pow(double x, double y);
 or_c64 or_cexp(or_c64 z);
 
 /* src/fdfd/three.rs:70-89 */

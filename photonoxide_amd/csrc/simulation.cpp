@@ -292,8 +292,8 @@ void build_cpml(pxfdtd_simulation *s) {
         const double h = hs[axis];
         auto profile = [&](double depth, int cells, double &sig, double &kap) {
             const double d = cells * h;
-            const double x = std::pow(std::clamp(depth / cells, 0.0, 1.0), cp.order);
-            const double smax = cp.has_sigma ? cp.sigma : (cp.order + 1.0) * (-std::log(cp.reflection)) / (2.0 * d);
+            const double x = pxfdtd::ppow(std::clamp(depth / cells, 0.0, 1.0), cp.order);
+            const double smax = cp.has_sigma ? cp.sigma : (cp.order + 1.0) * (-pxfdtd::pln(cp.reflection)) / (2.0 * d);
             sig = smax * x;
             kap = 1.0 + (cp.kappa - 1.0) * x;
         };
@@ -2094,8 +2094,8 @@ extern "C" int32_t pxfdtd_simulation_flux_frequencies(const pxfdtd_simulation *s
 // the grid's boundaries as fdfd_boundaries gives them, src/fdtd/monitors.rs:642-651): a CPML of
 // low and high cells is a PML of as many (0 cells a wall) graded with the CPML's reflection and
 // order and no real stretch, a periodic axis Bloch with k = 0. Its stretches at the nodes and
-// halfway after them (graded, src/fdfd/mod.rs:219-242, with the C library's log and pow where
-// the reference has its portable ones); shift, fixed, neighbour, curl_e, curl_h and
+// halfway after them (graded, src/fdfd/mod.rs:219-242, with the reference's portable ln and
+// powf); shift, fixed, neighbour, curl_e, curl_h and
 // curl_curl_into (three.rs:310-418) and volume (src/fdfd/three/ports.rs:228-240). Values are
 // numbered as Grid3d::index numbers them: component, then k, j, i.
 struct ModeLattice {
@@ -2131,10 +2131,10 @@ struct ModeLattice {
     C64 graded(double pos, double span0, double span1, int low, int high, double step, double R,
                double order) const {
         auto profile = [&](double depth, double d) -> C64 {
-            const double sigma = (order + 1.0) * (-std::log(R)) / (2.0 * k0 * d);
+            const double sigma = (order + 1.0) * (-pxfdtd::pln(R)) / (2.0 * k0 * d);
             double q = depth / d;
             q = q < 0.0 ? 0.0 : (q > 1.0 ? 1.0 : q);
-            return {1.0, sigma * std::pow(q, order)};
+            return {1.0, sigma * pxfdtd::ppow(q, order)};
         };
         const double dl = (double)low * step, dh = (double)high * step;
         if (dl > 0.0 && pos < span0 + dl) return profile(span0 + dl - pos, dl);

@@ -35,6 +35,8 @@ import oracle_lib as O
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import modemonitor as MM  # noqa: E402
 
+MM.use_portable(O.lib.or_ln, O.lib.or_pow)
+
 
 def sincos(x):
     s, c = O.C.c_double(), O.C.c_double()

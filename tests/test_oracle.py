@@ -69,7 +69,7 @@ def test_portable_exp_and_sincos():
 # x86-64 with libm 0.2.16. Pairs are (grid[i], grid[n - 1 - i]).
 GRID_HASHES = {"exp": 0x413FBC04627E9685, "sin_cos.0": 0xDA525217FA01FBF0,
                "sin_cos.1": 0xDA5C8775269B6B93, "powi": 0x0193356F37480A41,
-               "c_exp": 0x9FC0EEDECCB4CA6E}
+               "c_exp": 0x9FC0EEDECCB4CA6E, "ln": 0x0BF1E9D3216EFC53, "powf": 0x9133A6ECA7E0BB06}
 
 
 def reference_grid():
@@ -90,7 +90,7 @@ def grid_fold(h, v):
 
 
 def test_portable_bits_are_the_references_over_its_grid():
-    """exp, sin_cos (both parts), powi and complex exp give the reference's bits at every
+    """exp, ln, sin_cos (both parts), powi, powf and complex exp give the reference's bits at every
     argument of its grid, large ones (1e22, the largest double: Payne-Hanek) and special ones
     (inf, NaN, subnormals) included: the hashes are the reference's own."""
     xs = reference_grid()
@@ -104,15 +104,17 @@ def test_portable_bits_are_the_references_over_its_grid():
     def as_i32(v):  # Rust's saturating `as i32`
         return 0 if math.isnan(v) else int(max(min(v, 2**31 - 1), -2**31))
 
-    got = {"exp": 0, "sin_cos.0": 0, "sin_cos.1": 0, "powi": 0, "c_exp": 0}
+    got = {"exp": 0, "sin_cos.0": 0, "sin_cos.1": 0, "powi": 0, "c_exp": 0, "ln": 0, "powf": 0}
     for x in xs:
         got["exp"] = grid_fold(got["exp"], O.lib.or_exp(x))
+        got["ln"] = grid_fold(got["ln"], O.lib.or_ln(x))
         sv, cv = sincos(x)
         got["sin_cos.0"] = grid_fold(got["sin_cos.0"], sv)
         got["sin_cos.1"] = grid_fold(got["sin_cos.1"], cv)
     for a, b in pairs:
         n = as_i32(b * 7.0)
         got["powi"] = grid_fold(got["powi"], O.lib.or_powi(a, int(math.copysign(abs(n) % 64, n))))
+        got["powf"] = grid_fold(got["powf"], O.lib.or_pow(a, b))
         r = O.lib.or_exp(a)  # portable::complex::exp: exp(re) (cos im, sin im), im 0 kept
         if b == 0.0:
             w = (r, b)

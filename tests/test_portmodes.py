@@ -32,6 +32,8 @@ import oracle_lib as O
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import modemonitor as MM  # noqa: E402
 
+MM.use_portable(O.lib.or_ln, O.lib.or_pow)
+
 H = 0.05
 SILICON, OXIDE = 3.48, 1.444
 
