@@ -665,6 +665,36 @@ static void media_remove(or_sim *s, int c, long r) {
     gr->n = w;
 }
 
+/* A planted mistake in the pole coefficients, for tests that show a check fails on it: 0 none,
+ * 1 the damping term of xi with its sign flipped, 2 alpha without its denominator, 3 the damping
+ * rate doubled in every coefficient, 4 chi (E^(n-1)'s weight in the update) halved. */
+static int media_plant = 0;
+void or_media_plant(int code) { media_plant = code; }
+
+/* Pole::coefficients (media.rs:101-109) from Pole::parts (media.rs:74-87): a pole is
+ * (0, strength, f0, damping) for a Lorentz term a / (w0^2 - w^2 - i G w) with a = strength w0^2,
+ * or (1, fp, damping, 0) for a Drude term (a = wp^2, w0 = 0); frequencies in c/um, w = 2 pi f.
+ * d = 1 + G dt / 2, alpha = (2 - w0^2 dt^2) / d, xi = (G dt / 2 - 1) / d, gamma = a dt^2 / d. */
+void or_pole_coefficients(const double *p, double dt, double out[3]) {
+    double a, w0sq, gamma;
+    if (p[0] == 0.0) {
+        const double w0 = 6.283185307179586 * p[2];
+        a = p[1] * w0 * w0;
+        w0sq = w0 * w0;
+        gamma = 6.283185307179586 * p[3];
+    } else {
+        const double wp = 6.283185307179586 * p[1];
+        a = wp * wp;
+        w0sq = 0.0;
+        gamma = 6.283185307179586 * p[2];
+    }
+    if (media_plant == 3) gamma = 2.0 * gamma;
+    const double d = 1.0 + gamma * dt / 2.0;
+    out[0] = media_plant == 2 ? 2.0 - w0sq * dt * dt : (2.0 - w0sq * dt * dt) / d;
+    out[1] = media_plant == 1 ? (-gamma * dt / 2.0 - 1.0) / d : (gamma * dt / 2.0 - 1.0) / d;
+    out[2] = a * dt * dt / d;
+}
+
 int or_sim_add_medium(or_sim *s, double eps_inf, int np, const double *poles,
                       const or_scene *inside_scene) {
     if (s->steps > 0) { fail("a dispersive medium goes in before the run starts"); return -1; }
@@ -673,26 +703,10 @@ int or_sim_add_medium(or_sim *s, double eps_inf, int np, const double *poles,
     double *co = malloc((np ? np : 1) * 3 * sizeof(double));
     double chi_sum = 0.0;
     for (int q = 0; q < np; q++) {
-        const double *p = poles + 4 * q;
-        double a, w0sq, gamma;
-        if (p[0] == 0.0) {
-            const double w0 = 6.283185307179586 * p[2];
-            a = p[1] * w0 * w0;
-            w0sq = w0 * w0;
-            gamma = 6.283185307179586 * p[3];
-        } else {
-            const double wp = 6.283185307179586 * p[1];
-            a = wp * wp;
-            w0sq = 0.0;
-            gamma = 6.283185307179586 * p[2];
-        }
-        const double d = 1.0 + gamma * dt / 2.0;
-        co[3 * q] = (2.0 - w0sq * dt * dt) / d;
-        co[3 * q + 1] = (gamma * dt / 2.0 - 1.0) / d;
-        co[3 * q + 2] = a * dt * dt / d;
+        or_pole_coefficients(poles + 4 * q, dt, co + 3 * q);
         chi_sum += co[3 * q + 2];
     }
-    const double chi = chi_sum / (4.0 * dt);
+    const double chi = media_plant == 4 ? chi_sum / (8.0 * dt) : chi_sum / (4.0 * dt);
     state_64 *st = &s->st64;
     const or_grid *g = &s->g;
     long cap = 1024, n = 0;

@@ -47,6 +47,8 @@ double or_sin(double x);
 double or_cos(double x);
 double or_powi(double x, int n);
 double or_ln(double x);
+void or_pole_coefficients(const double *pole, double dt, double out[3]);
+void or_media_plant(int code);
 double or_pow(double x, double y);
 or_c64 or_cexp(or_c64 z);
 

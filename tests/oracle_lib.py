@@ -86,6 +86,8 @@ for name, res, args in [
     ("or_sincos", None, [C.c_double, pd, pd]),
     ("or_powi", C.c_double, [C.c_double, C.c_int]),
     ("or_ln", C.c_double, [C.c_double]),
+    ("or_pole_coefficients", None, [pd, C.c_double, pd]),
+    ("or_media_plant", None, [C.c_int]),
     ("or_pow", C.c_double, [C.c_double, C.c_double]),
     ("or_waveform_at", C.c_double, [C.POINTER(Wave), C.c_double]),
     ("or_cpu_rate_f32", C.c_double, [C.c_int] * 7),
