@@ -88,3 +88,41 @@ class DesignProblem:
     def chain(self, g_eps):
         """dJ/drho per design cell: the sum over the cell's three E values times eps_Si - eps_SiO2."""
         return np.asarray(g_eps).reshape(-1, 3).sum(axis=1).reshape(self.shape) * (EPS_SI - EPS_OX)
+
+
+class ModeDesignProblem(DesignProblem):
+    """The same design problem with the objective the adjoint workload names: J = |a|^2, a the
+    amplitude of the strip guide's fundamental (this build's port_modes, solved in a window of
+    2 window[0] x 2 window[1] nodes around the guide) going forward on the output plane, as the
+    mode monitor measures it; the adjoint takes the mode's Lorentz form as weights on the three
+    E components over the mode's two planes (Simulation.mode_weights)."""
+
+    def __init__(self, gpu, window=None, **kw):
+        super().__init__(gpu, **kw)
+        self.window = window
+
+    def mode(self, s):
+        nx, ny, nz = self.n
+        i_mon = nx - self.pml - 3
+        win = (None, None)
+        if self.window:
+            win = ((ny // 2 - self.window[0], ny // 2 + self.window[0]),
+                   (nz // 2 - self.window[1], nz // 2 + self.window[1]))
+        return s.port_modes(0, i_mon, 1 / 1.55, window=win)[0]
+
+    def build(self, rho):
+        from photonoxide_amd.adjoint import Adjoint
+        s = self.simulation(rho)
+        m = self.mode(s)
+        freq = s.mode_frequencies(s.add_mode_monitor([m]))[0]
+        lo, hi = s.mode_box(m)
+        d = s.add_dft(lo, hi, [freq])
+        adj = Adjoint(s, monitor=d, component=-1, frequency=0, weights=s.mode_weights(m, 0),
+                      design=self.design_values(), checkpoint=self.checkpoint)
+        return s, adj, self.steps
+
+    def objective(self, rho):
+        s = self.simulation(rho)
+        mon = s.add_mode_monitor([self.mode(s)])
+        s.run(self.steps)
+        return float(abs(s.mode_amplitudes(mon)[0][0]) ** 2)

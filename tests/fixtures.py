@@ -112,6 +112,8 @@ def oracle(spec, with_fields=True):
     if spec.get("dft"):
         lo, hi, fr = spec["dft"]
         s.add_dft(lo, hi, fr)
+    for lo, hi, fr in spec.get("dfts", []):
+        s.add_dft(lo, hi, fr)
     return s
 
 
@@ -155,6 +157,8 @@ def product(spec, with_fields=True):
         s.add_probe(f, c, at)
     if spec.get("dft"):
         lo, hi, fr = spec["dft"]
+        s.add_dft(lo, hi, fr)
+    for lo, hi, fr in spec.get("dfts", []):
         s.add_dft(lo, hi, fr)
     return s
 
