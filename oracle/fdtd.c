@@ -899,3 +899,89 @@ double or_sim_electric_energy(const or_sim *s) {
         }
     return 0.5 * sum * volume;
 }
+
+/* ------------------------------------------------------------ dipoles (sources.rs:242-315) */
+/* Simulation::add_dipole's restriction to the grid: along each axis that varies (not one cell
+ * with a periodic side), the position s = (x - first) / step from the component's first value
+ * (half a step on along the component for E, across it for H), the two values floor(s) and
+ * floor(s) + 1 (wrapped on a periodic axis, the last one's twin clamped at a wall) with weights
+ * 1 - f and f; each of the 8 products wx wy wz that is not zero adds a (w / volume) to its value
+ * (volume the product of the varying axes' steps), values in index order as a BTreeMap keeps
+ * them. Writes up to 8 values (index, amplitude re, im); returns their count, or -1 for a
+ * position outside the component's values along a walled axis or one that isn't finite. */
+static int cmp_i64(const void *a, const void *b) {
+    const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return x < y ? -1 : (x > y);
+}
+
+int or_dipole_values(const or_grid *g, const or_boundaries *b, int field, int comp, const double pos[3],
+                     double amp_re, double amp_im, int64_t *index_out, double *amp_out) {
+    if (!isfinite(amp_re) || !isfinite(amp_im) || !isfinite(pos[0]) || !isfinite(pos[1]) || !isfinite(pos[2])) {
+        fail("a dipole must be finite");
+        return -1;
+    }
+    const int n[3] = {g->nx, g->ny, g->nz};
+    const double h[3] = {g->dx, g->dy, g->dz}, o[3] = {g->x0, g->y0, g->z0};
+    int64_t at[3][2];
+    double w[3][2];
+    double volume = 1.0;
+    for (int a = 0; a < 3; a++) {
+        if (n[a] == 1 && b->periodic[a]) {
+            at[a][0] = at[a][1] = 0;
+            w[a][0] = 1.0;
+            w[a][1] = 0.0;
+            continue;
+        }
+        volume *= h[a];
+        const int half = field == 0 ? comp == a : comp != a;
+        const double first = o[a] + 0.0 * h[a] + (half ? 0.5 * h[a] : 0.0);
+        const double s = (pos[a] - first) / h[a];
+        if (!b->periodic[a] && !(s >= 0.0 && s <= (double)(n[a] - 1))) {
+            fail("the dipole is outside its component's values along an axis");
+            return -1;
+        }
+        const double m = floor(s), f = s - m;
+        const int64_t mi = (int64_t)m;
+        if (b->periodic[a]) {
+            at[a][0] = ((mi % n[a]) + n[a]) % n[a];
+            at[a][1] = (((mi + 1) % n[a]) + n[a]) % n[a];
+        } else {
+            at[a][0] = mi;
+            at[a][1] = mi + 1 < n[a] - 1 ? mi + 1 : n[a] - 1;
+        }
+        w[a][0] = 1.0 - f;
+        w[a][1] = f;
+    }
+    int64_t keys[8];
+    double re[8], im[8];
+    int count = 0;
+    for (int x = 0; x < 2; x++)
+        for (int y = 0; y < 2; y++)
+            for (int z = 0; z < 2; z++) {
+                const double wv = w[0][x] * w[1][y] * w[2][z];
+                if (wv == 0.0) continue;
+                const int64_t r = (at[2][z] * n[1] + at[1][y]) * n[0] + at[0][x];
+                const double q = wv / volume;
+                int e = 0;
+                while (e < count && keys[e] != r) e++;
+                if (e == count) {
+                    keys[count] = r;
+                    re[count] = 0.0;
+                    im[count] = 0.0;
+                    count++;
+                }
+                re[e] += amp_re * q - amp_im * 0.0;
+                im[e] += amp_re * 0.0 + amp_im * q;
+            }
+    int64_t order[8];
+    for (int e = 0; e < count; e++) order[e] = keys[e];
+    qsort(order, (size_t)count, sizeof(int64_t), cmp_i64);
+    for (int e = 0; e < count; e++) {
+        int q = 0;
+        while (keys[q] != order[e]) q++;
+        index_out[e] = order[e];
+        amp_out[2 * e] = re[q];
+        amp_out[2 * e + 1] = im[q];
+    }
+    return count;
+}

@@ -99,6 +99,8 @@ or_sim *or_sim_new(const or_grid *g, const or_boundaries *b, const or_scene *eps
 or_sim *or_sim_new_eps(const or_grid *g, const or_boundaries *b, const double *eps3n,
                        double courant);
 void or_sim_free(or_sim *s);
+int or_dipole_values(const or_grid *g, const or_boundaries *b, int field, int comp, const double pos[3],
+                     double amp_re, double amp_im, int64_t *index_out, double *amp_out);
 int or_sim_conductivity(or_sim *s, const or_scene *sigma);
 void or_sim_conductor(or_sim *s, int comp, int i, int j, int k);
 int or_sim_add_source(or_sim *s, int field, int comp, int i, int j, int k,

@@ -251,6 +251,25 @@ static int default_rows_2d(int nx, int ny) {
 
 static int pitch_of(int nx) { return (nx + 63) / 64 * 64; }
 
+// The CPML's psi values per copy: each axis side's slab holds 2 fields x 2 components.
+static long long psi_values(const pxfdtd_problem *p) {
+    long long n = 0;
+    for (int a = 0; a < 3; a++) n += 4 * slab_volume(p, a, p->pml_low[a] + p->pml_high[a]);
+    return n;
+}
+
+// Whether step kernel `kind` addresses the buffers of a problem with nval values per field
+// component and psi_len psi values, esz bytes each: kinds 1 and 2 read each component through a
+// buffer resource of 32-bit byte offsets, with a wall's neighbour at OUTSIDE (2^31) beyond it;
+// kinds 3 and 4 read a field's three components, and psi, the same way; kind 0 indexes by
+// element (int) from plain pointers. (Every kind needs nval < 2^30 and psi_len < 2^31, checked
+// at creation.)
+static bool kernel_fits(int kind, long long nval, long long psi_len, long long esz) {
+    if (kind == 1 || kind == 2) return nval * esz < (1LL << 31);
+    if (kind == 3 || kind == 4) return 3 * nval * esz < (1LL << 31) && psi_len * esz < (1LL << 31);
+    return true;
+}
+
 extern "C" int pxfdtd_resident_footprint(const pxfdtd_problem *p, size_t *bytes) {
     TRY(check_problem(p));
     const size_t esz = p->precision == PXFDTD_F64 ? 8 : 4;
@@ -500,8 +519,8 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
         }
     }
     r->psi_len = off;
-    if (r->kernel >= 3 && off * (long long)sizeof(T) >= (1LL << 31)) r->kernel = 1;  // psi beyond 32-bit offsets
-    if (off >= (1LL << 31)) return px_fail(PXFDTD_ERR_UNSUPPORTED, "the CPML's psi is beyond 32-bit indexing");
+    if (!kernel_fits(r->kernel, r->nval, off, (long long)sizeof(T)))
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "kernel %d cannot address %lld psi values", r->kernel, off);
     for (int q = 0; q < 2; q++) {
         TRY(r->psi[q].alloc(std::max<long long>(off, 1) * sizeof(T)));
         HIPCHK(hipMemsetAsync(r->psi[q].p, 0, std::max<long long>(off, 1) * sizeof(T), s));
@@ -576,25 +595,35 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
         r->step[a] = p->step[a];
     }
     r->dt = p->dt;
-    if (r->nval * 2 >= (1LL << 31))
-        return px_fail(PXFDTD_ERR_UNSUPPORTED, "a grid of %lld values per component is beyond 32-bit indexing", r->nval);
-    // defaults: the aligned march where a field's three components fit its 32-bit buffer
-    // offsets (else the 62-wide one), 2 rows per strip, and plane chunks short enough that the
-    // chip sees several rounds of wavefronts (8 planes, fewer on small grids)
+    if (r->nval * 2 >= (1LL << 31)) {
+        const long long nval = r->nval;
+        delete r;
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "a grid of %lld values per component is beyond 32-bit indexing", nval);
+    }
+    const long long psi_len = psi_values(p);
+    if (psi_len >= (1LL << 31)) {
+        delete r;
+        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the CPML's %lld psi values are beyond 32-bit indexing", psi_len);
+    }
+    // defaults: the aligned march (the 2D march for a grid one cell thick with z periodic)
+    // where its 32-bit buffer offsets reach every value, else the 62-wide march, else the LDS
+    // tiles; 2 rows per strip, and plane chunks short enough that the chip sees several rounds
+    // of wavefronts (8 planes, fewer on small grids)
     r->ty = 2;
-    r->kernel = 3 * r->nval * (long long)r->esz < (1LL << 31) ? 3 : 1;
-    if (r->kernel == 3 && p->nz == 1 && p->periodic[2]) r->kernel = 4;  // a 2D grid
+    const int first = p->nz == 1 && p->periodic[2] ? 4 : 3;
+    r->kernel = kernel_fits(first, r->nval, psi_len, (long long)r->esz) ? first
+                : (kernel_fits(1, r->nval, psi_len, (long long)r->esz) ? 1 : 0);
+    if (r->kernel == 0) r->ty = 8;
     r->concurrent = r->kernel == 3 ? 1 : 0;
-    {
-        const long long strips = (long long)((p->nx + (r->kernel == 3 ? SW : RW) - 1) / (r->kernel == 3 ? SW : RW)) *
-                                 ((p->ny + r->ty - 1) / r->ty);
+    if (r->kernel != 4) {
+        const int width = r->kernel == 3 ? SW : (r->kernel == 0 ? TX : RW);
+        const long long strips = (long long)((p->nx + width - 1) / width) * ((p->ny + r->ty - 1) / r->ty);
         int kz = std::min(p->nz, 8);
         while (kz > 2 && strips * ((p->nz + kz - 1) / kz) < 4096) kz = (kz + 1) / 2;
         r->kz = std::max(1, kz);
-        if (r->kernel == 4) {
-            r->ty = 1;
-            r->kz = default_rows_2d(p->nx, p->ny);
-        }
+    } else {
+        r->ty = 1;
+        r->kz = default_rows_2d(p->nx, p->ny);
     }
     hipError_t e = hipStreamCreate(&r->stream);
     if (e != hipSuccess) {
@@ -1202,8 +1231,10 @@ extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
     if (kind < 0 || kind > 4) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0 to 4");
     if (kind == 4 && !(r->nz == 1 && r->periodic[2]))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the 2D march steps grids one cell thick with z periodic");
-    if (kind >= 3 && (3 * r->nval * (long long)r->esz >= (1LL << 31) || r->psi_len * (long long)r->esz >= (1LL << 31)))
-        return px_fail(PXFDTD_ERR_UNSUPPORTED, "the aligned march needs a field's three components within 2 GiB");
+    if (!kernel_fits(kind, r->nval, r->psi_len, (long long)r->esz))
+        return px_fail(PXFDTD_ERR_UNSUPPORTED,
+                       kind >= 3 ? "kernel %d needs a field's three components and psi each within 2 GiB"
+                                 : "kernel %d needs each field component within 2 GiB", kind);
     r->kernel = kind;
     r->ty = kind == 0 ? 8 : (kind == 4 ? 1 : 2);
     if (kind == 4) r->kz = default_rows_2d(r->nx, r->ny);

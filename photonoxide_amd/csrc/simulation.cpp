@@ -468,23 +468,40 @@ extern "C" int pxfdtd_simulation_conductivity(pxfdtd_simulation *s, const pxfdtd
     if (!s->groups.empty())
         return px_fail(PXFDTD_ERR_INVALID,
                        "the conductivity comes before the dispersive media: conductivity, then medium");
+    // cb is dt/eps until the conductivity is set, so it is set once (the reference's builder
+    // would compound a second conductivity into cb while ca took only the second)
+    if (!s->sigma.empty())
+        return px_fail(PXFDTD_ERR_INVALID, "the conductivity is set once, for the whole grid");
     TRY(sync_host(s));
     const long long N = s->N;
-    for (int c = 0; c < 3; c++)
-        for (long long r = 0; r < N; r++) {
-            double &cb = s->cb[c * N + r];
-            if (cb == 0.0) continue;
-            double p[3];
-            s->e_position(c, r, p);
-            const double sg = scene_value(sigma, 0, p[0], p[1], p[2]);
-            if (!(std::isfinite(sg) && sg >= 0))
-                return px_fail(PXFDTD_ERR_INVALID, "the conductivity must be finite and >= 0, got %g", sg);
-            const double half = sg * cb / 2.0;
-            s->ca[c * N + r] = (1.0 - half) / (1.0 + half);
-            cb /= 1.0 + half;
-            if (s->sigma.empty()) s->sigma.assign((size_t)(3 * N), 0.0);
-            s->sigma[(size_t)(c * N + r)] = sg;
+    // every value's conductivity first, so that a refused one leaves the simulation as it was
+    std::vector<double> given((size_t)(3 * N), 0.0);
+    int bad = 0;
+    double bad_value = 0.0;
+#pragma omp parallel for schedule(static) if (N > (1 << 16))
+    for (long long q = 0; q < 3 * N; q++) {
+        if (s->cb[q] == 0.0) continue;
+        double p[3];
+        s->e_position((int)(q / N), q % N, p);
+        const double sg = scene_value(sigma, 0, p[0], p[1], p[2]);
+        given[(size_t)q] = sg;
+        if (!(std::isfinite(sg) && sg >= 0)) {
+#pragma omp critical
+            {
+                bad = 1;
+                bad_value = sg;
+            }
         }
+    }
+    if (bad) return px_fail(PXFDTD_ERR_INVALID, "the conductivity must be finite and >= 0, got %g", bad_value);
+    for (long long q = 0; q < 3 * N; q++) {
+        double &cb = s->cb[q];
+        if (cb == 0.0) continue;
+        const double half = given[(size_t)q] * cb / 2.0;
+        s->ca[q] = (1.0 - half) / (1.0 + half);
+        cb /= 1.0 + half;
+    }
+    s->sigma = std::move(given);
     drop_resident(s);
     return PXFDTD_OK;
 }

@@ -49,6 +49,10 @@ class Wave(C.Structure):
                 ("ramp", C.c_double)]
 
 
+class C64(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
 vp = C.c_void_p
 for name, res, args in [
     ("or_last_error", C.c_char_p, []),
@@ -88,8 +92,11 @@ for name, res, args in [
     ("or_ln", C.c_double, [C.c_double]),
     ("or_pole_coefficients", None, [pd, C.c_double, pd]),
     ("or_media_plant", None, [C.c_int]),
+    ("or_dipole_values", C.c_int, [C.POINTER(Grid), C.POINTER(Bounds), C.c_int, C.c_int, pd, C.c_double, C.c_double,
+                                   C.POINTER(C.c_int64), pd]),
     ("or_pow", C.c_double, [C.c_double, C.c_double]),
     ("or_waveform_at", C.c_double, [C.POINTER(Wave), C.c_double]),
+    ("or_waveform_complex_at", C64, [C.POINTER(Wave), C.c_double]),
     ("or_cpu_rate_f32", C.c_double, [C.c_int] * 7),
 ]:
     f = getattr(lib, name)
@@ -247,6 +254,19 @@ class Sim:
         out = np.zeros(n)
         lib.or_sim_plane_aux(self.h, p, field, comp, out.ctypes.data_as(pd))
         return out
+
+
+def dipole_values(grid, b, field, comp, position, amp):
+    """The oracle's restriction of a dipole to the grid: (indices, complex amplitudes)."""
+    g = Grid(*grid)
+    idx = np.zeros(8, np.int64)
+    a = np.zeros(16)
+    pos = np.ascontiguousarray(position, dtype=np.float64)
+    n = lib.or_dipole_values(C.byref(g), C.byref(b), field, comp, pos.ctypes.data_as(pd), complex(amp).real,
+                             complex(amp).imag, idx.ctypes.data_as(C.POINTER(C.c_int64)), a.ctypes.data_as(pd))
+    if n < 0:
+        raise ValueError(lib.or_last_error().decode())
+    return idx[:n].copy(), a[0:2 * n:2] + 1j * a[1:2 * n:2]
 
 
 def wave(kind=0, frequency=1.0, width=0.6, delay=1.8, amp=1 + 0j, ramp=0.0):

@@ -122,3 +122,36 @@ def test_bytes_per_step_is_the_references_model():
     want = 24 * cells + 6 * psi[0] + 5 * psi[1]
     assert p.bytes_per_step(P.Precision.F32) == 4 * want
     assert p.bytes_per_step(P.Precision.F64) == 8 * want
+
+
+def test_a_refused_conductivity_leaves_the_simulation_as_it_was():
+    """src/fdtd.rs:949-992: a conductivity that is negative or not finite anywhere is refused;
+    here the simulation keeps its coefficients (no value half-converted), and takes a good one
+    afterwards."""
+    import photonoxide_amd as P
+    from photonoxide_amd import _native as N
+    g = P.Grid3d(12, 10, 8, 0.05, 0.05, 0.05, -0.3, -0.25, -0.2)
+    s = P.Simulation.new(g, P.Scene(2.0), P.Boundaries.cpml_all(2), 0.9)
+    before = [s.coefficients(c) for c in range(3)]
+    with pytest.raises(N.NativeError):
+        s.with_conductivity(P.Scene(1.0, [P.Box((0.1, -INF, -INF), (INF, INF, INF), -0.5)]))
+    for c in range(3):
+        ca, cb = s.coefficients(c)
+        assert np.array_equal(ca, before[c][0]) and np.array_equal(cb, before[c][1]), c
+    s.with_conductivity(P.Scene(1.0))
+    assert not np.array_equal(s.coefficients(0)[1], before[0][1])
+
+
+def test_the_conductivity_is_set_once():
+    """A second conductivity would divide cb again while ca took only the second value (the
+    reference's builder would compound them): refused, the first kept."""
+    import photonoxide_amd as P
+    from photonoxide_amd import _native as N
+    g = P.Grid3d(12, 10, 8, 0.05, 0.05, 0.05, -0.3, -0.25, -0.2)
+    s = P.Simulation.new(g, P.Scene(2.0), P.Boundaries.cpml_all(2), 0.9)
+    s.with_conductivity(P.Scene(1.0))
+    once = [s.coefficients(c) for c in range(3)]
+    with pytest.raises(N.NativeError):
+        s.with_conductivity(P.Scene(0.5))
+    for c in range(3):
+        assert np.array_equal(s.coefficients(c)[0], once[c][0]) and np.array_equal(s.coefficients(c)[1], once[c][1])
