@@ -391,7 +391,11 @@ def main():
     bytes_step, psi = algorithmic_bytes_per_step(nx, ny, nz, pml, {0: 1, 1: 24, 2: 12}[mode])
     media_bytes = ade_bytes_per_step(s) if args.workload == "dispersive" else 0
     bytes_step += media_bytes
-    achieved = bytes_step / (kernel_ms * 1e-3) / 1e9
+    # the two-step march (kernel 5) takes two steps per pass over the same bytes as one step:
+    # a launch is a pair, and kernel_ms is its duration per step
+    steps_per_launch = 2 if s.kernel() == 5 else 1
+    launch_ms = kernel_ms * steps_per_launch
+    achieved = bytes_step / (launch_ms * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_march64_step_mmi_f32.json")
     if os.path.exists(tf) and args.workload == "mmi":
@@ -416,14 +420,17 @@ def main():
         "config": {"workload": (f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else "")
                                 if args.workload == "mmi" else
                                 "dispersive_512x512x256_dx15nm_40si_lorentz2_10au_drude_lorentz2_cpml16_f32"),
-                   "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch", "aligned_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": {0: "uint8 ids", 1: "ca, cb arrays", 2: "cb arrays"}[mode],
+                   "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch", "aligned_march", "march_2d",
+                              "two_step_march"][s.kernel()], "tiling": list(s.tiling()), "coefficients": {0: "uint8 ids", 1: "ca, cb arrays", 2: "cb arrays"}[mode],
                    "monitors": ("2 ports x 6 components x 50x30 values x 21 frequencies, FP64 sums"
                                 if args.workload == "mmi" else
                                 "a plane of 6 components x 480x224 values x 5 frequencies, FP64 sums"),
                    "psi_values": psi},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "kernel": ["fused_step", "march_step", "march_step", "march64_step"][s.kernel()], "kernel_ms": round(kernel_ms, 5),
+                     "kernel": ["fused_step", "march_step", "march_step", "march64_step", "march2d_step",
+                                "march2s_step"][s.kernel()],
+                     "kernel_ms": round(launch_ms, 5), "steps_per_launch": steps_per_launch,
                      "bytes_per_launch": bytes_step,
                      "gcell_per_s_kernel": round(cells / (kernel_ms * 1e-3) / 1e9, 3),
                      # the reference's own byte model (two-pass, f32 ca/cb arrays), for comparison

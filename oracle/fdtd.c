@@ -796,7 +796,7 @@ void or_sim_use_f32(or_sim *s) {
         b->cb[c] = to_f32(a->cb[c], N);
         b->fe[c] = to_f32(a->fe[c], s->n[c]);
         b->fh[c] = to_f32(a->fh[c], s->n[c]);
-        b->step[c] = (float)a->step[c];
+        b->step[c] = (float)(1.0 / a->step[c]); /* f32 psi multiplies by 1/step (yee.wgsl) */
     }
     b->dt = (float)a->dt;
     for (int q = 0; q < s->nslabs; q++) {

@@ -25,8 +25,10 @@
  *   - f32 stepping: the same arithmetic rounded to f32 (src/fdtd/kernel.rs `Real for f32`),
  *     checked against the f64 run to the reference's own growth law 2e-8 per step
  *     (src/fdtd/kernel_tests.rs:235-258).
- * Parity unpinned: the CPML profile's pow and ln use the C library's (the reference's are
- * fdlibm/musl ports); no golden vector of the reference covers CPML bits.
+ *   - the CPML profile's pow and ln: fdlibm ports (portable.c), pinned by the reference's grid
+ *     hashes 0x0bf1e9d3216efc53 and 0x9133a6eca7e0bb06 (glibc's pow and log give other bits);
+ *   - f32 psi terms multiply by the host's 1/step rounded to f32, as the reference's device
+ *     kernel does (src/fdtd/gpu/yee.wgsl).
  */
 #ifndef PXFDTD_ORACLE_H
 #define PXFDTD_ORACLE_H

@@ -39,7 +39,7 @@ template <typename T>
 struct AdjArgs {
     AdjGrid g;
     T dt;
-    T step[3];
+    T step[3];  // f64: dx, dy, dz; f32: their reciprocals (see per_step)
     const T *coef;  // the forward's factors and CPML b, a
     const T *ca, *cb;  // [3][nval], pitched
     T *eb, *hb;        // Ebar, Hbar [3][nval]
@@ -170,13 +170,13 @@ __global__ __launch_bounds__(ADJ_BX *ADJ_BY) void adj_h(AdjArgs<T> A) {
             const T s = t == 0 ? T(1) : T(-1);
             const T g1 = up[w] >= 0 ? cb1[c][t] * fe[1][w] * eb1[c][t] : T(0);
             T q1 = T(0);
-            if (pi[1][c][t] >= 0) q1 = aq[1][w] * (pb[1][c][t] + s * cb1[c][t] * eb1[c][t]) / A.step[w];
+            if (pi[1][c][t] >= 0) q1 = per_step(aq[1][w] * (pb[1][c][t] + s * cb1[c][t] * eb1[c][t]), A.step[w]);
             const T g0 = cb0[c] * fe[0][w] * eb0[c];
             T q0 = T(0);
             if (pi[0][c][t] >= 0) {
                 const T tot = pb[0][c][t] + s * cb0[c] * eb0[c];
                 A.pb_out[pi[0][c][t]] = bq[0][w] * tot;
-                q0 = aq[0][w] * tot / A.step[w];
+                q0 = per_step(aq[0][w] * tot, A.step[w]);
             }
             acc[o] = acc[o] - s * (g1 - g0);
             acc[o] = acc[o] - (q1 - q0);
@@ -250,11 +250,11 @@ __global__ __launch_bounds__(ADJ_BX *ADJ_BY) void adj_e(AdjArgs<T> A) {
             if (pi[0][c][t] >= 0) {
                 const T tot = pb[0][c][t] - A.dt * s * hb0[c];
                 A.pb_out[pi[0][c][t]] = bq[0][w] * tot;
-                q0 = aq[0][w] * tot / A.step[w];
+                q0 = per_step(aq[0][w] * tot, A.step[w]);
             }
             const T h1 = down[w] >= 0 ? fh[1][w] * hb1[c][t] : T(0);
             T q1 = T(0);
-            if (pi[1][c][t] >= 0) q1 = aq[1][w] * (pb[1][c][t] - A.dt * s * hb1[c][t]) / A.step[w];
+            if (pi[1][c][t] >= 0) q1 = per_step(aq[1][w] * (pb[1][c][t] - A.dt * s * hb1[c][t]), A.step[w]);
             acc[o] = acc[o] + A.dt * s * (h0 - h1);
             acc[o] = acc[o] - (q0 - q1);
         }

@@ -56,7 +56,7 @@ static AdjArgs<T> adj_args(pxfdtd_adjoint *a) {
         g.high[w] = r->high[w];
         g.fe_off[w] = r->fe_off[w];
         g.fh_off[w] = r->fh_off[w];
-        A.step[w] = (T)r->step[w];
+        A.step[w] = sizeof(T) == 4 ? (T)(1.0 / r->step[w]) : (T)r->step[w];  // see per_step
     }
     memcpy(g.psi_off, r->psi_off, sizeof g.psi_off);
     memcpy(g.cpml_off, r->cpml_off, sizeof g.cpml_off);

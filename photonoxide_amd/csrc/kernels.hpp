@@ -101,7 +101,7 @@ struct StepArgs {
     const T *cacb;
     const int8_t *grp;
     T dt;
-    T step[3];
+    T step[3];                // f64: dx, dy, dz; f32: 1/dx, 1/dy, 1/dz (see per_step)
     // kappa/step at the nodes (fe) and halves (fh) along x, y, z: fe_x, fh_x, fe_y, fh_y,
     // fe_z, fh_z, n_axis values each; then the CPML's b and a
     const T *coef;
@@ -110,6 +110,19 @@ struct StepArgs {
     const StepCold<T> *cold;  // the one for this step's parity
     int wave_step;            // this step's row of cold->waves
 };
+
+// The CPML's psi takes a field difference over the step. The reference's CPU loops divide
+// (src/fdtd/kernel/reference.rs:102-180); its GPU kernel multiplies by the host's 1/step rounded
+// to the working precision (src/fdtd/gpu/yee.wgsl update_h/update_e, with reals from
+// src/fdtd/gpu/resident.rs:170). f64 here does the first (the CPU path's bits), f32 the second
+// (one multiplication in place of an IEEE division): StepArgs::step holds the steps in f64 and
+// their reciprocals in f32.
+template <typename T>
+__device__ __forceinline__ T per_step(T d, T s);
+template <>
+__device__ __forceinline__ float per_step<float>(float d, float s) { return d * s; }
+template <>
+__device__ __forceinline__ double per_step<double>(double d, double s) { return d / s; }
 
 // A workgroup barrier that waits only for this wave's LDS operations, not for its global
 // loads: the loads issued for the next plane stay in flight across it. (__syncthreads also
@@ -177,7 +190,7 @@ __device__ __forceinline__ T psi_at(const StepArgs<T> &A, const StepCold<T> &K, 
 template <typename T, int FIELD>
 __device__ __forceinline__ void apply_injection(const StepArgs<T> &A, const StepCold<T> &K,
                                                 const Injection &in, int c, T &v0, T &v1, T &v2,
-                                                T cb0, T cb1, T cb2) {
+                                                T cb0, T cb1, T cb2, int ws) {
     if (in.emitter < 0) {
         const T f = T(K.aux[-1LL - in.emitter]);
         const T wt = T(in.a_re);
@@ -193,7 +206,7 @@ __device__ __forceinline__ void apply_injection(const StepArgs<T> &A, const Step
         }
         return;
     }
-    const double *w = K.waves + 2 * ((long long)A.wave_step * K.emitters + in.emitter);
+    const double *w = K.waves + 2 * ((long long)ws * K.emitters + in.emitter);
     const T amount = T(in.a_re * w[0] - in.a_im * w[1]);
     if (FIELD == 1) {
         const T d = A.dt * amount;
@@ -208,18 +221,25 @@ __device__ __forceinline__ void apply_injection(const StepArgs<T> &A, const Step
 }
 
 // the injections of plane k on the value (field, i, j) in `mask`, in the tile's list order
+// the sources, currents and couplings of tile `tile`'s list at value (i, j, k), wave row ws
 template <typename T, int FIELD>
-__device__ __forceinline__ void inject(const StepArgs<T> &A, const StepCold<T> &K, int tile, int i,
-                                       int j, int k, int mask, T &v0, T &v1, T &v2, T cb0, T cb1,
-                                       T cb2) {
+__device__ __forceinline__ void inject_ws(const StepArgs<T> &A, const StepCold<T> &K, int tile, int ws, int i,
+                                          int j, int k, int mask, T &v0, T &v1, T &v2, T cb0, T cb1,
+                                          T cb2) {
     const int32_t *off = K.inj_off + (long long)tile * (A.nz + 1);
     const int lo = off[k], hi = off[k + 1];
     for (int q = lo; q < hi; q++) {
         const Injection in = K.inj[q];
         const int c = in.code & 3;
         if ((in.code >> 2) != FIELD || in.i != i || in.j != j || !((mask >> c) & 1)) continue;
-        apply_injection<T, FIELD>(A, K, in, c, v0, v1, v2, cb0, cb1, cb2);
+        apply_injection<T, FIELD>(A, K, in, c, v0, v1, v2, cb0, cb1, cb2, ws);
     }
+}
+template <typename T, int FIELD>
+__device__ __forceinline__ void inject(const StepArgs<T> &A, const StepCold<T> &K, int tile, int i,
+                                       int j, int k, int mask, T &v0, T &v1, T &v2, T cb0, T cb1,
+                                       T cb2) {
+    inject_ws<T, FIELD>(A, K, tile, A.wave_step, i, j, k, mask, v0, v1, v2, cb0, cb1, cb2);
 }
 
 // the ADE rest of E's update and the polarization currents' step at one value
@@ -477,12 +497,12 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
             if (slabs) {
                 const SlabAt P = slab_at(A, K, oi, oj, k);
                 const bool wr = !pro;
-                if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, dyx / sty, wr));
-                if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, dzx / stz, wr));
-                if (P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, dxy / stx, wr));
-                if (P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, dzy / stz, wr));
-                if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, wr));
-                if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, wr));
+                if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, per_step(dyx, sty), wr));
+                if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, per_step(dzx, stz), wr));
+                if (P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, per_step(dxy, stx), wr));
+                if (P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, per_step(dzy, stz), wr));
+                if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, per_step(dxz, stx), wr));
+                if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, per_step(dyz, sty), wr));
             }
             if (injects) inject<T, 1>(A, K, t, oi, oj, k, 7, v0, v1, v2, T(0), T(0), T(0));
             if (!pro) {
@@ -507,10 +527,10 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
                 T v2 = R.r2 - dt * (dxz * fxh - dyz * fyh);
                 if (slabs) {
                     const SlabAt P = slab_at(A, K, oi, hrj, k);
-                    if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, dyx / sty, false));
-                    if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, dzx / stz, false));
-                    if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, false));
-                    if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, false));
+                    if (P.side[1] >= 0) v0 = v0 - dt * (T(1) * psi_at<1, 0, 1>(A, K, P, per_step(dyx, sty), false));
+                    if (P.side[2] >= 0) v0 = v0 - dt * (T(-1) * psi_at<1, 0, 2>(A, K, P, per_step(dzx, stz), false));
+                    if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, per_step(dxz, stx), false));
+                    if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, per_step(dyz, sty), false));
                 }
                 T v1 = T(0);
                 if (injects) inject<T, 1>(A, K, t, oi, hrj, k, 5, v0, v1, v2, T(0), T(0), T(0));
@@ -531,10 +551,10 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
                 T v2 = R.l2 - dt * (dxz * fxh - dyz * fyh);
                 if (slabs) {
                     const SlabAt P = slab_at(A, K, hci, hcj, k);
-                    if (P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, dxy / stx, false));
-                    if (P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, dzy / stz, false));
-                    if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, dxz / stx, false));
-                    if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, dyz / sty, false));
+                    if (P.side[0] >= 0) v1 = v1 - dt * (T(-1) * psi_at<1, 1, 0>(A, K, P, per_step(dxy, stx), false));
+                    if (P.side[2] >= 0) v1 = v1 - dt * (T(1) * psi_at<1, 1, 2>(A, K, P, per_step(dzy, stz), false));
+                    if (P.side[0] >= 0) v2 = v2 - dt * (T(1) * psi_at<1, 2, 0>(A, K, P, per_step(dxz, stx), false));
+                    if (P.side[1] >= 0) v2 = v2 - dt * (T(-1) * psi_at<1, 2, 1>(A, K, P, per_step(dyz, sty), false));
                 }
                 T v0 = T(0);
                 if (injects) inject<T, 1>(A, K, t, hci, hcj, k, 6, v0, v1, v2, T(0), T(0), T(0));
@@ -579,12 +599,12 @@ __global__ __launch_bounds__(TX *(TY + 1)) void fused_step(StepArgs<T> A) {
             T v2 = ca2 * ec2 + cb2 * (dxz * fxe - dyz * fye);
             if (slabs) {
                 const SlabAt P = slab_at(A, K, oi, oj, k);
-                if (P.side[1] >= 0) v0 = v0 + cb0 * (T(1) * psi_at<0, 0, 1>(A, K, P, dyx / sty, true));
-                if (P.side[2] >= 0) v0 = v0 + cb0 * (T(-1) * psi_at<0, 0, 2>(A, K, P, dzx / stz, true));
-                if (P.side[0] >= 0) v1 = v1 + cb1 * (T(-1) * psi_at<0, 1, 0>(A, K, P, dxy / stx, true));
-                if (P.side[2] >= 0) v1 = v1 + cb1 * (T(1) * psi_at<0, 1, 2>(A, K, P, dzy / stz, true));
-                if (P.side[0] >= 0) v2 = v2 + cb2 * (T(1) * psi_at<0, 2, 0>(A, K, P, dxz / stx, true));
-                if (P.side[1] >= 0) v2 = v2 + cb2 * (T(-1) * psi_at<0, 2, 1>(A, K, P, dyz / sty, true));
+                if (P.side[1] >= 0) v0 = v0 + cb0 * (T(1) * psi_at<0, 0, 1>(A, K, P, per_step(dyx, sty), true));
+                if (P.side[2] >= 0) v0 = v0 + cb0 * (T(-1) * psi_at<0, 0, 2>(A, K, P, per_step(dzx, stz), true));
+                if (P.side[0] >= 0) v1 = v1 + cb1 * (T(-1) * psi_at<0, 1, 0>(A, K, P, per_step(dxy, stx), true));
+                if (P.side[2] >= 0) v1 = v1 + cb1 * (T(1) * psi_at<0, 1, 2>(A, K, P, per_step(dzy, stz), true));
+                if (P.side[0] >= 0) v2 = v2 + cb2 * (T(1) * psi_at<0, 2, 0>(A, K, P, per_step(dxz, stx), true));
+                if (P.side[1] >= 0) v2 = v2 + cb2 * (T(-1) * psi_at<0, 2, 1>(A, K, P, per_step(dyz, sty), true));
             }
             if (injects) inject<T, 0>(A, K, t, oi, oj, k, 7, v0, v1, v2, cb0, cb1, cb2);
             if (g0 >= 0) v0 = media_update(A, K, g0, 0, r, v0, cb0, ec0);
@@ -648,7 +668,7 @@ __device__ __forceinline__ void psi_finish(const StepCold<T> &K, const T *cp, in
                                            T d_zx, T d_xy, T d_zy, T d_xz, T d_yz, T stx, T sty,
                                            T stz, bool write, T u[6]) {
     constexpr int CC[6] = {0, 0, 1, 1, 2, 2}, WW[6] = {1, 2, 0, 2, 0, 1};
-    const T d[6] = {d_yx / sty, d_zx / stz, d_xy / stx, d_zy / stz, d_xz / stx, d_yz / sty};
+    const T d[6] = {per_step(d_yx, sty), per_step(d_zx, stz), per_step(d_xy, stx), per_step(d_zy, stz), per_step(d_xz, stx), per_step(d_yz, sty)};
 #pragma unroll
     for (int n = 0; n < 6; n++) {
         const int c = CC[n], w = WW[n];

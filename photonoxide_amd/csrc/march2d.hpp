@@ -29,7 +29,7 @@ __device__ __forceinline__ void inject2d(const StepArgs<T> &A, const StepCold<T>
         const Injection in = K.inj[q];
         const int c = in.code & 3;
         if ((in.code >> 2) != FIELD || in.i != i || !((mask >> c) & 1)) continue;
-        apply_injection<T, FIELD>(A, K, in, c, v0, v1, v2, cb0, cb1, cb2);
+        apply_injection<T, FIELD>(A, K, in, c, v0, v1, v2, cb0, cb1, cb2, A.wave_step);
     }
 }
 
@@ -193,13 +193,13 @@ __device__ __forceinline__ void march2d_unit(const StepArgs<T> &A, const int4 un
             hz = hz_o - dt * (dxz * fxh - dyz * fyh);
             if (SLABS) {
                 if (xs >= 0 || xsl >= 0) {
-                    const T u = bxh * qxh[1] + axh * (dxz / stx);
+                    const T u = bxh * qxh[1] + axh * (per_step(dxz, stx));
                     hz = xs >= 0 ? hz - dt * (T(1) * u) : hz;
                     if (wr) bstore<T>(u, rPo, xo(1, 2, j), 0);
                 }
                 if (ys >= 0) {
                     const int co = K.cpml_off[1][ys][1] - cbase;
-                    const T u = cp[co + yl] * qyh[1] + cp[co + ycnt + yl] * (dyz / sty);
+                    const T u = cp[co + yl] * qyh[1] + cp[co + ycnt + yl] * (per_step(dyz, sty));
                     hz = hz - dt * (T(-1) * u);
                     if (wr) bstore<T>(u, rPo, vown, yo(1, 2, ys, yl));
                 }
@@ -212,13 +212,13 @@ __device__ __forceinline__ void march2d_unit(const StepArgs<T> &A, const int4 un
             hy = hy_o - dt * (T(0) - dxy * fxh);
             if (SLABS) {
                 if (xs >= 0 || xsl >= 0) {
-                    const T u = bxh * qxh[0] + axh * (dxy / stx);
+                    const T u = bxh * qxh[0] + axh * (per_step(dxy, stx));
                     hy = xs >= 0 ? hy - dt * (T(-1) * u) : hy;
                     if (wr) bstore<T>(u, rPo, xo(1, 1, j), 0);
                 }
                 if (ys >= 0) {
                     const int co = K.cpml_off[1][ys][1] - cbase;
-                    const T u = cp[co + yl] * qyh[0] + cp[co + ycnt + yl] * (dyx / sty);
+                    const T u = cp[co + yl] * qyh[0] + cp[co + ycnt + yl] * (per_step(dyx, sty));
                     hx = hx - dt * (T(1) * u);
                     if (wr) bstore<T>(u, rPo, vown, yo(1, 0, ys, yl));
                 }
@@ -236,10 +236,10 @@ __device__ __forceinline__ void march2d_unit(const StepArgs<T> &A, const int4 un
                 const T dxz = cur.ey - cur.gy, dyz = nxt.gx - cur.gx;
                 T v = ghz - dt * (dxz * fxh_l - dyz * fyh);
                 if (SLABS) {
-                    if (xsl >= 0) v = v - dt * (T(1) * (bxl * hq[1] + axl * (dxz / stx)));
+                    if (xsl >= 0) v = v - dt * (T(1) * (bxl * hq[1] + axl * (per_step(dxz, stx))));
                     if (ys >= 0) {
                         const int co = K.cpml_off[1][ys][1] - cbase;
-                        v = v - dt * (T(-1) * (cp[co + yl] * hq[2] + cp[co + ycnt + yl] * (dyz / sty)));
+                        v = v - dt * (T(-1) * (cp[co + yl] * hq[2] + cp[co + ycnt + yl] * (per_step(dyz, sty))));
                     }
                 }
                 hz_h = v;
@@ -247,7 +247,7 @@ __device__ __forceinline__ void march2d_unit(const StepArgs<T> &A, const int4 un
             if (TM) {
                 const T dxy = cur.ez - cur.gz;
                 T v = ghy - dt * (T(0) - dxy * fxh_l);
-                if (SLABS && xsl >= 0) v = v - dt * (T(-1) * (bxl * hq[0] + axl * (dxy / stx)));
+                if (SLABS && xsl >= 0) v = v - dt * (T(-1) * (bxl * hq[0] + axl * (per_step(dxy, stx))));
                 hy_h = v;
             }
             if (injects && hok) {
@@ -307,12 +307,12 @@ __device__ __forceinline__ void march2d_unit(const StepArgs<T> &A, const int4 un
             if (SLABS && own) {
                 if (xs >= 0) {
                     if (TE) {
-                        const T u = bxe * qxe[0] + axe * (dxy2 / stx);
+                        const T u = bxe * qxe[0] + axe * (per_step(dxy2, stx));
                         ey = ey + cb1 * (T(-1) * u);
                         bstore<T>(u, rPo, xo(0, 1, j), 0);
                     }
                     if (TM) {
-                        const T u = bxe * qxe[1] + axe * (dxz2 / stx);
+                        const T u = bxe * qxe[1] + axe * (per_step(dxz2, stx));
                         ez = ez + cb2 * (T(1) * u);
                         bstore<T>(u, rPo, xo(0, 2, j), 0);
                     }
@@ -321,12 +321,12 @@ __device__ __forceinline__ void march2d_unit(const StepArgs<T> &A, const int4 un
                     const int co = K.cpml_off[1][ys][0] - cbase;
                     const T bb = cp[co + yl], aa = cp[co + ycnt + yl];
                     if (TE) {
-                        const T u = bb * qye[0] + aa * (dyx / sty);
+                        const T u = bb * qye[0] + aa * (per_step(dyx, sty));
                         ex = ex + cb0 * (T(1) * u);
                         bstore<T>(u, rPo, vown, yo(0, 0, ys, yl));
                     }
                     if (TM) {
-                        const T u = bb * qye[1] + aa * (dyz2 / sty);
+                        const T u = bb * qye[1] + aa * (per_step(dyz2, sty));
                         ez = ez + cb2 * (T(-1) * u);
                         bstore<T>(u, rPo, vown, yo(0, 2, ys, yl));
                     }

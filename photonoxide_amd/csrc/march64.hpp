@@ -409,11 +409,11 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                 if (XS && xany) {
                     const bool in = xs >= 0;
                     if (r > 0) {
-                        const T u = bxh * qx[r][0] + axh * (dxy / stx);
+                        const T u = bxh * qx[r][0] + axh * (per_step(dxy, stx));
                         v1 = in ? v1 - dt * (T(-1) * u) : v1;
                         if (wr) bstore<T>(u, rPo, xo(1, 0, r), 0);
                     }
-                    const T u = bxh * qx[r][1] + axh * (dxz / stx);
+                    const T u = bxh * qx[r][1] + axh * (per_step(dxz, stx));
                     v2 = in ? v2 - dt * (T(1) * u) : v2;
                     if (wr) bstore<T>(u, rPo, xo(1, 1, r), 0);
                 }
@@ -421,8 +421,8 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                     const int cnt = ys[r] == 1 ? hy : ly;
                     const int co = K.cpml_off[1][ys[r]][1] - cbase;
                     const T bb = cp[co + yl[r]], aa = cp[co + cnt + yl[r]];
-                    const T u0 = bb * qy[r][0] + aa * (dyx / sty);
-                    const T u2 = bb * qy[r][1] + aa * (dyz / sty);
+                    const T u0 = bb * qy[r][0] + aa * (per_step(dyx, sty));
+                    const T u2 = bb * qy[r][1] + aa * (per_step(dyz, sty));
                     v0 = v0 - dt * (T(1) * u0);
                     v2 = v2 - dt * (T(-1) * u2);
                     if (wr) {
@@ -433,11 +433,11 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                 if (ZS && zs >= 0 && yj[r] >= 0) {
                     const int co = K.cpml_off[2][zs][1] - cbase;
                     const T bb = cp[co + zsl], aa = cp[co + zcnt + zsl];
-                    const T u0 = bb * qz[r][0] + aa * (dzx / stz);
+                    const T u0 = bb * qz[r][0] + aa * (per_step(dzx, stz));
                     v0 = v0 - dt * (T(-1) * u0);
                     if (wr) bstore<T>(u0, rPo, vown, zo(1, 0, r));
                     if (r > 0) {
-                        const T u1 = bb * qz[r][1] + aa * (dzy / stz);
+                        const T u1 = bb * qz[r][1] + aa * (per_step(dzy, stz));
                         v1 = v1 - dt * (T(1) * u1);
                         if (wr) bstore<T>(u1, rPo, vown, zo(1, 1, r));
                     }
@@ -467,18 +467,18 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
             T v2 = ghz - dt * (dxz * fxh_l - dyz * fyh_l);
             if (SLABS) {
                 if (XS && xsl >= 0) {
-                    v1 = v1 - dt * (T(-1) * (bxl * hq[0] + axl * (dxy / stx)));
-                    v2 = v2 - dt * (T(1) * (bxl * hq[1] + axl * (dxz / stx)));
+                    v1 = v1 - dt * (T(-1) * (bxl * hq[0] + axl * (per_step(dxy, stx))));
+                    v2 = v2 - dt * (T(1) * (bxl * hq[1] + axl * (per_step(dxz, stx))));
                 }
                 const int sd = YS && hok ? side_of(gyl, ny, ly, hy) : -1;
                 if (sd >= 0) {
                     const int cnt = sd == 1 ? hy : ly, s = gyl - (sd == 1 ? ny - hy : 0);
                     const int co = K.cpml_off[1][sd][1] - cbase;
-                    v2 = v2 - dt * (T(-1) * (cp[co + s] * hq[2] + cp[co + cnt + s] * (dyz / sty)));
+                    v2 = v2 - dt * (T(-1) * (cp[co + s] * hq[2] + cp[co + cnt + s] * (per_step(dyz, sty))));
                 }
                 if (ZS && zs >= 0) {
                     const int co = K.cpml_off[2][zs][1] - cbase;
-                    v1 = v1 - dt * (T(1) * (cp[co + zsl] * hq[3] + cp[co + zcnt + zsl] * (dzy / stz)));
+                    v1 = v1 - dt * (T(1) * (cp[co + zsl] * hq[3] + cp[co + zcnt + zsl] * (per_step(dzy, stz))));
                 }
             }
             T v0 = T(0);
@@ -534,8 +534,8 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                     const bool wr = j0 - 1 + r < ny;
                     if (XS && xany) {
                         const bool in = xs >= 0;
-                        const T u1 = bxe * qxe[r - 1][0] + axe * (dxy2 / stx);
-                        const T u2 = bxe * qxe[r - 1][1] + axe * (dxz2 / stx);
+                        const T u1 = bxe * qxe[r - 1][0] + axe * (per_step(dxy2, stx));
+                        const T u2 = bxe * qxe[r - 1][1] + axe * (per_step(dxz2, stx));
                         w1 = in ? w1 + cb1 * (T(-1) * u1) : w1;
                         w2 = in ? w2 + cb2 * (T(1) * u2) : w2;
                         if (wr) {
@@ -547,8 +547,8 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                         const int cnt = ys[r] == 1 ? hy : ly;
                         const int co = K.cpml_off[1][ys[r]][0] - cbase;
                         const T bb = cp[co + yl[r]], aa = cp[co + cnt + yl[r]];
-                        const T u0 = bb * qye[r - 1][0] + aa * (dyx / sty);
-                        const T u2 = bb * qye[r - 1][1] + aa * (dyz2 / sty);
+                        const T u0 = bb * qye[r - 1][0] + aa * (per_step(dyx, sty));
+                        const T u2 = bb * qye[r - 1][1] + aa * (per_step(dyz2, sty));
                         v0 = v0 + cb0 * (T(1) * u0);
                         w2 = w2 + cb2 * (T(-1) * u2);
                         if (wr) {
@@ -559,8 +559,8 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
                     if (ZS && zs >= 0 && j >= 0) {
                         const int co = K.cpml_off[2][zs][0] - cbase;
                         const T bb = cp[co + zsl], aa = cp[co + zcnt + zsl];
-                        const T u0 = bb * qze[r - 1][0] + aa * (dzx / stz);
-                        const T u1 = bb * qze[r - 1][1] + aa * (dzy2 / stz);
+                        const T u0 = bb * qze[r - 1][0] + aa * (per_step(dzx, stz));
+                        const T u1 = bb * qze[r - 1][1] + aa * (per_step(dzy2, stz));
                         v0 = v0 + cb0 * (T(-1) * u0);
                         w1 = w1 + cb1 * (T(1) * u1);
                         if (wr) {

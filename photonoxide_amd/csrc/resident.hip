@@ -22,6 +22,7 @@
 #include "kernels.hpp"
 #include "march64.hpp"
 #include "march2d.hpp"
+#include "march2s.hpp"
 #include "step_launch.hpp"
 
 using namespace pxfdtd;
@@ -192,6 +193,13 @@ struct pxfdtd_resident {
     // launched on the stream after each step's kernels, with the step's number in the call (the
     // adjoint's record of each recomputed step); none when empty
     std::function<void(hipStream_t, uint32_t)> after_step;
+    // the two-step march (kind 5): pairs of steps in one pass where the problem allows, with
+    // its own units (strips of 60 x ty2 rows, chunks of kz2 planes); single steps run kind 3
+    int pair = 0, ty2 = 2, kz2 = 16;
+    Buf units2, inj2_off, inj2, tap2_off, tap2;
+    int n2[5] = {0, 0, 0, 0, 0}, tiles2_x = 0;  // units: lean, x, y, z slabs only, the rest
+    PairStreams ps = {{nullptr, nullptr, nullptr, nullptr}, nullptr, {nullptr, nullptr, nullptr, nullptr}};
+    bool units2_dirty = true, have_tap2 = false;
     // per-launch timing of the fused kernel, when asked for
     bool profile = false;
     std::vector<hipEvent_t> kev;
@@ -661,7 +669,13 @@ extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
     if (r->side) (void)hipStreamSynchronize(r->side);
     if (r->fork_ev) (void)hipEventDestroy(r->fork_ev);
     if (r->join_ev) (void)hipEventDestroy(r->join_ev);
+    for (int c = 0; c < 4; c++) {
+        if (r->ps.s[c]) (void)hipStreamSynchronize(r->ps.s[c]);
+        if (r->ps.join[c]) (void)hipEventDestroy(r->ps.join[c]);
+    }
+    if (r->ps.fork) (void)hipEventDestroy(r->ps.fork);
     hipStream_t s = r->stream, side = r->side;
+    PairStreams ps = r->ps;
     {
         std::lock_guard<std::mutex> g(specs_mutex());
         specs().erase(r);
@@ -669,6 +683,8 @@ extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
     delete r;  // buffers free themselves
     if (s) (void)hipStreamDestroy(s);
     if (side) (void)hipStreamDestroy(side);
+    for (int c = 0; c < 4; c++)
+        if (ps.s[c]) (void)hipStreamDestroy(ps.s[c]);
 }
 
 extern "C" int64_t pxfdtd_resident_psi_len(const pxfdtd_resident *r) { return r->psi_len; }
@@ -869,6 +885,7 @@ static int build_injection_lists_2d(pxfdtd_resident *r) {
 }
 
 static int build_injection_lists(pxfdtd_resident *r) {
+    r->units2_dirty = true;  // the two-step march's lists follow
     r->have_inj = !r->inj_host.empty() || !r->coup_host.empty();
     r->inj_set = 0;
     auto mark = [&](const pxfdtd_injection &s) {
@@ -1061,6 +1078,159 @@ static int build_units(pxfdtd_resident *r) {
     return PXFDTD_OK;
 }
 
+// The two-step march's tables: its units (strips of SW2 own columns by ty2 rows, chunks of kz2
+// planes), each in the CPML variant of the slabs its margin meets (the values it computes reach
+// 2 columns left and 2 right of its strip, 2 rows below and 2 above, 2 planes below and 2
+// above); the sources, currents and couplings of each tile's margin, sorted by plane (every
+// value either step computes takes its own); and each tile's monitored values, sorted by plane.
+static int build_pair_tables(pxfdtd_resident *r) {
+    const int nx = r->nx, ny = r->ny, nz = r->nz, th = r->ty2;
+    const int tx = (nx + SW2 - 1) / SW2, tyn = (ny + th - 1) / th;
+    r->tiles2_x = tx;
+    std::vector<int> cuts = {0};
+    const int lz = r->low[2], hz = r->high[2];
+    auto cut = [&](int z) {
+        if (z > cuts.back() && z < nz) cuts.push_back(z);
+    };
+    const int lo_end = lz > 0 ? lz + 2 : 0, hi_start = hz > 0 ? nz - hz - 2 : nz;
+    for (int z = r->kz2; z < lo_end; z += r->kz2) cut(z);
+    cut(lo_end);
+    for (int z = cuts.back() + r->kz2; z < hi_start; z += r->kz2) cut(z);
+    cut(hi_start);
+    for (int z = cuts.back() + r->kz2; z < nz; z += r->kz2) cut(z);
+    cuts.push_back(nz);
+    auto in_slab = [](int a, int b, int n, int lo, int hi) {  // [a, b) meets [0, lo) or [n-hi, n)
+        return (lo > 0 && a < lo) || (hi > 0 && b > n - hi);
+    };
+    std::vector<int4> lean, xonly, yonly, zonly, slab;
+    for (size_t c = 0; c + 1 < cuts.size(); c++) {
+        const int a = cuts[c], b = cuts[c + 1];
+        const bool zs = !r->periodic[2] && in_slab(std::max(a - 2, 0), std::min(b + 2, nz), nz, lz, hz);
+        for (int by = 0; by < tyn; by++) {
+            const int j0 = by * th;
+            const bool ys = !r->periodic[1] && in_slab(std::max(j0 - 2, 0), std::min(j0 + th + 2, ny), ny, r->low[1], r->high[1]);
+            for (int bx = 0; bx < tx; bx++) {
+                const int i0 = bx * SW2;
+                const bool xs = !r->periodic[0] && in_slab(std::max(i0 - 2, 0), std::min(i0 + SW2 + 2, nx), nx, r->low[0], r->high[0]);
+                const int4 u = make_int4(bx, by, a, b);
+                const int nslabs = (int)xs + (int)ys + (int)zs;
+                if (nslabs == 0)
+                    lean.push_back(u);
+                else if (nslabs > 1)
+                    slab.push_back(u);
+                else
+                    (xs ? xonly : (ys ? yonly : zonly)).push_back(u);
+            }
+        }
+    }
+    r->n2[0] = (int)lean.size();
+    r->n2[1] = (int)xonly.size();
+    r->n2[2] = (int)yonly.size();
+    r->n2[3] = (int)zonly.size();
+    r->n2[4] = (int)slab.size();
+    for (auto *v : {&xonly, &yonly, &zonly, &slab}) lean.insert(lean.end(), v->begin(), v->end());
+    TRY(r->units2.alloc(std::max<size_t>(lean.size(), 1) * sizeof(int4)));
+    HIPCHK(hipMemcpy(r->units2.p, lean.data(), lean.size() * sizeof(int4), hipMemcpyHostToDevice));
+
+    // the sources and currents per (tile, plane, margin row, field): a tile's margin rows are
+    // j0 - 2 + r, r < th + 4 (wrapped on a periodic axis), its lanes i0 - 2 .. i0 + 61
+    const int NE = th + 4;
+    const long long tiles = (long long)tx * tyn;
+    auto xin = [&](int i, int i0) {
+        for (int d : {0, nx, -nx}) {
+            if (d && !r->periodic[0]) continue;
+            if (i + d >= i0 - 2 && i + d < i0 + SW2 + 2) return true;
+        }
+        return false;
+    };
+    auto yrow = [&](int jr) {  // the grid row of margin row jr (unwrapped), -1 beyond a wall
+        if (jr >= 0 && jr < ny) return jr;
+        if (!r->periodic[1]) return -1;
+        return ((jr % ny) + ny) % ny;
+    };
+    if (r->have_inj) {
+        const std::vector<pxfdtd_injection> all = every_injection(r);
+        // entries per (tile, k, margin row, field) in the list's order
+        std::vector<std::vector<std::pair<long long, Injection>>> at((size_t)tiles);
+        for (size_t q = 0; q < all.size(); q++) {
+            const pxfdtd_injection &sj = all[q];
+            const long long v = sj.index;
+            const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((long long)nx * ny));
+            Injection in;
+            in.k = k; in.i = i; in.j = j; in.code = sj.field * 4 + sj.component; in.emitter = sj.emitter;
+            in.pad = 0; in.a_re = sj.a_re; in.a_im = sj.a_im;
+            for (int by = 0; by < tyn; by++)
+                for (int rr = 0; rr < NE; rr++) {
+                    if (yrow(by * th - 2 + rr) != j) continue;
+                    for (int bx = 0; bx < tx; bx++) {
+                        if (!xin(i, bx * SW2)) continue;
+                        const long long key = (((long long)k * NE + rr) * 2 + sj.field);
+                        at[(size_t)(by * tx + bx)].push_back({key, in});
+                    }
+                }
+        }
+        const long long per = (long long)nz * NE * 2;
+        std::vector<int32_t> off(tiles * per + 1, 0);
+        std::vector<Injection> flat;
+        for (long long t = 0; t < tiles; t++) {
+            auto &L = at[(size_t)t];
+            std::stable_sort(L.begin(), L.end(), [](const std::pair<long long, Injection> &a,
+                                                    const std::pair<long long, Injection> &b) { return a.first < b.first; });
+            size_t q = 0;
+            for (long long key = 0; key < per; key++) {
+                off[t * per + key] = (int32_t)flat.size();
+                while (q < L.size() && L[q].first == key) flat.push_back(L[q++].second);
+            }
+        }
+        off[tiles * per] = (int32_t)flat.size();
+        TRY(r->inj2_off.alloc(off.size() * sizeof(int32_t)));
+        TRY(r->inj2.alloc(std::max<size_t>(flat.size(), 1) * sizeof(Injection)));
+        HIPCHK(hipMemcpy(r->inj2_off.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        if (!flat.empty()) HIPCHK(hipMemcpy(r->inj2.p, flat.data(), flat.size() * sizeof(Injection), hipMemcpyHostToDevice));
+    }
+    // the monitored values per (tile, plane, own row, component), each listed by its one owner
+    r->have_tap2 = r->ntaps > 0;
+    if (r->have_tap2) {
+        const long long per = (long long)nz * th * 6;
+        std::vector<std::vector<std::pair<long long, TapAt>>> at((size_t)tiles);
+        for (long long q = 0; q < r->ntaps; q++) {
+            const long long v = r->tap_index_h[(size_t)q];
+            const int k = (int)(v / r->plane), j = (int)((v % r->plane) / r->pitch), i = (int)(v % r->pitch);
+            const int c = r->tap_which_h[(size_t)q];
+            const TapAt a = {i, 0, q};
+            const long long key = ((long long)k * th + j % th) * 6 + c;
+            at[(size_t)((j / th) * tx + i / SW2)].push_back({key, a});
+        }
+        std::vector<int32_t> off(tiles * per + 1, 0);
+        std::vector<TapAt> flat;
+        for (long long t = 0; t < tiles; t++) {
+            auto &L = at[(size_t)t];
+            std::stable_sort(L.begin(), L.end(), [](const std::pair<long long, TapAt> &a,
+                                                    const std::pair<long long, TapAt> &b) { return a.first < b.first; });
+            size_t q = 0;
+            for (long long key = 0; key < per; key++) {
+                off[t * per + key] = (int32_t)flat.size();
+                while (q < L.size() && L[q].first == key) flat.push_back(L[q++].second);
+            }
+        }
+        off[tiles * per] = (int32_t)flat.size();
+        TRY(r->tap2_off.alloc(off.size() * sizeof(int32_t)));
+        TRY(r->tap2.alloc(std::max<size_t>(flat.size(), 1) * sizeof(TapAt)));
+        HIPCHK(hipMemcpy(r->tap2_off.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        if (!flat.empty()) HIPCHK(hipMemcpy(r->tap2.p, flat.data(), flat.size() * sizeof(TapAt), hipMemcpyHostToDevice));
+    }
+    r->units2_dirty = false;
+    return PXFDTD_OK;
+}
+
+// whether the problem's next two steps can run as one pass of the two-step march: no plane
+// waves (their auxiliary grids step between the grid's steps), no media (a pass of their own
+// after each step), no per-step hook, and the two steps' taps in one staging block
+static bool pair_ok(const pxfdtd_resident *r, int in_block) {
+    return r->pair && r->nz > 1 && r->planes_h.empty() && !r->nmvals && !r->ngroups && !r->after_step &&
+           (r->ntaps == 0 || r->quiet || in_block + 2 <= r->block_steps);
+}
+
 extern "C" int pxfdtd_resident_set_injections(pxfdtd_resident *r, const pxfdtd_injection *inj,
                                               int64_t n, int32_t emitters) {
     for (int64_t q = 0; q < n; q++) {
@@ -1209,6 +1379,15 @@ extern "C" int pxfdtd_resident_set_plane_state(pxfdtd_resident *r, int32_t n, co
 }
 
 extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int32_t planes) {
+    if (r->pair) {  // the two-step march's pairs (single steps keep kind 3's tiling)
+        if (rows && (rows < 2 || rows > 3))
+            return px_fail(PXFDTD_ERR_INVALID, "rows per strip must be 2 or 3 for the two-step march, got %d", rows);
+        if (planes < 0) return px_fail(PXFDTD_ERR_INVALID, "planes per chunk must be >= 1");
+        if (rows) r->ty2 = rows;
+        if (planes) r->kz2 = std::min(planes, r->nz);
+        r->units2_dirty = true;
+        return PXFDTD_OK;
+    }
     if (rows) {
         const bool ok = r->kernel == 0 ? (rows == 2 || rows == 4 || rows == 8)
                         : r->kernel == 4 ? rows == 1  // the 2D march: one row at a time
@@ -1228,7 +1407,24 @@ extern "C" int pxfdtd_resident_set_tiling(pxfdtd_resident *r, int32_t rows, int3
 }
 
 extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
-    if (kind < 0 || kind > 4) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0 to 4");
+    if (kind < 0 || kind > 5) return px_fail(PXFDTD_ERR_INVALID, "kernel kind must be 0 to 5");
+    if (kind == 5) {
+        if (r->nz == 1 || !kernel_fits(3, r->nval, r->psi_len, (long long)r->esz))
+            return px_fail(PXFDTD_ERR_UNSUPPORTED, "the two-step march steps 3D grids whose buffers kernel 3 addresses");
+        TRY(pxfdtd_resident_set_kernel(r, 3));
+        if (!r->ps.fork) {
+            HIPCHK(hipSetDevice(r->dev->ordinal));
+            HIPCHK(hipEventCreateWithFlags(&r->ps.fork, hipEventDisableTiming));
+            for (int c = 0; c < 4; c++) {
+                HIPCHK(hipStreamCreate(&r->ps.s[c]));
+                HIPCHK(hipEventCreateWithFlags(&r->ps.join[c], hipEventDisableTiming));
+            }
+        }
+        r->pair = 1;
+        r->units2_dirty = true;
+        return PXFDTD_OK;
+    }
+    r->pair = 0;
     if (kind == 4 && !(r->nz == 1 && r->periodic[2]))
         return px_fail(PXFDTD_ERR_UNSUPPORTED, "the 2D march steps grids one cell thick with z periodic");
     if (!kernel_fits(kind, r->nval, r->psi_len, (long long)r->esz))
@@ -1243,7 +1439,7 @@ extern "C" int pxfdtd_resident_set_kernel(pxfdtd_resident *r, int32_t kind) {
     return build_injection_lists(r);
 }
 
-extern "C" int pxfdtd_resident_get_kernel(const pxfdtd_resident *r) { return r->kernel; }
+extern "C" int pxfdtd_resident_get_kernel(const pxfdtd_resident *r) { return r->pair ? 5 : r->kernel; }
 
 extern "C" int pxfdtd_resident_set_concurrent(pxfdtd_resident *r, int32_t mode) {
     if (mode < 0 || mode > 2) return px_fail(PXFDTD_ERR_INVALID, "the launch mode must be 0, 1 or 2");
@@ -1253,14 +1449,15 @@ extern "C" int pxfdtd_resident_set_concurrent(pxfdtd_resident *r, int32_t mode) 
 }
 
 extern "C" int pxfdtd_resident_get_tiling(const pxfdtd_resident *r, int32_t *rows, int32_t *planes) {
-    *rows = r->ty;
-    *planes = r->kz;
+    *rows = r->pair ? r->ty2 : r->ty;
+    *planes = r->pair ? r->kz2 : r->kz;
     return PXFDTD_OK;
 }
 
 // ---------------------------------------------------------------- probes and transforms
 static int build_taps(pxfdtd_resident *r, const std::vector<int64_t> &index,
                       const std::vector<int8_t> &which) {
+    r->units2_dirty = true;  // the two-step march's lists follow
     r->ntaps = (long long)index.size();
     r->tap_index_h = index;
     r->tap_which_h = which;
@@ -1444,7 +1641,7 @@ static StepArgs<T> args_of(const pxfdtd_resident *r) {
     bool slabs = false;
     for (int a = 0; a < 3; a++) {
         A.flags |= r->periodic[a] << a;
-        A.step[a] = (T)r->step[a];
+        A.step[a] = sizeof(T) == 4 ? (T)(1.0 / r->step[a]) : (T)r->step[a];  // see per_step
         slabs = slabs || r->low[a] || r->high[a];
     }
     if (slabs) A.flags |= 8;
@@ -1473,9 +1670,11 @@ static StepArgs<T> args_of(const pxfdtd_resident *r) {
 
 // the parameters slab cells, injections and media read, one set per copy parity
 template <typename T>
-static void fill_cold(const pxfdtd_resident *r, StepCold<T> K[2]) {
-    memset(K, 0, 2 * sizeof(StepCold<T>));
-    for (int p = 0; p < 2; p++) {
+// the step kernels' cold parameters per parity: [0, 1] the one-step kernels', [2, 3] the
+// two-step march's (its tiles' source lists)
+static void fill_cold(const pxfdtd_resident *r, StepCold<T> *K, int n) {
+    memset(K, 0, n * sizeof(StepCold<T>));
+    for (int p = 0; p < n; p++) {
         StepCold<T> &k = K[p];
         for (int a = 0; a < 3; a++) {
             k.low[a] = r->low[a];
@@ -1483,10 +1682,10 @@ static void fill_cold(const pxfdtd_resident *r, StepCold<T> K[2]) {
         }
         memcpy(k.cpml_off, r->cpml_off, sizeof k.cpml_off);
         memcpy(k.psi_off, r->psi_off, sizeof k.psi_off);
-        k.psi_in = r->psi[p].as<T>();
-        k.psi_out = r->psi[1 - p].as<T>();
-        k.inj_off = r->have_inj ? r->inj_off.as<int32_t>() : nullptr;
-        k.inj = r->inj.as<Injection>();
+        k.psi_in = r->psi[p & 1].as<T>();
+        k.psi_out = r->psi[1 - (p & 1)].as<T>();
+        k.inj_off = !r->have_inj ? nullptr : (p < 2 ? r->inj_off.as<int32_t>() : r->inj2_off.as<int32_t>());
+        k.inj = p < 2 ? r->inj.as<Injection>() : r->inj2.as<Injection>();
         k.waves = r->waves.as<double>();
         k.emitters = r->emitters;
         k.groups = r->groups.as<MediumGroup>();
@@ -1501,8 +1700,8 @@ static void fill_cold(const pxfdtd_resident *r, StepCold<T> K[2]) {
 
 template <typename T>
 static int upload_cold(pxfdtd_resident *r) {
-    StepCold<T> K[2];
-    fill_cold<T>(r, K);
+    StepCold<T> K[4];
+    fill_cold<T>(r, K, 4);
     TRY(r->cold.reserve(sizeof K));
     HIPCHK(hipMemcpy(r->cold.p, K, sizeof K, hipMemcpyHostToDevice));
     return PXFDTD_OK;
@@ -1543,6 +1742,8 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         }
     }
     const size_t lds = r->mode == 0 ? (size_t)256 * (6 * sizeof(T) + 3) : 0;
+    // the two-step march's lists before the cold parameters, which point at them
+    if (r->pair && r->units2_dirty) TRY(build_pair_tables(r));
     TRY(upload_cold<T>(r));
     if (r->kernel != 0 && r->units_dirty) TRY(build_units(r));
     int in_block = 0;
@@ -1555,6 +1756,70 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         }
     HIPCHK(hipEventRecord(r->ev0, s));
     for (uint32_t n = 0; n < count; n++) {
+        if (n + 1 < count && pair_ok(r, in_block)) {  // steps n and n + 1 in one pass
+            StepArgs<T> A = args_of<T>(r);
+            const int p = r->p, q = 1 - p;
+            A.ein = r->field<T>(p, 0);
+            A.hin = r->field<T>(p, 3);
+            A.eout = r->field<T>(q, 0);
+            A.hout = r->field<T>(q, 3);
+            A.cold = r->cold.as<StepCold<T>>() + 2 + p;
+            A.wave_step = (int)n;
+            A.units = r->units2.as<int4>();
+            A.tiles_x = r->tiles2_x;
+            const bool tapping = r->ntaps && !r->quiet;
+            PairTaps<T> TP;
+            TP.off = tapping && r->have_tap2 ? r->tap2_off.as<int32_t>() : nullptr;
+            TP.at = r->tap2.as<TapAt>();
+            TP.stage = r->staging.as<T>() + (size_t)in_block * r->ntaps;
+            if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
+            const PairStreams *sp = r->concurrent == 1 && r->ps.fork ? &r->ps : nullptr;
+            const PairCounts uc = {r->n2[0], r->n2[1], r->n2[2], r->n2[3], r->n2[4]};
+            if (r->mode == 0)
+                TRY((launch_pair<T, 0>(A, TP, r->ty2, s, lds, sp, uc)));
+            else if (r->mode == 2)
+                TRY((launch_pair<T, 2>(A, TP, r->ty2, s, lds, sp, uc)));
+            else
+                TRY((launch_pair<T, 1>(A, TP, r->ty2, s, lds, sp, uc)));
+            HIPCHK(hipGetLastError());
+            if (r->profile) {
+                HIPCHK(hipEventRecord(r->kev[2 * n + 1], s));
+                HIPCHK(hipEventRecord(r->kev[2 * n + 2], s));
+                HIPCHK(hipEventRecord(r->kev[2 * n + 3], s));
+            }
+            r->p = q;  // the pass read copy p and wrote copy 1 - p
+            if (tapping) {  // the second step's monitored values, from the fields stored
+                const int threads = 256;
+                const long long blocks = (r->ntaps + threads - 1) / threads;
+                hipLaunchKernelGGL((tap_values<T>), dim3((unsigned)blocks), dim3(threads), 0, s,
+                                   r->tap_fields[q].as<const T *const>(), r->tap_index.as<int64_t>(),
+                                   r->tap_which.as<int8_t>(), r->ntaps,
+                                   r->staging.as<T>() + (size_t)(in_block + 1) * r->ntaps);
+                HIPCHK(hipGetLastError());
+                in_block += 2;
+                if (in_block == r->block_steps || n + 2 == count) {
+                    if (r->nvals) {
+                        const long long blocks2 = (r->nvals + threads - 1) / threads;
+                        hipLaunchKernelGGL((dft_accumulate<T>), dim3((unsigned)blocks2), dim3(threads), 0, s,
+                                           r->staging.as<T>() + r->nprobes, r->ntaps, r->nvals, in_block,
+                                           r->series_of.as<int32_t>(), r->series_info.as<SeriesInfo>(),
+                                           r->phases.as<double>() + (size_t)block_first * r->slots * 2,
+                                           r->slots, r->sums.as<double>());
+                        HIPCHK(hipGetLastError());
+                    }
+                    if (r->nprobes) {
+                        HIPCHK(hipMemcpy2DAsync(r->probe_rec.as<T>() + (size_t)r->probe_rows * r->nprobes,
+                                                r->nprobes * sizeof(T), r->staging.p, r->ntaps * sizeof(T),
+                                                r->nprobes * sizeof(T), in_block, hipMemcpyDeviceToDevice, s));
+                        r->probe_rows += in_block;
+                    }
+                    block_first = n + 2;
+                    in_block = 0;
+                }
+            }
+            n++;
+            continue;
+        }
         StepArgs<T> A = args_of<T>(r);
         const int p = r->p, q = 1 - p;
         A.ein = r->field<T>(p, 0);

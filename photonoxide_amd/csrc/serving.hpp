@@ -288,7 +288,7 @@ static int server_rebuild(pxfdtd_server *S) {
         ServerJob *J = S->table[j];
         pxfdtd_resident *r = J->r;
         if (r->units_dirty) TRY(build_units(r));
-        fill_cold<float>(r, &colds[2 * j]);
+        fill_cold<float>(r, &colds[2 * j], 2);
         for (int q = 0; q < 2; q++) {
             colds[2 * j + q].waves = S->d_waves.as<double>() + J->wave_off;
             StepArgs<float> A = args_of<float>(r);

@@ -4,4 +4,6 @@
 namespace pxfdtd {
 template int launch_mode<float, 2>(const StepArgs<float> &, int, int, hipStream_t, size_t, const SideStream *,
                                   const UnitCounts &, int);
+template int launch_pair<float, 2>(const StepArgs<float> &, const PairTaps<float> &, int, hipStream_t, size_t,
+                                  const PairStreams *, const PairCounts &);
 }  // namespace pxfdtd

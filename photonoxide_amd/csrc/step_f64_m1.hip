@@ -4,4 +4,6 @@
 namespace pxfdtd {
 template int launch_mode<double, 1>(const StepArgs<double> &, int, int, hipStream_t, size_t, const SideStream *,
                                   const UnitCounts &, int);
+template int launch_pair<double, 1>(const StepArgs<double> &, const PairTaps<double> &, int, hipStream_t, size_t,
+                                  const PairStreams *, const PairCounts &);
 }  // namespace pxfdtd

@@ -4,6 +4,7 @@
 #include "step_launch.hpp"
 #include "march64.hpp"
 #include "march2d.hpp"
+#include "march2s.hpp"
 
 namespace pxfdtd {
 
@@ -120,6 +121,47 @@ int launch_mode(const StepArgs<T> &A, int kernel, int ty, hipStream_t s, size_t 
     case 4: return launch_ty<T, MODE, 4>(A, s, lds);
     default: return launch_ty<T, MODE, 8>(A, s, lds);
     }
+}
+
+template <typename T, int MODE, int RY, int AX>
+static void launch_pair_units(const StepArgs<T> &A0, const PairTaps<T> &TP, int first, int count, hipStream_t s,
+                              size_t lds) {
+    if (!count) return;
+    StepArgs<T> A = A0;
+    A.units = A0.units + first;
+    A.blocks = count;
+    A.per_xcd = ((count + 3) / 4 + 7) / 8;
+    hipLaunchKernelGGL((march2s_step<T, MODE, RY, AX>), dim3(8 * A.per_xcd), dim3(256), lds, s, A, TP);
+}
+
+template <typename T, int MODE, int RY>
+static int launch_pair_ry(const StepArgs<T> &A, const PairTaps<T> &TP, hipStream_t s, size_t lds,
+                          const PairStreams *side, const PairCounts &n) {
+    const int count[4] = {n.x, n.y, n.z, n.rest};
+    const int first[4] = {n.lean, n.lean + n.x, n.lean + n.x + n.y, n.lean + n.x + n.y + n.z};
+    if (side) HIPCHK(hipEventRecord(side->fork, s));
+    for (int c = 0; c < 4; c++) {
+        if (!count[c]) continue;
+        hipStream_t sc = side ? side->s[c] : s;
+        if (side) HIPCHK(hipStreamWaitEvent(sc, side->fork, 0));
+        if (c == 0) launch_pair_units<T, MODE, RY, 1>(A, TP, first[c], count[c], sc, lds);
+        if (c == 1) launch_pair_units<T, MODE, RY, 2>(A, TP, first[c], count[c], sc, lds);
+        if (c == 2) launch_pair_units<T, MODE, RY, 4>(A, TP, first[c], count[c], sc, lds);
+        if (c == 3) launch_pair_units<T, MODE, RY, 7>(A, TP, first[c], count[c], sc, lds);
+        if (side) HIPCHK(hipEventRecord(side->join[c], sc));
+    }
+    launch_pair_units<T, MODE, RY, 0>(A, TP, 0, n.lean, s, lds);
+    if (side)
+        for (int c = 0; c < 4; c++)
+            if (count[c]) HIPCHK(hipStreamWaitEvent(s, side->join[c], 0));
+    return PXFDTD_OK;
+}
+
+template <typename T, int MODE>
+int launch_pair(const StepArgs<T> &A, const PairTaps<T> &TP, int ry, hipStream_t s, size_t lds,
+                const PairStreams *side, const PairCounts &n) {
+    if (ry == 3) return launch_pair_ry<T, MODE, 3>(A, TP, s, lds, side, n);
+    return launch_pair_ry<T, MODE, 2>(A, TP, s, lds, side, n);
 }
 
 }  // namespace pxfdtd
