@@ -153,6 +153,13 @@ int pxfdtd_resident_read_probes(pxfdtd_resident *r, double *out, int64_t capacit
 /* the transform sums, FP64, series after series, each [f][k][j][i] complex */
 int pxfdtd_resident_read_sums(pxfdtd_resident *r, double *out, int64_t capacity);
 int64_t pxfdtd_resident_sums_len(const pxfdtd_resident *r);
+/* The transform sums where they live, in the device's memory (the layout above), for
+ * post-processing on the device (pxfdtd_flux_device) without a copy to the host. Valid until
+ * the series are set again or the resident is destroyed; read it on the resident's stream
+ * (pxfdtd_resident_stream) or after pxfdtd_resident_synchronize. */
+int pxfdtd_resident_sums_device(const pxfdtd_resident *r, const double **sums);
+/* the resident's HIP stream (a hipStream_t) */
+void *pxfdtd_resident_stream(const pxfdtd_resident *r);
 /* sets the transform sums the device goes on adding to (a run continued from sums held
  * elsewhere: each sum stays one running FP64 sum in step order, as the reference's) */
 int pxfdtd_resident_write_sums(pxfdtd_resident *r, const double *in, int64_t count);
@@ -313,6 +320,34 @@ int pxfdtd_simulation_add_flux_box(pxfdtd_simulation *s, const int32_t planes[3]
  * from the FP64 transform sums (out holds pxfdtd_simulation_flux_frequencies values) */
 int pxfdtd_simulation_flux(pxfdtd_simulation *s, int32_t n, double *out);
 int32_t pxfdtd_simulation_flux_frequencies(const pxfdtd_simulation *s, int32_t n);
+/* A flux plane over transform sums held in device memory (FluxPlane, src/fdtd/monitors.rs:
+ * 210-350), halfway between nodes `plane` and plane + 1 along `axis`. Along b = (axis + 1) % 3
+ * (w = 0) and c = (axis + 2) % 3 (w = 1) the sum runs over values from[w] <= m < to[w]: the
+ * window's wlo[w] .. whi[w] + 1 when windowed[w] (FluxPlane::range), which then also weighs the
+ * values (FluxPlane::weight), else the grid's 0 .. n. series[6] are the transform's series in
+ * device memory, E x, y, z then H x, y, z, each [f][k][j][i] complex over the nodes
+ * lo[a] <= n < hi[a] (the resident's layout); series[axis] and series[3 + axis] are not read.
+ * sign: the face's sign in its monitor (1, or -1 for a box's low faces); step_b, step_c: the
+ * grid's steps along b and c. */
+typedef struct {
+    int32_t axis, plane;
+    int32_t windowed[2], wlo[2], whi[2];
+    int32_t from[2], to[2];
+    int32_t lo[3], hi[3];
+    int32_t nf;
+    double sign, step_b, step_c;
+    const double *series[6];
+} pxfdtd_flux_face;
+enum { PXFDTD_FLUX_MAX_FACES = 6 };
+/* Post-processing on the device (Simulation::flux, src/fdtd/monitors.rs:530-547, over
+ * FluxPlane::flux, :318-350): the flux of n faces (1 to 6, one frequency count) summed per
+ * frequency, sign times each face's flux added to -0.0 in face order, each face's terms in the
+ * reference's order, so the result carries the sequential host sum's bits. Written to out[nf]
+ * in device memory, queued on `stream` (a hipStream_t; null: the device's default stream).
+ * The library allocates a terms buffer of 8 B x nf x 2 x (to[0] - from[0]) x (to[1] - from[1])
+ * per face for the call, and waits for the stream before it frees it and returns. */
+int pxfdtd_flux_device(pxfdtd_device *dev, const pxfdtd_flux_face *faces, int32_t n, double *out,
+                       void *stream);
 /* A port's mode for Simulation::add_mode_source (src/fdtd/sources.rs:550-596), given by its
  * profile as the reference's PortMode3d holds it (src/fdfd/three/ports.rs:46-60): the axis the
  * waveguide runs along (0 x, 1 y, 2 z), the plane of nodes it was solved on, the cells along

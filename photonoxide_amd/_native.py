@@ -132,6 +132,13 @@ class Coupling(C.Structure):
                 ("sample", C.c_int64)]
 
 
+class FluxFace(C.Structure):
+    _fields_ = [("axis", C.c_int32), ("plane", C.c_int32), ("windowed", C.c_int32 * 2), ("wlo", C.c_int32 * 2),
+                ("whi", C.c_int32 * 2), ("from_", C.c_int32 * 2), ("to", C.c_int32 * 2), ("lo", C.c_int32 * 3),
+                ("hi", C.c_int32 * 3), ("nf", C.c_int32), ("sign", C.c_double), ("step_b", C.c_double),
+                ("step_c", C.c_double), ("series", C.c_void_p * 6)]
+
+
 vp = C.c_void_p
 SIGNATURES = {
     "pxfdtd_last_error": (C.c_char_p, []),
@@ -154,6 +161,9 @@ SIGNATURES = {
     "pxfdtd_resident_read_sums": (C.c_int, [vp, pd, C.c_int64]),
     "pxfdtd_resident_sums_len": (C.c_int64, [vp]),
     "pxfdtd_resident_write_sums": (C.c_int, [vp, pd, C.c_int64]),
+    "pxfdtd_resident_sums_device": (C.c_int, [vp, C.POINTER(vp)]),
+    "pxfdtd_resident_stream": (vp, [vp]),
+    "pxfdtd_flux_device": (C.c_int, [vp, C.POINTER(FluxFace), C.c_int32, vp, vp]),
     "pxfdtd_resident_set_plane_waves": (C.c_int, [vp, C.POINTER(PlaneWaveAux), C.c_int32, C.POINTER(Coupling), C.c_int64]),
     "pxfdtd_resident_plane_state": (C.c_int, [vp, C.c_int32, pd, pd]),
     "pxfdtd_resident_set_plane_state": (C.c_int, [vp, C.c_int32, pd, pd]),

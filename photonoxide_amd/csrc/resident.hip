@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/pxfdtd.h"
+#include "device.hpp"
 #include "kernels.hpp"
 #include "march64.hpp"
 #include "march2d.hpp"
@@ -44,10 +45,6 @@ int px_fail(int code, const char *fmt, ...) {
 
 
 // ------------------------------------------------------------------------------ device
-struct pxfdtd_device {
-    int ordinal;
-    hipDeviceProp_t prop;
-};
 
 extern "C" int pxfdtd_device_open(int ordinal, pxfdtd_device **out) {
     int count = 0;
@@ -690,6 +687,14 @@ extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
 extern "C" int64_t pxfdtd_resident_psi_len(const pxfdtd_resident *r) { return r->psi_len; }
 extern "C" int32_t pxfdtd_resident_coefficient_mode(const pxfdtd_resident *r) { return r->mode; }
 extern "C" double pxfdtd_resident_last_kernel_ms(const pxfdtd_resident *r) { return r->last_ms; }
+
+extern "C" int pxfdtd_resident_sums_device(const pxfdtd_resident *r, const double **sums) {
+    if (!r->sums_len) return px_fail(PXFDTD_ERR_INVALID, "the resident holds no transform sums");
+    *sums = (const double *)r->sums.p;
+    return PXFDTD_OK;
+}
+
+extern "C" void *pxfdtd_resident_stream(const pxfdtd_resident *r) { return (void *)r->stream; }
 
 extern "C" int pxfdtd_resident_synchronize(pxfdtd_resident *r) {
     HIPCHK(hipStreamSynchronize(r->stream));

@@ -29,6 +29,7 @@
 using pxfdtd::C64;
 
 int px_fail(int code, const char *fmt, ...);  // resident.hip
+int flux_to_host(pxfdtd_device *dev, const pxfdtd_flux_face *faces, int32_t n, double *out, void *stream);  // flux.hip
 
 #define TRY(x)                            \
     do {                                  \
@@ -2078,11 +2079,47 @@ static double face_flux(pxfdtd_simulation *s, const FluxFace &F, const std::vect
     return total * steps[b] * steps[c];
 }
 
+// the flux from the device's sums, on the device (pxfdtd_flux_device): only nf values come back
+static int device_flux(pxfdtd_simulation *s, const FluxRec &R, int nf, double *out) {
+    const double *sums = nullptr;
+    TRY(pxfdtd_resident_sums_device(s->res, &sums));
+    std::vector<pxfdtd_flux_face> faces(R.faces.size());
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    for (size_t q = 0; q < R.faces.size(); q++) {
+        const FluxFace &F = R.faces[q];
+        pxfdtd_flux_face &D = faces[q];
+        D = pxfdtd_flux_face{};
+        D.axis = F.axis;
+        D.plane = F.plane;
+        for (int w = 0; w < 2; w++) {
+            D.windowed[w] = F.windowed[w];
+            D.wlo[w] = F.wlo[w];
+            D.whi[w] = F.whi[w];
+            flux_range(s, F, w, D.from[w], D.to[w]);
+        }
+        const DftRec &d = s->dfts[F.dft];
+        for (int a = 0; a < 3; a++) {
+            D.lo[a] = d.lo[a];
+            D.hi[a] = d.hi[a];
+        }
+        D.nf = nf;
+        D.sign = F.sign;
+        D.step_b = steps[(F.axis + 1) % 3];
+        D.step_c = steps[(F.axis + 2) % 3];
+        size_t at = 0;
+        for (int m = 0; m < F.dft; m++) at += s->dfts[m].acc.size();
+        const size_t len = d.acc.size() / 6;
+        for (int t = 0; t < 6; t++) D.series[t] = sums + at + t * len;
+    }
+    return flux_to_host(s->dev, faces.data(), (int32_t)faces.size(), out, pxfdtd_resident_stream(s->res));
+}
+
 // Simulation::flux (monitors.rs:530-547): per frequency, the faces' signed fluxes summed
 extern "C" int pxfdtd_simulation_flux(pxfdtd_simulation *s, int32_t n, double *out) {
     if (n < 0 || n >= (int32_t)s->fluxes.size()) return px_fail(PXFDTD_ERR_INVALID, "no flux monitor %d", n);
     const FluxRec &R = s->fluxes[n];
     const int nf = (int)s->dfts[R.faces[0].dft].freqs.size();
+    if (s->res && !s->taps_dirty) return device_flux(s, R, nf, out);
     std::vector<std::array<std::vector<double>, 6>> sums(R.faces.size());
     for (size_t q = 0; q < R.faces.size(); q++) {
         const DftRec &d = s->dfts[R.faces[q].dft];
