@@ -348,6 +348,16 @@ enum { PXFDTD_FLUX_MAX_FACES = 6 };
  * per face for the call, and waits for the stream before it frees it and returns. */
 int pxfdtd_flux_device(pxfdtd_device *dev, const pxfdtd_flux_face *faces, int32_t n, double *out,
                        void *stream);
+/* pxfdtd_flux_device without an allocation. It has no counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). It calls no hipMalloc, hipFree or other allocating routine
+ * and does not wait: its terms go to a 16 MiB array the library holds on each GPU until the
+ * process ends, so it takes at most 2^21 terms (2 x nf x values per face, summed over the
+ * faces; more is PXFDTD_ERR_INVALID), and out[nf] is ready when `stream` reaches it. Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (on different streams
+ * or host threads) race and give wrong values. Same results as pxfdtd_flux_device. */
+int pxfdtd_flux_device_no_alloc(pxfdtd_device *dev, const pxfdtd_flux_face *faces, int32_t n,
+                                double *out, void *stream);
 /* A port's mode for Simulation::add_mode_source (src/fdtd/sources.rs:550-596), given by its
  * profile as the reference's PortMode3d holds it (src/fdfd/three/ports.rs:46-60): the axis the
  * waveguide runs along (0 x, 1 y, 2 z), the plane of nodes it was solved on, the cells along

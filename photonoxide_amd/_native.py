@@ -164,6 +164,7 @@ SIGNATURES = {
     "pxfdtd_resident_sums_device": (C.c_int, [vp, C.POINTER(vp)]),
     "pxfdtd_resident_stream": (vp, [vp]),
     "pxfdtd_flux_device": (C.c_int, [vp, C.POINTER(FluxFace), C.c_int32, vp, vp]),
+    "pxfdtd_flux_device_no_alloc": (C.c_int, [vp, C.POINTER(FluxFace), C.c_int32, vp, vp]),
     "pxfdtd_resident_set_plane_waves": (C.c_int, [vp, C.POINTER(PlaneWaveAux), C.c_int32, C.POINTER(Coupling), C.c_int64]),
     "pxfdtd_resident_plane_state": (C.c_int, [vp, C.c_int32, pd, pd]),
     "pxfdtd_resident_set_plane_state": (C.c_int, [vp, C.c_int32, pd, pd]),

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/pxfdtd.h"
 #include "common.hpp"
@@ -102,6 +103,35 @@ __global__ void __launch_bounds__(64) flux_sum(FacesK P, const double *__restric
 
 }  // namespace
 
+// The allocation-free flux's terms buffer: one array per GPU, held until the process ends.
+// Host code uses the address hipGetSymbolAddress gives for the current device (scratch_of),
+// never the array's name, which on the host is only a stand-in.
+static constexpr long long kScratchValues = 1LL << 21;  // 16 MiB of FP64 terms
+static __device__ __align__(256) double scratch[kScratchValues];
+
+static std::mutex g_scratch_mutex;
+static double *g_scratch_at[64] = {};
+
+// Once per device, with that device current: the device's own address of `scratch`.
+int flux_scratch_setup(int ordinal, double **out) {
+    if (ordinal < 0 || ordinal >= 64) return px_fail(PXFDTD_ERR_INVALID, "no scratch for device %d", ordinal);
+    std::lock_guard<std::mutex> g(g_scratch_mutex);
+    if (!g_scratch_at[ordinal]) {
+        int prev = 0;
+        HIPCHK(hipGetDevice(&prev));
+        HIPCHK(hipSetDevice(ordinal));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        (void)hipSetDevice(prev);
+        if (e != hipSuccess || !p)
+            return px_fail(PXFDTD_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", ordinal,
+                           hipGetErrorString(e));
+        g_scratch_at[ordinal] = (double *)p;
+    }
+    *out = g_scratch_at[ordinal];
+    return PXFDTD_OK;
+}
+
 // Checks the faces and lays out their kernel parameters and terms.
 static int flux_plan(const pxfdtd_flux_face *faces, int32_t n, FacesK &P) {
     if (n < 1 || n > PXFDTD_FLUX_MAX_FACES)
@@ -188,6 +218,21 @@ extern "C" int pxfdtd_flux_device(pxfdtd_device *dev, const pxfdtd_flux_face *fa
     TRY(rc);
     if (e != hipSuccess) return px_fail(PXFDTD_ERR_DEVICE, "the flux: %s", hipGetErrorString(e));
     return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_flux_device_no_alloc(pxfdtd_device *dev, const pxfdtd_flux_face *faces, int32_t n,
+                                           double *out, void *stream) {
+    FacesK P;
+    TRY(flux_plan(faces, n, P));
+    if (!out) return px_fail(PXFDTD_ERR_INVALID, "no output for the flux");
+    if (P.total > kScratchValues)
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "the flux needs %lld terms; the allocation-free flux holds at most %lld (16 MiB)", P.total,
+                       kScratchValues);
+    if (!dev->scratch) return px_fail(PXFDTD_ERR_DEVICE, "device %d has no scratch address", dev->ordinal);
+    HIPCHK(hipSetDevice(dev->ordinal));
+    // every term is written before it is read: nothing in the scratch needs clearing
+    return flux_launch(P, dev->scratch, out, (hipStream_t)stream);
 }
 
 // The host mirror's flux: on the device, nf values copied back to `out` (host memory).

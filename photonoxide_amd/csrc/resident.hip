@@ -58,6 +58,11 @@ extern "C" int pxfdtd_device_open(int ordinal, pxfdtd_device **out) {
         delete d;
         return px_fail(PXFDTD_ERR_DEVICE, "hipGetDeviceProperties: %s", hipGetErrorString(e));
     }
+    const int rc = flux_scratch_setup(ordinal, &d->scratch);
+    if (rc != PXFDTD_OK) {
+        delete d;
+        return rc;
+    }
     *out = d;
     return PXFDTD_OK;
 }
