@@ -141,36 +141,118 @@ def algorithmic_bytes_per_step(nx, ny, nz, pml, coef_bytes=1):
     return n * (48 + coef_bytes) + 8 * psi, psi
 
 
+def host_threads():
+    """The threads the CPU baseline uses: the cores this process may run on, within the
+    host's thread budget (OMP_NUM_THREADS where it is set)."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    budget = os.environ.get("OMP_NUM_THREADS", "")
+    return min(cores, int(budget)) if budget.isdigit() and int(budget) > 0 else cores
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(nx, ny, nz, pml):
+    """The reference's CPU kernel restated (oracle/cpu_baseline.c), timed with the reference's
+    own protocol, on the host's threads, with the host's STREAM triad beside it."""
+    import ctypes
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    threads = min(16, os.cpu_count() or 1)
-    steps = 3
-    sec = O.lib.or_cpu_rate_f32(nx, ny, nz, pml, steps, 3, threads)
-    rate = nx * ny * nz / sec / 1e9
+    threads = host_threads()
+    steps = ctypes.c_long()
+    rate = O.lib.or_cpu_rate_f32(nx, ny, nz, pml, threads, ctypes.byref(steps)) / 1e9
+    triad = O.lib.or_cpu_triad(1 << 26, 5, threads)
     return {"value": round(rate, 4), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"the reference's CPU row kernel restated in C/OpenMP (oracle/cpu_baseline.c), f32, "
-                      f"{nx}x{ny}x{nz} cells with a {pml}-cell CPML, best of 3 runs of {steps} steps"}
+                      f"{nx}x{ny}x{nz} cells with a {pml}-cell CPML, timed as src/fdtd/gpu/rates.rs times: "
+                      f"steps doubled from 4 until a run takes > 0.3 s, best of 3 runs of {steps.value} steps, "
+                      f"fields first touched by the threads that step them",
+            "nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+            "cpu_model": cpu_model(), "triad_gb_per_s": round(triad, 1),
+            "triad": "a = b + s c over 2^26 doubles, 24 B an element, best of 5 (src/traffic.rs:59-80)"}
 
 
-def copy_rate(mib, steps, warmup):
-    """A plain copy of `mib` MiB between two device buffers: bytes read + written per second."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(warmup):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    return {"copy_mib": mib, "ms_per_copy": round(ms, 5), "bytes_per_copy": 2 * n * 4,
-            "gb_per_s": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1)}
+# the files the step kernels and their launches are built from
+STEP_SOURCES = ["common.hpp", "kernels.hpp", "march64.hpp", "march2d.hpp", "march2s.hpp", "resident.hip",
+                "step_launch.hpp", "step_launch_impl.hpp"] + [f"step_f{b}_m{m}.hip" for b in (32, 64) for m in range(3)]
+
+
+def source_hash():
+    """A hash of the step kernels' sources: byte counts from rocprofv3 hold for the sources they
+    were counted on."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "photonoxide_amd", "csrc")
+    for name in STEP_SOURCES:
+        h.update(name.encode())
+        h.update(open(os.path.join(csrc, name), "rb").read())
+    return h.hexdigest()[:16]
+
+
+def counted_kib(d, name):
+    """Per kernel variant (template arguments) of the kernels named `name`: the mean over its
+    dispatches of each counter rocprofv3 --pmc wrote under directory d, in KiB."""
+    import csv
+    import glob
+    import re
+    sums = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for row in csv.DictReader(open(f)):
+            k = row["Kernel_Name"]
+            if name not in k:
+                continue
+            m = re.search(r"<([^>]*)>", k)
+            variant = m.group(1) if m else k
+            key = (variant, row["Counter_Name"])
+            sums.setdefault(key, []).append(float(row["Counter_Value"]))
+    out = {}
+    for (variant, counter), vals in sums.items():
+        out.setdefault(variant, {})[counter] = sum(vals) / len(vals)
+    return out
+
+
+def traffic_summary(out, fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_mib, kind, tiling,
+                    kernel, commands):
+    """The step's HBM bytes from rocprofv3's FETCH_SIZE and WRITE_SIZE (one pass each), each
+    scaled by what the counter reads for the library's copy kernel of known size."""
+    fetch, write = counted_kib(fetch_dir, kernel), counted_kib(write_dir, kernel)
+    cf = counted_kib(copy_fetch_dir, "hbm_copy")
+    cw = counted_kib(copy_write_dir, "hbm_copy")
+    copy_kib = copy_mib * 1024.0
+    f_copy = sum(v["FETCH_SIZE"] for v in cf.values()) / len(cf)
+    w_copy = sum(v["WRITE_SIZE"] for v in cw.values()) / len(cw)
+    fetch_scale, write_scale = copy_kib / f_copy, copy_kib / w_copy
+    f_step = sum(v["FETCH_SIZE"] for v in fetch.values())
+    w_step = sum(v["WRITE_SIZE"] for v in write.values())
+    read_b, write_b = f_step * fetch_scale * 1024, w_step * write_scale * 1024
+    alg, _ = algorithmic_bytes_per_step(384, 256, 192, 12)
+    rec = {"kernel": f"{kernel} (the launches of one step)", "workload": "mmi_1x2_soi_384x256x192_dx20nm_cpml12_f32",
+           "kernel_kind": kind, "tiling": list(tiling), "source_hash": source_hash(),
+           "fetch_kib_per_step": round(f_step, 1), "write_kib_per_step": round(w_step, 1),
+           "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
+           "bytes_per_launch": int(read_b + write_b), "read_bytes": int(read_b), "write_bytes": int(write_b),
+           "algorithmic_bytes": alg, "traffic_over_algorithmic": round((read_b + write_b) / alg, 4),
+           "per_variant_kib": {k: {c: round(x, 1) for c, x in v.items()} for k, v in fetch.items()},
+           "per_variant_write_kib": {k: {c: round(x, 1) for c, x in v.items()} for k, v in write.items()},
+           "scale_from": f"the library's copy kernel over {copy_mib} MiB (bench.py --copy-mib {copy_mib}): "
+                         f"FETCH_SIZE {f_copy:.1f} KiB per dispatch for {copy_kib:.0f} KiB read, WRITE_SIZE "
+                         f"{w_copy:.1f} KiB for {copy_kib:.0f} KiB written",
+           "commands": commands}
+    json.dump(rec, open(out, "w"), indent=1)
+    return rec
+
+
+def hbm_rates(gpu, mib=1024, reps=10):
+    """The device's achievable HBM rates from the library's own streaming kernels."""
+    r = gpu.hbm_rates(mib << 20, reps)
+    return {k: round(v, 1) for k, v in r.items()} | {"buffer_mib": mib, "unit": "GB/s"}
 
 
 def serving(gpu, n2d, n3d, rate, seed=2, autotune=2, direct=False):
@@ -324,8 +406,9 @@ def main():
                     help="the aligned march's launch mode: 0 three launches, 1 the slab one on a "
                          "second stream, 2 one launch")
     ap.add_argument("--copy-mib", type=int, default=0,
-                    help="only time a plain device-to-device copy of this many MiB (the reference "
-                         "kernel for scaling the HBM byte counters and for the achievable copy rate)")
+                    help="only time the library's streaming kernels (read, write, copy, triad) over "
+                         "buffers of this many MiB, --steps times each (the kernels of known size for "
+                         "scaling the HBM byte counters, and the achievable rates)")
     ap.add_argument("--serving", choices=["small", "full"], default=None,
                     help="the batched-serving workload instead: small is 64 2D + 4 3D jobs, full "
                          "256 2D + 16 3D")
@@ -339,14 +422,26 @@ def main():
                     help="adjoint: design cells checked by finite differences (0: none)")
     ap.add_argument("--direct", action="store_true",
                     help="serving: launch each segment's kernels directly instead of a hipGraph")
+    ap.add_argument("--traffic", nargs=5, metavar=("OUT", "FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
+                    help="write the step's counted HBM traffic to OUT from four rocprofv3 --pmc runs (FETCH_SIZE "
+                         "then WRITE_SIZE of the bench, then of --copy-mib 1024); no GPU needed")
+    ap.add_argument("--traffic-commands", nargs="*", default=[], help="the commands those runs were")
     args = ap.parse_args()
+    if args.traffic:
+        out, fd, wd, cfd, cwd = args.traffic
+        kind = args.kernel if args.kernel >= 0 else 3
+        tiling = (args.rows or 2, args.planes or 8)
+        name = {3: "march64_step", 5: "march2s_step"}.get(kind, "march")
+        print(json.dumps(traffic_summary(out, fd, wd, cfd, cwd, 1024, kind, tiling, name, args.traffic_commands)))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: each box has one MI355X, and "
                           "--gpus above 1 is not supported (a grid would shard by z slabs with a "
                           "one-plane halo exchange per half step)"}))
         sys.exit(2)
     if args.copy_mib:
-        print(json.dumps(copy_rate(args.copy_mib, args.steps, args.warmup)), flush=True)
+        import photonoxide_amd as P
+        print(json.dumps(hbm_rates(P.Gpu(0), args.copy_mib, max(1, args.steps))), flush=True)
         return
     if args.adjoint:
         import photonoxide_amd as P
@@ -396,14 +491,21 @@ def main():
     steps_per_launch = 2 if s.kernel() == 5 else 1
     launch_ms = kernel_ms * steps_per_launch
     achieved = bytes_step / (launch_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, "no counted profile for this workload"
     tf = os.path.join(ROOT, "profiles", "traffic_march64_step_mmi_f32.json")
     if os.path.exists(tf) and args.workload == "mmi":
-        # the counted bytes hold for the workload and tiling they were counted on
+        # the counted bytes hold for the sources, workload and tiling they were counted on
         t = json.load(open(tf))
-        if (not args.plain and pml == 12 and s.kernel() == t.get("kernel_kind")
-                and list(s.tiling()) == t.get("tiling")):
+        if t.get("source_hash") != source_hash():
+            traffic_note = "counted on other kernel sources: dropped"
+        elif not (not args.plain and pml == 12 and s.kernel() == t.get("kernel_kind")
+                  and list(s.tiling()) == t.get("tiling")):
+            traffic_note = "counted on another kernel or tiling: dropped"
+        else:
             traffic = t.get("bytes_per_launch")
+            traffic_note = (f"rocprofv3 FETCH_SIZE and WRITE_SIZE of the step's launches, scaled by what they "
+                            f"read for the library's copy kernel of known size; sources {t['source_hash']}")
+    measured = hbm_rates(gpu)
     out = {
         "metric": "3D FDTD Gcell-updates/s (% HBM roofline); batched-job p50/p99 latency",
         "value": round(value, 3),
@@ -427,7 +529,9 @@ def main():
                                 "a plane of 6 components x 480x224 values x 5 frequencies, FP64 sums"),
                    "psi_values": psi},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic, "traffic_note": traffic_note,
+                     "measured_hbm": measured,
+                     "frac_of_measured_copy": round(achieved / measured["copy"], 4),
                      "kernel": ["fused_step", "march_step", "march_step", "march64_step", "march2d_step",
                                 "march2s_step"][s.kernel()],
                      "kernel_ms": round(launch_ms, 5), "steps_per_launch": steps_per_launch,

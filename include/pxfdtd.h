@@ -66,6 +66,12 @@ typedef struct {
 int pxfdtd_device_open(int ordinal, pxfdtd_device **out);
 int pxfdtd_device_query(const pxfdtd_device *dev, pxfdtd_device_info *info);
 void pxfdtd_device_close(pxfdtd_device *dev);
+/* The achievable HBM rate (no counterpart in the reference: the stepper's roofline is also
+ * reported against it): streaming kernels over buffers of `bytes` each (at least 1 MiB, a
+ * multiple of 16), 16-byte accesses, the best of `reps` timed runs each. out: read, write,
+ * copy (2 x bytes) and triad (3 x bytes) in GB/s of bytes read plus written. Allocates three
+ * buffers for the call. */
+int pxfdtd_device_hbm_rates(pxfdtd_device *dev, int64_t bytes, int32_t reps, double out[4]);
 
 /* A dispersive medium's values and poles (src/fdtd/media.rs:592-607 Group): E's update there
  * uses p_e, p_m as its ca, cb (given in the problem's ca, cb), then the ADE rest. */

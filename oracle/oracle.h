@@ -154,9 +154,14 @@ const double *or_sim_dft_series(const or_sim *s, int n, int series, int64_t *len
 double or_sim_electric_energy(const or_sim *s);
 
 /* the CPU speed baseline: the reference CPU kernel's row loops (src/fdtd/kernel.rs), rows in
- * chunks of ceil(4096/nx) shared by OpenMP threads, f32, stepping `steps` times a uniform
- * medium of n^3 cells with a CPML of `pml` cells; returns seconds, best of `repeats`. */
-double or_cpu_rate_f32(int nx, int ny, int nz, int pml, int steps, int repeats, int threads);
+ * chunks of ceil(4096/nx) shared by `threads` OpenMP threads, f32, a uniform medium of
+ * nx x ny x nz cells with a CPML of `pml` cells, timed as src/fdtd/gpu/rates.rs:49-100 times
+ * (steps doubled from 4 until a run takes > 0.3 s, best of 3); returns cell-updates per second
+ * and the steps of a run in *steps_out. */
+double or_cpu_rate_f32(int nx, int ny, int nz, int pml, int threads, long *steps_out);
+/* the STREAM triad a = b + s c over `len` doubles on `threads` threads (src/traffic.rs:59-80):
+ * GB/s at 24 bytes an element, best of `repeats` */
+double or_cpu_triad(long len, int repeats, int threads);
 
 #ifdef __cplusplus
 }

@@ -145,6 +145,7 @@ SIGNATURES = {
     "pxfdtd_device_open": (C.c_int, [C.c_int, C.POINTER(vp)]),
     "pxfdtd_device_query": (C.c_int, [vp, C.POINTER(DeviceInfo)]),
     "pxfdtd_device_close": (None, [vp]),
+    "pxfdtd_device_hbm_rates": (C.c_int, [vp, C.c_int64, C.c_int32, C.POINTER(C.c_double * 4)]),
     "pxfdtd_resident_footprint": (C.c_int, [C.POINTER(Problem), C.POINTER(C.c_size_t)]),
     "pxfdtd_resident_create": (C.c_int, [vp, C.POINTER(Problem), C.POINTER(vp)]),
     "pxfdtd_resident_destroy": (None, [vp]),

@@ -360,6 +360,13 @@ class Gpu:
                 "compute_units": i.compute_units, "hbm_bytes": i.hbm_bytes,
                 "free_bytes": i.free_bytes}
 
+    def hbm_rates(self, nbytes: int = 1 << 30, reps: int = 10) -> dict:
+        """The device's achievable HBM rates in GB/s (bytes read plus written), from streaming
+        kernels over buffers of nbytes each: read, write, copy and triad."""
+        out = (C.c_double * 4)()
+        N.check(N.lib.pxfdtd_device_hbm_rates(self.handle, nbytes, reps, C.byref(out)))
+        return dict(zip(("read", "write", "copy", "triad"), (float(v) for v in out)))
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:

@@ -97,7 +97,8 @@ for name, res, args in [
     ("or_pow", C.c_double, [C.c_double, C.c_double]),
     ("or_waveform_at", C.c_double, [C.POINTER(Wave), C.c_double]),
     ("or_waveform_complex_at", C64, [C.POINTER(Wave), C.c_double]),
-    ("or_cpu_rate_f32", C.c_double, [C.c_int] * 7),
+    ("or_cpu_rate_f32", C.c_double, [C.c_int] * 5 + [C.POINTER(C.c_long)]),
+    ("or_cpu_triad", C.c_double, [C.c_long, C.c_int, C.c_int]),
 ]:
     f = getattr(lib, name)
     f.restype = res
