@@ -538,6 +538,14 @@ typedef struct {
     int32_t direct;      /* 1: launch each segment's kernels directly instead of a hipGraph */
     int32_t autotune;    /* trial launches per tiling for the first job of each class of grid
                             shape (2D or 3D, each extent's power of two); 0: default tiling */
+    int32_t order;       /* 0: every admitted job steps in every segment, jobs admitted in
+                            arrival order; 1: shortest remaining work first: a segment steps the
+                            admitted jobs with the fewest cell-updates left until their cells
+                            reach fill_cells (at least one job), and the queue admits its
+                            smallest job first. The others wait resident, so short jobs finish
+                            early; results are the same bits either way */
+    int32_t pad;
+    int64_t fill_cells;  /* order 1: cells a segment's jobs fill the device with; 0: 2^24 */
 } pxfdtd_server_config;
 enum {
     PXFDTD_JOB_QUEUED = 0,
@@ -564,6 +572,12 @@ int pxfdtd_server_submit(pxfdtd_server *s, pxfdtd_simulation *sim, int64_t steps
 int pxfdtd_server_state(pxfdtd_server *s, int64_t job, int32_t *state);
 /* waits for the job (timeout_seconds <= 0: no limit); PXFDTD_OK once it is done */
 int pxfdtd_server_wait(pxfdtd_server *s, int64_t job, double timeout_seconds);
+/* Cancels a queued or running job (no counterpart in the reference). A queued job leaves the
+ * queue at once; a running one stops at the next segment boundary, its simulation holding the
+ * fields and sums of the step it reached, with no device, and its device memory freed. Its
+ * state becomes PXFDTD_JOB_CANCELLED (pxfdtd_server_wait reports it); the other jobs go on,
+ * their results the same bits. A job already done, failed or cancelled is left as it is. */
+int pxfdtd_server_cancel(pxfdtd_server *s, int64_t job);
 /* seconds since the server started: submission, admission, completion */
 int pxfdtd_server_times(pxfdtd_server *s, int64_t job, double *submitted, double *started,
                         double *finished);

@@ -62,7 +62,8 @@ class Series(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_jobs", C.c_int32), ("segment", C.c_int32),
-                ("direct", C.c_int32), ("autotune", C.c_int32)]
+                ("direct", C.c_int32), ("autotune", C.c_int32), ("order", C.c_int32), ("pad", C.c_int32),
+                ("fill_cells", C.c_int64)]
 
 
 class AdjointSpec(C.Structure):
@@ -185,6 +186,7 @@ SIGNATURES = {
     "pxfdtd_server_submit": (C.c_int, [vp, vp, C.c_int64, pi64]),
     "pxfdtd_server_state": (C.c_int, [vp, C.c_int64, pi32]),
     "pxfdtd_server_wait": (C.c_int, [vp, C.c_int64, C.c_double]),
+    "pxfdtd_server_cancel": (C.c_int, [vp, C.c_int64]),
     "pxfdtd_server_times": (C.c_int, [vp, C.c_int64, pd, pd, pd]),
     "pxfdtd_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
     "pxfdtd_adjoint_create": (C.c_int, [vp, C.POINTER(AdjointSpec), C.POINTER(vp)]),

@@ -18,6 +18,7 @@
 // job of each class of grid shape (2D or 3D, each extent's power of two) and are cached per
 // class.
 #pragma once
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -115,6 +116,8 @@ struct ServerJob {
     int32_t state = PXFDTD_JOB_QUEUED;
     double t_submit = 0, t_start = 0, t_end = 0;
     size_t foot = 0;
+    double work = 0;                 // cells x steps at submission
+    std::atomic<bool> cancel{false};  // asked to stop: it steps no more, stops at the boundary
     pxfdtd_resident *r = nullptr;
     int emitters = 0, slots = 0;
     std::string error;
@@ -399,14 +402,35 @@ static int server_segment_begin(pxfdtd_server *S) {
     const int K = S->cfg.segment;
     const int nj = (int)S->table.size();
     double cells = 0;
+    // which jobs step: all that can, or (shortest first) those with the least work left until
+    // their cells fill the device
+    std::vector<char> go(nj, 0);
+    std::vector<int> cand;
+    for (int j = 0; j < nj; j++) {
+        const ServerJob *J = S->table[j];
+        if (J->state == PXFDTD_JOB_RUNNING && J->left > 0 && J->r && !J->cancel.load()) cand.push_back(j);
+    }
+    if (S->cfg.order == 1) {
+        auto rest = [&](int j) { return (double)S->table[j]->left * (double)S->table[j]->r->ncells; };
+        std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return rest(a) < rest(b); });
+        const double fill = S->cfg.fill_cells > 0 ? (double)S->cfg.fill_cells : (double)(1 << 24);
+        double filled = 0;
+        for (int j : cand) {
+            if (filled >= fill) break;
+            go[j] = 1;
+            filled += (double)S->table[j]->r->ncells;
+        }
+    } else {
+        for (int j : cand) go[j] = 1;
+    }
     // every job's waves and phases for the segment (host FP64, each job's own slice of the
     // tables): on up to 8 host threads, the device waiting on them
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : cells) num_threads(8) if (nj > 8)
     for (int j = 0; j < nj; j++) {
         ServerJob *J = S->table[j];
-        const bool on = J->state == PXFDTD_JOB_RUNNING && J->left > 0 && J->r;
+        const bool on = go[j] != 0;
         const int a = on ? (int)std::min<int64_t>(K, J->left) : 0;
-        S->h_seg[j] = {a, on ? J->r->p : 0};
+        S->h_seg[j] = {a, J->r ? J->r->p : 0};
         if (a) {
             pxfdtd_host::sim_tables(J->sim, pxfdtd_host::sim_steps(J->sim), a, J->slots, S->h_waves + J->wave_off,
                                     S->h_phases + J->phase_off);
@@ -464,9 +488,14 @@ static void server_loop(pxfdtd_server *S) {
         std::vector<ServerJob *> admit;
         while (!S->stop && !S->queue.empty() &&
                (int)(S->running.size() + admit.size() + fetch.size()) < S->cfg.max_jobs) {
-            ServerJob *J = S->queue.front();
+            // arrival order, or (shortest first) the smallest queued job
+            auto at = S->queue.begin();
+            if (S->cfg.order == 1)
+                at = std::min_element(S->queue.begin(), S->queue.end(),
+                                      [](const ServerJob *a, const ServerJob *b) { return a->work < b->work; });
+            ServerJob *J = *at;
             if (S->used + J->foot > (size_t)S->cfg.hbm_budget) break;
-            S->queue.pop_front();
+            S->queue.erase(at);
             J->state = PXFDTD_JOB_RUNNING;
             J->t_start = server_now(S);
             S->used += J->foot;
@@ -559,14 +588,19 @@ static void server_loop(pxfdtd_server *S) {
         for (auto &f : fetched) {
             f.first->r = nullptr;
             S->used -= f.first->foot;
-            server_finish(S, f.first, f.second.empty() ? PXFDTD_JOB_DONE : PXFDTD_JOB_FAILED, f.second);
+            if (!f.second.empty())
+                server_finish(S, f.first, PXFDTD_JOB_FAILED, f.second);
+            else if (f.first->left > 0)
+                server_finish(S, f.first, PXFDTD_JOB_CANCELLED, "cancelled");
+            else
+                server_finish(S, f.first, PXFDTD_JOB_DONE, "");
         }
         fetch.clear();
         const bool device_error = rc != PXFDTD_OK || rc4 != PXFDTD_OK;
         // the jobs this segment finished (or all of them, after a device error) go to fetch
         for (auto it = S->running.begin(); it != S->running.end();) {
             ServerJob *J = *it;
-            if (!device_error && J->left > 0) {
+            if (!device_error && J->left > 0 && !J->cancel.load()) {
                 ++it;
                 continue;
             }
@@ -608,6 +642,8 @@ extern "C" int pxfdtd_server_create(pxfdtd_device *dev, const pxfdtd_server_conf
     std::memset(&S->stats, 0, sizeof S->stats);
     if (S->cfg.segment < 0 || S->cfg.max_jobs < 0 || S->cfg.hbm_budget < 0 || S->cfg.autotune < 0)
         return px_fail(PXFDTD_ERR_INVALID, "a server setting is negative");
+    if (S->cfg.order < 0 || S->cfg.order > 1 || S->cfg.fill_cells < 0)
+        return px_fail(PXFDTD_ERR_INVALID, "a server's order is 0 or 1, and its fill a cell count");
     if (!S->cfg.segment) S->cfg.segment = 32;
     if (S->cfg.segment > 4096) return px_fail(PXFDTD_ERR_INVALID, "segments of at most 4096 steps");
     if (!S->cfg.max_jobs) S->cfg.max_jobs = 256;
@@ -663,6 +699,7 @@ extern "C" int pxfdtd_server_submit(pxfdtd_server *S, pxfdtd_simulation *sim, in
     J->sim = sim;
     J->steps = J->left = steps;
     J->foot = foot;
+    J->work = (double)steps * (double)pxfdtd_simulation_cells(sim);
     J->t_submit = server_now(S);
     ServerJob *raw = J.get();
     S->jobs[raw->id] = std::move(J);
@@ -696,6 +733,25 @@ extern "C" int pxfdtd_server_wait(pxfdtd_server *S, int64_t job, double timeout_
     if (J->state != PXFDTD_JOB_DONE)
         return px_fail(J->state == PXFDTD_JOB_FAILED ? PXFDTD_ERR_DEVICE : PXFDTD_ERR_INVALID, "job %lld %s: %s",
                        (long long)job, J->state == PXFDTD_JOB_FAILED ? "failed" : "was cancelled", J->error.c_str());
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_server_cancel(pxfdtd_server *S, int64_t job) {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return px_fail(PXFDTD_ERR_INVALID, "no job %lld", (long long)job);
+    ServerJob *J = it->second.get();
+    if (J->state == PXFDTD_JOB_QUEUED) {
+        for (auto q = S->queue.begin(); q != S->queue.end(); ++q)
+            if (*q == J) {
+                S->queue.erase(q);
+                break;
+            }
+        server_finish(S, J, PXFDTD_JOB_CANCELLED, "cancelled before it was admitted");
+    } else if (J->state == PXFDTD_JOB_RUNNING) {
+        J->cancel.store(true);
+        S->wake.notify_all();
+    }
     return PXFDTD_OK;
 }
 

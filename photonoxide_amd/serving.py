@@ -1,10 +1,11 @@
 """The serving loop: continuous batching of FDTD jobs on one MI355X (pxfdtd_server_*).
 
 A job is a :class:`~photonoxide_amd.fdtd.Simulation` built with no device, handed over with a
-step count. The native loop thread admits jobs in arrival order while the admitted jobs'
-device footprint fits the HBM budget, steps all admitted jobs together in segments (batched
-launches replayed from a captured hipGraph), admits new arrivals at each segment boundary, and
-hands each finished job's fields and FP64 transform sums back to its simulation. A job's
+step count. The native loop thread admits jobs while the admitted jobs' device footprint fits
+the HBM budget, steps the admitted jobs together in segments (batched launches replayed from a
+captured hipGraph; by default those with the least work left, enough of them to fill the
+device), admits new arrivals at each segment boundary, and hands each finished job's fields
+and FP64 transform sums back to its simulation. A job's
 results are the same bits as running it alone. photonoxide has no counterpart; its nearest is
 the process-level farm (src/farm.rs:1-56).
 """
@@ -25,14 +26,17 @@ class Server:
     max_jobs: jobs admitted at once (0: 256); segment: steps per batched segment (0: 32);
     direct: launch each segment's kernels directly instead of replaying a hipGraph;
     autotune: trial launches per tiling for the first job of each class of grid shape (2D or 3D,
-    each extent's power of two; 0: the default tiling).
+    each extent's power of two; 0: the default tiling);
+    shortest_first: each segment steps the admitted jobs with the least work left until their
+    cells reach fill_cells (0: 2^24), and the queue admits its smallest job first (False: every
+    admitted job steps every segment, admitted in arrival order).
     """
 
     def __init__(self, gpu: Gpu, hbm_budget: int = 0, max_jobs: int = 0, segment: int = 0,
-                 direct: bool = False, autotune: int = 0):
+                 direct: bool = False, autotune: int = 0, shortest_first: bool = True, fill_cells: int = 0):
         self._gpu = gpu
         cfg = N.ServerConfig(int(hbm_budget), int(max_jobs), int(segment), 1 if direct else 0,
-                             int(autotune))
+                             int(autotune), 1 if shortest_first else 0, 0, int(fill_cells))
         h = C.c_void_p()
         N.check(N.lib.pxfdtd_server_create(gpu.handle, C.byref(cfg), C.byref(h)))
         self._h = h
@@ -54,6 +58,10 @@ class Server:
         """Blocks until the job is done (raises if it failed, was cancelled or timed out)."""
         N.check(N.lib.pxfdtd_server_wait(self._h, int(job), float(timeout)))
         return self._sims[job]
+
+    def cancel(self, job: int):
+        """Cancels a queued or running job; a running one stops at the next segment boundary."""
+        N.check(N.lib.pxfdtd_server_cancel(self._h, int(job)))
 
     def times(self, job: int):
         """(submitted, admitted, finished), seconds since the server started."""

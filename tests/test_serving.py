@@ -144,3 +144,66 @@ def test_autotuned_tiling_changes_no_bit(gpu):
         t = product_job(sp)
         srv.wait(srv.submit(t, sp["steps"]), timeout=60)
     assert same_bits(spectra(t), spectra(base))
+
+
+def test_cancel_a_queued_and_a_running_job(gpu):
+    """One job at a time (max_jobs 1), admitted in arrival order: a long job is cancelled while
+    it runs, and stops at a segment boundary with the fields of that many steps alone; the job
+    queued behind it is cancelled before admission and never steps; a third job still finishes
+    with its own bits."""
+    from photonoxide_amd import _native as N
+    from photonoxide_amd.serving import CANCELLED, DONE, RUNNING, Server
+    print("case cancel: a running 512x512 job of 20000 steps, a queued one, a finishing one", flush=True)
+    long_spec = job_spec(512, 512, 1, 20000, 51)
+    queued_spec = job_spec(100, 80, 1, 50, 52)
+    last_spec = job_spec(90, 70, 1, 120, 53)
+    K = 16
+    with Server(gpu, segment=K, max_jobs=1, shortest_first=False) as srv:
+        s_long, s_q, s_last = product_job(long_spec), product_job(queued_spec), product_job(last_spec)
+        j_long = srv.submit(s_long, long_spec["steps"])
+        j_q = srv.submit(s_q, queued_spec["steps"])
+        j_last = srv.submit(s_last, last_spec["steps"])
+        t0 = time.perf_counter()
+        while srv.state(j_long) != RUNNING and time.perf_counter() - t0 < 60:
+            time.sleep(0.002)
+        time.sleep(0.05)
+        srv.cancel(j_q)
+        assert srv.state(j_q) == CANCELLED
+        srv.cancel(j_long)
+        with pytest.raises(N.NativeError, match="cancelled"):
+            srv.wait(j_long, timeout=120)
+        srv.wait(j_last, timeout=120)
+        assert srv.state(j_long) == CANCELLED and srv.state(j_last) == DONE
+        srv.cancel(j_last)  # done: left as it is
+        assert srv.state(j_last) == DONE
+        st = srv.stats()
+    reached = s_long.steps()
+    print(f"the long job stopped after {reached} of {long_spec['steps']} steps; {st}", flush=True)
+    assert 0 < reached < long_spec["steps"] and reached % K == 0
+    assert s_q.steps() == 0
+    assert same_bits(fields_of(s_long), fields_of(alone(gpu, dict(long_spec, steps=reached))))
+    assert same_bits(spectra(s_last), spectra(alone(gpu, last_spec)))
+
+
+def test_shortest_first_finishes_short_jobs_first_with_the_same_bits(gpu):
+    """Shortest remaining work first with a fill of one cell: each segment steps the one job
+    with the least work left, so the jobs finish in order of their work (the long one, first
+    submitted, last), each with the bits of its alone run."""
+    from photonoxide_amd.serving import Server
+    print("case shortest first, fill 1", flush=True)
+    specs = [job_spec(300, 200, 1, 400, 61), job_spec(80, 60, 1, 100, 62), job_spec(120, 90, 1, 60, 63),
+             job_spec(48, 40, 32, 40, 64)]
+    ref = [spectra(alone(gpu, sp)) for sp in specs]
+    with Server(gpu, segment=16, shortest_first=True, fill_cells=1) as srv:
+        sims = [product_job(sp) for sp in specs]
+        jobs = [srv.submit(s, sp["steps"]) for s, sp in zip(sims, specs)]
+        for j in jobs:
+            srv.wait(j, timeout=120)
+        ends = [srv.times(j)[2] for j in jobs]
+    work = [sp["n"][0] * sp["n"][1] * sp["n"][2] * sp["steps"] for sp in specs]
+    print("work", work, "ends", ends, flush=True)
+    assert np.argmax(ends) == 0
+    order = np.argsort(ends)
+    assert all(work[order[q]] <= work[order[q + 1]] for q in range(len(specs) - 1)), (work, ends)
+    for q, s in enumerate(sims):
+        assert same_bits(spectra(s), ref[q]), q
