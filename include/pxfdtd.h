@@ -545,7 +545,7 @@ typedef struct {
                             smallest job first. The others wait resident, so short jobs finish
                             early; results are the same bits either way */
     int32_t pad;
-    int64_t fill_cells;  /* order 1: cells a segment's jobs fill the device with; 0: 2^24 */
+    int64_t fill_cells;  /* order 1: cells a segment's jobs fill the device with; 0: 2^26 */
 } pxfdtd_server_config;
 enum {
     PXFDTD_JOB_QUEUED = 0,

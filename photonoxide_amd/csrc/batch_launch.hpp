@@ -20,10 +20,11 @@ struct BatchBlock {
     int32_t job, first;  // a workgroup's job and its first of four units
 };
 
-// One step s of a segment for the workgroups [0, count) of a kernel class: `kind` 3 (the
-// aligned march, v rows per strip) or 4 (the 2D march, v its field set), coefficient mode
-// `mode`, CPML variant `ax`.
+// One step s of a segment for a kernel class: `kind` 3 (the aligned march, v rows per strip) or
+// 4 (the 2D march, v its field set), coefficient mode `mode`, CPML variant `ax`. blocks[0,
+// *count) (device memory: the active jobs' workgroups this segment) out of at most `capacity`.
 void launch_batch_class(int kind, int mode, int v, int ax, const JobStep<float> *jobs, const JobSeg *seg,
-                        const BatchBlock *blocks, const int4 *units, int count, int s, hipStream_t stream);
+                        const BatchBlock *blocks, const int *count, const int4 *units, int capacity, int s,
+                        hipStream_t stream);
 
 }  // namespace pxfdtd

@@ -148,7 +148,11 @@ struct pxfdtd_server {
         int first, count;       // blocks
     };
     std::vector<Class> classes;
-    Buf d_jobs, d_colds, d_seg, d_blocks, d_units, d_waves, d_phases;
+    std::vector<BatchBlock> all_blocks;  // every table job's workgroups, class after class
+    Buf d_jobs, d_colds, d_seg, d_blocks, d_counts, d_units, d_waves, d_phases;
+    BatchBlock *h_blocks = nullptr;  // pinned: this segment's active workgroups, per class
+    int32_t *h_counts = nullptr;     // pinned: their number per class
+    size_t cap_blocks = 0, cap_counts = 0;
     Buf d_tapjobs, d_tjob, d_tlocal, d_tindex, d_twhich, d_dftjobs, d_djob, d_dlocal;
     long long ntaps = 0, nvals = 0;
     double *h_waves = nullptr, *h_phases = nullptr;  // pinned
@@ -225,16 +229,18 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
     return PXFDTD_OK;
 }
 
-static void launch_class(pxfdtd_server *S, const pxfdtd_server::Class &c, int s) {
+static void launch_class(pxfdtd_server *S, int q, int s) {
+    const pxfdtd_server::Class &c = S->classes[q];
     launch_batch_class(c.kind, c.mode, c.v, c.ax, S->d_jobs.as<JobStep<float>>(), S->d_seg.as<JobSeg>(),
-                       S->d_blocks.as<BatchBlock>() + c.first, S->d_units.as<int4>(), c.count, s, S->stream);
+                       S->d_blocks.as<BatchBlock>() + c.first, S->d_counts.as<int32_t>() + q,
+                       S->d_units.as<int4>(), c.count, s, S->stream);
 }
 
 // one segment's launches on the server's stream (captured into the graph, or run directly)
 static int launch_segment(pxfdtd_server *S) {
     const int K = S->cfg.segment;
     for (int s = 0; s < K; s++) {
-        for (const auto &c : S->classes) launch_class(S, c, s);
+        for (int q = 0; q < (int)S->classes.size(); q++) launch_class(S, q, s);
         if (S->ntaps) {
             const long long blocks = (S->ntaps + 255) / 256;
             hipLaunchKernelGGL((tap_batch<float>), dim3((unsigned)blocks), dim3(256), 0, S->stream,
@@ -363,6 +369,10 @@ static int server_rebuild(pxfdtd_server *S) {
     TRY(upload_vec(S->d_jobs, steps, st));
     if (!colds.empty()) HIPCHK(hipMemcpyAsync(S->d_colds.p, colds.data(), colds.size() * sizeof(StepCold<float>), hipMemcpyHostToDevice, st));
     TRY(upload_vec(S->d_blocks, all, st));
+    S->all_blocks = all;
+    TRY(S->d_counts.reserve(std::max<size_t>(S->classes.size(), 1) * sizeof(int32_t)));
+    TRY(pinned(S->h_blocks, S->cap_blocks, all.size()));
+    TRY(pinned(S->h_counts, S->cap_counts, S->classes.size()));
     TRY(upload_vec(S->d_units, units, st));
     TRY(upload_vec(S->d_tapjobs, tapjobs, st));
     TRY(upload_vec(S->d_tjob, tjob, st));
@@ -413,7 +423,7 @@ static int server_segment_begin(pxfdtd_server *S) {
     if (S->cfg.order == 1) {
         auto rest = [&](int j) { return (double)S->table[j]->left * (double)S->table[j]->r->ncells; };
         std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return rest(a) < rest(b); });
-        const double fill = S->cfg.fill_cells > 0 ? (double)S->cfg.fill_cells : (double)(1 << 24);
+        const double fill = S->cfg.fill_cells > 0 ? (double)S->cfg.fill_cells : (double)(1 << 26);
         double filled = 0;
         for (int j : cand) {
             if (filled >= fill) break;
@@ -437,7 +447,20 @@ static int server_segment_begin(pxfdtd_server *S) {
             cells += (double)a * J->r->ncells;
         }
     }
+    // each class's workgroups of the jobs that step, in table order
+    for (size_t q = 0; q < S->classes.size(); q++) {
+        const pxfdtd_server::Class &c = S->classes[q];
+        int n = 0;
+        for (int k = c.first; k < c.first + c.count; k++)
+            if (go[S->all_blocks[k].job]) S->h_blocks[c.first + n++] = S->all_blocks[k];
+        S->h_counts[q] = n;
+    }
     hipStream_t st = S->stream;
+    if (!S->all_blocks.empty())
+        HIPCHK(hipMemcpyAsync(S->d_blocks.p, S->h_blocks, S->all_blocks.size() * sizeof(BatchBlock),
+                              hipMemcpyHostToDevice, st));
+    if (!S->classes.empty())
+        HIPCHK(hipMemcpyAsync(S->d_counts.p, S->h_counts, S->classes.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     if (S->cap_waves) HIPCHK(hipMemcpyAsync(S->d_waves.p, S->h_waves, S->cap_waves * 8, hipMemcpyHostToDevice, st));
     if (S->cap_phases) HIPCHK(hipMemcpyAsync(S->d_phases.p, S->h_phases, S->cap_phases * 8, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(S->d_seg.p, S->h_seg, (size_t)nj * sizeof(JobSeg), hipMemcpyHostToDevice, st));
@@ -674,6 +697,8 @@ extern "C" void pxfdtd_server_destroy(pxfdtd_server *S) {
     if (S->h_waves) (void)hipHostFree(S->h_waves);
     if (S->h_phases) (void)hipHostFree(S->h_phases);
     if (S->h_seg) (void)hipHostFree(S->h_seg);
+    if (S->h_blocks) (void)hipHostFree(S->h_blocks);
+    if (S->h_counts) (void)hipHostFree(S->h_counts);
     hipStream_t st = S->stream;
     delete S;  // device tables free themselves
     if (st) (void)hipStreamDestroy(st);

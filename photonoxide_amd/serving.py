@@ -28,7 +28,7 @@ class Server:
     autotune: trial launches per tiling for the first job of each class of grid shape (2D or 3D,
     each extent's power of two; 0: the default tiling);
     shortest_first: each segment steps the admitted jobs with the least work left until their
-    cells reach fill_cells (0: 2^24), and the queue admits its smallest job first (False: every
+    cells reach fill_cells (0: 2^26), and the queue admits its smallest job first (False: every
     admitted job steps every segment, admitted in arrival order).
     """
 
