@@ -289,6 +289,11 @@ typedef struct {
  * (src/fdtd.rs:570-653) */
 int pxfdtd_simulation_new(const pxfdtd_grid *g, const pxfdtd_boundaries *b,
                           const pxfdtd_scene *eps, double courant, pxfdtd_simulation **out);
+/* pxfdtd_simulation_new with the averaging done on `dev` (coefficient construction on the
+ * device: the 512 samples per value and shape are the set-up's dominant cost in 3D), the same
+ * bits as the host's: each value's samples are the host's sequence of IEEE 754 operations */
+int pxfdtd_simulation_new_on_device(pxfdtd_device *dev, const pxfdtd_grid *g, const pxfdtd_boundaries *b,
+                                    const pxfdtd_scene *eps, double courant, pxfdtd_simulation **out);
 /* the permittivity each value of E sees given directly, [3][N] */
 int pxfdtd_simulation_new_eps(const pxfdtd_grid *g, const pxfdtd_boundaries *b,
                               const double *eps3n, double courant, pxfdtd_simulation **out);

@@ -196,6 +196,8 @@ SIGNATURES = {
     "pxfdtd_adjoint_destroy": (None, [vp]),
     "pxfdtd_simulation_new": (C.c_int, [C.POINTER(Grid), C.POINTER(Bounds), C.POINTER(Scene), C.c_double, C.POINTER(vp)]),
     "pxfdtd_simulation_new_eps": (C.c_int, [C.POINTER(Grid), C.POINTER(Bounds), pd, C.c_double, C.POINTER(vp)]),
+    "pxfdtd_simulation_new_on_device": (C.c_int, [vp, C.POINTER(Grid), C.POINTER(Bounds), C.POINTER(Scene), C.c_double,
+                                                  C.POINTER(vp)]),
     "pxfdtd_simulation_destroy": (None, [vp]),
     "pxfdtd_simulation_conductivity": (C.c_int, [vp, C.POINTER(Scene)]),
     "pxfdtd_simulation_conductor": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32]),

@@ -30,6 +30,7 @@ using pxfdtd::C64;
 
 int px_fail(int code, const char *fmt, ...);  // resident.hip
 int flux_to_host(pxfdtd_device *dev, const pxfdtd_flux_face *faces, int32_t n, double *out, void *stream);  // flux.hip
+int averaged_on_device(pxfdtd_device *dev, const pxfdtd_grid *g, const pxfdtd_scene *sc, double *eps3n);  // coefficients.hip
 
 #define TRY(x)                            \
     do {                                  \
@@ -432,6 +433,26 @@ extern "C" int pxfdtd_simulation_new(const pxfdtd_grid *g, const pxfdtd_boundari
         p[c] += 0.5 * hs[c];
         return averaged(eps, c, p, hs);
     }, out);
+}
+
+namespace pxfdtd_host {
+double averaged_at(const pxfdtd_scene *sc, int comp, const double centre[3], const double h[3]) {
+    return averaged(sc, comp, centre, h);
+}
+}  // namespace pxfdtd_host
+
+// Simulation::diagonal with the averaging on the device (coefficients.hip): the same bits as
+// pxfdtd_simulation_new
+extern "C" int pxfdtd_simulation_new_on_device(pxfdtd_device *dev, const pxfdtd_grid *g, const pxfdtd_boundaries *b,
+                                               const pxfdtd_scene *eps, double courant, pxfdtd_simulation **out) {
+    if (!eps) return px_fail(PXFDTD_ERR_INVALID, "a scene is required");
+    if (!dev) return px_fail(PXFDTD_ERR_INVALID, "a device is required");
+    TRY(check(g, b, courant));
+    const long long N = (long long)g->nx * g->ny * g->nz;
+    std::unique_ptr<double[]> e(new (std::nothrow) double[(size_t)(3 * N)]);
+    if (!e) return px_fail(PXFDTD_ERR_MEMORY, "no host memory for %lld averaged values", 3 * N);
+    TRY(averaged_on_device(dev, g, eps, e.get()));
+    return make(g, b, courant, [&](int c, long long r) { return e[c * N + r]; }, out);
 }
 
 extern "C" int pxfdtd_simulation_new_eps(const pxfdtd_grid *g, const pxfdtd_boundaries *b,

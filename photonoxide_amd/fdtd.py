@@ -385,13 +385,20 @@ class Simulation:
 
     # -- construction ---------------------------------------------------------------------
     @staticmethod
-    def new(grid: Grid3d, eps, boundaries: Boundaries, courant: float) -> "Simulation":
+    def new(grid: Grid3d, eps, boundaries: Boundaries, courant: float, device: "Gpu" = None) -> "Simulation":
         """The permittivity averaged per component over 8^3 samples of each cell
-        (Simulation::new / diagonal, src/fdtd.rs:570-653). `eps` is a :class:`Scene`."""
+        (Simulation::new / diagonal, src/fdtd.rs:570-653). `eps` is a :class:`Scene`. With a
+        `device`, the averaging runs there (the same bits); the simulation still has no device
+        to step on until set_device."""
         h = C.c_void_p()
         scene, keep = eps.native()
-        N.check(N.lib.pxfdtd_simulation_new(C.byref(_grid(grid)), C.byref(boundaries.native()),
-                                            C.byref(scene), courant, C.byref(h)))
+        if device is not None:
+            N.check(N.lib.pxfdtd_simulation_new_on_device(device.handle, C.byref(_grid(grid)),
+                                                          C.byref(boundaries.native()), C.byref(scene),
+                                                          courant, C.byref(h)))
+        else:
+            N.check(N.lib.pxfdtd_simulation_new(C.byref(_grid(grid)), C.byref(boundaries.native()),
+                                                C.byref(scene), courant, C.byref(h)))
         return Simulation(h, grid, boundaries)
 
     @staticmethod
