@@ -316,6 +316,14 @@ int pxfdtd_simulation_add_dft(pxfdtd_simulation *s, const int32_t low[3], const 
 /* poles [np][4]: (0, strength, resonance, damping) Lorentz or (1, plasma, damping, 0) Drude */
 int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf, int32_t np,
                                  const double *poles, const pxfdtd_scene *inside);
+/* Dispersive::amplification (src/fdtd/media.rs:258-299): the largest |z| over the modes of a
+ * plane wave stepped in the medium eps_inf with poles [np][4] (as pxfdtd_simulation_add_medium
+ * takes them) and conductivity sigma at time step dt, for (K dt)^2 = u: the roots of the
+ * update's characteristic polynomial, by Aberth's iteration. pxfdtd_simulation_add_medium
+ * admits a medium when this is at most 1 + 1e-6 at 200 values of u up to 4 sum 1/h^2 dt^2,
+ * without conductivity and with the largest inside (Dispersive::stable, media.rs:300-309). */
+int pxfdtd_medium_amplification(double eps_inf, int32_t np, const double *poles, double sigma, double dt,
+                                double u, double *out);
 /* Simulation::add_flux (src/fdtd/monitors.rs:470-491): the flux towards +axis through the plane
  * halfway between nodes `plane` and plane + 1, within window[w] = {lo, hi} (the half-planes
  * after nodes lo and hi) along the w-th of the axes after `axis` (lo < 0: the whole axis), at

@@ -226,6 +226,7 @@ SIGNATURES = {
     "pxfdtd_simulation_add_probe": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, pi32]),
     "pxfdtd_simulation_add_dft": (C.c_int, [vp, pi32, pi32, C.c_int32, pd, pi32]),
     "pxfdtd_simulation_add_medium": (C.c_int, [vp, C.c_double, C.c_int32, pd, C.POINTER(Scene)]),
+    "pxfdtd_medium_amplification": (C.c_int, [C.c_double, C.c_int32, pd, C.c_double, C.c_double, C.c_double, pd]),
     "pxfdtd_simulation_set_device": (C.c_int, [vp, vp, C.c_int32]),
     "pxfdtd_simulation_run": (C.c_int, [vp, C.c_int64]),
     "pxfdtd_simulation_run_until": (C.c_int, [vp, C.c_double]),

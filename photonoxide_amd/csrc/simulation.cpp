@@ -1475,6 +1475,145 @@ extern "C" int pxfdtd_simulation_add_flux_box(pxfdtd_simulation *s, const int32_
     return PXFDTD_OK;
 }
 
+// ---------------------------------------------------------------- media stability
+// |z| (the reference's portable::complex::abs is a hypot; this is the C library's)
+static double cabs_(C64 z) { return std::hypot(z.re, z.im); }
+
+// media.rs:446-455: the product of two polynomials, coefficients from the constant term up
+static std::vector<double> poly_multiply(const std::vector<double> &a, const std::vector<double> &b) {
+    std::vector<double> out(a.size() + b.size() - 1, 0.0);
+    for (size_t i = 0; i < a.size(); i++)
+        for (size_t j = 0; j < b.size(); j++) out[i + j] += a[i] * b[j];
+    return out;
+}
+
+// media.rs:457-510: the roots of a real polynomial by Aberth's simultaneous iteration (O. Aberth,
+// Math. Comput. 27, 339 (1973)), from a circle of the roots' geometric mean modulus turned off
+// the real axis, at most 500 sweeps, until no root moves by more than 1e-15 of its modulus
+static std::vector<C64> poly_roots(std::vector<double> p) {
+    while (p.size() > 1 && p.back() == 0.0) p.pop_back();
+    const int n = (int)p.size() - 1;
+    if (n <= 0) return {};
+    const double lead = p[n];
+    for (double &c : p) c = c / lead;
+    const double radius = std::max(pxfdtd::ppow(std::fabs(p[0]), 1.0 / n), 1e-3);
+    std::vector<C64> z(n);
+    for (int k = 0; k < n; k++) {
+        double sn = 0.0, cs = 1.0;
+        pxfdtd::psincos(TAU * (k + 0.25) / n, sn, cs);
+        z[k] = {radius * cs, radius * sn};
+    }
+    auto eval = [&](C64 x, C64 &v, C64 &d) {
+        v = {0.0, 0.0};
+        d = {0.0, 0.0};
+        for (int q = n; q >= 0; q--) {
+            const C64 dx = cmul(d, x), vx = cmul(v, x);
+            d = {dx.re + v.re, dx.im + v.im};
+            v = {vx.re + p[q], vx.im};
+        }
+    };
+    for (int it = 0; it < 500; it++) {
+        double largest = 0.0;
+        for (int k = 0; k < n; k++) {
+            C64 v, d;
+            eval(z[k], v, d);
+            if (v.re == 0.0 && v.im == 0.0) continue;
+            const C64 ratio = cdiv(v, d);
+            C64 rep = {0.0, 0.0};
+            for (int j = 0; j < n; j++) {
+                if (j == k) continue;
+                const C64 t = cdiv({1.0, 0.0}, {z[k].re - z[j].re, z[k].im - z[j].im});
+                rep = {rep.re + t.re, rep.im + t.im};
+            }
+            const C64 rr = cmul(ratio, rep);
+            const C64 step = cdiv(ratio, {1.0 - rr.re, -rr.im});
+            if (std::isfinite(step.re) && std::isfinite(step.im)) {
+                z[k] = {z[k].re - step.re, z[k].im - step.im};
+                largest = std::max(largest, cabs_(step) / std::max(cabs_(z[k]), 1e-300));
+            }
+        }
+        if (largest < 1e-15) break;
+    }
+    return z;
+}
+
+// Dispersive::amplification (media.rs:258-299): the largest |z| over the modes of a plane wave
+// stepped in the medium (eps_inf, terms {alpha, xi, gamma}) with conductivity sigma at time
+// step dt, for (K dt)^2 = u: the roots of the update's characteristic polynomial,
+// eps_inf (z - 1)^2 + (sigma dt / 2)(z^2 - 1) + u z times every term's z^2 - alpha z - xi, plus
+// each term's (gamma / 4)(z^2 - 1)^2 times the others'
+static double medium_amplification(double eps_inf, const std::vector<std::array<double, 3>> &terms, double sigma,
+                                   double dt, double u) {
+    auto product = [&](int skip) {
+        std::vector<double> acc = {1.0};
+        for (int i = 0; i < (int)terms.size(); i++)
+            if (i != skip) acc = poly_multiply(acc, {-terms[i][1], -terms[i][0], 1.0});
+        return acc;
+    };
+    const double e = eps_inf, half = sigma * dt / 2.0;
+    std::vector<double> poly = poly_multiply({e - half, u - 2.0 * e, e + half}, product(-1));
+    for (int i = 0; i < (int)terms.size(); i++) {
+        const double g = terms[i][2];
+        const std::vector<double> t = poly_multiply({g / 4.0, 0.0, -g / 2.0, 0.0, g / 4.0}, product(i));
+        for (size_t c = 0; c < poly.size() && c < t.size(); c++) poly[c] += t[c];
+    }
+    double m = 0.0;
+    for (const C64 &z : poly_roots(poly)) m = std::max(m, cabs_(z));
+    return m;
+}
+
+// Dispersive::stable (media.rs:300-309): every (K dt)^2 up to top dt^2 in 200 steps (K = 0,
+// a static field's double root at z = 1, left out) amplifies by at most 1 + 1e-6
+static bool medium_stable(double eps_inf, const std::vector<std::array<double, 3>> &terms, double sigma, double dt,
+                          double top) {
+    const int points = 200;
+    const double most = top * dt * dt;
+    for (int i = 1; i <= points; i++) {
+        const double u = most * i / points;
+        if (!(medium_amplification(eps_inf, terms, sigma, dt, u) <= 1.0 + 1e-6)) return false;
+    }
+    return true;
+}
+
+// a medium's terms {alpha, xi, gamma} at dt from poles [np][4] (Pole::coefficients, media.rs:
+// 61-109): kind 0 Lorentz (strength, resonance, damping), 1 Drude (plasma, damping)
+static int medium_terms(int32_t np, const double *poles, double dt, std::vector<std::array<double, 3>> &out) {
+    out.clear();
+    for (int q = 0; q < np; q++) {
+        const double *p = poles + 4 * q;
+        double a, w0sq, gamma;
+        if (p[0] == 0.0) {
+            if (!(p[1] >= 0 && std::isfinite(p[1])) || !(p[2] > 0 && std::isfinite(p[2])) ||
+                !(p[3] >= 0 && std::isfinite(p[3])))
+                return px_fail(PXFDTD_ERR_INVALID, "a Lorentz term needs strength >= 0, resonance > 0, damping >= 0");
+            const double w0 = TAU * p[2];
+            a = p[1] * w0 * w0;
+            w0sq = w0 * w0;
+            gamma = TAU * p[3];
+        } else {
+            if (!(p[1] > 0 && std::isfinite(p[1])) || !(p[2] >= 0 && std::isfinite(p[2])))
+                return px_fail(PXFDTD_ERR_INVALID, "a Drude term needs plasma > 0, damping >= 0");
+            const double wp = TAU * p[1];
+            a = wp * wp;
+            w0sq = 0.0;
+            gamma = TAU * p[2];
+        }
+        const double d = 1.0 + gamma * dt / 2.0;
+        out.push_back({(2.0 - w0sq * dt * dt) / d, (gamma * dt / 2.0 - 1.0) / d, a * dt * dt / d});
+    }
+    return PXFDTD_OK;
+}
+
+extern "C" int pxfdtd_medium_amplification(double eps_inf, int32_t np, const double *poles, double sigma, double dt,
+                                           double u, double *out) {
+    if (!(eps_inf > 0 && std::isfinite(eps_inf)) || np < 0 || !(dt > 0) || !std::isfinite(u))
+        return px_fail(PXFDTD_ERR_INVALID, "eps_inf > 0, np >= 0, dt > 0 and a finite u are needed");
+    std::vector<std::array<double, 3>> terms;
+    TRY(medium_terms(np, poles, dt, terms));
+    *out = medium_amplification(eps_inf, terms, sigma, dt, u);
+    return PXFDTD_OK;
+}
+
 // src/fdtd/media.rs:61-109,733-825
 extern "C" int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf, int32_t np,
                                             const double *poles, const pxfdtd_scene *inside_) {
@@ -1486,43 +1625,20 @@ extern "C" int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf
     const double dt = s->dt;
     Group G;
     G.poles = np;
+    std::vector<std::array<double, 3>> terms;
+    TRY(medium_terms(np, poles, dt, terms));
     double gsum = 0.0;
-    double top_w0dt = 0.0;
     for (int q = 0; q < np; q++) {
-        const double *p = poles + 4 * q;
-        double a, w0sq, gamma;
-        if (p[0] == 0.0) {
-            if (!(p[1] >= 0) || !(p[2] > 0) || !(p[3] >= 0))
-                return px_fail(PXFDTD_ERR_INVALID, "a Lorentz term needs strength >= 0, resonance > 0, damping >= 0");
-            const double w0 = TAU * p[2];
-            a = p[1] * w0 * w0;
-            w0sq = w0 * w0;
-            gamma = TAU * p[3];
-            top_w0dt = std::max(top_w0dt, w0 * dt);
-        } else {
-            if (!(p[1] > 0) || !(p[2] >= 0))
-                return px_fail(PXFDTD_ERR_INVALID, "a Drude term needs plasma > 0, damping >= 0");
-            const double wp = TAU * p[1];
-            a = wp * wp;
-            w0sq = 0.0;
-            gamma = TAU * p[2];
-        }
-        const double d = 1.0 + gamma * dt / 2.0;
-        G.coef[q][0] = (2.0 - w0sq * dt * dt) / d;
-        G.coef[q][1] = (gamma * dt / 2.0 - 1.0) / d;
-        G.coef[q][2] = a * dt * dt / d;
+        for (int m = 0; m < 3; m++) G.coef[q][m] = terms[q][m];
         gsum += G.coef[q][2];
     }
-    // stability, as the reference states it in practice (media.rs:36-40): w0 dt <= 2 for each
-    // term and a Courant number within sqrt(eps_inf)
+    // the curl's largest eigenvalue, 4 sum 1/h^2 over the axes a field varies along
+    // (bloch::curl_bound, src/fdtd/bloch.rs:31-43, with k = 0 on periodic sides)
     double curl = 0.0;
     const double hs[3] = {s->g.dx, s->g.dy, s->g.dz};
     for (int a = 0; a < 3; a++)
-        if (!(s->b.periodic[a] && s->n[a] == 1)) curl += 1.0 / (hs[a] * hs[a]);
-    if (top_w0dt > 2.0 || dt * std::sqrt(curl) > std::sqrt(eps_inf) * (1.0 + 1e-12))
-        return px_fail(PXFDTD_ERR_INVALID,
-                       "the medium isn't stable at the time step %g um/c (a term's w0 dt above 2, or a "
-                       "Courant number above sqrt(eps_inf)): take a smaller step", dt);
+        if (!(s->b.periodic[a] && s->n[a] == 1)) curl += pxfdtd::ppowi(hs[a], -2);
+    const double top = 4.0 * curl;
     G.chi = gsum / (4.0 * dt);
     const long long N = s->N;
     std::vector<double> sig;
@@ -1558,6 +1674,14 @@ extern "C" int pxfdtd_simulation_add_medium(pxfdtd_simulation *s, double eps_inf
             sig.insert(sig.end(), fsig[t].begin(), fsig[t].end());
         }
     }
+    // stability (media.rs:783-793): without conductivity and with the largest inside
+    double largest = 0.0;
+    for (double v : sig) largest = std::max(largest, v);
+    for (double sigma : {0.0, largest})
+        if (!medium_stable(eps_inf, terms, sigma, dt, top))
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "the medium isn't stable at the time step %g um/c (a term's w0 dt above 2, or a "
+                           "Courant number above sqrt(eps_inf)): take a smaller step", dt);
     const double e = eps_inf;
     // the values taken from earlier media leave those, each group compacted once
     std::vector<char> dirty(s->groups.size(), 0);
