@@ -627,6 +627,63 @@ int pxfdtd_adjoint_info(const pxfdtd_adjoint *a, int64_t *checkpoints, int64_t *
                         int64_t *recomputed_steps);
 void pxfdtd_adjoint_destroy(pxfdtd_adjoint *a);
 
+/* ------------------------------------- designs and the frequency-domain adjoint (two runs) */
+/* Design (src/fdtd/adjoint.rs:63-75): the cells [lo, hi) along each axis, each with a density
+ * rho and eps(rho) = eps[0] + rho (eps[1] - eps[0]); densities numbered i fastest, then j, k. */
+typedef struct {
+    int32_t lo[3], hi[3];
+    double eps[2];
+} pxfdtd_design;
+/* Simulation::with_design (adjoint.rs:366-423): every E value whose cell reaches the design's
+ * cells gets the average (8^3 samples, harmonic along the component) of eps(rho) in the design's
+ * cells and `background` elsewhere; before the run starts. Refused: a design off the grid or
+ * whose values reach a CPML, densities of the wrong count or giving eps <= 0, values with
+ * conductivity or a dispersive medium. */
+int pxfdtd_simulation_with_design(pxfdtd_simulation *s, const pxfdtd_design *d, const double *density,
+                                  int64_t n, const pxfdtd_scene *background);
+/* Simulation::add_design_monitor (adjoint.rs:435-470): E's transforms over the values the
+ * design's cells reach at the objective's frequencies (c/um), from the next step on. */
+int pxfdtd_simulation_add_design_monitor(pxfdtd_simulation *s, const pxfdtd_design *d, int32_t nf,
+                                         const double *freqs, int32_t *number);
+/* Design::values (adjoint.rs:196-255): how many values design monitor n reads (returned), and
+ * with comp_ijk non-null each one's (component, i, j, k), in the order the gradients take them. */
+int64_t pxfdtd_simulation_design_values(const pxfdtd_simulation *s, int32_t n, int32_t *comp_ijk);
+/* A design monitor's transform of each value it reads at frequency f, out [values][2]. */
+int pxfdtd_simulation_design_transform(pxfdtd_simulation *s, int32_t n, int32_t f, double *out);
+/* Term (adjoint.rs:285-338), what an objective's derivative dF/da (Wirtinger, conj(a) held) is
+ * taken of: kind 0 a mode monitor's amplitude (monitor, mode, direction 0 forward 1 backward),
+ * 1 a flux monitor's flux at its frequency number `frequency` (derivative_re real), 2 a
+ * transform monitor's value of field's component at `at`, frequency number `frequency`. */
+typedef struct {
+    int32_t kind;
+    int32_t monitor, mode, direction, frequency, field, component, at[3];
+    double derivative_re, derivative_im;
+} pxfdtd_term;
+/* Simulation::adjoint (adjoint.rs:496-569): the adjoint run, at rest, for the objective whose
+ * derivatives the terms give: the same grid, media and boundaries, no sources or monitors but
+ * design monitor `monitor` (transforms at zero), and currents J = -g/D, M = g/D at the values
+ * the terms read, D the CPMLs' symmetrizer (adjoint.rs:704-734), each a sum of Gaussian pulses
+ * `bandwidth` wide (c/um), two a quarter period apart per frequency, whose transforms at the
+ * design monitor's frequencies are the targets (the real system of adjoint.rs:575-621). Run the
+ * forward and the adjoint until their fields die away, then pxfdtd_simulation_design_gradient. */
+int pxfdtd_simulation_adjoint(pxfdtd_simulation *s, const pxfdtd_term *terms, int32_t n, int32_t monitor,
+                              double bandwidth, pxfdtd_simulation **out);
+/* Simulation::permittivity_gradient (adjoint.rs:831-855): dF/deps per value design monitor
+ * `monitor` reads (pxfdtd_simulation_design_values' order), sum over the frequencies of
+ * 2 Re(i w~ E_adj E), w~ = (2/dt) sin(w dt/2), from this run (the forward) and `adjoint`. */
+int pxfdtd_simulation_permittivity_gradient(pxfdtd_simulation *s, pxfdtd_simulation *adjoint, int32_t monitor,
+                                            double *out);
+/* Simulation::gradient (adjoint.rs:857-879): dF/drho per design cell, the permittivity gradient
+ * times (eps[1] - eps[0]) times each cell's share of each value's cell. */
+int pxfdtd_simulation_design_gradient(pxfdtd_simulation *s, pxfdtd_simulation *adjoint, int32_t monitor,
+                                      double *out);
+/* Simulation::run_until_converged (adjoint.rs:904-964): runs in blocks of `interval` (um/c)
+ * until, after every source has finished, no series of any transform monitor has moved over a
+ * block by more than `tolerance` of its own largest value, or until the time reaches `limit`;
+ * *settled whether they had. */
+int pxfdtd_simulation_run_until_converged(pxfdtd_simulation *s, double tolerance, double interval, double limit,
+                                          int32_t *settled);
+
 #ifdef __cplusplus
 }
 #endif

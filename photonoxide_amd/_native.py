@@ -140,6 +140,16 @@ class FluxFace(C.Structure):
                 ("step_c", C.c_double), ("series", C.c_void_p * 6)]
 
 
+class Design(C.Structure):
+    _fields_ = [("lo", i3), ("hi", i3), ("eps", C.c_double * 2)]
+
+
+class Term(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("monitor", C.c_int32), ("mode", C.c_int32), ("direction", C.c_int32),
+                ("frequency", C.c_int32), ("field", C.c_int32), ("component", C.c_int32), ("at", i3),
+                ("derivative_re", C.c_double), ("derivative_im", C.c_double)]
+
+
 vp = C.c_void_p
 SIGNATURES = {
     "pxfdtd_last_error": (C.c_char_p, []),
@@ -245,6 +255,14 @@ SIGNATURES = {
     "pxfdtd_simulation_footprint": (C.c_int, [vp, C.c_int32, pi64]),
     "pxfdtd_simulation_bytes_per_step": (C.c_int, [vp, C.c_int32, pi64]),
     "pxfdtd_simulation_media_bytes_per_step": (C.c_int, [vp, C.c_int32, pi64]),
+    "pxfdtd_simulation_with_design": (C.c_int, [vp, C.POINTER(Design), pd, C.c_int64, C.POINTER(Scene)]),
+    "pxfdtd_simulation_add_design_monitor": (C.c_int, [vp, C.POINTER(Design), C.c_int32, pd, pi32]),
+    "pxfdtd_simulation_design_values": (C.c_int64, [vp, C.c_int32, pi32]),
+    "pxfdtd_simulation_design_transform": (C.c_int, [vp, C.c_int32, C.c_int32, pd]),
+    "pxfdtd_simulation_adjoint": (C.c_int, [vp, C.POINTER(Term), C.c_int32, C.c_int32, C.c_double, C.POINTER(vp)]),
+    "pxfdtd_simulation_permittivity_gradient": (C.c_int, [vp, vp, C.c_int32, pd]),
+    "pxfdtd_simulation_design_gradient": (C.c_int, [vp, vp, C.c_int32, pd]),
+    "pxfdtd_simulation_run_until_converged": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, pi32]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

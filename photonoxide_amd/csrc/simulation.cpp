@@ -123,6 +123,24 @@ struct ModeMonRec {
     std::vector<ModeEntry> modes;
 };
 
+// A value of E whose cell reaches a design's cells, with each design cell's share of that cell
+// (Touched, src/fdtd/adjoint.rs:77-86)
+struct Touched {
+    int comp;
+    int at[3];
+    long long r;
+    std::vector<std::pair<long long, double>> shares;
+};
+
+// A design monitor (DesignMonitor, adjoint.rs:263-283): the design, the values its cells reach
+// and the transform monitor of E over each component's box of them
+struct DesignRec {
+    pxfdtd_design d;
+    std::vector<Touched> touched;
+    std::vector<double> freqs;
+    int dft[3];  // per component, an index into the simulation's transform monitors, or -1
+};
+
 struct Group {
     std::vector<int32_t> comp;
     std::vector<int64_t> r;
@@ -188,10 +206,11 @@ double scene_value(const pxfdtd_scene *sc, int comp, double x, double y, double 
     return v;
 }
 
-// the permittivity of E's value's cell from samples^3 points, harmonic along the component,
-// arithmetic across, with num-complex's division of a real by a complex, e / (e e + 0 0)
-// (src/fdfd/three.rs:563-588)
-double averaged(const pxfdtd_scene *sc, int comp, const double centre[3], const double h[3]) {
+// the permittivity of E's value's cell from samples^3 points of eps(x, y, z), harmonic along
+// the component, arithmetic across, with num-complex's division of a real by a complex,
+// e / (e e + 0 0) (src/fdfd/three.rs:563-588)
+template <typename F>
+double averaged_with(const F &eps, int comp, const double centre[3], const double h[3]) {
     const int samples = 8;
     const double n = samples;
     const int a = comp, b = (comp + 1) % 3, c = (comp + 2) % 3;
@@ -203,13 +222,18 @@ double averaged(const pxfdtd_scene *sc, int comp, const double centre[3], const 
             double inverse = 0.0;
             for (int sa = 0; sa < samples; sa++) {
                 p[a] = centre[a] + ((sa + 0.5) / n - 0.5) * h[a];
-                const double e = scene_value(sc, comp, p[0], p[1], p[2]);
+                const double e = eps(p[0], p[1], p[2]);
                 inverse += 1.0 * e / (e * e + 0.0 * 0.0);
             }
             sum += n * inverse / (inverse * inverse + 0.0 * 0.0);
         }
     }
     return sum / (n * n);
+}
+
+double averaged(const pxfdtd_scene *sc, int comp, const double centre[3], const double h[3]) {
+    return averaged_with([&](double x, double y, double z) { return scene_value(sc, comp, x, y, z); }, comp, centre,
+                         h);
 }
 
 }  // namespace
@@ -237,6 +261,7 @@ struct pxfdtd_simulation {
     std::vector<int> shown;  // the caller's transform monitors: their indices in dfts
     std::vector<FluxRec> fluxes;
     std::vector<ModeMonRec> modemons;
+    std::vector<DesignRec> designs;
     std::vector<Group> groups;
     std::vector<int32_t> member;  // [3][N] group or -1, when media exist
     // the device
@@ -3156,5 +3181,730 @@ extern "C" int pxfdtd_simulation_bytes_per_step(const pxfdtd_simulation *s, int3
     long long slabs = 0;
     for (const Slab &sl : s->slabs) slabs += (long long)sl.psi.size() * (sl.field == PXFDTD_E ? 6 : 5);
     *bytes = (int64_t)((s->N * (9 + 15) + slabs) * (precision == PXFDTD_F32 ? 4 : 8));
+    return PXFDTD_OK;
+}
+
+// ---------------------------------------------------------------- designs and the frequency-domain adjoint
+// src/fdtd/adjoint.rs: a design region's densities, the transforms of E over the values its
+// cells reach, and the adjoint run whose transforms, with the forward run's, give the gradient.
+
+namespace {
+
+long long design_len(const pxfdtd_design &d) {
+    return (long long)(d.hi[0] - d.lo[0]) * (d.hi[1] - d.lo[1]) * (d.hi[2] - d.lo[2]);
+}
+
+// Design::index (adjoint.rs:95-103): i fastest, then j, then k; -1 outside the region
+long long design_index(const pxfdtd_design &d, long long i, long long j, long long k) {
+    if (i < d.lo[0] || i >= d.hi[0] || j < d.lo[1] || j >= d.hi[1] || k < d.lo[2] || k >= d.hi[2]) return -1;
+    const long long nx = d.hi[0] - d.lo[0], ny = d.hi[1] - d.lo[1];
+    return ((k - d.lo[2]) * ny + (j - d.lo[1])) * nx + (i - d.lo[0]);
+}
+
+// Design::permittivity (adjoint.rs:105-108)
+double design_eps(const pxfdtd_design &d, double rho) { return d.eps[0] + rho * (d.eps[1] - d.eps[0]); }
+
+// Design::cell_at (adjoint.rs:148-167): the design cell holding p, across a periodic side wrapped
+// onto the grid; -1 for none
+long long design_cell_at(const pxfdtd_simulation *s, const pxfdtd_design &d, const double p[3]) {
+    const double origin[3] = {s->g.x0, s->g.y0, s->g.z0}, hs[3] = {s->g.dx, s->g.dy, s->g.dz};
+    long long at[3];
+    for (int a = 0; a < 3; a++) {
+        const long long n = s->n[a];
+        long long m = (long long)std::floor((p[a] - origin[a]) / hs[a]);
+        if (s->b.periodic[a])
+            m = ((m % n) + n) % n;
+        else if (m < 0 || m >= n)
+            return -1;
+        at[a] = m;
+    }
+    return design_index(d, at[0], at[1], at[2]);
+}
+
+// Design::check (adjoint.rs:110-140): on the grid, clear of the CPMLs with the values of E its
+// cells reach, its permittivities finite and positive
+int design_check(const pxfdtd_simulation *s, const pxfdtd_design *d) {
+    if (!d) return px_fail(PXFDTD_ERR_INVALID, "a design is required");
+    for (double e : d->eps)
+        if (!(std::isfinite(e) && e > 0.0))
+            return px_fail(PXFDTD_ERR_INVALID, "a design's permittivities must be finite and positive, got %g and %g",
+                           d->eps[0], d->eps[1]);
+    for (int a = 0; a < 3; a++) {
+        const int n = s->n[a];
+        if (d->lo[a] < 0 || d->lo[a] >= d->hi[a] || d->hi[a] > n)
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "a design's cells along axis %d must be a nonempty range within the grid's %d, got %d..%d", a,
+                           n, d->lo[a], d->hi[a]);
+        const int low = s->b.low[a], high = s->b.high[a];
+        if (!s->b.periodic[a] && (low > 0 || high > 0) && (d->lo[a] < low || d->hi[a] + high >= n))
+            return px_fail(PXFDTD_ERR_INVALID,
+                           "a design's cells along axis %d, %d..%d, and the values of E they reach must be clear of the "
+                           "CPMLs (%d and %d of %d cells)", a, d->lo[a], d->hi[a], low, high, n);
+    }
+    return PXFDTD_OK;
+}
+
+// Design::check_density (adjoint.rs:142-146)
+int design_check_density(const pxfdtd_design &d, const double *density, int64_t n) {
+    if (n != design_len(d) || (n > 0 && !density))
+        return px_fail(PXFDTD_ERR_INVALID, "a design of %lld cells needs as many densities, got %lld", design_len(d),
+                       (long long)n);
+    for (int64_t v = 0; v < n; v++)
+        if (!(std::isfinite(density[v]) && design_eps(d, density[v]) > 0.0))
+            return px_fail(PXFDTD_ERR_INVALID, "a density must be finite and give a positive permittivity, got %g",
+                           density[v]);
+    return PXFDTD_OK;
+}
+
+// Design::values (adjoint.rs:196-255): every value of E whose cell reaches the design's cells,
+// each design cell's share of it (the 8 x 8 samples across the component; along it the cell lies
+// in one design cell), component by component, then k, j, i
+std::vector<Touched> design_values(const pxfdtd_simulation *s, const pxfdtd_design &d) {
+    const int samples = 8;
+    const double n = samples;
+    auto offset = [&](int q) { return (q + 0.5) / n - 0.5; };
+    const double h[3] = {s->g.dx, s->g.dy, s->g.dz};
+    std::vector<Touched> out;
+    for (int c = 0; c < 3; c++) {
+        std::vector<int> cand[3];
+        for (int a = 0; a < 3; a++) {
+            std::vector<int> set;
+            const int len = s->n[a];
+            const int last = a == c ? d.hi[a] - 1 : d.hi[a];
+            for (int m = d.lo[a]; m <= last; m++) {
+                if (m < len)
+                    set.push_back(m);
+                else if (s->b.periodic[a])
+                    set.push_back(m % len);
+            }
+            std::sort(set.begin(), set.end());
+            set.erase(std::unique(set.begin(), set.end()), set.end());
+            cand[a] = set;
+        }
+        const int b = (c + 1) % 3, cc = (c + 2) % 3;
+        for (int k : cand[2])
+            for (int j : cand[1])
+                for (int i : cand[0]) {
+                    const long long r = ((long long)k * s->n[1] + j) * s->n[0] + i;
+                    double centre[3];
+                    s->e_position(c, r, centre);
+                    std::map<long long, double> shares;
+                    double p[3] = {centre[0], centre[1], centre[2]};
+                    for (int sb = 0; sb < samples; sb++) {
+                        p[b] = centre[b] + offset(sb) * h[b];
+                        for (int sc = 0; sc < samples; sc++) {
+                            p[cc] = centre[cc] + offset(sc) * h[cc];
+                            p[c] = centre[c];
+                            const long long v = design_cell_at(s, d, p);
+                            if (v >= 0) shares[v] += 1.0 / (n * n);
+                        }
+                    }
+                    if (!shares.empty()) {
+                        Touched t;
+                        t.comp = c;
+                        t.at[0] = i;
+                        t.at[1] = j;
+                        t.at[2] = k;
+                        t.r = r;
+                        t.shares.assign(shares.begin(), shares.end());
+                        out.push_back(std::move(t));
+                    }
+                }
+    }
+    return out;
+}
+
+// every transform monitor's sums, concatenated in the order of s->dfts (the device's, which went
+// on from the host's, when the device holds them)
+int all_sums(pxfdtd_simulation *s, std::vector<double> &out) {
+    if (s->res && !s->taps_dirty) {
+        out.resize(pxfdtd_resident_sums_len(s->res));
+        return pxfdtd_resident_read_sums(s->res, out.data(), (int64_t)out.size());
+    }
+    out.clear();
+    for (const DftRec &d : s->dfts) out.insert(out.end(), d.acc.begin(), d.acc.end());
+    return PXFDTD_OK;
+}
+
+size_t dft_offset(const pxfdtd_simulation *s, int index) {
+    size_t at = 0;
+    for (int m = 0; m < index; m++) at += s->dfts[m].acc.size();
+    return at;
+}
+
+// DesignMonitor::value (adjoint.rs:285-291): the transform at frequency f of the value t reads,
+// from the sums `all`
+C64 design_value(const pxfdtd_simulation *s, const DesignRec &R, const std::vector<double> &all, const Touched &t,
+                 int f) {
+    const int index = R.dft[t.comp];
+    const DftRec &d = s->dfts[index];
+    const size_t len = d.acc.size() / 6;
+    const double *series = all.data() + dft_offset(s, index) + (size_t)(PXFDTD_E * 3 + t.comp) * len;
+    const long long ex = d.hi[0] - d.lo[0], ey = d.hi[1] - d.lo[1], ez = d.hi[2] - d.lo[2];
+    const long long q = (((long long)f * ez + (t.at[2] - d.lo[2])) * ey + (t.at[1] - d.lo[1])) * ex + (t.at[0] - d.lo[0]);
+    return {series[2 * q], series[2 * q + 1]};
+}
+
+// num-complex's quotient of a real by a complex
+C64 rcdiv(double x, C64 z) {
+    const double n = z.re * z.re + z.im * z.im;
+    return {x * z.re / n, (0.0 - x * z.im) / n};
+}
+
+// inverse_stretch (adjoint.rs:682-702): 1/s~ of the CPML along `axis` at index m, on a node
+// (E's convolutions) or halfway after it (H~'s), 1/kappa + a/(1 - b e^(i w dt)); 1 outside
+C64 inverse_stretch(const pxfdtd_simulation *s, int axis, int m, bool half, double omega) {
+    C64 v = {half ? s->kh[axis][m] : s->kn[axis][m], 0.0};
+    const int field = half ? PXFDTD_H : PXFDTD_E;
+    for (const Slab &sl : s->slabs)
+        if (sl.field == field && sl.axis == axis && m >= sl.from && m < sl.from + sl.count) {
+            const int q = m - sl.from;
+            double sn, cs;
+            pxfdtd::psincos(omega * s->dt, sn, cs);
+            const C64 bz = {sl.b[q] * cs, sl.b[q] * sn};
+            const C64 t = rcdiv(sl.a[q], {1.0 - bz.re, 0.0 - bz.im});
+            v = {v.re + t.re, v.im + t.im};
+            break;
+        }
+    return v;
+}
+
+// symmetrizer (adjoint.rs:704-718): D at field's component at `at`, the product of the CPMLs'
+// stretches along the three axes there; 1 outside the CPMLs
+C64 symmetrizer(const pxfdtd_simulation *s, int field, int comp, const int at[3], double omega) {
+    C64 d = {1.0, 0.0};
+    for (int axis = 0; axis < 3; axis++) {
+        const bool half = field == PXFDTD_E ? axis == comp : axis != comp;
+        d = cdiv(d, inverse_stretch(s, axis, at[axis], half, omega));
+    }
+    return d;
+}
+
+// Waveform::spectrum (src/fdtd.rs:353-396): sum of w(t_n) e^(i w t_n) dt over the first `steps`
+// steps, at the times a current on `field` takes its values (E: (n + 1/2) dt, H~: n dt)
+C64 waveform_spectrum(const pxfdtd_waveform &w, int field, double dt, long long steps, double frequency) {
+    const double omega = TAU * frequency;
+    const double offset = field == PXFDTD_E ? 0.5 * dt : 0.0;
+    C64 sum = {0.0, 0.0};
+    for (long long n = 0; n < steps; n++) {
+        const double t = (double)n * dt + offset;
+        const double v = wave_at(w, t);
+        const C64 e = pxfdtd::pcexp({0.0, omega * t});
+        const C64 p = {v * e.re - 0.0 * e.im, v * e.im + 0.0 * e.re};
+        sum = {sum.re + p.re * dt, sum.im + p.im * dt};
+    }
+    return sum;
+}
+
+// basis (adjoint.rs:1011-1031): per frequency, Waveform::pulse on it and the same a quarter
+// period later
+int adjoint_basis(const std::vector<double> &freqs, double bandwidth, std::vector<pxfdtd_waveform> &out) {
+    if (!(std::isfinite(bandwidth) && bandwidth > 0.0))
+        return px_fail(PXFDTD_ERR_INVALID, "a pulse's bandwidth must be positive and finite, got %g", bandwidth);
+    out.clear();
+    for (double f : freqs) {
+        const double width = std::sqrt(2.0 * 0.6931471805599453) / (3.141592653589793 * bandwidth);
+        for (double shift : {0.0, 0.25 / f}) {
+            pxfdtd_waveform w{};
+            w.kind = 0;
+            w.frequency = f;
+            w.width = width;
+            w.delay = 6.0 * width + shift;
+            w.amp_re = 1.0;
+            out.push_back(w);
+        }
+    }
+    return PXFDTD_OK;
+}
+
+// Simulation::amplitudes (adjoint.rs:571-621): the real amplitudes b_m of each value's current
+// sum b_m w_m(t) whose transform at each frequency is its target there, the real system
+// sum b_m W_m(w_j) = target_j (real and imaginary rows) solved by LU with partial pivoting
+int adjoint_amplitudes(const pxfdtd_simulation *s, int field, const std::vector<pxfdtd_waveform> &waves,
+                       const std::vector<double> &freqs, const std::vector<const std::vector<C64> *> &targets,
+                       std::vector<std::vector<double>> &out) {
+    const int m = (int)waves.size();
+    long long steps = 0;
+    for (const pxfdtd_waveform &w : waves)
+        steps = std::max(steps, (long long)std::ceil((w.delay + 8.0 * w.width) / s->dt) + 1);
+    std::vector<double> a((size_t)m * m), rhs((size_t)m * targets.size());
+    for (int q = 0; q < m; q++)
+        for (size_t j = 0; j < freqs.size(); j++) {
+            const C64 t = waveform_spectrum(waves[q], field, s->dt, steps, freqs[j]);
+            a[(2 * j) * m + q] = t.re;
+            a[(2 * j + 1) * m + q] = t.im;
+        }
+    const size_t nt = targets.size();
+    for (size_t col = 0; col < nt; col++)
+        for (size_t j = 0; j < freqs.size(); j++) {
+            rhs[(2 * j) * nt + col] = (*targets[col])[j].re;
+            rhs[(2 * j + 1) * nt + col] = (*targets[col])[j].im;
+        }
+    // Gaussian elimination with partial pivoting on [a | rhs]
+    for (int k = 0; k < m; k++) {
+        int p = k;
+        for (int i = k + 1; i < m; i++)
+            if (std::fabs(a[(size_t)i * m + k]) > std::fabs(a[(size_t)p * m + k])) p = i;
+        if (p != k) {
+            for (int c = 0; c < m; c++) std::swap(a[(size_t)k * m + c], a[(size_t)p * m + c]);
+            for (size_t c = 0; c < nt; c++) std::swap(rhs[k * nt + c], rhs[p * nt + c]);
+        }
+        const double piv = a[(size_t)k * m + k];
+        for (int i = k + 1; i < m; i++) {
+            const double l = a[(size_t)i * m + k] / piv;
+            if (l == 0.0) continue;
+            for (int c = k; c < m; c++) a[(size_t)i * m + c] -= l * a[(size_t)k * m + c];
+            for (size_t c = 0; c < nt; c++) rhs[i * nt + c] -= l * rhs[k * nt + c];
+        }
+    }
+    out.assign(nt, std::vector<double>(m));
+    for (size_t col = 0; col < nt; col++)
+        for (int k = m - 1; k >= 0; k--) {
+            double v = rhs[k * nt + col];
+            for (int c = k + 1; c < m; c++) v -= a[(size_t)k * m + c] * out[col][c];
+            out[col][k] = v / a[(size_t)k * m + k];
+            if (!std::isfinite(out[col][k]))
+                return px_fail(PXFDTD_ERR_INVALID,
+                               "the adjoint's pulses can't give its sources' transforms: frequencies too close for the "
+                               "bandwidth");
+        }
+    return PXFDTD_OK;
+}
+
+struct Derivative {
+    int field, comp;
+    long long r;
+    C64 g;
+};
+
+// flux_derivatives (adjoint.rs:1033-1082): a flux face's derivative with respect to each
+// transform its sum pairs, times `scale`: the flux is sum kappa Re(E conj(H~)), so d/dE = kappa
+// conj(H~) / 2 and d/dH~ = kappa conj(E) / 2
+void flux_derivatives(const pxfdtd_simulation *s, const FluxFace &F, const double *series0, size_t len, int f,
+                      double scale, std::vector<Derivative> &out) {
+    const DftRec &d = s->dfts[F.dft];
+    const int a = F.axis, b = (a + 1) % 3, c = (a + 2) % 3;
+    const long long ex = d.hi[0] - d.lo[0], ey = d.hi[1] - d.lo[1], ez = d.hi[2] - d.lo[2];
+    auto value = [&](int field, int comp, const int at[3]) {
+        const double *v = series0 + (size_t)(field * 3 + comp) * len;
+        const long long q = (((long long)f * ez + (at[2] - d.lo[2])) * ey + (at[1] - d.lo[1])) * ex + (at[0] - d.lo[0]);
+        return C64{v[2 * q], v[2 * q + 1]};
+    };
+    auto index = [&](const int at[3]) { return ((long long)at[2] * s->n[1] + at[1]) * s->n[0] + at[0]; };
+    const double steps[3] = {s->g.dx, s->g.dy, s->g.dz};
+    const double area = steps[b] * steps[c];
+    int bfrom, bto, cfrom, cto;
+    flux_range(s, F, 0, bfrom, bto);
+    flux_range(s, F, 1, cfrom, cto);
+    const int comps[2] = {b, c}, hs[2] = {c, b};
+    const double signs[2] = {0.5, -0.5};
+    for (int q = 0; q < 2; q++) {
+        const int comp = comps[q], h = hs[q];
+        for (int v = cfrom; v < cto; v++)
+            for (int u = bfrom; u < bto; u++) {
+                const double weight = flux_weight(F, 0, u, comp == b) * flux_weight(F, 1, v, comp == c);
+                if (weight == 0.0) continue;
+                const double kappa = scale * weight * signs[q] * 0.5 * area;
+                int at[3];
+                at[a] = F.plane;
+                at[b] = u;
+                at[c] = v;
+                const C64 hv = value(PXFDTD_H, h, at), low = value(PXFDTD_E, comp, at);
+                const long long h_at = index(at);
+                at[a] = F.plane + 1;
+                const C64 high = value(PXFDTD_E, comp, at);
+                const double k2 = 0.5 * kappa;
+                out.push_back({PXFDTD_E, comp, h_at, {k2 * hv.re, k2 * -hv.im}});
+                out.push_back({PXFDTD_E, comp, index(at), {k2 * hv.re, k2 * -hv.im}});
+                const C64 sum = {low.re + high.re, low.im + high.im};
+                out.push_back({PXFDTD_H, h, h_at, {k2 * sum.re, k2 * -sum.im}});
+            }
+    }
+}
+
+}  // namespace
+
+// Simulation::term_derivatives (adjoint.rs:623-680): a term's frequency and the objective's
+// derivative with respect to each transform it reads
+static int term_derivatives(pxfdtd_simulation *s, const pxfdtd_term &t, const std::vector<double> &all, double &freq,
+                            std::vector<Derivative> &out) {
+    out.clear();
+    if (t.kind == 0) {
+        if (t.monitor < 0 || t.monitor >= (int32_t)s->modemons.size() || t.mode < 0 ||
+            t.mode >= (int32_t)s->modemons[t.monitor].modes.size() || t.direction < 0 || t.direction > 1)
+            return px_fail(PXFDTD_ERR_INVALID, "the objective's mode term reads no mode monitor here");
+        const ModeEntry &e = s->modemons[t.monitor].modes[t.mode];
+        const pxfdtd_port_mode mode = e.view();
+        int bl[3] = {0, 0, 0}, bn[3] = {s->n[0], s->n[1], s->n[2]};
+        bl[e.axis] = e.plane;
+        bn[e.axis] = 2;
+        const long long vol = (long long)bn[0] * bn[1] * bn[2];
+        std::vector<double> w((size_t)(6 * vol));
+        TRY(pxfdtd_simulation_mode_weights(s, &mode, t.direction, w.data()));
+        const C64 g = {t.derivative_re, t.derivative_im};
+        for (int c = 0; c < 3; c++)
+            for (long long v = 0; v < vol; v++) {
+                const C64 wv = {w[2 * (c * vol + v)], w[2 * (c * vol + v) + 1]};
+                if (wv.re == 0.0 && wv.im == 0.0) continue;
+                const long long i = v % bn[0] + bl[0], j = (v / bn[0]) % bn[1] + bl[1], k = v / ((long long)bn[0] * bn[1]) + bl[2];
+                out.push_back({PXFDTD_E, c, (k * s->n[1] + j) * s->n[0] + i, cmul(g, wv)});
+            }
+        freq = e.frequency;
+        return PXFDTD_OK;
+    }
+    if (t.kind == 1) {
+        if (t.monitor < 0 || t.monitor >= (int32_t)s->fluxes.size())
+            return px_fail(PXFDTD_ERR_INVALID, "the objective's flux term reads no flux monitor here");
+        const FluxRec &R = s->fluxes[t.monitor];
+        const DftRec &d0 = s->dfts[R.faces[0].dft];
+        if (t.frequency < 0 || t.frequency >= (int32_t)d0.freqs.size())
+            return px_fail(PXFDTD_ERR_INVALID, "the objective's flux term names no frequency of its monitor");
+        freq = d0.freqs[t.frequency];
+        for (const FluxFace &F : R.faces) {
+            const double *series0 = all.data() + dft_offset(s, F.dft);
+            flux_derivatives(s, F, series0, s->dfts[F.dft].acc.size() / 6, t.frequency, F.sign * t.derivative_re, out);
+        }
+        return PXFDTD_OK;
+    }
+    if (t.kind == 2) {
+        if (t.monitor < 0 || t.monitor >= (int32_t)s->shown.size() || t.field < 0 || t.field > 1 || t.component < 0 ||
+            t.component > 2)
+            return px_fail(PXFDTD_ERR_INVALID, "the objective's transform term reads no transform monitor here");
+        const DftRec &d = s->dfts[s->shown[t.monitor]];
+        if (t.frequency < 0 || t.frequency >= (int32_t)d.freqs.size())
+            return px_fail(PXFDTD_ERR_INVALID, "the objective's transform term names no frequency of its monitor");
+        for (int a = 0; a < 3; a++)
+            if (t.at[a] < d.lo[a] || t.at[a] >= d.hi[a])
+                return px_fail(PXFDTD_ERR_INVALID, "the objective's transform term reads no value of its monitor");
+        freq = d.freqs[t.frequency];
+        out.push_back({t.field, t.component, ((long long)t.at[2] * s->n[1] + t.at[1]) * s->n[0] + t.at[0],
+                       {t.derivative_re, t.derivative_im}});
+        return PXFDTD_OK;
+    }
+    return px_fail(PXFDTD_ERR_INVALID, "no term kind %d", t.kind);
+}
+
+// the transform monitors of design `R`'s values on `t` (its at-rest copy's among them)
+static int add_design_transforms(pxfdtd_simulation *t, DesignRec &R) {
+    for (int c = 0; c < 3; c++) {
+        R.dft[c] = -1;
+        int32_t lo[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, hi[3] = {-1, -1, -1};
+        for (const Touched &v : R.touched)
+            if (v.comp == c)
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = std::min(lo[a], v.at[a]);
+                    hi[a] = std::max(hi[a], v.at[a]);
+                }
+        if (hi[0] < 0) continue;
+        TRY(add_transform(t, lo, hi, (int32_t)R.freqs.size(), R.freqs.data(), &R.dft[c]));
+    }
+    return PXFDTD_OK;
+}
+
+// Simulation::at_rest (adjoint.rs:720-785): the same problem at rest from the start, the grid,
+// media and boundaries, no sources, probes or monitors but design monitor `keep`, at zero
+static int sim_at_rest(const pxfdtd_simulation *s, int keep, pxfdtd_simulation **out) {
+    std::unique_ptr<pxfdtd_simulation> t(new pxfdtd_simulation());
+    t->g = s->g;
+    t->b = s->b;
+    for (int a = 0; a < 3; a++) t->n[a] = s->n[a];
+    t->N = s->N;
+    t->dt = s->dt;
+    t->steps = 0;
+    t->e.assign(3 * s->N, 0.0);
+    t->h.assign(3 * s->N, 0.0);
+    t->ca = s->ca;
+    t->cb = s->cb;
+    t->eps.reset(new double[(size_t)(3 * s->N)]);
+    std::memcpy(t->eps.get(), s->eps.get(), (size_t)(3 * s->N) * sizeof(double));
+    t->sigma = s->sigma;
+    for (int a = 0; a < 3; a++) {
+        t->kn[a] = s->kn[a];
+        t->kh[a] = s->kh[a];
+        t->fe[a] = s->fe[a];
+        t->fh[a] = s->fh[a];
+        for (int side = 0; side < 2; side++)
+            for (int f = 0; f < 2; f++) {
+                t->cpml_b[a][side][f] = s->cpml_b[a][side][f];
+                t->cpml_a[a][side][f] = s->cpml_a[a][side][f];
+            }
+    }
+    t->slabs = s->slabs;
+    for (Slab &sl : t->slabs) std::fill(sl.psi.begin(), sl.psi.end(), 0.0);
+    t->groups = s->groups;
+    t->member = s->member;
+    t->dev = s->dev;
+    t->precision = s->precision;
+    for (size_t m = 0; m < s->designs.size(); m++) {
+        DesignRec R;
+        R.d = s->designs[m].d;
+        R.freqs = s->designs[m].freqs;
+        R.dft[0] = R.dft[1] = R.dft[2] = -1;
+        if ((int)m == keep) {
+            R.touched = s->designs[m].touched;
+            TRY(add_design_transforms(t.get(), R));
+        }
+        t->designs.push_back(std::move(R));
+    }
+    *out = t.release();
+    return PXFDTD_OK;
+}
+
+// Simulation::with_design (adjoint.rs:366-423); every value is computed and checked before any
+// changes
+extern "C" int pxfdtd_simulation_with_design(pxfdtd_simulation *s, const pxfdtd_design *d, const double *density,
+                                             int64_t n, const pxfdtd_scene *background) {
+    TRY(design_check(s, d));
+    TRY(design_check_density(*d, density, n));
+    if (!background) return px_fail(PXFDTD_ERR_INVALID, "a design needs the background's scene");
+    if (s->steps > 0) return px_fail(PXFDTD_ERR_INVALID, "a design is set before the run starts");
+    const long long N = s->N;
+    const double h[3] = {s->g.dx, s->g.dy, s->g.dz};
+    const std::vector<Touched> touched = design_values(s, *d);
+    std::vector<double> values(touched.size(), 0.0);
+    for (size_t q = 0; q < touched.size(); q++) {
+        const Touched &t = touched[q];
+        const long long cr = t.comp * N + t.r;
+        if (s->cb[cr] == 0.0) continue;  // held at zero: a wall or a conductor
+        if (s->ca[cr] != 1.0 || (!s->member.empty() && s->member[cr] >= 0))
+            return px_fail(PXFDTD_ERR_INVALID, "a design's values must be free of conductivity and dispersive media");
+        double centre[3];
+        s->e_position(t.comp, t.r, centre);
+        const int comp = t.comp;
+        const double value = averaged_with(
+            [&](double x, double y, double z) {
+                const double p[3] = {x, y, z};
+                const long long v = design_cell_at(s, *d, p);
+                return v >= 0 ? design_eps(*d, density[v]) : scene_value(background, comp, x, y, z);
+            },
+            comp, centre, h);
+        if (!(std::isfinite(value) && value > 0.0))
+            return px_fail(PXFDTD_ERR_INVALID, "the permittivity must be finite and positive, got %g", value);
+        values[q] = value;
+    }
+    for (size_t q = 0; q < touched.size(); q++) {
+        const long long cr = touched[q].comp * N + touched[q].r;
+        if (s->cb[cr] == 0.0) continue;
+        s->cb[cr] = s->dt / values[q];
+        s->eps[cr] = values[q];
+    }
+    if (s->res) drop_resident(s);  // the device's coefficients are the old ones
+    return PXFDTD_OK;
+}
+
+// Simulation::add_design_monitor (adjoint.rs:435-470)
+extern "C" int pxfdtd_simulation_add_design_monitor(pxfdtd_simulation *s, const pxfdtd_design *d, int32_t nf,
+                                                    const double *freqs, int32_t *number) {
+    TRY(check_frequencies(s, nf, freqs));
+    TRY(design_check(s, d));
+    DesignRec R;
+    R.d = *d;
+    R.touched = design_values(s, *d);
+    R.freqs.assign(freqs, freqs + nf);
+    TRY(add_design_transforms(s, R));
+    s->designs.push_back(std::move(R));
+    *number = (int32_t)s->designs.size() - 1;
+    return PXFDTD_OK;
+}
+
+extern "C" int64_t pxfdtd_simulation_design_values(const pxfdtd_simulation *s, int32_t n, int32_t *comp_ijk) {
+    if (n < 0 || n >= (int32_t)s->designs.size()) {
+        px_fail(PXFDTD_ERR_INVALID, "no design monitor %d", n);
+        return -1;
+    }
+    const std::vector<Touched> &T = s->designs[n].touched;
+    if (comp_ijk)
+        for (size_t q = 0; q < T.size(); q++) {
+            comp_ijk[4 * q] = T[q].comp;
+            for (int a = 0; a < 3; a++) comp_ijk[4 * q + 1 + a] = T[q].at[a];
+        }
+    return (int64_t)T.size();
+}
+
+extern "C" int pxfdtd_simulation_design_transform(pxfdtd_simulation *s, int32_t n, int32_t f, double *out) {
+    if (n < 0 || n >= (int32_t)s->designs.size()) return px_fail(PXFDTD_ERR_INVALID, "no design monitor %d", n);
+    const DesignRec &R = s->designs[n];
+    if (f < 0 || f >= (int32_t)R.freqs.size()) return px_fail(PXFDTD_ERR_INVALID, "no frequency %d", f);
+    std::vector<double> all;
+    TRY(all_sums(s, all));
+    for (size_t q = 0; q < R.touched.size(); q++) {
+        const C64 v = design_value(s, R, all, R.touched[q], f);
+        out[2 * q] = v.re;
+        out[2 * q + 1] = v.im;
+    }
+    return PXFDTD_OK;
+}
+
+// Simulation::adjoint (adjoint.rs:496-569)
+extern "C" int pxfdtd_simulation_adjoint(pxfdtd_simulation *s, const pxfdtd_term *terms, int32_t n, int32_t monitor,
+                                         double bandwidth, pxfdtd_simulation **out) {
+    if (monitor < 0 || monitor >= (int32_t)s->designs.size())
+        return px_fail(PXFDTD_ERR_INVALID, "there is no design monitor %d", monitor);
+    if (n < 0 || (n > 0 && !terms)) return px_fail(PXFDTD_ERR_INVALID, "the adjoint needs its terms");
+    const std::vector<double> freqs = s->designs[monitor].freqs;
+    const size_t count = freqs.size();
+    std::vector<double> all;
+    TRY(all_sums(s, all));
+    // each value's transform targets, frequency by frequency, keyed (field, component, index)
+    std::map<std::tuple<int, int, long long>, std::vector<C64>> targets;
+    std::vector<Derivative> ders;
+    for (int32_t q = 0; q < n; q++) {
+        double frequency;
+        TRY(term_derivatives(s, terms[q], all, frequency, ders));
+        int f = -1;
+        for (size_t m = 0; m < count; m++)
+            if (freqs[m] == frequency) {
+                f = (int)m;
+                break;
+            }
+        if (f < 0)
+            return px_fail(PXFDTD_ERR_INVALID, "the objective's frequency %.17g isn't among design monitor %d's",
+                           frequency, monitor);
+        const double omega = TAU * frequency;
+        for (const Derivative &dv : ders) {
+            const int at[3] = {(int)(dv.r % s->n[0]), (int)((dv.r / s->n[0]) % s->n[1]),
+                               (int)(dv.r / ((long long)s->n[0] * s->n[1]))};
+            const C64 d = symmetrizer(s, dv.field, dv.comp, at, omega);
+            // the adjoint's sources: J = -g_E, M = g_H, over D
+            const C64 value = dv.field == PXFDTD_E ? cdiv({-dv.g.re, -dv.g.im}, d) : cdiv(dv.g, d);
+            auto it = targets.find({dv.field, dv.comp, dv.r});
+            if (it == targets.end())
+                it = targets.emplace(std::make_tuple(dv.field, dv.comp, dv.r), std::vector<C64>(count, C64{0.0, 0.0}))
+                         .first;
+            it->second[f] = {it->second[f].re + value.re, it->second[f].im + value.im};
+        }
+    }
+    std::vector<pxfdtd_waveform> waves;
+    TRY(adjoint_basis(freqs, bandwidth, waves));
+    pxfdtd_simulation *adj = nullptr;
+    TRY(sim_at_rest(s, monitor, &adj));
+    std::unique_ptr<pxfdtd_simulation, void (*)(pxfdtd_simulation *)> guard(adj, pxfdtd_simulation_destroy);
+    for (int field : {PXFDTD_E, PXFDTD_H}) {
+        std::vector<std::tuple<int, long long>> keys;
+        std::vector<const std::vector<C64> *> mine;
+        for (const auto &kv : targets)
+            if (std::get<0>(kv.first) == field) {
+                keys.push_back({std::get<1>(kv.first), std::get<2>(kv.first)});
+                mine.push_back(&kv.second);
+            }
+        if (mine.empty()) continue;
+        std::vector<std::vector<double>> amps;
+        TRY(adjoint_amplitudes(s, field, waves, freqs, mine, amps));
+        for (size_t k = 0; k < waves.size(); k++) {
+            CurrentRec c;
+            c.field = field;
+            c.w = waves[k];
+            for (size_t v = 0; v < keys.size(); v++) {
+                if (amps[v][k] == 0.0) continue;
+                c.comps.push_back(std::get<0>(keys[v]));
+                c.r.push_back(std::get<1>(keys[v]));
+                c.amps.push_back({amps[v][k], 0.0});
+            }
+            if (!c.r.empty()) adj->currents.push_back(std::move(c));
+        }
+    }
+    adj->inj_dirty = true;
+    *out = guard.release();
+    return PXFDTD_OK;
+}
+
+// design_pair (adjoint.rs:881-902): design monitor `monitor` of both runs, the same in both
+static int design_pair(const pxfdtd_simulation *s, const pxfdtd_simulation *adj, int32_t monitor) {
+    bool same = monitor >= 0 && monitor < (int32_t)s->designs.size() && monitor < (int32_t)adj->designs.size();
+    if (same) {
+        const DesignRec &F = s->designs[monitor], &B = adj->designs[monitor];
+        same = std::memcmp(F.d.lo, B.d.lo, sizeof F.d.lo) == 0 && std::memcmp(F.d.hi, B.d.hi, sizeof F.d.hi) == 0 &&
+               F.d.eps[0] == B.d.eps[0] && F.d.eps[1] == B.d.eps[1] && F.freqs == B.freqs &&
+               F.touched.size() == B.touched.size();
+        for (size_t q = 0; same && q < F.touched.size(); q++) {
+            const Touched &a = F.touched[q], &b = B.touched[q];
+            same = a.comp == b.comp && a.r == b.r && a.shares == b.shares;
+        }
+    }
+    if (!same)
+        return px_fail(PXFDTD_ERR_INVALID, "design monitor %d must be the same in the forward and the adjoint runs",
+                       monitor);
+    return PXFDTD_OK;
+}
+
+// Simulation::permittivity_gradient (adjoint.rs:831-855)
+extern "C" int pxfdtd_simulation_permittivity_gradient(pxfdtd_simulation *s, pxfdtd_simulation *adj, int32_t monitor,
+                                                       double *out) {
+    TRY(design_pair(s, adj, monitor));
+    const DesignRec &F = s->designs[monitor], &B = adj->designs[monitor];
+    std::vector<double> fa, ba;
+    TRY(all_sums(s, fa));
+    TRY(all_sums(adj, ba));
+    std::vector<double> tilde;
+    for (double f : F.freqs) {  // Simulation::leapfrog_wavenumber (src/fdtd/sources.rs:171-173)
+        double sn, cs;
+        pxfdtd::psincos(TAU * f * s->dt / 2.0, sn, cs);
+        tilde.push_back(2.0 / s->dt * sn);
+    }
+    for (size_t q = 0; q < F.touched.size(); q++) {
+        double sum = -0.0;  // a sum of f64s, as Rust's Sum folds them
+        for (size_t f = 0; f < tilde.size(); f++) {
+            const C64 b = design_value(adj, B, ba, B.touched[q], (int)f);
+            const C64 e = design_value(s, F, fa, F.touched[q], (int)f);
+            sum += 2.0 * cmul(cmul({0.0, tilde[f]}, b), e).re;
+        }
+        out[q] = sum;
+    }
+    return PXFDTD_OK;
+}
+
+// Simulation::gradient (adjoint.rs:857-879)
+extern "C" int pxfdtd_simulation_design_gradient(pxfdtd_simulation *s, pxfdtd_simulation *adj, int32_t monitor,
+                                                 double *out) {
+    TRY(design_pair(s, adj, monitor));
+    const DesignRec &R = s->designs[monitor];
+    std::vector<double> values(R.touched.size());
+    TRY(pxfdtd_simulation_permittivity_gradient(s, adj, monitor, values.data()));
+    const double scale = R.d.eps[1] - R.d.eps[0];
+    std::fill(out, out + design_len(R.d), 0.0);
+    for (size_t q = 0; q < R.touched.size(); q++) {
+        const double g = values[q];
+        for (const auto &vs : R.touched[q].shares) out[vs.first] += g * vs.second * scale;
+    }
+    return PXFDTD_OK;
+}
+
+// Simulation::run_until_converged (adjoint.rs:904-964), over every series of every transform
+// monitor the simulation holds
+extern "C" int pxfdtd_simulation_run_until_converged(pxfdtd_simulation *s, double tolerance, double interval,
+                                                     double limit, int32_t *settled) {
+    if (!(std::isfinite(tolerance) && tolerance > 0.0) || !(std::isfinite(interval) && interval > 0.0) ||
+        !std::isfinite(limit))
+        return px_fail(PXFDTD_ERR_INVALID,
+                       "running until the transforms settle needs a positive tolerance and interval and a finite "
+                       "limit, got %g, %g and %g", tolerance, interval, limit);
+    if (s->dfts.empty()) return px_fail(PXFDTD_ERR_INVALID, "running until the transforms settle needs a monitor");
+    *settled = 0;
+    double finished;
+    if (!sources_end(s, finished)) finished = HUGE_VAL;
+    std::vector<double> before, now;
+    TRY(all_sums(s, before));
+    while ((double)s->steps * s->dt < limit) {
+        TRY(pxfdtd_simulation_run_until(s, std::min((double)s->steps * s->dt + interval, limit)));
+        TRY(all_sums(s, now));
+        bool all = true;
+        size_t at = 0;
+        for (const DftRec &d : s->dfts) {
+            const size_t len = d.acc.size() / 6;
+            for (int t = 0; t < 6; t++, at += len) {
+                double largest = 0.0, change = 0.0;
+                for (size_t q = 0; q < len; q += 2) {
+                    largest = std::max(largest, std::hypot(now[at + q], now[at + q + 1]));
+                    change = std::max(change, std::hypot(now[at + q] - before[at + q], now[at + q + 1] - before[at + q + 1]));
+                }
+                all = all && change <= tolerance * largest;
+            }
+        }
+        if ((double)s->steps * s->dt >= finished && all) {
+            *settled = 1;
+            return PXFDTD_OK;
+        }
+        before.swap(now);
+    }
     return PXFDTD_OK;
 }

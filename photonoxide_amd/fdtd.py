@@ -374,6 +374,63 @@ class Gpu:
             self.handle = None
 
 
+@dataclass
+class Design:
+    """A design region (src/fdtd/adjoint.rs:63-75): the cells [lo, hi) along x, y and z, each with
+    a density rho and the permittivity eps[0] + rho (eps[1] - eps[0]); densities are numbered i
+    fastest, then j, then k."""
+    cells: tuple  # ((lo, hi), (lo, hi), (lo, hi))
+    eps: tuple    # (eps at density 0, eps at density 1)
+
+    def __len__(self):
+        return int(np.prod([hi - lo for lo, hi in self.cells]))
+
+    @property
+    def shape(self):
+        """The densities' array shape, (nz, ny, nx) of the region."""
+        return tuple(hi - lo for lo, hi in reversed(self.cells))
+
+    def native(self) -> N.Design:
+        return N.Design(N.i3(*[c[0] for c in self.cells]), N.i3(*[c[1] for c in self.cells]),
+                        (C.c_double * 2)(*self.eps))
+
+
+@dataclass
+class Term:
+    """What an objective's derivative dF/da is taken of (Term, src/fdtd/adjoint.rs:285-338), with
+    dF/da the Wirtinger derivative (conj(a) held; conj(a) for F = |a|^2). Build with
+    Term.mode, Term.flux or Term.transform."""
+    kind: int
+    monitor: int
+    derivative: complex
+    mode: int = 0
+    direction: int = 0
+    frequency: int = 0
+    field: int = 0
+    component: int = 0
+    at: tuple = (0, 0, 0)
+
+    @staticmethod
+    def mode_amplitude(monitor: int, mode: int, direction: int, derivative: complex) -> "Term":
+        """A mode monitor's amplitude of `mode` going `direction` (0 forward, 1 backward)."""
+        return Term(0, monitor, complex(derivative), mode=mode, direction=direction)
+
+    @staticmethod
+    def flux(monitor: int, frequency: int, derivative: float) -> "Term":
+        """A flux monitor's flux at its frequency number `frequency`; dF/dP real."""
+        return Term(1, monitor, complex(derivative), frequency=frequency)
+
+    @staticmethod
+    def transform(monitor: int, field, component, at, frequency: int, derivative: complex) -> "Term":
+        """A transform monitor's value of field's component at `at`, frequency number `frequency`."""
+        return Term(2, monitor, complex(derivative), frequency=frequency, field=int(field), component=int(component),
+                    at=tuple(at))
+
+    def native(self) -> N.Term:
+        return N.Term(self.kind, self.monitor, self.mode, self.direction, self.frequency, self.field, self.component,
+                      N.i3(*self.at), self.derivative.real, self.derivative.imag)
+
+
 class Simulation:
     """An FDTD problem (src/fdtd.rs:470-512) on the host, stepping on a device."""
 
@@ -694,6 +751,81 @@ class Simulation:
         N.check(N.lib.pxfdtd_simulation_dft(self._h, n, series, N.dptr(out)))
         nf, (lx, ly, lz) = self._dft_shapes[n]
         return (out[0::2] + 1j * out[1::2]).reshape(nf, lz, ly, lx)
+
+    # -- designs and the frequency-domain adjoint -------------------------------------------
+    def with_design(self, design: Design, density, background: Scene) -> "Simulation":
+        """Simulation::with_design (src/fdtd/adjoint.rs:366-423): every value of E whose cell
+        reaches the design's cells takes the average of eps(rho) there and `background`
+        elsewhere. `density` is one value per cell, [nz][ny][nx] of the region or flat."""
+        d = np.ascontiguousarray(np.asarray(density, dtype=np.float64).ravel())
+        sc, keep = background.native()
+        N.check(N.lib.pxfdtd_simulation_with_design(self._h, C.byref(design.native()), N.dptr(d), len(d),
+                                                    C.byref(sc)))
+        return self
+
+    def add_design_monitor(self, design: Design, frequencies) -> int:
+        """Simulation::add_design_monitor (adjoint.rs:435-470): E's transforms over the values the
+        design's cells reach, at the objective's frequencies."""
+        f = np.ascontiguousarray(np.atleast_1d(np.asarray(frequencies, dtype=np.float64)))
+        n = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_add_design_monitor(self._h, C.byref(design.native()), len(f), N.dptr(f),
+                                                           C.byref(n)))
+        return n.value
+
+    def design_values(self, n: int) -> np.ndarray:
+        """The values design monitor n reads, [(component, i, j, k)], in the gradients' order."""
+        m = N.lib.pxfdtd_simulation_design_values(self._h, n, None)
+        if m < 0:
+            raise IndexError(f"no design monitor {n}")
+        out = np.zeros((m, 4), dtype=np.int32)
+        N.lib.pxfdtd_simulation_design_values(self._h, n, out.ctypes.data_as(N.pi32))
+        return out
+
+    def design_transform(self, n: int, frequency: int = 0) -> np.ndarray:
+        """Design monitor n's transform of each value it reads at its frequency number `frequency`."""
+        m = N.lib.pxfdtd_simulation_design_values(self._h, n, None)
+        if m < 0:
+            raise IndexError(f"no design monitor {n}")
+        out = np.zeros(2 * m)
+        N.check(N.lib.pxfdtd_simulation_design_transform(self._h, n, frequency, N.dptr(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def adjoint(self, terms: Sequence[Term], monitor: int, bandwidth: float) -> "Simulation":
+        """Simulation::adjoint (adjoint.rs:496-569): the adjoint run, at rest, whose currents'
+        transforms at design monitor `monitor`'s frequencies are the objective's derivatives.
+        It steps on this simulation's device; run both until their fields die away
+        (run_until_converged), then gradient()."""
+        arr = (N.Term * max(1, len(terms)))(*[t.native() for t in terms])
+        h = C.c_void_p()
+        N.check(N.lib.pxfdtd_simulation_adjoint(self._h, arr, len(terms), monitor, float(bandwidth), C.byref(h)))
+        adj = Simulation(h, self.grid, self.boundaries)
+        adj._device = self._device
+        return adj
+
+    def permittivity_gradient(self, adjoint: "Simulation", monitor: int) -> np.ndarray:
+        """Simulation::permittivity_gradient (adjoint.rs:831-855): dF/deps per value of
+        design_values(monitor)."""
+        m = N.lib.pxfdtd_simulation_design_values(self._h, monitor, None)
+        if m < 0:
+            raise IndexError(f"no design monitor {monitor}")
+        out = np.zeros(m)
+        N.check(N.lib.pxfdtd_simulation_permittivity_gradient(self._h, adjoint._h, monitor, N.dptr(out)))
+        return out
+
+    def gradient(self, adjoint: "Simulation", monitor: int, design: Design) -> np.ndarray:
+        """Simulation::gradient (adjoint.rs:857-879): dF/drho per design cell, [nz][ny][nx] of
+        the region."""
+        out = np.zeros(len(design))
+        N.check(N.lib.pxfdtd_simulation_design_gradient(self._h, adjoint._h, monitor, N.dptr(out)))
+        return out.reshape(design.shape)
+
+    def run_until_converged(self, tolerance: float, interval: float, limit: float) -> bool:
+        """Simulation::run_until_converged (adjoint.rs:904-964): whether every transform had
+        settled to `tolerance` of its largest value over a block of `interval` before `limit`."""
+        s = C.c_int32()
+        N.check(N.lib.pxfdtd_simulation_run_until_converged(self._h, float(tolerance), float(interval), float(limit),
+                                                            C.byref(s)))
+        return bool(s.value)
 
     # -- stepping -------------------------------------------------------------------------
     def set_device(self, device: Optional[Gpu], precision: Precision = Precision.F32):
