@@ -55,7 +55,7 @@ def test_a_straight_guide_job_records_its_events_and_carries_its_power(gpu, tmp_
     print("transmission", t, "flux", through, "reflection", refl, flush=True)
     assert np.all(np.abs(t - 1.0) < 1e-5), t
     assert np.all(np.abs(through - t) < 2e-4), (through, t)
-    assert np.all(back < 1e-8) and np.all(refl < 1e-8)
+    assert np.all(back < 1e-5) and np.all(refl < 1e-8)
     assert np.allclose(out["wavelengths_um"], [1.5 + 0.1 * k / 5 for k in range(6)], rtol=1e-15, atol=0)
 
 
