@@ -167,13 +167,18 @@ def cpu_baseline(nx, ny, nz, pml):
     import oracle_lib as O
     threads = host_threads()
     steps = ctypes.c_long()
+    s, rows = ctypes.c_int(), ctypes.c_int()
+    O.lib.or_cpu_auto_blocking(nx, ny, nz, threads, ctypes.byref(s), ctypes.byref(rows))
     rate = O.lib.or_cpu_rate_f32(nx, ny, nz, pml, threads, ctypes.byref(steps)) / 1e9
     triad = O.lib.or_cpu_triad(1 << 26, 5, threads)
+    how = (f"Malas diamonds of {s.value} steps in tiles of {rows.value} rows (Blocking::auto, "
+           f"src/fdtd/kernel/blocked.rs)" if s.value else "the whole grid a step at a time")
     return {"value": round(rate, 4), "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
             "sample": f"the reference's CPU row kernel restated in C/OpenMP (oracle/cpu_baseline.c), f32, "
-                      f"{nx}x{ny}x{nz} cells with a {pml}-cell CPML, timed as src/fdtd/gpu/rates.rs times: "
-                      f"steps doubled from 4 until a run takes > 0.3 s, best of 3 runs of {steps.value} steps, "
-                      f"fields first touched by the threads that step them",
+                      f"{nx}x{ny}x{nz} cells with a {pml}-cell CPML, stepped by {how}, timed as "
+                      f"src/fdtd/gpu/rates.rs times: steps doubled from 4 until a run takes > 0.3 s, best of 3 "
+                      f"runs of {steps.value} steps, fields first touched by the threads that step them",
+            "blocking": {"steps": s.value, "rows": rows.value},
             "nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
             "cpu_model": cpu_model(), "triad_gb_per_s": round(triad, 1),
             "triad": "a = b + s c over 2^26 doubles, 24 B an element, best of 5 (src/traffic.rs:59-80)"}

@@ -159,6 +159,14 @@ double or_sim_electric_energy(const or_sim *s);
  * (steps doubled from 4 until a run takes > 0.3 s, best of 3); returns cell-updates per second
  * and the steps of a run in *steps_out. */
 double or_cpu_rate_f32(int nx, int ny, int nz, int pml, int threads, long *steps_out);
+/* the same with the blocking given: s steps at a time in tiles of about `rows` rows (s = 0: the
+ * whole grid each step; s < 0: Blocking::auto's choice) */
+double or_cpu_rate_blocked_f32(int nx, int ny, int nz, int pml, int threads, int s, int rows, long *steps_out);
+/* Blocking::auto's choice for f32 on `threads` threads (s = 0: the whole grid) */
+void or_cpu_auto_blocking(int nx, int ny, int nz, int threads, int *s, int *rows);
+/* the values of E, H and psi that differ after `steps` steps by diamonds of s steps and about
+ * `rows` rows against the whole grid a step at a time (-1 if the fields did not move) */
+long or_cpu_blocked_differs(int nx, int ny, int nz, int pml, int steps, int s, int rows, int threads);
 /* the STREAM triad a = b + s c over `len` doubles on `threads` threads (src/traffic.rs:59-80):
  * GB/s at 24 bytes an element, best of `repeats` */
 double or_cpu_triad(long len, int repeats, int threads);

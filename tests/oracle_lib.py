@@ -99,6 +99,9 @@ for name, res, args in [
     ("or_waveform_complex_at", C64, [C.POINTER(Wave), C.c_double]),
     ("or_cpu_rate_f32", C.c_double, [C.c_int] * 5 + [C.POINTER(C.c_long)]),
     ("or_cpu_triad", C.c_double, [C.c_long, C.c_int, C.c_int]),
+    ("or_cpu_rate_blocked_f32", C.c_double, [C.c_int] * 7 + [C.POINTER(C.c_long)]),
+    ("or_cpu_auto_blocking", None, [C.c_int] * 4 + [C.POINTER(C.c_int)] * 2),
+    ("or_cpu_blocked_differs", C.c_long, [C.c_int] * 8),
 ]:
     f = getattr(lib, name)
     f.restype = res
