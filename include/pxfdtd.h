@@ -438,6 +438,14 @@ int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, int32_t pla
  * Nyquist frequency, fewer than 8 samples or one not finite. */
 int pxfdtd_harmonic_inversion(const double *signal, int64_t n, double dt, double low, double high,
                               int32_t capacity, int32_t *found, double *out);
+/* pxfdtd_harmonic_inversion of a signal in device memory (`signal`: n complex FP64 samples on
+ * `dev`, read on `stream`, a hipStream_t or null), for post-processing on the device: the sums
+ * over the samples at each of the filter diagonalization's points run there and only they come
+ * back; the small eigenproblem runs on the host. The same resonances as the host's, bit for bit,
+ * and the same refusals. It allocates 8 B x 12 x points of device memory per call and waits for
+ * the stream. */
+int pxfdtd_harmonic_inversion_device(pxfdtd_device *dev, const double *signal, int64_t n, double dt, double low,
+                                     double high, int32_t capacity, int32_t *found, double *out, void *stream);
 /* Lattice::mode_amplitude_weights (src/fdfd/three/ports.rs:293-345): the mode monitor's
  * amplitude of `mode` going `direction` (0 forward, 1 backward) as weights on E's transform:
  * out [3][box] complex, the box the mode's plane and the next along its axis and the whole grid

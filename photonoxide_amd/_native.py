@@ -225,6 +225,8 @@ SIGNATURES = {
     "pxfdtd_simulation_mode_weights": (C.c_int, [vp, C.POINTER(PortMode), C.c_int32, C.POINTER(C.c_double)]),
     "pxfdtd_harmonic_inversion": (C.c_int, [C.POINTER(C.c_double), C.c_int64, C.c_double, C.c_double, C.c_double,
                                             C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "pxfdtd_harmonic_inversion_device": (C.c_int, [vp, vp, C.c_int64, C.c_double, C.c_double, C.c_double,
+                                                   C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), vp]),
     "pxfdtd_simulation_permittivity": (C.c_int, [vp, C.POINTER(C.c_double)]),
     "pxfdtd_simulation_port_modes": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_void_p, C.c_double, C.c_int32,
                                                C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double),

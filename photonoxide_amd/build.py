@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpxfdtd.so")
-SOURCES = ["resident.hip", "simulation.cpp", "portable.cpp", "batch_launch.hip", "flux.hip", "hbm.hip", "coefficients.hip",
+SOURCES = ["resident.hip", "simulation.cpp", "portable.cpp", "batch_launch.hip", "flux.hip", "harmonic_device.hip", "hbm.hip", "coefficients.hip",
            "step_f32_m0.hip", "step_f32_m1.hip", "step_f32_m2.hip",
            "step_f64_m0.hip", "step_f64_m1.hip", "step_f64_m2.hip"]
 FLAGS = [

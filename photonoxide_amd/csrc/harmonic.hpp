@@ -69,11 +69,6 @@ inline cd diagonal(const std::vector<cd> &c, int m, int p, cd z) {
     return total;
 }
 
-inline cd element(const std::vector<cd> &c, int m, int p, cd z, const Sums &sz, cd w, const Sums &sw) {
-    if (std::abs(z - w) <= 1e-13 * std::max(std::abs(z), std::abs(w))) return diagonal(c, m, p, z);
-    return (z * sw.f - w * sz.f - sz.zm * sw.g + sw.zm * sz.g) / (z - w);
-}
-
 // A = W S V^H by one-sided Jacobi on A's columns (n x n, row-major): the singular values in
 // descending order with W's and V's columns in the same order
 inline void svd(const std::vector<cd> &A, int n, std::vector<cd> &W, std::vector<double> &S, std::vector<cd> &V) {
@@ -191,10 +186,44 @@ inline bool eigen(const std::vector<cd> &A, int n, std::vector<cd> &values, std:
     return true;
 }
 
-// harmonic_inversion: false with the reason in *why for a refused input or a failed eigenproblem
-inline bool harmonic_inversion(const std::vector<cd> &signal, double dt, double f_low, double f_high,
-                               std::vector<Resonance> &found, std::string *why) {
-    found.clear();
+// The signal's sums at points z for shift p (sums_at) and the diagonal weighted sum (diagonal),
+// each point's from the same operations in the same order wherever they are computed: the host
+// below, or the device (harmonic_device.hip) from a signal held there.
+struct Evaluator {
+    virtual ~Evaluator() = default;
+    // for each point: its sums and its diagonal; false with the reason in *why on failure
+    virtual bool evaluate(int p, const std::vector<cd> &z, std::vector<Sums> &sums, std::vector<cd> &diag,
+                          std::string *why) const = 0;
+};
+
+struct HostSignal : Evaluator {
+    const std::vector<cd> &c;
+    int m;
+    HostSignal(const std::vector<cd> &signal, int m_) : c(signal), m(m_) {}
+    bool evaluate(int p, const std::vector<cd> &z, std::vector<Sums> &sums, std::vector<cd> &diag,
+                  std::string *) const override {
+        const int n = (int)z.size();
+        sums.resize(n);
+        diag.resize(n);
+#pragma omp parallel for schedule(static) if (n * (long long)m > 200000)
+        for (int q = 0; q < n; q++) {
+            sums[q] = sums_at(c, m, p, z[q]);
+            diag[q] = diagonal(c, m, p, z[q]);
+        }
+        return true;
+    }
+};
+
+// element of U^(p) at (z, w) from the points' sums, z's diagonal where the points meet
+inline cd element_of(cd z, const Sums &sz, cd dz, cd w, const Sums &sw) {
+    if (std::abs(z - w) <= 1e-13 * std::max(std::abs(z), std::abs(w))) return dz;
+    return (z * sw.f - w * sz.f - sz.zm * sw.g + sw.zm * sz.g) / (z - w);
+}
+
+// harmonic inversion's checks, and its parameters for n samples every dt over [f_low, f_high]:
+// M and the points z_j; false with the reason in *why
+inline bool harmonic_setup(long long n, bool finite, double dt, double f_low, double f_high, int &m,
+                           std::vector<cd> &z, std::string *why) {
     if (!(std::isfinite(dt) && dt > 0.0)) {
         *why = "harmonic inversion needs a positive, finite time step";
         return false;
@@ -204,36 +233,40 @@ inline bool harmonic_inversion(const std::vector<cd> &signal, double dt, double 
                std::to_string(0.5 / dt) + " c/um";
         return false;
     }
-    const long long n = (long long)signal.size();
-    bool finite = true;
-    for (const cd &v : signal) finite = finite && std::isfinite(v.real()) && std::isfinite(v.imag());
     if (n < 8 || !finite) {
         *why = "harmonic inversion needs at least 8 finite samples";
         return false;
     }
     const double TAU_ = 6.283185307179586, PI_ = 3.141592653589793;
-    const double tau = dt;
-    const int m = (int)((n - 3) / 2);
-    const double phi_low = TAU_ * f_low * tau, phi_high = TAU_ * f_high * tau;
+    m = (int)((n - 3) / 2);
+    const double phi_low = TAU_ * f_low * dt, phi_high = TAU_ * f_high * dt;
     const int j = std::max(8, (int)std::ceil((double)n * (phi_high - phi_low) / (4.0 * PI_)));
-    std::vector<cd> z(j);
+    z.resize(j);
     for (int q = 0; q < j; q++) {
         const double phi = phi_low + ((double)q + 0.5) * (phi_high - phi_low) / (double)j;
         z[q] = std::exp(cd(0.0, -phi));
     }
+    return true;
+}
+
+// the filter diagonalization from the signal's sums (taken by `ev`) at the points z
+inline bool harmonic_solve(const Evaluator &ev, int m, const std::vector<cd> &z, double dt, double f_low,
+                           double f_high, std::vector<Resonance> &found, std::string *why) {
+    found.clear();
+    const double TAU_ = 6.283185307179586;
+    const double tau = dt;
+    const int j = (int)z.size();
     std::vector<Sums> sums[3];
-    for (int p = 0; p < 3; p++) {
-        sums[p].resize(j);
-#pragma omp parallel for schedule(static) if (j * (long long)m > 200000)
-        for (int q = 0; q < j; q++) sums[p][q] = sums_at(signal, m, p, z[q]);
-    }
+    std::vector<cd> diag[3];
+    for (int p = 0; p < 3; p++)
+        if (!ev.evaluate(p, z, sums[p], diag[p], why)) return false;
     std::vector<cd> U[3];
     for (int p = 0; p < 3; p++) {
         U[p].assign((size_t)j * j, 0.0);
-#pragma omp parallel for schedule(dynamic) if (j * (long long)m > 200000)
+#pragma omp parallel for schedule(dynamic) if (j * (long long)j > 40000)
         for (int a = 0; a < j; a++)
             for (int b = 0; b < j; b++)
-                U[p][(size_t)a * j + b] = element(signal, m, p, z[a], sums[p][a], z[b], sums[p][b]);
+                U[p][(size_t)a * j + b] = element_of(z[a], sums[p][a], diag[p][a], z[b], sums[p][b]);
     }
     std::vector<cd> W, Vs;
     std::vector<double> S;
@@ -261,12 +294,26 @@ inline bool harmonic_inversion(const std::vector<cd> &signal, double dt, double 
         *why = "harmonic inversion's eigenproblem failed";
         return false;
     }
+    // the eigenvalues in the window, then the signal's sums at each at once
+    std::vector<int> kept;
+    std::vector<cd> uks;
     for (int k = 0; k < rank; k++) {
         const cd uk = values[k];
         if (std::abs(uk) == 0.0) continue;
         const cd omega = cd(-std::arg(uk), std::log(std::abs(uk))) / tau;
         const double frequency = omega.real() / TAU_;
         if (!(frequency >= f_low && frequency <= f_high)) continue;
+        kept.push_back(k);
+        uks.push_back(uk);
+    }
+    std::vector<Sums> su;
+    std::vector<cd> du;
+    if (!uks.empty() && !ev.evaluate(0, uks, su, du, why)) return false;
+    for (size_t e = 0; e < kept.size(); e++) {
+        const int k = kept[e];
+        const cd uk = values[k];
+        const cd omega = cd(-std::arg(uk), std::log(std::abs(uk))) / tau;
+        const double frequency = omega.real() / TAU_;
         std::vector<cd> b((size_t)j, 0.0);
         for (int r = 0; r < j; r++)
             for (int c = 0; c < rank; c++) b[r] += Vs[(size_t)r * j + c] * vectors[(size_t)c * rank + k];
@@ -289,14 +336,25 @@ inline bool harmonic_inversion(const std::vector<cd> &signal, double dt, double 
             residual += std::norm(u2b[r] - uk2 * u0b[r]);
             size += std::norm(uk2 * u0b[r]);
         }
-        const Sums su = sums_at(signal, m, 0, uk);
         cd sum = 0.0;
-        for (int a = 0; a < j; a++) sum += b[a] * element(signal, m, 0, z[a], sums[0][a], uk, su);
+        for (int a = 0; a < j; a++) sum += b[a] * element_of(z[a], sums[0][a], diag[0][a], uk, su[e]);
         const cd root = sum / (double)(m + 1);
         found.push_back({frequency, -omega.imag(), root * root, std::sqrt(residual) / std::sqrt(size)});
     }
     std::sort(found.begin(), found.end(), [](const Resonance &x, const Resonance &y) { return x.frequency < y.frequency; });
     return true;
+}
+
+// harmonic_inversion: false with the reason in *why for a refused input or a failed eigenproblem
+inline bool harmonic_inversion(const std::vector<cd> &signal, double dt, double f_low, double f_high,
+                               std::vector<Resonance> &found, std::string *why) {
+    found.clear();
+    bool finite = true;
+    for (const cd &v : signal) finite = finite && std::isfinite(v.real()) && std::isfinite(v.imag());
+    int m;
+    std::vector<cd> z;
+    if (!harmonic_setup((long long)signal.size(), finite, dt, f_low, f_high, m, z, why)) return false;
+    return harmonic_solve(HostSignal(signal, m), m, z, dt, f_low, f_high, found, why);
 }
 
 }  // namespace pxharm

@@ -264,6 +264,23 @@ def harmonic_inversion(signal, dt: float, low: float, high: float):
                       float(out[5 * q + 4])) for q in range(found.value)]
 
 
+def harmonic_inversion_device(gpu: "Gpu", signal_ptr, n: int, dt: float, low: float, high: float, stream=None):
+    """harmonic_inversion of n complex FP64 samples held in `gpu`'s memory at `signal_ptr` (a
+    device address): the sums over the samples run on the device, only they come back
+    (pxfdtd_harmonic_inversion_device); the same resonances as harmonic_inversion."""
+    capacity = 64
+    while True:
+        out = np.zeros(5 * capacity)
+        found = C.c_int32()
+        N.check(N.lib.pxfdtd_harmonic_inversion_device(gpu.handle, signal_ptr, int(n), float(dt), float(low),
+                                                       float(high), capacity, C.byref(found), N.dptr(out), stream))
+        if found.value <= capacity:
+            break
+        capacity = found.value
+    return [Resonance(float(out[5 * q]), float(out[5 * q + 1]), complex(out[5 * q + 2], out[5 * q + 3]),
+                      float(out[5 * q + 4])) for q in range(found.value)]
+
+
 def _port_mode(mode: PortMode):
     """The C ABI's pxfdtd_port_mode for `mode`, and the arrays it points into (kept alive by
     the caller while the struct is in use)."""

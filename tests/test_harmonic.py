@@ -146,3 +146,44 @@ def test_a_slabs_resonance_and_q_converge_to_the_continuums_at_second_order():
     for c, f in zip(coarse, fine):
         assert abs(c / f - 4.0) < 0.1
     assert fine[0] < 1e-3 and fine[1] < 1e-2
+
+
+@pytest.mark.gpu
+def test_harmonic_inversion_on_the_device_is_the_hosts_bit_for_bit():
+    """A signal held in device memory (pxfdtd_harmonic_inversion_device): the sums over its
+    samples run on the device and give every resonance of the host's, bit for bit, on the
+    synthetic two-term signal and on a lossy cavity's probe; the refusals are the host's."""
+    import photonoxide_amd as P
+    from photonoxide_amd.fdtd import harmonic_inversion_device
+    from photonoxide_amd._native import NativeError
+    from devmem import Buffer
+    print("case harmonic inversion on the device", flush=True)
+    gpu = P.Gpu(0)
+    tau, n = 0.1, 3001
+    sig = signal([(0.95, 0.01, 1.0), (0.96, 0.02, 0.5j), (1.3, 0.0, 0.2)], tau, n)
+    nx, ny, h = 24, 18, 0.05
+    g = P.Grid3d(nx, ny, 1, h, h, h, -nx * h / 2, -ny * h / 2, -h / 2)
+    s = P.Simulation.new(g, P.Scene(2.0), P.Boundaries(P.Pml(0, 0), P.Pml(0, 0), P.Periodic()), 0.9)
+    s.with_conductivity(P.Scene(0.05))
+    pulse = P.Waveform.pulse(0.8, 0.8)
+    s.add_source(P.Source(P.Field.E, P.Axis.Z, (7, 5, 0), pulse))
+    p = s.add_probe(P.Field.E, P.Axis.Z, (16, 11, 0))
+    s.set_device(gpu, P.Precision.F64)
+    s.run_until(pulse.delay + 6.0 * pulse.width)
+    start = len(s.probe(p))
+    s.run(4000)
+    cavity = s.probe(p)[start:].astype(complex)
+    for x, dt, lo, hi in [(sig, tau, 0.9, 1.1), (cavity, s.dt(), 0.4, 1.2)]:
+        host = P.harmonic_inversion(x, dt, lo, hi)
+        buf = Buffer.of(np.ascontiguousarray(x).view(np.float64))
+        dev = harmonic_inversion_device(gpu, buf.ptr, len(x), dt, lo, hi)
+        assert len(dev) == len(host) > 0
+        for a, b in zip(dev, host):
+            assert (a.frequency, a.decay, a.amplitude, a.error) == (b.frequency, b.decay, b.amplitude, b.error)
+    bad = sig.copy()
+    bad[17] = np.nan
+    buf = Buffer.of(bad.view(np.float64))
+    with pytest.raises(NativeError, match="finite samples"):
+        harmonic_inversion_device(gpu, buf.ptr, len(bad), tau, 0.9, 1.1)
+    with pytest.raises(NativeError, match="below the Nyquist"):
+        harmonic_inversion_device(gpu, buf.ptr, len(bad), tau, 1.1, 0.9)
