@@ -11,7 +11,9 @@ block at random densities, a pulsed E_z point source at 1.55 um:
   restatement's formula on the device's transforms;
 - the flux through a whole plane (a flux term, its values reaching into the CPMLs, where the
   adjoint's sources are divided by the symmetrizer D): the density gradient matches central
-  differences at design cells.
+  differences at design cells;
+- with a Drude + Lorentz block beside the design, whose ADE the adjoint run steps as the
+  forward does, the density gradient matches central differences.
 """
 import math
 import os
@@ -129,3 +131,45 @@ def test_flux_term_gradient_matches_central_differences(gpu):
         print(f"cell {cell}: adjoint {g_rho[cell]:.9e}, central difference {fd:.9e}, {rel:.2e} of the largest",
               flush=True)
     assert worst < 1e-6, worst
+
+
+def test_gradient_with_a_dispersive_medium_matches_central_differences(gpu):
+    """The frequency-domain adjoint with a Drude + Lorentz block beside the design: its adjoint
+    run steps the same medium (the ADE terms sit on K's diagonal blocks, symmetric,
+    src/fdtd/adjoint.rs:9-12), and the density gradient matches central differences."""
+    import photonoxide_amd as P
+    print("case design beside a Drude + Lorentz block, FP64", flush=True)
+
+    def with_media(rho):
+        s, design, mon = build(rho)
+        box = P.Scene(0.0, [P.Box((0.2, -0.25, -0.15), (0.45, 0.25, 0.15), 1.0)])
+        s.with_medium(2.0, [("drude", 1.2, 0.05), ("lorentz", 0.8, 1.1, 0.02)], box)
+        return s, design, mon
+
+    rho = np.random.default_rng(13).random((3, 5, 5))
+    s, design, mon = with_media(rho)
+    dm = s.add_design_monitor(design, [F0])
+    s.set_device(gpu, P.Precision.F64)
+    assert s.run_until_converged(TOL, INTERVAL, LIMIT)
+    steps = s.steps()
+    S = s.dft(mon, P.Field.E, P.Axis.Z)[0, 0, 0, 0]
+    adj = s.adjoint([P.Term.transform(mon, P.Field.E, P.Axis.Z, (21, 13, 10), 0, np.conj(S))], dm, 0.15)
+    assert adj.run_until_converged(TOL, INTERVAL, LIMIT)
+    g_rho = s.gradient(adj, dm, design)
+    d = 1e-3
+    worst = 0.0
+    for cell in [(1, 2, 2), (2, 0, 4)]:
+        vals = []
+        for sign in (1, -1):
+            r = rho.copy()
+            r[cell] += sign * d
+            t, _, m = with_media(r)
+            t.set_device(gpu, P.Precision.F64)
+            t.run(steps)
+            vals.append(abs(t.dft(m, P.Field.E, P.Axis.Z)[0, 0, 0, 0]) ** 2)
+        fd = (vals[0] - vals[1]) / (2 * d)
+        rel = abs(g_rho[cell] - fd) / np.max(np.abs(g_rho))
+        worst = max(worst, rel)
+        print(f"cell {cell}: adjoint {g_rho[cell]:.9e}, central difference {fd:.9e}, {rel:.2e} of the largest",
+              flush=True)
+    assert worst < 1e-5, worst
