@@ -692,6 +692,15 @@ int pxfdtd_simulation_design_gradient(pxfdtd_simulation *s, pxfdtd_simulation *a
 int pxfdtd_simulation_run_until_converged(pxfdtd_simulation *s, double tolerance, double interval, double limit,
                                           int32_t *settled);
 
+/* The library's portable maths, evaluated once (no counterpart in the reference's API: these are
+ * the functions of src/portable that the host layer computes its waveforms, coefficients, beams,
+ * port modes and resonances with), so that tests can hold their bits to the reference's hashes over
+ * its argument grid (src/portable/tests.rs:400-548). `name` is the reference's: "exp", "ln",
+ * "ln_1p", "asin", "atan", "acosh" (of a); "atan2" (y = a, x = b), "hypot", "powf", "powi" (n = b,
+ * an integer); "c_sqrt", "c_asin", "c_ln", "c_polar" (|z| and arg z) of z = a + ib. A real result
+ * in *re, *im = 0. PXFDTD_ERR_INVALID for another name. */
+int pxfdtd_portable(const char *name, double a, double b, double *re, double *im);
+
 #ifdef __cplusplus
 }
 #endif

@@ -265,6 +265,7 @@ SIGNATURES = {
     "pxfdtd_simulation_permittivity_gradient": (C.c_int, [vp, vp, C.c_int32, pd]),
     "pxfdtd_simulation_design_gradient": (C.c_int, [vp, vp, C.c_int32, pd]),
     "pxfdtd_simulation_run_until_converged": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, pi32]),
+    "pxfdtd_portable": (C.c_int, [C.c_char_p, C.c_double, C.c_double, pd, pd]),
 }
 
 for _name, (_res, _args) in SIGNATURES.items():

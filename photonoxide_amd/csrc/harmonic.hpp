@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "eigen.hpp"
+#include "portable.hpp"
 
 namespace pxharm {
 
@@ -244,9 +245,17 @@ inline bool harmonic_setup(long long n, bool finite, double dt, double f_low, do
     z.resize(j);
     for (int q = 0; q < j; q++) {
         const double phi = phi_low + ((double)q + 0.5) * (phi_high - phi_low) / (double)j;
-        z[q] = std::exp(cd(0.0, -phi));
+        const pxfdtd::C64 p = pxfdtd::pcexp({0.0, -phi});  // portable::complex::exp
+        z[q] = cd(p.re, p.im);
     }
     return true;
+}
+
+// w = (-arg u, ln |u|) / tau, with the portable modulus, argument and logarithm
+inline cd angular(const cd &u, double tau) {
+    const pxfdtd::C64 v = {u.real(), u.imag()};
+    const double re = -pxfdtd::pcarg(v), im = pxfdtd::pln(pxfdtd::pcabs(v));
+    return cd(re / tau, im / tau);
 }
 
 // the filter diagonalization from the signal's sums (taken by `ev`) at the points z
@@ -299,8 +308,8 @@ inline bool harmonic_solve(const Evaluator &ev, int m, const std::vector<cd> &z,
     std::vector<cd> uks;
     for (int k = 0; k < rank; k++) {
         const cd uk = values[k];
-        if (std::abs(uk) == 0.0) continue;
-        const cd omega = cd(-std::arg(uk), std::log(std::abs(uk))) / tau;
+        if (pxfdtd::pcabs({uk.real(), uk.imag()}) == 0.0) continue;
+        const cd omega = angular(uk, tau);
         const double frequency = omega.real() / TAU_;
         if (!(frequency >= f_low && frequency <= f_high)) continue;
         kept.push_back(k);
@@ -312,7 +321,7 @@ inline bool harmonic_solve(const Evaluator &ev, int m, const std::vector<cd> &z,
     for (size_t e = 0; e < kept.size(); e++) {
         const int k = kept[e];
         const cd uk = values[k];
-        const cd omega = cd(-std::arg(uk), std::log(std::abs(uk))) / tau;
+        const cd omega = angular(uk, tau);
         const double frequency = omega.real() / TAU_;
         std::vector<cd> b((size_t)j, 0.0);
         for (int r = 0; r < j; r++)
@@ -326,7 +335,9 @@ inline bool harmonic_solve(const Evaluator &ev, int m, const std::vector<cd> &z,
         std::vector<cd> u0b = times(0, b);
         cd norm = 0.0;
         for (int r = 0; r < j; r++) norm += b[r] * u0b[r];
-        const cd scale = 1.0 / std::sqrt(norm);
+        const pxfdtd::C64 sq = pxfdtd::pcsqrt({norm.real(), norm.imag()});
+        const double sn = sq.re * sq.re + sq.im * sq.im;  // num-complex's inv
+        const cd scale = cd(sq.re / sn, 0.0 - sq.im / sn);
         for (cd &x : b) x *= scale;
         u0b = times(0, b);
         const std::vector<cd> u2b = times(2, b);

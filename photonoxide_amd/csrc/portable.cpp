@@ -3,6 +3,11 @@
 
 #include <cmath>
 #include <cstring>
+#include <string>
+
+#include "../../include/pxfdtd.h"
+
+int px_fail(int code, const char *fmt, ...);  // resident.hip
 
 namespace pxfdtd {
 namespace {
@@ -443,4 +448,429 @@ double ppow(double x, double y) {
     return s * z;
 }
 
+/* sqrt(x^2 + y^2) with no overflow or underflow in between, as the reference computes it
+ * (src/portable/mod.rs:246-248 takes the `libm` crate 0.2.16's hypot, its port of musl's
+ * hypot.c): |x| >= |y| ordered by their bits, inf before NaN, the larger alone where the
+ * exponents are more than 64 apart, a scale of 2^-+700 where a square could leave the range, and
+ * each square as an exact head and tail by Dekker's split at 2^27 + 1. */
+double phypot(double x, double y) {
+    uint64_t ux = bits(x) & 0x7fffffffffffffffULL, uy = bits(y) & 0x7fffffffffffffffULL;
+    if (ux < uy) {
+        const uint64_t t = ux;
+        ux = uy;
+        uy = t;
+    }
+    const int ex = (int)(ux >> 52), ey = (int)(uy >> 52);
+    x = from_bits(ux);
+    y = from_bits(uy);
+    if (ey == 0x7ff) return y;
+    if (ex == 0x7ff || uy == 0) return x;
+    if (ex - ey > 64) return x + y;
+    double z = 1.0;
+    if (ex > 0x3ff + 510) {
+        z = 0x1p700;
+        x *= 0x1p-700;
+        y *= 0x1p-700;
+    } else if (ey < 0x3ff - 450) {
+        z = 0x1p-700;
+        x *= 0x1p700;
+        y *= 0x1p700;
+    }
+    auto square = [](double v, double &hi, double &lo) {
+        const double vc = v * 134217729.0;  // 2^27 + 1
+        const double vh = v - vc + vc, vl = v - vh;
+        hi = v * v;
+        lo = vh * vh - hi + 2.0 * vh * vl + vl * vl;
+    };
+    double hx, lx, hy, ly;
+    square(x, hx, lx);
+    square(y, hy, ly);
+    return z * std::sqrt(ly + lx + hy + hx);
+}
+
+/* The arctangent (src/portable/mod.rs:253-255, the `libm` crate's port of fdlibm's s_atan.c, Sun
+ * Microsystems 1993): |x| reduced to [-7/16, 7/16] about 0, 0.5, 1, 1.5 or inf, atan of the
+ * reduced argument by fdlibm's odd degree-23 polynomial in two halves, plus atan of the centre as
+ * a head and tail. */
+double patan(double x) {
+    static const uint64_t hi_bits[4] = {0x3fddac670561bb4fULL, 0x3fe921fb54442d18ULL, 0x3fef730bd281f69bULL,
+                                        0x3ff921fb54442d18ULL};
+    static const uint64_t lo_bits[4] = {0x3c7a2b7f222f65e2ULL, 0x3c81a62633145c07ULL, 0x3c7007887af0cbbdULL,
+                                        0x3c91a62633145c07ULL};
+    static const uint64_t at_bits[11] = {0x3fd555555555550dULL, 0xbfc999999998ebc4ULL, 0x3fc24924920083ffULL,
+                                         0xbfbc71c6fe231671ULL, 0x3fb745cdc54c206eULL, 0xbfb3b0f2af749a6dULL,
+                                         0x3fb10d66a0d03d51ULL, 0xbfadde2d52defd9aULL, 0x3fa97b4b24760debULL,
+                                         0xbfa2b4442c6a6c2fULL, 0x3f90ad3ae322da11ULL};
+    double at[11];
+    for (int q = 0; q < 11; q++) at[q] = from_bits(at_bits[q]);
+    uint32_t ix = hi_word(x);
+    const bool negative = (ix >> 31) != 0;
+    ix &= 0x7fffffffu;
+    int id;
+    if (ix >= 0x44100000u) {  // |x| >= 2^66 or NaN
+        if (std::isnan(x)) return x;
+        const double z = from_bits(hi_bits[3]);
+        return negative ? -z : z;
+    }
+    if (ix < 0x3fdc0000u) {  // |x| < 7/16
+        if (ix < 0x3e400000u) return x;
+        id = -1;
+    } else {
+        x = std::fabs(x);
+        if (ix < 0x3ff30000u) {
+            if (ix < 0x3fe60000u) {
+                id = 0;
+                x = (2.0 * x - 1.0) / (2.0 + x);
+            } else {
+                id = 1;
+                x = (x - 1.0) / (x + 1.0);
+            }
+        } else if (ix < 0x40038000u) {
+            id = 2;
+            x = (x - 1.5) / (1.0 + 1.5 * x);
+        } else {
+            id = 3;
+            x = -1.0 / x;
+        }
+    }
+    const double z = x * x, w = z * z;
+    const double s1 = z * (at[0] + w * (at[2] + w * (at[4] + w * (at[6] + w * (at[8] + w * at[10])))));
+    const double s2 = w * (at[1] + w * (at[3] + w * (at[5] + w * (at[7] + w * at[9]))));
+    if (id < 0) return x - x * (s1 + s2);
+    const double r = from_bits(hi_bits[id]) - (x * (s1 + s2) - from_bits(lo_bits[id]) - x);
+    return negative ? -r : r;
+}
+
+/* The angle of (x, y) in [-pi, pi] (src/portable/mod.rs:265-266, the `libm` crate's port of
+ * fdlibm's e_atan2.c): C's cases for zeros and infinities by the signs, pi/2 where |y/x| > 2^64,
+ * else atan|y/x| moved to the quadrant with pi as a head and tail. */
+double patan2(double y, double x) {
+    const double pi = from_bits(0x400921fb54442d18ULL), pi_lo = from_bits(0x3ca1a62633145c07ULL);
+    if (std::isnan(x) || std::isnan(y)) return x + y;
+    uint32_t ix = hi_word(x), iy = hi_word(y);
+    const uint32_t lx = lo_word(x), ly = lo_word(y);
+    if (((ix - 0x3ff00000u) | lx) == 0) return patan(y);  // x = 1
+    const uint32_t m = ((iy >> 31) & 1u) | ((ix >> 30) & 2u);  // 2 sign(x) + sign(y)
+    ix &= 0x7fffffffu;
+    iy &= 0x7fffffffu;
+    if ((iy | ly) == 0) {
+        switch (m) {
+        case 0:
+        case 1: return y;
+        case 2: return pi;
+        default: return -pi;
+        }
+    }
+    if ((ix | lx) == 0) return (m & 1u) ? -pi / 2.0 : pi / 2.0;
+    if (ix == 0x7ff00000u) {
+        if (iy == 0x7ff00000u) {
+            switch (m) {
+            case 0: return pi / 4.0;
+            case 1: return -pi / 4.0;
+            case 2: return 3.0 * pi / 4.0;
+            default: return -3.0 * pi / 4.0;
+            }
+        }
+        switch (m) {
+        case 0: return 0.0;
+        case 1: return -0.0;
+        case 2: return pi;
+        default: return -pi;
+        }
+    }
+    if (ix + (64u << 20) < iy || iy == 0x7ff00000u) return (m & 1u) ? -pi / 2.0 : pi / 2.0;
+    const double z = ((m & 2u) && iy + (64u << 20) < ix) ? 0.0 : patan(std::fabs(y / x));
+    switch (m) {
+    case 0: return z;
+    case 1: return -z;
+    case 2: return pi - (z - pi_lo);
+    default: return (z - pi_lo) - pi;
+    }
+}
+
+/* The arcsine (src/portable/mod.rs:249-251, the `libm` crate's port of fdlibm's e_asin.c):
+ * x + x R(x^2) below 0.5 with fdlibm's rational R, pi/2 - 2 asin(sqrt((1 - |x|)/2)) above, the
+ * square root split into a 32-bit head and its remainder between 0.5 and 0.975. */
+double pasin(double x) {
+    const double pio2_hi = from_bits(0x3ff921fb54442d18ULL), pio2_lo = from_bits(0x3c91a62633145c07ULL);
+    auto rational = [](double z) {
+        const double p0 = from_bits(0x3fc5555555555555ULL), p1 = from_bits(0xbfd4d61203eb6f7dULL),
+                     p2 = from_bits(0x3fc9c1550e884455ULL), p3 = from_bits(0xbfa48228b5688f3bULL),
+                     p4 = from_bits(0x3f49efe07501b288ULL), p5 = from_bits(0x3f023de10dfdf709ULL);
+        const double q1 = from_bits(0xc0033a271c8a2d4bULL), q2 = from_bits(0x40002ae59c598ac8ULL),
+                     q3 = from_bits(0xbfe6066c1b8d0159ULL), q4 = from_bits(0x3fb3b8c5b12e9282ULL);
+        const double p = z * (p0 + z * (p1 + z * (p2 + z * (p3 + z * (p4 + z * p5)))));
+        const double q = 1.0 + z * (q1 + z * (q2 + z * (q3 + z * q4)));
+        return p / q;
+    };
+    const uint32_t hx = hi_word(x), ix = hx & 0x7fffffffu;
+    if (ix >= 0x3ff00000u) {  // |x| >= 1 or NaN
+        if (((ix - 0x3ff00000u) | lo_word(x)) == 0) return x * pio2_hi;
+        return 0.0 / (x - x);
+    }
+    if (ix < 0x3fe00000u) {  // |x| < 0.5
+        if (ix < 0x3e500000u && ix >= 0x00100000u) return x;
+        return x + x * rational(x * x);
+    }
+    const double z = (1.0 - std::fabs(x)) * 0.5, s = std::sqrt(z), r = rational(z);
+    double v;
+    if (ix >= 0x3fef3333u) {  // |x| > 0.975
+        v = pio2_hi - (2.0 * (s + s * r) - pio2_lo);
+    } else {
+        const double f = with_lo(s, 0), c = (z - f * f) / (s + f);
+        v = 0.5 * pio2_hi - (2.0 * s * r - (pio2_lo - 2.0 * c) - (0.5 * pio2_hi - 2.0 * f));
+    }
+    return (hx >> 31) ? -v : v;
+}
+
+namespace {
+// log(1 + f) - f + f^2/2 for 1 + f in [sqrt(2)/2, sqrt(2)): fdlibm's polynomial in s^2,
+// s = f / (2 + f), the part musl's log and log1p share
+struct LogPoly {
+    double s, hfsq, r;
+};
+LogPoly log_poly(double f) {
+    const double lg1 = from_bits(0x3fe5555555555593ULL), lg2 = from_bits(0x3fd999999997fa04ULL);
+    const double lg3 = from_bits(0x3fd2492494229359ULL), lg4 = from_bits(0x3fcc71c51d8e78afULL);
+    const double lg5 = from_bits(0x3fc7466496cb03deULL), lg6 = from_bits(0x3fc39a09d078c69fULL);
+    const double lg7 = from_bits(0x3fc2f112df3e5244ULL);
+    LogPoly q;
+    q.hfsq = 0.5 * f * f;
+    q.s = f / (2.0 + f);
+    const double z = q.s * q.s, w = z * z;
+    const double t1 = w * (lg2 + w * (lg4 + w * lg6));
+    const double t2 = z * (lg1 + w * (lg3 + w * (lg5 + w * lg7)));
+    q.r = t2 + t1;
+    return q;
+}
+const double LN2_HI = from_bits(0x3fe62e42fee00000ULL), LN2_LO = from_bits(0x3dea39ef35793c76ULL);
+
+/* The natural logarithm as the `libm` crate 0.2.16 computes it (musl's log.c, its shorter
+ * rewrite of fdlibm's e_log.c, whose bits differ at times from the fdlibm port `pln`): the
+ * reference's acosh takes it through `libm::acosh` (src/portable/mod.rs:310-315). */
+double musl_log(double x) {
+    uint64_t u = bits(x);
+    uint32_t hx = (uint32_t)(u >> 32);
+    int k = 0;
+    if (hx < 0x00100000u || (hx >> 31)) {
+        if ((u << 1) == 0) return -1.0 / (x * x);
+        if (hx >> 31) return (x - x) / 0.0;
+        k -= 54;  // subnormal: scale up
+        x *= 0x1p54;
+        u = bits(x);
+        hx = (uint32_t)(u >> 32);
+    } else if (hx >= 0x7ff00000u) {
+        return x;
+    } else if (hx == 0x3ff00000u && (u << 32) == 0) {
+        return 0.0;
+    }
+    hx += 0x3ff00000u - 0x3fe6a09eu;
+    k += (int)(hx >> 20) - 0x3ff;
+    hx = (hx & 0x000fffffu) + 0x3fe6a09eu;
+    x = from_bits((uint64_t)hx << 32 | (u & 0xffffffffULL));
+    const double f = x - 1.0;
+    const LogPoly q = log_poly(f);
+    const double dk = (double)k;
+    return q.s * (q.hfsq + q.r) + dk * LN2_LO - q.hfsq + f + dk * LN2_HI;
+}
+}  // namespace
+
+/* ln(1 + x), accurate near 0 (src/portable/mod.rs:236-238, the `libm` crate's port of musl's
+ * log1p.c after fdlibm's s_log1p.c): 1 + x = 2^k (1 + f) with a correction c for the rounding of
+ * 1 + x, then the logarithm's polynomial. */
+double plog1p(double x) {
+    const uint32_t hx = hi_word(x);
+    int k = 1;
+    double c = 0.0, f = 0.0;
+    if (hx < 0x3fda827au || (hx >> 31)) {  // 1 + x < sqrt(2)
+        if (hx >= 0xbff00000u) {            // x <= -1
+            if (x == -1.0) return x / 0.0;
+            return (x - x) / 0.0;
+        }
+        if ((hx << 1) < (0x3ca00000u << 1)) return x;  // |x| < 2^-53
+        if (hx <= 0xbfd2bec4u) {                        // sqrt(2)/2 <= 1 + x < sqrt(2)
+            k = 0;
+            c = 0.0;
+            f = x;
+        }
+    } else if (hx >= 0x7ff00000u) {
+        return x;
+    }
+    if (k) {
+        const double u = 1.0 + x;
+        uint64_t ui = bits(u);
+        uint32_t hu = (uint32_t)(ui >> 32);
+        hu += 0x3ff00000u - 0x3fe6a09eu;
+        k = (int)(hu >> 20) - 0x3ff;
+        if (k < 54) {
+            c = k >= 2 ? 1.0 - (u - x) : x - (u - 1.0);
+            c /= u;
+        } else {
+            c = 0.0;
+        }
+        hu = (hu & 0x000fffffu) + 0x3fe6a09eu;
+        ui = (uint64_t)hu << 32 | (ui & 0xffffffffULL);
+        f = from_bits(ui) - 1.0;
+    }
+    const LogPoly q = log_poly(f);
+    const double dk = (double)k;
+    return q.s * (q.hfsq + q.r) + (dk * LN2_LO + c) - q.hfsq + f + dk * LN2_HI;
+}
+
+/* The inverse hyperbolic cosine (src/portable/mod.rs:310-315): NaN below 1, else the `libm`
+ * crate's port of musl's acosh.c, log1p(t + sqrt(t^2 + 2t)) with t = x - 1 below 2,
+ * log(2x - 1/(x + sqrt(x^2 - 1))) below 2^26 and log x + ln 2 above, with musl's log. */
+double pacosh(double x) {
+    if (x < 1.0) return NAN;
+    const unsigned e = (unsigned)(bits(x) >> 52) & 0x7ffu;
+    if (e < 0x3ff + 1) return plog1p(x - 1.0 + std::sqrt((x - 1.0) * (x - 1.0) + 2.0 * (x - 1.0)));
+    if (e < 0x3ff + 26) return musl_log(2.0 * x - 1.0 / (x + std::sqrt(x * x - 1.0)));
+    return musl_log(x) + from_bits(0x3fe62e42fefa39efULL);
+}
+
+double pcabs(C64 z) { return phypot(z.re, z.im); }
+double pcarg(C64 z) { return patan2(z.im, z.re); }
+
+/* The principal square root (src/portable/complex.rs:132-180): Kahan's formulas, t =
+ * sqrt((|a| + |z|)/2), then (t, b/2t) for a >= 0 and (|b|/2t, +-t) below, scaled by 1/4 (and 2
+ * after) near overflow and by 2^54 (and 2^-27) near underflow; C99's special values, a zero
+ * imaginary part's sign kept on the cut. */
+C64 pcsqrt(C64 z) {
+    const double big = from_bits(0x7fc0000000000000ULL), small = from_bits(0x0170000000000000ULL);
+    const double up = from_bits(0x4350000000000000ULL), down = from_bits(0x3e40000000000000ULL);
+    double x = z.re, y = z.im;
+    if (y == 0.0) {
+        if (!std::signbit(x)) return {std::sqrt(x), y};
+        return {0.0, std::copysign(std::sqrt(-x), y)};
+    }
+    if (std::isinf(y)) return {INFINITY, y};
+    if (std::isnan(x) || std::isnan(y)) return {NAN, NAN};
+    if (std::isinf(x)) return x > 0.0 ? C64{x, std::copysign(0.0, y)} : C64{0.0, std::copysign(INFINITY, y)};
+    const double m = std::fmax(std::fabs(x), std::fabs(y));
+    double scale = 1.0;
+    if (m >= big) {
+        x *= 0.25;
+        y *= 0.25;
+        scale = 2.0;
+    } else if (m < small) {
+        x *= up;
+        y *= up;
+        scale = down;
+    }
+    const double t = std::sqrt(0.5 * (std::fabs(x) + phypot(x, y)));
+    double re, im;
+    if (x >= 0.0) {
+        re = t;
+        im = y / (2.0 * t);
+    } else {
+        re = std::fabs(y) / (2.0 * t);
+        im = std::copysign(t, y);
+    }
+    return {re * scale, im * scale};
+}
+
+/* The principal logarithm (src/portable/complex.rs:106-116): ln|z| + i arg z, the real part as
+ * ln_1p((a - 1)(a + 1) + b^2)/2 where |z| is within [1/sqrt(2), sqrt(2)] (a >= b the parts'
+ * sizes). */
+C64 pcln(C64 z) {
+    const double r = pcabs(z), theta = pcarg(z);
+    if (r >= from_bits(0x3fe6a09e667f3bcdULL) && r <= from_bits(0x3ff6a09e667f3bcdULL)) {
+        const double x = std::fabs(z.re), y = std::fabs(z.im);
+        const double a = x >= y ? x : y, b = x >= y ? y : x;
+        const double t = (a - 1.0) * (a + 1.0) + b * b;
+        return {0.5 * plog1p(t), theta};
+    }
+    return {pln(r), theta};
+}
+
+/* The principal arcsine (src/portable/complex.rs:344-407): Hull, Fairgrieve and Tang's algorithm
+ * (ACM TOMS 23, 299, 1997) on the first quadrant, r = |z + 1|, s = |z - 1|, A = (r + s)/2,
+ * B = x/A; the real part asin B up to B = 0.6417 and an arctangent of positive sums above, the
+ * imaginary ln_1p of positive sums up to A = 1.5 and ln(A + sqrt(A^2 - 1)) above; z itself below
+ * 2^-27, atan2 and ln|z| + ln 2 beyond 2^500, the limits below an imaginary part of 2^-500; the
+ * signs put back by asin(-z) = -asin z and asin(conj z) = conj asin z. */
+C64 pcasin(C64 z) {
+    const double large = from_bits(0x5f30000000000000ULL), tiny = from_bits(0x20b0000000000000ULL);
+    const double small = from_bits(0x3e40000000000000ULL), half_pi = from_bits(0x3ff921fb54442d18ULL);
+    const double x = std::fabs(z.re), y = std::fabs(z.im);
+    if (std::isnan(x) || std::isnan(y)) {
+        if (std::isinf(y)) return {NAN, z.im};
+        if (x == 0.0) return {z.re, NAN};
+        if (std::isinf(x)) return {NAN, INFINITY};
+        return {NAN, NAN};
+    }
+    double re, im;
+    const double big = std::fmax(x, y);
+    if (big < small) {
+        re = x;
+        im = y;
+    } else if (big >= large) {
+        re = patan2(x, y);
+        im = pln(phypot(x, y)) + from_bits(0x3fe62e42fefa39efULL);
+    } else if (y < tiny) {
+        if (x < 1.0) {
+            re = pasin(x);
+            im = y / std::sqrt((1.0 - x) * (1.0 + x));
+        } else if (x == 1.0) {
+            re = half_pi;
+            im = std::sqrt(y);
+        } else {
+            re = half_pi;
+            im = pacosh(x);
+        }
+    } else {
+        const double xp1 = x + 1.0;
+        const double r = phypot(xp1, y), s = phypot(x - 1.0, y);
+        const double a = 0.5 * (r + s), b = x / a, yy = y * y;
+        if (b <= 0.6417) {
+            re = pasin(b);
+        } else {
+            const double apx = a + x;
+            if (x <= 1.0)
+                re = patan(x / std::sqrt(0.5 * apx * (yy / (r + xp1) + (s + (1.0 - x)))));
+            else
+                re = patan(x / (y * std::sqrt(0.5 * (apx / (r + xp1) + apx / (s + (x - 1.0))))));
+        }
+        if (a <= 1.5) {
+            const double am1 = x < 1.0 ? 0.5 * (yy / (r + xp1) + yy / (s + (1.0 - x)))
+                                       : 0.5 * (yy / (r + xp1) + (s + (x - 1.0)));
+            im = plog1p(am1 + std::sqrt(am1 * (a + 1.0)));
+        } else {
+            im = pln(a + std::sqrt(a * a - 1.0));
+        }
+    }
+    return {std::copysign(re, z.re), std::copysign(im, z.im)};
+}
+
 }  // namespace pxfdtd
+
+extern "C" int pxfdtd_portable(const char *name, double a, double b, double *re, double *im) {
+    using namespace pxfdtd;
+    if (!name || !re || !im) return px_fail(PXFDTD_ERR_INVALID, "pxfdtd_portable needs a name and two outputs");
+    const std::string f(name);
+    double r = 0.0, i = 0.0;
+    C64 w{0.0, 0.0};
+    bool complex = false;
+    if (f == "exp") r = pexp(a);
+    else if (f == "ln") r = pln(a);
+    else if (f == "ln_1p") r = plog1p(a);
+    else if (f == "asin") r = pasin(a);
+    else if (f == "atan") r = patan(a);
+    else if (f == "acosh") r = pacosh(a);
+    else if (f == "atan2") r = patan2(a, b);
+    else if (f == "hypot") r = phypot(a, b);
+    else if (f == "powf") r = ppow(a, b);
+    else if (f == "powi") r = ppowi(a, (int)b);
+    else {
+        complex = true;
+        const C64 z{a, b};
+        if (f == "c_sqrt") w = pcsqrt(z);
+        else if (f == "c_asin") w = pcasin(z);
+        else if (f == "c_ln") w = pcln(z);
+        else if (f == "c_polar") w = {pcabs(z), pcarg(z)};
+        else return px_fail(PXFDTD_ERR_INVALID, "no portable function named '%s'", name);
+    }
+    *re = complex ? w.re : r;
+    *im = complex ? w.im : i;
+    return PXFDTD_OK;
+}

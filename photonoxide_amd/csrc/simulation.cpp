@@ -945,11 +945,10 @@ extern "C" int pxfdtd_simulation_add_mode_source(pxfdtd_simulation *s, const pxf
 }
 
 // Simulation::add_beam (src/fdtd/sources.rs:618-699) with beam_profile (:1043-1108) and
-// PlaneSpectrum (:1110-1350). Complex sqrt, asin and abs come from the C++ library (the
-// reference's are its portable ones); every other operation is the reference's.
+// PlaneSpectrum (:1110-1350), with the reference's operations and its portable complex square
+// root, arcsine and modulus.
 extern "C" int pxfdtd_simulation_add_beam(pxfdtd_simulation *s, const pxfdtd_gaussian_beam *beam,
                                           const pxfdtd_waveform *w) {
-    using cd = std::complex<double>;
     TRY(check_waveform(w));
     if (!beam) return px_fail(PXFDTD_ERR_INVALID, "a beam is required");
     const int axis = beam->axis, tilt = beam->tilt;
@@ -1023,8 +1022,7 @@ extern "C" int pxfdtd_simulation_add_beam(pxfdtd_simulation *s, const pxfdtd_gau
         const C64 ratio = cdiv({0.0, -rayleigh}, qq);
         C64 envelope = ratio;
         if (transverse < 2) {
-            const cd root = std::sqrt(cd(ratio.re, ratio.im));
-            envelope = {root.real(), root.imag()};
+            envelope = pxfdtd::pcsqrt(ratio);
         }
         // i (k r2 / (2 q) + k z)
         const C64 t1 = cdiv({kk * r2, 0.0}, {2.0 * qq.re, 2.0 * qq.im});
@@ -1120,8 +1118,8 @@ extern "C" int pxfdtd_simulation_add_beam(pxfdtd_simulation *s, const pxfdtd_gau
             const double square = beam->eps * k0 * k0 - kb * kb - kc * kc;
             C64 normal = square >= 0.0 ? C64{std::sqrt(square), 0.0} : C64{0.0, std::sqrt(-square)};
             normal = {normal.re * sign, normal.im * sign};
-            const cd as = std::asin(cd(normal.re * step_a / 2.0, normal.im * step_a / 2.0));
-            const C64 k_normal = {2.0 / step_a * as.real(), 2.0 / step_a * as.imag()};
+            const C64 as = pxfdtd::pcasin({normal.re * step_a / 2.0, normal.im * step_a / 2.0});
+            const C64 k_normal = {2.0 / step_a * as.re, 2.0 / step_a * as.im};
             const size_t wv = (size_t)(mc * nw[0] + mb);
             const C64 eb = coefficients[0][wv], ec = coefficients[1][wv];
             std::array<C64, 3> vec = {C64{0.0, 0.0}, C64{0.0, 0.0}, C64{0.0, 0.0}};
@@ -1129,7 +1127,7 @@ extern "C" int pxfdtd_simulation_add_beam(pxfdtd_simulation *s, const pxfdtd_gau
             kvec[axis] = normal;
             kvec[ab] = {kb, 0.0};
             kvec[ac] = {kc, 0.0};
-            if (std::abs(cd(normal.re, normal.im)) > 1e-12 * k0) {
+            if (pxfdtd::pcabs(normal) > 1e-12 * k0) {
                 vec[ab] = eb;
                 vec[ac] = ec;
                 const C64 num = {kb * eb.re + kc * ec.re, kb * eb.im + kc * ec.im};
@@ -1501,8 +1499,8 @@ extern "C" int pxfdtd_simulation_add_flux_box(pxfdtd_simulation *s, const int32_
 }
 
 // ---------------------------------------------------------------- media stability
-// |z| (the reference's portable::complex::abs is a hypot; this is the C library's)
-static double cabs_(C64 z) { return std::hypot(z.re, z.im); }
+// |z|, portable::complex::abs
+static double cabs_(C64 z) { return pxfdtd::pcabs(z); }
 
 // media.rs:446-455: the product of two polynomials, coefficients from the constant term up
 static std::vector<double> poly_multiply(const std::vector<double> &a, const std::vector<double> &b) {
@@ -2559,12 +2557,11 @@ struct ModeLattice {
 };
 
 // Simulation::leapfrog_frequency (src/fdtd/monitors.rs:560-567): the frequency whose leapfrog
-// wavenumber is k0, (2 / dt) asin(k0 dt / 2) / 2 pi (the C library's asin), or false beyond the
-// Nyquist frequency
+// wavenumber is k0, (2 / dt) asin(k0 dt / 2) / 2 pi, or false beyond the Nyquist frequency
 static bool leapfrog_frequency(const pxfdtd_simulation *s, double k0, double *f) {
     const double x = 0.5 * k0 * s->dt;
     if (!(x >= 0.0 && x < 1.0)) return false;
-    *f = 2.0 / s->dt * std::asin(x) / TAU;
+    *f = 2.0 / s->dt * pxfdtd::pasin(x) / TAU;
     return true;
 }
 
@@ -2799,7 +2796,7 @@ static double lattice_flux(const ModeLattice &L, int axis, int plane, const E &e
 // permittivity, at the leapfrog wavenumber of `frequency`. The unknowns are E across the plane
 // inside the window; the rows are k0^2 eps E_t - curl_t curl_t E_t + grad_t(eps_a^-1 div_t(eps
 // E_t)); the eigenvalues nearest k0^2 max(eps, 1) are beta_d^2, beta = (2/h) asin(beta_d h / 2)
-// (complex asin from the C++ library). E_a = i div_t(eps E_t) / (beta_d eps_a) e^(i beta h / 2)
+// (the portable complex arcsine). E_a = i div_t(eps E_t) / (beta_d eps_a) e^(i beta h / 2)
 // on the half-plane after; each mode normalized to N(f, g) = 1 with its largest tangential
 // value's real part positive, going the way its power goes (propagating) or decays; sorted
 // propagating first by |Re beta|, then the others by Im beta. The plane must be clear of the
@@ -2941,9 +2938,8 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
     auto propagates = [](C64 bt) { return std::fabs(bt.im) < std::fabs(bt.re); };
     auto build = [&](const pxeig::Pair &pair, C64 beta_d) -> Built {
         Built m;
-        const std::complex<double> bd(beta_d.re, beta_d.im);
-        const std::complex<double> beta = 2.0 / hstep * std::asin(bd * hstep / 2.0);
-        m.beta = {beta.real(), beta.imag()};
+        const C64 as = pxfdtd::pcasin({beta_d.re * hstep / 2.0, beta_d.im * hstep / 2.0});
+        m.beta = {2.0 / hstep * as.re, 2.0 / hstep * as.im};
         m.tangential.assign((size_t)(4 * per), 0.0);
         m.normal.assign((size_t)(2 * per), 0.0);
         auto at_plane = [&](long long r) -> C64 {
@@ -2995,35 +2991,34 @@ extern "C" int pxfdtd_simulation_port_modes(pxfdtd_simulation *s, int32_t axis, 
             return mode_value(&view, cc, at3, false);
         };
         const C64 norm = L.lorentz(axis, plane, fwd, bwd);
-        const std::complex<double> sq = std::sqrt(std::complex<double>(norm.re, norm.im));
-        std::complex<double> scale = 1.0 / sq;
-        std::complex<double> peak(1.0, 0.0);
+        C64 scale = cdiv({1.0, 0.0}, pxfdtd::pcsqrt(norm));
+        C64 peak = {1.0, 0.0};
         double best = -1.0;
-        for (long long t = 0; t < 2 * per; t++) {
-            const std::complex<double> x(m.tangential[2 * t], m.tangential[2 * t + 1]);
-            if (std::abs(x) >= best) {
-                best = std::abs(x);
+        for (long long t = 0; t < 2 * per; t++) {  // the last of the largest, as max_by gives
+            const C64 x = {m.tangential[2 * t], m.tangential[2 * t + 1]};
+            const double size = pxfdtd::pcabs(x);
+            if (size >= best) {
+                best = size;
                 peak = x;
             }
         }
-        if ((peak * scale).real() < 0.0) scale = -scale;
+        if (cmul(peak, scale).re < 0.0) scale = {-scale.re, -scale.im};
         for (long long t = 0; t < 2 * per; t++) {
-            const std::complex<double> x = std::complex<double>(m.tangential[2 * t], m.tangential[2 * t + 1]) * scale;
-            m.tangential[2 * t] = x.real();
-            m.tangential[2 * t + 1] = x.imag();
+            const C64 x = cmul({m.tangential[2 * t], m.tangential[2 * t + 1]}, scale);
+            m.tangential[2 * t] = x.re;
+            m.tangential[2 * t + 1] = x.im;
         }
         for (long long t = 0; t < per; t++) {
-            const std::complex<double> x = std::complex<double>(m.normal[2 * t], m.normal[2 * t + 1]) * scale;
-            m.normal[2 * t] = x.real();
-            m.normal[2 * t + 1] = x.imag();
+            const C64 x = cmul({m.normal[2 * t], m.normal[2 * t + 1]}, scale);
+            m.normal[2 * t] = x.re;
+            m.normal[2 * t + 1] = x.im;
         }
         m.power = lattice_flux(L, axis, plane, fwd);
         return m;
     };
     std::vector<Built> modes;
     for (const pxeig::Pair &pair : pairs) {
-        const std::complex<double> root = std::sqrt(pair.value);
-        C64 bd = {root.real(), root.imag()};
+        C64 bd = pxfdtd::pcsqrt({pair.value.real(), pair.value.imag()});
         const bool prop = propagates(bd);
         if ((!prop && bd.im < 0.0) || (prop && bd.re < 0.0)) bd = {-bd.re, -bd.im};
         Built m = build(pair, bd);
