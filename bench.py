@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 
 
-def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12, plain=False):
+def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12, plain=False, pml_axes="xyz"):
     import photonoxide_amd as P
     g = P.Grid3d(nx, ny, nz, h, h, h, 0.0, -ny * h / 2, -nz * h / 2)
     n_si, n_ox = 3.48, 1.444
@@ -58,7 +58,11 @@ def mmi_problem(nx=384, ny=256, nz=192, h=0.02, pml=12, plain=False):
         return np.where(si, e_si, e_ox).astype(np.float64)
 
     eps = np.stack([np.broadcast_to(eps_at(c), (nz, ny, nx)) for c in range(3)])
-    s = P.Simulation.with_eps(g, eps, P.Boundaries.cpml_all(pml), 0.9)
+    if pml_axes == "xyz":
+        edges = P.Boundaries.cpml_all(pml)
+    else:  # the other axes periodic: one CPML slab variant's units at a time, for characterization
+        edges = P.Boundaries(*[P.Pml(pml, pml) if a in pml_axes else P.Periodic() for a in "xyz"])
+    s = P.Simulation.with_eps(g, eps, edges, 0.9)
     if plain:
         return s, g
     # the source: E_y across the input arm with a Gaussian profile, a pulse at 1.55 um
@@ -411,6 +415,9 @@ def main():
     ap.add_argument("--plain", action="store_true",
                     help="the kernel alone: no source, no monitors (for characterization)")
     ap.add_argument("--pml", type=int, default=12)
+    ap.add_argument("--pml-axes", default="xyz",
+                    help="with --plain: the axes with a CPML, the others periodic (e.g. y: only the "
+                         "y-slab units take a slab variant)")
     ap.add_argument("--kernel", type=int, default=-1)
     ap.add_argument("--launch", type=int, default=-1,
                     help="the aligned march's launch mode: 0 three launches, 1 the slab one on a "
@@ -477,7 +484,9 @@ def main():
         t_setup = time.perf_counter() - t_setup
     else:
         nx, ny, nz, pml = 384, 256, 192, args.pml
-        s, g = mmi_problem(nx, ny, nz, 0.02, pml, plain=args.plain)
+        if args.pml_axes != "xyz" and not (args.plain and set(args.pml_axes) <= set("xyz")):
+            ap.error("--pml-axes takes a subset of xyz and needs --plain")
+        s, g = mmi_problem(nx, ny, nz, 0.02, pml, plain=args.plain, pml_axes=args.pml_axes)
     gpu = P.Gpu(0)
     s.set_device(gpu, P.Precision.F32)
     if args.kernel >= 0:
@@ -535,6 +544,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": (f"mmi_1x2_soi_384x256x192_dx20nm_cpml{pml}_f32" + ("_plain" if args.plain else "")
+                                + ("" if args.pml_axes == "xyz" else f"_cpml_{args.pml_axes}_only")
                                 if args.workload == "mmi" else
                                 "dispersive_512x512x256_dx15nm_40si_lorentz2_10au_drude_lorentz2_cpml16_f32"),
                    "cells": cells, "kernel": ["lds_tiles", "register_march", "register_march_noprefetch", "aligned_march", "march_2d",

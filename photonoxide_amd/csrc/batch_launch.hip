@@ -18,10 +18,14 @@ __device__ __forceinline__ int batch_block(int it, int n) {
     return j < per ? x * per + j : n;  // n: none
 }
 
+// The tables the kernels read are never written while they run (__restrict__): their loads are
+// scalar, and a job's parameters take SGPRs, not a VGPR each. The general slab variant at one row
+// is held to 128 VGPRs, four waves per SIMD, where it fits without spilling (not with uint8 ids).
 template <typename T, int MODE, int RY, int AX>
-__global__ __launch_bounds__(256) void march64_batch(const JobStep<T> *jobs, const JobSeg *seg,
-                                                     const BatchBlock *blocks, const int *count,
-                                                     const int4 *units, int s) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RY == 1 && AX == 7 && MODE != 0 ? 4 : 1))) void march64_batch(const JobStep<T> *__restrict__ jobs, const JobSeg *__restrict__ seg,
+                                                     const BatchBlock *__restrict__ blocks,
+                                                     const int *__restrict__ count,
+                                                     const int4 *__restrict__ units, int s) {
     __shared__ MarchShared<T> S;
     extern __shared__ unsigned char smem_raw[];
     T *tab = reinterpret_cast<T *>(smem_raw);
@@ -45,9 +49,10 @@ __global__ __launch_bounds__(256) void march64_batch(const JobStep<T> *jobs, con
 }
 
 template <typename T, int MODE, int AX, int SET>
-__global__ __launch_bounds__(256) void march2d_batch(const JobStep<T> *jobs, const JobSeg *seg,
-                                                     const BatchBlock *blocks, const int *count,
-                                                     const int4 *units, int s) {
+__global__ __launch_bounds__(256) void march2d_batch(const JobStep<T> *__restrict__ jobs, const JobSeg *__restrict__ seg,
+                                                     const BatchBlock *__restrict__ blocks,
+                                                     const int *__restrict__ count,
+                                                     const int4 *__restrict__ units, int s) {
     __shared__ MarchShared<T> S;
     extern __shared__ unsigned char smem_raw[];
     T *tab = reinterpret_cast<T *>(smem_raw);

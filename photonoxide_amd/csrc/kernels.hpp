@@ -87,6 +87,8 @@ struct StepArgs {
     int flags;                // bit a: axis a periodic; bit 3: CPML slabs; bit 4: injections;
                               // bit 5: media in media_pass after the step kernel
     int tiles_x, tiles_y, kz;
+    int tile_rows;            // rows of the tiles the injection lists are kept by (a march unit
+                              // may be fewer rows: its tile is the one holding its first row)
     int blocks;               // tiles_x * tiles_y * chunks
     int per_xcd;              // blocks dealt to each XCD, contiguous in the tile order
     const T *ein, *hin;       // copy p, [3][nval] each

@@ -133,8 +133,8 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
     const int lane = tid & 63;
     const int cbase = 2 * (A.nx + A.ny + A.nz);
     const int bx = unit.x, by = unit.y, kz0 = unit.z, kz1 = unit.w;
-    const int t = by * A.tiles_x + bx;
     const int i0 = bx * SW, j0 = by * RY;
+    const int t = (RY == A.tile_rows ? by : j0 / A.tile_rows) * A.tiles_x + bx;  // its injections' tile
     const int nx = A.nx, ny = A.ny, nz = A.nz, pitch = A.pitch;
     const int perx = A.flags & 1, pery = (A.flags >> 1) & 1, perz = (A.flags >> 2) & 1;
     bool injects = false;
@@ -618,7 +618,7 @@ __device__ __forceinline__ void march64_unit(const StepArgs<T> &A, const int4 un
 }
 
 template <typename T, int MODE, int RY, int AX>
-__global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
+__device__ __forceinline__ void march64_kernel(const StepArgs<T> &A) {
     __shared__ MarchShared<T> S;
     extern __shared__ unsigned char smem_raw[];
     T *tab = reinterpret_cast<T *>(smem_raw);                 // [256][3][2]
@@ -631,6 +631,18 @@ __global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
     const int w = uniform(b * 4 + (int)(threadIdx.x >> 6));
     if (w >= A.blocks) return;
     march64_unit<T, MODE, RY, AX>(A, A.units[w], S, tab, grp);
+}
+
+template <typename T, int MODE, int RY, int AX>
+__global__ __launch_bounds__(256) void march64_step(StepArgs<T> A) {
+    march64_kernel<T, MODE, RY, AX>(A);
+}
+
+// The same held to 128 VGPRs in f32, four waves per SIMD: the general slab variant at one row per
+// strip, which needs 130 unbounded (three waves) and fits 128 without spilling
+template <typename T, int MODE, int RY, int AX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void march64_step_4w(StepArgs<T> A) {
+    march64_kernel<T, MODE, RY, AX>(A);
 }
 
 

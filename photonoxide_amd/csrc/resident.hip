@@ -190,6 +190,7 @@ struct pxfdtd_resident {
     Buf units;
     std::vector<int4> units_h;  // the same, on the host (the server batches them)
     int n_lean = 0, n_x = 0, n_slab = 0;  // the march's units: no slab, x slabs only, others
+    int slab_rows = 2;                      // rows per unit of the others (the aligned march)
     bool units_dirty = true;
     bool quiet = false;  // steps record no probe or transform values (the adjoint's recomputation)
     // launched on the stream after each step's kernels, with the step's number in the call (the
@@ -1076,6 +1077,19 @@ static int build_units(pxfdtd_resident *r) {
         });
         lean = merged;
     }
+    // the slab units of a two-row tiling march one row each, in the general variant held to four
+    // waves per SIMD (at two rows its registers leave three): each unit's half as its own unit,
+    // the two adjacent in the list
+    r->slab_rows = th;
+    if (r->kernel == 3 && r->concurrent != 2 && th == 2) {
+        std::vector<int4> halves;
+        halves.reserve(2 * slab.size());
+        for (const int4 &u : slab)
+            for (int h = 0; h < 2; h++)
+                if (2 * u.y + h < r->ny) halves.push_back(make_int4(u.x, 2 * u.y + h, u.z, u.w));
+        slab.swap(halves);
+        r->slab_rows = 1;
+    }
     r->n_lean = (int)lean.size();
     r->n_x = (int)xonly.size();
     r->n_slab = (int)slab.size();
@@ -1662,6 +1676,7 @@ static StepArgs<T> args_of(const pxfdtd_resident *r) {
     tile_shape(r, tw, th);
     A.tiles_x = (r->nx + tw - 1) / tw;
     A.tiles_y = (r->ny + th - 1) / th;
+    A.tile_rows = th;
     A.kz = r->kz;
     A.blocks = A.tiles_x * A.tiles_y * ((r->nz + r->kz - 1) / r->kz);
     A.per_xcd = (A.blocks + 7) / 8;
@@ -1845,7 +1860,7 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
         const SideStream side = {r->side, r->fork_ev, r->join_ev};
         const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
-        const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab};
+        const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab, r->slab_rows};
         const int set = step_set(r);
         const dim3 aux_grid((unsigned)((r->aux_max + 255) / 256), (unsigned)r->planes_h.size());
         // the plane waves' H~ to t + dt/2 before the grid's step reads it (E's couplings);
@@ -1945,7 +1960,7 @@ static int trial_ms(pxfdtd_resident *r, int trials, double *ms) {
     A.wave_step = 0;
     const SideStream side = {r->side, r->fork_ev, r->join_ev};
     const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
-    const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab};
+    const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab, r->slab_rows};
     const int set = step_set(r);
     for (int n = 0; n <= trials; n++) {
         if (n == 1) HIPCHK(hipEventRecord(r->ev0, s));

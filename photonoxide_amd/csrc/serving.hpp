@@ -318,7 +318,9 @@ static int server_rebuild(pxfdtd_server *S) {
         const int v = two_d ? step_set(r) : r->ty;
         int at = 0;
         for (int c = 0; c < 3; c++) {
-            auto &bl = blocks[std::make_tuple(r->kernel, r->mode, v, axes[c])];
+            // the aligned march's slab units may be fewer rows than its tiling's
+            const int rows = !two_d && c == 2 ? r->slab_rows : v;
+            auto &bl = blocks[std::make_tuple(r->kernel, r->mode, rows, axes[c])];
             for (int u = 0; u < counts[c]; u += 4) {
                 bl.push_back({j, (int32_t)units.size()});
                 for (int w = 0; w < 4; w++)

@@ -21,6 +21,7 @@ struct PairStreams {
 
 struct UnitCounts {
     int lean, x, slab;  // units in the lists, in this order
+    int slab_rows;      // rows per unit of the slab list (the aligned march: 1 below the tiling's 2)
 };
 
 // One step's launches of kernel `kernel` (0 LDS tiles, 1-2 the 62-wide march, 3 the aligned

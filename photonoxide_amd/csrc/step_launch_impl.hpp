@@ -46,6 +46,17 @@ static void launch_units(const StepArgs<T> &A0, int first, int count, hipStream_
     hipLaunchKernelGGL((march64_step<T, MODE, RY, AX>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
 }
 
+// the slab units at one row each, in the kernel held to four waves per SIMD
+template <typename T, int MODE>
+static void launch_units_4w(const StepArgs<T> &A0, int first, int count, hipStream_t s, size_t lds) {
+    if (!count) return;
+    StepArgs<T> A = A0;
+    A.units = A0.units + first;
+    A.blocks = count;
+    A.per_xcd = ((count + 3) / 4 + 7) / 8;
+    hipLaunchKernelGGL((march64_step_4w<T, MODE, 1, 7>), dim3(8 * A.per_xcd), dim3(256), lds, s, A);
+}
+
 // The units touching no CPML slab, those touching only the x slabs and the others, each in the
 // variant that compiles in just that psi work. With a side stream the slab launch runs there
 // beside the others; the main stream waits for it before the step's next work.
@@ -58,7 +69,10 @@ static int launch_march64(const StepArgs<T> &A, hipStream_t s, size_t lds, const
         HIPCHK(hipEventRecord(side->fork, s));
         HIPCHK(hipStreamWaitEvent(ss, side->fork, 0));
     }
-    launch_units<T, MODE, RY, 7>(A, n.lean + n.x, n.slab, ss, lds);
+    if (n.slab_rows == 1)
+        launch_units_4w<T, MODE>(A, n.lean + n.x, n.slab, ss, lds);
+    else
+        launch_units<T, MODE, RY, 7>(A, n.lean + n.x, n.slab, ss, lds);
     launch_units<T, MODE, RY, 0>(A, 0, n.lean, s, lds);
     launch_units<T, MODE, RY, 1>(A, n.lean, n.x, s, lds);
     if (both) {
