@@ -578,13 +578,16 @@ enum {
 typedef struct {
     int64_t submitted, admitted, completed, failed;
     int64_t segments, captures; /* segments stepped; hipGraphs captured (one per job set) */
-    double busy_seconds;        /* wall time of the segments on the device */
+    double busy_seconds;        /* wall time from each segment's launch to the host seeing its end */
     double cell_updates;        /* cells x steps over all jobs */
     int64_t hbm_in_use;         /* the admitted jobs' footprint now */
     int64_t peak_jobs;          /* the most jobs admitted at once */
     double admit_seconds;       /* host time putting jobs on the device (tables, uploads, tuning) */
-    double retire_seconds;      /* host time bringing finished jobs' results back */
+    double retire_seconds;      /* host time bringing finished jobs' results back and waiting for the segment */
     double rebuild_seconds;     /* host time building the batched tables and capturing graphs */
+    double prepare_seconds;     /* host time choosing a segment's jobs and filling its tables */
+    double gap_seconds;         /* host time from one segment's end to the next one's launch */
+    double device_seconds;      /* the segments' time on the device, first copy to last kernel */
 } pxfdtd_server_stats;
 int pxfdtd_server_create(pxfdtd_device *dev, const pxfdtd_server_config *cfg, pxfdtd_server **out);
 /* stops the loop: queued jobs are cancelled, running ones cancelled after their segment */

@@ -79,7 +79,8 @@ class ServerStats(C.Structure):
                 ("busy_seconds", C.c_double), ("cell_updates", C.c_double),
                 ("hbm_in_use", C.c_int64), ("peak_jobs", C.c_int64),
                 ("admit_seconds", C.c_double), ("retire_seconds", C.c_double),
-                ("rebuild_seconds", C.c_double)]
+                ("rebuild_seconds", C.c_double), ("prepare_seconds", C.c_double),
+                ("gap_seconds", C.c_double), ("device_seconds", C.c_double)]
 
 
 class Grid(C.Structure):

@@ -128,6 +128,7 @@ struct pxfdtd_server {
     pxfdtd_device *dev = nullptr;
     pxfdtd_server_config cfg;
     hipStream_t stream = nullptr;
+    hipEvent_t seg_start = nullptr, seg_stop = nullptr;  // a segment's span on the device
     std::thread loop;
     std::mutex mu;
     std::condition_variable wake, changed;
@@ -162,6 +163,7 @@ struct pxfdtd_server {
     hipGraphExec_t exec = nullptr;
     std::chrono::steady_clock::time_point seg_t0;  // the segment in flight
     double seg_cells = 0;
+    double t_seg_end = -1;  // when the last segment ended (-1: the loop has waited since)
     int64_t captures = 0;  // read into stats under the lock
 };
 
@@ -410,7 +412,8 @@ static int server_rebuild(pxfdtd_server *S) {
 
 // one segment: every running job's waves and phases, then the batched steps, launched on the
 // server's stream without waiting (server_segment_end waits)
-static int server_segment_begin(pxfdtd_server *S) {
+static int server_segment_begin(pxfdtd_server *S, double *prepare, double *gap) {
+    const double t_begin = server_now(S);
     const int K = S->cfg.segment;
     const int nj = (int)S->table.size();
     double cells = 0;
@@ -458,6 +461,7 @@ static int server_segment_begin(pxfdtd_server *S) {
         S->h_counts[q] = n;
     }
     hipStream_t st = S->stream;
+    HIPCHK(hipEventRecord(S->seg_start, st));
     if (!S->all_blocks.empty())
         HIPCHK(hipMemcpyAsync(S->d_blocks.p, S->h_blocks, S->all_blocks.size() * sizeof(BatchBlock),
                               hipMemcpyHostToDevice, st));
@@ -468,16 +472,24 @@ static int server_segment_begin(pxfdtd_server *S) {
     HIPCHK(hipMemcpyAsync(S->d_seg.p, S->h_seg, (size_t)nj * sizeof(JobSeg), hipMemcpyHostToDevice, st));
     S->seg_t0 = std::chrono::steady_clock::now();
     S->seg_cells = cells;
+    const double t_launch = server_now(S);
+    *prepare = t_launch - t_begin;
+    *gap = S->t_seg_end >= 0 ? t_launch - S->t_seg_end : 0.0;
     if (S->exec)
         HIPCHK(hipGraphLaunch(S->exec, st));
     else
         TRY(launch_segment(S));
+    HIPCHK(hipEventRecord(S->seg_stop, st));
     return PXFDTD_OK;
 }
 
-static int server_segment_end(pxfdtd_server *S, double *busy) {
+static int server_segment_end(pxfdtd_server *S, double *busy, double *device) {
     HIPCHK(hipStreamSynchronize(S->stream));
     *busy = std::chrono::duration<double>(std::chrono::steady_clock::now() - S->seg_t0).count();
+    S->t_seg_end = server_now(S);
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, S->seg_start, S->seg_stop));
+    *device = 1e-3 * (double)ms;
     const int nj = (int)S->table.size();
     for (int j = 0; j < nj; j++) {
         ServerJob *J = S->table[j];
@@ -538,13 +550,14 @@ static void server_loop(pxfdtd_server *S) {
             break;
         }
         if (S->running.empty() && admit.empty() && fetch.empty()) {
+            S->t_seg_end = -1;
             S->wake.wait(lk);
             continue;
         }
         lk.unlock();
         int rc = PXFDTD_OK;
         bool launched = false;
-        double t_rebuild = 0, t_admit = 0, t_retire = 0, t_busy = 0;
+        double t_rebuild = 0, t_admit = 0, t_retire = 0, t_busy = 0, t_prepare = 0, t_gap = 0, t_device = 0;
         if (!S->running.empty() && !S->stop) {
             if (dirty) {
                 const double tb = server_now(S);
@@ -552,7 +565,7 @@ static void server_loop(pxfdtd_server *S) {
                 t_rebuild = server_now(S) - tb;
                 dirty = false;
             }
-            if (rc == PXFDTD_OK) rc = server_segment_begin(S);
+            if (rc == PXFDTD_OK) rc = server_segment_begin(S, &t_prepare, &t_gap);
             launched = rc == PXFDTD_OK;
         }
         const std::string why = rc ? pxfdtd_last_error() : "";
@@ -593,16 +606,19 @@ static void server_loop(pxfdtd_server *S) {
             }
         }
         int rc4 = PXFDTD_OK;
-        if (launched) rc4 = server_segment_end(S, &t_busy);
+        if (launched) rc4 = server_segment_end(S, &t_busy, &t_device);
         const std::string why4 = rc4 ? pxfdtd_last_error() : "";
         for (auto &f : fetched) pxfdtd_host::sim_release(f.first->sim);
         t_retire = server_now(S) - tf;
         lk.lock();
         S->stats.rebuild_seconds += t_rebuild;
+        S->stats.prepare_seconds += t_prepare;
+        S->stats.gap_seconds += t_gap;
         S->stats.admit_seconds += t_admit;
         S->stats.retire_seconds += t_retire;
         if (launched && rc4 == PXFDTD_OK) {
             S->stats.busy_seconds += t_busy;
+            S->stats.device_seconds += t_device;
             S->stats.segments++;
             S->stats.cell_updates += S->seg_cells;
         }
@@ -678,6 +694,8 @@ extern "C" int pxfdtd_server_create(pxfdtd_device *dev, const pxfdtd_server_conf
         S->cfg.hbm_budget = (int64_t)(0.9 * (double)free_b);
     }
     HIPCHK(hipStreamCreate(&S->stream));
+    HIPCHK(hipEventCreate(&S->seg_start));
+    HIPCHK(hipEventCreate(&S->seg_stop));
     S->t0 = std::chrono::steady_clock::now();
     pxfdtd_server *raw = S.release();
     raw->loop = std::thread(server_loop, raw);
@@ -702,7 +720,10 @@ extern "C" void pxfdtd_server_destroy(pxfdtd_server *S) {
     if (S->h_blocks) (void)hipHostFree(S->h_blocks);
     if (S->h_counts) (void)hipHostFree(S->h_counts);
     hipStream_t st = S->stream;
+    hipEvent_t e0 = S->seg_start, e1 = S->seg_stop;
     delete S;  // device tables free themselves
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     if (st) (void)hipStreamDestroy(st);
 }
 
