@@ -191,6 +191,7 @@ struct pxfdtd_resident {
     std::vector<int4> units_h;  // the same, on the host (the server batches them)
     int n_lean = 0, n_x = 0, n_slab = 0;  // the march's units: no slab, x slabs only, others
     int slab_rows = 2;                      // rows per unit of the others (the aligned march)
+    bool batched = false;                   // stepped by the server's batched kernels
     bool units_dirty = true;
     bool quiet = false;  // steps record no probe or transform values (the adjoint's recomputation)
     // launched on the stream after each step's kernels, with the step's number in the call (the
@@ -997,8 +998,10 @@ static int build_units_2d(pxfdtd_resident *r) {
         }
     }
     // a grid too small to fill the chip several times over steps in one launch (the slab
-    // variant's psi work is behind uniform branches): its step time is one launch's latency
-    if ((long long)(lean.size() + slab.size()) < 4LL * 4096) {
+    // variant's psi work is behind uniform branches): its step time is one launch's latency.
+    // Not in the server's batches, where the jobs fill the chip together and each class runs in
+    // one launch whatever its jobs' sizes: there the lean units keep their cheaper variant.
+    if ((long long)(lean.size() + slab.size()) < 4LL * 4096 && !r->batched) {
         slab.insert(slab.begin(), lean.begin(), lean.end());
         lean.clear();
     }

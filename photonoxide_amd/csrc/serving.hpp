@@ -225,6 +225,7 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
         S->tuned[key] = {rows, planes};
     }
     if (r->kernel == 3 && r->ty > 2) TRY(pxfdtd_resident_set_tiling(r, 2, 0));
+    r->batched = true;
     TRY(build_units(r));
     J->emitters = std::max(1, pxfdtd_host::sim_emitters(J->sim));
     J->slots = std::max(1, (int)r->slots);
