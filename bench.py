@@ -264,7 +264,7 @@ def hbm_rates(gpu, mib=1024, reps=10):
     return {k: round(v, 1) for k, v in r.items()} | {"buffer_mib": mib, "unit": "GB/s"}
 
 
-def serving(gpu, n2d, n3d, rate, seed=2, autotune=2, direct=False, shortest_first=True, fill_cells=0):
+def serving(gpu, n2d, n3d, rate, seed=2, autotune=2, direct=False, shortest_first=True, fill_cells=0, segment=0):
     """The batched-serving workload: a seeded Poisson stream of 2D (U[128,1024]^2) and 3D
     (U[64,256]^3) design jobs, 1000-4000 steps, 5-frequency monitors, f32. Each job first runs
     alone on the plain path (one at a time: the sequential throughput), then the stream goes
@@ -296,7 +296,8 @@ def serving(gpu, n2d, n3d, rate, seed=2, autotune=2, direct=False, shortest_firs
         del sim
     # the stream through the server
     sims = [SJ.product_job(sp) for sp in specs]
-    srv = Server(gpu, autotune=autotune, direct=direct, shortest_first=shortest_first, fill_cells=fill_cells)
+    srv = Server(gpu, autotune=autotune, direct=direct, shortest_first=shortest_first, fill_cells=fill_cells,
+                 segment=segment)
     ids = [None] * len(specs)
     t0 = time.perf_counter()
 
@@ -441,6 +442,7 @@ def main():
                     help="serving: launch each segment's kernels directly instead of a hipGraph")
     ap.add_argument("--arrival-order", action="store_true",
                     help="serving: every admitted job steps every segment (no shortest-first order)")
+    ap.add_argument("--segment", type=int, default=0, help="serving: steps per segment (0: the server's default, 32)")
     ap.add_argument("--fill", type=int, default=0,
                     help="serving, shortest first: the cells a segment's jobs fill the device with (0: the server's default)")
     ap.add_argument("--traffic", nargs=5, metavar=("OUT", "FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR"),
@@ -472,7 +474,8 @@ def main():
         import photonoxide_amd as P
         n2d, n3d = (256, 16) if args.serving == "full" else (64, 4)
         print(json.dumps(serving(P.Gpu(0), n2d, n3d, args.rate, direct=args.direct,
-                                 shortest_first=not args.arrival_order, fill_cells=args.fill)), flush=True)
+                                 shortest_first=not args.arrival_order, fill_cells=args.fill,
+                                 segment=args.segment)), flush=True)
         return
     import photonoxide_amd as P
     from photonoxide_amd import _native as N

@@ -83,6 +83,57 @@ extern "C" int pxfdtd_device_query(const pxfdtd_device *d, pxfdtd_device_info *i
 
 extern "C" void pxfdtd_device_close(pxfdtd_device *d) { delete d; }
 
+// Streams and events of residents that are gone, kept per device for the next ones: creating and
+// destroying them costs more than the set-up of a small job (a server sets up and retires
+// hundreds). Held until the process ends.
+struct HipPool {
+    std::mutex mu;
+    std::map<int, std::vector<hipStream_t>> streams;
+    std::map<std::pair<int, bool>, std::vector<hipEvent_t>> events;  // (ordinal, timing)
+};
+static HipPool &hip_pool() {
+    static HipPool *p = new HipPool();  // never destroyed: the handles live as long as the process
+    return *p;
+}
+static hipError_t take_stream(int ordinal, hipStream_t *s) {
+    {
+        HipPool &P = hip_pool();
+        std::lock_guard<std::mutex> g(P.mu);
+        auto &v = P.streams[ordinal];
+        if (!v.empty()) {
+            *s = v.back();
+            v.pop_back();
+            return hipSuccess;
+        }
+    }
+    return hipStreamCreate(s);
+}
+static void give_stream(int ordinal, hipStream_t s) {  // s has no work left
+    if (!s) return;
+    HipPool &P = hip_pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    P.streams[ordinal].push_back(s);
+}
+static hipError_t take_event(int ordinal, bool timing, hipEvent_t *e) {
+    {
+        HipPool &P = hip_pool();
+        std::lock_guard<std::mutex> g(P.mu);
+        auto &v = P.events[{ordinal, timing}];
+        if (!v.empty()) {
+            *e = v.back();
+            v.pop_back();
+            return hipSuccess;
+        }
+    }
+    return hipEventCreateWithFlags(e, timing ? hipEventDefault : hipEventDisableTiming);
+}
+static void give_event(int ordinal, bool timing, hipEvent_t e) {
+    if (!e) return;
+    HipPool &P = hip_pool();
+    std::lock_guard<std::mutex> g(P.mu);
+    P.events[{ordinal, timing}].push_back(e);
+}
+
 // ------------------------------------------------------------------------------ buffers
 struct Buf {
     void *p = nullptr;
@@ -191,6 +242,7 @@ struct pxfdtd_resident {
     std::vector<int4> units_h;  // the same, on the host (the server batches them)
     int n_lean = 0, n_x = 0, n_slab = 0;  // the march's units: no slab, x slabs only, others
     int slab_rows = 2;                      // rows per unit of the others (the aligned march)
+    int ordinal = 0;                        // the device's, for its pooled streams and events
     bool batched = false;                   // stepped by the server's batched kernels
     bool units_dirty = true;
     bool quiet = false;  // steps record no probe or transform values (the adjoint's recomputation)
@@ -591,6 +643,7 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
     HIPCHK(hipSetDevice(dev->ordinal));
     auto *r = new pxfdtd_resident();
     r->dev = dev;
+    r->ordinal = dev->ordinal;
     r->prec = p->precision;
     r->esz = p->precision == PXFDTD_F64 ? 8 : 4;
     r->nx = p->nx;
@@ -637,22 +690,23 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
         r->ty = 1;
         r->kz = default_rows_2d(p->nx, p->ny);
     }
-    hipError_t e = hipStreamCreate(&r->stream);
+    const int ord = dev->ordinal;
+    hipError_t e = take_stream(ord, &r->stream);
     if (e != hipSuccess) {
         delete r;
         return px_fail(PXFDTD_ERR_DEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
-    if (hipStreamCreate(&r->side) != hipSuccess || hipEventCreateWithFlags(&r->fork_ev, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&r->join_ev, hipEventDisableTiming) != hipSuccess) {
-        if (r->side) (void)hipStreamDestroy(r->side);
-        if (r->fork_ev) (void)hipEventDestroy(r->fork_ev);
-        if (r->join_ev) (void)hipEventDestroy(r->join_ev);
-        (void)hipStreamDestroy(r->stream);
+    if (take_stream(ord, &r->side) != hipSuccess || take_event(ord, false, &r->fork_ev) != hipSuccess ||
+        take_event(ord, false, &r->join_ev) != hipSuccess) {
+        give_stream(ord, r->side);
+        give_event(ord, false, r->fork_ev);
+        give_event(ord, false, r->join_ev);
+        give_stream(ord, r->stream);
         delete r;
         return px_fail(PXFDTD_ERR_DEVICE, "the side stream or its events could not be created");
     }
-    (void)hipEventCreate(&r->ev0);
-    (void)hipEventCreate(&r->ev1);
+    (void)take_event(ord, true, &r->ev0);
+    (void)take_event(ord, true, &r->ev1);
     int rc = r->prec == PXFDTD_F64 ? build<double>(r, p) : build<float>(r, p);
     if (rc != PXFDTD_OK) {
         std::string msg = g_error;
@@ -666,13 +720,14 @@ extern "C" int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *
 
 extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
     if (!r) return;
+    const int ord = r->ordinal;  // not r->dev: a resident may outlive its device handle
     if (r->stream) (void)hipStreamSynchronize(r->stream);
-    if (r->ev0) (void)hipEventDestroy(r->ev0);
-    if (r->ev1) (void)hipEventDestroy(r->ev1);
+    give_event(ord, true, r->ev0);
+    give_event(ord, true, r->ev1);
     for (auto e : r->kev) (void)hipEventDestroy(e);
     if (r->side) (void)hipStreamSynchronize(r->side);
-    if (r->fork_ev) (void)hipEventDestroy(r->fork_ev);
-    if (r->join_ev) (void)hipEventDestroy(r->join_ev);
+    give_event(ord, false, r->fork_ev);
+    give_event(ord, false, r->join_ev);
     for (int c = 0; c < 4; c++) {
         if (r->ps.s[c]) (void)hipStreamSynchronize(r->ps.s[c]);
         if (r->ps.join[c]) (void)hipEventDestroy(r->ps.join[c]);
@@ -685,8 +740,8 @@ extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
         specs().erase(r);
     }
     delete r;  // buffers free themselves
-    if (s) (void)hipStreamDestroy(s);
-    if (side) (void)hipStreamDestroy(side);
+    give_stream(ord, s);
+    give_stream(ord, side);
     for (int c = 0; c < 4; c++)
         if (ps.s[c]) (void)hipStreamDestroy(ps.s[c]);
 }
