@@ -144,7 +144,7 @@ static int adj_save(pxfdtd_adjoint *a, size_t slot) {
     const size_t fb = 6 * (size_t)r->nval * sizeof(T), pbytes = (size_t)r->psi_len * sizeof(T);
     if (a->ckpt.size() <= slot) {
         std::vector<Buf> grown(slot + 1);
-        for (size_t q = 0; q < a->ckpt.size(); q++) std::swap(grown[q].p, a->ckpt[q].p), std::swap(grown[q].bytes, a->ckpt[q].bytes);
+        for (size_t q = 0; q < a->ckpt.size(); q++) grown[q].swap(a->ckpt[q]);
         a->ckpt.swap(grown);
     }
     TRY(a->ckpt[slot].reserve(fb + pbytes));

@@ -67,6 +67,7 @@ extern "C" int pxfdtd_device_open(int ordinal, pxfdtd_device **out) {
     return PXFDTD_OK;
 }
 
+static size_t block_held(int ordinal);
 extern "C" int pxfdtd_device_query(const pxfdtd_device *d, pxfdtd_device_info *info) {
     memset(info, 0, sizeof *info);
     snprintf(info->name, sizeof info->name, "%s", d->prop.name);
@@ -77,7 +78,7 @@ extern "C" int pxfdtd_device_query(const pxfdtd_device *d, pxfdtd_device_info *i
     size_t free_b = 0, total = 0;
     HIPCHK(hipSetDevice(d->ordinal));
     HIPCHK(hipMemGetInfo(&free_b, &total));
-    info->free_bytes = (int64_t)free_b;
+    info->free_bytes = (int64_t)(free_b + block_held(d->ordinal));  // cached blocks are free to residents
     return PXFDTD_OK;
 }
 
@@ -135,30 +136,103 @@ static void give_event(int ordinal, bool timing, hipEvent_t e) {
 }
 
 // ------------------------------------------------------------------------------ buffers
+// Device blocks of residents that are gone, kept per device for the next ones: a hipMalloc and
+// hipFree pair (the free waits for the device) costs more than the set-up of a small job, and a
+// server sets up and retires hundreds. Only a resident's destruction gives blocks back, after
+// its streams are synchronized, so no work on the device still reads them; at most 8 GiB a
+// device is held, and an allocation that fails returns them all first.
+constexpr size_t kBlockCacheBytes = size_t(8) << 30;
+struct BlockCache {
+    std::mutex mu;
+    std::map<int, std::multimap<size_t, void *>> free;  // ordinal -> block size -> block
+    std::map<int, size_t> held;
+};
+static BlockCache &block_cache() {
+    static BlockCache *c = new BlockCache();  // never destroyed: the blocks live as long as the process
+    return *c;
+}
+static thread_local bool tl_give_blocks = false;  // set while a resident's buffers are released
+static size_t block_size(size_t n) {               // 64 KiB steps, 2 MiB above 16 MiB
+    const size_t q = n >= (size_t(16) << 20) ? (size_t(2) << 20) : (size_t(64) << 10);
+    return (n + q - 1) / q * q;
+}
+static bool block_take(int ordinal, size_t n, void **p, size_t *size) {
+    BlockCache &C = block_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    auto &m = C.free[ordinal];
+    auto it = m.lower_bound(n);
+    if (it == m.end() || it->first > n + n / 4 + (size_t(64) << 10)) return false;
+    *p = it->second;
+    *size = it->first;
+    C.held[ordinal] -= it->first;
+    m.erase(it);
+    return true;
+}
+static bool block_give(int ordinal, void *p, size_t size) {
+    BlockCache &C = block_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    if (C.held[ordinal] + size > kBlockCacheBytes) return false;
+    C.free[ordinal].insert({size, p});
+    C.held[ordinal] += size;
+    return true;
+}
+static size_t block_held(int ordinal) {
+    BlockCache &C = block_cache();
+    std::lock_guard<std::mutex> g(C.mu);
+    return C.held[ordinal];
+}
+static void block_flush(int ordinal) {
+    std::multimap<size_t, void *> m;
+    {
+        BlockCache &C = block_cache();
+        std::lock_guard<std::mutex> g(C.mu);
+        m.swap(C.free[ordinal]);
+        C.held[ordinal] = 0;
+    }
+    for (auto &b : m) (void)hipFree(b.second);
+}
+
 struct Buf {
     void *p = nullptr;
-    size_t bytes = 0;
+    size_t bytes = 0;   // asked for
+    size_t block = 0;   // the block's own size
+    int ordinal = 0;
     Buf() = default;
     Buf(const Buf &) = delete;
     Buf &operator=(const Buf &) = delete;
     ~Buf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p && !(tl_give_blocks && block_give(ordinal, p, block))) (void)hipFree(p);
         p = nullptr;
-        bytes = 0;
+        bytes = block = 0;
     }
     int alloc(size_t n) {
         release();
         if (n == 0) return PXFDTD_OK;
-        hipError_t e = hipMalloc(&p, n);
+        (void)hipGetDevice(&ordinal);
+        if (block_take(ordinal, n, &p, &block)) {
+            bytes = n;
+            return PXFDTD_OK;
+        }
+        block = block_size(n);
+        hipError_t e = hipMalloc(&p, block);
+        if (e != hipSuccess) {  // the cached blocks back first
+            (void)hipGetLastError();
+            block_flush(ordinal);
+            e = hipMalloc(&p, block);
+        }
         if (e != hipSuccess) {
             p = nullptr;
+            block = 0;
             return px_fail(PXFDTD_ERR_MEMORY, "hipMalloc of %zu bytes: %s", n, hipGetErrorString(e));
         }
         bytes = n;
         return PXFDTD_OK;
     }
     // grows to at least n bytes, keeping nothing
+    void swap(Buf &o) {
+        std::swap(p, o.p), std::swap(bytes, o.bytes), std::swap(block, o.block), std::swap(ordinal, o.ordinal);
+    }
     int reserve(size_t n) { return n <= bytes ? PXFDTD_OK : alloc(n); }
     template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
 };
@@ -739,7 +813,9 @@ extern "C" void pxfdtd_resident_destroy(pxfdtd_resident *r) {
         std::lock_guard<std::mutex> g(specs_mutex());
         specs().erase(r);
     }
-    delete r;  // buffers free themselves
+    tl_give_blocks = true;  // its streams are synchronized: its blocks can go to the next ones
+    delete r;               // buffers free themselves
+    tl_give_blocks = false;
     give_stream(ord, s);
     give_stream(ord, side);
     for (int c = 0; c < 4; c++)
@@ -1820,8 +1896,7 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
                 HIPCHK(hipMemcpyAsync(grown.p, r->probe_rec.p, r->probe_rows * r->nprobes * sizeof(T),
                                       hipMemcpyDeviceToDevice, s));
             HIPCHK(hipStreamSynchronize(s));
-            std::swap(grown.p, r->probe_rec.p);
-            std::swap(grown.bytes, r->probe_rec.bytes);
+            grown.swap(r->probe_rec);
         }
     }
     const size_t lds = r->mode == 0 ? (size_t)256 * (6 * sizeof(T) + 3) : 0;
