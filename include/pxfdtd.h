@@ -133,7 +133,7 @@ int pxfdtd_resident_create(pxfdtd_device *dev, const pxfdtd_problem *p, pxfdtd_r
 void pxfdtd_resident_destroy(pxfdtd_resident *r);
 /* Fields [3][N] and the CPML's psi, all slabs concatenated in the reference's slab order
  * (axis x, y, z; side low, high; field E, H; component axis+1, axis+2; each [k][j][i] over its
- * box): Resident::upload / download (resident.rs:378-413). psi may be NULL (zeros). */
+ * box): Resident::upload / download (resident.rs:378-413). e, h or psi may be NULL (zeros). */
 int pxfdtd_resident_upload(pxfdtd_resident *r, const double *e, const double *h,
                            const double *psi);
 int pxfdtd_resident_download(pxfdtd_resident *r, double *e, double *h, double *psi);

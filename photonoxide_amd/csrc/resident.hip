@@ -868,20 +868,24 @@ static int upload_fields(pxfdtd_resident *r, const double *e, const double *h, c
                 }
         }
     r->fields_set = set;
-    std::vector<T> buf(r->nval, T(0));
+    std::vector<T> buf;
     for (int q = 0; q < 6; q++) {
         const double *src = (q < 3 ? e : h);
-        if (!src) continue;
+        if (!src) {  // zeros
+            HIPCHK(hipMemsetAsync(r->field<T>(r->p, q), 0, r->nval * sizeof(T), r->stream));
+            continue;
+        }
         src += (q % 3) * N;
+        buf.assign(r->nval, T(0));
         to_pitched<T>(r, src, buf.data());
         HIPCHK(hipMemcpyAsync(r->field<T>(r->p, q), buf.data(), r->nval * sizeof(T),
                               hipMemcpyHostToDevice, r->stream));
         HIPCHK(hipStreamSynchronize(r->stream));
     }
-    if (r->psi_len) {
-        std::vector<T> ps(r->psi_len, T(0));
-        if (psi)
-            for (long long q = 0; q < r->psi_len; q++) ps[q] = (T)psi[q];
+    if (r->psi_len && !psi) HIPCHK(hipMemsetAsync(r->psi[r->p].p, 0, r->psi_len * sizeof(T), r->stream));
+    if (r->psi_len && psi) {
+        std::vector<T> ps(r->psi_len);
+        for (long long q = 0; q < r->psi_len; q++) ps[q] = (T)psi[q];
         HIPCHK(hipMemcpyAsync(r->psi[r->p].p, ps.data(), r->psi_len * sizeof(T),
                               hipMemcpyHostToDevice, r->stream));
         HIPCHK(hipStreamSynchronize(r->stream));

@@ -270,6 +270,7 @@ struct pxfdtd_simulation {
     pxfdtd_resident *res = nullptr;
     bool device_newer = false;  // the device's fields are newer than the host's
     bool host_newer = false;    // the host's fields changed since the last upload
+    bool fields_zero = true;    // e, h and every slab's psi hold +0.0 alone, as made
     bool inj_dirty = true, taps_dirty = true;
     int planes_on_device = 0;      // the plane waves the resident steps
     bool planes_device_newer = false;  // their auxiliary fields there are newer than the host's
@@ -1775,6 +1776,7 @@ static int sync_host(pxfdtd_simulation *s) {
     if (!s->res || !s->device_newer) return PXFDTD_OK;
     std::vector<double> psi(pxfdtd_resident_psi_len(s->res));
     TRY(pxfdtd_resident_download(s->res, s->e.data(), s->h.data(), psi.data()));
+    s->fields_zero = false;
     size_t at = 0;
     for (auto &sl : s->slabs)
         for (auto &v : sl.psi) v = psi[at++];
@@ -1838,6 +1840,10 @@ static int ensure_resident(pxfdtd_simulation *s) {
         TRY(pxfdtd_resident_create(s->dev, &p, &s->res));
         s->host_newer = true;
         s->inj_dirty = s->taps_dirty = true;
+    }
+    if (s->host_newer && s->fields_zero) {  // fields as made: zeros, without their bytes
+        TRY(pxfdtd_resident_upload(s->res, nullptr, nullptr, nullptr));
+        s->host_newer = false;
     }
     if (s->host_newer) {
         std::vector<double> psi;
@@ -2157,6 +2163,7 @@ extern "C" int pxfdtd_simulation_set_field(pxfdtd_simulation *s, int32_t which, 
     double *dst = (which == 0 ? s->e.data() : s->h.data()) + comp * s->N;
     std::memcpy(dst, values, s->N * sizeof(double));
     s->host_newer = true;
+    s->fields_zero = false;
     return PXFDTD_OK;
 }
 
