@@ -264,7 +264,7 @@ def hbm_rates(gpu, mib=1024, reps=10):
     return {k: round(v, 1) for k, v in r.items()} | {"buffer_mib": mib, "unit": "GB/s"}
 
 
-def serving(gpu, n2d, n3d, rate, seed=2, autotune=2, direct=False, shortest_first=True, fill_cells=0, segment=0):
+def serving(gpu, n2d, n3d, rate, seed=2, autotune=0, direct=False, shortest_first=True, fill_cells=0, segment=0):
     """The batched-serving workload: a seeded Poisson stream of 2D (U[128,1024]^2) and 3D
     (U[64,256]^3) design jobs, 1000-4000 steps, 5-frequency monitors, f32. Each job first runs
     alone on the plain path (one at a time: the sequential throughput), then the stream goes

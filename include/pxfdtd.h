@@ -557,8 +557,10 @@ typedef struct {
     int32_t max_jobs;    /* jobs admitted at once; 0: 256 */
     int32_t segment;     /* steps per batched segment; 0: 32 */
     int32_t direct;      /* 1: launch each segment's kernels directly instead of a hipGraph */
-    int32_t autotune;    /* trial launches per tiling for the first job of each class of grid
-                            shape (2D or 3D, each extent's power of two); 0: default tiling */
+    int32_t autotune;    /* trial launches per tiling for the first 3D job of each class of
+                            grid shape (each extent's power of two); 0: two rows per strip and
+                            8-plane chunks, which suit a full batch (2D: one row, chunks of up
+                            to 16 rows either way) */
     int32_t order;       /* 0: every admitted job steps in every segment, jobs admitted in
                             arrival order; 1: shortest remaining work first: a segment steps the
                             admitted jobs with the fewest cell-updates left until their cells

@@ -14,9 +14,10 @@
 //
 // Every cell is updated by the same device code as when the job runs alone (the march's unit
 // body, the taps, the step-ordered FP64 sums, no atomics), so a job's fields and spectra are
-// the same bits alone and in any mix. Tilings come from pxfdtd_resident_autotune on the first
-// job of each class of grid shape (2D or 3D, each extent's power of two) and are cached per
-// class.
+// the same bits alone and in any mix. Tilings suit a full batch (2D: one row, chunks of up to 16
+// rows; 3D: two rows, 8-plane chunks) or, with autotune trials asked for, come from
+// pxfdtd_resident_autotune on the first 3D job of each class of grid shape (each extent's power
+// of two) and are cached per class.
 #pragma once
 #include <atomic>
 #include <chrono>
@@ -218,8 +219,11 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
         if (S->cfg.autotune > 0) {
             TRY(pxfdtd_resident_autotune(r, S->cfg.autotune, 2, &rows, &planes, &ms));
         } else {
-            rows = r->ty;
-            planes = r->kz;
+            // in a batch the device is full: two rows per strip and 8-plane chunks (less H
+            // recomputed than the short chunks that fill the chip for one small grid alone)
+            rows = 2;
+            planes = std::min(r->nz, 8);
+            TRY(pxfdtd_resident_set_tiling(r, rows, planes));
         }
         std::lock_guard<std::mutex> g(S->tuned_mu);
         S->tuned[key] = {rows, planes};

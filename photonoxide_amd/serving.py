@@ -25,8 +25,9 @@ class Server:
     hbm_budget: device bytes the admitted jobs may hold (0: 90 % of the free HBM);
     max_jobs: jobs admitted at once (0: 256); segment: steps per batched segment (0: 32);
     direct: launch each segment's kernels directly instead of replaying a hipGraph;
-    autotune: trial launches per tiling for the first job of each class of grid shape (2D or 3D,
-    each extent's power of two; 0: the default tiling);
+    autotune: trial launches per tiling for the first 3D job of each class of grid shape (each
+    extent's power of two; 0: two rows per strip and 8-plane chunks, which suit a full batch);
+    2D jobs take one row and chunks of up to 16 rows either way;
     shortest_first: each segment steps the admitted jobs with the least work left until their
     cells reach fill_cells (0: 2^26), and the queue admits its smallest job first (False: every
     admitted job steps every segment, admitted in arrival order).
