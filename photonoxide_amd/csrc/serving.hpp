@@ -24,6 +24,7 @@
 #include <condition_variable>
 #include <deque>
 #include <memory>
+#include <numeric>
 #include <mutex>
 #include <thread>
 
@@ -231,6 +232,10 @@ static int server_admit(pxfdtd_server *S, ServerJob *J) {
     if (r->kernel == 3 && r->ty > 2) TRY(pxfdtd_resident_set_tiling(r, 2, 0));
     r->batched = true;
     TRY(build_units(r));
+    // the staging holds a whole segment's taps (here, beside the running segment, rather than
+    // in the rebuild between segments)
+    if ((long long)r->staging.bytes < (long long)S->cfg.segment * r->ntaps * 4)
+        TRY(r->staging.alloc((size_t)S->cfg.segment * r->ntaps * 4));
     J->emitters = std::max(1, pxfdtd_host::sim_emitters(J->sim));
     J->slots = std::max(1, (int)r->slots);
     return PXFDTD_OK;
@@ -289,14 +294,20 @@ static int server_rebuild(pxfdtd_server *S) {
     std::vector<int8_t> twhich;
     std::map<std::tuple<int, int, int, int>, std::vector<BatchBlock>> blocks;
     std::vector<int4> units;
-    size_t wv = 0, ph = 0;
+    size_t wv = 0, ph = 0, nt = 0, nd = 0;
     for (int j = 0; j < nj; j++) {
         ServerJob *J = S->table[j];
         J->wave_off = wv;
         J->phase_off = ph;
         wv += (size_t)K * J->emitters * 2;
         ph += (size_t)K * J->slots * 2;
+        nt += (size_t)J->r->ntaps;
+        nd += (size_t)J->r->nvals;
     }
+    // every tap and transform value of every job: whole runs per job, not value by value
+    tjob.resize(nt), tlocal.resize(nt), tindex.resize(nt), twhich.resize(nt);
+    djob.resize(nd), dlocal.resize(nd);
+    size_t at_t = 0, at_d = 0;
     TRY(S->d_waves.reserve(std::max<size_t>(wv, 1) * 8));
     TRY(S->d_phases.reserve(std::max<size_t>(ph, 1) * 8));
     TRY(S->d_colds.reserve(std::max<size_t>(colds.size(), 1) * sizeof(StepCold<float>)));
@@ -341,12 +352,14 @@ static int server_rebuild(pxfdtd_server *S) {
             for (int f = 0; f < 6; f++) tj.fields[q][f] = r->field<float>(q, f);
         tj.staging = r->staging.as<float>();
         tj.ntaps = r->ntaps;
-        for (long long v = 0; v < r->ntaps; v++) {
-            tjob.push_back(j);
-            tlocal.push_back(v);
-            tindex.push_back(r->tap_index_h[v]);
-            twhich.push_back(r->tap_which_h[v]);
+        const size_t n_t = (size_t)r->ntaps;
+        std::fill_n(tjob.begin() + at_t, n_t, j);
+        std::iota(tlocal.begin() + at_t, tlocal.begin() + at_t + n_t, (int64_t)0);
+        if (n_t) {
+            std::memcpy(tindex.data() + at_t, r->tap_index_h.data(), n_t * sizeof(int64_t));
+            std::memcpy(twhich.data() + at_t, r->tap_which_h.data(), n_t * sizeof(int8_t));
         }
+        at_t += n_t;
         DftJob<float> &dj = dftjobs[j];
         dj.staged = r->staging.as<float>() + r->nprobes;
         dj.row = r->ntaps;
@@ -355,10 +368,9 @@ static int server_rebuild(pxfdtd_server *S) {
         dj.phases = S->d_phases.as<double>() + J->phase_off;
         dj.slots = J->slots;
         dj.sums = r->sums.as<double>();
-        for (long long v = 0; v < r->nvals; v++) {
-            djob.push_back(j);
-            dlocal.push_back(v);
-        }
+        std::fill_n(djob.begin() + at_d, (size_t)r->nvals, j);
+        std::iota(dlocal.begin() + at_d, dlocal.begin() + at_d + r->nvals, (int64_t)0);
+        at_d += (size_t)r->nvals;
         // the staging must hold a whole segment
         if ((long long)r->staging.bytes < (long long)K * r->ntaps * 4) {
             TRY(r->staging.alloc((size_t)K * r->ntaps * 4));
