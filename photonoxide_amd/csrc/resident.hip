@@ -14,6 +14,7 @@
 #include <functional>
 #include <map>
 #include <mutex>
+#include <set>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -519,17 +520,31 @@ static int build(pxfdtd_resident *r, const pxfdtd_problem *p) {
     std::vector<double> tab(256 * 6, 0.0);
     std::vector<int8_t> tgrp(256 * 3, -1);
     bool table_ok = true;
+    auto key_at = [&](long long q) {
+        Key key;
+        for (int c = 0; c < 3; c++) {
+            memcpy(&key.v[2 * c], &p->ca[c * N + q], 8);
+            memcpy(&key.v[2 * c + 1], &p->cb[c * N + q], 8);
+            key.g[c] = member.empty() ? (int8_t)-1 : (int8_t)(member[c * N + q] - 1);
+        }
+        return key;
+    };
+    {
+        // more than 256 distinct triples among 4096 evenly spaced cells settle it without the
+        // serial scan of every cell (a smooth design region has thousands)
+        std::set<Key> few;
+        const long long stride = std::max<long long>(1, N / 4096);
+        for (long long q = 0; q < N && table_ok; q += stride) {
+            few.insert(key_at(q));
+            if (few.size() > 256) table_ok = false;
+        }
+    }
     {
         std::map<Key, int> seen;
         Key last;
         int last_id = -1;
         for (long long q = 0; q < N && table_ok; q++) {
-            Key key;
-            for (int c = 0; c < 3; c++) {
-                memcpy(&key.v[2 * c], &p->ca[c * N + q], 8);
-                memcpy(&key.v[2 * c + 1], &p->cb[c * N + q], 8);
-                key.g[c] = member.empty() ? (int8_t)-1 : (int8_t)(member[c * N + q] - 1);
-            }
+            const Key key = key_at(q);
             int id;
             if (last_id >= 0 && key == last) {
                 id = last_id;
