@@ -1932,7 +1932,17 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
             r->kev.push_back(e);
         }
     HIPCHK(hipEventRecord(r->ev0, s));
+    // the aligned march with a side stream takes each step's taps (and a block's transform
+    // sums and probe rows) into the next step, on the side stream: they read the copy that step
+    // reads and writes nothing it reads, and the step's join orders them before the step after
+    // it, which overwrites that copy
+    const bool defer_taps = r->kernel == 3 && r->concurrent == 1;
+    std::function<int(hipStream_t)> pending;
     for (uint32_t n = 0; n < count; n++) {
+        if (pending && (n + 1 < count && pair_ok(r, in_block))) {
+            TRY(pending(s));
+            pending = nullptr;
+        }
         if (n + 1 < count && pair_ok(r, in_block)) {  // steps n and n + 1 in one pass
             StepArgs<T> A = args_of<T>(r);
             const int p = r->p, q = 1 - p;
@@ -2010,8 +2020,11 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
             A.grp = nullptr;
         }
         if (r->profile) HIPCHK(hipEventRecord(r->kev[2 * n], s));
-        const SideStream side = {r->side, r->fork_ev, r->join_ev};
+        SideStream side = {r->side, r->fork_ev, r->join_ev};
+        side.first = std::move(pending);
+        pending = nullptr;
         const SideStream *sp = r->concurrent == 1 ? &side : nullptr;
+        if (side.first && !(r->kernel == 3 && sp)) TRY(side.first(s));
         const UnitCounts uc = {r->n_lean, r->n_x, r->n_slab, r->slab_rows};
         const int set = step_set(r);
         const dim3 aux_grid((unsigned)((r->aux_max + 255) / 256), (unsigned)r->planes_h.size());
@@ -2045,35 +2058,44 @@ static int step_t(pxfdtd_resident *r, uint32_t count, const double *waves, const
             HIPCHK(hipGetLastError());
         }
         if (r->ntaps && !r->quiet) {
-            const int threads = 256;
-            const long long blocks = (r->ntaps + threads - 1) / threads;
-            hipLaunchKernelGGL((tap_values<T>), dim3((unsigned)blocks), dim3(threads), 0, s,
-                               r->tap_fields[q].as<const T *const>(), r->tap_index.as<int64_t>(),
-                               r->tap_which.as<int8_t>(), r->ntaps,
-                               r->staging.as<T>() + (size_t)in_block * r->ntaps);
-            HIPCHK(hipGetLastError());
+            const T *const *tf = r->tap_fields[q].as<const T *const>();
+            T *slot = r->staging.as<T>() + (size_t)in_block * r->ntaps;
             in_block++;
-            if (in_block == r->block_steps || n + 1 == count) {
-                if (r->nvals) {
+            const bool block_end = in_block == r->block_steps || n + 1 == count;
+            const int rows = in_block;
+            const double *ph = r->phases.as<double>() + (size_t)block_first * r->slots * 2;
+            T *rec = r->probe_rec.as<T>() + (size_t)r->probe_rows * r->nprobes;
+            auto work = [r, tf, slot, block_end, rows, ph, rec](hipStream_t on) -> int {
+                const int threads = 256;
+                const long long blocks = (r->ntaps + threads - 1) / threads;
+                hipLaunchKernelGGL((tap_values<T>), dim3((unsigned)blocks), dim3(threads), 0, on, tf,
+                                   r->tap_index.as<int64_t>(), r->tap_which.as<int8_t>(), r->ntaps, slot);
+                HIPCHK(hipGetLastError());
+                if (block_end && r->nvals) {
                     const long long blocks2 = (r->nvals + threads - 1) / threads;
-                    hipLaunchKernelGGL((dft_accumulate<T>), dim3((unsigned)blocks2), dim3(threads), 0, s,
-                                       r->staging.as<T>() + r->nprobes, r->ntaps, r->nvals, in_block,
-                                       r->series_of.as<int32_t>(), r->series_info.as<SeriesInfo>(),
-                                       r->phases.as<double>() + (size_t)block_first * r->slots * 2,
+                    hipLaunchKernelGGL((dft_accumulate<T>), dim3((unsigned)blocks2), dim3(threads), 0, on,
+                                       r->staging.as<T>() + r->nprobes, r->ntaps, r->nvals, rows,
+                                       r->series_of.as<int32_t>(), r->series_info.as<SeriesInfo>(), ph,
                                        r->slots, r->sums.as<double>());
                     HIPCHK(hipGetLastError());
                 }
-                if (r->nprobes) {
-                    HIPCHK(hipMemcpy2DAsync(r->probe_rec.as<T>() + (size_t)r->probe_rows * r->nprobes,
-                                            r->nprobes * sizeof(T), r->staging.p, r->ntaps * sizeof(T),
-                                            r->nprobes * sizeof(T), in_block, hipMemcpyDeviceToDevice, s));
-                    r->probe_rows += in_block;
-                }
+                if (block_end && r->nprobes)
+                    HIPCHK(hipMemcpy2DAsync(rec, r->nprobes * sizeof(T), r->staging.p, r->ntaps * sizeof(T),
+                                            r->nprobes * sizeof(T), rows, hipMemcpyDeviceToDevice, on));
+                return PXFDTD_OK;
+            };
+            if (defer_taps)
+                pending = work;
+            else
+                TRY(work(s));
+            if (block_end) {
+                if (r->nprobes) r->probe_rows += in_block;
                 block_first = n + 1;
                 in_block = 0;
             }
         }
     }
+    if (pending) TRY(pending(s));
     HIPCHK(hipEventRecord(r->ev1, s));
     HIPCHK(hipEventSynchronize(r->ev1));
     float ms = 0;

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -11,6 +13,9 @@ namespace pxfdtd {
 struct SideStream {
     hipStream_t s;
     hipEvent_t fork, join;
+    // work of the step before that only reads the fields this step reads (its monitor taps):
+    // run on the side stream after the fork, in the slack beside the longer main-stream launches
+    std::function<int(hipStream_t)> first = nullptr;
 };
 // the two-step march's CPML classes each on a stream of its own beside the lean launch, forked
 // from and joined to the main stream by events (all at the default priority)

@@ -69,6 +69,7 @@ static int launch_march64(const StepArgs<T> &A, hipStream_t s, size_t lds, const
         HIPCHK(hipEventRecord(side->fork, s));
         HIPCHK(hipStreamWaitEvent(ss, side->fork, 0));
     }
+    if (side && side->first) TRY(side->first(ss));
     if (n.slab_rows == 1)
         launch_units_4w<T, MODE>(A, n.lean + n.x, n.slab, ss, lds);
     else
