@@ -248,6 +248,25 @@ def test_2d_march_steps_one_field_set_bit_for_bit(gpu, which, prec):
             assert same_bits(o.dft(0, f, c).view(np.float64), p.dft(0, f, c).view(np.float64)), (f, c)
 
 
+def test_device_again_midrun_changes_no_bit(gpu):
+    """Fields, psi and monitor sums brought back to the host and put on a new resident carry on
+    as if the run had not stopped (the host's fields are no longer the zeros a new simulation
+    holds, so they go up in full)."""
+    import photonoxide_amd as P
+    spec = busy_spec(False)
+    a = product(spec)
+    a.set_device(gpu, P.Precision.F32)
+    a.run(60)
+    b = product(spec)
+    b.set_device(gpu, P.Precision.F32)
+    b.run(23)
+    b.set_device(gpu, P.Precision.F32)
+    b.run(37)
+    assert same_bits(product_fields(a), product_fields(b))
+    for n in range(len(spec["probes"])):
+        assert same_bits(a.probe(n), b.probe(n))
+
+
 @pytest.mark.parametrize("rows_per_chunk", [1, 5, 45])
 def test_2d_march_chunks_change_no_bit(gpu, rows_per_chunk):
     import photonoxide_amd as P
